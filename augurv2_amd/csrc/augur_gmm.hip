@@ -1,0 +1,635 @@
+// augur_gmm.hip -- Gibbs sampler for the multivariate Gaussian mixture
+// (hmvgmm.rv; mvgmm.rv with fixed pis and covs) on gfx950.
+//
+// Sweep, schedule ConjGibbs [pis] (*) ConjGibbs [mu] (*) ConjGibbs [cov] (*)
+// DiscGibbs [z] (experiments/experiment.py:127, order CudaC/CgKern.hs:192-238):
+//
+//   k_gmm_conj  one block per (component, chain): reduces the sufficient
+//               statistics that the previous z step left per block, draws
+//               pis_k's Gamma (Core/CgConj.hs:171-183), mu_k | cov_k
+//               (cbits/augur_dist.c:633-669), cov_k | mu_k
+//               (cbits/augur_dist.c:740-757, Bartlett), all in FP64, and builds
+//               the component's table for the z step: L_k^-1 (packed lower
+//               triangle), mu_k and -sum log diag L_k, stored FP32.
+//   k_gmm_z     point-parallel categorical draw (Core/CgDisc.hs:122-144) in
+//               log space with max subtraction: logit_k = log pis_k
+//               - 0.5 log det cov_k - 0.5 |L_k^-1 (y - mu_k)|^2, tables in
+//               LDS; fused with the statistics of the new z for the next
+//               sweep, centred at the current mu_k: count, sum (y - mu_k) and
+//               sum (y - mu_k)(y - mu_k)^T (LDS atomics, FP32 per block, FP64
+//               across blocks).  The next sweep's scatter around the new mu is
+//               S + s d^T + d s^T + n d d^T with d = mu_old - mu_new, so the
+//               data are read once per sweep.
+//
+// The reference multiplies raw pdfs in linear space (cbits/augur_dist.c:
+// 798-813), which underflows at D = 16; the log-space draw targets the same
+// distribution.
+#include "augur_common.h"
+
+namespace augur {
+
+constexpr int kGmmThreads = 256;
+constexpr int kMaxDim = 16;
+constexpr int kMaxComp = 64;
+
+__host__ __device__ constexpr int tri(int d) { return d * (d + 1) / 2; }
+
+struct GmmDev {
+  int n, d, k, G, S;       // S = 1 + d + d(d+1)/2 statistics per component
+  const float* y;          // n x d
+  const double* alpha;     // k
+  const double* mu0;       // d
+  const double* cov0;      // d x d
+  const double* scale0;    // d x d
+  int df, sample_pi, sample_cov;
+  double* gpi;     // C x k unnormalised Dirichlet gammas (or pis when fixed)
+  double* pis64;   // C x k normalised
+  double* mu64;    // C x k x d
+  double* cov64;   // C x k x d x d
+  float* tab;      // C x k x TS   (TS = tri(DM) + DM + 1, DM = padded dim)
+  int TS, DM;
+  float* part;     // C x G x k x S
+  int32_t* z;      // C x n
+};
+
+// ---- small FP64 linear algebra (cbits/augur_matop.c:312-406), row-major ----
+__device__ void chol(double* a, int n) {
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j <= i; ++j) {
+      double s = 0.0;
+      for (int k = 0; k < j; ++k) s += a[i * n + k] * a[j * n + k];
+      if (i == j) a[i * n + i] = sqrt(a[i * n + i] - s);
+      else a[i * n + j] = (a[i * n + j] - s) / a[j * n + j];
+    }
+}
+__device__ void lower_trmi(double* l, int n) {
+  for (int c = 0; c < n; ++c)
+    for (int m = 0; m < n; ++m) {
+      double b = (c == m) ? 1.0 : 0.0;
+      for (int i = 0; i < m; ++i) b -= l[m * n + i] * l[i * n + c];
+      l[m * n + c] = b / l[m * n + m];
+    }
+}
+__device__ void chol_invert(double* a, int n) {
+  lower_trmi(a, n);
+  for (int c = 0; c < n; ++c)
+    for (int r = c; r < n; ++r) {
+      double acc = 0.0;
+      for (int k = c; k < n; ++k) acc += a[k * n + r] * a[k * n + c];
+      a[r * n + c] = acc;
+    }
+  for (int c = 0; c < n; ++c)
+    for (int r = c + 1; r < n; ++r) a[c * n + r] = a[r * n + c];
+}
+__device__ void trtrmm(const double* l1, double* l2, int n) {
+  for (int i = n - 1; i >= 0; --i)
+    for (int j = 0; j < n; ++j) {
+      double t = 0.0;
+      for (int k = 0; k < i; ++k) t += l1[i * n + k] * l2[k * n + j];
+      t += l1[i * n + i] * l2[i * n + j];
+      l2[i * n + j] = t;
+    }
+}
+
+// z-step table of one component from (pis_k, mu_k, cov_k).
+__device__ void build_table(const GmmDev& m, int c, int k, double logpi_unused, double* work) {
+  const int d = m.d, DM = m.DM;
+  const double* cov = m.cov64 + ((size_t)c * m.k + k) * d * d;
+  for (int i = 0; i < d * d; ++i) work[i] = cov[i];
+  chol(work, d);
+  double ldet = 0.0;
+  for (int i = 0; i < d; ++i) ldet += log(work[i * d + i]);
+  lower_trmi(work, d);
+  float* t = m.tab + ((size_t)c * m.k + k) * m.TS;
+  for (int i = 0; i < DM; ++i)
+    for (int j = 0; j <= i; ++j) t[tri(i) + j] = (i < d && j < d) ? (float)work[i * d + j] : 0.0f;
+  const double* mu = m.mu64 + ((size_t)c * m.k + k) * d;
+  for (int i = 0; i < DM; ++i) t[tri(DM) + i] = i < d ? (float)mu[i] : 0.0f;
+  t[tri(DM) + DM] = (float)(-ldet);  // -0.5 log det cov
+}
+
+// One block per (component k, chain c).
+__global__ __launch_bounds__(kGmmThreads) void k_gmm_conj(GmmDev m, uint32_t seed, uint32_t sweep,
+                                                          int use_stats) {
+  const int k = blockIdx.x, c = blockIdx.y, tid = threadIdx.x;
+  const int d = m.d, S = m.S;
+  __shared__ double st[1 + kMaxDim + kMaxDim * kMaxDim];
+  __shared__ double work[5 * kMaxDim * kMaxDim + 4 * kMaxDim];
+  if (use_stats) {
+    for (int i = tid; i < S; i += kGmmThreads) {
+      double s = 0.0;
+      for (int g = 0; g < m.G; ++g) s += (double)m.part[(((size_t)c * m.G + g) * m.k + k) * S + i];
+      st[i] = s;
+    }
+  }
+  __syncthreads();
+  if (tid != 0 || !use_stats) return;
+  // unpack: st[0] count, st[1..d] sum (y - centre), packed lower S after
+  const int cnt = (int)llrint(st[0]);
+  double* sc = work;                     // d
+  double* Sc = sc + kMaxDim;             // d x d
+  double* L0 = Sc + kMaxDim * kMaxDim;   // d x d
+  double* L = L0 + kMaxDim * kMaxDim;    // d x d
+  double* A = L + kMaxDim * kMaxDim;     // d x d
+  double* v = A + kMaxDim * kMaxDim;     // d
+  double* mean = v + kMaxDim;            // d
+  double* munew = mean + kMaxDim;        // d
+  double* B = munew + kMaxDim;           // d x d
+  for (int i = 0; i < d; ++i) sc[i] = st[1 + i];
+  for (int i = 0; i < d; ++i)
+    for (int j = 0; j <= i; ++j) {
+      const double val = st[1 + d + tri(i) + j];
+      Sc[i * d + j] = val;
+      Sc[j * d + i] = val;
+    }
+  double* mu = m.mu64 + ((size_t)c * m.k + k) * d;  // current mu = centre
+  double* cov = m.cov64 + ((size_t)c * m.k + k) * d * d;
+  if (m.sample_pi) {
+    const Stream sp{seed, (uint32_t)c, sweep, TAG_GMM_PI};
+    m.gpi[(size_t)c * m.k + k] = gamma_at(sp, (uint32_t)k, m.alpha[k] + (double)cnt);
+  }
+  // ---- mu_k | cov_k: cbits/augur_dist.c:633-669 ----
+  for (int i = 0; i < d * d; ++i) L0[i] = m.cov0[i];
+  chol(L0, d);
+  chol_invert(L0, d);  // cov0^-1
+  for (int i = 0; i < d * d; ++i) L[i] = cov[i];
+  chol(L, d);
+  chol_invert(L, d);
+  for (int i = 0; i < d * d; ++i) L[i] *= (double)cnt;  // n cov^-1
+  for (int i = 0; i < d; ++i) {
+    double t = 0.0;
+    for (int j = 0; j < d; ++j) t += L0[i * d + j] * m.mu0[j];
+    v[i] = t;
+  }
+  // the z step centred its statistics at the FP32 table copy of mu
+  double cen[kMaxDim];
+  for (int i = 0; i < d; ++i) cen[i] = (double)(float)mu[i];
+  // mean of the assigned points: centre + s / n (the sum itself when n = 0)
+  const double inv = cnt == 0 ? 1.0 : 1.0 / (double)cnt;
+  for (int i = 0; i < d; ++i) mean[i] = ((double)cnt * cen[i] + sc[i]) * inv;
+  for (int i = 0; i < d; ++i) {
+    double t = 0.0;
+    for (int j = 0; j < d; ++j) t += L[i * d + j] * mean[j];
+    v[i] += t;
+  }
+  for (int i = 0; i < d * d; ++i) L0[i] += L[i];
+  chol(L0, d);
+  chol_invert(L0, d);  // A = (cov0^-1 + n cov^-1)^-1
+  for (int i = 0; i < d; ++i) {
+    double t = 0.0;
+    for (int j = 0; j < d; ++j) t += L0[i * d + j] * v[j];
+    mean[i] = t;
+  }
+  {
+    const Stream sm{seed, (uint32_t)c, sweep, TAG_GMM_MU};
+    for (int i = 0; i < d; ++i) munew[i] = normal_at(sm, (uint32_t)(k * d + i), 0);
+    for (int i = 0; i < d * d; ++i) L[i] = L0[i];
+    chol(L, d);
+    for (int i = d - 1; i >= 0; --i) {  // trvm
+      double t = 0.0;
+      for (int j = 0; j <= i; ++j) t += L[i * d + j] * munew[j];
+      munew[i] = t;
+    }
+    for (int i = 0; i < d; ++i) munew[i] += mean[i];
+  }
+  // ---- cov_k | mu_k: IW(df + n, scale0 + sum (y - mu)(y - mu)^T) ----
+  if (m.sample_cov) {
+    double* dl = v;  // d = centre - mu_new
+    for (int i = 0; i < d; ++i) dl[i] = cen[i] - munew[i];
+    for (int i = 0; i < d; ++i)
+      for (int j = 0; j < d; ++j)
+        A[i * d + j] = Sc[i * d + j] + sc[i] * dl[j] + dl[i] * sc[j] + (double)cnt * dl[i] * dl[j] +
+                       m.scale0[i * d + j];
+    // augur_invwishart_sample (cbits/augur_dist.c:740-749)
+    chol(A, d);
+    chol_invert(A, d);
+    chol(A, d);
+    const Stream sc_s{seed, (uint32_t)c, sweep, TAG_GMM_COV};
+    const int dfn = m.df + cnt;
+    for (int i = 0; i < d; ++i)
+      for (int j = 0; j < d; ++j) {
+        const uint32_t site = (uint32_t)(k * d * d + i * d + j);
+        if (j < i) B[i * d + j] = normal_at(sc_s, site, 0);
+        else if (j > i) B[i * d + j] = 0.0;
+        else B[i * d + j] = sqrt(2.0 * gamma_at(sc_s, site, 0.5 * (double)(dfn - i)));
+      }
+    trtrmm(A, B, d);
+    chol_invert(B, d);
+    for (int i = 0; i < d * d; ++i) cov[i] = B[i];
+  }
+  for (int i = 0; i < d; ++i) mu[i] = munew[i];
+  build_table(m, c, k, 0.0, L);
+}
+
+// Tables from the current parameters (after set_pt).
+__global__ void k_gmm_tables(GmmDev m) {
+  const int k = blockIdx.x, c = blockIdx.y;
+  __shared__ double work[kMaxDim * kMaxDim];
+  if (threadIdx.x == 0) build_table(m, c, k, 0.0, work);
+}
+
+// DM: padded dimension, KM: maximum number of components held in registers.
+template <int DM, int KM>
+__global__ __launch_bounds__(kGmmThreads) void k_gmm_z(GmmDev m, uint32_t seed, uint32_t sweep,
+                                                      uint32_t chain0, int with_stats) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int TS = tri(DM) + DM + 1;
+  const int K = m.k, d = m.d, S = m.S, c = blockIdx.y, tid = threadIdx.x;
+  float* tabs = (float*)smem;        // K x TS
+  float* acc = tabs + K * TS;        // K x S statistics
+  float* lpi = acc + K * S;          // K
+  const float* gtab = m.tab + (size_t)c * K * m.TS;
+  for (int i = tid; i < K * TS; i += kGmmThreads) tabs[i] = gtab[i];
+  for (int i = tid; i < K * S; i += kGmmThreads) acc[i] = 0.0f;
+  if (tid == 0) {
+    const double* g = m.gpi + (size_t)c * K;
+    double tot = 0.0;
+    for (int i = 0; i < K; ++i) tot += g[i];
+    for (int i = 0; i < K; ++i) {
+      lpi[i] = (float)log(g[i] / tot);
+      if (blockIdx.x == 0) m.pis64[(size_t)c * K + i] = g[i] / tot;
+    }
+  }
+  __syncthreads();
+  const Stream zs{seed, chain0 + (uint32_t)c, sweep, TAG_GMM_Z};
+  const int stride = gridDim.x * kGmmThreads;
+  for (int n = blockIdx.x * kGmmThreads + tid; n < m.n; n += stride) {
+    float yv[DM];
+    const float* yr = m.y + (size_t)n * d;
+    if (d == DM && (DM % 4) == 0) {
+#pragma unroll
+      for (int i = 0; i < DM; i += 4) {
+        const float4 q = *(const float4*)(yr + i);
+        yv[i] = q.x; yv[i + 1] = q.y; yv[i + 2] = q.z; yv[i + 3] = q.w;
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < DM; ++i) yv[i] = i < d ? yr[i] : 0.0f;
+    }
+    float lg[KM];
+    float mx = -INFINITY;
+#pragma unroll
+    for (int kk = 0; kk < KM; ++kk) {
+      if (kk < K) {
+        const float* t = tabs + kk * TS;
+        float dv[DM];
+#pragma unroll
+        for (int i = 0; i < DM; ++i) dv[i] = yv[i] - t[tri(DM) + i];
+        float q = 0.0f;
+#pragma unroll
+        for (int i = 0; i < DM; ++i) {
+          float w = 0.0f;
+#pragma unroll
+          for (int j = 0; j <= i; ++j) w = fmaf(t[tri(i) + j], dv[j], w);
+          q = fmaf(w, w, q);
+        }
+        const float l = lpi[kk] + t[tri(DM) + DM] - 0.5f * q;
+        lg[kk] = l;
+        mx = fmaxf(mx, l);
+      }
+    }
+    float tot = 0.0f;
+#pragma unroll
+    for (int kk = 0; kk < KM; ++kk)
+      if (kk < K) {
+        lg[kk] = expf(lg[kk] - mx);
+        tot += lg[kk];
+      }
+    const float u = u01f(zs.draw((uint32_t)n, 0).x);
+    const float tt = u * tot;
+    int zk = K - 1;
+    float run = 0.0f;
+#pragma unroll
+    for (int kk = 0; kk < KM; ++kk)
+      if (kk < K - 1) {
+        run += lg[kk];
+        if (zk == K - 1 && tt < run) zk = kk;
+      }
+    m.z[(size_t)c * m.n + n] = zk;
+    if (with_stats) {
+      const float* t = tabs + zk * TS;
+      float dv[DM];
+#pragma unroll
+      for (int i = 0; i < DM; ++i) dv[i] = yv[i] - t[tri(DM) + i];
+      float* a = acc + zk * S;
+      atomicAdd(a, 1.0f);
+      for (int i = 0; i < d; ++i) {
+        atomicAdd(a + 1 + i, dv[i]);
+        for (int j = 0; j <= i; ++j) atomicAdd(a + 1 + d + tri(i) + j, dv[i] * dv[j]);
+      }
+    }
+  }
+  if (!with_stats) return;
+  __syncthreads();
+  float* dst = m.part + ((size_t)c * m.G + blockIdx.x) * K * S;
+  for (int i = tid; i < K * S; i += kGmmThreads) dst[i] = acc[i];
+}
+
+// Statistics of the current z around the current mu (after set_pt).
+__global__ void k_gmm_stats(GmmDev m) {
+  const int c = blockIdx.y, tid = threadIdx.x;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int K = m.k, d = m.d, S = m.S;
+  float* acc = (float*)smem;
+  for (int i = tid; i < K * S; i += blockDim.x) acc[i] = 0.0f;
+  __syncthreads();
+  for (int n = blockIdx.x * blockDim.x + tid; n < m.n; n += gridDim.x * blockDim.x) {
+    const int zk = m.z[(size_t)c * m.n + n];
+    const double* mu = m.mu64 + ((size_t)c * K + zk) * d;
+    float dv[kMaxDim];
+    for (int i = 0; i < d; ++i) dv[i] = m.y[(size_t)n * d + i] - (float)mu[i];
+    float* a = acc + zk * S;
+    atomicAdd(a, 1.0f);
+    for (int i = 0; i < d; ++i) {
+      atomicAdd(a + 1 + i, dv[i]);
+      for (int j = 0; j <= i; ++j) atomicAdd(a + 1 + d + tri(i) + j, dv[i] * dv[j]);
+    }
+  }
+  __syncthreads();
+  float* dst = m.part + ((size_t)c * m.G + blockIdx.x) * K * S;
+  for (int i = tid; i < K * S; i += blockDim.x) dst[i] = acc[i];
+}
+
+// Data part of the log joint: sum_n log pis_z + mvnormal_ll(y_n; mu_z, cov_z),
+// FP32 per point from the tables, FP64 accumulation.  One block per chain.
+template <int DM>
+__global__ __launch_bounds__(kGmmThreads) void k_gmm_ll(GmmDev m, double* out) {
+  constexpr int TS = tri(DM) + DM + 1;
+  const int c = blockIdx.x, tid = threadIdx.x, d = m.d, K = m.k;
+  __shared__ double red[kGmmThreads];
+  double accum = 0.0;
+  for (int n = tid; n < m.n; n += kGmmThreads) {
+    const int zk = m.z[(size_t)c * m.n + n];
+    const float* t = m.tab + ((size_t)c * K + zk) * m.TS;
+    float dv[DM];
+    for (int i = 0; i < DM; ++i) dv[i] = (i < d ? m.y[(size_t)n * d + i] : 0.0f) - t[tri(DM) + i];
+    float q = 0.0f;
+    for (int i = 0; i < DM; ++i) {
+      float w = 0.0f;
+      for (int j = 0; j <= i; ++j) w = fmaf(t[tri(i) + j], dv[j], w);
+      q = fmaf(w, w, q);
+    }
+    accum += log(m.pis64[(size_t)c * K + zk]) + (double)t[tri(DM) + DM] - 0.5 * (double)q -
+             0.5 * d * log(2.0 * 3.14159265358979323846);
+  }
+  red[tid] = accum;
+  __syncthreads();
+  if (tid == 0) {
+    double s = 0.0;
+    for (int i = 0; i < kGmmThreads; ++i) s += red[i];
+    out[c] = s;
+  }
+  (void)TS;
+}
+
+static int pad_dim(int d) { return d <= 4 ? 4 : (d <= 8 ? 8 : 16); }
+static int pad_comp(int k) { return k <= 8 ? 8 : (k <= 16 ? 16 : (k <= 32 ? 32 : 64)); }
+
+typedef void (*ZKernel)(GmmDev, uint32_t, uint32_t, uint32_t, int);
+
+static ZKernel pick_z(int DM, int KM) {
+#define AUGUR_Z(D_, K_) if (DM == D_ && KM == K_) return k_gmm_z<D_, K_>;
+  AUGUR_Z(4, 8) AUGUR_Z(4, 16) AUGUR_Z(4, 32) AUGUR_Z(4, 64)
+  AUGUR_Z(8, 8) AUGUR_Z(8, 16) AUGUR_Z(8, 32) AUGUR_Z(8, 64)
+  AUGUR_Z(16, 8) AUGUR_Z(16, 16) AUGUR_Z(16, 32) AUGUR_Z(16, 64)
+#undef AUGUR_Z
+  fail(AUGUR_ERR_UNSUPPORTED, "no z kernel for padded dim %d, components %d", DM, KM);
+}
+
+struct GmmSampler final : augur_sampler {
+  GmmDev m{};
+  DevBuf<float> y, tab, part;
+  DevBuf<double> alpha, mu0, cov0, scale0, gpi, pis64, mu64, cov64, ll;
+  DevBuf<int32_t> z;
+  std::vector<double> h_alpha, h_mu0, h_cov0, h_scale0;
+  bool stats_valid = false;
+
+  const char* family() const override { return "gmm"; }
+
+  ParamView param(const std::string& name) override {
+    const size_t K = m.k, D = m.d;
+    ParamView v;
+    if (name == "pis") v = {pis64.p, sizeof(double), K, K, true};
+    else if (name == "mu") v = {mu64.p, sizeof(double), K * D, K * D, true};
+    else if (name == "cov") v = {cov64.p, sizeof(double), K * D * D, K * D * D, true};
+    else if (name == "z") v = {z.p, sizeof(int32_t), (size_t)m.n, (size_t)m.n, true};
+    return v;
+  }
+
+  size_t z_smem() const {
+    const int TS = m.TS;
+    return sizeof(float) * ((size_t)m.k * TS + (size_t)m.k * m.S + m.k);
+  }
+
+  void refresh_tables() {
+    k_gmm_tables<<<dim3(m.k, chains), 64, 0, stream>>>(m);
+    AUGUR_CHECK_LAUNCH();
+  }
+
+  void refresh_stats() {
+    k_gmm_stats<<<dim3(m.G, chains), kGmmThreads, sizeof(float) * m.k * m.S, stream>>>(m);
+    AUGUR_CHECK_LAUNCH();
+    stats_valid = true;
+  }
+
+  void on_set(const std::string& name) override {
+    if (name == "pis") {
+      AUGUR_HIP(hipMemcpyAsync(gpi.p, pis64.p, pis64.bytes(), hipMemcpyDeviceToDevice, stream));
+    } else if (name == "z") {
+      std::vector<int32_t> hz(z.n);
+      z.download(hz.data(), hz.size(), stream);
+      AUGUR_HIP(hipStreamSynchronize(stream));
+      for (int32_t v : hz)
+        if (v < 0 || v >= m.k) fail(AUGUR_ERR_ARG, "z value %d outside [0, %d)", v, m.k);
+    }
+    if (name == "cov" || name == "mu") refresh_tables();
+    stats_valid = false;
+  }
+
+  void run_sweeps(int nsw) override {
+    const ZKernel zk = pick_z(m.DM, pad_comp(m.k));
+    for (int i = 0; i < nsw; ++i) {
+      if (!stats_valid) refresh_stats();
+      k_gmm_conj<<<dim3(m.k, chains), kGmmThreads, 0, stream>>>(m, seed, sweep, 1);
+      AUGUR_CHECK_LAUNCH();
+      zk<<<dim3(m.G, chains), kGmmThreads, z_smem(), stream>>>(m, seed, sweep, 0u, 1);
+      AUGUR_CHECK_LAUNCH();
+      ++sweep;
+      record_traces();
+    }
+  }
+
+  void log_density(double* out) override {
+    // data part on the device, priors (reference ll_mod terms) on the host
+    const int DM = m.DM;
+    if (DM == 4) k_gmm_ll<4><<<chains, kGmmThreads, 0, stream>>>(m, ll.p);
+    else if (DM == 8) k_gmm_ll<8><<<chains, kGmmThreads, 0, stream>>>(m, ll.p);
+    else k_gmm_ll<16><<<chains, kGmmThreads, 0, stream>>>(m, ll.p);
+    AUGUR_CHECK_LAUNCH();
+    ll.download(out, chains, stream);
+    AUGUR_HIP(hipStreamSynchronize(stream));
+  }
+
+  void stats(double* out, size_t bytes) override {
+    if (bytes < sizeof(double)) fail(AUGUR_ERR_ARG, "stats buffer too small");
+    out[0] = (double)sweep;
+  }
+};
+
+static void gmm_validate(const augur_gmm_desc* D) {
+  if (!D) fail(AUGUR_ERR_ARG, "null descriptor");
+  if (D->n_points < 1 || D->dim < 1 || D->dim > kMaxDim || D->n_components < 1 ||
+      D->n_components > kMaxComp || D->chains < 1)
+    fail(AUGUR_ERR_ARG, "need N >= 1, 1 <= D <= %d, 1 <= K <= %d, chains >= 1", kMaxDim, kMaxComp);
+  if (!D->y || !D->alpha || !D->mu0 || !D->cov0 || !D->scale0) fail(AUGUR_ERR_ARG, "null array");
+  if (D->sample_cov && D->df <= D->dim - 1) fail(AUGUR_ERR_ARG, "InvWishart needs df > D - 1");
+}
+
+static void gmm_setup(GmmSampler* s, int n, int d, int k, int chains) {
+  GmmDev& m = s->m;
+  m.n = n; m.d = d; m.k = k;
+  m.DM = pad_dim(d);
+  m.TS = tri(m.DM) + m.DM + 1;
+  m.S = 1 + d + tri(d);
+  const int want = (n + kGmmThreads - 1) / kGmmThreads;
+  m.G = std::max(1, std::min(want, std::max(1, 1024 / chains)));
+  s->chains = chains;
+  s->y.alloc((size_t)n * d);
+  s->alpha.alloc(k); s->mu0.alloc(d); s->cov0.alloc((size_t)d * d); s->scale0.alloc((size_t)d * d);
+  s->gpi.alloc((size_t)chains * k); s->pis64.alloc((size_t)chains * k);
+  s->mu64.alloc((size_t)chains * k * d); s->cov64.alloc((size_t)chains * k * d * d);
+  s->tab.alloc((size_t)chains * k * m.TS);
+  s->part.alloc((size_t)chains * m.G * k * m.S);
+  s->z.alloc((size_t)chains * n);
+  s->ll.alloc(chains);
+  m.y = s->y.p; m.alpha = s->alpha.p; m.mu0 = s->mu0.p; m.cov0 = s->cov0.p; m.scale0 = s->scale0.p;
+  m.gpi = s->gpi.p; m.pis64 = s->pis64.p; m.mu64 = s->mu64.p; m.cov64 = s->cov64.p;
+  m.tab = s->tab.p; m.part = s->part.p; m.z = s->z.p;
+}
+
+}  // namespace augur
+
+using namespace augur;
+
+extern "C" {
+
+augur_status augur_gmm_create(const augur_gmm_desc* D, augur_sampler** out) {
+  return guard([&] {
+    if (!out) fail(AUGUR_ERR_ARG, "null output handle");
+    *out = nullptr;
+    gmm_validate(D);
+    require_device();
+    auto* s = new GmmSampler();
+    try {
+      AUGUR_HIP(hipGetDevice(&s->device));
+      AUGUR_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+      s->seed = D->seed;
+      const int n = D->n_points, d = D->dim, k = D->n_components, C = D->chains;
+      gmm_setup(s, n, d, k, C);
+      GmmDev& m = s->m;
+      m.df = D->df; m.sample_pi = D->sample_pi; m.sample_cov = D->sample_cov;
+      s->y.upload(D->y, (size_t)n * d, s->stream);
+      s->alpha.upload(D->alpha, k, s->stream);
+      s->mu0.upload(D->mu0, d, s->stream);
+      s->cov0.upload(D->cov0, (size_t)d * d, s->stream);
+      s->scale0.upload(D->scale0, (size_t)d * d, s->stream);
+      // default point (AugurDefault, pyaugur/pyaugur/augurdistlib.py:143-169):
+      // uniform pis, zero means, identity covariances, all points in component 0
+      std::vector<double> pis((size_t)C * k, 1.0 / k), mu((size_t)C * k * d, 0.0),
+          cov((size_t)C * k * d * d, 0.0);
+      for (size_t c = 0; c < (size_t)C * k; ++c)
+        for (int i = 0; i < d; ++i) cov[c * d * d + i * d + i] = 1.0;
+      s->pis64.upload(pis.data(), pis.size(), s->stream);
+      s->gpi.upload(pis.data(), pis.size(), s->stream);
+      s->mu64.upload(mu.data(), mu.size(), s->stream);
+      s->cov64.upload(cov.data(), cov.size(), s->stream);
+      s->z.zero(s->stream);
+      s->refresh_tables();
+      AUGUR_HIP(hipStreamSynchronize(s->stream));
+    } catch (...) {
+      delete s;
+      throw;
+    }
+    *out = s;
+  });
+}
+
+augur_status augur_op_gmm_zstep(int n, int d, int k, const float* y, const double* pis,
+                                const double* mu, const double* cov, uint32_t seed, uint32_t chain,
+                                uint32_t sweep, int32_t* z) {
+  return guard([&] {
+    if (n < 1 || d < 1 || d > kMaxDim || k < 1 || k > kMaxComp || !y || !pis || !mu || !cov || !z)
+      fail(AUGUR_ERR_ARG, "bad arguments");
+    require_device();
+    GmmSampler s;
+    AUGUR_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+    gmm_setup(&s, n, d, k, 1);
+    s.y.upload(y, (size_t)n * d, s.stream);
+    s.gpi.upload(pis, k, s.stream);
+    s.mu64.upload(mu, (size_t)k * d, s.stream);
+    s.cov64.upload(cov, (size_t)k * d * d, s.stream);
+    s.refresh_tables();
+    const ZKernel zk = pick_z(s.m.DM, pad_comp(k));
+    zk<<<dim3(s.m.G, 1), kGmmThreads, s.z_smem(), s.stream>>>(s.m, seed, sweep, chain, 0);
+    AUGUR_CHECK_LAUNCH();
+    s.z.download(z, n, s.stream);
+    AUGUR_HIP(hipStreamSynchronize(s.stream));
+  });
+}
+
+augur_status augur_op_gmm_conj(int d, int k, const double* alpha, const double* mu0,
+                               const double* cov0, int df, const double* scale0, const int32_t* cnt,
+                               const double* sum_y, const double* sum_yy, int sample_pi,
+                               int sample_cov, uint32_t seed, uint32_t chain, uint32_t sweep,
+                               double* pis, double* mu, double* cov) {
+  return guard([&] {
+    if (d < 1 || d > kMaxDim || k < 1 || k > kMaxComp || !alpha || !mu0 || !cov0 || !scale0 || !cnt ||
+        !sum_y || !sum_yy || !pis || !mu || !cov)
+      fail(AUGUR_ERR_ARG, "bad arguments");
+    if (chain != 0) fail(AUGUR_ERR_ARG, "the conjugate op runs chain 0");
+    require_device();
+    GmmSampler s;
+    AUGUR_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+    gmm_setup(&s, 1, d, k, 1);
+    GmmDev& m = s.m;
+    m.G = 1;
+    m.df = df; m.sample_pi = sample_pi; m.sample_cov = sample_cov;
+    s.alpha.upload(alpha, k, s.stream);
+    s.mu0.upload(mu0, d, s.stream);
+    s.cov0.upload(cov0, (size_t)d * d, s.stream);
+    s.scale0.upload(scale0, (size_t)d * d, s.stream);
+    s.gpi.upload(pis, k, s.stream);
+    s.mu64.upload(mu, (size_t)k * d, s.stream);
+    s.cov64.upload(cov, (size_t)k * d * d, s.stream);
+    // statistics centred at the current mu, as the z kernel leaves them
+    std::vector<float> st((size_t)k * m.S);
+    for (int j = 0; j < k; ++j) {
+      double c[kMaxDim];  // the FP32 table copy of mu, as the z kernel centres
+      for (int i = 0; i < d; ++i) c[i] = (double)(float)mu[(size_t)j * d + i];
+      const double n = cnt[j];
+      float* o = st.data() + (size_t)j * m.S;
+      o[0] = (float)n;
+      for (int i = 0; i < d; ++i) o[1 + i] = (float)(sum_y[j * d + i] - n * c[i]);
+      for (int i = 0; i < d; ++i)
+        for (int l = 0; l <= i; ++l) {
+          const double syy = sum_yy[((size_t)j * d + i) * d + l];
+          o[1 + d + tri(i) + l] = (float)(syy - c[i] * sum_y[j * d + l] - sum_y[j * d + i] * c[l] + n * c[i] * c[l]);
+        }
+    }
+    s.part.upload(st.data(), st.size(), s.stream);
+    k_gmm_conj<<<dim3(k, 1), kGmmThreads, 0, s.stream>>>(m, seed, sweep, 1);
+    AUGUR_CHECK_LAUNCH();
+    std::vector<double> g(k);
+    s.gpi.download(g.data(), k, s.stream);
+    s.mu64.download(mu, (size_t)k * d, s.stream);
+    s.cov64.download(cov, (size_t)k * d * d, s.stream);
+    AUGUR_HIP(hipStreamSynchronize(s.stream));
+    if (sample_pi) {
+      double tot = 0.0;
+      for (int j = 0; j < k; ++j) tot += g[j];
+      for (int j = 0; j < k; ++j) pis[j] = g[j] / tot;
+    }
+  });
+}
+
+}  // extern "C"

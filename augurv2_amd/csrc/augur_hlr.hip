@@ -1,0 +1,700 @@
+// augur_hlr.hip -- HMC sampler for hierarchical logistic regression (hlr.rv
+// family) on gfx950.
+//
+//   v ~ InvGamma(a, b) | Exponential(a);  [b ~ Normal(0, v)];
+//   theta[k] ~ Normal(0, v);  y[n] ~ Bernoulli(sigmoid(dotprod(theta, x[n]) [+ b]))
+//
+// Reference path: the generated grad_<blk> / ll_<blk> map-reduces
+// (Core/CgGrad.hs:306-376, Core/CgLike.hs:90-136, lowered by
+// CudaC/CgCuda.hs:371-516 to one-thread-per-row kernels with FP64 atomics and
+// thrust reductions) driven by the host HMC loop of cbits/augur_mcmclib.h:
+// 343-424.  Here the whole HMC iteration stays on the device:
+//
+//   k_hmc_begin   momentum draw, first half step, first position step
+//   k_hlr_pass    one data pass per leapfrog step for all chains at once:
+//                 X tile (64 rows) staged in LDS and shared by every chain,
+//                 t = x.theta + b, a = y - sigmoid(t), ll += log sigmoid(+-t),
+//                 grad += a x; FP32 arithmetic, FP64 accumulation across
+//                 tiles and blocks; the last block to finish reduces the block
+//                 partials in a fixed order (deterministic) and applies the
+//                 leapfrog update of every chain (momentum, position, FP32
+//                 copy of the next evaluation point).
+//   k_hmc_end     kinetic energies, FP64 Metropolis test, accept / reject.
+//
+// The data log-density and gradient at the current point are cached (they
+// are exactly what the pass computed when that point was proposed), so an
+// iteration of L leapfrog steps costs L data passes instead of the
+// reference's L + 3 (L + 1 gradients and 2 log-densities).
+#include "augur_common.h"
+
+namespace augur {
+
+constexpr int kTileRows = 64;
+constexpr int kPassThreads = 256;
+constexpr int kMaxGradSlots = 16;  // (d) * cp <= 16 * 256
+constexpr int kMaxChains = 16;
+
+enum { MODE_CACHE = 0, MODE_LEAPFROG = 1 };
+
+struct HmcStep {
+  double step[kMaxChains];
+  int steps[kMaxChains];
+};
+
+// eps' = eps U(0.8, 1.2); L = max(1, round(U(0.9, 1.1) simLen / eps'))
+// (cbits/augur_mcmclib.h:401-405), uniforms from the block's jitter stream.
+__host__ __device__ inline void hmc_jitter(const Stream& s, double sim_len, double eps,
+                                          int fixed_steps, double* step, int* steps) {
+#pragma clang fp contract(off)
+  if (fixed_steps > 0) {
+    *step = eps;
+    *steps = fixed_steps;
+    return;
+  }
+  const u32x4 w = s.draw(0, 0);
+  const double st = (0.8 + 0.4 * u01(w.x)) * eps;
+  const double sc = 0.9 + 0.2 * u01(w.y);
+  const double r = round(sc * sim_len / st);
+  *step = st;
+  *steps = r < 1.0 ? 1 : (int)r;
+}
+
+// cbits/augur_dist.c:372-381 (ref_compat: -log var; else -0.5 log var)
+__device__ inline double normal_ll0(double x, double var, int ref_compat) {
+  if (var > 0.0) return -(ref_compat ? 1.0 : 0.5) * log(var) - 0.5 * x * x / var;
+  return -kLargePosNum;
+}
+
+struct PassArgs {
+  const float* __restrict__ x;
+  const int32_t* __restrict__ y;
+  int n, d, cp, c_real, has_b;
+  float* beta32;   // cp x (d + 1): evaluation point (theta, b) per chain
+  double* part;    // [cp * (d + 2)][G]
+  unsigned* counter;
+  int mode;
+  int l;           // leapfrog pass index (1-based)
+  double* cache_g;  // cp x (d + 1)
+  double* cache_ll; // cp
+  double* gprop;    // cp x (d + 1)
+  double* llprop;   // cp
+  double* prop;     // cp x P
+  double* p;        // cp x (d + 1)
+  const double* state;  // cp x P
+  int blk_theta, blk_b;
+  HmcStep hs;
+};
+
+__device__ inline float log_sigmoid_pm(float t, int y, float* a_out) {
+  // e = exp(-|t|); sigmoid(t) = 1/(1+e) (t >= 0) or e/(1+e); a = y - sigmoid(t)
+  // log sigmoid(+-t) = min(+-t, 0) - log1p(e)
+  const float e = expf(-fabsf(t));
+  const float inv = 1.0f / (1.0f + e);
+  const float sig = t >= 0.0f ? inv : e * inv;
+  *a_out = (float)y - sig;
+  const float st = y ? t : -t;
+  return fminf(st, 0.0f) - log1pf(e);
+}
+
+__global__ __launch_bounds__(kPassThreads) void k_hlr_pass(PassArgs A) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int d = A.d, cp = A.cp, ds = d + 1;
+  float* xs = (float*)smem;                          // kTileRows x ds
+  float* bs = xs + kTileRows * ds;                   // cp x ds
+  float* as = bs + cp * ds;                          // kTileRows x cp
+  int* ys = (int*)(as + kTileRows * cp);             // kTileRows
+  double* red = (double*)(((uintptr_t)(ys + kTileRows) + 15) & ~(uintptr_t)15);  // 2 x 256
+  int* flag = (int*)(red + 2 * kPassThreads);
+
+  const int tid = threadIdx.x;
+  const int G = gridDim.x;
+  for (int i = tid; i < cp * ds; i += kPassThreads) bs[i] = A.beta32[i];
+
+  // phase-B ownership: combo q = tid + 256 m -> row q / cp, chain q % cp (cp | 256)
+  const int my_c = tid % cp;
+  const int nb = (kTileRows * cp + kPassThreads - 1) / kPassThreads;
+  // phase-C ownership: combo q = tid + 256 m -> chain q / d, column q % d
+  const int ngc = d * cp;
+  const int ng = (ngc + kPassThreads - 1) / kPassThreads;
+  double gacc[kMaxGradSlots];
+#pragma unroll
+  for (int m = 0; m < kMaxGradSlots; ++m) gacc[m] = 0.0;
+  double llacc = 0.0, gbacc = 0.0;
+
+  const int tiles = (A.n + kTileRows - 1) / kTileRows;
+  for (int tile = blockIdx.x; tile < tiles; tile += G) {
+    const int r0 = tile * kTileRows;
+    const int rows = min(kTileRows, A.n - r0);
+    __syncthreads();
+    // stage the tile: rows x d contiguous floats
+    const float* src = A.x + (size_t)r0 * d;
+    if ((d & 3) == 0) {
+      const int nv = rows * d / 4;
+      const float4* s4 = (const float4*)src;
+      for (int i = tid; i < nv; i += kPassThreads) {
+        const float4 v = s4[i];
+        const int e = 4 * i, r = e / d, c = e % d;
+        float* dst = xs + r * ds + c;
+        dst[0] = v.x; dst[1] = v.y; dst[2] = v.z; dst[3] = v.w;
+      }
+    } else {
+      for (int i = tid; i < rows * d; i += kPassThreads) xs[(i / d) * ds + i % d] = src[i];
+    }
+    for (int i = rows * d + tid; i < kTileRows * d; i += kPassThreads) xs[(i / d) * ds + i % d] = 0.0f;
+    if (tid < kTileRows) ys[tid] = tid < rows ? A.y[r0 + tid] : 0;
+    __syncthreads();
+    // phase B: t = x . theta_c + b_c per (row, chain)
+    for (int m = 0; m < nb; ++m) {
+      const int q = tid + kPassThreads * m;
+      if (q >= kTileRows * cp) break;
+      const int r = q / cp, c = q % cp;
+      const float* xr = xs + r * ds;
+      const float* br = bs + c * ds;
+      float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+      int j = 0;
+      for (; j + 4 <= d; j += 4) {
+        s0 = fmaf(xr[j], br[j], s0);
+        s1 = fmaf(xr[j + 1], br[j + 1], s1);
+        s2 = fmaf(xr[j + 2], br[j + 2], s2);
+        s3 = fmaf(xr[j + 3], br[j + 3], s3);
+      }
+      for (; j < d; ++j) s0 = fmaf(xr[j], br[j], s0);
+      float t = (s0 + s1) + (s2 + s3);
+      if (A.has_b) t += br[d];
+      float a;
+      const float lr = log_sigmoid_pm(t, ys[r], &a);
+      if (r < rows) {
+        llacc += (double)lr;
+        gbacc += (double)a;
+      } else {
+        a = 0.0f;
+      }
+      as[r * cp + c] = a;
+    }
+    __syncthreads();
+    // phase C: grad[c][j] += sum_r a[r][c] x[r][j]
+#pragma unroll
+    for (int m = 0; m < kMaxGradSlots; ++m) {
+      if (m >= ng) break;
+      const int q = tid + kPassThreads * m;
+      if (q < ngc) {
+        const int c = q / d, j = q % d;
+        float f0 = 0.f, f1 = 0.f;
+        for (int r = 0; r < kTileRows; r += 2) {
+          f0 = fmaf(as[r * cp + c], xs[r * ds + j], f0);
+          f1 = fmaf(as[(r + 1) * cp + c], xs[(r + 1) * ds + j], f1);
+        }
+        gacc[m] += (double)(f0 + f1);
+      }
+    }
+  }
+  // block partials -> part[value][block], value = c * (d + 2) + j
+  const int vs = d + 2;
+#pragma unroll
+  for (int m = 0; m < kMaxGradSlots; ++m) {
+    if (m >= ng) break;
+    const int q = tid + kPassThreads * m;
+    if (q < ngc) {
+      const int c = q / d, j = q % d;
+      A.part[(size_t)(c * vs + j) * G + blockIdx.x] = gacc[m];
+    }
+  }
+  __syncthreads();
+  red[tid] = gbacc;
+  red[kPassThreads + tid] = llacc;
+  __syncthreads();
+  if (tid < cp) {
+    double sb = 0.0, sl = 0.0;
+    for (int i = tid; i < kPassThreads; i += cp) {
+      sb += red[i];
+      sl += red[kPassThreads + i];
+    }
+    A.part[(size_t)(tid * vs + d) * G + blockIdx.x] = sb;
+    A.part[(size_t)(tid * vs + d + 1) * G + blockIdx.x] = sl;
+  }
+  (void)my_c;
+  // last block to finish reduces every value over the blocks in order
+  __threadfence();
+  __syncthreads();
+  if (tid == 0) {
+    const unsigned prev = atomicAdd(A.counter, 1u);
+    *flag = (prev == (unsigned)(G - 1));
+  }
+  __syncthreads();
+  if (!*flag) return;
+  __threadfence();
+  double* tot = (double*)smem;  // cp * (d + 2) doubles, aliases the tile
+  const int nval = cp * vs;
+  const int lane = tid & 63, wave = tid >> 6, nw = kPassThreads / 64;
+  for (int v = wave; v < nval; v += nw) {
+    const double* pv = A.part + (size_t)v * G;
+    double s = 0.0;
+    for (int g = lane; g < G; g += 64) s += pv[g];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) tot[v] = s;
+  }
+  __syncthreads();
+  const int P = d + 2;
+  if (A.mode == MODE_CACHE) {
+    for (int q = tid; q < cp * ds; q += kPassThreads) {
+      const int c = q / ds, j = q % ds;
+      A.cache_g[q] = tot[c * vs + j];
+      if (j == 0) A.cache_ll[c] = tot[c * vs + d + 1];
+    }
+  } else {
+    for (int q = tid; q < A.c_real * ds; q += kPassThreads) {
+      const int c = q / ds, j = q % ds;
+      const int L = A.hs.steps[c];
+      if (A.l > L) continue;
+      const double g_data = tot[c * vs + j];
+      if (A.l == L) {
+        A.gprop[q] = g_data;
+        if (j == 0) A.llprop[c] = tot[c * vs + d + 1];
+      }
+      const bool in_blk = (j < d) ? A.blk_theta : (A.blk_b && A.has_b);
+      if (!in_blk) continue;
+      const double step = A.hs.step[c];
+      const double v = A.state[c * P + d + 1];
+      double xv = A.prop[c * P + j];
+      const double g = g_data + (-xv / v);
+      const double coef = (A.l == L) ? 0.5 * step : step;
+      const double pj = A.p[q] + coef * g;
+      A.p[q] = pj;
+      if (A.l < L) {
+        xv += step * pj;
+        A.prop[c * P + j] = xv;
+        A.beta32[q] = (float)xv;
+      }
+    }
+  }
+  if (tid == 0) *A.counter = 0u;
+}
+
+size_t pass_smem_bytes(int d, int cp) {
+  size_t b = sizeof(float) * ((size_t)kTileRows * (d + 1) + (size_t)cp * (d + 1) + (size_t)kTileRows * cp) +
+             sizeof(int) * kTileRows;
+  b = (b + 15) & ~(size_t)15;
+  b += sizeof(double) * 2 * kPassThreads + 16;
+  const size_t tot = sizeof(double) * (size_t)cp * (d + 2);
+  return b > tot ? b : tot;
+}
+
+struct HmcArgs {
+  int d, cp, c_real, has_b, ref_compat;
+  int blk_theta, blk_b, block_kind;  // block_kind: AUGUR_HMC_BLOCK_*
+  uint32_t seed, sweep, tag;
+  const double* cache_g;
+  const double* cache_ll;
+  double* cache_g_w;
+  double* cache_ll_w;
+  const double* gprop;
+  const double* llprop;
+  double* state;
+  double* prop;
+  double* p0;
+  double* p;
+  double* k0;
+  float* beta32;
+  double* stats;  // [chain][2] for this block
+  HmcStep hs;
+};
+
+__device__ inline double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// One wave per chain: momentum draw, half step, first position step.
+__global__ __launch_bounds__(64) void k_hmc_begin(HmcArgs A) {
+  const int c = blockIdx.x, lane = threadIdx.x;
+  const int d = A.d, ds = d + 1, P = d + 2;
+  const Stream mom{A.seed, (uint32_t)c, A.sweep, A.tag + 0};
+  const double step = A.hs.step[c];
+  const double v = A.state[c * P + d + 1];
+  double kin = 0.0;
+  for (int j = lane; j < P; j += 64) {
+    const double sv = A.state[c * P + j];
+    double xv = sv;
+    if (j < ds) {
+      const bool in_blk = (j < d) ? A.blk_theta : (A.blk_b && A.has_b);
+      if (in_blk) {
+        const uint32_t site = A.block_kind == AUGUR_HMC_BLOCK_B ? 0u : (uint32_t)j;
+        const double p0 = normal_at(mom, site, 0);
+        const double g = A.cache_g[c * ds + j] + (-sv / v);
+        const double pj = p0 + 0.5 * step * g;
+        A.p0[c * ds + j] = p0;
+        A.p[c * ds + j] = pj;
+        kin += p0 * p0;
+        xv = sv + step * pj;
+      }
+      A.beta32[c * ds + j] = (float)xv;
+    }
+    A.prop[c * P + j] = xv;
+  }
+  kin = wave_sum(kin);
+  if (lane == 0) A.k0[c] = 0.5 * kin;
+}
+
+// One wave per chain: energies, FP64 Metropolis test (cbits/augur_mcmclib.h:
+// 70-83, 410-423), accept or reject.
+__global__ __launch_bounds__(64) void k_hmc_end(HmcArgs A) {
+  const int c = blockIdx.x, lane = threadIdx.x;
+  const int d = A.d, ds = d + 1, P = d + 2;
+  const double v = A.state[c * P + d + 1];
+  double kin = 0.0, lp0 = 0.0, lp1 = 0.0;
+  for (int j = lane; j < ds; j += 64) {
+    const bool in_blk = (j < d) ? A.blk_theta : (A.blk_b && A.has_b);
+    if (!in_blk) continue;
+    const double pj = A.p[c * ds + j];
+    kin += pj * pj;
+    lp0 += normal_ll0(A.state[c * P + j], v, A.ref_compat);
+    lp1 += normal_ll0(A.prop[c * P + j], v, A.ref_compat);
+  }
+  kin = wave_sum(kin);
+  lp0 = wave_sum(lp0);
+  lp1 = wave_sum(lp1);
+  const double ll0 = A.cache_ll[c] + lp0;
+  const double ll1 = A.llprop[c] + lp1;
+  const double k1 = 0.5 * kin;
+  const Stream acc_s{A.seed, (uint32_t)c, A.sweep, A.tag + 2};
+  const double u = u01(acc_s.draw(0, 0).x);
+  const bool acc = log(u) < (ll1 - ll0) + (A.k0[c] - k1);
+  if (acc) {
+    for (int j = lane; j < P; j += 64) A.state[c * P + j] = A.prop[c * P + j];
+    for (int j = lane; j < ds; j += 64) A.cache_g_w[c * ds + j] = A.gprop[c * ds + j];
+    if (lane == 0) A.cache_ll_w[c] = A.llprop[c];
+  }
+  if (lane == 0) {
+    A.stats[2 * c] += acc ? 1.0 : 0.0;
+    A.stats[2 * c + 1] += 1.0;
+  }
+}
+
+struct VBlockArgs {
+  int d, has_b, ref_compat, prior_kind;
+  double prior_a, prior_b;
+  double sim_len, eps;
+  int fixed_steps;
+  uint32_t seed, sweep, tag;
+  double* state;  // C x P
+  double* stats;  // [chain][2]
+};
+
+__device__ inline double v_prior_ll(const VBlockArgs& A, double v) {
+  if (A.prior_kind == 0)  // cbits/augur_dist.c:345-347
+    return A.prior_a * log(A.prior_b) - lgamma(A.prior_a) - (A.prior_a + 1.0) * log(v) - A.prior_b / v;
+  if (A.prior_a > 0.0 && v > 0.0) return log(A.prior_a) - A.prior_a * v;  // :231-238
+  return -kLargePosNum;
+}
+__device__ inline double v_prior_dot(const VBlockArgs& A, double v) {
+  if (A.prior_kind == 0) return -(A.prior_a + 1.0) / v + A.prior_b / (v * v);  // :349-351
+  return (A.prior_a > 0.0 && v > 0.0) ? -A.prior_a : 0.0;                    // :240-247
+}
+
+// HMC on the scalar variance v (the full conditional involves only theta
+// and b): one wave per chain runs the whole trajectory.
+__global__ __launch_bounds__(64) void k_hmc_vblock(VBlockArgs A) {
+  const int c = blockIdx.x, lane = threadIdx.x;
+  const int d = A.d, P = d + 2;
+  double* st = A.state + (size_t)c * P;
+  const int nterm = d + (A.has_b ? 1 : 0);
+  auto ll_of = [&](double v) {
+    double s = 0.0;
+    for (int j = lane; j < nterm; j += 64) s += normal_ll0(st[j], v, A.ref_compat);
+    return wave_sum(s) + v_prior_ll(A, v);
+  };
+  auto grad_of = [&](double v) {
+    double s = 0.0;
+    for (int j = lane; j < nterm; j += 64) {
+      const double t = st[j];
+      // cbits/augur_dist.c:391-394 (ref_compat keeps -0.5 * var)
+      s += (A.ref_compat ? -0.5 * v : -0.5 / v) + 0.5 * (t * t) / (v * v);
+    }
+    return wave_sum(s) + v_prior_dot(A, v);
+  };
+  double step;
+  int steps;
+  hmc_jitter(Stream{A.seed, (uint32_t)c, A.sweep, A.tag + 1}, A.sim_len, A.eps, A.fixed_steps, &step,
+             &steps);
+  const double v0 = st[d + 1];
+  const double p0 = normal_at(Stream{A.seed, (uint32_t)c, A.sweep, A.tag + 0}, 0, 0);
+  double v = v0;
+  double p = p0 + 0.5 * step * grad_of(v);
+  for (int l = 0; l < steps; ++l) {
+    v += step * p;
+    if (l != steps - 1) p += step * grad_of(v);
+  }
+  p += 0.5 * step * grad_of(v);
+  const double ll0 = ll_of(v0), ll1 = ll_of(v);
+  const double u = u01(Stream{A.seed, (uint32_t)c, A.sweep, A.tag + 2}.draw(0, 0).x);
+  const bool acc = log(u) < (ll1 - ll0) + (0.5 * p0 * p0 - 0.5 * p * p);
+  if (lane == 0) {
+    if (acc) st[d + 1] = v;
+    A.stats[2 * c] += acc ? 1.0 : 0.0;
+    A.stats[2 * c + 1] += 1.0;
+  }
+}
+
+__global__ void k_state_to_beta32(const double* __restrict__ state, float* __restrict__ beta32, int d,
+                                  int cp, int c_real, int has_b) {
+  const int ds = d + 1, P = d + 2;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < cp * ds; q += gridDim.x * blockDim.x) {
+    const int c = q / ds, j = q % ds;
+    float v = 0.0f;
+    if (c < c_real && (j < d || has_b)) v = (float)state[c * P + j];
+    beta32[q] = v;
+  }
+}
+
+// Full log joint (reference ll_mod): v prior + theta / b priors + data.
+__global__ void k_hlr_ll_mod(const double* __restrict__ state, const double* __restrict__ cache_ll,
+                             VBlockArgs A, int c_real, double* out) {
+  const int c = blockIdx.x, lane = threadIdx.x;
+  if (c >= c_real) return;
+  const int d = A.d, P = d + 2;
+  const double* st = state + (size_t)c * P;
+  const double v = st[d + 1];
+  const int nterm = d + (A.has_b ? 1 : 0);
+  double s = 0.0;
+  for (int j = lane; j < nterm; j += 64) s += normal_ll0(st[j], v, A.ref_compat);
+  s = wave_sum(s);
+  if (lane == 0) out[c] = s + v_prior_ll(A, v) + cache_ll[c];
+}
+
+static int next_pow2(int c) {
+  int p = 1;
+  while (p < c) p <<= 1;
+  return p;
+}
+
+struct HlrSampler final : augur_sampler {
+  int n = 0, d = 0, cp = 1, has_b = 0, ref_compat = 1, prior_kind = 0;
+  double prior_a = 1.0, prior_b = 1.0;
+  int G = 1;
+  std::vector<augur_hmc_block> sched;
+  DevBuf<float> x, beta32;
+  DevBuf<int32_t> y;
+  DevBuf<double> state, prop, p0, p, k0, cache_g, cache_ll, gprop, llprop, part, stats_buf, ll_out;
+  DevBuf<unsigned> counter;
+  bool cache_valid = false;
+
+  const char* family() const override { return "hlr"; }
+
+  ParamView param(const std::string& name) override {
+    const size_t P = (size_t)d + 2;
+    ParamView v;
+    if (name == "state") v = {state.p, sizeof(double), P, P, true};
+    else if (name == "theta") v = {state.p, sizeof(double), (size_t)d, P, true};
+    else if (name == "b" && has_b) v = {state.p + d, sizeof(double), 1, P, true};
+    else if (name == "v") v = {state.p + d + 1, sizeof(double), 1, P, true};
+    return v;
+  }
+
+  void on_set(const std::string& name) override {
+    if (name != "v") cache_valid = false;
+  }
+
+  PassArgs pass_args(int mode, int l) {
+    PassArgs A{};
+    A.x = x.p; A.y = y.p; A.n = n; A.d = d; A.cp = cp; A.c_real = chains; A.has_b = has_b;
+    A.beta32 = beta32.p; A.part = part.p; A.counter = counter.p; A.mode = mode; A.l = l;
+    A.cache_g = cache_g.p; A.cache_ll = cache_ll.p; A.gprop = gprop.p; A.llprop = llprop.p;
+    A.prop = prop.p; A.p = p.p; A.state = state.p;
+    return A;
+  }
+
+  void launch_pass(const PassArgs& A) {
+    k_hlr_pass<<<G, kPassThreads, pass_smem_bytes(d, cp), stream>>>(A);
+    AUGUR_CHECK_LAUNCH();
+  }
+
+  void refresh_cache() {
+    k_state_to_beta32<<<8, 256, 0, stream>>>(state.p, beta32.p, d, cp, chains, has_b);
+    AUGUR_CHECK_LAUNCH();
+    launch_pass(pass_args(MODE_CACHE, 0));
+    cache_valid = true;
+  }
+
+  void data_block(int bi, const augur_hmc_block& blk) {
+    if (!cache_valid) refresh_cache();
+    const uint32_t tag = TAG_HMC_BASE + 4u * (uint32_t)bi;
+    HmcArgs H{};
+    H.d = d; H.cp = cp; H.c_real = chains; H.has_b = has_b; H.ref_compat = ref_compat;
+    H.blk_theta = blk.block == AUGUR_HMC_BLOCK_THETA || blk.block == AUGUR_HMC_BLOCK_B_THETA;
+    H.blk_b = blk.block == AUGUR_HMC_BLOCK_B || blk.block == AUGUR_HMC_BLOCK_B_THETA;
+    H.block_kind = blk.block;
+    H.seed = seed; H.sweep = sweep; H.tag = tag;
+    H.cache_g = cache_g.p; H.cache_ll = cache_ll.p; H.cache_g_w = cache_g.p; H.cache_ll_w = cache_ll.p;
+    H.gprop = gprop.p; H.llprop = llprop.p; H.state = state.p; H.prop = prop.p; H.p0 = p0.p;
+    H.p = p.p; H.k0 = k0.p; H.beta32 = beta32.p; H.stats = stats_buf.p + (size_t)bi * chains * 2;
+    int lmax = 0;
+    for (int c = 0; c < chains; ++c) {
+      hmc_jitter(Stream{seed, (uint32_t)c, sweep, tag + 1}, blk.sim_len, blk.step_size, blk.fixed_steps,
+                 &H.hs.step[c], &H.hs.steps[c]);
+      lmax = std::max(lmax, H.hs.steps[c]);
+    }
+    k_hmc_begin<<<chains, 64, 0, stream>>>(H);
+    AUGUR_CHECK_LAUNCH();
+    PassArgs A = pass_args(MODE_LEAPFROG, 0);
+    A.blk_theta = H.blk_theta;
+    A.blk_b = H.blk_b;
+    A.hs = H.hs;
+    for (int l = 1; l <= lmax; ++l) {
+      A.l = l;
+      launch_pass(A);
+    }
+    k_hmc_end<<<chains, 64, 0, stream>>>(H);
+    AUGUR_CHECK_LAUNCH();
+  }
+
+  VBlockArgs vargs(int bi, const augur_hmc_block& blk) {
+    VBlockArgs V{};
+    V.d = d; V.has_b = has_b; V.ref_compat = ref_compat; V.prior_kind = prior_kind;
+    V.prior_a = prior_a; V.prior_b = prior_b; V.sim_len = blk.sim_len; V.eps = blk.step_size;
+    V.fixed_steps = blk.fixed_steps; V.seed = seed; V.sweep = sweep;
+    V.tag = TAG_HMC_BASE + 4u * (uint32_t)bi; V.state = state.p;
+    V.stats = stats_buf.p + (size_t)bi * chains * 2;
+    return V;
+  }
+
+  void run_sweeps(int nsw) override {
+    for (int i = 0; i < nsw; ++i) {
+      for (int bi = 0; bi < (int)sched.size(); ++bi) {
+        const augur_hmc_block& blk = sched[bi];
+        if (blk.block == AUGUR_HMC_BLOCK_V) {
+          k_hmc_vblock<<<chains, 64, 0, stream>>>(vargs(bi, blk));
+          AUGUR_CHECK_LAUNCH();
+        } else {
+          data_block(bi, blk);
+        }
+      }
+      ++sweep;
+      record_traces();
+    }
+  }
+
+  void log_density(double* out) override {
+    if (!cache_valid) refresh_cache();
+    augur_hmc_block dummy{};
+    k_hlr_ll_mod<<<cp, 64, 0, stream>>>(state.p, cache_ll.p, vargs(0, dummy), chains, ll_out.p);
+    AUGUR_CHECK_LAUNCH();
+    ll_out.download(out, chains, stream);
+    AUGUR_HIP(hipStreamSynchronize(stream));
+  }
+
+  void stats(double* out, size_t bytes) override {
+    const size_t need = sizeof(double) * sched.size() * chains * 2;
+    if (bytes < need) fail(AUGUR_ERR_ARG, "stats need %zu bytes", need);
+    stats_buf.download(out, sched.size() * chains * 2, stream);
+    AUGUR_HIP(hipStreamSynchronize(stream));
+  }
+
+  void alloc(int n_, int d_, int chains_) {
+    n = n_; d = d_; chains = chains_;
+    cp = next_pow2(chains);
+    const int tiles = (n + kTileRows - 1) / kTileRows;
+    G = std::max(1, std::min(tiles, 256));
+    const size_t P = (size_t)d + 2, ds = (size_t)d + 1;
+    x.alloc((size_t)n * d);
+    y.alloc(std::max(n, 1));
+    beta32.alloc(cp * ds);
+    state.alloc(cp * P); prop.alloc(cp * P);
+    p0.alloc(cp * ds); p.alloc(cp * ds); k0.alloc(cp);
+    cache_g.alloc(cp * ds); cache_ll.alloc(cp); gprop.alloc(cp * ds); llprop.alloc(cp);
+    part.alloc((size_t)cp * (d + 2) * G);
+    counter.alloc(1);
+    ll_out.alloc(cp);
+    stats_buf.alloc(std::max<size_t>(1, 4 * (size_t)chains * 2));
+    state.zero(stream); prop.zero(stream); beta32.zero(stream); counter.zero(stream);
+    stats_buf.zero(stream); p.zero(stream); p0.zero(stream);
+  }
+};
+
+static void validate_shapes(int n, int d, int chains) {
+  if (n < 1 || d < 1 || d > 256) fail(AUGUR_ERR_ARG, "need n >= 1 and 1 <= d <= 256 (got %d, %d)", n, d);
+  if (chains < 1 || chains > kMaxChains) fail(AUGUR_ERR_ARG, "chains must be in [1, %d]", kMaxChains);
+  const int cp = next_pow2(chains);
+  if (d * cp > kMaxGradSlots * kPassThreads)
+    fail(AUGUR_ERR_ARG, "d * chains (padded to a power of two) must be <= %d", kMaxGradSlots * kPassThreads);
+}
+
+}  // namespace augur
+
+using namespace augur;
+
+extern "C" {
+
+augur_status augur_hlr_create(const augur_hlr_desc* D, augur_sampler** out) {
+  return guard([&] {
+    if (!out || !D) fail(AUGUR_ERR_ARG, "null argument");
+    *out = nullptr;
+    validate_shapes(D->n_rows, D->n_features, D->chains);
+    if (!D->x || !D->y) fail(AUGUR_ERR_ARG, "null data");
+    if (D->prior_kind != 0 && D->prior_kind != 1) fail(AUGUR_ERR_ARG, "prior_kind must be 0 or 1");
+    if (D->n_blocks < 1 || D->n_blocks > 4) fail(AUGUR_ERR_ARG, "1..4 HMC blocks");
+    for (int i = 0; i < D->n_blocks; ++i) {
+      const augur_hmc_block& b = D->blocks[i];
+      if (b.block < 0 || b.block > 3) fail(AUGUR_ERR_ARG, "bad HMC block kind %d", b.block);
+      if ((b.block == AUGUR_HMC_BLOCK_B || b.block == AUGUR_HMC_BLOCK_B_THETA) && !D->has_intercept)
+        fail(AUGUR_ERR_ARG, "HMC block over b without an intercept");
+      if (!(b.step_size > 0.0) || (b.fixed_steps <= 0 && !(b.sim_len > 0.0)) || b.fixed_steps > 100000)
+        fail(AUGUR_ERR_ARG, "HMC block %d needs step_size > 0 and sim_len > 0 or fixed_steps", i);
+      if (b.fixed_steps <= 0 && b.sim_len / (0.8 * b.step_size) > 1.0e5)
+        fail(AUGUR_ERR_ARG, "HMC block %d: sim_len / step_size too large", i);
+    }
+    for (int i = 0; i < D->n_rows; ++i)
+      if (D->y[i] != 0 && D->y[i] != 1) fail(AUGUR_ERR_ARG, "y[%d] = %d is not 0 or 1", i, D->y[i]);
+    require_device();
+    auto* s = new HlrSampler();
+    try {
+      AUGUR_HIP(hipGetDevice(&s->device));
+      AUGUR_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+      s->seed = D->seed;
+      s->has_b = D->has_intercept;
+      s->ref_compat = D->ref_compat;
+      s->prior_kind = D->prior_kind;
+      s->prior_a = D->prior_a;
+      s->prior_b = D->prior_b;
+      s->sched.assign(D->blocks, D->blocks + D->n_blocks);
+      s->alloc(D->n_rows, D->n_features, D->chains);
+      s->x.upload(D->x, (size_t)D->n_rows * D->n_features, s->stream);
+      s->y.upload(D->y, D->n_rows, s->stream);
+      AUGUR_HIP(hipStreamSynchronize(s->stream));
+    } catch (...) {
+      delete s;
+      throw;
+    }
+    *out = s;
+  });
+}
+
+augur_status augur_op_hlr_ll_grad(int n_rows, int d, int chains, const float* x, const int32_t* y,
+                                  const double* beta, int has_intercept, double* ll, double* grad) {
+  return guard([&] {
+    validate_shapes(n_rows, d, chains);
+    if (!x || !y || !beta || !ll || !grad) fail(AUGUR_ERR_ARG, "null argument");
+    require_device();
+    HlrSampler s;
+    AUGUR_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+    s.has_b = has_intercept;
+    s.alloc(n_rows, d, chains);
+    s.x.upload(x, (size_t)n_rows * d, s.stream);
+    s.y.upload(y, n_rows, s.stream);
+    const int P = d + 2;
+    std::vector<double> st((size_t)s.cp * P, 0.0);
+    for (int c = 0; c < chains; ++c)
+      for (int j = 0; j <= d; ++j) st[(size_t)c * P + j] = (j < d || has_intercept) ? beta[(size_t)c * (d + 1) + j] : 0.0;
+    s.state.upload(st.data(), st.size(), s.stream);
+    s.refresh_cache();
+    std::vector<double> g((size_t)s.cp * (d + 1)), l(s.cp);
+    s.cache_g.download(g.data(), g.size(), s.stream);
+    s.cache_ll.download(l.data(), l.size(), s.stream);
+    AUGUR_HIP(hipStreamSynchronize(s.stream));
+    for (int c = 0; c < chains; ++c) {
+      ll[c] = l[c];
+      for (int j = 0; j <= d; ++j) grad[(size_t)c * (d + 1) + j] = g[(size_t)c * (d + 1) + j];
+    }
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,386 @@
+// augur_lda.hip -- LDA Gibbs sampler (lda.rv) on gfx950.
+//
+// Sweep (schedule ConjGibbs [theta] (*) ConjGibbs [phi] (*) DiscGibbs [z],
+// order of CudaC/CgKern.hs:192-238) as two kernels per sweep:
+//   k_lda_dirichlet : theta_d ~ Dir(alpha + cnt_theta[d]), phi_k ~ Dir(beta +
+//                     cnt_phi[k]) (Core/CgConj.hs:171-183, 278-306 ->
+//                     cbits/augur_dist.c:526-538); each count is read once and
+//                     cleared by the thread that reads it.
+//   k_lda_z         : z[d, n] ~ Cat(theta[d][k] * phi[k][w]) (Core/CgDisc.hs:
+//                     122-144 -> cbits/augur_dist.c:798-813), fused with the
+//                     count accumulation of the new z for the next sweep.
+// Layout in HBM (per chain): w, z int32 in CSR token order; theta FP32 D x K
+// (doc-major, one row per wave); phi_t FP32 V x K (word-major, so the K
+// probabilities of a token are contiguous); FP64 copies of theta, phi for
+// the caller; cnt_theta D x K, cnt_phi K x V int32.
+#include "augur_common.h"
+
+namespace augur {
+
+// FP32 categorical draw with a fixed operation order (no contraction):
+// pmf_i = a_i * b_i, running sums c_i; the first i < k-1 with u * c_{k-1} < c_i,
+// else k - 1.  This is augur_unnorm_disc_samp with the normalisation moved
+// to the uniform side.
+__device__ __noinline__ int draw_cat_f32(const float* __restrict__ a, const float* __restrict__ b,
+                                         int k, float u) {
+#pragma clang fp contract(off)
+  float c = 0.0f;
+  for (int i = 0; i < k; ++i) c = c + a[i] * b[i];
+  const float t = u * c;
+  float run = 0.0f;
+  for (int i = 0; i < k - 1; ++i) {
+    run = run + a[i] * b[i];
+    if (t < run) return i;
+  }
+  return k - 1;
+}
+
+struct LdaDev {
+  int K, D, V, T;
+  const int32_t* doc_off;
+  const int32_t* w;
+  const double* alpha;
+  const double* beta;
+  int32_t* z;          // C x T
+  int32_t* cnt_theta;  // C x D x K
+  int32_t* cnt_phi;    // C x K x V
+  double* theta64;     // C x D x K
+  float* theta32;      // C x D x K
+  double* phi64;       // C x K x V
+  float* phit32;       // C x V x K
+};
+
+__device__ inline double block_sum(double v, double* red) {
+  // deterministic: wave butterfly, then wave partials summed in order
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  const int wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[wv] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int i = 0; i < nw; ++i) s += red[i];
+  return s;
+}
+
+// blocks [0, K): phi rows; blocks [K, K + ceil(D / waves)): theta rows, one wave each.
+__global__ __launch_bounds__(256) void k_lda_dirichlet(LdaDev m, uint32_t seed, uint32_t sweep) {
+  __shared__ double red[16];
+  const int c = blockIdx.y;
+  const int b = blockIdx.x;
+  if (b < m.K) {
+    const int k = b;
+    const Stream s{seed, (uint32_t)c, sweep, TAG_LDA_PHI};
+    int32_t* cnt = m.cnt_phi + ((size_t)c * m.K + k) * m.V;
+    double* row = m.phi64 + ((size_t)c * m.K + k) * m.V;
+    double part = 0.0;
+    for (int v = threadIdx.x; v < m.V; v += blockDim.x) {
+      const double g = gamma_at(s, (uint32_t)(k * m.V + v), m.beta[v] + (double)cnt[v]);
+      cnt[v] = 0;
+      row[v] = g;
+      part += g;
+    }
+    const double tot = block_sum(part, red);
+    float* pt = m.phit32 + (size_t)c * m.V * m.K;
+    for (int v = threadIdx.x; v < m.V; v += blockDim.x) {
+      const double p = row[v] / tot;
+      row[v] = p;
+      pt[(size_t)v * m.K + k] = (float)p;
+    }
+    return;
+  }
+  const int waves = blockDim.x >> 6;
+  const int d = (b - m.K) * waves + (threadIdx.x >> 6);
+  if (d >= m.D) return;
+  const int lane = threadIdx.x & 63;
+  const Stream s{seed, (uint32_t)c, sweep, TAG_LDA_THETA};
+  const size_t off = ((size_t)c * m.D + d) * m.K;
+  double part = 0.0;
+  for (int k = lane; k < m.K; k += 64) {
+    const double g = gamma_at(s, (uint32_t)(d * m.K + k), m.alpha[k] + (double)m.cnt_theta[off + k]);
+    m.cnt_theta[off + k] = 0;
+    m.theta64[off + k] = g;
+    part += g;
+  }
+  for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o);
+  for (int k = lane; k < m.K; k += 64) {
+    const double p = m.theta64[off + k] / part;
+    m.theta64[off + k] = p;
+    m.theta32[off + k] = (float)p;
+  }
+}
+
+// One wave per document (grid-stride).  Lane j takes token j of each
+// 64-token chunk.  Topic counts of the document are formed with ballots
+// (lane k owns topic k) and stored once; word-topic counts use global
+// atomics (order-independent integer adds, so the table is exact).
+// The Philox key of table row c is chain0 + c.
+__global__ __launch_bounds__(256) void k_lda_z(LdaDev m, uint32_t seed, uint32_t sweep,
+                                               uint32_t chain0) {
+  const int c = blockIdx.y;
+  const int lane = threadIdx.x & 63;
+  const int waves = blockDim.x >> 6;
+  const Stream s{seed, chain0 + (uint32_t)c, sweep, TAG_LDA_Z};
+  const float* phit = m.phit32 + (size_t)c * m.V * m.K;
+  int32_t* cphi = m.cnt_phi + (size_t)c * m.K * m.V;
+  int32_t* zc = m.z + (size_t)c * m.T;
+  for (int d = blockIdx.x * waves + (threadIdx.x >> 6); d < m.D; d += gridDim.x * waves) {
+    const int du = __builtin_amdgcn_readfirstlane(d);
+    const float* th = m.theta32 + ((size_t)c * m.D + du) * m.K;
+    const int t0 = m.doc_off[du], t1 = m.doc_off[du + 1];
+    int my_count = 0;  // count of topic `lane` in this document (K <= 64)
+    for (int base = t0; base < t1; base += 64) {
+      const int t = base + lane;
+      const bool valid = t < t1;
+      int znew = -1;
+      if (valid) {
+        const int wd = m.w[t];
+        const float u = u01f(s.draw((uint32_t)t, 0).x);
+        znew = draw_cat_f32(th, phit + (size_t)wd * m.K, m.K, u);
+        zc[t] = znew;
+        atomicAdd(cphi + (size_t)znew * m.V + wd, 1);
+      }
+      if (m.K <= 64) {
+        for (int k = 0; k < m.K; ++k) {
+          const unsigned long long bal = __ballot(valid && znew == k);
+          if (lane == k) my_count += __popcll(bal);
+        }
+      } else if (valid) {
+        atomicAdd(m.cnt_theta + ((size_t)c * m.D + du) * m.K + znew, 1);
+      }
+    }
+    if (m.K <= 64 && lane < m.K) m.cnt_theta[((size_t)c * m.D + du) * m.K + lane] = my_count;
+  }
+}
+
+// Count tables of the current z (after set_pt / at creation).
+__global__ void k_lda_count(LdaDev m) {
+  const int c = blockIdx.y;
+  for (int d = blockIdx.x * blockDim.x + threadIdx.x; d < m.D; d += gridDim.x * blockDim.x) {
+    for (int t = m.doc_off[d]; t < m.doc_off[d + 1]; ++t) {
+      const int zz = m.z[(size_t)c * m.T + t];
+      atomicAdd(m.cnt_theta + ((size_t)c * m.D + d) * m.K + zz, 1);
+      atomicAdd(m.cnt_phi + ((size_t)c * m.K + zz) * m.V + m.w[t], 1);
+    }
+  }
+}
+
+__global__ void k_to_f32(const double* __restrict__ in, float* __restrict__ out, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    out[i] = (float)in[i];
+}
+
+__global__ void k_phi_transpose(const double* __restrict__ phi64, float* __restrict__ phit32, int K,
+                                int V, int C) {
+  const size_t n = (size_t)C * K * V;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const size_t c = i / ((size_t)K * V), r = i % ((size_t)K * V);
+    const int k = (int)(r / V), v = (int)(r % V);
+    phit32[(c * V + v) * K + k] = (float)phi64[i];
+  }
+}
+
+// Reference ll_mod: dirichlet_ll terms (cbits/augur_dist.c:488-496, no
+// normalising constant) + categorical_ll of z and w.  One block per chain.
+__global__ __launch_bounds__(256) void k_lda_ll(LdaDev m, double* out) {
+  __shared__ double red[16];
+  const int c = blockIdx.x;
+  double acc = 0.0;
+  for (size_t i = threadIdx.x; i < (size_t)m.D * m.K; i += blockDim.x)
+    acc += (m.alpha[i % m.K] - 1.0) * log(m.theta64[(size_t)c * m.D * m.K + i]);
+  for (size_t i = threadIdx.x; i < (size_t)m.K * m.V; i += blockDim.x)
+    acc += (m.beta[i % m.V] - 1.0) * log(m.phi64[(size_t)c * m.K * m.V + i]);
+  for (int d = 0; d < m.D; ++d)
+    for (int t = m.doc_off[d] + threadIdx.x; t < m.doc_off[d + 1]; t += blockDim.x) {
+      const int zz = m.z[(size_t)c * m.T + t];
+      acc += log(m.theta64[((size_t)c * m.D + d) * m.K + zz]) +
+             log(m.phi64[((size_t)c * m.K + zz) * m.V + m.w[t]]);
+    }
+  const double tot = block_sum(acc, red);
+  if (threadIdx.x == 0) out[c] = tot;
+}
+
+struct LdaSampler final : augur_sampler {
+  LdaDev m{};
+  DevBuf<int32_t> doc_off, w, z, cnt_theta, cnt_phi;
+  DevBuf<double> alpha, beta, theta64, phi64, ll;
+  DevBuf<float> theta32, phit32;
+
+  const char* family() const override { return "lda"; }
+
+  ParamView param(const std::string& name) override {
+    ParamView v;
+    if (name == "theta") v = {theta64.p, sizeof(double), (size_t)m.D * m.K, (size_t)m.D * m.K, true};
+    else if (name == "phi") v = {phi64.p, sizeof(double), (size_t)m.K * m.V, (size_t)m.K * m.V, true};
+    else if (name == "z") v = {z.p, sizeof(int32_t), (size_t)m.T, (size_t)m.T, true};
+    return v;
+  }
+
+  void recount() {
+    cnt_theta.zero(stream);
+    cnt_phi.zero(stream);
+    k_lda_count<<<dim3((m.D + 255) / 256, chains), 256, 0, stream>>>(m);
+    AUGUR_CHECK_LAUNCH();
+  }
+
+  void on_set(const std::string& name) override {
+    if (name == "z") {
+      // reject out-of-range topics before they index the count tables
+      std::vector<int32_t> hz((size_t)m.T * chains);
+      z.download(hz.data(), hz.size(), stream);
+      AUGUR_HIP(hipStreamSynchronize(stream));
+      for (int32_t v : hz)
+        if (v < 0 || v >= m.K) fail(AUGUR_ERR_ARG, "z value %d outside [0, %d)", v, m.K);
+      recount();
+    } else if (name == "theta") {
+      k_to_f32<<<256, 256, 0, stream>>>(theta64.p, theta32.p, theta64.n);
+      AUGUR_CHECK_LAUNCH();
+    } else if (name == "phi") {
+      k_phi_transpose<<<256, 256, 0, stream>>>(phi64.p, phit32.p, m.K, m.V, chains);
+      AUGUR_CHECK_LAUNCH();
+    }
+  }
+
+  void run_sweeps(int n) override {
+    const int waves = 4;
+    const dim3 gd(m.K + (m.D + waves - 1) / waves, chains);
+    const int zblocks = std::max(1, std::min((m.D + waves - 1) / waves, 2048));
+    for (int i = 0; i < n; ++i) {
+      k_lda_dirichlet<<<gd, 256, 0, stream>>>(m, seed, sweep);
+      AUGUR_CHECK_LAUNCH();
+      k_lda_z<<<dim3(zblocks, chains), 256, 0, stream>>>(m, seed, sweep, 0u);
+      AUGUR_CHECK_LAUNCH();
+      ++sweep;
+      record_traces();
+    }
+  }
+
+  void log_density(double* out) override {
+    k_lda_ll<<<chains, 256, 0, stream>>>(m, ll.p);
+    AUGUR_CHECK_LAUNCH();
+    ll.download(out, chains, stream);
+    AUGUR_HIP(hipStreamSynchronize(stream));
+  }
+
+  void stats(double* out, size_t bytes) override {
+    if (bytes < sizeof(double)) fail(AUGUR_ERR_ARG, "stats buffer too small");
+    out[0] = (double)sweep;
+  }
+};
+
+static void lda_validate(const augur_lda_desc* d) {
+  if (!d) fail(AUGUR_ERR_ARG, "null descriptor");
+  if (d->n_topics < 2 || d->n_docs < 1 || d->n_vocab < 1 || d->chains < 1)
+    fail(AUGUR_ERR_ARG, "need K >= 2, D >= 1, V >= 1, chains >= 1");
+  if (!d->doc_off || !d->w || !d->alpha || !d->beta) fail(AUGUR_ERR_ARG, "null array");
+  if (d->doc_off[0] != 0) fail(AUGUR_ERR_ARG, "doc_off[0] must be 0");
+  for (int i = 0; i < d->n_docs; ++i)
+    if (d->doc_off[i + 1] < d->doc_off[i]) fail(AUGUR_ERR_ARG, "doc_off must be non-decreasing");
+  const int T = d->doc_off[d->n_docs];
+  for (int t = 0; t < T; ++t)
+    if (d->w[t] < 0 || d->w[t] >= d->n_vocab)
+      fail(AUGUR_ERR_ARG, "word id %d at token %d outside [0, %d)", d->w[t], t, d->n_vocab);
+  for (int k = 0; k < d->n_topics; ++k)
+    if (!(d->alpha[k] > 0.0)) fail(AUGUR_ERR_ARG, "alpha must be positive");
+  for (int v = 0; v < d->n_vocab; ++v)
+    if (!(d->beta[v] > 0.0)) fail(AUGUR_ERR_ARG, "beta must be positive");
+}
+
+// The LDA DiscGibbs kernel on caller-supplied tables (one chain).
+static void lda_zstep_op(int n_docs, int k, int v, const int32_t* doc_off, const int32_t* w,
+                         const float* theta, const float* phi_t, uint32_t seed, uint32_t chain,
+                         uint32_t sweep, int32_t* z, int32_t* cnt_theta, int32_t* cnt_phi) {
+  require_device();
+  if (n_docs < 1 || k < 2 || v < 1 || !doc_off || !w || !theta || !phi_t || !z)
+    fail(AUGUR_ERR_ARG, "bad arguments");
+  const int T = doc_off[n_docs];
+  for (int t = 0; t < T; ++t)
+    if (w[t] < 0 || w[t] >= v) fail(AUGUR_ERR_ARG, "word id out of range");
+  DevBuf<int32_t> ddo(n_docs + 1), dw(std::max(T, 1)), dz(std::max(T, 1)), dct((size_t)n_docs * k),
+      dcp((size_t)k * v);
+  DevBuf<float> dth((size_t)n_docs * k), dph((size_t)v * k);
+  ddo.upload(doc_off, n_docs + 1, 0);
+  dw.upload(w, T, 0);
+  dth.upload(theta, (size_t)n_docs * k, 0);
+  dph.upload(phi_t, (size_t)v * k, 0);
+  dct.zero(0);
+  dcp.zero(0);
+  LdaDev m{};
+  m.K = k; m.D = n_docs; m.V = v; m.T = T;
+  m.doc_off = ddo.p; m.w = dw.p; m.z = dz.p; m.cnt_theta = dct.p; m.cnt_phi = dcp.p;
+  m.theta32 = dth.p; m.phit32 = dph.p;
+  const int zblocks = std::max(1, std::min((n_docs + 3) / 4, 2048));
+  k_lda_z<<<dim3(zblocks, 1), 256>>>(m, seed, sweep, chain);
+  AUGUR_CHECK_LAUNCH();
+  dz.download(z, T, 0);
+  if (cnt_theta) dct.download(cnt_theta, (size_t)n_docs * k, 0);
+  if (cnt_phi) dcp.download(cnt_phi, (size_t)k * v, 0);
+  AUGUR_HIP(hipDeviceSynchronize());
+}
+
+}  // namespace augur
+
+using namespace augur;
+
+extern "C" {
+
+augur_status augur_lda_create(const augur_lda_desc* d, augur_sampler** out) {
+  return guard([&] {
+    if (!out) fail(AUGUR_ERR_ARG, "null output handle");
+    *out = nullptr;
+    lda_validate(d);
+    require_device();
+    auto* s = new LdaSampler();
+    try {
+      AUGUR_HIP(hipGetDevice(&s->device));
+      AUGUR_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+      s->chains = d->chains;
+      s->seed = d->seed;
+      const int K = d->n_topics, D = d->n_docs, V = d->n_vocab, C = d->chains;
+      const int T = d->doc_off[D];
+      s->doc_off.alloc(D + 1);
+      s->w.alloc(std::max(T, 1));
+      s->alpha.alloc(K);
+      s->beta.alloc(V);
+      s->z.alloc((size_t)C * std::max(T, 1));
+      s->cnt_theta.alloc((size_t)C * D * K);
+      s->cnt_phi.alloc((size_t)C * K * V);
+      s->theta64.alloc((size_t)C * D * K);
+      s->theta32.alloc((size_t)C * D * K);
+      s->phi64.alloc((size_t)C * K * V);
+      s->phit32.alloc((size_t)C * V * K);
+      s->ll.alloc(C);
+      s->doc_off.upload(d->doc_off, D + 1, s->stream);
+      s->w.upload(d->w, T, s->stream);
+      s->alpha.upload(d->alpha, K, s->stream);
+      s->beta.upload(d->beta, V, s->stream);
+      s->z.zero(s->stream);
+      s->theta64.zero(s->stream);
+      s->theta32.zero(s->stream);
+      s->phi64.zero(s->stream);
+      s->phit32.zero(s->stream);
+      LdaDev& m = s->m;
+      m.K = K; m.D = D; m.V = V; m.T = T;
+      m.doc_off = s->doc_off.p; m.w = s->w.p; m.alpha = s->alpha.p; m.beta = s->beta.p;
+      m.z = s->z.p; m.cnt_theta = s->cnt_theta.p; m.cnt_phi = s->cnt_phi.p;
+      m.theta64 = s->theta64.p; m.theta32 = s->theta32.p; m.phi64 = s->phi64.p;
+      m.phit32 = s->phit32.p;
+      s->recount();
+      AUGUR_HIP(hipStreamSynchronize(s->stream));
+    } catch (...) {
+      delete s;
+      throw;
+    }
+    *out = s;
+  });
+}
+
+augur_status augur_op_lda_zstep(int n_docs, int k, int v, const int32_t* doc_off,
+                                const int32_t* w, const float* theta, const float* phi_t,
+                                uint32_t seed, uint32_t chain, uint32_t sweep, int32_t* z,
+                                int32_t* cnt_theta, int32_t* cnt_phi) {
+  return guard([&] {
+    lda_zstep_op(n_docs, k, v, doc_off, w, theta, phi_t, seed, chain, sweep, z, cnt_theta, cnt_phi);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,157 @@
+"""Python handles over the C-ABI samplers (one object per augur_sampler*)."""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+
+
+class Sampler:
+    """Common operations of an augur_sampler handle."""
+
+    family = None
+
+    def __init__(self, handle, chains, shapes):
+        self._h = handle
+        self.chains = chains
+        self._shapes = shapes  # name -> (shape per chain, dtype)
+
+    # -- lifecycle --
+    def close(self):
+        if self._h:
+            L.call("augur_destroy", self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- parameters --
+    def _spec(self, name):
+        if name not in self._shapes:
+            raise KeyError(f"{self.family} sampler has no parameter {name!r}")
+        return self._shapes[name]
+
+    def get(self, name, chain=0):
+        """augur_cpy: current value of `name` for one chain."""
+        shape, dt = self._spec(name)
+        out = np.empty(shape, dtype=dt)
+        L.call("augur_cpy", self._h, name.encode(), chain, L.ptr(out), out.nbytes)
+        return out
+
+    def set(self, name, value, chain=-1):
+        """augur_set_pt: write `name` for one chain (-1: every chain)."""
+        shape, dt = self._spec(name)
+        a = np.ascontiguousarray(value, dtype=dt).reshape(shape)
+        L.call("augur_set_pt", self._h, name.encode(), chain, L.ptr(a), a.nbytes)
+
+    def step(self, n=1):
+        """augur_step: run n sweeps on every chain."""
+        L.call("augur_step", self._h, int(n))
+
+    def log_density(self):
+        out = np.zeros(self.chains)
+        L.call("augur_get_curr_ll", self._h, L.ptr(out))
+        return out
+
+    def trace(self, name, capacity):
+        L.call("augur_trace", self._h, name.encode(), int(capacity))
+
+    def read_trace(self, name):
+        shape, dt = self._spec(name)
+        n = C.c_int(0)
+        L.call("augur_trace_read", self._h, name.encode(), None, 0, C.byref(n))
+        out = np.empty((n.value, self.chains) + tuple(shape), dtype=dt)
+        L.call("augur_trace_read", self._h, name.encode(), L.ptr(out), out.nbytes, C.byref(n))
+        return out
+
+    def sweeps(self):
+        v = C.c_uint32(0)
+        L.call("augur_sweep_count", self._h, C.byref(v))
+        return v.value
+
+
+class LdaSampler(Sampler):
+    family = "lda"
+
+    def __init__(self, n_topics, doc_off, w, alpha, beta, chains=1, seed=0):
+        L.require_gpu()
+        self.doc_off = L.i32(doc_off)
+        self.w = L.i32(w)
+        self.alpha = L.f64(alpha)
+        self.beta = L.f64(beta)
+        D = len(self.doc_off) - 1
+        V = len(self.beta)
+        d = L.LdaDesc(n_topics, D, V, chains, seed & 0xFFFFFFFF, L.ptr(self.doc_off).value,
+                      L.ptr(self.w).value, L.ptr(self.alpha).value, L.ptr(self.beta).value)
+        h = C.c_void_p()
+        L.call("augur_lda_create", C.byref(d), C.byref(h))
+        T = int(self.doc_off[-1])
+        super().__init__(h, chains, {"theta": ((D, n_topics), np.float64),
+                                     "phi": ((n_topics, V), np.float64),
+                                     "z": ((T,), np.int32)})
+
+
+class HlrSampler(Sampler):
+    family = "hlr"
+
+    def __init__(self, x, y, blocks, chains=1, seed=0, prior_kind=0, prior_a=1.0, prior_b=1.0,
+                 has_intercept=False, ref_compat=True):
+        """blocks: list of (block_kind, sim_len, step_size, fixed_steps)."""
+        L.require_gpu()
+        self.x = L.f32(x)
+        self.y = L.i32(y)
+        n, dim = self.x.shape
+        desc = L.HlrDesc()
+        desc.n_rows, desc.n_features, desc.chains, desc.seed = n, dim, chains, seed & 0xFFFFFFFF
+        desc.x, desc.y = L.ptr(self.x).value, L.ptr(self.y).value
+        desc.prior_kind, desc.prior_a, desc.prior_b = prior_kind, prior_a, prior_b
+        desc.has_intercept, desc.ref_compat = int(has_intercept), int(ref_compat)
+        if not 1 <= len(blocks) <= 4:
+            raise ValueError("1..4 HMC blocks")
+        desc.n_blocks = len(blocks)
+        for i, (kind, sim_len, eps, fixed) in enumerate(blocks):
+            desc.blocks[i] = L.HmcBlock(kind, sim_len, eps, int(fixed))
+        h = C.c_void_p()
+        L.call("augur_hlr_create", C.byref(desc), C.byref(h))
+        self.n_blocks = len(blocks)
+        shapes = {"state": ((dim + 2,), np.float64), "theta": ((dim,), np.float64),
+                  "v": ((1,), np.float64)}
+        if has_intercept:
+            shapes["b"] = ((1,), np.float64)
+        super().__init__(h, chains, shapes)
+
+    def acceptance(self):
+        """[block][chain] accepted / proposed."""
+        out = np.zeros((self.n_blocks, self.chains, 2))
+        L.call("augur_stats", self._h, L.ptr(out), out.nbytes)
+        with np.errstate(invalid="ignore"):
+            return out[..., 0] / out[..., 1]
+
+
+class GmmSampler(Sampler):
+    family = "gmm"
+
+    def __init__(self, y, alpha, mu0, cov0, df, scale0, chains=1, seed=0, sample_pi=True,
+                 sample_cov=True):
+        L.require_gpu()
+        self.y = L.f32(y)
+        n, dim = self.y.shape
+        self.alpha, self.mu0 = L.f64(alpha), L.f64(mu0)
+        self.cov0, self.scale0 = L.f64(cov0), L.f64(scale0)
+        k = len(self.alpha)
+        desc = L.GmmDesc(n, dim, k, chains, seed & 0xFFFFFFFF, L.ptr(self.y).value,
+                         L.ptr(self.alpha).value, L.ptr(self.mu0).value, L.ptr(self.cov0).value,
+                         int(df), L.ptr(self.scale0).value, int(sample_pi), int(sample_cov))
+        h = C.c_void_p()
+        L.call("augur_gmm_create", C.byref(desc), C.byref(h))
+        super().__init__(h, chains, {"pis": ((k,), np.float64), "mu": ((k, dim), np.float64),
+                                     "cov": ((k, dim, dim), np.float64), "z": ((n,), np.int32)})

@@ -1,0 +1,202 @@
+/*
+ * augur_hip.h -- C-ABI of the MI355X (gfx950) sampler runtime.
+ *
+ * This library replaces, for the LDA, hierarchical-logistic-regression and
+ * Gaussian-mixture families, the runtime-generated sampler library that
+ * AugurV2 builds and drives through ctypes, together with the cbits GPU
+ * library it links (augur_blkop, augur_vecop, augur_matop, augur_dist).
+ *
+ * Conventions
+ *   - Every function returns an augur_status; on failure a message is
+ *     available from augur_last_error() (per calling thread).
+ *   - Host pointers only: the caller owns every buffer passed in or out;
+ *     the library owns all device memory of a sampler.
+ *   - Handles are independent: several samplers may live in one process.
+ *     Calls on one handle must not run concurrently; different handles may.
+ *   - Parameter names are the family's canonical names:
+ *       LDA : "theta" (D x K f64), "phi" (K x V f64), "z" (T i32)
+ *       HLR : "theta" (D f64), "b" (1 f64), "v" (1 f64), "state" (D + 2 f64)
+ *       GMM : "pis" (K f64), "mu" (K x D f64), "cov" (K x D x D f64), "z" (N i32)
+ *     `chain` selects one chain's copy.
+ *   - Random numbers: Philox4x32-10 keyed by (seed, chain); see augur_rng.h.
+ */
+#ifndef AUGUR_HIP_H
+#define AUGUR_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int augur_status;
+#define AUGUR_OK 0
+#define AUGUR_ERR_ARG 1         /* invalid argument or shape */
+#define AUGUR_ERR_HIP 2         /* HIP runtime error (no device, launch failure, ...) */
+#define AUGUR_ERR_STATE 3       /* call not valid in the sampler's current state */
+#define AUGUR_ERR_UNSUPPORTED 4 /* model / schedule outside the supported families */
+
+typedef struct augur_sampler augur_sampler;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* augur_last_error(void);
+/* Library version string. */
+const char* augur_version(void);
+/* Number of visible HIP devices (0 when none); never fails. */
+augur_status augur_device_count(int* n);
+
+/* ------------------------------------------------------------------------
+ * Sampler construction.  Each replaces the generated augur_iface_init for
+ * its family (compiler/augur/src/CudaC/CgStInit.hs:474-510, called from
+ * pyaugur/pyaugur/augurlib.py:445-475): hyper-parameters and data are
+ * copied to device memory; parameters start at zero until augur_set_pt.
+ * ---------------------------------------------------------------------- */
+
+/* lda.rv: theta[d] ~ Dirichlet(alpha); phi[k] ~ Dirichlet(beta);
+ * z[d, n] ~ Categorical(theta[d]); w[d, n] ~ Categorical(phi[z[d, n]]).
+ * Schedule ConjGibbs [theta] (*) ConjGibbs [phi] (*) DiscGibbs [z]. */
+typedef struct {
+  int n_topics;          /* K */
+  int n_docs;            /* D */
+  int n_vocab;           /* V */
+  int chains;
+  uint32_t seed;
+  const int32_t* doc_off; /* D + 1 offsets into w (CSR of the ragged w[d, n]) */
+  const int32_t* w;       /* T = doc_off[D] word ids in [0, V) */
+  const double* alpha;    /* K */
+  const double* beta;     /* V */
+} augur_lda_desc;
+augur_status augur_lda_create(const augur_lda_desc* desc, augur_sampler** out);
+
+/* hlr.rv family: v ~ InvGamma(a, b) | Exponential(a); [b ~ Normal(0, v)];
+ * theta[k] ~ Normal(0, v); y[n] ~ Bernoulli(sigmoid(dotprod(theta, x[n]) [+ b])).
+ * A schedule of up to 4 HMC blocks run in order each sweep. */
+#define AUGUR_HMC_BLOCK_V 0
+#define AUGUR_HMC_BLOCK_B 1
+#define AUGUR_HMC_BLOCK_THETA 2
+#define AUGUR_HMC_BLOCK_B_THETA 3
+typedef struct {
+  int block;        /* AUGUR_HMC_BLOCK_* */
+  double sim_len;   /* reference simLen (Core/KernSyn.hs:64-75 default 1.0) */
+  double step_size; /* reference eps (default 0.05) */
+  int fixed_steps;  /* > 0: fixed step size and this many leapfrog steps;
+                       0: reference jitter (cbits/augur_mcmclib.h:401-405) */
+} augur_hmc_block;
+typedef struct {
+  int n_rows;       /* N */
+  int n_features;   /* D (K in hlr.rv), 1..256 */
+  int chains;       /* 1..16 */
+  uint32_t seed;
+  const float* x;   /* N x D row-major FP32 */
+  const int32_t* y; /* N labels in {0, 1} */
+  int prior_kind;   /* 0: v ~ InvGamma(prior_a, prior_b); 1: v ~ Exponential(prior_a) */
+  double prior_a, prior_b;
+  int has_intercept;
+  int ref_compat;   /* 1: reference normal_ll (-log var) and normal_dotarg2 forms */
+  int n_blocks;
+  augur_hmc_block blocks[4];
+} augur_hlr_desc;
+augur_status augur_hlr_create(const augur_hlr_desc* desc, augur_sampler** out);
+
+/* hmvgmm.rv: pis ~ Dirichlet(alpha); mu[k] ~ MvNormal(mu0, cov0);
+ * cov[k] ~ InvWishart(df, scale0); z[n] ~ Categorical(pis);
+ * y[n] ~ MvNormal(mu[z[n]], cov[z[n]]).  sample_pi / sample_cov = 0 keep
+ * pis / cov fixed at their set_pt values (mvgmm.rv). */
+typedef struct {
+  int n_points;      /* N */
+  int dim;           /* D, 1..16 */
+  int n_components;  /* K, 1..64 */
+  int chains;
+  uint32_t seed;
+  const float* y;        /* N x D FP32 */
+  const double* alpha;   /* K */
+  const double* mu0;     /* D */
+  const double* cov0;    /* D x D */
+  int df;
+  const double* scale0;  /* D x D */
+  int sample_pi;
+  int sample_cov;
+} augur_gmm_desc;
+augur_status augur_gmm_create(const augur_gmm_desc* desc, augur_sampler** out);
+
+/* ------------------------------------------------------------------------
+ * Sampler operation (generic over families).
+ * ---------------------------------------------------------------------- */
+
+/* Replaces augur_set_pt (CgStInit.hs:518-531): write one chain's value of a
+ * parameter; chain = -1 writes the same value to every chain. */
+augur_status augur_set_pt(augur_sampler* s, const char* name, int chain, const void* src,
+                          size_t bytes);
+/* Replaces augur_cpy (CgStInit.hs:535-546): read one chain's current value. */
+augur_status augur_cpy(augur_sampler* s, const char* name, int chain, void* dst, size_t bytes);
+/* Replaces augur_step (CudaC/CgKern.hs:232-238): run n sweeps of the
+ * schedule on every chain; returns when they are complete. */
+augur_status augur_step(augur_sampler* s, int n_sweeps);
+/* Replaces augur_get_curr_ll (CgStInit.hs:597-602): log-density of the
+ * current state, one value per chain (FP64). */
+augur_status augur_get_curr_ll(augur_sampler* s, double* out_per_chain);
+/* Record `name` after every sweep into a device trace of `capacity` sweeps
+ * (capacity 0 disables).  Recording starts with the next augur_step. */
+augur_status augur_trace(augur_sampler* s, const char* name, int capacity);
+/* Copy the recorded sweeps (oldest first, all chains: [sweep][chain][...]). */
+augur_status augur_trace_read(augur_sampler* s, const char* name, void* dst, size_t bytes,
+                              int* n_recorded);
+/* Per-chain sampler statistics: HMC blocks report accepted proposals and
+ * proposals for each block ([block][chain][2] as double); Gibbs samplers
+ * report the sweep count. */
+augur_status augur_stats(augur_sampler* s, double* out, size_t bytes);
+/* Number of sweeps run so far (the Philox sweep counter). */
+augur_status augur_sweep_count(augur_sampler* s, uint32_t* out);
+augur_status augur_destroy(augur_sampler* s);
+
+/* ------------------------------------------------------------------------
+ * Kernel-level entry points (host buffers in and out).  Each runs one HIP
+ * kernel of the sampler on device 0 and exists so the kernels can be
+ * checked one by one against the CPU restatement.
+ * ---------------------------------------------------------------------- */
+
+/* n Philox4x32-10 blocks: ctr[4 n], key[2 n] -> out[4 n]. */
+augur_status augur_op_philox(const uint32_t* ctr, const uint32_t* key, int n, uint32_t* out);
+/* out[i] = Gamma(shape[i], 1) at site i (cbits/augur_dist.c:302-312). */
+augur_status augur_op_gamma(uint32_t seed, uint32_t chain, uint32_t sweep, uint32_t tag,
+                            const double* shape, int n, double* out);
+/* rows independent Dirichlet(alpha + cnt[r]) draws, row r at sites r * k + i
+ * (cbits/augur_dist.c:526-538). */
+augur_status augur_op_dirichlet_conj(uint32_t seed, uint32_t chain, uint32_t sweep,
+                                     uint32_t tag, const double* alpha, const int32_t* cnt,
+                                     int rows, int k, double* out);
+/* The LDA DiscGibbs [z] kernel on given FP32 theta (D x K) and word-major
+ * phi_t (V x K) tables (Core/CgDisc.hs:122-144); also returns the count
+ * tables of the new z (Core/CgConj.hs:171-183). */
+augur_status augur_op_lda_zstep(int n_docs, int k, int v, const int32_t* doc_off,
+                                const int32_t* w, const float* theta, const float* phi_t,
+                                uint32_t seed, uint32_t chain, uint32_t sweep, int32_t* z,
+                                int32_t* cnt_theta, int32_t* cnt_phi);
+/* HLR data term for `chains` parameter vectors beta[c] = (theta[0..D), b):
+ * ll[c] = sum_n log Bernoulli(y_n | sigmoid(x_n . theta + b)) and its
+ * gradient grad[c] = (d/dtheta, d/db) (Core/CgLike.hs:90-136,
+ * Core/CgGrad.hs:165-233).  FP32 arithmetic, FP64 accumulation. */
+augur_status augur_op_hlr_ll_grad(int n_rows, int d, int chains, const float* x,
+                                  const int32_t* y, const double* beta, int has_intercept,
+                                  double* ll, double* grad);
+/* The GMM DiscGibbs [z] kernel for one chain given pis, mu, cov (FP64, as
+ * drawn by the conjugate kernels; cbits/augur_dist.c:548-568). */
+augur_status augur_op_gmm_zstep(int n, int d, int k, const float* y, const double* pis,
+                                const double* mu, const double* cov, uint32_t seed,
+                                uint32_t chain, uint32_t sweep, int32_t* z);
+/* Conjugate draws of one GMM sweep on the device for one chain, from given
+ * counts and sums (cbits/augur_dist.c:633-669 and :740-757): updates pis
+ * (if sample_pi), mu and cov in place. sum_y: K x D sums of assigned points;
+ * scatter: K x D x D, sum over assigned points of (y - mu_new)(y - mu_new)^T
+ * is formed on the device from sum_yy (K x D x D, sum of y y^T). */
+augur_status augur_op_gmm_conj(int d, int k, const double* alpha, const double* mu0,
+                               const double* cov0, int df, const double* scale0,
+                               const int32_t* cnt, const double* sum_y, const double* sum_yy,
+                               int sample_pi, int sample_cov, uint32_t seed, uint32_t chain,
+                               uint32_t sweep, double* pis, double* mu, double* cov);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AUGUR_HIP_H */

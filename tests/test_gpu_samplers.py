@@ -1,0 +1,159 @@
+"""Sampler-level parity through the C-ABI: one sweep of each family on the
+GPU against the same sweep of the CPU FP64 restatement, started from the
+same state with the same Philox streams."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def rt():
+    from augurv2_amd import _lib, runtime
+    _lib.require_gpu()
+    return runtime
+
+
+def _corpus(rng, D, V, mean_len):
+    lens = np.maximum(rng.poisson(mean_len, D), 1)
+    doc_off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    w = rng.integers(0, V, doc_off[-1]).astype(np.int32)
+    return doc_off, w
+
+
+def test_lda_two_sweeps_match_oracle(rt, oracle):
+    rng = np.random.default_rng(0)
+    K, D, V, C = 6, 40, 120, 2
+    doc_off, w = _corpus(rng, D, V, 60)
+    alpha, beta = np.full(K, 0.1), np.full(V, 0.01)
+    s = rt.LdaSampler(K, doc_off, w, alpha, beta, chains=C, seed=77)
+    z0 = [rng.integers(0, K, len(w)).astype(np.int32) for _ in range(C)]
+    for c in range(C):
+        s.set("z", z0[c], chain=c)
+    for sweep in range(2):
+        s.step(1)
+        for c in range(C):
+            theta = np.zeros((D, K))
+            phi = np.zeros((K, V))
+            z = z0[c].copy()
+            oracle.lda_sweep(doc_off, w, alpha, beta, 77, c, sweep, theta, phi, z)
+            np.testing.assert_allclose(s.get("theta", c), theta, rtol=1e-9, atol=1e-300)
+            np.testing.assert_allclose(s.get("phi", c), phi, rtol=1e-9, atol=1e-300)
+            zg = s.get("z", c)
+            assert (zg == z).mean() >= 0.999
+            z0[c] = zg  # continue both from the GPU's assignment
+    ll = s.log_density()
+    assert np.isfinite(ll).all()
+    s.close()
+
+
+def test_lda_counts_exact_after_set(rt, oracle):
+    """Counts are recomputed from a set z; the Dirichlet draws then consume
+    them, so equal theta/phi to the oracle imply equal count tables."""
+    rng = np.random.default_rng(3)
+    K, D, V = 12, 30, 64
+    doc_off, w = _corpus(rng, D, V, 40)
+    alpha, beta = np.full(K, 0.2), np.full(V, 0.05)
+    s = rt.LdaSampler(K, doc_off, w, alpha, beta, seed=1)
+    z = rng.integers(0, K, len(w)).astype(np.int32)
+    s.set("z", z)
+    s.step(1)
+    theta = np.zeros((D, K))
+    phi = np.zeros((K, V))
+    zz = z.copy()
+    oracle.lda_sweep(doc_off, w, alpha, beta, 1, 0, 0, theta, phi, zz)
+    np.testing.assert_allclose(s.get("theta"), theta, rtol=1e-9)
+    np.testing.assert_allclose(s.get("phi"), phi, rtol=1e-9)
+    with pytest.raises(Exception):
+        s.set("z", np.full(len(w), K, np.int32))  # out of range topic
+
+
+def _hlr_data(rng, n, d):
+    x = rng.standard_normal((n, d)).astype(np.float32)
+    beta = rng.standard_normal(d) * 0.5
+    y = (rng.random(n) < 1 / (1 + np.exp(-(x @ beta)))).astype(np.int32)
+    return x, y
+
+
+@pytest.mark.parametrize("fixed", [0, 12])
+@pytest.mark.parametrize("intercept", [False, True])
+def test_hlr_iterations_match_oracle(rt, oracle, fixed, intercept):
+    from augurv2_amd._lib import HMC_BLOCK_V, HMC_BLOCK_B, HMC_BLOCK_THETA
+    rng = np.random.default_rng(10 + fixed)
+    n, d, C = 3000, 8, 4
+    x, y = _hlr_data(rng, n, d)
+    blocks = [(HMC_BLOCK_V, 1.0, 0.05, fixed)]
+    if intercept:
+        blocks.append((HMC_BLOCK_B, 0.1, 0.01, fixed))
+    blocks.append((HMC_BLOCK_THETA, 0.1, 0.01, fixed))
+    s = rt.HlrSampler(x, y, blocks, chains=C, seed=123, has_intercept=intercept)
+    m = oracle.Hlr(x.astype(np.float64), y, has_intercept=intercept)
+    states = []
+    for c in range(C):
+        st = np.concatenate([rng.standard_normal(d) * 0.3, [0.1 * intercept], [0.8 + 0.1 * c]])
+        s.set("state", st, chain=c)
+        states.append(st)
+    kinds = {HMC_BLOCK_V: 0, HMC_BLOCK_B: 1, HMC_BLOCK_THETA: 2}
+    for sweep in range(3):
+        s.step(1)
+        for c in range(C):
+            for bi, (kind, sl, eps, fx) in enumerate(blocks):
+                m.hmc_step(states[c], kinds[kind], bi, sl, eps, fx, 123, c, sweep)
+            np.testing.assert_allclose(s.get("state", c), states[c], rtol=2e-5, atol=1e-7)
+    acc = s.acceptance()
+    assert acc.shape == (len(blocks), C)
+    s.close()
+
+
+def test_hlr_curr_ll(rt, oracle):
+    from augurv2_amd._lib import HMC_BLOCK_V, HMC_BLOCK_THETA
+    rng = np.random.default_rng(4)
+    x, y = _hlr_data(rng, 2000, 6)
+    s = rt.HlrSampler(x, y, [(HMC_BLOCK_V, 1.0, 0.05, 0), (HMC_BLOCK_THETA, 0.1, 0.01, 0)],
+                      chains=2, seed=3)
+    m = oracle.Hlr(x.astype(np.float64), y)
+    st = np.concatenate([rng.standard_normal(6) * 0.2, [0.0], [1.3]])
+    s.set("state", st)
+    ll = s.log_density()
+    ref = m.block_ll(st, 0) + m.data_ll(st)  # v prior + theta priors + data
+    np.testing.assert_allclose(ll, ref, rtol=1e-6)
+
+
+def _gmm_setup(rng, n, d, k):
+    pis = rng.dirichlet(np.ones(k))
+    mu = rng.standard_normal((k, d)) * 3
+    zt = rng.choice(k, n, p=pis)
+    y = (mu[zt] + rng.standard_normal((n, d))).astype(np.float32)
+    return y, zt
+
+
+@pytest.mark.parametrize("sample_pi,sample_cov", [(True, True), (False, False)])
+def test_gmm_sweeps_match_oracle(rt, oracle, sample_pi, sample_cov):
+    rng = np.random.default_rng(8)
+    n, d, k, C = 3000, 4, 3, 2
+    y, zt = _gmm_setup(rng, n, d, k)
+    alpha, mu0, cov0, df, scale0 = np.ones(k), np.zeros(d), 10 * np.eye(d), d + 2, np.eye(d)
+    s = rt.GmmSampler(y, alpha, mu0, cov0, df, scale0, chains=C, seed=9, sample_pi=sample_pi,
+                      sample_cov=sample_cov)
+    g = oracle.Gmm(y.astype(np.float64), alpha, mu0, cov0, df, scale0, sample_pi, sample_cov)
+    st = []
+    for c in range(C):
+        pis = np.full(k, 1.0 / k)
+        mu = rng.standard_normal((k, d))
+        cov = np.stack([np.eye(d) * (1 + 0.2 * j) for j in range(k)])
+        z = rng.integers(0, k, n).astype(np.int32)
+        for name, val in (("pis", pis), ("mu", mu), ("cov", cov), ("z", z)):
+            s.set(name, val, chain=c)
+        st.append([pis, mu, cov, z])
+    for sweep in range(2):
+        s.step(1)
+        for c in range(C):
+            pis, mu, cov, z = st[c]
+            g.sweep(pis, mu, cov, z, 9, c, sweep)
+            np.testing.assert_allclose(s.get("pis", c), pis, rtol=1e-9)
+            np.testing.assert_allclose(s.get("mu", c), mu, rtol=1e-5, atol=1e-5)
+            np.testing.assert_allclose(s.get("cov", c), cov, rtol=1e-4, atol=1e-6)
+            zg = s.get("z", c)
+            assert (zg == z).mean() >= 0.999
+            st[c][3] = zg.copy()
+    s.close()
