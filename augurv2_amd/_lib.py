@@ -72,6 +72,8 @@ SIGNATURES = {
     "augur_trace_read": [_H, C.c_char_p, _P, _S, _P],
     "augur_stats": [_H, _P, _S],
     "augur_sweep_count": [_H, _P],
+    "augur_profile": [_H, _I],
+    "augur_profile_read": [_H, _P, _P],
     "augur_destroy": [_H],
     "augur_op_philox": [_P, _P, _I, _P],
     "augur_op_gamma": [_U, _U, _U, _U, _P, _I, _P],

@@ -56,7 +56,28 @@ using namespace augur;
 augur_sampler::augur_sampler() {}
 
 augur_sampler::~augur_sampler() {
+  for (auto& e : prof_events) {
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
+  }
   if (stream) (void)hipStreamDestroy(stream);
+}
+
+void augur_sampler::prof_begin() {
+  if (!prof_on()) return;
+  if (prof_used == prof_events.size()) {
+    std::pair<hipEvent_t, hipEvent_t> e;
+    AUGUR_HIP(hipEventCreate(&e.first));
+    AUGUR_HIP(hipEventCreate(&e.second));
+    prof_events.push_back(e);
+  }
+  AUGUR_HIP(hipEventRecord(prof_events[prof_used].first, stream));
+}
+
+void augur_sampler::prof_end() {
+  if (!prof_on()) return;
+  AUGUR_HIP(hipEventRecord(prof_events[prof_used].second, stream));
+  ++prof_used;
 }
 
 void augur_sampler::record_traces() {
@@ -191,6 +212,30 @@ augur_status augur_sweep_count(augur_sampler* s, uint32_t* out) {
   if (!s || !out) return AUGUR_ERR_ARG;
   *out = s->sweep;
   return AUGUR_OK;
+}
+
+augur_status augur_profile(augur_sampler* s, int period) {
+  return guard([&] {
+    if (!s || period < 0) fail(AUGUR_ERR_ARG, "bad arguments");
+    s->prof_period = period;
+    s->prof_used = 0;
+  });
+}
+
+augur_status augur_profile_read(augur_sampler* s, double* avg_us, int* n_launches) {
+  return guard([&] {
+    if (!s || !avg_us || !n_launches) fail(AUGUR_ERR_ARG, "null argument");
+    AUGUR_HIP(hipStreamSynchronize(s->stream));
+    double tot = 0.0;
+    for (size_t i = 0; i < s->prof_used; ++i) {
+      float ms = 0.0f;
+      AUGUR_HIP(hipEventElapsedTime(&ms, s->prof_events[i].first, s->prof_events[i].second));
+      tot += ms;
+    }
+    *n_launches = (int)s->prof_used;
+    *avg_us = s->prof_used ? 1000.0 * tot / (double)s->prof_used : 0.0;
+    s->prof_used = 0;
+  });
 }
 
 augur_status augur_destroy(augur_sampler* s) {

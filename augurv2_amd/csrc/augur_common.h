@@ -134,4 +134,13 @@ struct augur_sampler {
   virtual void on_set(const std::string& name) = 0;
   // Called after every sweep: copies traced parameters.
   void record_traces();
+
+  // Event timing of the family's dominant kernel: every prof_period-th
+  // sweep, each launch of it is bracketed by an event pair on `stream`.
+  int prof_period = 0;
+  size_t prof_used = 0;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_events;
+  bool prof_on() const { return prof_period > 0 && (sweep % (uint32_t)prof_period) == 0; }
+  void prof_begin();
+  void prof_end();
 };

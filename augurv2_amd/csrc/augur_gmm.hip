@@ -452,8 +452,10 @@ struct GmmSampler final : augur_sampler {
       if (!stats_valid) refresh_stats();
       k_gmm_conj<<<dim3(m.k, chains), kGmmThreads, 0, stream>>>(m, seed, sweep, 1);
       AUGUR_CHECK_LAUNCH();
+      prof_begin();
       zk<<<dim3(m.G, chains), kGmmThreads, z_smem(), stream>>>(m, seed, sweep, 0u, 1);
       AUGUR_CHECK_LAUNCH();
+      prof_end();
       ++sweep;
       record_traces();
     }

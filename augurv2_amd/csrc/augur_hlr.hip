@@ -504,8 +504,10 @@ struct HlrSampler final : augur_sampler {
   }
 
   void launch_pass(const PassArgs& A) {
+    prof_begin();
     k_hlr_pass<<<G, kPassThreads, pass_smem_bytes(d, cp), stream>>>(A);
     AUGUR_CHECK_LAUNCH();
+    prof_end();
   }
 
   void refresh_cache() {

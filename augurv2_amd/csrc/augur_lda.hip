@@ -247,8 +247,10 @@ struct LdaSampler final : augur_sampler {
     for (int i = 0; i < n; ++i) {
       k_lda_dirichlet<<<gd, 256, 0, stream>>>(m, seed, sweep);
       AUGUR_CHECK_LAUNCH();
+      prof_begin();
       k_lda_z<<<dim3(zblocks, chains), 256, 0, stream>>>(m, seed, sweep, 0u);
       AUGUR_CHECK_LAUNCH();
+      prof_end();
       ++sweep;
       record_traces();
     }

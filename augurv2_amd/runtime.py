@@ -73,6 +73,17 @@ class Sampler:
         L.call("augur_trace_read", self._h, name.encode(), L.ptr(out), out.nbytes, C.byref(n))
         return out
 
+    def profile(self, period):
+        """Event-time the dominant kernel every `period`-th sweep (0: off)."""
+        L.call("augur_profile", self._h, int(period))
+
+    def profile_read(self):
+        """(average microseconds per timed launch, number of launches)."""
+        avg = C.c_double(0.0)
+        n = C.c_int(0)
+        L.call("augur_profile_read", self._h, C.byref(avg), C.byref(n))
+        return avg.value, n.value
+
     def sweeps(self):
         v = C.c_uint32(0)
         L.call("augur_sweep_count", self._h, C.byref(v))

@@ -1,0 +1,162 @@
+"""Benchmark: min-ESS/sec of the HMC sampler for hierarchical logistic
+regression (N = 100k rows, D = 64, 4 chains, HMC [v] (*) HMC [beta] with 20
+leapfrog steps of 0.005) on one MI355X, at R-hat < 1.01; beside it LDA
+tokens/sec (K = 10, V = 1000, D = 200, ~20k tokens, 100 sweeps, 1 chain).
+
+A step is one sweep of the schedule (one HMC iteration of every block on
+every chain).  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_hlr(steps, warmup, seed=2024):
+    from augurv2_amd import diagnostics, runtime, workloads
+    from augurv2_amd._lib import HMC_BLOCK_THETA, HMC_BLOCK_V
+    N, D, C, L, EPS = 100_000, 64, 4, 20, 0.005
+    x, y, v_true, beta_true = workloads.hlr_data(N, D, seed=0)
+    blocks = [(HMC_BLOCK_V, 1.0, 0.05, 0), (HMC_BLOCK_THETA, 0.1, EPS, L)]
+    s = runtime.HlrSampler(x, y, blocks, chains=C, seed=seed, prior_kind=0, prior_a=1.0,
+                           prior_b=1.0, ref_compat=True)
+    for c, st in enumerate(workloads.hlr_init(D, C)):
+        s.set("state", st, chain=c)
+    log(f"hlr: warmup {warmup} sweeps")
+    s.step(warmup)
+    s.trace("state", steps)
+    s.profile(10)
+    log(f"hlr: timed {steps} sweeps")
+    t0 = time.perf_counter()
+    s.step(steps)  # returns after the stream is synchronised
+    t1 = time.perf_counter()
+    avg_us, n_timed = s.profile_read()
+    draws = s.read_trace("state")  # (steps, C, D + 2)
+    keep = list(range(D)) + [D + 1]  # theta and v (no intercept in this model)
+    summ = diagnostics.summarize(draws[:, :, keep])
+    acc = s.acceptance()
+    s.close()
+    elapsed = t1 - t0
+    min_ess = float(np.min(summ["ess_bulk"]))
+    return {
+        "elapsed": elapsed, "min_ess": min_ess, "rhat_max": float(np.max(summ["rhat"])),
+        "min_ess_tail": float(np.min(summ["ess_tail"])),
+        "accept": [float(a) for a in acc.mean(axis=1)], "pass_avg_us": avg_us,
+        "pass_launches_timed": n_timed, "N": N, "D": D, "C": C, "L": L, "eps": EPS,
+        "x": x, "y": y, "blocks": blocks, "post_mean_v": float(summ["mean"][-1]),
+    }
+
+
+def run_lda(sweeps=100, warmup=10):
+    from augurv2_amd import runtime, workloads
+    doc_off, w, alpha, beta, _ = workloads.lda_corpus(K=10, V=1000, D=200, mean_len=100, seed=0)
+    s = runtime.LdaSampler(10, doc_off, w, alpha, beta, chains=1, seed=7)
+    rng = np.random.default_rng(3)
+    s.set("z", rng.integers(0, 10, len(w)).astype(np.int32))
+    s.step(warmup)
+    s.profile(1)
+    t0 = time.perf_counter()
+    s.step(sweeps)
+    t1 = time.perf_counter()
+    zavg, _ = s.profile_read()
+    s.close()
+    T = int(doc_off[-1])
+    return {"tokens": T, "sweeps": sweeps, "tokens_per_s": T * sweeps / (t1 - t0),
+            "sweep_us": 1e6 * (t1 - t0) / sweeps, "zstep_avg_us": zavg}
+
+
+def cpu_baseline(hlr, min_ess_per_iter, budget_s=12.0):
+    """The FP64 oracle (a restatement of the reference's libcpu sampler: same
+    schedule, same RNG contract, L + 3 data passes per HMC iteration), one
+    thread, on the same data; timed over whole iterations of all chains
+    until the budget is spent."""
+    from oracle import oracle as orc
+    x64 = hlr["x"].astype(np.float64)
+    m = orc.Hlr(x64, hlr["y"], prior_kind=0, prior_a=1.0, prior_b=1.0, ref_compat=True)
+    from augurv2_amd import workloads
+    states = [st.copy() for st in workloads.hlr_init(hlr["D"], hlr["C"])]
+    kinds = {0: 0, 2: 2}
+    it = 0
+    t0 = time.perf_counter()
+    while True:
+        for c in range(hlr["C"]):
+            for bi, (kind, sl, eps, fx) in enumerate(hlr["blocks"]):
+                m.hmc_step(states[c], kinds[kind], bi, sl, eps, fx, 2024, c, it)
+        it += 1
+        if time.perf_counter() - t0 >= budget_s:
+            break
+    el = time.perf_counter() - t0
+    return {"value": min_ess_per_iter * it / el, "unit": "min-ESS/s", "cores": 1, "kind": "port",
+            "sample": f"{it} HMC sweeps x {hlr['C']} chains of the same workload on the FP64 "
+                      f"oracle in {el:.1f} s ({el / it:.2f} s/sweep); min-ESS per sweep taken "
+                      "from the GPU run (same algorithm and RNG streams)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=500)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the sampler "
+              "runtime runs on a single MI355X)", file=sys.stderr)
+        sys.exit(2)
+    hlr = run_hlr(args.steps, args.warmup)
+    log(f"hlr: {hlr['elapsed']:.3f} s, min ESS {hlr['min_ess']:.1f}, R-hat {hlr['rhat_max']:.4f}")
+    lda = run_lda()
+    log(f"lda: {lda['tokens_per_s']:.3e} tokens/s")
+    value = hlr["min_ess"] / hlr["elapsed"]
+    N, D = hlr["N"], hlr["D"]
+    bytes_per_pass = N * D * 4 + N * 4  # X (FP32) + y (int32): the algorithmic bytes
+    achieved = bytes_per_pass / (hlr["pass_avg_us"] * 1e-6) / 1e9
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(hlr, hlr["min_ess"] / args.steps)
+        log(f"cpu: {cpu['sample']}")
+    out = {
+        "metric": "min-ESS/sec on 1 GPU (HLR HMC, R-hat<1.01); LDA tokens/sec",
+        "value": value,
+        "unit": "min-ESS/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * hlr["elapsed"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "HLR HMC: N=100000 rows, D=64, 4 chains, HMC [v] (*) HMC [beta] "
+                               "with 20 leapfrog steps of 0.005, FP32 X",
+                   "global_batch": N, "chains": 4, "parallelism": "single GPU"},
+        "rhat_max": hlr["rhat_max"],
+        "min_ess": hlr["min_ess"],
+        "min_ess_tail": hlr["min_ess_tail"],
+        "accept_rate": hlr["accept"],
+        "lda_tokens_per_s": lda["tokens_per_s"],
+        "lda": {k: lda[k] for k in ("tokens", "sweeps", "sweep_us", "zstep_avg_us")},
+        "roofline": {"kernel": "k_hlr_pass", "bound": "hbm", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "avg_us": hlr["pass_avg_us"],
+                     "launches_timed": hlr["pass_launches_timed"]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

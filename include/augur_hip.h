@@ -148,6 +148,15 @@ augur_status augur_trace_read(augur_sampler* s, const char* name, void* dst, siz
 augur_status augur_stats(augur_sampler* s, double* out, size_t bytes);
 /* Number of sweeps run so far (the Philox sweep counter). */
 augur_status augur_sweep_count(augur_sampler* s, uint32_t* out);
+/* No counterpart in the reference (its only timing is time.clock() around
+ * samplen, experiments/util/timer.py:7-14).  Times the family's dominant
+ * kernel (HLR: the data pass; LDA and GMM: the z step) with HIP events on the
+ * sampler's stream: during every `period`-th sweep (0 disables) each launch
+ * of that kernel is bracketed by an event pair. */
+augur_status augur_profile(augur_sampler* s, int period);
+/* Average duration in microseconds of the launches timed since the last
+ * read, and their number; resets the count. */
+augur_status augur_profile_read(augur_sampler* s, double* avg_us, int* n_launches);
 augur_status augur_destroy(augur_sampler* s);
 
 /* ------------------------------------------------------------------------
