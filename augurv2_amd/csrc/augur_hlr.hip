@@ -15,10 +15,11 @@
 //                 X tile (64 rows) staged in LDS and shared by every chain,
 //                 t = x.theta + b, a = y - sigmoid(t), ll += log sigmoid(+-t),
 //                 grad += a x; FP32 arithmetic, FP64 accumulation across
-//                 tiles and blocks; the last block to finish reduces the block
-//                 partials in a fixed order (deterministic) and applies the
-//                 leapfrog update of every chain (momentum, position, FP32
-//                 copy of the next evaluation point).
+//                 tiles and blocks; block partials are reduced in two levels
+//                 in a fixed order (deterministic): the last block of each
+//                 group of 16 sums its group, the last group sums the groups
+//                 and applies the leapfrog update of every chain (momentum,
+//                 position, FP32 copy of the next evaluation point).
 //   k_hmc_end     kinetic energies, FP64 Metropolis test, accept / reject.
 //
 // The data log-density and gradient at the current point are cached (they
@@ -33,6 +34,7 @@ constexpr int kTileRows = 64;
 constexpr int kPassThreads = 256;
 constexpr int kMaxGradSlots = 16;  // (d) * cp <= 16 * 256
 constexpr int kMaxChains = 16;
+constexpr int kGroup = 16;  // blocks per first-level reduction group
 
 enum { MODE_CACHE = 0, MODE_LEAPFROG = 1 };
 
@@ -71,6 +73,8 @@ struct PassArgs {
   int n, d, cp, c_real, has_b;
   float* beta32;   // cp x (d + 1): evaluation point (theta, b) per chain
   double* part;    // [cp * (d + 2)][G]
+  double* gpart;   // [cp * (d + 2)][ceil(G / kGroup)]
+  unsigned* gcounter;  // per group of blocks
   unsigned* counter;
   int mode;
   int l;           // leapfrog pass index (1-based)
@@ -213,25 +217,41 @@ __global__ __launch_bounds__(kPassThreads) void k_hlr_pass(PassArgs A) {
     A.part[(size_t)(tid * vs + d + 1) * G + blockIdx.x] = sl;
   }
   (void)my_c;
-  // last block to finish reduces every value over the blocks in order
+  // Two-level deterministic reduction over blocks: the last block of each
+  // group of kGroup consecutive blocks sums the group's partials in block
+  // order; the last group to finish sums the group sums in group order.
+  // Each thread owns whole values, so its loads are independent.
+  const int nval = cp * vs;
+  const int grp = blockIdx.x / kGroup;
+  const int ngroups = (G + kGroup - 1) / kGroup;
+  const int g0 = grp * kGroup, gn = min(kGroup, G - g0);
   __threadfence();
   __syncthreads();
-  if (tid == 0) {
-    const unsigned prev = atomicAdd(A.counter, 1u);
-    *flag = (prev == (unsigned)(G - 1));
+  if (tid == 0) *flag = (atomicAdd(A.gcounter + grp, 1u) == (unsigned)(gn - 1));
+  __syncthreads();
+  if (!*flag) return;
+  __threadfence();
+  for (int v = tid; v < nval; v += kPassThreads) {
+    const double* pv = A.part + (size_t)v * G + g0;
+    double s = 0.0;
+#pragma unroll 16
+    for (int i = 0; i < gn; ++i) s += pv[i];
+    A.gpart[(size_t)v * ngroups + grp] = s;
   }
+  if (tid == 0) A.gcounter[grp] = 0u;
+  __threadfence();
+  __syncthreads();
+  if (tid == 0) *flag = (atomicAdd(A.counter, 1u) == (unsigned)(ngroups - 1));
   __syncthreads();
   if (!*flag) return;
   __threadfence();
   double* tot = (double*)smem;  // cp * (d + 2) doubles, aliases the tile
-  const int nval = cp * vs;
-  const int lane = tid & 63, wave = tid >> 6, nw = kPassThreads / 64;
-  for (int v = wave; v < nval; v += nw) {
-    const double* pv = A.part + (size_t)v * G;
+  for (int v = tid; v < nval; v += kPassThreads) {
+    const double* pv = A.gpart + (size_t)v * ngroups;
     double s = 0.0;
-    for (int g = lane; g < G; g += 64) s += pv[g];
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-    if (lane == 0) tot[v] = s;
+#pragma unroll 16
+    for (int i = 0; i < ngroups; ++i) s += pv[i];
+    tot[v] = s;
   }
   __syncthreads();
   const int P = d + 2;
@@ -475,7 +495,8 @@ struct HlrSampler final : augur_sampler {
   DevBuf<float> x, beta32;
   DevBuf<int32_t> y;
   DevBuf<double> state, prop, p0, p, k0, cache_g, cache_ll, gprop, llprop, part, stats_buf, ll_out;
-  DevBuf<unsigned> counter;
+  DevBuf<double> gpart;
+  DevBuf<unsigned> counter, gcounter;
   bool cache_valid = false;
 
   const char* family() const override { return "hlr"; }
@@ -497,7 +518,8 @@ struct HlrSampler final : augur_sampler {
   PassArgs pass_args(int mode, int l) {
     PassArgs A{};
     A.x = x.p; A.y = y.p; A.n = n; A.d = d; A.cp = cp; A.c_real = chains; A.has_b = has_b;
-    A.beta32 = beta32.p; A.part = part.p; A.counter = counter.p; A.mode = mode; A.l = l;
+    A.beta32 = beta32.p; A.part = part.p; A.gpart = gpart.p; A.gcounter = gcounter.p;
+    A.counter = counter.p; A.mode = mode; A.l = l;
     A.cache_g = cache_g.p; A.cache_ll = cache_ll.p; A.gprop = gprop.p; A.llprop = llprop.p;
     A.prop = prop.p; A.p = p.p; A.state = state.p;
     return A;
@@ -595,7 +617,9 @@ struct HlrSampler final : augur_sampler {
     n = n_; d = d_; chains = chains_;
     cp = next_pow2(chains);
     const int tiles = (n + kTileRows - 1) / kTileRows;
-    G = std::max(1, std::min(tiles, 256));
+    int want = 256;
+    if (const char* e = getenv("AUGUR_HLR_BLOCKS")) want = std::max(1, atoi(e));
+    G = std::max(1, std::min(tiles, want));
     const size_t P = (size_t)d + 2, ds = (size_t)d + 1;
     x.alloc((size_t)n * d);
     y.alloc(std::max(n, 1));
@@ -603,8 +627,12 @@ struct HlrSampler final : augur_sampler {
     state.alloc(cp * P); prop.alloc(cp * P);
     p0.alloc(cp * ds); p.alloc(cp * ds); k0.alloc(cp);
     cache_g.alloc(cp * ds); cache_ll.alloc(cp); gprop.alloc(cp * ds); llprop.alloc(cp);
+    const int ngroups = (G + kGroup - 1) / kGroup;
     part.alloc((size_t)cp * (d + 2) * G);
+    gpart.alloc((size_t)cp * (d + 2) * ngroups);
     counter.alloc(1);
+    gcounter.alloc(ngroups);
+    gcounter.zero(stream);
     ll_out.alloc(cp);
     stats_buf.alloc(std::max<size_t>(1, 4 * (size_t)chains * 2));
     state.zero(stream); prop.zero(stream); beta32.zero(stream); counter.zero(stream);

@@ -63,7 +63,8 @@ __device__ inline double block_sum(double v, double* red) {
 }
 
 // blocks [0, K): phi rows; blocks [K, K + ceil(D / waves)): theta rows, one wave each.
-__global__ __launch_bounds__(256) void k_lda_dirichlet(LdaDev m, uint32_t seed, uint32_t sweep) {
+// 1024-thread blocks: one Gamma draw per thread for a row of up to 1024 words.
+__global__ __launch_bounds__(1024) void k_lda_dirichlet(LdaDev m, uint32_t seed, uint32_t sweep) {
   __shared__ double red[16];
   const int c = blockIdx.y;
   const int b = blockIdx.x;
@@ -241,11 +242,12 @@ struct LdaSampler final : augur_sampler {
   }
 
   void run_sweeps(int n) override {
+    const int dwaves = 16;
+    const dim3 gd(m.K + (m.D + dwaves - 1) / dwaves, chains);
     const int waves = 4;
-    const dim3 gd(m.K + (m.D + waves - 1) / waves, chains);
     const int zblocks = std::max(1, std::min((m.D + waves - 1) / waves, 2048));
     for (int i = 0; i < n; ++i) {
-      k_lda_dirichlet<<<gd, 256, 0, stream>>>(m, seed, sweep);
+      k_lda_dirichlet<<<gd, 1024, 0, stream>>>(m, seed, sweep);
       AUGUR_CHECK_LAUNCH();
       prof_begin();
       k_lda_z<<<dim3(zblocks, chains), 256, 0, stream>>>(m, seed, sweep, 0u);

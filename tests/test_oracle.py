@@ -1,0 +1,296 @@
+"""Pins the CPU oracle against sources independent of it: published
+known-answer vectors, numpy / scipy closed forms, the golden fixtures of
+tests/golden/, finite differences and exact-target statistical tests."""
+import os
+
+import numpy as np
+import pytest
+import scipy.stats as sps
+
+GOLD = np.load(os.path.join(os.path.dirname(__file__), "golden", "golden.npz"))
+
+
+def test_philox_known_answers(oracle):
+    for c, k, o in zip(GOLD["philox_ctr"], GOLD["philox_key"], GOLD["philox_out"]):
+        assert (oracle.philox(c, k) == o).all()
+
+
+def test_uniform_conversions(oracle):
+    L = oracle.lib()
+    assert L.or_u01(0) == 0.5 * 2.0**-32
+    assert L.or_u01(0xFFFFFFFF) == (2.0**32 - 0.5) * 2.0**-32 < 1.0
+    assert L.or_u01f(0) == 0.0
+    assert L.or_u01f(0xFFFFFFFF) == np.float32((2**24 - 1) * 2.0**-24)
+    assert L.or_u01f(0x80000000) == 0.5
+
+
+def test_normal_distribution(oracle):
+    z = np.array([oracle.lib().or_normal(1, 2, i, 0, 3, 9) for i in range(40000)])
+    assert abs(z.mean()) < 4 / np.sqrt(len(z))
+    assert abs(z.var() - 1) < 0.03
+    assert sps.kstest(z, "norm").pvalue > 1e-3
+
+
+@pytest.mark.parametrize("shape", [0.05, 0.5, 1.0, 2.5, 40.0])
+def test_gamma_distribution(oracle, shape):
+    g = np.array([oracle.gamma(3, 0, i, 5, 1, shape) for i in range(20000)])
+    assert np.all(g >= 0)
+    assert sps.kstest(g, sps.gamma(shape).cdf).pvalue > 1e-3
+    assert abs(g.mean() - shape) < 5 * np.sqrt(shape / len(g))
+
+
+def test_gamma_invalid_shape(oracle):
+    assert np.isnan(oracle.gamma(0, 0, 0, 0, 1, 0.0))
+    assert np.isnan(oracle.gamma(0, 0, 0, 0, 1, -2.0))
+    assert np.isnan(oracle.gamma(0, 0, 0, 0, 1, float("nan")))
+
+
+def test_dirichlet_conj_moments(oracle):
+    alpha = np.array([0.1, 0.5, 2.0, 1.0])
+    cnt = np.array([3, 0, 7, 1], np.int32)
+    a = alpha + cnt
+    draws = np.array([oracle.dirichlet_conj(1, 0, s, 1, 0, alpha, cnt) for s in range(20000)])
+    np.testing.assert_allclose(draws.sum(1), 1.0, rtol=1e-12)
+    mean = a / a.sum()
+    var = mean * (1 - mean) / (a.sum() + 1)
+    assert np.all(np.abs(draws.mean(0) - mean) < 5 * np.sqrt(var / len(draws)))
+    assert np.all(np.abs(draws.var(0) / var - 1) < 0.1)
+
+
+def test_categorical_reference_form(oracle):
+    """augur_unnorm_disc_samp == inverse CDF of the normalised pmf."""
+    rng = np.random.default_rng(0)
+    L = oracle.lib()
+    for _ in range(2000):
+        k = rng.integers(1, 12)
+        pmf = rng.random(k) * (rng.random(k) < 0.8)
+        if pmf.sum() == 0:
+            pmf[0] = 1.0
+        u = rng.random()
+        cdf = np.cumsum(pmf) / pmf.sum()
+        expect = min(int(np.searchsorted(cdf, u, side="right")), k - 1)
+        assert L.or_categorical_ref(np.ascontiguousarray(pmf), k, u) == expect
+
+
+def test_categorical_nan_pmf_falls_to_last(oracle):
+    pmf = np.zeros(4)  # all-zero pmf: norm 0 -> NaN comparisons -> last bin
+    assert oracle.lib().or_categorical_ref(pmf, 4, 0.3) == 3
+
+
+def test_categorical_kernel_forms_agree(oracle):
+    """The FP32 kernel form and the log-space form pick the reference bin
+    except within rounding distance of a bin edge."""
+    rng = np.random.default_rng(1)
+    L = oracle.lib()
+    bad = 0
+    for _ in range(3000):
+        k = rng.integers(2, 40)
+        a = rng.random(k).astype(np.float32)
+        b = rng.random(k).astype(np.float32)
+        u = np.float32(rng.random())
+        pmf = a.astype(np.float64) * b.astype(np.float64)
+        r = L.or_categorical_ref(np.ascontiguousarray(pmf), k, float(u))
+        f = L.or_categorical_f32(a, b, k, u)
+        lg = L.or_categorical_logits(np.ascontiguousarray(np.log(pmf)), k, float(u))
+        assert lg == r or np.min(np.abs(np.cumsum(pmf) / pmf.sum() - u)) < 1e-12
+        if f != r:
+            bad += 1
+            assert np.min(np.abs(np.cumsum(pmf) / pmf.sum() - u)) < 1e-5
+    assert bad <= 3
+
+
+@pytest.mark.parametrize("n", [2, 5, 16])
+def test_cholesky_family_golden(oracle, n):
+    spd = GOLD[f"spd{n}"]
+    l = oracle.cholesky(spd)
+    np.testing.assert_allclose(np.tril(l), GOLD[f"chol{n}"], rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(np.triu(l, 1), np.triu(spd, 1))  # upper part untouched
+    li = oracle.lower_trmi(GOLD[f"chol{n}"])
+    np.testing.assert_allclose(li, GOLD[f"linv{n}"], rtol=1e-10, atol=1e-12)
+    inv = oracle.cholesky_invert(l)
+    np.testing.assert_allclose(inv, GOLD[f"inv{n}"], rtol=1e-10, atol=1e-12)
+    l1 = GOLD[f"chol{n}"]
+    l2 = np.tril(np.random.default_rng(n).standard_normal((n, n)))
+    np.testing.assert_allclose(oracle.trtrmm(l1, l2), l1 @ l2, rtol=1e-12, atol=1e-12)
+    v = np.arange(1.0, n + 1)
+    np.testing.assert_allclose(oracle.trvm(l1, v), l1 @ v, rtol=1e-12)
+    ll = oracle.mvnormal_ll(GOLD[f"mvn_pt{n}"], GOLD[f"mvn_mean{n}"], spd)
+    np.testing.assert_allclose(ll, GOLD[f"mvn_ll{n}"], rtol=1e-12)
+
+
+def test_mvnormal_conj_posterior(oracle):
+    """mu | y ~ N(A (S0^-1 mu0 + n S^-1 ybar), A), A = (S0^-1 + n S^-1)^-1."""
+    rng = np.random.default_rng(2)
+    d, n = 3, 7
+    mu0 = np.array([1.0, -1.0, 0.5])
+    a = rng.standard_normal((d, d))
+    cov0 = a @ a.T + np.eye(d)
+    b = rng.standard_normal((d, d))
+    cov = b @ b.T + 0.5 * np.eye(d)
+    s = rng.standard_normal(d) * 4
+    A = np.linalg.inv(np.linalg.inv(cov0) + n * np.linalg.inv(cov))
+    mean = A @ (np.linalg.inv(cov0) @ mu0 + np.linalg.inv(cov) @ s)
+    draws = np.array([oracle.mvnormal_conj(5, 0, i, 0, mu0, cov0, s, n, cov) for i in range(20000)])
+    se = np.sqrt(np.diag(A) / len(draws))
+    assert np.all(np.abs(draws.mean(0) - mean) < 5 * se)
+    np.testing.assert_allclose(np.cov(draws.T), A, rtol=0.06, atol=0.02 * np.abs(A).max())
+    # n = 0: the prior
+    d0 = np.array([oracle.mvnormal_conj(6, 0, i, 0, mu0, cov0, np.zeros(d), 0, cov)
+                   for i in range(20000)])
+    assert np.all(np.abs(d0.mean(0) - mu0) < 5 * np.sqrt(np.diag(cov0) / len(d0)))
+
+
+def test_invwishart_marginal(oracle):
+    """For Sigma ~ IW(nu, Psi): Sigma_ii ~ InvGamma((nu - p + 1) / 2, Psi_ii / 2),
+    E[Sigma] = Psi / (nu - p - 1)."""
+    rng = np.random.default_rng(3)
+    p, nu = 4, 9
+    a = rng.standard_normal((p, p))
+    psi = a @ a.T + np.eye(p)
+    draws = np.array([oracle.invwishart(7, 0, i, 0, nu, psi) for i in range(20000)])
+    np.testing.assert_allclose(draws, np.transpose(draws, (0, 2, 1)), rtol=1e-10, atol=1e-12)
+    for i in range(p):
+        ref = sps.invgamma((nu - p + 1) / 2, scale=psi[i, i] / 2)
+        assert sps.kstest(draws[:, i, i], ref.cdf).pvalue > 1e-3
+    np.testing.assert_allclose(draws.mean(0), psi / (nu - p - 1), rtol=0.12,
+                               atol=0.05 * np.abs(psi).max())
+
+
+def test_hlr_ll_grad_golden_and_fd(oracle):
+    x, y = GOLD["hlr_x"], GOLD["hlr_y"]
+    d = x.shape[1]
+    m = oracle.Hlr(x, y, has_intercept=True)
+    st = np.concatenate([GOLD["hlr_beta"], [float(GOLD["hlr_b"])], [1.0]])
+    np.testing.assert_allclose(m.data_ll(st, stable=True), GOLD["hlr_ll"], rtol=1e-12)
+    np.testing.assert_allclose(m.data_ll(st, stable=False), GOLD["hlr_ll"], rtol=1e-9)
+    g, gb = m.data_grad(st)
+    np.testing.assert_allclose(g, GOLD["hlr_grad"], rtol=1e-10, atol=1e-10)
+    np.testing.assert_allclose(gb, GOLD["hlr_grad_b"], rtol=1e-10, atol=1e-10)
+    gr, gbr = m.data_grad(st, ref_form=True)  # bernoulli_dotarg1 * expit_grad_0
+    np.testing.assert_allclose(gr, g, rtol=1e-8, atol=1e-8)
+    # central differences of the block log-density (theta, b), textbook mode
+    m2 = oracle.Hlr(x, y, has_intercept=True, ref_compat=False)
+    gfull = m2.block_grad(st, 3)
+    h = 1e-6
+    for j in list(range(d)) + [d]:
+        e = np.zeros_like(st)
+        e[j] = h
+        fd = (m2.block_ll(st + e, 3) - m2.block_ll(st - e, 3)) / (2 * h)
+        assert abs(fd - gfull[j]) < 1e-4 * max(1.0, abs(fd))
+
+
+def test_normal_ll_and_quirks(oracle):
+    L = oracle.lib()
+    # textbook mode is the scipy density up to the -0.5 log(2 pi) constant
+    for x, var in [(0.3, 2.0), (-1.5, 0.25)]:
+        ref = sps.norm(0, np.sqrt(var)).logpdf(x) + 0.5 * np.log(2 * np.pi)
+        np.testing.assert_allclose(L.or_normal_ll(x, 0.0, var, 0), ref, rtol=1e-13)
+        # reference mode: -log(var) (cbits/augur_dist.c:376)
+        np.testing.assert_allclose(L.or_normal_ll(x, 0.0, var, 1), -np.log(var) - 0.5 * x * x / var)
+        # textbook dotarg2 is d/dvar of the textbook ll
+        h = 1e-6
+        fd = (L.or_normal_ll(x, 0.0, var + h, 0) - L.or_normal_ll(x, 0.0, var - h, 0)) / (2 * h)
+        np.testing.assert_allclose(L.or_normal_dotarg2(x, 0.0, var, 0), fd, rtol=1e-6)
+        # reference dotarg2 keeps -0.5 * var (cbits/augur_dist.c:393)
+        np.testing.assert_allclose(L.or_normal_dotarg2(x, 0.0, var, 1),
+                                   -0.5 * var + 0.5 * x * x / var**2)
+    assert L.or_normal_ll(1.0, 0.0, -1.0, 1) == -1e8
+    np.testing.assert_allclose(L.or_invgamma_ll(0.7, 2.0, 3.0), sps.invgamma(2.0, scale=3.0).logpdf(0.7))
+    np.testing.assert_allclose(L.or_exponential_ll(0.7, 2.0), sps.expon(scale=0.5).logpdf(0.7))
+    assert L.or_exponential_ll(-1.0, 2.0) == -1e8
+
+
+def _hmc_chain(oracle, m, st, block, iters, sim_len, eps, fixed, seed=11):
+    out = []
+    for it in range(iters):
+        m.hmc_step(st, block, 0, sim_len, eps, fixed, seed, 0, it)
+        out.append(st.copy())
+    return np.array(out)
+
+
+def test_hmc_exact_gaussian_target(oracle):
+    """With no data the theta block targets prod N(theta_j; 0, v) exactly."""
+    x = np.zeros((0, 3))
+    y = np.zeros(0, np.int32)
+    m = oracle.Hlr(x, y)
+    v = 2.0
+    st = np.array([3.0, -2.0, 0.5, 0.0, v])
+    draws = _hmc_chain(oracle, m, st, 2, 6000, 1.5, 0.2, 0)[1000:, :3]
+    from augurv2_amd import diagnostics
+    for j in range(3):
+        xj = draws[:, j][None, :]
+        mcse = diagnostics.mcse_mean(xj)
+        assert abs(xj.mean()) < 4 * mcse
+    assert abs(draws.var() / v - 1) < 0.1
+
+
+@pytest.mark.parametrize("ref_compat,shape_add", [(False, 0.5), (True, 1.0)])
+def test_hmc_variance_block_target(oracle, ref_compat, shape_add):
+    """v | theta ~ InvGamma(a + s K, b + sum theta^2 / 2) with s = 1/2 for the
+    textbook Normal and s = 1 for the reference's -log(var) normal_ll."""
+    theta = np.array([0.5, -1.0, 1.5, 0.2, -0.7, 1.1])
+    m = oracle.Hlr(np.zeros((0, 6)), np.zeros(0, np.int32), prior_kind=0, prior_a=2.0,
+                   prior_b=1.0, ref_compat=ref_compat)
+    st = np.concatenate([theta, [0.0], [1.0]])
+    draws = _hmc_chain(oracle, m, st, 0, 8000, 1.0, 0.05, 0)[1000:, -1]
+    a, b = 2.0 + shape_add * len(theta), 1.0 + 0.5 * np.sum(theta**2)
+    from augurv2_amd import diagnostics
+    mcse = diagnostics.mcse_mean(draws[None, :])
+    assert abs(draws.mean() - b / (a - 1)) < 4 * mcse
+
+
+def test_lda_counts_brute_force(oracle):
+    w = GOLD["lda_w"]
+    doc_off = np.arange(0, w.size + 1, w.shape[1]).astype(np.int32)
+    rng = np.random.default_rng(4)
+    z = rng.integers(0, 4, w.size).astype(np.int32)
+    ct, cp = oracle.lda_counts(4, 4, 8, doc_off, w.ravel(), z)
+    ct_ref = np.zeros((4, 4), int)
+    cp_ref = np.zeros((4, 8), int)
+    d_of = np.repeat(np.arange(4), w.shape[1])
+    np.add.at(ct_ref, (d_of, z), 1)
+    np.add.at(cp_ref, (z, w.ravel()), 1)
+    assert (ct == ct_ref).all() and (cp == cp_ref).all()
+
+
+def test_lda_recovers_designed_topics(oracle):
+    """examples/lda.py:41-70: words (0,1), (2,3), (4,5), (6,7) are designed
+    topic pairs; the posterior keeps each pair in one topic."""
+    w = GOLD["lda_w"].ravel().astype(np.int32)
+    doc_off = np.arange(0, 41, 10).astype(np.int32)
+    K, V = 4, 8
+    alpha, beta = np.full(K, 0.1), np.full(V, 0.01)
+    same = []
+    for chain in range(4):
+        rng = np.random.default_rng(chain)
+        theta, phi = np.zeros((4, K)), np.zeros((K, V))
+        z = rng.integers(0, K, len(w)).astype(np.int32)
+        for s in range(300):
+            oracle.lda_sweep(doc_off, w, alpha, beta, 3, chain, s, theta, phi, z)
+            if s >= 100:
+                top = phi.argmax(0)
+                same.append([top[0] == top[1], top[2] == top[3], top[4] == top[5],
+                             top[6] == top[7]])
+    same = np.array(same)
+    # (2, 3) and (4, 5) co-occur in two documents each and are never split;
+    # (0, 1) and (6, 7) also admit the two-topic explanation of document 4
+    assert same[:, 1].mean() > 0.9 and same[:, 2].mean() > 0.9
+    assert same.mean() > 0.75
+
+
+def test_gmm_sweep_recovers_clusters(oracle):
+    """examples/mvgmm.py:39-62 generator (K = 3, D = 2, covs 0.1 I) with the
+    hmvgmm schedule: means recovered up to label switching."""
+    rng = np.random.default_rng(5)
+    centers = np.array([[0.0, 2.0], [2.0, -2.0], [-2.0, -2.0]])
+    n = 300
+    y = np.array([rng.multivariate_normal(centers[i % 3], 0.1 * np.eye(2)) for i in range(n)])
+    g = oracle.Gmm(y, np.ones(3), np.zeros(2), np.eye(2) * 10, 4, np.eye(2))
+    pis = np.full(3, 1 / 3)
+    mu = rng.standard_normal((3, 2))
+    cov = np.stack([np.eye(2)] * 3)
+    z = rng.integers(0, 3, n).astype(np.int32)
+    for s in range(60):
+        g.sweep(pis, mu, cov, z, 1, 0, s)
+    found = sorted(map(tuple, np.round(mu, 0)))
+    assert found == sorted(map(tuple, centers))
