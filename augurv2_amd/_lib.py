@@ -42,6 +42,14 @@ class HlrDesc(C.Structure):
                 ("blocks", HmcBlock * 4)]
 
 
+class BhlrDesc(C.Structure):
+    _fields_ = [("n_datasets", C.c_int), ("n_rows", C.c_void_p), ("n_features", C.c_void_p),
+                ("x", C.c_void_p), ("y", C.c_void_p), ("chains", C.c_int), ("seed", C.c_uint32),
+                ("prior_kind", C.c_int), ("prior_a", C.c_double), ("prior_b", C.c_double),
+                ("ref_compat", C.c_int), ("v_block", HmcBlock), ("theta_block", HmcBlock),
+                ("rows_per_block", C.c_int), ("use_graph", C.c_int)]
+
+
 class GmmDesc(C.Structure):
     _fields_ = [("n_points", C.c_int), ("dim", C.c_int), ("n_components", C.c_int),
                 ("chains", C.c_int), ("seed", C.c_uint32), ("y", C.c_void_p),
@@ -64,6 +72,8 @@ SIGNATURES = {
     "augur_lda_create": [_P, _P],
     "augur_hlr_create": [_P, _P],
     "augur_gmm_create": [_P, _P],
+    "augur_bhlr_create": [_P, _P],
+    "augur_bhlr_tuning": [_H, _P, _P, _P, _P, _I],
     "augur_set_pt": [_H, C.c_char_p, _I, _P, _S],
     "augur_cpy": [_H, C.c_char_p, _I, _P, _S],
     "augur_step": [_H, _I],

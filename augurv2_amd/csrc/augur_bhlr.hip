@@ -1,0 +1,757 @@
+// augur_bhlr.hip -- batched HMC for many independent hierarchical logistic
+// regressions (datasets of uneven size and width) in one launch per leapfrog
+// step, on gfx950.
+//
+// Each dataset i follows hlr.rv without intercept: v_i ~ InvGamma(a, b) |
+// Exponential(a); theta_i[k] ~ Normal(0, v_i); y_i[n] ~ Bernoulli(sigmoid(
+// dotprod(theta_i, x_i[n]))), with C chains per dataset and the schedule
+// HMC [v] (*) HMC [theta] (fixed leapfrog count for theta).
+//
+// Layout in HBM: every dataset's X stored BF16, row-major, back to back; y as
+// uint8.  The pass kernel gives each dataset ceil(N_i / R) blocks of R rows (R
+// chosen by the autotuner at creation); a block stages 64-row tiles in LDS
+// (BF16 -> FP32), computes t = x . theta for all chains of its dataset,
+// a = y - sigmoid(t), log sigmoid(+-t) and a x (FP32 arithmetic, FP64
+// accumulation), and writes its partials; the last block of each dataset sums
+// that dataset's partials in block order (deterministic) and applies the
+// leapfrog update of its chains.  One sweep (v block, momentum draw, L
+// passes, Metropolis step, sweep counter) can be captured in a hipGraph and
+// replayed: every kernel reads the sweep counter from device memory.
+//
+// Philox chain key of (dataset i, chain c) is i * C + c.
+#include "augur_hmc.h"
+
+namespace augur {
+
+constexpr int kBTile = 64;
+constexpr int kBThreads = 256;
+
+struct BDs {
+  int n, d;
+  int nblk, blk0;  // pass blocks of this dataset and the first one
+  int poff;        // partials: [value][nblk] from here, value = c * (d + 2) + j
+  int soff;        // offset in the per-chain state vector ([theta, b, v] per dataset)
+  int goff;        // offset in the per-chain (d + 1)-wide vectors
+  const uint16_t* x;
+  const uint8_t* y;
+};
+
+struct BArgs {
+  const BDs* ds;
+  const int* blk_ds;
+  int n_ds, C, SW, GW, rows_per_blk;
+  float* beta32;     // [C][GW]
+  double* part;
+  unsigned* counter; // per dataset
+  int mode, l;
+  double* cache_g;   // [C][GW]
+  double* cache_ll;  // [C][n_ds]
+  double* gprop;     // [C][GW]
+  double* llprop;    // [C][n_ds]
+  double* prop;      // [C][SW]
+  double* p;         // [C][GW]
+  double* p0;        // [C][GW]
+  double* k0;        // [C][n_ds]
+  double* state;     // [C][SW]
+  double* stats;     // [block 0 v | 1 theta][C][n_ds][2]
+  const uint32_t* sweep;  // device sweep counter
+  uint32_t seed;
+  int prior_kind, ref_compat;
+  double prior_a, prior_b;
+  double step;       // theta block: fixed step size and count
+  int steps;
+  double v_sim_len, v_eps;
+  int v_fixed;
+};
+
+enum { BMODE_CACHE = 0, BMODE_LEAPFROG = 1 };
+
+__device__ inline float bf16f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+template <int D, int CP>
+__device__ void bhlr_body(const BArgs& A, const BDs& ds, int di, unsigned char* smem) {
+  constexpr int DS = D + 4;
+  constexpr int TR = kBTile;
+  constexpr int NV = TR * D / 8;  // uint4 (8 BF16) per tile
+  constexpr int NVT = (NV + kBThreads - 1) / kBThreads;
+  constexpr int NB = (TR * CP + kBThreads - 1) / kBThreads;
+  constexpr int NP = CP * (D / 4);
+  constexpr int RS = NP >= kBThreads ? 1 : kBThreads / NP;
+  constexpr int NS = NP > kBThreads ? NP / kBThreads : 1;
+  constexpr int VS = D + 2;
+  float* xs = (float*)smem;                 // TR x DS
+  float* bs = xs + TR * DS;                 // CP x DS
+  float* as = bs + CP * DS;                 // TR x CP
+  int* ys = (int*)(as + TR * CP);           // TR
+  double* red = (double*)(((uintptr_t)(ys + TR) + 15) & ~(uintptr_t)15);  // 2 x 256
+  double* red2 = red + 2 * kBThreads;       // 4 x 256
+  int* flag = (int*)(red2 + 4 * kBThreads);
+
+  const int tid = threadIdx.x;
+  const int local = blockIdx.x - ds.blk0;
+  const int rbeg = local * A.rows_per_blk;
+  const int rend = min(ds.n, rbeg + A.rows_per_blk);
+  for (int i = tid; i < CP * D; i += kBThreads) {
+    const int c = i / D, j = i % D;
+    bs[c * DS + j] = c < A.C ? A.beta32[(size_t)c * A.GW + ds.goff + j] : 0.0f;
+  }
+  __syncthreads();
+  const int my_c = tid % CP;
+  constexpr bool BREG = D <= 64;  // theta_c in registers, else read from LDS
+  float breg[BREG ? D : 1];
+  if constexpr (BREG) {
+#pragma unroll
+    for (int j = 0; j < D; ++j) breg[j] = bs[my_c * DS + j];
+  }
+  double gacc[NS][4];
+#pragma unroll
+  for (int s = 0; s < NS; ++s)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) gacc[s][k] = 0.0;
+  double llacc = 0.0, gbacc = 0.0;
+  uint4 pre[NVT];
+  int ypre = 0;
+  auto prefetch = [&](int r0) {
+    const int rows = min(TR, rend - r0);
+    const uint4* s8 = (const uint4*)(ds.x + (size_t)r0 * D);
+#pragma unroll
+    for (int k = 0; k < NVT; ++k) {
+      const int i = tid + k * kBThreads;
+      pre[k] = (i < NV && i < rows * (D / 8)) ? s8[i] : make_uint4(0u, 0u, 0u, 0u);
+    }
+    ypre = (tid < TR && tid < rows) ? (int)ds.y[r0 + tid] : 0;
+  };
+  if (rbeg < rend) prefetch(rbeg);
+  for (int r0 = rbeg; r0 < rend; r0 += TR) {
+    const int rows = min(TR, rend - r0);
+#pragma unroll
+    for (int k = 0; k < NVT; ++k) {
+      const int i = tid + k * kBThreads;
+      if (i < NV) {
+        const int e = 8 * i;
+        float* dst = xs + (e / D) * DS + e % D;
+        const uint4 v = pre[k];
+        *(float4*)dst = make_float4(bf16f(v.x & 0xffff), bf16f(v.x >> 16), bf16f(v.y & 0xffff),
+                                    bf16f(v.y >> 16));
+        *(float4*)(dst + 4) = make_float4(bf16f(v.z & 0xffff), bf16f(v.z >> 16),
+                                          bf16f(v.w & 0xffff), bf16f(v.w >> 16));
+      }
+    }
+    if (tid < TR) ys[tid] = ypre;
+    __syncthreads();
+    if (r0 + TR < rend) prefetch(r0 + TR);
+#pragma unroll
+    for (int m = 0; m < NB; ++m) {
+      const int q = tid + kBThreads * m;
+      if (q < TR * CP) {
+        const int r = q / CP;
+        const float* xr = xs + r * DS;
+        float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+#pragma unroll
+        for (int j = 0; j < D; j += 4) {
+          const float4 xv = *(const float4*)(xr + j);
+          float4 bv;
+          if constexpr (BREG) bv = make_float4(breg[j], breg[j + 1], breg[j + 2], breg[j + 3]);
+          else bv = *(const float4*)(bs + my_c * DS + j);
+          s0 = fmaf(xv.x, bv.x, s0);
+          s1 = fmaf(xv.y, bv.y, s1);
+          s2 = fmaf(xv.z, bv.z, s2);
+          s3 = fmaf(xv.w, bv.w, s3);
+        }
+        const float t = (s0 + s1) + (s2 + s3);
+        float a;
+        const float lr = log_sigmoid_pm(t, ys[r], &a);
+        if (r < rows) {
+          llacc += (double)lr;
+          gbacc += (double)a;
+        } else {
+          a = 0.0f;
+        }
+        as[r * CP + my_c] = a;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const int pair = (NS > 1) ? tid + s * kBThreads : tid % NP;
+      const int rs = (NS > 1) ? 0 : tid / NP;
+      const int c = pair / (D / 4), jq = pair % (D / 4);
+      float4 f = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll 4
+      for (int r = rs * (TR / RS); r < (rs + 1) * (TR / RS); ++r) {
+        const float av = as[r * CP + c];
+        const float4 xv = *(const float4*)(xs + r * DS + 4 * jq);
+        f.x = fmaf(av, xv.x, f.x);
+        f.y = fmaf(av, xv.y, f.y);
+        f.z = fmaf(av, xv.z, f.z);
+        f.w = fmaf(av, xv.w, f.w);
+      }
+      gacc[s][0] += (double)f.x;
+      gacc[s][1] += (double)f.y;
+      gacc[s][2] += (double)f.z;
+      gacc[s][3] += (double)f.w;
+    }
+    __syncthreads();
+  }
+  // partials of this block: part[poff + value * nblk + local]
+  double* part = A.part + ds.poff;
+  if constexpr (RS > 1) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) red2[tid * 4 + k] = gacc[0][k];
+    __syncthreads();
+    if (tid < NP) {
+      const int c = tid / (D / 4), jq = tid % (D / 4);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        double s = 0.0;
+        for (int rs = 0; rs < RS; ++rs) s += red2[(rs * NP + tid) * 4 + k];
+        part[(size_t)(c * VS + 4 * jq + k) * ds.nblk + local] = s;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const int pair = tid + s * kBThreads;
+      const int c = pair / (D / 4), jq = pair % (D / 4);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) part[(size_t)(c * VS + 4 * jq + k) * ds.nblk + local] = gacc[s][k];
+    }
+  }
+  red[tid] = gbacc;
+  red[kBThreads + tid] = llacc;
+  __syncthreads();
+  if (tid < CP) {
+    double sb = 0.0, sl = 0.0;
+    for (int i = tid; i < kBThreads; i += CP) {
+      sb += red[i];
+      sl += red[kBThreads + i];
+    }
+    part[(size_t)(tid * VS + D) * ds.nblk + local] = sb;
+    part[(size_t)(tid * VS + D + 1) * ds.nblk + local] = sl;
+  }
+  // the last block of the dataset reduces its partials in block order
+  __threadfence();
+  __syncthreads();
+  if (tid == 0) *flag = (atomicAdd(A.counter + di, 1u) == (unsigned)(ds.nblk - 1));
+  __syncthreads();
+  if (!*flag) return;
+  __threadfence();
+  double* tot = (double*)smem;  // CP * VS doubles, aliases the tile
+  for (int v = tid; v < A.C * VS; v += kBThreads) {
+    const double* pv = part + (size_t)v * ds.nblk;
+    double s = 0.0;
+#pragma unroll 8
+    for (int b = 0; b < ds.nblk; ++b) s += pv[b];
+    tot[v] = s;
+  }
+  __syncthreads();
+  const int C = A.C;
+  for (int q = tid; q < C * D; q += kBThreads) {
+    const int c = q / D, j = q % D;
+    const size_t gi = (size_t)c * A.GW + ds.goff + j;
+    const double g_data = tot[c * VS + j];
+    if (A.mode == BMODE_CACHE) {
+      A.cache_g[gi] = g_data;
+      if (j == 0) A.cache_ll[(size_t)c * A.n_ds + di] = tot[c * VS + D + 1];
+      continue;
+    }
+    const int L = A.steps;
+    if (A.l == L) {
+      A.gprop[gi] = g_data;
+      if (j == 0) A.llprop[(size_t)c * A.n_ds + di] = tot[c * VS + D + 1];
+    }
+    const size_t si = (size_t)c * A.SW + ds.soff;
+    const double v = A.state[si + D + 1];
+    double xv = A.prop[si + j];
+    const double g = g_data + (-xv / v);
+    const double coef = (A.l == L) ? 0.5 * A.step : A.step;
+    const double pj = A.p[gi] + coef * g;
+    A.p[gi] = pj;
+    if (A.l < L) {
+      xv += A.step * pj;
+      A.prop[si + j] = xv;
+      A.beta32[gi] = (float)xv;
+    }
+  }
+  if (tid == 0) A.counter[di] = 0u;
+}
+
+template <int CP>
+__global__ __launch_bounds__(kBThreads) void k_bhlr_pass(BArgs A) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int di = A.blk_ds[blockIdx.x];
+  const BDs ds = A.ds[di];
+  switch (ds.d) {
+    case 16: bhlr_body<16, CP>(A, ds, di, smem); break;
+    case 32: bhlr_body<32, CP>(A, ds, di, smem); break;
+    case 64: bhlr_body<64, CP>(A, ds, di, smem); break;
+    default: bhlr_body<128, CP>(A, ds, di, smem); break;
+  }
+}
+
+size_t bhlr_smem_bytes(int dmax, int cp) {
+  size_t b = sizeof(float) * ((size_t)kBTile * (dmax + 4) + (size_t)cp * (dmax + 4) + (size_t)kBTile * cp) +
+             sizeof(int) * kBTile;
+  b = (b + 15) & ~(size_t)15;
+  b += sizeof(double) * 6 * kBThreads + 16;
+  const size_t tot = sizeof(double) * (size_t)cp * (dmax + 2);
+  return b > tot ? b : tot;
+}
+
+// One wave per (dataset, chain): momentum draw, half step, first position step.
+__global__ __launch_bounds__(64) void k_bhmc_begin(BArgs A) {
+  const int di = blockIdx.x, c = blockIdx.y, lane = threadIdx.x;
+  const BDs ds = A.ds[di];
+  const int D = ds.d;
+  const uint32_t gc = (uint32_t)(di * A.C + c);
+  const Stream mom{A.seed, gc, *A.sweep, TAG_HMC_BASE + 4u + 0u};
+  const size_t si = (size_t)c * A.SW + ds.soff, gi = (size_t)c * A.GW + ds.goff;
+  const double v = A.state[si + D + 1];
+  double kin = 0.0;
+  for (int j = lane; j < D + 2; j += 64) {
+    const double sv = A.state[si + j];
+    double xv = sv;
+    if (j < D) {
+      const double p0 = normal_at(mom, (uint32_t)j, 0);
+      const double g = A.cache_g[gi + j] + (-sv / v);
+      const double pj = p0 + 0.5 * A.step * g;
+      A.p0[gi + j] = p0;
+      A.p[gi + j] = pj;
+      kin += p0 * p0;
+      xv = sv + A.step * pj;
+      A.beta32[gi + j] = (float)xv;
+    }
+    A.prop[si + j] = xv;
+  }
+  kin = wave_sum(kin);
+  if (lane == 0) A.k0[(size_t)c * A.n_ds + di] = 0.5 * kin;
+}
+
+// One wave per (dataset, chain): energies, FP64 Metropolis test, accept.
+__global__ __launch_bounds__(64) void k_bhmc_end(BArgs A) {
+  const int di = blockIdx.x, c = blockIdx.y, lane = threadIdx.x;
+  const BDs ds = A.ds[di];
+  const int D = ds.d;
+  const uint32_t gc = (uint32_t)(di * A.C + c);
+  const size_t si = (size_t)c * A.SW + ds.soff, gi = (size_t)c * A.GW + ds.goff;
+  const size_t ci = (size_t)c * A.n_ds + di;
+  const double v = A.state[si + D + 1];
+  double kin = 0.0, lp0 = 0.0, lp1 = 0.0;
+  for (int j = lane; j < D; j += 64) {
+    const double pj = A.p[gi + j];
+    kin += pj * pj;
+    lp0 += normal_ll0(A.state[si + j], v, A.ref_compat);
+    lp1 += normal_ll0(A.prop[si + j], v, A.ref_compat);
+  }
+  kin = wave_sum(kin);
+  lp0 = wave_sum(lp0);
+  lp1 = wave_sum(lp1);
+  const double ll0 = A.cache_ll[ci] + lp0, ll1 = A.llprop[ci] + lp1;
+  const Stream acc_s{A.seed, gc, *A.sweep, TAG_HMC_BASE + 4u + 2u};
+  const double u = u01(acc_s.draw(0, 0).x);
+  const bool acc = log(u) < (ll1 - ll0) + (A.k0[ci] - 0.5 * kin);
+  if (acc) {
+    for (int j = lane; j < D; j += 64) {
+      A.state[si + j] = A.prop[si + j];
+      A.cache_g[gi + j] = A.gprop[gi + j];
+    }
+    if (lane == 0) A.cache_ll[ci] = A.llprop[ci];
+  }
+  if (lane == 0) {
+    double* st = A.stats + ((size_t)1 * A.C * A.n_ds + ci) * 2;
+    st[0] += acc ? 1.0 : 0.0;
+    st[1] += 1.0;
+  }
+}
+
+// HMC on each dataset's variance v (block 0): one wave per (dataset, chain).
+__global__ __launch_bounds__(64) void k_bhmc_vblock(BArgs A) {
+  const int di = blockIdx.x, c = blockIdx.y, lane = threadIdx.x;
+  const BDs ds = A.ds[di];
+  const int D = ds.d;
+  const uint32_t gc = (uint32_t)(di * A.C + c);
+  const uint32_t sw = *A.sweep;
+  double* st = A.state + (size_t)c * A.SW + ds.soff;
+  auto ll_of = [&](double v) {
+    double s = 0.0;
+    for (int j = lane; j < D; j += 64) s += normal_ll0(st[j], v, A.ref_compat);
+    return wave_sum(s) + v_prior_ll0(A.prior_kind, A.prior_a, A.prior_b, v);
+  };
+  auto grad_of = [&](double v) {
+    double s = 0.0;
+    for (int j = lane; j < D; j += 64) s += normal_dotarg2_0(st[j], v, A.ref_compat);
+    return wave_sum(s) + v_prior_dot0(A.prior_kind, A.prior_a, A.prior_b, v);
+  };
+  double step;
+  int steps;
+  hmc_jitter(Stream{A.seed, gc, sw, TAG_HMC_BASE + 1u}, A.v_sim_len, A.v_eps, A.v_fixed, &step, &steps);
+  const double v0 = st[D + 1];
+  const double p0 = normal_at(Stream{A.seed, gc, sw, TAG_HMC_BASE + 0u}, 0, 0);
+  double v = v0;
+  double p = p0 + 0.5 * step * grad_of(v);
+  for (int l = 0; l < steps; ++l) {
+    v += step * p;
+    if (l != steps - 1) p += step * grad_of(v);
+  }
+  p += 0.5 * step * grad_of(v);
+  const double ll0 = ll_of(v0), ll1 = ll_of(v);
+  const double u = u01(Stream{A.seed, gc, sw, TAG_HMC_BASE + 2u}.draw(0, 0).x);
+  const bool acc = log(u) < (ll1 - ll0) + (0.5 * p0 * p0 - 0.5 * p * p);
+  if (lane == 0) {
+    if (acc) st[D + 1] = v;
+    double* s2 = A.stats + ((size_t)c * A.n_ds + di) * 2;
+    s2[0] += acc ? 1.0 : 0.0;
+    s2[1] += 1.0;
+  }
+}
+
+__global__ void k_bump_sweep(uint32_t* sweep) { *sweep += 1u; }
+
+__global__ void k_bstate_to_beta32(BArgs A) {
+  const int di = blockIdx.x, c = blockIdx.y;
+  const BDs ds = A.ds[di];
+  for (int j = threadIdx.x; j < ds.d; j += blockDim.x)
+    A.beta32[(size_t)c * A.GW + ds.goff + j] = (float)A.state[(size_t)c * A.SW + ds.soff + j];
+}
+
+static uint16_t to_bf16(float f) {  // round to nearest even
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+typedef void (*BPassKernel)(BArgs);
+
+static BPassKernel pick_bpass(int cp) {
+  switch (cp) {
+    case 1: return k_bhlr_pass<1>;
+    case 2: return k_bhlr_pass<2>;
+    case 4: return k_bhlr_pass<4>;
+    case 8: return k_bhlr_pass<8>;
+    default: return k_bhlr_pass<16>;
+  }
+}
+
+struct BhlrSampler final : augur_sampler {
+  int n_ds = 0, C = 1, SW = 0, GW = 0, dmax = 16, R = 4096, nblk_total = 0;
+  std::vector<BDs> hds;
+  std::vector<int> nrows, dims;
+  DevBuf<uint16_t> x;
+  DevBuf<uint8_t> y;
+  DevBuf<BDs> dds;
+  DevBuf<int> blk_ds;
+  DevBuf<float> beta32;
+  DevBuf<double> part, cache_g, cache_ll, gprop, llprop, prop, p, p0, k0, state, stats_buf;
+  DevBuf<unsigned> counter;
+  DevBuf<uint32_t> sweep_dev;
+  BArgs base{};
+  BPassKernel pass = nullptr;
+  bool cache_valid = false, use_graph = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  std::vector<float> tune_us;
+  std::vector<int> tune_rows;
+
+  ~BhlrSampler() override {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+
+  const char* family() const override { return "bhlr"; }
+
+  ParamView param(const std::string& name) override {
+    ParamView v;
+    if (name == "state") return {state.p, sizeof(double), (size_t)SW, (size_t)SW, true};
+    if (name.rfind("state:", 0) == 0) {
+      const int i = atoi(name.c_str() + 6);
+      if (i >= 0 && i < n_ds) v = {state.p + hds[i].soff, sizeof(double), (size_t)hds[i].d + 2, (size_t)SW, true};
+    }
+    return v;
+  }
+
+  void on_set(const std::string&) override {
+    cache_valid = false;
+    drop_graph();
+  }
+
+  void drop_graph() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    graph_exec = nullptr;
+    graph = nullptr;
+  }
+
+  // Blocks of R rows per dataset; partial offsets follow.
+  void layout(int rows) {
+    R = rows;
+    int blk = 0, poff = 0;
+    std::vector<int> table;
+    for (int i = 0; i < n_ds; ++i) {
+      BDs& d = hds[i];
+      d.nblk = (d.n + R - 1) / R;
+      d.blk0 = blk;
+      d.poff = poff;
+      for (int b = 0; b < d.nblk; ++b) table.push_back(i);
+      blk += d.nblk;
+      poff += C * (d.d + 2) * d.nblk;
+    }
+    nblk_total = blk;
+    part.alloc((size_t)poff);
+    blk_ds.alloc(table.size());
+    blk_ds.upload(table.data(), table.size(), stream);
+    dds.upload(hds.data(), hds.size(), stream);
+    base.ds = dds.p;
+    base.blk_ds = blk_ds.p;
+    base.rows_per_blk = R;
+    base.part = part.p;
+  }
+
+  void launch_pass(int mode, int l) {
+    BArgs A = base;
+    A.mode = mode;
+    A.l = l;
+    prof_begin();
+    pass<<<nblk_total, kBThreads, bhlr_smem_bytes(dmax, next_cp()), stream>>>(A);
+    AUGUR_CHECK_LAUNCH();
+    prof_end();
+  }
+
+  int next_cp() const {
+    int p = 1;
+    while (p < C) p <<= 1;
+    return p;
+  }
+
+  void refresh_cache() {
+    k_bstate_to_beta32<<<dim3(n_ds, C), 64, 0, stream>>>(base);
+    AUGUR_CHECK_LAUNCH();
+    launch_pass(BMODE_CACHE, 0);
+    cache_valid = true;
+  }
+
+  // Times the cache pass for candidate rows-per-block and keeps the fastest.
+  void autotune() {
+    const int cands[] = {1024, 2048, 4096, 8192, 16384};
+    hipEvent_t e0, e1;
+    AUGUR_HIP(hipEventCreate(&e0));
+    AUGUR_HIP(hipEventCreate(&e1));
+    int best = 4096;
+    float best_us = 1e30f;
+    k_bstate_to_beta32<<<dim3(n_ds, C), 64, 0, stream>>>(base);
+    AUGUR_CHECK_LAUNCH();
+    for (int rows : cands) {
+      layout(rows);
+      BArgs A = base;
+      A.mode = BMODE_CACHE;
+      pass<<<nblk_total, kBThreads, bhlr_smem_bytes(dmax, next_cp()), stream>>>(A);  // warm
+      AUGUR_CHECK_LAUNCH();
+      AUGUR_HIP(hipEventRecord(e0, stream));
+      for (int r = 0; r < 3; ++r) pass<<<nblk_total, kBThreads, bhlr_smem_bytes(dmax, next_cp()), stream>>>(A);
+      AUGUR_CHECK_LAUNCH();
+      AUGUR_HIP(hipEventRecord(e1, stream));
+      AUGUR_HIP(hipEventSynchronize(e1));
+      float ms = 0.0f;
+      AUGUR_HIP(hipEventElapsedTime(&ms, e0, e1));
+      const float us = 1000.0f * ms / 3.0f;
+      tune_rows.push_back(rows);
+      tune_us.push_back(us);
+      if (us < best_us) {
+        best_us = us;
+        best = rows;
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    layout(best);
+  }
+
+  void enqueue_sweep() {
+    k_bhmc_vblock<<<dim3(n_ds, C), 64, 0, stream>>>(base);
+    AUGUR_CHECK_LAUNCH();
+    k_bhmc_begin<<<dim3(n_ds, C), 64, 0, stream>>>(base);
+    AUGUR_CHECK_LAUNCH();
+    for (int l = 1; l <= base.steps; ++l) launch_pass(BMODE_LEAPFROG, l);
+    k_bhmc_end<<<dim3(n_ds, C), 64, 0, stream>>>(base);
+    AUGUR_CHECK_LAUNCH();
+    k_bump_sweep<<<1, 1, 0, stream>>>(sweep_dev.p);
+    AUGUR_CHECK_LAUNCH();
+  }
+
+  void run_sweeps(int nsw) override {
+    if (nsw <= 0) return;
+    if (!cache_valid) refresh_cache();
+    if (use_graph && !graph_exec && prof_period == 0) {
+      AUGUR_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      enqueue_sweep();
+      AUGUR_HIP(hipStreamEndCapture(stream, &graph));
+      AUGUR_HIP(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+    }
+    for (int i = 0; i < nsw; ++i) {
+      if (graph_exec && prof_period == 0) AUGUR_HIP(hipGraphLaunch(graph_exec, stream));
+      else enqueue_sweep();
+      ++sweep;
+      record_traces();
+    }
+  }
+
+  void log_density(double* out) override {
+    if (!cache_valid) refresh_cache();
+    std::vector<double> ll((size_t)C * n_ds), st((size_t)C * SW);
+    cache_ll.download(ll.data(), ll.size(), stream);
+    state.download(st.data(), st.size(), stream);
+    AUGUR_HIP(hipStreamSynchronize(stream));
+    // per chain: sum over datasets of data ll (device) + priors (host, FP64)
+    for (int c = 0; c < chains; ++c) {
+      double acc = 0.0;
+      for (int i = 0; i < n_ds; ++i) {
+        const double* s = st.data() + (size_t)c * SW + hds[i].soff;
+        const int D = hds[i].d;
+        const double v = s[D + 1];
+        const double cc = base.ref_compat ? 1.0 : 0.5;
+        for (int j = 0; j < D; ++j) acc += v > 0.0 ? -cc * log(v) - 0.5 * s[j] * s[j] / v : -kLargePosNum;
+        if (base.prior_kind == 0)
+          acc += base.prior_a * log(base.prior_b) - lgamma(base.prior_a) - (base.prior_a + 1.0) * log(v) - base.prior_b / v;
+        else
+          acc += (v > 0.0) ? log(base.prior_a) - base.prior_a * v : -kLargePosNum;
+        acc += ll[(size_t)c * n_ds + i];
+      }
+      out[c] = acc;
+    }
+  }
+
+  void stats(double* out, size_t bytes) override {
+    const size_t need = sizeof(double) * 2 * (size_t)C * n_ds * 2;
+    if (bytes < need) fail(AUGUR_ERR_ARG, "stats need %zu bytes", need);
+    stats_buf.download(out, 2 * (size_t)C * n_ds * 2, stream);
+    AUGUR_HIP(hipStreamSynchronize(stream));
+  }
+};
+
+}  // namespace augur
+
+using namespace augur;
+
+extern "C" {
+
+augur_status augur_bhlr_create(const augur_bhlr_desc* D, augur_sampler** out) {
+  return guard([&] {
+    if (!out || !D) fail(AUGUR_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (D->n_datasets < 1 || !D->n_rows || !D->n_features || !D->x || !D->y)
+      fail(AUGUR_ERR_ARG, "need n_datasets >= 1 and per-dataset arrays");
+    if (D->chains < 1 || D->chains > kMaxChains) fail(AUGUR_ERR_ARG, "chains must be in [1, %d]", kMaxChains);
+    if (D->theta_block.fixed_steps < 1 || D->theta_block.fixed_steps > 100000 || !(D->theta_block.step_size > 0.0))
+      fail(AUGUR_ERR_ARG, "the batched theta block needs fixed_steps >= 1 and step_size > 0");
+    if (!(D->v_block.step_size > 0.0) || (D->v_block.fixed_steps <= 0 && !(D->v_block.sim_len > 0.0)))
+      fail(AUGUR_ERR_ARG, "bad v block settings");
+    if (D->prior_kind != 0 && D->prior_kind != 1) fail(AUGUR_ERR_ARG, "prior_kind must be 0 or 1");
+    for (int i = 0; i < D->n_datasets; ++i) {
+      const int d = D->n_features[i];
+      if (D->n_rows[i] < 1 || !(d == 16 || d == 32 || d == 64 || d == 128))
+        fail(AUGUR_ERR_ARG, "dataset %d: need n >= 1 and d in {16, 32, 64, 128}", i);
+      if (!D->x[i] || !D->y[i]) fail(AUGUR_ERR_ARG, "dataset %d: null data", i);
+      for (int r = 0; r < D->n_rows[i]; ++r)
+        if (D->y[i][r] != 0 && D->y[i][r] != 1) fail(AUGUR_ERR_ARG, "dataset %d: y must be 0 or 1", i);
+    }
+    require_device();
+    auto* s = new BhlrSampler();
+    try {
+      AUGUR_HIP(hipGetDevice(&s->device));
+      AUGUR_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+      s->seed = D->seed;
+      s->n_ds = D->n_datasets;
+      s->C = s->chains = D->chains;
+      s->use_graph = D->use_graph != 0;
+      size_t xtot = 0, ytot = 0;
+      int soff = 0, goff = 0;
+      s->hds.resize(s->n_ds);
+      for (int i = 0; i < s->n_ds; ++i) {
+        BDs& b = s->hds[i];
+        b.n = D->n_rows[i];
+        b.d = D->n_features[i];
+        b.soff = soff;
+        b.goff = goff;
+        soff += b.d + 2;
+        goff += b.d + 1;
+        s->dmax = std::max(s->dmax, b.d);
+        xtot += (size_t)b.n * b.d;
+        ytot += (size_t)b.n;
+      }
+      s->SW = soff;
+      s->GW = goff;
+      s->x.alloc(xtot);
+      s->y.alloc(ytot);
+      {
+        std::vector<uint16_t> hx;
+        std::vector<uint8_t> hy;
+        size_t xo = 0, yo = 0;
+        for (int i = 0; i < s->n_ds; ++i) {
+          BDs& b = s->hds[i];
+          const size_t ne = (size_t)b.n * b.d;
+          hx.resize(ne);
+          for (size_t e = 0; e < ne; ++e) hx[e] = to_bf16(D->x[i][e]);
+          hy.resize(b.n);
+          for (int r = 0; r < b.n; ++r) hy[r] = (uint8_t)D->y[i][r];
+          AUGUR_HIP(hipMemcpy(s->x.p + xo, hx.data(), ne * 2, hipMemcpyHostToDevice));
+          AUGUR_HIP(hipMemcpy(s->y.p + yo, hy.data(), b.n, hipMemcpyHostToDevice));
+          b.x = s->x.p + xo;
+          b.y = s->y.p + yo;
+          xo += ne;
+          yo += b.n;
+        }
+      }
+      const size_t C = s->C;
+      s->dds.alloc(s->n_ds);
+      s->beta32.alloc(C * s->GW);
+      s->cache_g.alloc(C * s->GW); s->gprop.alloc(C * s->GW); s->p.alloc(C * s->GW); s->p0.alloc(C * s->GW);
+      s->cache_ll.alloc(C * s->n_ds); s->llprop.alloc(C * s->n_ds); s->k0.alloc(C * s->n_ds);
+      s->state.alloc(C * s->SW); s->prop.alloc(C * s->SW);
+      s->stats_buf.alloc(2 * C * s->n_ds * 2);
+      s->counter.alloc(s->n_ds);
+      s->sweep_dev.alloc(1);
+      s->state.zero(s->stream); s->prop.zero(s->stream); s->stats_buf.zero(s->stream);
+      s->counter.zero(s->stream); s->sweep_dev.zero(s->stream); s->beta32.zero(s->stream);
+      BArgs& A = s->base;
+      A.n_ds = s->n_ds; A.C = s->C; A.SW = s->SW; A.GW = s->GW;
+      A.beta32 = s->beta32.p; A.counter = s->counter.p;
+      A.cache_g = s->cache_g.p; A.cache_ll = s->cache_ll.p; A.gprop = s->gprop.p; A.llprop = s->llprop.p;
+      A.prop = s->prop.p; A.p = s->p.p; A.p0 = s->p0.p; A.k0 = s->k0.p; A.state = s->state.p;
+      A.stats = s->stats_buf.p; A.sweep = s->sweep_dev.p; A.seed = s->seed;
+      A.prior_kind = D->prior_kind; A.ref_compat = D->ref_compat;
+      A.prior_a = D->prior_a; A.prior_b = D->prior_b;
+      A.step = D->theta_block.step_size; A.steps = D->theta_block.fixed_steps;
+      A.v_sim_len = D->v_block.sim_len; A.v_eps = D->v_block.step_size; A.v_fixed = D->v_block.fixed_steps;
+      s->pass = pick_bpass(s->next_cp());
+      // state starts at theta = 0, v = 1
+      std::vector<double> st(C * s->SW, 0.0);
+      for (size_t c = 0; c < C; ++c)
+        for (int i = 0; i < s->n_ds; ++i) st[c * s->SW + s->hds[i].soff + s->hds[i].d + 1] = 1.0;
+      s->state.upload(st.data(), st.size(), s->stream);
+      if (D->rows_per_block > 0) s->layout(((D->rows_per_block + kBTile - 1) / kBTile) * kBTile);
+      else s->autotune();
+      AUGUR_HIP(hipStreamSynchronize(s->stream));
+    } catch (...) {
+      delete s;
+      throw;
+    }
+    *out = s;
+  });
+}
+
+augur_status augur_bhlr_tuning(augur_sampler* s, int* rows_per_block, int* n_candidates,
+                               int* cand_rows, float* cand_us, int cap) {
+  return guard([&] {
+    auto* b = dynamic_cast<BhlrSampler*>(s);
+    if (!b) fail(AUGUR_ERR_ARG, "not a batched HLR sampler");
+    if (rows_per_block) *rows_per_block = b->R;
+    if (n_candidates) *n_candidates = (int)b->tune_rows.size();
+    for (int i = 0; i < (int)b->tune_rows.size() && i < cap; ++i) {
+      if (cand_rows) cand_rows[i] = b->tune_rows[i];
+      if (cand_us) cand_us[i] = b->tune_us[i];
+    }
+  });
+}
+
+}  // extern "C"

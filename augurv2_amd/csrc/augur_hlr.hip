@@ -26,46 +26,16 @@
 // are exactly what the pass computed when that point was proposed), so an
 // iteration of L leapfrog steps costs L data passes instead of the
 // reference's L + 3 (L + 1 gradients and 2 log-densities).
-#include "augur_common.h"
+#include "augur_hmc.h"
 
 namespace augur {
 
 constexpr int kTileRows = 64;
 constexpr int kPassThreads = 256;
 constexpr int kMaxGradSlots = 16;  // (d) * cp <= 16 * 256
-constexpr int kMaxChains = 16;
 constexpr int kGroup = 16;  // blocks per first-level reduction group
 
 enum { MODE_CACHE = 0, MODE_LEAPFROG = 1 };
-
-struct HmcStep {
-  double step[kMaxChains];
-  int steps[kMaxChains];
-};
-
-// eps' = eps U(0.8, 1.2); L = max(1, round(U(0.9, 1.1) simLen / eps'))
-// (cbits/augur_mcmclib.h:401-405), uniforms from the block's jitter stream.
-__host__ __device__ inline void hmc_jitter(const Stream& s, double sim_len, double eps,
-                                          int fixed_steps, double* step, int* steps) {
-#pragma clang fp contract(off)
-  if (fixed_steps > 0) {
-    *step = eps;
-    *steps = fixed_steps;
-    return;
-  }
-  const u32x4 w = s.draw(0, 0);
-  const double st = (0.8 + 0.4 * u01(w.x)) * eps;
-  const double sc = 0.9 + 0.2 * u01(w.y);
-  const double r = round(sc * sim_len / st);
-  *step = st;
-  *steps = r < 1.0 ? 1 : (int)r;
-}
-
-// cbits/augur_dist.c:372-381 (ref_compat: -log var; else -0.5 log var)
-__device__ inline double normal_ll0(double x, double var, int ref_compat) {
-  if (var > 0.0) return -(ref_compat ? 1.0 : 0.5) * log(var) - 0.5 * x * x / var;
-  return -kLargePosNum;
-}
 
 struct PassArgs {
   const float* __restrict__ x;
@@ -89,15 +59,82 @@ struct PassArgs {
   HmcStep hs;
 };
 
-__device__ inline float log_sigmoid_pm(float t, int y, float* a_out) {
-  // e = exp(-|t|); sigmoid(t) = 1/(1+e) (t >= 0) or e/(1+e); a = y - sigmoid(t)
-  // log sigmoid(+-t) = min(+-t, 0) - log1p(e)
-  const float e = expf(-fabsf(t));
-  const float inv = 1.0f / (1.0f + e);
-  const float sig = t >= 0.0f ? inv : e * inv;
-  *a_out = (float)y - sig;
-  const float st = y ? t : -t;
-  return fminf(st, 0.0f) - log1pf(e);
+
+// Two-level deterministic reduction over blocks, called by every block after
+// it has written its partials: the last block of each group of kGroup
+// consecutive blocks sums the group's partials in block order; the last group
+// to finish sums the group sums in group order and applies the epilogue.
+// Each thread owns whole values, so its loads are independent.
+__device__ void pass_finish(const PassArgs& A, unsigned char* smem, int* flag) {
+  const int tid = threadIdx.x, G = gridDim.x;
+  const int d = A.d, cp = A.cp, ds = d + 1, vs = d + 2;
+  const int nval = cp * vs;
+  const int grp = blockIdx.x / kGroup;
+  const int ngroups = (G + kGroup - 1) / kGroup;
+  const int g0 = grp * kGroup, gn = min(kGroup, G - g0);
+  __threadfence();
+  __syncthreads();
+  if (tid == 0) *flag = (atomicAdd(A.gcounter + grp, 1u) == (unsigned)(gn - 1));
+  __syncthreads();
+  if (!*flag) return;
+  __threadfence();
+  for (int v = tid; v < nval; v += kPassThreads) {
+    const double* pv = A.part + (size_t)v * G + g0;
+    double s = 0.0;
+#pragma unroll 16
+    for (int i = 0; i < gn; ++i) s += pv[i];
+    A.gpart[(size_t)v * ngroups + grp] = s;
+  }
+  if (tid == 0) A.gcounter[grp] = 0u;
+  __threadfence();
+  __syncthreads();
+  if (tid == 0) *flag = (atomicAdd(A.counter, 1u) == (unsigned)(ngroups - 1));
+  __syncthreads();
+  if (!*flag) return;
+  __threadfence();
+  double* tot = (double*)smem;  // cp * (d + 2) doubles, aliases the tile
+  for (int v = tid; v < nval; v += kPassThreads) {
+    const double* pv = A.gpart + (size_t)v * ngroups;
+    double s = 0.0;
+#pragma unroll 16
+    for (int i = 0; i < ngroups; ++i) s += pv[i];
+    tot[v] = s;
+  }
+  __syncthreads();
+  const int P = d + 2;
+  if (A.mode == MODE_CACHE) {
+    for (int q = tid; q < cp * ds; q += kPassThreads) {
+      const int c = q / ds, j = q % ds;
+      A.cache_g[q] = tot[c * vs + j];
+      if (j == 0) A.cache_ll[c] = tot[c * vs + d + 1];
+    }
+  } else {
+    for (int q = tid; q < A.c_real * ds; q += kPassThreads) {
+      const int c = q / ds, j = q % ds;
+      const int L = A.hs.steps[c];
+      if (A.l > L) continue;
+      const double g_data = tot[c * vs + j];
+      if (A.l == L) {
+        A.gprop[q] = g_data;
+        if (j == 0) A.llprop[c] = tot[c * vs + d + 1];
+      }
+      const bool in_blk = (j < d) ? A.blk_theta : (A.blk_b && A.has_b);
+      if (!in_blk) continue;
+      const double step = A.hs.step[c];
+      const double v = A.state[c * P + d + 1];
+      double xv = A.prop[c * P + j];
+      const double g = g_data + (-xv / v);
+      const double coef = (A.l == L) ? 0.5 * step : step;
+      const double pj = A.p[q] + coef * g;
+      A.p[q] = pj;
+      if (A.l < L) {
+        xv += step * pj;
+        A.prop[c * P + j] = xv;
+        A.beta32[q] = (float)xv;
+      }
+    }
+  }
+  if (tid == 0) *A.counter = 0u;
 }
 
 __global__ __launch_bounds__(kPassThreads) void k_hlr_pass(PassArgs A) {
@@ -217,77 +254,207 @@ __global__ __launch_bounds__(kPassThreads) void k_hlr_pass(PassArgs A) {
     A.part[(size_t)(tid * vs + d + 1) * G + blockIdx.x] = sl;
   }
   (void)my_c;
-  // Two-level deterministic reduction over blocks: the last block of each
-  // group of kGroup consecutive blocks sums the group's partials in block
-  // order; the last group to finish sums the group sums in group order.
-  // Each thread owns whole values, so its loads are independent.
-  const int nval = cp * vs;
-  const int grp = blockIdx.x / kGroup;
-  const int ngroups = (G + kGroup - 1) / kGroup;
-  const int g0 = grp * kGroup, gn = min(kGroup, G - g0);
-  __threadfence();
+  pass_finish(A, smem, flag);
+}
+
+// Specialised data pass for D % 4 == 0 and a power-of-two chain count CP.
+//   - rows are staged with 16-byte-aligned stride D + 4, so tile stores and
+//     row reads are 128-bit LDS operations;
+//   - the next tile is prefetched into registers while the current one is
+//     processed;
+//   - phase B keeps each thread's theta_c in registers (D <= 64);
+//   - phase C gives each thread 4 adjacent columns of one chain and a slice of
+//     the tile's rows (a 128-bit x read and one broadcast a read feed 4 FMAs);
+//     the row slices are combined once per block, in a fixed order.
+template <int D, int CP>
+__global__ __launch_bounds__(kPassThreads) void k_hlr_pass_t(PassArgs A) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int DS = D + 4;
+  constexpr int TR = kTileRows;
+  constexpr int NV = TR * D / 4;                              // float4 per tile
+  constexpr int NVT = (NV + kPassThreads - 1) / kPassThreads;  // per thread
+  constexpr int NB = (TR * CP + kPassThreads - 1) / kPassThreads;
+  constexpr int NP = CP * (D / 4);                             // (chain, column quad) pairs
+  constexpr int RS = NP >= kPassThreads ? 1 : kPassThreads / NP;  // row slices
+  constexpr int NS = NP > kPassThreads ? NP / kPassThreads : 1;  // pairs per thread
+  constexpr bool BREG = D <= 64;
+  static_assert(D % 4 == 0 && (CP & (CP - 1)) == 0 && CP <= kMaxChains, "shape");
+  static_assert(TR % RS == 0, "row slices");
+
+  float* xs = (float*)smem;                 // TR x DS
+  float* bs = xs + TR * DS;                 // CP x (D + 4)
+  float* as = bs + CP * DS;                 // TR x CP
+  int* ys = (int*)(as + TR * CP);           // TR
+  double* red = (double*)(((uintptr_t)(ys + TR) + 15) & ~(uintptr_t)15);  // 2 x 256
+  double* red2 = red + 2 * kPassThreads;    // RS x NP x 4 (RS * NP <= 256)
+  int* flag = (int*)(red2 + 4 * kPassThreads);
+
+  const int tid = threadIdx.x;
+  const int G = gridDim.x;
+  const int n = A.n;
+  const int tiles = (n + TR - 1) / TR;
+  for (int i = tid; i < CP * (D + 1); i += kPassThreads) bs[(i / (D + 1)) * DS + i % (D + 1)] = A.beta32[i];
   __syncthreads();
-  if (tid == 0) *flag = (atomicAdd(A.gcounter + grp, 1u) == (unsigned)(gn - 1));
-  __syncthreads();
-  if (!*flag) return;
-  __threadfence();
-  for (int v = tid; v < nval; v += kPassThreads) {
-    const double* pv = A.part + (size_t)v * G + g0;
-    double s = 0.0;
-#pragma unroll 16
-    for (int i = 0; i < gn; ++i) s += pv[i];
-    A.gpart[(size_t)v * ngroups + grp] = s;
+
+  const int my_c = tid % CP;  // phase-B chain of this thread
+  float breg[BREG ? D : 1];
+  if constexpr (BREG) {
+#pragma unroll
+    for (int j = 0; j < D; ++j) breg[j] = bs[my_c * DS + j];
   }
-  if (tid == 0) A.gcounter[grp] = 0u;
-  __threadfence();
-  __syncthreads();
-  if (tid == 0) *flag = (atomicAdd(A.counter, 1u) == (unsigned)(ngroups - 1));
-  __syncthreads();
-  if (!*flag) return;
-  __threadfence();
-  double* tot = (double*)smem;  // cp * (d + 2) doubles, aliases the tile
-  for (int v = tid; v < nval; v += kPassThreads) {
-    const double* pv = A.gpart + (size_t)v * ngroups;
-    double s = 0.0;
-#pragma unroll 16
-    for (int i = 0; i < ngroups; ++i) s += pv[i];
-    tot[v] = s;
+  const float b_c = A.has_b ? bs[my_c * DS + D] : 0.0f;
+
+  double gacc[NS][4];
+#pragma unroll
+  for (int s = 0; s < NS; ++s)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) gacc[s][k] = 0.0;
+  double llacc = 0.0, gbacc = 0.0;
+
+  float4 pre[NVT];
+  int ypre = 0;
+  auto prefetch = [&](int tile) {
+    const int r0 = tile * TR;
+    const int rows = min(TR, n - r0);
+    const float4* s4 = (const float4*)(A.x + (size_t)r0 * D);
+#pragma unroll
+    for (int k = 0; k < NVT; ++k) {
+      const int i = tid + k * kPassThreads;
+      pre[k] = (i < NV && i < rows * (D / 4)) ? s4[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    ypre = (tid < TR && tid < rows) ? A.y[r0 + tid] : 0;
+  };
+
+  int tile = blockIdx.x;
+  if (tile < tiles) prefetch(tile);
+  for (; tile < tiles; tile += G) {
+    const int rows = min(TR, n - tile * TR);
+#pragma unroll
+    for (int k = 0; k < NVT; ++k) {
+      const int i = tid + k * kPassThreads;
+      if (i < NV) {
+        const int e = 4 * i;
+        *(float4*)(xs + (e / D) * DS + e % D) = pre[k];
+      }
+    }
+    if (tid < TR) ys[tid] = ypre;
+    __syncthreads();
+    if (tile + G < tiles) prefetch(tile + G);
+    // phase B: t = x . theta_c + b_c
+#pragma unroll
+    for (int m = 0; m < NB; ++m) {
+      const int q = tid + kPassThreads * m;
+      if (q < TR * CP) {
+        const int r = q / CP;
+        const float* xr = xs + r * DS;
+        const float* br = bs + my_c * DS;
+        float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+#pragma unroll
+        for (int j = 0; j < D; j += 4) {
+          const float4 xv = *(const float4*)(xr + j);
+          float4 bv;
+          if constexpr (BREG) bv = make_float4(breg[j], breg[j + 1], breg[j + 2], breg[j + 3]);
+          else bv = *(const float4*)(br + j);
+          s0 = fmaf(xv.x, bv.x, s0);
+          s1 = fmaf(xv.y, bv.y, s1);
+          s2 = fmaf(xv.z, bv.z, s2);
+          s3 = fmaf(xv.w, bv.w, s3);
+        }
+        const float t = ((s0 + s1) + (s2 + s3)) + b_c;
+        float a;
+        const float lr = log_sigmoid_pm(t, ys[r], &a);
+        if (r < rows) {
+          llacc += (double)lr;
+          gbacc += (double)a;
+        } else {
+          a = 0.0f;
+        }
+        as[r * CP + my_c] = a;
+      }
+    }
+    __syncthreads();
+    // phase C: grad[c][4 jq .. 4 jq + 3] += sum over the thread's rows
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const int pair = (NS > 1) ? tid + s * kPassThreads : tid % NP;
+      const int rs = (NS > 1) ? 0 : tid / NP;
+      const int c = pair / (D / 4), jq = pair % (D / 4);
+      float4 f = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll 4
+      for (int r = rs * (TR / RS); r < (rs + 1) * (TR / RS); ++r) {
+        const float av = as[r * CP + c];
+        const float4 xv = *(const float4*)(xs + r * DS + 4 * jq);
+        f.x = fmaf(av, xv.x, f.x);
+        f.y = fmaf(av, xv.y, f.y);
+        f.z = fmaf(av, xv.z, f.z);
+        f.w = fmaf(av, xv.w, f.w);
+      }
+      gacc[s][0] += (double)f.x;
+      gacc[s][1] += (double)f.y;
+      gacc[s][2] += (double)f.z;
+      gacc[s][3] += (double)f.w;
+    }
+    __syncthreads();
   }
-  __syncthreads();
-  const int P = d + 2;
-  if (A.mode == MODE_CACHE) {
-    for (int q = tid; q < cp * ds; q += kPassThreads) {
-      const int c = q / ds, j = q % ds;
-      A.cache_g[q] = tot[c * vs + j];
-      if (j == 0) A.cache_ll[c] = tot[c * vs + d + 1];
+  // block partials -> part[value][block], value = c * (D + 2) + j
+  constexpr int VS = D + 2;
+  if constexpr (RS > 1) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) red2[((tid / NP) * NP + tid % NP) * 4 + k] = gacc[0][k];
+    __syncthreads();
+    if (tid < NP) {
+      const int c = tid / (D / 4), jq = tid % (D / 4);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        double s = 0.0;
+        for (int rs = 0; rs < RS; ++rs) s += red2[(rs * NP + tid) * 4 + k];
+        A.part[(size_t)(c * VS + 4 * jq + k) * G + blockIdx.x] = s;
+      }
     }
   } else {
-    for (int q = tid; q < A.c_real * ds; q += kPassThreads) {
-      const int c = q / ds, j = q % ds;
-      const int L = A.hs.steps[c];
-      if (A.l > L) continue;
-      const double g_data = tot[c * vs + j];
-      if (A.l == L) {
-        A.gprop[q] = g_data;
-        if (j == 0) A.llprop[c] = tot[c * vs + d + 1];
-      }
-      const bool in_blk = (j < d) ? A.blk_theta : (A.blk_b && A.has_b);
-      if (!in_blk) continue;
-      const double step = A.hs.step[c];
-      const double v = A.state[c * P + d + 1];
-      double xv = A.prop[c * P + j];
-      const double g = g_data + (-xv / v);
-      const double coef = (A.l == L) ? 0.5 * step : step;
-      const double pj = A.p[q] + coef * g;
-      A.p[q] = pj;
-      if (A.l < L) {
-        xv += step * pj;
-        A.prop[c * P + j] = xv;
-        A.beta32[q] = (float)xv;
-      }
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const int pair = tid + s * kPassThreads;
+      const int c = pair / (D / 4), jq = pair % (D / 4);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) A.part[(size_t)(c * VS + 4 * jq + k) * G + blockIdx.x] = gacc[s][k];
     }
   }
-  if (tid == 0) *A.counter = 0u;
+  red[tid] = gbacc;
+  red[kPassThreads + tid] = llacc;
+  __syncthreads();
+  if (tid < CP) {
+    double sb = 0.0, sl = 0.0;
+    for (int i = tid; i < kPassThreads; i += CP) {
+      sb += red[i];
+      sl += red[kPassThreads + i];
+    }
+    A.part[(size_t)(tid * VS + D) * G + blockIdx.x] = sb;
+    A.part[(size_t)(tid * VS + D + 1) * G + blockIdx.x] = sl;
+  }
+  pass_finish(A, smem, flag);
+}
+
+size_t pass_t_smem_bytes(int d, int cp) {
+  size_t b = sizeof(float) * ((size_t)kTileRows * (d + 4) + (size_t)cp * (d + 4) + (size_t)kTileRows * cp) +
+             sizeof(int) * kTileRows;
+  b = (b + 15) & ~(size_t)15;
+  b += sizeof(double) * 6 * kPassThreads + 16;
+  const size_t tot = sizeof(double) * (size_t)cp * (d + 2);
+  return b > tot ? b : tot;
+}
+
+typedef void (*PassKernel)(PassArgs);
+
+// Specialised pass for (d, cp) when one is compiled, else null.
+PassKernel pick_pass(int d, int cp) {
+#define AUGUR_PASS(D_, C_) if (d == D_ && cp == C_) return k_hlr_pass_t<D_, C_>;
+  AUGUR_PASS(16, 1) AUGUR_PASS(16, 2) AUGUR_PASS(16, 4) AUGUR_PASS(16, 8) AUGUR_PASS(16, 16)
+  AUGUR_PASS(32, 1) AUGUR_PASS(32, 2) AUGUR_PASS(32, 4) AUGUR_PASS(32, 8) AUGUR_PASS(32, 16)
+  AUGUR_PASS(64, 1) AUGUR_PASS(64, 2) AUGUR_PASS(64, 4) AUGUR_PASS(64, 8) AUGUR_PASS(64, 16)
+  AUGUR_PASS(128, 1) AUGUR_PASS(128, 2) AUGUR_PASS(128, 4) AUGUR_PASS(128, 8) AUGUR_PASS(128, 16)
+#undef AUGUR_PASS
+  return nullptr;
 }
 
 size_t pass_smem_bytes(int d, int cp) {
@@ -319,10 +486,6 @@ struct HmcArgs {
   HmcStep hs;
 };
 
-__device__ inline double wave_sum(double v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  return v;
-}
 
 // One wave per chain: momentum draw, half step, first position step.
 __global__ __launch_bounds__(64) void k_hmc_begin(HmcArgs A) {
@@ -401,14 +564,10 @@ struct VBlockArgs {
 };
 
 __device__ inline double v_prior_ll(const VBlockArgs& A, double v) {
-  if (A.prior_kind == 0)  // cbits/augur_dist.c:345-347
-    return A.prior_a * log(A.prior_b) - lgamma(A.prior_a) - (A.prior_a + 1.0) * log(v) - A.prior_b / v;
-  if (A.prior_a > 0.0 && v > 0.0) return log(A.prior_a) - A.prior_a * v;  // :231-238
-  return -kLargePosNum;
+  return v_prior_ll0(A.prior_kind, A.prior_a, A.prior_b, v);
 }
 __device__ inline double v_prior_dot(const VBlockArgs& A, double v) {
-  if (A.prior_kind == 0) return -(A.prior_a + 1.0) / v + A.prior_b / (v * v);  // :349-351
-  return (A.prior_a > 0.0 && v > 0.0) ? -A.prior_a : 0.0;                    // :240-247
+  return v_prior_dot0(A.prior_kind, A.prior_a, A.prior_b, v);
 }
 
 // HMC on the scalar variance v (the full conditional involves only theta
@@ -425,11 +584,7 @@ __global__ __launch_bounds__(64) void k_hmc_vblock(VBlockArgs A) {
   };
   auto grad_of = [&](double v) {
     double s = 0.0;
-    for (int j = lane; j < nterm; j += 64) {
-      const double t = st[j];
-      // cbits/augur_dist.c:391-394 (ref_compat keeps -0.5 * var)
-      s += (A.ref_compat ? -0.5 * v : -0.5 / v) + 0.5 * (t * t) / (v * v);
-    }
+    for (int j = lane; j < nterm; j += 64) s += normal_dotarg2_0(st[j], v, A.ref_compat);
     return wave_sum(s) + v_prior_dot(A, v);
   };
   double step;
@@ -491,6 +646,7 @@ struct HlrSampler final : augur_sampler {
   int n = 0, d = 0, cp = 1, has_b = 0, ref_compat = 1, prior_kind = 0;
   double prior_a = 1.0, prior_b = 1.0;
   int G = 1;
+  PassKernel fast_pass = nullptr;  // specialised pass for (d, cp), if compiled
   std::vector<augur_hmc_block> sched;
   DevBuf<float> x, beta32;
   DevBuf<int32_t> y;
@@ -527,7 +683,10 @@ struct HlrSampler final : augur_sampler {
 
   void launch_pass(const PassArgs& A) {
     prof_begin();
-    k_hlr_pass<<<G, kPassThreads, pass_smem_bytes(d, cp), stream>>>(A);
+    if (fast_pass)
+      fast_pass<<<G, kPassThreads, pass_t_smem_bytes(d, cp), stream>>>(A);
+    else
+      k_hlr_pass<<<G, kPassThreads, pass_smem_bytes(d, cp), stream>>>(A);
     AUGUR_CHECK_LAUNCH();
     prof_end();
   }
@@ -620,6 +779,8 @@ struct HlrSampler final : augur_sampler {
     int want = 256;
     if (const char* e = getenv("AUGUR_HLR_BLOCKS")) want = std::max(1, atoi(e));
     G = std::max(1, std::min(tiles, want));
+    const char* gen = getenv("AUGUR_HLR_GENERIC");
+    fast_pass = (gen && atoi(gen)) ? nullptr : pick_pass(d, cp);
     const size_t P = (size_t)d + 2, ds = (size_t)d + 1;
     x.alloc((size_t)n * d);
     y.alloc(std::max(n, 1));

@@ -1,0 +1,74 @@
+// augur_hmc.h -- device helpers shared by the HMC samplers (plain and batched
+// hierarchical logistic regression).
+#pragma once
+#include "augur_common.h"
+
+namespace augur {
+
+constexpr int kMaxChains = 16;
+
+struct HmcStep {
+  double step[kMaxChains];
+  int steps[kMaxChains];
+};
+
+// eps' = eps U(0.8, 1.2); L = max(1, round(U(0.9, 1.1) simLen / eps'))
+// (cbits/augur_mcmclib.h:401-405), uniforms from the block's jitter stream.
+__host__ __device__ inline void hmc_jitter(const Stream& s, double sim_len, double eps,
+                                          int fixed_steps, double* step, int* steps) {
+#pragma clang fp contract(off)
+  if (fixed_steps > 0) {
+    *step = eps;
+    *steps = fixed_steps;
+    return;
+  }
+  const u32x4 w = s.draw(0, 0);
+  const double st = (0.8 + 0.4 * u01(w.x)) * eps;
+  const double sc = 0.9 + 0.2 * u01(w.y);
+  const double r = round(sc * sim_len / st);
+  *step = st;
+  *steps = r < 1.0 ? 1 : (int)r;
+}
+
+// cbits/augur_dist.c:372-381 (ref_compat: -log var; else -0.5 log var)
+__device__ inline double normal_ll0(double x, double var, int ref_compat) {
+  if (var > 0.0) return -(ref_compat ? 1.0 : 0.5) * log(var) - 0.5 * x * x / var;
+  return -kLargePosNum;
+}
+
+// cbits/augur_dist.c:391-394 (ref_compat keeps -0.5 * var)
+__device__ inline double normal_dotarg2_0(double x, double var, int ref_compat) {
+  return (ref_compat ? -0.5 * var : -0.5 / var) + 0.5 * (x * x) / (var * var);
+}
+
+// Prior of the variance: InvGamma(a, b) (cbits/augur_dist.c:345-351) or
+// Exponential(a) (:231-247).
+__device__ inline double v_prior_ll0(int kind, double a, double b, double v) {
+  if (kind == 0) return a * log(b) - lgamma(a) - (a + 1.0) * log(v) - b / v;
+  if (a > 0.0 && v > 0.0) return log(a) - a * v;
+  return -kLargePosNum;
+}
+__device__ inline double v_prior_dot0(int kind, double a, double b, double v) {
+  if (kind == 0) return -(a + 1.0) / v + b / (v * v);
+  return (a > 0.0 && v > 0.0) ? -a : 0.0;
+}
+
+// Data term of one row: e = exp(-|t|); sigmoid(t) = 1/(1+e) (t >= 0) or
+// e/(1+e); a = y - sigmoid(t) (= bernoulli_dotarg1 * expit_grad_0,
+// cbits/augur_dist.c:68-71, cbits/augur_math.c:56-59); returns
+// log sigmoid(+-t) = min(+-t, 0) - log1p(e).
+__device__ inline float log_sigmoid_pm(float t, int y, float* a_out) {
+  const float e = expf(-fabsf(t));
+  const float inv = 1.0f / (1.0f + e);
+  const float sig = t >= 0.0f ? inv : e * inv;
+  *a_out = (float)y - sig;
+  const float st = y ? t : -t;
+  return fminf(st, 0.0f) - log1pf(e);
+}
+
+__device__ inline double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+}  // namespace augur

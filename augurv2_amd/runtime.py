@@ -148,6 +148,58 @@ class HlrSampler(Sampler):
             return out[..., 0] / out[..., 1]
 
 
+class BhlrSampler(Sampler):
+    """Many independent HLR datasets sampled together (BF16 X on the device)."""
+
+    family = "bhlr"
+
+    def __init__(self, xs, ys, chains=4, seed=0, v_block=(1.0, 0.05, 0), theta_block=(0.1, 0.005, 20),
+                 prior_kind=0, prior_a=1.0, prior_b=1.0, ref_compat=False, rows_per_block=0,
+                 use_graph=True):
+        L.require_gpu()
+        self.xs = [L.f32(x) for x in xs]
+        self.ys = [L.i32(y).ravel() for y in ys]
+        n = np.array([x.shape[0] for x in self.xs], np.int32)
+        d = np.array([x.shape[1] for x in self.xs], np.int32)
+        self.n_rows, self.dims = n, d
+        xp = (C.c_void_p * len(xs))(*[L.ptr(x).value for x in self.xs])
+        yp = (C.c_void_p * len(ys))(*[L.ptr(y).value for y in self.ys])
+        self._keep = (xp, yp)
+        desc = L.BhlrDesc()
+        desc.n_datasets = len(xs)
+        desc.n_rows, desc.n_features = L.ptr(n).value, L.ptr(d).value
+        desc.x, desc.y = C.cast(xp, C.c_void_p).value, C.cast(yp, C.c_void_p).value
+        desc.chains, desc.seed = chains, seed & 0xFFFFFFFF
+        desc.prior_kind, desc.prior_a, desc.prior_b = prior_kind, prior_a, prior_b
+        desc.ref_compat = int(ref_compat)
+        desc.v_block = L.HmcBlock(L.HMC_BLOCK_V, *v_block[:2], int(v_block[2]))
+        desc.theta_block = L.HmcBlock(L.HMC_BLOCK_THETA, *theta_block[:2], int(theta_block[2]))
+        desc.rows_per_block, desc.use_graph = int(rows_per_block), int(use_graph)
+        h = C.c_void_p()
+        L.call("augur_bhlr_create", C.byref(desc), C.byref(h))
+        self.offsets = np.concatenate([[0], np.cumsum(d + 2)])[:-1]
+        shapes = {"state": ((int(np.sum(d + 2)),), np.float64)}
+        for i in range(len(xs)):
+            shapes[f"state:{i}"] = ((int(d[i]) + 2,), np.float64)
+        super().__init__(h, chains, shapes)
+
+    def tuning(self):
+        rows = C.c_int(0)
+        n = C.c_int(0)
+        cr = np.zeros(16, np.int32)
+        cu = np.zeros(16, np.float32)
+        L.call("augur_bhlr_tuning", self._h, C.byref(rows), C.byref(n), L.ptr(cr), L.ptr(cu), 16)
+        return rows.value, dict(zip(cr[:n.value].tolist(), cu[:n.value].tolist()))
+
+    def acceptance(self):
+        """[block (v, theta)][chain][dataset] acceptance rates."""
+        k = len(self.xs)
+        out = np.zeros((2, self.chains, k, 2))
+        L.call("augur_stats", self._h, L.ptr(out), out.nbytes)
+        with np.errstate(invalid="ignore"):
+            return out[..., 0] / out[..., 1]
+
+
 class GmmSampler(Sampler):
     family = "gmm"
 

@@ -99,6 +99,40 @@ typedef struct {
 } augur_hlr_desc;
 augur_status augur_hlr_create(const augur_hlr_desc* desc, augur_sampler** out);
 
+/* Batched form of the hlr.rv family without intercept: n_datasets
+ * independent regressions of uneven N_i and D_i in {16, 32, 64, 128}, `chains`
+ * chains each, schedule HMC [v] (*) HMC [theta] with a fixed leapfrog count
+ * for theta.  X is stored BF16 on the device (FP32 arithmetic, FP64
+ * accumulation and Metropolis test).  No counterpart in the reference, which
+ * runs one model per loaded library (cbits/augur_mcmclib.h:44); each dataset
+ * follows the single-model semantics of augur_hlr_create.  Parameters:
+ * "state" ([chain][sum_i (D_i + 2)] f64, dataset i at offset sum_{i'<i}
+ * (D_i' + 2), each slice [theta, b = 0, v]) and "state:<i>".  Philox chain
+ * key of (dataset i, chain c) is i * chains + c.  rows_per_block = 0 lets the
+ * autotuner time candidate block sizes at creation; use_graph captures one
+ * sweep in a hipGraph and replays it. */
+typedef struct {
+  int n_datasets;
+  const int32_t* n_rows;            /* per dataset */
+  const int32_t* n_features;        /* per dataset */
+  const float* const* x;            /* per dataset, N_i x D_i FP32 (rounded to BF16) */
+  const int32_t* const* y;          /* per dataset, N_i labels in {0, 1} */
+  int chains;                       /* 1..16 */
+  uint32_t seed;
+  int prior_kind;                   /* as augur_hlr_desc */
+  double prior_a, prior_b;
+  int ref_compat;
+  augur_hmc_block v_block;          /* HMC [v] */
+  augur_hmc_block theta_block;      /* HMC [theta]: fixed_steps >= 1 required */
+  int rows_per_block;               /* 0: autotune */
+  int use_graph;
+} augur_bhlr_desc;
+augur_status augur_bhlr_create(const augur_bhlr_desc* desc, augur_sampler** out);
+/* The rows-per-block the batched sampler runs with, and the autotuner's
+ * timings of its candidates (microseconds per data pass, up to `cap`). */
+augur_status augur_bhlr_tuning(augur_sampler* s, int* rows_per_block, int* n_candidates,
+                               int* cand_rows, float* cand_us, int cap);
+
 /* hmvgmm.rv: pis ~ Dirichlet(alpha); mu[k] ~ MvNormal(mu0, cov0);
  * cov[k] ~ InvWishart(df, scale0); z[n] ~ Categorical(pis);
  * y[n] ~ MvNormal(mu[z[n]], cov[z[n]]).  sample_pi / sample_cov = 0 keep
