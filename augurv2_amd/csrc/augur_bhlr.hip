@@ -32,6 +32,7 @@ struct BDs {
   int poff;        // partials: [value][nblk] from here, value = c * (d + 2) + j
   int soff;        // offset in the per-chain state vector ([theta, b, v] per dataset)
   int goff;        // offset in the per-chain (d + 1)-wide vectors
+  int voff;        // first global value index of this dataset
   const uint16_t* x;
   const uint8_t* y;
 };
@@ -39,10 +40,10 @@ struct BDs {
 struct BArgs {
   const BDs* ds;
   const int* blk_ds;
-  int n_ds, C, SW, GW, rows_per_blk;
+  const int* val_ds;  // global value -> dataset
+  int n_ds, C, SW, GW, rows_per_blk, nval;
   float* beta32;     // [C][GW]
   double* part;
-  unsigned* counter; // per dataset
   int mode, l;
   double* cache_g;   // [C][GW]
   double* cache_ll;  // [C][n_ds]
@@ -229,51 +230,53 @@ __device__ void bhlr_body(const BArgs& A, const BDs& ds, int di, unsigned char* 
     part[(size_t)(tid * VS + D) * ds.nblk + local] = sb;
     part[(size_t)(tid * VS + D + 1) * ds.nblk + local] = sl;
   }
-  // the last block of the dataset reduces its partials in block order
-  __threadfence();
-  __syncthreads();
-  if (tid == 0) *flag = (atomicAdd(A.counter + di, 1u) == (unsigned)(ds.nblk - 1));
-  __syncthreads();
-  if (!*flag) return;
-  __threadfence();
-  double* tot = (double*)smem;  // CP * VS doubles, aliases the tile
-  for (int v = tid; v < A.C * VS; v += kBThreads) {
-    const double* pv = part + (size_t)v * ds.nblk;
-    double s = 0.0;
-#pragma unroll 8
-    for (int b = 0; b < ds.nblk; ++b) s += pv[b];
-    tot[v] = s;
+  (void)flag;
+  (void)di;
+}
+
+// One wave per value (dataset i, chain c, coordinate j; j = d_i + 1: the
+// log-density): sums the dataset's block partials in block order (fixed
+// butterfly), then applies the leapfrog update of that coordinate.
+__global__ __launch_bounds__(256) void k_bhlr_finish(BArgs A) {
+  const int lane = threadIdx.x & 63;
+  const int v = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (v >= A.nval) return;
+  const int di = A.val_ds[v];
+  const BDs ds = A.ds[di];
+  const int D = ds.d, VS = D + 2;
+  const int lv = v - ds.voff;
+  const double* pv = A.part + ds.poff + (size_t)lv * ds.nblk;
+  double s = 0.0;
+  for (int b = lane; b < ds.nblk; b += 64) s += pv[b];
+  s = wave_sum(s);
+  if (lane != 0) return;
+  const int c = lv / VS, j = lv % VS;
+  const size_t ci = (size_t)c * A.n_ds + di;
+  if (j == D) return;  // no intercept in the batched model
+  if (j == D + 1) {
+    if (A.mode == BMODE_CACHE) A.cache_ll[ci] = s;
+    else if (A.l == A.steps) A.llprop[ci] = s;
+    return;
   }
-  __syncthreads();
-  const int C = A.C;
-  for (int q = tid; q < C * D; q += kBThreads) {
-    const int c = q / D, j = q % D;
-    const size_t gi = (size_t)c * A.GW + ds.goff + j;
-    const double g_data = tot[c * VS + j];
-    if (A.mode == BMODE_CACHE) {
-      A.cache_g[gi] = g_data;
-      if (j == 0) A.cache_ll[(size_t)c * A.n_ds + di] = tot[c * VS + D + 1];
-      continue;
-    }
-    const int L = A.steps;
-    if (A.l == L) {
-      A.gprop[gi] = g_data;
-      if (j == 0) A.llprop[(size_t)c * A.n_ds + di] = tot[c * VS + D + 1];
-    }
-    const size_t si = (size_t)c * A.SW + ds.soff;
-    const double v = A.state[si + D + 1];
-    double xv = A.prop[si + j];
-    const double g = g_data + (-xv / v);
-    const double coef = (A.l == L) ? 0.5 * A.step : A.step;
-    const double pj = A.p[gi] + coef * g;
-    A.p[gi] = pj;
-    if (A.l < L) {
-      xv += A.step * pj;
-      A.prop[si + j] = xv;
-      A.beta32[gi] = (float)xv;
-    }
+  const size_t gi = (size_t)c * A.GW + ds.goff + j;
+  if (A.mode == BMODE_CACHE) {
+    A.cache_g[gi] = s;
+    return;
   }
-  if (tid == 0) A.counter[di] = 0u;
+  const int L = A.steps;
+  if (A.l == L) A.gprop[gi] = s;
+  const size_t si = (size_t)c * A.SW + ds.soff;
+  const double var = A.state[si + D + 1];
+  double xv = A.prop[si + j];
+  const double g = s + (-xv / var);
+  const double coef = (A.l == L) ? 0.5 * A.step : A.step;
+  const double pj = A.p[gi] + coef * g;
+  A.p[gi] = pj;
+  if (A.l < L) {
+    xv += A.step * pj;
+    A.prop[si + j] = xv;
+    A.beta32[gi] = (float)xv;
+  }
 }
 
 template <int CP>
@@ -372,15 +375,14 @@ __global__ __launch_bounds__(64) void k_bhmc_vblock(BArgs A) {
   const uint32_t gc = (uint32_t)(di * A.C + c);
   const uint32_t sw = *A.sweep;
   double* st = A.state + (size_t)c * A.SW + ds.soff;
+  double S = 0.0;
+  for (int j = lane; j < D; j += 64) S += st[j] * st[j];
+  S = wave_sum(S);
   auto ll_of = [&](double v) {
-    double s = 0.0;
-    for (int j = lane; j < D; j += 64) s += normal_ll0(st[j], v, A.ref_compat);
-    return wave_sum(s) + v_prior_ll0(A.prior_kind, A.prior_a, A.prior_b, v);
+    return normal_sum_ll(D, S, v, A.ref_compat) + v_prior_ll0(A.prior_kind, A.prior_a, A.prior_b, v);
   };
   auto grad_of = [&](double v) {
-    double s = 0.0;
-    for (int j = lane; j < D; j += 64) s += normal_dotarg2_0(st[j], v, A.ref_compat);
-    return wave_sum(s) + v_prior_dot0(A.prior_kind, A.prior_a, A.prior_b, v);
+    return normal_sum_dv(D, S, v, A.ref_compat) + v_prior_dot0(A.prior_kind, A.prior_a, A.prior_b, v);
   };
   double step;
   int steps;
@@ -444,7 +446,7 @@ struct BhlrSampler final : augur_sampler {
   DevBuf<int> blk_ds;
   DevBuf<float> beta32;
   DevBuf<double> part, cache_g, cache_ll, gprop, llprop, prop, p, p0, k0, state, stats_buf;
-  DevBuf<unsigned> counter;
+  DevBuf<int> val_ds;
   DevBuf<uint32_t> sweep_dev;
   BArgs base{};
   BPassKernel pass = nullptr;
@@ -486,24 +488,32 @@ struct BhlrSampler final : augur_sampler {
   // Blocks of R rows per dataset; partial offsets follow.
   void layout(int rows) {
     R = rows;
-    int blk = 0, poff = 0;
-    std::vector<int> table;
+    int blk = 0, poff = 0, voff = 0;
+    std::vector<int> table, vtable;
     for (int i = 0; i < n_ds; ++i) {
       BDs& d = hds[i];
       d.nblk = (d.n + R - 1) / R;
       d.blk0 = blk;
       d.poff = poff;
+      d.voff = voff;
       for (int b = 0; b < d.nblk; ++b) table.push_back(i);
+      for (int v = 0; v < C * (d.d + 2); ++v) vtable.push_back(i);
       blk += d.nblk;
       poff += C * (d.d + 2) * d.nblk;
+      voff += C * (d.d + 2);
     }
     nblk_total = blk;
     part.alloc((size_t)poff);
     blk_ds.alloc(table.size());
     blk_ds.upload(table.data(), table.size(), stream);
+    val_ds.alloc(vtable.size());
+    val_ds.upload(vtable.data(), vtable.size(), stream);
     dds.upload(hds.data(), hds.size(), stream);
+    AUGUR_HIP(hipStreamSynchronize(stream));  // host tables are rebuilt by the next call
     base.ds = dds.p;
     base.blk_ds = blk_ds.p;
+    base.val_ds = val_ds.p;
+    base.nval = voff;
     base.rows_per_blk = R;
     base.part = part.p;
   }
@@ -516,6 +526,8 @@ struct BhlrSampler final : augur_sampler {
     pass<<<nblk_total, kBThreads, bhlr_smem_bytes(dmax, next_cp()), stream>>>(A);
     AUGUR_CHECK_LAUNCH();
     prof_end();
+    k_bhlr_finish<<<(A.nval + 3) / 4, 256, 0, stream>>>(A);
+    AUGUR_CHECK_LAUNCH();
   }
 
   int next_cp() const {
@@ -709,13 +721,12 @@ augur_status augur_bhlr_create(const augur_bhlr_desc* D, augur_sampler** out) {
       s->cache_ll.alloc(C * s->n_ds); s->llprop.alloc(C * s->n_ds); s->k0.alloc(C * s->n_ds);
       s->state.alloc(C * s->SW); s->prop.alloc(C * s->SW);
       s->stats_buf.alloc(2 * C * s->n_ds * 2);
-      s->counter.alloc(s->n_ds);
       s->sweep_dev.alloc(1);
       s->state.zero(s->stream); s->prop.zero(s->stream); s->stats_buf.zero(s->stream);
-      s->counter.zero(s->stream); s->sweep_dev.zero(s->stream); s->beta32.zero(s->stream);
+      s->sweep_dev.zero(s->stream); s->beta32.zero(s->stream);
       BArgs& A = s->base;
       A.n_ds = s->n_ds; A.C = s->C; A.SW = s->SW; A.GW = s->GW;
-      A.beta32 = s->beta32.p; A.counter = s->counter.p;
+      A.beta32 = s->beta32.p;
       A.cache_g = s->cache_g.p; A.cache_ll = s->cache_ll.p; A.gprop = s->gprop.p; A.llprop = s->llprop.p;
       A.prop = s->prop.p; A.p = s->p.p; A.p0 = s->p0.p; A.k0 = s->k0.p; A.state = s->state.p;
       A.stats = s->stats_buf.p; A.sweep = s->sweep_dev.p; A.seed = s->seed;

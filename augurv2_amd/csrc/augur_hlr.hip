@@ -33,7 +33,6 @@ namespace augur {
 constexpr int kTileRows = 64;
 constexpr int kPassThreads = 256;
 constexpr int kMaxGradSlots = 16;  // (d) * cp <= 16 * 256
-constexpr int kGroup = 16;  // blocks per first-level reduction group
 
 enum { MODE_CACHE = 0, MODE_LEAPFROG = 1 };
 
@@ -43,9 +42,7 @@ struct PassArgs {
   int n, d, cp, c_real, has_b;
   float* beta32;   // cp x (d + 1): evaluation point (theta, b) per chain
   double* part;    // [cp * (d + 2)][G]
-  double* gpart;   // [cp * (d + 2)][ceil(G / kGroup)]
-  unsigned* gcounter;  // per group of blocks
-  unsigned* counter;
+  int G;           // blocks of the pass kernel
   int mode;
   int l;           // leapfrog pass index (1-based)
   double* cache_g;  // cp x (d + 1)
@@ -60,81 +57,52 @@ struct PassArgs {
 };
 
 
-// Two-level deterministic reduction over blocks, called by every block after
-// it has written its partials: the last block of each group of kGroup
-// consecutive blocks sums the group's partials in block order; the last group
-// to finish sums the group sums in group order and applies the epilogue.
-// Each thread owns whole values, so its loads are independent.
-__device__ void pass_finish(const PassArgs& A, unsigned char* smem, int* flag) {
-  const int tid = threadIdx.x, G = gridDim.x;
-  const int d = A.d, cp = A.cp, ds = d + 1, vs = d + 2;
-  const int nval = cp * vs;
-  const int grp = blockIdx.x / kGroup;
-  const int ngroups = (G + kGroup - 1) / kGroup;
-  const int g0 = grp * kGroup, gn = min(kGroup, G - g0);
-  __threadfence();
-  __syncthreads();
-  if (tid == 0) *flag = (atomicAdd(A.gcounter + grp, 1u) == (unsigned)(gn - 1));
-  __syncthreads();
-  if (!*flag) return;
-  __threadfence();
-  for (int v = tid; v < nval; v += kPassThreads) {
-    const double* pv = A.part + (size_t)v * G + g0;
-    double s = 0.0;
-#pragma unroll 16
-    for (int i = 0; i < gn; ++i) s += pv[i];
-    A.gpart[(size_t)v * ngroups + grp] = s;
-  }
-  if (tid == 0) A.gcounter[grp] = 0u;
-  __threadfence();
-  __syncthreads();
-  if (tid == 0) *flag = (atomicAdd(A.counter, 1u) == (unsigned)(ngroups - 1));
-  __syncthreads();
-  if (!*flag) return;
-  __threadfence();
-  double* tot = (double*)smem;  // cp * (d + 2) doubles, aliases the tile
-  for (int v = tid; v < nval; v += kPassThreads) {
-    const double* pv = A.gpart + (size_t)v * ngroups;
-    double s = 0.0;
-#pragma unroll 16
-    for (int i = 0; i < ngroups; ++i) s += pv[i];
-    tot[v] = s;
-  }
-  __syncthreads();
-  const int P = d + 2;
+// Reduction of the block partials and the leapfrog epilogue, one wave per
+// value (chain c, coordinate j; j = d: intercept, j = d + 1: log-density).
+// Lane l sums blocks l, l + 64, ... in order, then a fixed butterfly: the
+// result is deterministic.  The update of coordinate j needs only value j,
+// so every wave finishes independently.  (A separate kernel instead of a
+// last-block epilogue: the device-scope fence each block would need writes
+// back the L2 and serialises across blocks.)
+__global__ __launch_bounds__(256) void k_hlr_finish(PassArgs A) {
+  const int d = A.d, ds = d + 1, vs = d + 2, P = d + 2;
+  const int lane = threadIdx.x & 63;
+  const int v = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (v >= A.cp * vs) return;
+  const double* pv = A.part + (size_t)v * A.G;
+  double s = 0.0;
+  for (int g = lane; g < A.G; g += 64) s += pv[g];
+  s = wave_sum(s);
+  if (lane != 0) return;
+  const int c = v / vs, j = v % vs;
   if (A.mode == MODE_CACHE) {
-    for (int q = tid; q < cp * ds; q += kPassThreads) {
-      const int c = q / ds, j = q % ds;
-      A.cache_g[q] = tot[c * vs + j];
-      if (j == 0) A.cache_ll[c] = tot[c * vs + d + 1];
-    }
-  } else {
-    for (int q = tid; q < A.c_real * ds; q += kPassThreads) {
-      const int c = q / ds, j = q % ds;
-      const int L = A.hs.steps[c];
-      if (A.l > L) continue;
-      const double g_data = tot[c * vs + j];
-      if (A.l == L) {
-        A.gprop[q] = g_data;
-        if (j == 0) A.llprop[c] = tot[c * vs + d + 1];
-      }
-      const bool in_blk = (j < d) ? A.blk_theta : (A.blk_b && A.has_b);
-      if (!in_blk) continue;
-      const double step = A.hs.step[c];
-      const double v = A.state[c * P + d + 1];
-      double xv = A.prop[c * P + j];
-      const double g = g_data + (-xv / v);
-      const double coef = (A.l == L) ? 0.5 * step : step;
-      const double pj = A.p[q] + coef * g;
-      A.p[q] = pj;
-      if (A.l < L) {
-        xv += step * pj;
-        A.prop[c * P + j] = xv;
-        A.beta32[q] = (float)xv;
-      }
-    }
+    if (j <= d) A.cache_g[c * ds + j] = s;
+    else A.cache_ll[c] = s;
+    return;
   }
-  if (tid == 0) *A.counter = 0u;
+  if (c >= A.c_real) return;
+  const int L = A.hs.steps[c];
+  if (A.l > L) return;
+  if (j == d + 1) {
+    if (A.l == L) A.llprop[c] = s;
+    return;
+  }
+  const int q = c * ds + j;
+  if (A.l == L) A.gprop[q] = s;
+  const bool in_blk = (j < d) ? A.blk_theta : (A.blk_b && A.has_b);
+  if (!in_blk) return;
+  const double step = A.hs.step[c];
+  const double var = A.state[c * P + d + 1];
+  double xv = A.prop[c * P + j];
+  const double g = s + (-xv / var);
+  const double coef = (A.l == L) ? 0.5 * step : step;
+  const double pj = A.p[q] + coef * g;
+  A.p[q] = pj;
+  if (A.l < L) {
+    xv += step * pj;
+    A.prop[c * P + j] = xv;
+    A.beta32[q] = (float)xv;
+  }
 }
 
 __global__ __launch_bounds__(kPassThreads) void k_hlr_pass(PassArgs A) {
@@ -254,7 +222,7 @@ __global__ __launch_bounds__(kPassThreads) void k_hlr_pass(PassArgs A) {
     A.part[(size_t)(tid * vs + d + 1) * G + blockIdx.x] = sl;
   }
   (void)my_c;
-  pass_finish(A, smem, flag);
+  (void)flag;
 }
 
 // Specialised data pass for D % 4 == 0 and a power-of-two chain count CP.
@@ -432,7 +400,7 @@ __global__ __launch_bounds__(kPassThreads) void k_hlr_pass_t(PassArgs A) {
     A.part[(size_t)(tid * VS + D) * G + blockIdx.x] = sb;
     A.part[(size_t)(tid * VS + D + 1) * G + blockIdx.x] = sl;
   }
-  pass_finish(A, smem, flag);
+  (void)flag;
 }
 
 size_t pass_t_smem_bytes(int d, int cp) {
@@ -577,16 +545,11 @@ __global__ __launch_bounds__(64) void k_hmc_vblock(VBlockArgs A) {
   const int d = A.d, P = d + 2;
   double* st = A.state + (size_t)c * P;
   const int nterm = d + (A.has_b ? 1 : 0);
-  auto ll_of = [&](double v) {
-    double s = 0.0;
-    for (int j = lane; j < nterm; j += 64) s += normal_ll0(st[j], v, A.ref_compat);
-    return wave_sum(s) + v_prior_ll(A, v);
-  };
-  auto grad_of = [&](double v) {
-    double s = 0.0;
-    for (int j = lane; j < nterm; j += 64) s += normal_dotarg2_0(st[j], v, A.ref_compat);
-    return wave_sum(s) + v_prior_dot(A, v);
-  };
+  double S = 0.0;
+  for (int j = lane; j < nterm; j += 64) S += st[j] * st[j];
+  S = wave_sum(S);
+  auto ll_of = [&](double v) { return normal_sum_ll(nterm, S, v, A.ref_compat) + v_prior_ll(A, v); };
+  auto grad_of = [&](double v) { return normal_sum_dv(nterm, S, v, A.ref_compat) + v_prior_dot(A, v); };
   double step;
   int steps;
   hmc_jitter(Stream{A.seed, (uint32_t)c, A.sweep, A.tag + 1}, A.sim_len, A.eps, A.fixed_steps, &step,
@@ -651,8 +614,6 @@ struct HlrSampler final : augur_sampler {
   DevBuf<float> x, beta32;
   DevBuf<int32_t> y;
   DevBuf<double> state, prop, p0, p, k0, cache_g, cache_ll, gprop, llprop, part, stats_buf, ll_out;
-  DevBuf<double> gpart;
-  DevBuf<unsigned> counter, gcounter;
   bool cache_valid = false;
 
   const char* family() const override { return "hlr"; }
@@ -674,8 +635,7 @@ struct HlrSampler final : augur_sampler {
   PassArgs pass_args(int mode, int l) {
     PassArgs A{};
     A.x = x.p; A.y = y.p; A.n = n; A.d = d; A.cp = cp; A.c_real = chains; A.has_b = has_b;
-    A.beta32 = beta32.p; A.part = part.p; A.gpart = gpart.p; A.gcounter = gcounter.p;
-    A.counter = counter.p; A.mode = mode; A.l = l;
+    A.beta32 = beta32.p; A.part = part.p; A.G = G; A.mode = mode; A.l = l;
     A.cache_g = cache_g.p; A.cache_ll = cache_ll.p; A.gprop = gprop.p; A.llprop = llprop.p;
     A.prop = prop.p; A.p = p.p; A.state = state.p;
     return A;
@@ -689,6 +649,8 @@ struct HlrSampler final : augur_sampler {
       k_hlr_pass<<<G, kPassThreads, pass_smem_bytes(d, cp), stream>>>(A);
     AUGUR_CHECK_LAUNCH();
     prof_end();
+    k_hlr_finish<<<(cp * (d + 2) + 3) / 4, 256, 0, stream>>>(A);
+    AUGUR_CHECK_LAUNCH();
   }
 
   void refresh_cache() {
@@ -788,15 +750,10 @@ struct HlrSampler final : augur_sampler {
     state.alloc(cp * P); prop.alloc(cp * P);
     p0.alloc(cp * ds); p.alloc(cp * ds); k0.alloc(cp);
     cache_g.alloc(cp * ds); cache_ll.alloc(cp); gprop.alloc(cp * ds); llprop.alloc(cp);
-    const int ngroups = (G + kGroup - 1) / kGroup;
     part.alloc((size_t)cp * (d + 2) * G);
-    gpart.alloc((size_t)cp * (d + 2) * ngroups);
-    counter.alloc(1);
-    gcounter.alloc(ngroups);
-    gcounter.zero(stream);
     ll_out.alloc(cp);
     stats_buf.alloc(std::max<size_t>(1, 4 * (size_t)chains * 2));
-    state.zero(stream); prop.zero(stream); beta32.zero(stream); counter.zero(stream);
+    state.zero(stream); prop.zero(stream); beta32.zero(stream);
     stats_buf.zero(stream); p.zero(stream); p0.zero(stream);
   }
 };

@@ -41,6 +41,18 @@ __device__ inline double normal_dotarg2_0(double x, double var, int ref_compat) 
   return (ref_compat ? -0.5 * var : -0.5 / var) + 0.5 * (x * x) / (var * var);
 }
 
+// The v block's full conditional depends on the K coefficients only through
+// K and S = sum x_j^2: sum_j normal_ll(x_j; 0, v) and its v-derivative
+// (normal_dotarg2 summed) in closed form, so a v trajectory needs no
+// per-step reduction.
+__device__ inline double normal_sum_ll(int K, double S, double v, int ref_compat) {
+  if (v > 0.0) return (double)K * (-(ref_compat ? 1.0 : 0.5) * log(v)) - 0.5 * S / v;
+  return (double)K * -kLargePosNum;
+}
+__device__ inline double normal_sum_dv(int K, double S, double v, int ref_compat) {
+  return (double)K * (ref_compat ? -0.5 * v : -0.5 / v) + 0.5 * S / (v * v);
+}
+
 // Prior of the variance: InvGamma(a, b) (cbits/augur_dist.c:345-351) or
 // Exponential(a) (:231-247).
 __device__ inline double v_prior_ll0(int kind, double a, double b, double v) {

@@ -1,0 +1,320 @@
+// augur_mlp.hip -- training and batched evaluation of the gradient-surrogate
+// MLP: a strided tiled FP32 GEMM with fused bias / GELU epilogues, GELU
+// backward, column sums, MSE loss and Adam, all hand-written.
+#include "augur_mlp.h"
+
+namespace augur {
+
+constexpr float kGeluC = 0.7978845608028654f;  // sqrt(2 / pi)
+
+__device__ inline float gelu_f(float z) {
+  return 0.5f * z * (1.0f + tanhf(kGeluC * (z + 0.044715f * z * z * z)));
+}
+__device__ inline float gelu_grad_f(float z) {
+  const float u = kGeluC * (z + 0.044715f * z * z * z);
+  const float t = tanhf(u);
+  const float du = kGeluC * (1.0f + 3.0f * 0.044715f * z * z);
+  return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * du;
+}
+
+struct GemmArgs {
+  int M, N, K;
+  const float* A;
+  long sam, sak;  // A(m, k) = A[m * sam + k * sak]
+  const float* B;
+  long sbk, sbn;  // B(k, n) = B[k * sbk + n * sbn]
+  float* C;
+  int ldc;
+  const float* bias;  // [N] or null
+  float* zout;        // pre-activation copy when act != 0
+  int act;            // 0: none, 1: GELU
+};
+
+// C = A B (+ bias, GELU): 64 x 64 tile per 256-thread block, 4 x 4 per
+// thread, k in steps of 16 through LDS.  Fixed summation order.
+__global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
+  __shared__ float As[16][64 + 4];
+  __shared__ float Bs[16][64 + 4];
+  const int tm = blockIdx.y * 64, tn = blockIdx.x * 64;
+  const int tid = threadIdx.x, tx = tid % 16, ty = tid / 16;
+  float acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = 0.0f;
+  for (int k0 = 0; k0 < g.K; k0 += 16) {
+    for (int e = tid; e < 1024; e += 256) {
+      const int mm = e / 16, kk = e % 16;
+      const int m = tm + mm, k = k0 + kk;
+      As[kk][mm] = (m < g.M && k < g.K) ? g.A[m * g.sam + k * g.sak] : 0.0f;
+      const int kb = e / 64, nb = e % 64;
+      const int k2 = k0 + kb, n = tn + nb;
+      Bs[kb][nb] = (k2 < g.K && n < g.N) ? g.B[k2 * g.sbk + n * g.sbn] : 0.0f;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk) {
+      float a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = As[kk][ty * 4 + i];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = Bs[kk][tx * 4 + j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = fmaf(a[i], b[j], acc[i][j]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int m = tm + ty * 4 + i, n = tn + tx * 4 + j;
+      if (m >= g.M || n >= g.N) continue;
+      float v = acc[i][j] + (g.bias ? g.bias[n] : 0.0f);
+      if (g.act) {
+        g.zout[(size_t)m * g.ldc + n] = v;
+        v = gelu_f(v);
+      }
+      g.C[(size_t)m * g.ldc + n] = v;
+    }
+}
+
+static void gemm(hipStream_t st, int M, int N, int K, const float* A, long sam, long sak, const float* B,
+                 long sbk, long sbn, float* Cm, int ldc, const float* bias = nullptr, float* zout = nullptr,
+                 int act = 0) {
+  GemmArgs g{M, N, K, A, sam, sak, B, sbk, sbn, Cm, ldc, bias, zout, act};
+  k_gemm<<<dim3((N + 63) / 64, (M + 63) / 64), 256, 0, st>>>(g);
+  AUGUR_CHECK_LAUNCH();
+}
+
+// out[n] = sum over rows of x[m][n]
+__global__ void k_colsum(const float* __restrict__ x, int rows, int cols, float* __restrict__ out) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= cols) return;
+  float s = 0.0f;
+  for (int m = 0; m < rows; ++m) s += x[(size_t)m * cols + n];
+  out[n] = s;
+}
+
+__global__ void k_gelu_bwd(const float* __restrict__ dh, const float* __restrict__ z, float* __restrict__ dz,
+                           size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dz[i] = dh[i] * gelu_grad_f(z[i]);
+}
+
+// dout = 2 (out - t) / n and loss = mean (out - t)^2 (one block, fixed order).
+__global__ __launch_bounds__(1024) void k_mse(const float* __restrict__ out, const float* __restrict__ t,
+                                              float* __restrict__ dout, size_t n, double* loss) {
+  __shared__ double red[1024];
+  double s = 0.0;
+  for (size_t i = threadIdx.x; i < n; i += blockDim.x) {
+    const float df = out[i] - t[i];
+    dout[i] = 2.0f * df / (float)n;
+    s += (double)df * (double)df;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *loss = red[0] / (double)n;
+}
+
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, size_t n, float lr, float c1, float c2) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const float mi = 0.9f * m[i] + 0.1f * g[i];
+    const float vi = 0.999f * v[i] + 0.001f * g[i] * g[i];
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + 1e-8f);
+  }
+}
+
+void Mlp::init(int d, int h, int batch_rows, uint32_t seed, hipStream_t st) {
+  s = MlpShape{d, h};
+  batch = batch_rows;
+  const size_t P = s.count();
+  params.alloc(P);
+  grads.alloc(P);
+  m1.alloc(P);
+  m2.alloc(P);
+  z1.alloc((size_t)batch * h); h1.alloc((size_t)batch * h);
+  z2.alloc((size_t)batch * h); h2.alloc((size_t)batch * h);
+  out.alloc((size_t)batch * d); dout.alloc((size_t)batch * d);
+  dh.alloc((size_t)batch * h); dz.alloc((size_t)batch * h);
+  loss.alloc(1);
+  // Uniform(-1/sqrt(fan_in), 1/sqrt(fan_in)) weights from Philox, zero biases
+  std::vector<float> hp(P, 0.0f);
+  auto fill = [&](size_t off, int rows, int fan_in, uint32_t tag) {
+    const Stream rs{seed, 0u, 0u, tag};
+    const float sc = 1.0f / sqrtf((float)fan_in);
+    for (size_t i = 0; i < (size_t)rows * fan_in; ++i)
+      hp[off + i] = sc * (float)(2.0 * u01(rs.draw((uint32_t)i, 0).x) - 1.0);
+  };
+  fill(s.w1(), h, d, 101u);
+  fill(s.w2(), h, h, 102u);
+  fill(s.w3(), d, h, 103u);
+  params.upload(hp.data(), P, st);
+  m1.zero(st);
+  m2.zero(st);
+  adam_t = 0;
+}
+
+void Mlp::forward(const float* x, int rows, hipStream_t st) {
+  const int d = s.d, h = s.h;
+  const float* P = params.p;
+  gemm(st, rows, h, d, x, d, 1, P + s.w1(), 1, d, h1.p, h, P + s.b1(), z1.p, 1);
+  gemm(st, rows, h, h, h1.p, h, 1, P + s.w2(), 1, h, h2.p, h, P + s.b2(), z2.p, 1);
+  gemm(st, rows, d, h, h2.p, h, 1, P + s.w3(), 1, h, out.p, d, P + s.b3());
+}
+
+void Mlp::loss_grad(const float* x, const float* t, int rows, hipStream_t st) {
+  const int d = s.d, h = s.h;
+  forward(x, rows, st);
+  const float* P = params.p;
+  float* Gp = grads.p;
+  k_mse<<<1, 1024, 0, st>>>(out.p, t, dout.p, (size_t)rows * d, loss.p);
+  AUGUR_CHECK_LAUNCH();
+  // layer 3: dW3 = dout^T h2, db3 = colsum(dout), dh2 = dout W3
+  gemm(st, d, h, rows, dout.p, 1, d, h2.p, h, 1, Gp + s.w3(), h);
+  k_colsum<<<(d + 255) / 256, 256, 0, st>>>(dout.p, rows, d, Gp + s.b3());
+  gemm(st, rows, h, d, dout.p, d, 1, P + s.w3(), h, 1, dh.p, h);
+  k_gelu_bwd<<<256, 256, 0, st>>>(dh.p, z2.p, dz.p, (size_t)rows * h);
+  // layer 2
+  gemm(st, h, h, rows, dz.p, 1, h, h1.p, h, 1, Gp + s.w2(), h);
+  k_colsum<<<(h + 255) / 256, 256, 0, st>>>(dz.p, rows, h, Gp + s.b2());
+  gemm(st, rows, h, h, dz.p, h, 1, P + s.w2(), h, 1, dh.p, h);
+  k_gelu_bwd<<<256, 256, 0, st>>>(dh.p, z1.p, dz.p, (size_t)rows * h);
+  // layer 1
+  gemm(st, h, d, rows, dz.p, 1, h, x, d, 1, Gp + s.w1(), d);
+  k_colsum<<<(h + 255) / 256, 256, 0, st>>>(dz.p, rows, h, Gp + s.b1());
+  AUGUR_CHECK_LAUNCH();
+}
+
+void Mlp::adam(double lr, hipStream_t st) {
+  ++adam_t;
+  const float c1 = (float)(1.0 / (1.0 - pow(0.9, adam_t)));
+  const float c2 = (float)(1.0 / (1.0 - pow(0.999, adam_t)));
+  k_adam<<<512, 256, 0, st>>>(params.p, grads.p, m1.p, m2.p, s.count(), (float)lr, c1, c2);
+  AUGUR_CHECK_LAUNCH();
+}
+
+// Minibatch rows drawn uniformly (Philox, step `step`) from n pairs,
+// standardised on the way: xb = (x - mx) * ix, tb = (t - mt) * it.
+__global__ void k_gather_std(const float* __restrict__ x, const float* __restrict__ t, int n, int d,
+                             const float* __restrict__ stats, int rows, uint32_t seed, uint32_t step,
+                             float* __restrict__ xb, float* __restrict__ tb) {
+  const int r = blockIdx.x;
+  if (r >= rows) return;
+  const Stream s{seed, 0u, step, 900u};
+  const uint32_t w = s.draw((uint32_t)r, 0).x;
+  const int src = (int)(((uint64_t)w * (uint64_t)n) >> 32);
+  const float* mx = stats;
+  const float* ix = stats + d;
+  const float* mt = stats + 2 * d;
+  const float* it = stats + 3 * d;
+  for (int j = threadIdx.x; j < d; j += blockDim.x) {
+    xb[(size_t)r * d + j] = (x[(size_t)src * d + j] - mx[j]) * ix[j];
+    tb[(size_t)r * d + j] = (t[(size_t)src * d + j] - mt[j]) * it[j];
+  }
+}
+
+// Column means and inverse standard deviations of x and t (n x d each):
+// stats = [mean x | 1/sd x | mean t | 1/sd t].  One thread per column, FP64.
+__global__ void k_colstats(const float* __restrict__ x, const float* __restrict__ t, int n, int d,
+                           float* __restrict__ stats) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= 2 * d) return;
+  const float* a = j < d ? x : t;
+  const int col = j % d;
+  double s = 0.0, s2 = 0.0;
+  for (int i = 0; i < n; ++i) {
+    const double v = a[(size_t)i * d + col];
+    s += v;
+    s2 += v * v;
+  }
+  const double mean = s / n;
+  const double var = fmax(s2 / n - mean * mean, 1e-30);
+  const int base = j < d ? 0 : 2 * d;
+  stats[base + col] = (float)mean;
+  stats[base + d + col] = (float)(1.0 / sqrt(var));
+}
+
+}  // namespace augur
+
+using namespace augur;
+
+extern "C" {
+
+augur_status augur_op_mlp_loss_grad(int d, int h, int rows, const float* params, const float* x,
+                                    const float* t, float* out, double* loss, float* grads) {
+  return guard([&] {
+    if (d < 1 || h < 1 || rows < 1 || !params || !x || !t) fail(AUGUR_ERR_ARG, "bad arguments");
+    require_device();
+    hipStream_t st;
+    AUGUR_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    try {
+      Mlp m;
+      m.init(d, h, rows, 0, st);
+      m.params.upload(params, m.s.count(), st);
+      DevBuf<float> dx((size_t)rows * d), dt((size_t)rows * d);
+      dx.upload(x, (size_t)rows * d, st);
+      dt.upload(t, (size_t)rows * d, st);
+      m.loss_grad(dx.p, dt.p, rows, st);
+      if (out) m.out.download(out, (size_t)rows * d, st);
+      if (loss) m.loss.download(loss, 1, st);
+      if (grads) m.grads.download(grads, m.s.count(), st);
+      AUGUR_HIP(hipStreamSynchronize(st));
+    } catch (...) {
+      (void)hipStreamDestroy(st);
+      throw;
+    }
+    AUGUR_HIP(hipStreamDestroy(st));
+  });
+}
+
+augur_status augur_op_mlp_train(int d, int h, int n, const float* x, const float* t, int steps,
+                                int batch, double lr, uint32_t seed, float* params_inout,
+                                double* loss_hist) {
+  return guard([&] {
+    if (d < 1 || h < 1 || n < 1 || steps < 0 || batch < 1 || !x || !t || !params_inout)
+      fail(AUGUR_ERR_ARG, "bad arguments");
+    require_device();
+    hipStream_t st;
+    AUGUR_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    try {
+      Mlp m;
+      m.init(d, h, batch, seed, st);
+      m.params.upload(params_inout, m.s.count(), st);
+      DevBuf<float> dx((size_t)n * d), dt((size_t)n * d), xb((size_t)batch * d), tb((size_t)batch * d),
+          stats(4 * (size_t)d);
+      dx.upload(x, (size_t)n * d, st);
+      dt.upload(t, (size_t)n * d, st);
+      // identity standardisation: the caller passes standardised data
+      std::vector<float> hs(4 * (size_t)d);
+      for (int j = 0; j < d; ++j) {
+        hs[j] = 0.0f; hs[d + j] = 1.0f; hs[2 * d + j] = 0.0f; hs[3 * d + j] = 1.0f;
+      }
+      stats.upload(hs.data(), hs.size(), st);
+      for (int s = 0; s < steps; ++s) {
+        k_gather_std<<<batch, 128, 0, st>>>(dx.p, dt.p, n, d, stats.p, batch, seed, (uint32_t)s, xb.p, tb.p);
+        AUGUR_CHECK_LAUNCH();
+        m.loss_grad(xb.p, tb.p, batch, st);
+        if (loss_hist) AUGUR_HIP(hipMemcpyAsync(loss_hist + s, m.loss.p, sizeof(double), hipMemcpyDeviceToHost, st));
+        m.adam(lr, st);
+      }
+      m.params.download(params_inout, m.s.count(), st);
+      AUGUR_HIP(hipStreamSynchronize(st));
+    } catch (...) {
+      (void)hipStreamDestroy(st);
+      throw;
+    }
+    AUGUR_HIP(hipStreamDestroy(st));
+  });
+}
+
+}  // extern "C"

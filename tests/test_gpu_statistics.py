@@ -36,8 +36,10 @@ def test_hlr_posterior_matches_oracle(rt, oracle):
     y = (rng.random(n) < 1 / (1 + np.exp(-(x @ np.array([1.0, -0.5, 0.25]) + 0.3)))).astype(np.int32)
     blocks = [(HMC_BLOCK_V, 1.0, 0.05, 0), (HMC_BLOCK_B, 0.3, 0.05, 0),
               (HMC_BLOCK_THETA, 0.3, 0.05, 0)]
+    # textbook Normal density: the reference's normal_dotarg2 makes HMC on v
+    # accept rarely, and its slow mixing leaves too few effective draws
     s = rt.HlrSampler(x, y, blocks, chains=C, seed=5, prior_kind=1, prior_a=1.0,
-                      has_intercept=True)
+                      has_intercept=True, ref_compat=False)
     init = [np.array([0.1 * c, -0.1, 0.0, 0.0, 1.0]) for c in range(C)]
     for c in range(C):
         s.set("state", init[c], chain=c)
@@ -46,7 +48,8 @@ def test_hlr_posterior_matches_oracle(rt, oracle):
     s.step(iters)
     g = s.read_trace("state")  # (iters, C, d + 2)
     s.close()
-    m = oracle.Hlr(x.astype(np.float64), y, prior_kind=1, prior_a=1.0, has_intercept=True)
+    m = oracle.Hlr(x.astype(np.float64), y, prior_kind=1, prior_a=1.0, has_intercept=True,
+                   ref_compat=False)
     kinds = {HMC_BLOCK_V: 0, HMC_BLOCK_B: 1, HMC_BLOCK_THETA: 2}
     o = np.zeros_like(g)
     for c in range(C):
