@@ -50,6 +50,13 @@ class BhlrDesc(C.Structure):
                 ("rows_per_block", C.c_int), ("use_graph", C.c_int)]
 
 
+class SurrogateCfg(C.Structure):
+    _fields_ = [("block_index", C.c_int), ("hidden", C.c_int), ("train_sweeps", C.c_int),
+                ("max_pairs", C.c_int), ("epochs", C.c_int), ("batch", C.c_int),
+                ("lr", C.c_double), ("steps", C.c_int), ("step_size", C.c_double),
+                ("min_accept", C.c_double), ("window", C.c_int), ("seed", C.c_uint32)]
+
+
 class GmmDesc(C.Structure):
     _fields_ = [("n_points", C.c_int), ("dim", C.c_int), ("n_components", C.c_int),
                 ("chains", C.c_int), ("seed", C.c_uint32), ("y", C.c_void_p),
@@ -73,6 +80,10 @@ SIGNATURES = {
     "augur_hlr_create": [_P, _P],
     "augur_gmm_create": [_P, _P],
     "augur_bhlr_create": [_P, _P],
+    "augur_hlr_surrogate": [_H, _P],
+    "augur_hlr_surrogate_info": [_H, _P, _S],
+    "augur_op_mlp_loss_grad": [_I, _I, _I, _P, _P, _P, _P, _P, _P],
+    "augur_op_mlp_train": [_I, _I, _I, _P, _P, _I, _I, C.c_double, _U, _P, _P],
     "augur_bhlr_tuning": [_H, _P, _P, _P, _P, _I],
     "augur_set_pt": [_H, C.c_char_p, _I, _P, _S],
     "augur_cpy": [_H, C.c_char_p, _I, _P, _S],

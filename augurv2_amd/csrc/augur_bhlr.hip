@@ -194,13 +194,14 @@ __device__ void bhlr_body(const BArgs& A, const BDs& ds, int di, unsigned char* 
     }
     __syncthreads();
   }
-  // partials of this block: part[poff + value * nblk + local]
+  // partials of this block: part[poff + value * nblk + local]; the region
+  // holds the C real chains only (chains padded up to CP write nothing)
   double* part = A.part + ds.poff;
   if constexpr (RS > 1) {
 #pragma unroll
     for (int k = 0; k < 4; ++k) red2[tid * 4 + k] = gacc[0][k];
     __syncthreads();
-    if (tid < NP) {
+    if (tid < NP && tid / (D / 4) < A.C) {
       const int c = tid / (D / 4), jq = tid % (D / 4);
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
@@ -214,6 +215,7 @@ __device__ void bhlr_body(const BArgs& A, const BDs& ds, int di, unsigned char* 
     for (int s = 0; s < NS; ++s) {
       const int pair = tid + s * kBThreads;
       const int c = pair / (D / 4), jq = pair % (D / 4);
+      if (c >= A.C) continue;
 #pragma unroll
       for (int k = 0; k < 4; ++k) part[(size_t)(c * VS + 4 * jq + k) * ds.nblk + local] = gacc[s][k];
     }
@@ -221,7 +223,7 @@ __device__ void bhlr_body(const BArgs& A, const BDs& ds, int di, unsigned char* 
   red[tid] = gbacc;
   red[kBThreads + tid] = llacc;
   __syncthreads();
-  if (tid < CP) {
+  if (tid < A.C) {
     double sb = 0.0, sl = 0.0;
     for (int i = tid; i < kBThreads; i += CP) {
       sb += red[i];
@@ -551,10 +553,14 @@ struct BhlrSampler final : augur_sampler {
     AUGUR_HIP(hipEventCreate(&e1));
     int best = 4096;
     float best_us = 1e30f;
-    k_bstate_to_beta32<<<dim3(n_ds, C), 64, 0, stream>>>(base);
-    AUGUR_CHECK_LAUNCH();
+    bool first = true;
     for (int rows : cands) {
-      layout(rows);
+      layout(rows);  // sets base.ds and the block / value tables
+      if (first) {
+        k_bstate_to_beta32<<<dim3(n_ds, C), 64, 0, stream>>>(base);
+        AUGUR_CHECK_LAUNCH();
+        first = false;
+      }
       BArgs A = base;
       A.mode = BMODE_CACHE;
       pass<<<nblk_total, kBThreads, bhlr_smem_bytes(dmax, next_cp()), stream>>>(A);  // warm

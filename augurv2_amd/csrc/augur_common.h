@@ -97,6 +97,14 @@ struct DevBuf {
 
 void require_device();
 
+// Dynamic LDS beyond 64 KB must be granted per kernel (gfx950 has 160 KB).
+template <class K>
+inline void allow_lds(K kernel, size_t bytes) {
+  if (bytes > 65536)
+    AUGUR_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+}
+
 // A parameter the generic ABI can address by name: one contiguous block of
 // `elems` elements per chain, chain c at base + c * chain_stride.
 struct ParamView {

@@ -26,7 +26,11 @@
 // are exactly what the pass computed when that point was proposed), so an
 // iteration of L leapfrog steps costs L data passes instead of the
 // reference's L + 3 (L + 1 gradients and 2 log-densities).
+#include <chrono>
+
 #include "augur_hmc.h"
+#include "augur_mlp.h"
+#include "augur_surrogate.h"
 
 namespace augur {
 
@@ -54,6 +58,11 @@ struct PassArgs {
   const double* state;  // cp x P
   int blk_theta, blk_b;
   HmcStep hs;
+  // training-pair capture (null: off): row cap_row + c receives chain c's
+  // evaluation point (FP32) and data gradient
+  float* cap_x;
+  float* cap_y;
+  int cap_row;
 };
 
 
@@ -89,6 +98,11 @@ __global__ __launch_bounds__(256) void k_hlr_finish(PassArgs A) {
   }
   const int q = c * ds + j;
   if (A.l == L) A.gprop[q] = s;
+  if (A.cap_x && j < d) {
+    const size_t row = (size_t)(A.cap_row + c) * d + j;
+    A.cap_x[row] = A.beta32[q];  // the point this pass evaluated
+    A.cap_y[row] = (float)s;
+  }
   const bool in_blk = (j < d) ? A.blk_theta : (A.blk_b && A.has_b);
   if (!in_blk) return;
   const double step = A.hs.step[c];
@@ -512,7 +526,8 @@ __global__ __launch_bounds__(64) void k_hmc_end(HmcArgs A) {
   const bool acc = log(u) < (ll1 - ll0) + (A.k0[c] - k1);
   if (acc) {
     for (int j = lane; j < P; j += 64) A.state[c * P + j] = A.prop[c * P + j];
-    for (int j = lane; j < ds; j += 64) A.cache_g_w[c * ds + j] = A.gprop[c * ds + j];
+    if (A.gprop)  // null in surrogate mode: no exact gradient at the proposal
+      for (int j = lane; j < ds; j += 64) A.cache_g_w[c * ds + j] = A.gprop[c * ds + j];
     if (lane == 0) A.cache_ll_w[c] = A.llprop[c];
   }
   if (lane == 0) {
@@ -616,6 +631,17 @@ struct HlrSampler final : augur_sampler {
   DevBuf<double> state, prop, p0, p, k0, cache_g, cache_ll, gprop, llprop, part, stats_buf, ll_out;
   bool cache_valid = false;
 
+  // ---- surrogate-gradient HMC on one theta block ----
+  enum { SURR_OFF = 0, SURR_COLLECT = 1, SURR_ACTIVE = 2, SURR_FALLBACK = 3 };
+  int surr_phase = SURR_OFF;
+  augur_surrogate_cfg scfg{};
+  int surr_sweeps_collected = 0, n_pairs = 0, max_pairs = 0, surr_iters = 0, fallbacks = 0;
+  double surr_loss = 0.0, surr_train_ms = 0.0, window_accept = -1.0, win_acc0 = 0.0, win_prop0 = 0.0;
+  Mlp mlp;
+  DevBuf<float> cap_x, cap_y, sstats, xb, tb;
+  DevBuf<uint16_t> w1t, w2t, w3t;
+  DevBuf<double> ll64_cur, ll64_prop, ll64_part;
+
   const char* family() const override { return "hlr"; }
 
   ParamView param(const std::string& name) override {
@@ -643,10 +669,12 @@ struct HlrSampler final : augur_sampler {
 
   void launch_pass(const PassArgs& A) {
     prof_begin();
-    if (fast_pass)
+    if (fast_pass) {
       fast_pass<<<G, kPassThreads, pass_t_smem_bytes(d, cp), stream>>>(A);
-    else
+    } else {
+      allow_lds(k_hlr_pass, pass_smem_bytes(d, cp));
       k_hlr_pass<<<G, kPassThreads, pass_smem_bytes(d, cp), stream>>>(A);
+    }
     AUGUR_CHECK_LAUNCH();
     prof_end();
     k_hlr_finish<<<(cp * (d + 2) + 3) / 4, 256, 0, stream>>>(A);
@@ -660,7 +688,96 @@ struct HlrSampler final : augur_sampler {
     cache_valid = true;
   }
 
+  HmcArgs end_args_surrogate(int bi) {
+    HmcArgs H{};
+    H.d = d; H.cp = cp; H.c_real = chains; H.has_b = has_b; H.ref_compat = ref_compat;
+    H.blk_theta = 1; H.blk_b = 0; H.block_kind = AUGUR_HMC_BLOCK_THETA;
+    H.seed = seed; H.sweep = sweep; H.tag = TAG_HMC_BASE + 4u * (uint32_t)bi;
+    H.cache_g = cache_g.p; H.cache_ll = ll64_cur.p; H.cache_g_w = cache_g.p; H.cache_ll_w = ll64_cur.p;
+    H.gprop = nullptr; H.llprop = ll64_prop.p; H.state = state.p; H.prop = prop.p; H.p0 = p0.p;
+    H.p = p.p; H.k0 = k0.p; H.beta32 = beta32.p; H.stats = stats_buf.p + (size_t)bi * chains * 2;
+    return H;
+  }
+
+  Ll64Args ll64_args(const double* point, double* out) {
+    return Ll64Args{x.p, y.p, n, d, cp, has_b, d + 2, point, ll64_part.p, G, out};
+  }
+
+  // Fits the MLP to the captured (point, data gradient) pairs, then switches
+  // the block to surrogate trajectories.
+  void train_surrogate() {
+    AUGUR_HIP(hipStreamSynchronize(stream));
+    const auto t0 = std::chrono::steady_clock::now();
+    mlp_colstats(cap_x.p, cap_y.p, n_pairs, d, sstats.p, stream);
+    const long steps = std::max<long>(1, (long)scfg.epochs * n_pairs / scfg.batch);
+    double lsum = 0.0;
+    int lcount = 0;
+    for (long s = 0; s < steps; ++s) {
+      mlp_gather(cap_x.p, cap_y.p, n_pairs, d, sstats.p, scfg.batch, scfg.seed, (uint32_t)s, xb.p, tb.p, stream);
+      mlp.loss_grad(xb.p, tb.p, scfg.batch, stream);
+      if (s >= steps - 20) {
+        double l = 0.0;
+        mlp.loss.download(&l, 1, stream);
+        AUGUR_HIP(hipStreamSynchronize(stream));
+        lsum += l;
+        ++lcount;
+      }
+      mlp.adam(scfg.lr, stream);
+    }
+    surr_loss = lcount ? lsum / lcount : 0.0;
+    make_inference_weights(mlp.params.p, d, scfg.hidden, w1t.p, w2t.p, w3t.p, stream);
+    launch_ll64(ll64_args(state.p, ll64_cur.p), stream);
+    std::vector<double> st((size_t)sched.size() * chains * 2);
+    stats_buf.download(st.data(), st.size(), stream);
+    AUGUR_HIP(hipStreamSynchronize(stream));
+    surr_train_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    win_acc0 = win_prop0 = 0.0;
+    for (int c = 0; c < chains; ++c) {
+      win_acc0 += st[((size_t)scfg.block_index * chains + c) * 2];
+      win_prop0 += st[((size_t)scfg.block_index * chains + c) * 2 + 1];
+    }
+    surr_phase = SURR_ACTIVE;
+  }
+
+  void surrogate_iteration(int bi) {
+    SurrTrajArgs S{};
+    S.d = d; S.h = scfg.hidden; S.P = d + 2; S.ds = d + 1; S.c_real = chains;
+    S.step = scfg.step_size; S.steps = scfg.steps;
+    S.seed = seed; S.sweep = sweep; S.tag = TAG_HMC_BASE + 4u * (uint32_t)bi;
+    const MlpShape sh{d, scfg.hidden};
+    S.w1t = w1t.p; S.b1 = mlp.params.p + sh.b1(); S.w2t = w2t.p; S.b2 = mlp.params.p + sh.b2();
+    S.w3t = w3t.p; S.b3 = mlp.params.p + sh.b3(); S.stats = sstats.p;
+    S.state = state.p; S.prop = prop.p; S.p = p.p; S.k0 = k0.p; S.beta32 = beta32.p;
+    launch_surr_traj(S, chains, stream);
+    launch_ll64(ll64_args(prop.p, ll64_prop.p), stream);
+    k_hmc_end<<<chains, 64, 0, stream>>>(end_args_surrogate(bi));
+    AUGUR_CHECK_LAUNCH();
+    cache_valid = false;  // the FP32 data cache no longer tracks the state
+    if (++surr_iters % scfg.window == 0) {
+      std::vector<double> st((size_t)sched.size() * chains * 2);
+      stats_buf.download(st.data(), st.size(), stream);
+      AUGUR_HIP(hipStreamSynchronize(stream));
+      double a = 0.0, pr = 0.0;
+      for (int c = 0; c < chains; ++c) {
+        a += st[((size_t)bi * chains + c) * 2];
+        pr += st[((size_t)bi * chains + c) * 2 + 1];
+      }
+      window_accept = (a - win_acc0) / std::max(1.0, pr - win_prop0);
+      win_acc0 = a;
+      win_prop0 = pr;
+      if (window_accept < scfg.min_accept) {
+        surr_phase = SURR_FALLBACK;
+        ++fallbacks;
+      }
+    }
+  }
+
   void data_block(int bi, const augur_hmc_block& blk) {
+    const bool surr_blk = surr_phase != SURR_OFF && bi == scfg.block_index;
+    if (surr_blk && surr_phase == SURR_ACTIVE) {
+      surrogate_iteration(bi);
+      return;
+    }
     if (!cache_valid) refresh_cache();
     const uint32_t tag = TAG_HMC_BASE + 4u * (uint32_t)bi;
     HmcArgs H{};
@@ -684,12 +801,21 @@ struct HlrSampler final : augur_sampler {
     A.blk_theta = H.blk_theta;
     A.blk_b = H.blk_b;
     A.hs = H.hs;
+    const bool capture = surr_blk && surr_phase == SURR_COLLECT;
     for (int l = 1; l <= lmax; ++l) {
       A.l = l;
+      A.cap_x = nullptr;
+      if (capture && n_pairs + chains <= max_pairs) {
+        A.cap_x = cap_x.p;
+        A.cap_y = cap_y.p;
+        A.cap_row = n_pairs;
+        n_pairs += chains;
+      }
       launch_pass(A);
     }
     k_hmc_end<<<chains, 64, 0, stream>>>(H);
     AUGUR_CHECK_LAUNCH();
+    if (capture && ++surr_sweeps_collected >= scfg.train_sweeps && n_pairs > 0) train_surrogate();
   }
 
   VBlockArgs vargs(int bi, const augur_hmc_block& blk) {
@@ -842,6 +968,54 @@ augur_status augur_op_hlr_ll_grad(int n_rows, int d, int chains, const float* x,
       ll[c] = l[c];
       for (int j = 0; j <= d; ++j) grad[(size_t)c * (d + 1) + j] = g[(size_t)c * (d + 1) + j];
     }
+  });
+}
+
+augur_status augur_hlr_surrogate(augur_sampler* s, const augur_surrogate_cfg* cfg) {
+  return guard([&] {
+    auto* h = dynamic_cast<HlrSampler*>(s);
+    if (!h || !cfg) fail(AUGUR_ERR_ARG, "needs an HLR sampler and a configuration");
+    const augur_surrogate_cfg& c = *cfg;
+    if (c.block_index < 0 || c.block_index >= (int)h->sched.size() ||
+        h->sched[c.block_index].block != AUGUR_HMC_BLOCK_THETA)
+      fail(AUGUR_ERR_ARG, "block_index must name an HMC [theta] block");
+    if (h->has_b) fail(AUGUR_ERR_UNSUPPORTED, "the surrogate covers models without an intercept");
+    if (h->d > 256 || c.hidden < 1 || c.hidden > 256) fail(AUGUR_ERR_ARG, "need d <= 256 and 1 <= hidden <= 256");
+    if (c.train_sweeps < 1 || c.epochs < 1 || c.batch < 1 || c.batch > 65536 || !(c.lr > 0.0) ||
+        c.window < 1 || c.steps < 1 || !(c.step_size > 0.0) || c.max_pairs < h->chains)
+      fail(AUGUR_ERR_ARG, "bad surrogate settings");
+    if (h->surr_phase != HlrSampler::SURR_OFF) fail(AUGUR_ERR_STATE, "surrogate already configured");
+    h->scfg = c;
+    h->max_pairs = c.max_pairs;
+    h->cap_x.alloc((size_t)c.max_pairs * h->d);
+    h->cap_y.alloc((size_t)c.max_pairs * h->d);
+    h->sstats.alloc(4 * (size_t)h->d);
+    h->xb.alloc((size_t)c.batch * h->d);
+    h->tb.alloc((size_t)c.batch * h->d);
+    h->w1t.alloc((size_t)h->d * c.hidden);
+    h->w2t.alloc((size_t)c.hidden * c.hidden);
+    h->w3t.alloc((size_t)c.hidden * h->d);
+    h->ll64_cur.alloc(h->cp);
+    h->ll64_prop.alloc(h->cp);
+    h->ll64_part.alloc((size_t)h->cp * h->G);
+    h->mlp.init(h->d, c.hidden, c.batch, c.seed, h->stream);
+    AUGUR_HIP(hipStreamSynchronize(h->stream));
+    h->surr_phase = HlrSampler::SURR_COLLECT;
+  });
+}
+
+augur_status augur_hlr_surrogate_info(augur_sampler* s, double* info, size_t bytes) {
+  return guard([&] {
+    auto* h = dynamic_cast<HlrSampler*>(s);
+    if (!h || !info || bytes < 8 * sizeof(double)) fail(AUGUR_ERR_ARG, "bad arguments");
+    info[0] = h->surr_phase;
+    info[1] = h->n_pairs;
+    info[2] = h->surr_loss;
+    info[3] = h->surr_train_ms;
+    info[4] = h->fallbacks;
+    info[5] = h->surr_iters;
+    info[6] = h->window_accept;
+    info[7] = h->surr_sweeps_collected;
   });
 }
 

@@ -37,4 +37,10 @@ struct Mlp {
   void adam(double lr, hipStream_t st);
 };
 
+// stats = [mean x | 1/sd x | mean t | 1/sd t] over n rows of x and t (n x d)
+void mlp_colstats(const float* x, const float* t, int n, int d, float* stats, hipStream_t st);
+// `rows` rows drawn uniformly from the n pairs (Philox, step), standardised
+void mlp_gather(const float* x, const float* t, int n, int d, const float* stats, int rows,
+                uint32_t seed, uint32_t step, float* xb, float* tb, hipStream_t st);
+
 }  // namespace augur

@@ -161,6 +161,7 @@ void Mlp::init(int d, int h, int batch_rows, uint32_t seed, hipStream_t st) {
   params.upload(hp.data(), P, st);
   m1.zero(st);
   m2.zero(st);
+  AUGUR_HIP(hipStreamSynchronize(st));  // hp is released on return
   adam_t = 0;
 }
 
@@ -242,6 +243,17 @@ __global__ void k_colstats(const float* __restrict__ x, const float* __restrict_
   const int base = j < d ? 0 : 2 * d;
   stats[base + col] = (float)mean;
   stats[base + d + col] = (float)(1.0 / sqrt(var));
+}
+
+void mlp_colstats(const float* x, const float* t, int n, int d, float* stats, hipStream_t st) {
+  k_colstats<<<(2 * d + 63) / 64, 64, 0, st>>>(x, t, n, d, stats);
+  AUGUR_CHECK_LAUNCH();
+}
+
+void mlp_gather(const float* x, const float* t, int n, int d, const float* stats, int rows, uint32_t seed,
+                uint32_t step, float* xb, float* tb, hipStream_t st) {
+  k_gather_std<<<rows, 128, 0, st>>>(x, t, n, d, stats, rows, seed, step, xb, tb);
+  AUGUR_CHECK_LAUNCH();
 }
 
 }  // namespace augur

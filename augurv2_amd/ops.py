@@ -81,3 +81,25 @@ def gmm_conj(alpha, mu0, cov0, df, scale0, cnt, sum_y, sum_yy, pis, mu, cov, see
            L.ptr(scale0), L.ptr(cnt), L.ptr(sum_y), L.ptr(sum_yy), int(sample_pi),
            int(sample_cov), seed, 0, sweep, L.ptr(pis), L.ptr(mu), L.ptr(cov))
     return pis, mu, cov
+
+
+def mlp_loss_grad(params, x, t, d, h):
+    """Surrogate MLP: outputs, MSE loss and parameter gradient (FP32 kernels)."""
+    params, x, t = L.f32(params), L.f32(x), L.f32(t)
+    rows = x.shape[0]
+    out = np.zeros((rows, d), np.float32)
+    loss = np.zeros(1)
+    grads = np.zeros_like(params)
+    L.call("augur_op_mlp_loss_grad", d, h, rows, L.ptr(params), L.ptr(x), L.ptr(t), L.ptr(out),
+           L.ptr(loss), L.ptr(grads))
+    return out, float(loss[0]), grads
+
+
+def mlp_train(params, x, t, d, h, steps, batch, lr, seed):
+    """Adam over Philox-drawn minibatches; returns (params, loss per step)."""
+    p = L.f32(params).copy()
+    x, t = L.f32(x), L.f32(t)
+    hist = np.zeros(steps)
+    L.call("augur_op_mlp_train", d, h, x.shape[0], L.ptr(x), L.ptr(t), int(steps), int(batch),
+           float(lr), seed & 0xFFFFFFFF, L.ptr(p), L.ptr(hist))
+    return p, hist

@@ -99,6 +99,48 @@ typedef struct {
 } augur_hlr_desc;
 augur_status augur_hlr_create(const augur_hlr_desc* desc, augur_sampler** out);
 
+/* No counterpart in the reference.  Surrogate-gradient HMC for one
+ * HMC [theta] block of an HLR sampler without intercept:
+ *   1. the first train_sweeps sweeps run exact HMC and record every leapfrog
+ *      point with its data gradient (up to max_pairs rows);
+ *   2. an MLP d -> hidden -> hidden -> d (GELU) is then fitted to them on the
+ *      device (Adam, epochs passes of minibatches of `batch`, rate lr);
+ *   3. later sweeps run `steps` leapfrog steps of step_size with the
+ *      network's data gradient plus the exact prior gradient;
+ *   4. the proposal is accepted or rejected with the exact log-density (FP64
+ *      row arithmetic), so the target is unchanged;
+ *   5. if the acceptance rate over a window of `window` sweeps falls below
+ *      min_accept, the block returns to exact HMC for good. */
+typedef struct {
+  int block_index;   /* index in augur_hlr_desc.blocks of the HMC [theta] block */
+  int hidden;        /* 1..256 */
+  int train_sweeps;
+  int max_pairs;
+  int epochs;
+  int batch;
+  double lr;
+  int steps;
+  double step_size;
+  double min_accept;
+  int window;
+  uint32_t seed;     /* network initialisation and minibatch draws */
+} augur_surrogate_cfg;
+augur_status augur_hlr_surrogate(augur_sampler* s, const augur_surrogate_cfg* cfg);
+/* info[8]: phase (0 off, 1 collecting, 2 surrogate, 3 fell back), pairs
+ * collected, final training loss (standardised MSE), training time in ms,
+ * fallbacks, surrogate sweeps run, last window's acceptance, collection
+ * sweeps run. */
+augur_status augur_hlr_surrogate_info(augur_sampler* s, double* info, size_t bytes);
+/* The surrogate network's loss and gradient on host data (rows x d inputs
+ * and targets, parameters in the order W1, b1, W2, b2, W3, b3), and a run of
+ * Adam steps over minibatches drawn from n pairs: the kernels the sampler
+ * trains with, exposed for checks against the CPU restatement. */
+augur_status augur_op_mlp_loss_grad(int d, int h, int rows, const float* params, const float* x,
+                                    const float* t, float* out, double* loss, float* grads);
+augur_status augur_op_mlp_train(int d, int h, int n, const float* x, const float* t, int steps,
+                                int batch, double lr, uint32_t seed, float* params_inout,
+                                double* loss_hist);
+
 /* Batched form of the hlr.rv family without intercept: n_datasets
  * independent regressions of uneven N_i and D_i in {16, 32, 64, 128}, `chains`
  * chains each, schedule HMC [v] (*) HMC [theta] with a fixed leapfrog count

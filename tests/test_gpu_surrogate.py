@@ -1,0 +1,146 @@
+"""The gradient surrogate: its training kernels against the FP64 oracle, the
+exactness of surrogate HMC (same posterior as exact HMC), what the trained
+network gains, and the fallback when it is poor."""
+import numpy as np
+import pytest
+
+from augurv2_amd import diagnostics
+from oracle import mlp as omlp
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    from augurv2_amd import _lib, ops, runtime
+    _lib.require_gpu()
+    return ops, runtime
+
+
+def _flat(p):
+    return np.concatenate([p["W1"].ravel(), p["b1"], p["W2"].ravel(), p["b2"], p["W3"].ravel(),
+                           p["b3"]]).astype(np.float32)
+
+
+def _unflat(v, d, h):
+    v = np.asarray(v, np.float64)
+    o = 0
+    out = {}
+    for k, shape in (("W1", (h, d)), ("b1", (h,)), ("W2", (h, h)), ("b2", (h,)), ("W3", (d, h)),
+                     ("b3", (d,))):
+        n = int(np.prod(shape))
+        out[k] = v[o:o + n].reshape(shape)
+        o += n
+    return out
+
+
+def test_mlp_loss_and_gradient_match_oracle(gpu):
+    ops, _ = gpu
+    rng = np.random.default_rng(0)
+    d, h, rows = 16, 48, 300
+    p = omlp.init_params(d, h, rng)
+    for k in ("b1", "b2", "b3"):
+        p[k] = 0.1 * rng.standard_normal(p[k].shape)
+    x = rng.standard_normal((rows, d)).astype(np.float32)
+    t = rng.standard_normal((rows, d)).astype(np.float32)
+    p32 = _unflat(_flat(p), d, h)  # the FP32-rounded parameters the GPU sees
+    out, loss, g = ops.mlp_loss_grad(_flat(p), x, t, d, h)
+    ref_out, _ = omlp.forward(p32, x.astype(np.float64))
+    ref_loss, ref_g = omlp.loss_and_grads(p32, x.astype(np.float64), t.astype(np.float64))
+    np.testing.assert_allclose(out, ref_out, rtol=1e-4, atol=1e-5)
+    assert abs(loss - ref_loss) <= 1e-5 * ref_loss
+    gg = _unflat(g, d, h)
+    for k in ref_g:
+        scale = np.abs(ref_g[k]).max()
+        assert np.abs(gg[k] - ref_g[k]).max() <= 1e-4 * scale + 1e-7, k
+
+
+def test_mlp_adam_step_matches_oracle_and_training_converges(gpu, oracle):
+    ops, _ = gpu
+    rng = np.random.default_rng(1)
+    d, h, n, batch, lr = 8, 32, 2000, 64, 3e-3
+    x = rng.standard_normal((n, d)).astype(np.float32)
+    w = rng.standard_normal((d, d)) / np.sqrt(d)
+    t = np.tanh(x @ w).astype(np.float32)  # a smooth target the network can fit
+    p0 = omlp.init_params(d, h, rng)
+    p1, hist = ops.mlp_train(_flat(p0), x, t, d, h, steps=1, batch=batch, lr=lr, seed=5)
+    # the same minibatch the device drew: row floor(u32 * n / 2^32) of Philox
+    # (seed, chain 0) at counter (row, 0, step 0, tag 900)
+    idx = [int((int(oracle.draw4(5, 0, r, 0, 0, 900)[0]) * n) >> 32) for r in range(batch)]
+    q = _unflat(_flat(p0), d, h)
+    loss, g = omlp.loss_and_grads(q, x[idx].astype(np.float64), t[idx].astype(np.float64))
+    assert abs(hist[0] - loss) <= 1e-5 * loss
+    m = {k: np.zeros_like(v) for k, v in q.items()}
+    v = {k: np.zeros_like(v) for k, v in q.items()}
+    omlp.adam_step(q, g, m, v, 1, lr)
+    np.testing.assert_allclose(p1, _flat(q), rtol=0, atol=lr * 1e-2)
+    _, hist = ops.mlp_train(_flat(p0), x, t, d, h, steps=600, batch=batch, lr=lr, seed=6)
+    assert hist[-20:].mean() < 0.1 * hist[:20].mean()
+
+
+def _hlr_data(rng, n, d):
+    x = rng.standard_normal((n, d)).astype(np.float32)
+    beta = rng.standard_normal(d) * 0.4
+    y = (rng.random(n) < 1 / (1 + np.exp(-(x @ beta)))).astype(np.int32)
+    return x, y
+
+
+def _run(rt, x, y, surrogate, seed, n_sweeps, train=300, fallback_cfg=None):
+    from augurv2_amd._lib import HMC_BLOCK_THETA, HMC_BLOCK_V
+    C, L, EPS = 4, 10, 0.02
+    d = x.shape[1]
+    s = rt.HlrSampler(x, y, [(HMC_BLOCK_V, 1.0, 0.05, 0), (HMC_BLOCK_THETA, 0.2, EPS, L)],
+                      chains=C, seed=seed, ref_compat=False)
+    for c in range(C):
+        s.set("state", np.concatenate([np.zeros(d), [0.0], [1.0]]), chain=c)
+    if surrogate:
+        kw = dict(hidden=64, train_sweeps=train, max_pairs=200_000, epochs=40, batch=256,
+                  lr=2e-3, steps=L, step_size=EPS, min_accept=0.6, window=50, seed=3)
+        kw.update(fallback_cfg or {})
+        s.enable_surrogate(1, **kw)
+        s.step(train)  # collection with exact HMC, then training
+    s.step(100)
+    s.trace("state", n_sweeps)
+    s.step(n_sweeps)
+    draws = s.read_trace("state")
+    info = s.surrogate_info() if surrogate else None
+    acc = s.acceptance()
+    s.close()
+    return draws, info, acc
+
+
+def test_surrogate_hmc_is_exact_and_pays_off(gpu):
+    _, rt = gpu
+    rng = np.random.default_rng(2)
+    x, y = _hlr_data(rng, 20000, 16)
+    d = x.shape[1]
+    n = 2000
+    exact, _, acc_e = _run(rt, x, y, False, 11, n)
+    surr, info, acc_s = _run(rt, x, y, True, 12, n)
+    assert info["phase"] == 2 and info["fallbacks"] == 0, info
+    assert info["window_accept"] >= 0.6, info
+    keep = list(range(d)) + [d + 1]
+    for j in keep:
+        a, b = exact[:, :, j].T, surr[:, :, j].T
+        assert diagnostics.rhat(b) < 1.05
+        se = np.hypot(diagnostics.mcse_mean(a), diagnostics.mcse_mean(b))
+        assert abs(a.mean() - b.mean()) <= 3 * se, (j, a.mean(), b.mean(), se)
+    # the gain: effective draws per exact data pass.  Exact HMC spends L = 10
+    # passes per sweep; surrogate HMC spends one (the log-density of the
+    # proposal) per sweep.
+    ess_e = min(diagnostics.ess_bulk(exact[:, :, j].T) for j in keep)
+    ess_s = min(diagnostics.ess_bulk(surr[:, :, j].T) for j in keep)
+    assert (ess_s / (n * 1)) >= 3.0 * (ess_e / (n * 10)), (ess_s, ess_e)
+
+
+def test_untrained_surrogate_falls_back_to_exact_hmc(gpu):
+    _, rt = gpu
+    rng = np.random.default_rng(3)
+    x, y = _hlr_data(rng, 5000, 16)
+    # one tiny Adam step: the network is still random, trajectories fail
+    draws, info, acc = _run(rt, x, y, True, 13, 300, train=20,
+                            fallback_cfg=dict(epochs=1, batch=4096, lr=1e-7))
+    assert info["phase"] == 3 and info["fallbacks"] == 1, info
+    assert info["window_accept"] < 0.6
+    assert np.isfinite(draws).all()
+    assert acc[1].mean() > 0.3  # exact HMC took over
