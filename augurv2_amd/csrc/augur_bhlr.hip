@@ -59,8 +59,9 @@ struct BArgs {
   uint32_t seed;
   int prior_kind, ref_compat;
   double prior_a, prior_b;
-  double step;       // theta block: fixed step size and count
-  int steps;
+  double step;       // theta block: step size (jittered per chain and sweep
+  int steps;         // when step_jitter) and fixed count
+  int step_jitter;
   double v_sim_len, v_eps;
   int v_fixed;
 };
@@ -239,6 +240,15 @@ __device__ void bhlr_body(const BArgs& A, const BDs& ds, int di, unsigned char* 
 // One wave per value (dataset i, chain c, coordinate j; j = d_i + 1: the
 // log-density): sums the dataset's block partials in block order (fixed
 // butterfly), then applies the leapfrog update of that coordinate.
+// The theta block's step size of chain key gc this sweep: eps, or with
+// step_jitter eps U(0.8, 1.2) from the block's jitter stream, as the
+// reference draws it (cbits/augur_mcmclib.h:401-402).
+__device__ inline double theta_step(const BArgs& A, uint32_t gc) {
+  if (!A.step_jitter) return A.step;
+  const Stream js{A.seed, gc, *A.sweep, TAG_HMC_BASE + 4u + 1u};
+  return (0.8 + 0.4 * u01(js.draw(0, 0).x)) * A.step;
+}
+
 __global__ __launch_bounds__(256) void k_bhlr_finish(BArgs A) {
   const int lane = threadIdx.x & 63;
   const int v = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -271,11 +281,12 @@ __global__ __launch_bounds__(256) void k_bhlr_finish(BArgs A) {
   const double var = A.state[si + D + 1];
   double xv = A.prop[si + j];
   const double g = s + (-xv / var);
-  const double coef = (A.l == L) ? 0.5 * A.step : A.step;
+  const double step = theta_step(A, (uint32_t)(di * A.C + c));
+  const double coef = (A.l == L) ? 0.5 * step : step;
   const double pj = A.p[gi] + coef * g;
   A.p[gi] = pj;
   if (A.l < L) {
-    xv += A.step * pj;
+    xv += step * pj;
     A.prop[si + j] = xv;
     A.beta32[gi] = (float)xv;
   }
@@ -312,6 +323,7 @@ __global__ __launch_bounds__(64) void k_bhmc_begin(BArgs A) {
   const Stream mom{A.seed, gc, *A.sweep, TAG_HMC_BASE + 4u + 0u};
   const size_t si = (size_t)c * A.SW + ds.soff, gi = (size_t)c * A.GW + ds.goff;
   const double v = A.state[si + D + 1];
+  const double step = theta_step(A, gc);
   double kin = 0.0;
   for (int j = lane; j < D + 2; j += 64) {
     const double sv = A.state[si + j];
@@ -319,11 +331,11 @@ __global__ __launch_bounds__(64) void k_bhmc_begin(BArgs A) {
     if (j < D) {
       const double p0 = normal_at(mom, (uint32_t)j, 0);
       const double g = A.cache_g[gi + j] + (-sv / v);
-      const double pj = p0 + 0.5 * A.step * g;
+      const double pj = p0 + 0.5 * step * g;
       A.p0[gi + j] = p0;
       A.p[gi + j] = pj;
       kin += p0 * p0;
-      xv = sv + A.step * pj;
+      xv = sv + step * pj;
       A.beta32[gi + j] = (float)xv;
     }
     A.prop[si + j] = xv;
@@ -660,9 +672,10 @@ augur_status augur_bhlr_create(const augur_bhlr_desc* D, augur_sampler** out) {
     if (D->n_datasets < 1 || !D->n_rows || !D->n_features || !D->x || !D->y)
       fail(AUGUR_ERR_ARG, "need n_datasets >= 1 and per-dataset arrays");
     if (D->chains < 1 || D->chains > kMaxChains) fail(AUGUR_ERR_ARG, "chains must be in [1, %d]", kMaxChains);
-    if (D->theta_block.fixed_steps < 1 || D->theta_block.fixed_steps > 100000 || !(D->theta_block.step_size > 0.0))
-      fail(AUGUR_ERR_ARG, "the batched theta block needs fixed_steps >= 1 and step_size > 0");
-    if (!(D->v_block.step_size > 0.0) || (D->v_block.fixed_steps <= 0 && !(D->v_block.sim_len > 0.0)))
+    if (D->theta_block.fixed_steps == 0 || D->theta_block.fixed_steps > 100000 ||
+        D->theta_block.fixed_steps < -100000 || !(D->theta_block.step_size > 0.0))
+      fail(AUGUR_ERR_ARG, "the batched theta block needs fixed_steps != 0 and step_size > 0");
+    if (!(D->v_block.step_size > 0.0) || (D->v_block.fixed_steps == 0 && !(D->v_block.sim_len > 0.0)))
       fail(AUGUR_ERR_ARG, "bad v block settings");
     if (D->prior_kind != 0 && D->prior_kind != 1) fail(AUGUR_ERR_ARG, "prior_kind must be 0 or 1");
     for (int i = 0; i < D->n_datasets; ++i) {
@@ -738,7 +751,8 @@ augur_status augur_bhlr_create(const augur_bhlr_desc* D, augur_sampler** out) {
       A.stats = s->stats_buf.p; A.sweep = s->sweep_dev.p; A.seed = s->seed;
       A.prior_kind = D->prior_kind; A.ref_compat = D->ref_compat;
       A.prior_a = D->prior_a; A.prior_b = D->prior_b;
-      A.step = D->theta_block.step_size; A.steps = D->theta_block.fixed_steps;
+      A.step = D->theta_block.step_size; A.steps = std::abs(D->theta_block.fixed_steps);
+      A.step_jitter = D->theta_block.fixed_steps < 0;
       A.v_sim_len = D->v_block.sim_len; A.v_eps = D->v_block.step_size; A.v_fixed = D->v_block.fixed_steps;
       s->pass = pick_bpass(s->next_cp());
       // state starts at theta = 0, v = 1

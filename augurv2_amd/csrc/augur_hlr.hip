@@ -638,8 +638,10 @@ struct HlrSampler final : augur_sampler {
   int surr_sweeps_collected = 0, n_pairs = 0, max_pairs = 0, surr_iters = 0, fallbacks = 0;
   double surr_loss = 0.0, surr_train_ms = 0.0, window_accept = -1.0, win_acc0 = 0.0, win_prop0 = 0.0;
   Mlp mlp;
-  DevBuf<float> cap_x, cap_y, sstats, xb, tb;
-  DevBuf<uint16_t> w1t, w2t, w3t;
+  DevBuf<float> cap_x, cap_y, sstats, sstats2, xb, tb;
+  DevBuf<double> lin_b;
+  SurrInfer inf;
+  double resid_frac = -1.0;  // residual variance of the linear term / target variance
   DevBuf<double> ll64_cur, ll64_prop, ll64_part;
 
   const char* family() const override { return "hlr"; }
@@ -703,17 +705,21 @@ struct HlrSampler final : augur_sampler {
     return Ll64Args{x.p, y.p, n, d, cp, has_b, d + 2, point, ll64_part.p, G, out};
   }
 
-  // Fits the MLP to the captured (point, data gradient) pairs, then switches
-  // the block to surrogate trajectories.
+  // Fits the surrogate to the captured pairs (standardisation, least-squares
+  // linear term, MLP on the linear term's residual), folds it into the
+  // inference form, then switches the block to surrogate trajectories.
   void train_surrogate() {
     AUGUR_HIP(hipStreamSynchronize(stream));
     const auto t0 = std::chrono::steady_clock::now();
     mlp_colstats(cap_x.p, cap_y.p, n_pairs, d, sstats.p, stream);
+    surr_standardise(cap_x.p, cap_y.p, n_pairs, d, sstats.p, stream);
+    surr_linear_fit(cap_x.p, cap_y.p, n_pairs, d, 1e-6, lin_b.p, stream);
+    mlp_colstats(cap_x.p, cap_y.p, n_pairs, d, sstats2.p, stream);
     const long steps = std::max<long>(1, (long)scfg.epochs * n_pairs / scfg.batch);
     double lsum = 0.0;
     int lcount = 0;
     for (long s = 0; s < steps; ++s) {
-      mlp_gather(cap_x.p, cap_y.p, n_pairs, d, sstats.p, scfg.batch, scfg.seed, (uint32_t)s, xb.p, tb.p, stream);
+      mlp_gather(cap_x.p, cap_y.p, n_pairs, d, sstats2.p, scfg.batch, scfg.seed, (uint32_t)s, xb.p, tb.p, stream);
       mlp.loss_grad(xb.p, tb.p, scfg.batch, stream);
       if (s >= steps - 20) {
         double l = 0.0;
@@ -725,11 +731,16 @@ struct HlrSampler final : augur_sampler {
       mlp.adam(scfg.lr, stream);
     }
     surr_loss = lcount ? lsum / lcount : 0.0;
-    make_inference_weights(mlp.params.p, d, scfg.hidden, w1t.p, w2t.p, w3t.p, stream);
+    surr_fold(d, scfg.hidden, sstats.p, sstats2.p, lin_b.p, mlp.params.p, inf, stream);
     launch_ll64(ll64_args(state.p, ll64_cur.p), stream);
     std::vector<double> st((size_t)sched.size() * chains * 2);
     stats_buf.download(st.data(), st.size(), stream);
+    std::vector<float> s2(4 * (size_t)d);
+    sstats2.download(s2.data(), s2.size(), stream);
     AUGUR_HIP(hipStreamSynchronize(stream));
+    double rv = 0.0;
+    for (int j = 0; j < d; ++j) rv += 1.0 / ((double)s2[3 * d + j] * s2[3 * d + j]);
+    resid_frac = rv / d;
     surr_train_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     win_acc0 = win_prop0 = 0.0;
     for (int c = 0; c < chains; ++c) {
@@ -739,14 +750,24 @@ struct HlrSampler final : augur_sampler {
     surr_phase = SURR_ACTIVE;
   }
 
-  void surrogate_iteration(int bi) {
+  void surrogate_iteration(int bi, const augur_hmc_block& blk) {
     SurrTrajArgs S{};
     S.d = d; S.h = scfg.hidden; S.P = d + 2; S.ds = d + 1; S.c_real = chains;
-    S.step = scfg.step_size; S.steps = scfg.steps;
     S.seed = seed; S.sweep = sweep; S.tag = TAG_HMC_BASE + 4u * (uint32_t)bi;
+    // the block's own (jittered) schedule, or the configured fixed one
+    for (int c = 0; c < chains; ++c) {
+      if (scfg.steps > 0) {
+        S.hs.step[c] = scfg.step_size;
+        S.hs.steps[c] = scfg.steps;
+      } else {
+        hmc_jitter(Stream{seed, (uint32_t)c, sweep, S.tag + 1}, blk.sim_len, blk.step_size, blk.fixed_steps,
+                   &S.hs.step[c], &S.hs.steps[c]);
+      }
+    }
     const MlpShape sh{d, scfg.hidden};
-    S.w1t = w1t.p; S.b1 = mlp.params.p + sh.b1(); S.w2t = w2t.p; S.b2 = mlp.params.p + sh.b2();
-    S.w3t = w3t.p; S.b3 = mlp.params.p + sh.b3(); S.stats = sstats.p;
+    S.w1t = inf.w1t.p; S.b1 = mlp.params.p + sh.b1(); S.w2t = inf.w2t.p; S.b2 = mlp.params.p + sh.b2();
+    S.w3t = inf.w3t.p; S.b3 = inf.b3.p;
+    S.in_mean = inf.in_mean.p; S.in_scale = inf.in_scale.p; S.lin_mean = inf.lin_mean.p; S.lin_t = inf.lin_t.p;
     S.state = state.p; S.prop = prop.p; S.p = p.p; S.k0 = k0.p; S.beta32 = beta32.p;
     launch_surr_traj(S, chains, stream);
     launch_ll64(ll64_args(prop.p, ll64_prop.p), stream);
@@ -775,7 +796,7 @@ struct HlrSampler final : augur_sampler {
   void data_block(int bi, const augur_hmc_block& blk) {
     const bool surr_blk = surr_phase != SURR_OFF && bi == scfg.block_index;
     if (surr_blk && surr_phase == SURR_ACTIVE) {
-      surrogate_iteration(bi);
+      surrogate_iteration(bi, blk);
       return;
     }
     if (!cache_valid) refresh_cache();
@@ -864,7 +885,10 @@ struct HlrSampler final : augur_sampler {
     n = n_; d = d_; chains = chains_;
     cp = next_pow2(chains);
     const int tiles = (n + kTileRows - 1) / kTileRows;
+    // about 256 rows per block, between one and four blocks per CU (measured
+    // at N = 100k, D = 64: 256 blocks 19.6 us, 512 16.0 us, 1024 18.8 us)
     int want = 256;
+    while (want < 1024 && (size_t)want * 256 < (size_t)n) want *= 2;
     if (const char* e = getenv("AUGUR_HLR_BLOCKS")) want = std::max(1, atoi(e));
     G = std::max(1, std::min(tiles, want));
     const char* gen = getenv("AUGUR_HLR_GENERIC");
@@ -911,9 +935,10 @@ augur_status augur_hlr_create(const augur_hlr_desc* D, augur_sampler** out) {
       if (b.block < 0 || b.block > 3) fail(AUGUR_ERR_ARG, "bad HMC block kind %d", b.block);
       if ((b.block == AUGUR_HMC_BLOCK_B || b.block == AUGUR_HMC_BLOCK_B_THETA) && !D->has_intercept)
         fail(AUGUR_ERR_ARG, "HMC block over b without an intercept");
-      if (!(b.step_size > 0.0) || (b.fixed_steps <= 0 && !(b.sim_len > 0.0)) || b.fixed_steps > 100000)
+      if (!(b.step_size > 0.0) || (b.fixed_steps == 0 && !(b.sim_len > 0.0)) || b.fixed_steps > 100000 ||
+          b.fixed_steps < -100000)
         fail(AUGUR_ERR_ARG, "HMC block %d needs step_size > 0 and sim_len > 0 or fixed_steps", i);
-      if (b.fixed_steps <= 0 && b.sim_len / (0.8 * b.step_size) > 1.0e5)
+      if (b.fixed_steps == 0 && b.sim_len / (0.8 * b.step_size) > 1.0e5)
         fail(AUGUR_ERR_ARG, "HMC block %d: sim_len / step_size too large", i);
     }
     for (int i = 0; i < D->n_rows; ++i)
@@ -971,6 +996,33 @@ augur_status augur_op_hlr_ll_grad(int n_rows, int d, int chains, const float* x,
   });
 }
 
+augur_status augur_op_hlr_ll64(int n_rows, int d, int chains, const float* x, const int32_t* y,
+                               const double* beta, int has_intercept, double* ll) {
+  return guard([&] {
+    validate_shapes(n_rows, d, chains);
+    if (!x || !y || !beta || !ll) fail(AUGUR_ERR_ARG, "null argument");
+    require_device();
+    HlrSampler s;
+    AUGUR_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+    s.has_b = has_intercept;
+    s.alloc(n_rows, d, chains);
+    s.x.upload(x, (size_t)n_rows * d, s.stream);
+    s.y.upload(y, n_rows, s.stream);
+    const int P = d + 2;
+    std::vector<double> st((size_t)s.cp * P, 0.0);
+    for (int c = 0; c < chains; ++c)
+      for (int j = 0; j <= d; ++j) st[(size_t)c * P + j] = (j < d || has_intercept) ? beta[(size_t)c * (d + 1) + j] : 0.0;
+    s.state.upload(st.data(), st.size(), s.stream);
+    s.ll64_part.alloc((size_t)s.cp * s.G);
+    s.ll64_cur.alloc(s.cp);
+    launch_ll64(s.ll64_args(s.state.p, s.ll64_cur.p), s.stream);
+    std::vector<double> l(s.cp);
+    s.ll64_cur.download(l.data(), l.size(), s.stream);
+    AUGUR_HIP(hipStreamSynchronize(s.stream));
+    for (int c = 0; c < chains; ++c) ll[c] = l[c];
+  });
+}
+
 augur_status augur_hlr_surrogate(augur_sampler* s, const augur_surrogate_cfg* cfg) {
   return guard([&] {
     auto* h = dynamic_cast<HlrSampler*>(s);
@@ -982,7 +1034,7 @@ augur_status augur_hlr_surrogate(augur_sampler* s, const augur_surrogate_cfg* cf
     if (h->has_b) fail(AUGUR_ERR_UNSUPPORTED, "the surrogate covers models without an intercept");
     if (h->d > 256 || c.hidden < 1 || c.hidden > 256) fail(AUGUR_ERR_ARG, "need d <= 256 and 1 <= hidden <= 256");
     if (c.train_sweeps < 1 || c.epochs < 1 || c.batch < 1 || c.batch > 65536 || !(c.lr > 0.0) ||
-        c.window < 1 || c.steps < 1 || !(c.step_size > 0.0) || c.max_pairs < h->chains)
+        c.window < 1 || c.steps < 0 || (c.steps > 0 && !(c.step_size > 0.0)) || c.max_pairs < h->chains)
       fail(AUGUR_ERR_ARG, "bad surrogate settings");
     if (h->surr_phase != HlrSampler::SURR_OFF) fail(AUGUR_ERR_STATE, "surrogate already configured");
     h->scfg = c;
@@ -990,11 +1042,11 @@ augur_status augur_hlr_surrogate(augur_sampler* s, const augur_surrogate_cfg* cf
     h->cap_x.alloc((size_t)c.max_pairs * h->d);
     h->cap_y.alloc((size_t)c.max_pairs * h->d);
     h->sstats.alloc(4 * (size_t)h->d);
+    h->sstats2.alloc(4 * (size_t)h->d);
+    h->lin_b.alloc((size_t)h->d * h->d);
     h->xb.alloc((size_t)c.batch * h->d);
     h->tb.alloc((size_t)c.batch * h->d);
-    h->w1t.alloc((size_t)h->d * c.hidden);
-    h->w2t.alloc((size_t)c.hidden * c.hidden);
-    h->w3t.alloc((size_t)c.hidden * h->d);
+    h->inf.alloc(h->d, c.hidden);
     h->ll64_cur.alloc(h->cp);
     h->ll64_prop.alloc(h->cp);
     h->ll64_part.alloc((size_t)h->cp * h->G);
@@ -1016,6 +1068,7 @@ augur_status augur_hlr_surrogate_info(augur_sampler* s, double* info, size_t byt
     info[5] = h->surr_iters;
     info[6] = h->window_accept;
     info[7] = h->surr_sweeps_collected;
+    if (bytes >= 9 * sizeof(double)) info[8] = h->resid_frac;
   });
 }
 

@@ -23,6 +23,11 @@ __host__ __device__ inline void hmc_jitter(const Stream& s, double sim_len, doub
     return;
   }
   const u32x4 w = s.draw(0, 0);
+  if (fixed_steps < 0) {  // the reference's step jitter, a fixed step count
+    *step = (0.8 + 0.4 * u01(w.x)) * eps;
+    *steps = -fixed_steps;
+    return;
+  }
   const double st = (0.8 + 0.4 * u01(w.x)) * eps;
   const double sc = 0.9 + 0.2 * u01(w.y);
   const double r = round(sc * sim_len / st);

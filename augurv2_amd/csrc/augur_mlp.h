@@ -37,6 +37,11 @@ struct Mlp {
   void adam(double lr, hipStream_t st);
 };
 
+// C (M x N, row stride ldc) = A B with A(m, k) = A[m sam + k sak] and
+// B(k, n) = B[k sbk + n sbn]; FP32, fixed summation order.
+void gemm_f32(hipStream_t st, int M, int N, int K, const float* A, long sam, long sak, const float* B,
+              long sbk, long sbn, float* C, int ldc);
+
 // stats = [mean x | 1/sd x | mean t | 1/sd t] over n rows of x and t (n x d)
 void mlp_colstats(const float* x, const float* t, int n, int d, float* stats, hipStream_t st);
 // `rows` rows drawn uniformly from the n pairs (Philox, step), standardised

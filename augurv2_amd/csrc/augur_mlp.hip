@@ -89,6 +89,11 @@ static void gemm(hipStream_t st, int M, int N, int K, const float* A, long sam, 
   AUGUR_CHECK_LAUNCH();
 }
 
+void gemm_f32(hipStream_t st, int M, int N, int K, const float* A, long sam, long sak, const float* B,
+              long sbk, long sbn, float* C, int ldc) {
+  gemm(st, M, N, K, A, sam, sak, B, sbk, sbn, C, ldc);
+}
+
 // out[n] = sum over rows of x[m][n]
 __global__ void k_colsum(const float* __restrict__ x, int rows, int cols, float* __restrict__ out) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;

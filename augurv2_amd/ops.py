@@ -60,6 +60,17 @@ def hlr_ll_grad(x, y, beta, has_intercept=False):
     return ll, g
 
 
+def hlr_ll64(x, y, beta, has_intercept=False):
+    """The data log-density in FP64 row arithmetic.  beta: chains x (D + 1)."""
+    x, y = L.f32(x), L.i32(y)
+    n, d = x.shape
+    beta = L.f64(beta).reshape(-1, d + 1)
+    ll = np.zeros(beta.shape[0])
+    L.call("augur_op_hlr_ll64", n, d, beta.shape[0], L.ptr(x), L.ptr(y), L.ptr(beta),
+           int(has_intercept), L.ptr(ll))
+    return ll
+
+
 def gmm_zstep(y, pis, mu, cov, seed, chain, sweep):
     y = L.f32(y)
     n, d = y.shape

@@ -141,18 +141,20 @@ class HlrSampler(Sampler):
         super().__init__(h, chains, shapes)
 
     def enable_surrogate(self, block_index, hidden=256, train_sweeps=2000, max_pairs=500_000,
-                         epochs=20, batch=512, lr=1e-3, steps=30, step_size=0.005,
+                         epochs=20, batch=512, lr=1e-3, steps=0, step_size=0.0,
                          min_accept=0.6, window=50, seed=0):
-        """Surrogate-gradient HMC for block `block_index` (an HMC [theta] block)."""
+        """Surrogate-gradient HMC for block `block_index` (an HMC [theta] block).
+        steps = 0: surrogate trajectories follow the block's own (jittered)
+        schedule; steps > 0: that many leapfrog steps of step_size."""
         cfg = L.SurrogateCfg(block_index, hidden, train_sweeps, max_pairs, epochs, batch, lr, steps,
                              step_size, min_accept, window, seed & 0xFFFFFFFF)
         L.call("augur_hlr_surrogate", self._h, C.byref(cfg))
 
     def surrogate_info(self):
-        out = np.zeros(8)
+        out = np.zeros(9)
         L.call("augur_hlr_surrogate_info", self._h, L.ptr(out), out.nbytes)
         keys = ("phase", "pairs", "loss", "train_ms", "fallbacks", "surrogate_sweeps",
-                "window_accept", "collected_sweeps")
+                "window_accept", "collected_sweeps", "resid_frac")
         return dict(zip(keys, out.tolist()))
 
     def acceptance(self):

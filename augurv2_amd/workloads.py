@@ -30,7 +30,8 @@ def hlr_data(N=100_000, D=64, seed=0, dtype=np.float32):
     v = 1.0 / rng.gamma(1.0, 1.0)
     beta = rng.standard_normal(D) * np.sqrt(v)
     t = x.astype(np.float64) @ beta
-    y = (rng.random(N) < 1.0 / (1.0 + np.exp(-t))).astype(np.int32)
+    with np.errstate(over="ignore"):  # exp(-t) = inf gives probability 0, as intended
+        y = (rng.random(N) < 1.0 / (1.0 + np.exp(-t))).astype(np.int32)
     return x, y, v, beta
 
 
@@ -45,6 +46,21 @@ def hlr_init(D, chains, seed=1):
         theta = rng.standard_normal(D) * 0.1
         out.append(np.concatenate([theta, [0.0], [v]]))
     return out
+
+
+def batched_hlr_data(n_datasets=256, n_min=1000, n_max=200_000, dims=(16, 32, 64), seed=0):
+    """Datasets for the batched HLR sampler: N_i ~ U{n_min..n_max}, D_i
+    uniform over dims, each generated as hlr_data with seed 1000 + i.
+    Returns (xs, ys)."""
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(n_min, n_max + 1, n_datasets)
+    ds = rng.choice(np.asarray(dims), n_datasets)
+    xs, ys = [], []
+    for i in range(n_datasets):
+        x, y, _, _ = hlr_data(int(ns[i]), int(ds[i]), seed=1000 + i)
+        xs.append(x)
+        ys.append(y)
+    return xs, ys
 
 
 def gmm_data(N=1_000_000, D=16, K=32, seed=0):

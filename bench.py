@@ -29,7 +29,12 @@ def run_hlr(steps, warmup, seed=2024):
     from augurv2_amd._lib import HMC_BLOCK_THETA, HMC_BLOCK_V
     N, D, C, L, EPS = 100_000, 64, 4, 20, 0.005
     x, y, v_true, beta_true = workloads.hlr_data(N, D, seed=0)
-    blocks = [(HMC_BLOCK_V, 1.0, 0.05, 0), (HMC_BLOCK_THETA, 0.1, EPS, L)]
+    # the reference's HMC (cbits/augur_mcmclib.h:401-405): step jittered in
+    # [0.8, 1.2] eps, round(U[0.9, 1.1] * sim_len / step) steps, 20 of 0.005
+    # nominal.  Without the jitter the trajectory length 0.1 is a full period
+    # (omega T = 2 pi) of the posterior's directions with curvature ~3900 and
+    # those directions barely move.
+    blocks = [(HMC_BLOCK_V, 1.0, 0.05, 0), (HMC_BLOCK_THETA, L * EPS, EPS, 0)]
     # textbook Normal density: with the reference's normal_dotarg2 (-0.5 * var,
     # cbits/augur_dist.c:393) the HMC force on v is wrong by ~50 at the
     # posterior mode and the v block accepts almost nothing
@@ -111,7 +116,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -144,7 +149,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "HLR HMC: N=100000 rows, D=64, 4 chains, HMC [v] (*) HMC [beta] "
-                               "with 20 leapfrog steps of 0.005, FP32 X",
+                               "with 20 leapfrog steps of 0.005 (jittered as the reference), FP32 X",
                    "global_batch": N, "chains": 4, "parallelism": "single GPU"},
         "rhat_max": hlr["rhat_max"],
         "min_ess": hlr["min_ess"],

@@ -103,10 +103,14 @@ augur_status augur_hlr_create(const augur_hlr_desc* desc, augur_sampler** out);
  * HMC [theta] block of an HLR sampler without intercept:
  *   1. the first train_sweeps sweeps run exact HMC and record every leapfrog
  *      point with its data gradient (up to max_pairs rows);
- *   2. an MLP d -> hidden -> hidden -> d (GELU) is then fitted to them on the
- *      device (Adam, epochs passes of minibatches of `batch`, rate lr);
- *   3. later sweeps run `steps` leapfrog steps of step_size with the
- *      network's data gradient plus the exact prior gradient;
+ *   2. the data gradient is then fitted on the device: a least-squares
+ *      linear map of theta (FP64 normal equations) plus an MLP
+ *      d -> hidden -> hidden -> d (GELU) trained on the linear map's residual
+ *      (Adam, epochs passes of minibatches of `batch`, rate lr);
+ *   3. later sweeps run leapfrog trajectories with the fitted data gradient
+ *      (BF16 network weights, FP32 arithmetic) plus the exact prior
+ *      gradient: the block's own jittered schedule when steps = 0, else
+ *      `steps` steps of step_size;
  *   4. the proposal is accepted or rejected with the exact log-density (FP64
  *      row arithmetic), so the target is unchanged;
  *   5. if the acceptance rate over a window of `window` sweeps falls below
@@ -126,10 +130,11 @@ typedef struct {
   uint32_t seed;     /* network initialisation and minibatch draws */
 } augur_surrogate_cfg;
 augur_status augur_hlr_surrogate(augur_sampler* s, const augur_surrogate_cfg* cfg);
-/* info[8]: phase (0 off, 1 collecting, 2 surrogate, 3 fell back), pairs
- * collected, final training loss (standardised MSE), training time in ms,
- * fallbacks, surrogate sweeps run, last window's acceptance, collection
- * sweeps run. */
+/* info[8] (info[9] when bytes allow): phase (0 off, 1 collecting,
+ * 2 surrogate, 3 fell back), pairs collected, final training loss
+ * (standardised MSE of the residual), training time in ms, fallbacks,
+ * surrogate sweeps run, last window's acceptance, collection sweeps run,
+ * [the linear map's residual variance as a fraction of the gradient's]. */
 augur_status augur_hlr_surrogate_info(augur_sampler* s, double* info, size_t bytes);
 /* The surrogate network's loss and gradient on host data (rows x d inputs
  * and targets, parameters in the order W1, b1, W2, b2, W3, b3), and a run of
@@ -265,6 +270,11 @@ augur_status augur_op_lda_zstep(int n_docs, int k, int v, const int32_t* doc_off
 augur_status augur_op_hlr_ll_grad(int n_rows, int d, int chains, const float* x,
                                   const int32_t* y, const double* beta, int has_intercept,
                                   double* ll, double* grad);
+/* The same data log-density in FP64 row arithmetic (FP64 matrix cores for
+ * D in {16, 32, 64, 128} without intercept): the Metropolis term of
+ * surrogate HMC.  Same arguments as augur_op_hlr_ll_grad, no gradient. */
+augur_status augur_op_hlr_ll64(int n_rows, int d, int chains, const float* x, const int32_t* y,
+                               const double* beta, int has_intercept, double* ll);
 /* The GMM DiscGibbs [z] kernel for one chain given pis, mu, cov (FP64, as
  * drawn by the conjugate kernels; cbits/augur_dist.c:548-568). */
 augur_status augur_op_gmm_zstep(int n, int d, int k, const float* y, const double* pis,

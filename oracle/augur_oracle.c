@@ -562,7 +562,8 @@ static int or_hlr_block_coord(const or_hlr_model* m, int block, int i) {
 /* cbits/augur_mcmclib.h:389-424 (h_augur_mcmc_hmc) with the leapfrog of
  * :343-375 and the accept test of :70-83.  fixed_steps > 0 selects a fixed
  * step size eps and fixed_steps leapfrog steps instead of the reference's
- * jitter eps * U(0.8, 1.2), L = max(1, round(U(0.9, 1.1) simLen / eps')).
+ * jitter eps * U(0.8, 1.2), L = max(1, round(U(0.9, 1.1) simLen / eps'));
+ * fixed_steps < 0 keeps the reference's step jitter with -fixed_steps steps.
  * out_info (7): eps', L, K0, K1, ll0, ll1, accepted. */
 int or_hmc_step(const or_hlr_model* m, double* state, int block, int block_id,
                 double sim_len, double eps, int fixed_steps, uint32_t seed,
@@ -580,6 +581,11 @@ int or_hmc_step(const or_hlr_model* m, double* state, int block, int block_id,
   if (fixed_steps > 0) {
     step = eps;
     steps = fixed_steps;
+  } else if (fixed_steps < 0) {
+    uint32_t w[4];
+    or_draw4(seed, chain, 0, 0, sweep, tag + 1, w);
+    step = (0.8 + 0.4 * or_u01(w[0])) * eps;
+    steps = -fixed_steps;
   } else {
     uint32_t w[4];
     or_draw4(seed, chain, 0, 0, sweep, tag + 1, w);

@@ -44,10 +44,13 @@ def _init(s, rng, C):
     return starts
 
 
-def test_batched_matches_oracle(rt, oracle):
+@pytest.mark.parametrize("L", [8, -8])
+def test_batched_matches_oracle(rt, oracle, L):
+    """L > 0: fixed step and count; L < 0: the reference's step jitter with
+    |L| steps (every chain runs the same number of passes)."""
     rng = np.random.default_rng(0)
     xs, ys = _datasets(rng, SHAPES)
-    C, L, EPS = 2, 8, 0.01
+    C, EPS = 2, 0.01
     s = rt.BhlrSampler(xs, ys, chains=C, seed=7, v_block=(1.0, 0.05, 0), theta_block=(0.1, EPS, L),
                        rows_per_block=512, use_graph=True)
     starts = _init(s, rng, C)

@@ -135,6 +135,28 @@ def test_hlr_ll_grad_vs_fp64(ops, oracle, n, d, chains, intercept):
             assert abs(g[c, d] - gb_ref) <= 1e-5 * a.sum() + 1e-9
 
 
+@pytest.mark.parametrize("n,d,chains,intercept", [(5000, 16, 4, False), (4099, 64, 4, False),
+                                                  (1000, 128, 8, False), (2003, 128, 16, False),
+                                                  (15, 32, 2, False), (777, 5, 3, True),
+                                                  (300, 64, 4, True)])
+def test_hlr_ll64_vs_fp64(ops, oracle, n, d, chains, intercept):
+    """FP64 row arithmetic (matrix-core form for D in {16, 32, 64, 128}
+    without intercept, including row counts that leave a partial 16-row
+    tile): the oracle's FP64 log-density to 1e-11 relative."""
+    rng = np.random.default_rng(7 * n + d)
+    x = rng.standard_normal((n, d)).astype(np.float32)
+    beta_true = rng.standard_normal(d) * 0.3
+    y = (rng.random(n) < 1 / (1 + np.exp(-(x @ beta_true)))).astype(np.int32)
+    beta = rng.standard_normal((chains, d + 1)) * 0.3
+    if not intercept:
+        beta[:, d] = 0.0
+    ll = ops.hlr_ll64(x, y, beta, intercept)
+    m = oracle.Hlr(x.astype(np.float64), y, has_intercept=intercept)
+    for c in range(chains):
+        ll_ref = m.data_ll(np.concatenate([beta[c], [1.0]]))
+        assert abs(ll[c] - ll_ref) <= 1e-11 * abs(ll_ref), (c, ll[c], ll_ref)
+
+
 def _gmm_params(rng, k, d):
     pis = rng.dirichlet(np.ones(k))
     mu = rng.standard_normal((k, d)) * 3

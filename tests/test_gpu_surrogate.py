@@ -85,17 +85,20 @@ def _hlr_data(rng, n, d):
     return x, y
 
 
-def _run(rt, x, y, surrogate, seed, n_sweeps, train=300, fallback_cfg=None):
+def _run(rt, x, y, surrogate, seed, n_sweeps, train=300, hidden=64, fallback_cfg=None):
     from augurv2_amd._lib import HMC_BLOCK_THETA, HMC_BLOCK_V
-    C, L, EPS = 4, 10, 0.02
+    C, SIM, EPS = 4, 0.15, 0.015  # about 10 jittered leapfrog steps
     d = x.shape[1]
-    s = rt.HlrSampler(x, y, [(HMC_BLOCK_V, 1.0, 0.05, 0), (HMC_BLOCK_THETA, 0.2, EPS, L)],
+    s = rt.HlrSampler(x, y, [(HMC_BLOCK_V, 1.0, 0.05, 0), (HMC_BLOCK_THETA, SIM, EPS, 0)],
                       chains=C, seed=seed, ref_compat=False)
     for c in range(C):
         s.set("state", np.concatenate([np.zeros(d), [0.0], [1.0]]), chain=c)
+    # exact warmup first: training pairs must come from the typical set, not
+    # from the burn-in path, whose gradients are orders of magnitude larger
+    s.step(200)
     if surrogate:
-        kw = dict(hidden=64, train_sweeps=train, max_pairs=200_000, epochs=40, batch=256,
-                  lr=2e-3, steps=L, step_size=EPS, min_accept=0.6, window=50, seed=3)
+        kw = dict(hidden=hidden, train_sweeps=train, max_pairs=200_000, epochs=40, batch=256,
+                  lr=2e-3, min_accept=0.6, window=50, seed=3)
         kw.update(fallback_cfg or {})
         s.enable_surrogate(1, **kw)
         s.step(train)  # collection with exact HMC, then training
@@ -109,37 +112,55 @@ def _run(rt, x, y, surrogate, seed, n_sweeps, train=300, fallback_cfg=None):
     return draws, info, acc
 
 
-def test_surrogate_hmc_is_exact_and_pays_off(gpu):
-    _, rt = gpu
-    rng = np.random.default_rng(2)
-    x, y = _hlr_data(rng, 20000, 16)
-    d = x.shape[1]
-    n = 2000
-    exact, _, acc_e = _run(rt, x, y, False, 11, n)
-    surr, info, acc_s = _run(rt, x, y, True, 12, n)
-    assert info["phase"] == 2 and info["fallbacks"] == 0, info
-    assert info["window_accept"] >= 0.6, info
+def _compare(exact, surr, d, n):
+    """Posterior means of surrogate HMC against exact HMC: each within 4
+    MCSE (family-wise over the d + 1 parameters) and at most 5% (or one)
+    beyond 3 MCSE; returns the minimum bulk ESS of each run."""
     keep = list(range(d)) + [d + 1]
+    beyond3 = 0
     for j in keep:
         a, b = exact[:, :, j].T, surr[:, :, j].T
         assert diagnostics.rhat(b) < 1.05
         se = np.hypot(diagnostics.mcse_mean(a), diagnostics.mcse_mean(b))
-        assert abs(a.mean() - b.mean()) <= 3 * se, (j, a.mean(), b.mean(), se)
-    # the gain: effective draws per exact data pass.  Exact HMC spends L = 10
-    # passes per sweep; surrogate HMC spends one (the log-density of the
-    # proposal) per sweep.
+        z = abs(a.mean() - b.mean()) / se
+        assert z <= 4.0, (j, a.mean(), b.mean(), se)
+        beyond3 += z > 3.0
+    assert beyond3 <= max(1, 0.05 * len(keep)), beyond3
     ess_e = min(diagnostics.ess_bulk(exact[:, :, j].T) for j in keep)
     ess_s = min(diagnostics.ess_bulk(surr[:, :, j].T) for j in keep)
-    assert (ess_s / (n * 1)) >= 3.0 * (ess_e / (n * 10)), (ess_s, ess_e)
+    return ess_e, ess_s
 
 
-def test_untrained_surrogate_falls_back_to_exact_hmc(gpu):
+@pytest.mark.parametrize("d,hidden", [(16, 64), (64, 128)])
+def test_surrogate_hmc_is_exact_and_pays_off(gpu, d, hidden):
+    """d = 16: the generic trajectory kernel and FP64 log-density; d = 64,
+    hidden = 128: the register-resident BF16-dot trajectory kernel and the
+    FP64 matrix-core log-density."""
+    _, rt = gpu
+    rng = np.random.default_rng(2)
+    x, y = _hlr_data(rng, 20000, d)
+    n = 2000
+    exact, _, acc_e = _run(rt, x, y, False, 11, n)
+    surr, info, acc_s = _run(rt, x, y, True, 12, n, hidden=hidden)
+    assert info["phase"] == 2 and info["fallbacks"] == 0, (info, acc_e, acc_s)
+    assert info["window_accept"] >= 0.6, info
+    # near the posterior the logistic gradient is close to linear in theta
+    assert 0.0 < info["resid_frac"] < 0.2, info
+    ess_e, ess_s = _compare(exact, surr, d, n)
+    # the gain: effective draws per exact data pass.  Exact HMC spends about
+    # 10 passes per sweep (the jittered leapfrog count); surrogate HMC one
+    # (the log-density of the proposal).
+    assert ess_s >= 3.0 * ess_e / 10, (ess_s, ess_e, info)
+
+
+def test_failing_surrogate_trajectories_fall_back_to_exact_hmc(gpu):
     _, rt = gpu
     rng = np.random.default_rng(3)
     x, y = _hlr_data(rng, 5000, 16)
-    # one tiny Adam step: the network is still random, trajectories fail
+    # fixed surrogate steps of 0.2: eps * sqrt(largest curvature ~ 750) ~ 5.5,
+    # far past the leapfrog's stability limit of 2, so proposals are rejected
     draws, info, acc = _run(rt, x, y, True, 13, 300, train=20,
-                            fallback_cfg=dict(epochs=1, batch=4096, lr=1e-7))
+                            fallback_cfg=dict(epochs=1, steps=20, step_size=0.2))
     assert info["phase"] == 3 and info["fallbacks"] == 1, info
     assert info["window_accept"] < 0.6
     assert np.isfinite(draws).all()

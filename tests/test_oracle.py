@@ -224,6 +224,35 @@ def test_hmc_exact_gaussian_target(oracle):
     assert abs(draws.var() / v - 1) < 0.1
 
 
+def test_hmc_step_jitter_modes(oracle):
+    """fixed_steps < 0 draws the same jittered step as the reference's jitter
+    (fixed_steps = 0) at the same counters and keeps |fixed_steps| steps; the
+    step stays in [0.8, 1.2] eps; fixed_steps > 0 leaves it at eps."""
+    m = oracle.Hlr(np.zeros((0, 3)), np.zeros(0, np.int32))
+    for sweep in range(20):
+        st = np.array([0.3, -0.2, 0.1, 0.0, 1.0])
+        ref = m.hmc_step(st.copy(), 2, 1, 0.1, 0.01, 0, 5, 2, sweep)
+        neg = m.hmc_step(st.copy(), 2, 1, 0.1, 0.01, -7, 5, 2, sweep)
+        fix = m.hmc_step(st.copy(), 2, 1, 0.1, 0.01, 7, 5, 2, sweep)
+        assert neg[0] == ref[0] and 0.008 <= neg[0] <= 0.012 and neg[1] == 7
+        assert fix[0] == 0.01 and fix[1] == 7
+        # the reference's count: round(U(0.9, 1.1) * 0.1 / eps') lies in [7, 14]
+        assert 7 <= ref[1] <= 14
+
+
+def test_hmc_gaussian_target_with_step_jitter(oracle):
+    """The step-jittered fixed-count mode targets the same distribution."""
+    m = oracle.Hlr(np.zeros((0, 3)), np.zeros(0, np.int32))
+    v = 2.0
+    st = np.array([3.0, -2.0, 0.5, 0.0, v])
+    draws = _hmc_chain(oracle, m, st, 2, 6000, 0.0, 0.2, -7)[1000:, :3]
+    from augurv2_amd import diagnostics
+    for j in range(3):
+        xj = draws[:, j][None, :]
+        assert abs(xj.mean()) < 4 * diagnostics.mcse_mean(xj)
+    assert abs(draws.var() / v - 1) < 0.1
+
+
 @pytest.mark.parametrize("ref_compat,shape_add", [(False, 0.5), (True, 1.0)])
 def test_hmc_variance_block_target(oracle, ref_compat, shape_add):
     """v | theta ~ InvGamma(a + s K, b + sum theta^2 / 2) with s = 1/2 for the
