@@ -37,6 +37,7 @@ namespace augur {
 constexpr int kTileRows = 64;
 constexpr int kPassThreads = 256;
 constexpr int kMaxGradSlots = 16;  // (d) * cp <= 16 * 256
+constexpr int kPassBlocksPerCU = 3;  // register budget of the specialised pass (<= 168 VGPRs)
 
 enum { MODE_CACHE = 0, MODE_LEAPFROG = 1 };
 
@@ -66,23 +67,46 @@ struct PassArgs {
 };
 
 
-// Reduction of the block partials and the leapfrog epilogue, one wave per
-// value (chain c, coordinate j; j = d: intercept, j = d + 1: log-density).
-// Lane l sums blocks l, l + 64, ... in order, then a fixed butterfly: the
-// result is deterministic.  The update of coordinate j needs only value j,
-// so every wave finishes independently.  (A separate kernel instead of a
-// last-block epilogue: the device-scope fence each block would need writes
-// back the L2 and serialises across blocks.)
-__global__ __launch_bounds__(256) void k_hlr_finish(PassArgs A) {
+// Reduction of the block partials and the leapfrog epilogue for values
+// (chain c, coordinate j; j = d: intercept, j = d + 1: log-density); the
+// partials are part[block][value].  A 1024-thread block takes kFinVals
+// consecutive values: thread (group g, value i) sums rows g, g + 32, ... of
+// its column (each group reads contiguous row segments), then thread i adds
+// the 32 group sums in order: the result is deterministic.  The update of
+// coordinate j needs only value j, so every value finishes independently.
+// (A separate kernel instead of a last-block epilogue: the device-scope
+// fence each pass block would need writes back the L2 and serialises across
+// blocks.)
+constexpr int kFinVals = 16;
+constexpr int kFinGroups = 64;
+
+__global__ __launch_bounds__(1024) void k_hlr_finish(PassArgs A) {
+  __shared__ double red[kFinGroups][kFinVals + 1];
   const int d = A.d, ds = d + 1, vs = d + 2, P = d + 2;
-  const int lane = threadIdx.x & 63;
-  const int v = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (v >= A.cp * vs) return;
-  const double* pv = A.part + (size_t)v * A.G;
+  const int nval = A.cp * vs;
+  const int vi = threadIdx.x % kFinVals, grp = threadIdx.x / kFinVals;
+  const int v0 = blockIdx.x * kFinVals;
+  double acc = 0.0;
+  if (v0 + vi < nval) {
+    // eight independent loads in flight per round, summed in a fixed order
+    const double* col = A.part + v0 + vi;
+    for (int g0 = grp; g0 < A.G; g0 += 8 * kFinGroups) {
+      double t[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int g = g0 + k * kFinGroups;
+        t[k] = g < A.G ? col[(size_t)g * nval] : 0.0;
+      }
+      acc += ((t[0] + t[1]) + (t[2] + t[3])) + ((t[4] + t[5]) + (t[6] + t[7]));
+    }
+  }
+  red[grp][vi] = acc;
+  __syncthreads();
+  if (threadIdx.x >= kFinVals) return;
+  const int v = v0 + threadIdx.x;
+  if (v >= nval) return;
   double s = 0.0;
-  for (int g = lane; g < A.G; g += 64) s += pv[g];
-  s = wave_sum(s);
-  if (lane != 0) return;
+  for (int k = 0; k < kFinGroups; ++k) s += red[k][threadIdx.x];
   const int c = v / vs, j = v % vs;
   if (A.mode == MODE_CACHE) {
     if (j <= d) A.cache_g[c * ds + j] = s;
@@ -211,15 +235,16 @@ __global__ __launch_bounds__(kPassThreads) void k_hlr_pass(PassArgs A) {
       }
     }
   }
-  // block partials -> part[value][block], value = c * (d + 2) + j
+  // block partials -> part[block][value], value = c * (d + 2) + j
   const int vs = d + 2;
+  double* pb = A.part + (size_t)blockIdx.x * cp * vs;
 #pragma unroll
   for (int m = 0; m < kMaxGradSlots; ++m) {
     if (m >= ng) break;
     const int q = tid + kPassThreads * m;
     if (q < ngc) {
       const int c = q / d, j = q % d;
-      A.part[(size_t)(c * vs + j) * G + blockIdx.x] = gacc[m];
+      pb[c * vs + j] = gacc[m];
     }
   }
   __syncthreads();
@@ -232,10 +257,11 @@ __global__ __launch_bounds__(kPassThreads) void k_hlr_pass(PassArgs A) {
       sb += red[i];
       sl += red[kPassThreads + i];
     }
-    A.part[(size_t)(tid * vs + d) * G + blockIdx.x] = sb;
-    A.part[(size_t)(tid * vs + d + 1) * G + blockIdx.x] = sl;
+    pb[tid * vs + d] = sb;
+    pb[tid * vs + d + 1] = sl;
   }
   (void)my_c;
+  (void)G;
   (void)flag;
 }
 
@@ -249,7 +275,7 @@ __global__ __launch_bounds__(kPassThreads) void k_hlr_pass(PassArgs A) {
 //     the tile's rows (a 128-bit x read and one broadcast a read feed 4 FMAs);
 //     the row slices are combined once per block, in a fixed order.
 template <int D, int CP>
-__global__ __launch_bounds__(kPassThreads) void k_hlr_pass_t(PassArgs A) {
+__global__ __launch_bounds__(kPassThreads, kPassBlocksPerCU) void k_hlr_pass_t(PassArgs A) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int DS = D + 4;
   constexpr int TR = kTileRows;
@@ -260,6 +286,7 @@ __global__ __launch_bounds__(kPassThreads) void k_hlr_pass_t(PassArgs A) {
   constexpr int RS = NP >= kPassThreads ? 1 : kPassThreads / NP;  // row slices
   constexpr int NS = NP > kPassThreads ? NP / kPassThreads : 1;  // pairs per thread
   constexpr bool BREG = D <= 64;
+  constexpr int PD = D <= 64 ? 3 : 2;                          // tiles in flight (pass_ring)
   static_assert(D % 4 == 0 && (CP & (CP - 1)) == 0 && CP <= kMaxChains, "shape");
   static_assert(TR % RS == 0, "row slices");
 
@@ -293,35 +320,45 @@ __global__ __launch_bounds__(kPassThreads) void k_hlr_pass_t(PassArgs A) {
     for (int k = 0; k < 4; ++k) gacc[s][k] = 0.0;
   double llacc = 0.0, gbacc = 0.0;
 
-  float4 pre[NVT];
-  int ypre = 0;
-  auto prefetch = [&](int tile) {
-    const int r0 = tile * TR;
+  // a ring of PD tiles in flight: this block's tiles blockIdx.x + k G are
+  // loaded PD ahead of their use (all of them up front when the block has
+  // at most PD), so the pass waits on memory about once, not once per tile
+  float4 pre[PD][NVT];
+  int ypre[PD];
+  auto prefetch = [&](int slot_tile, float4 (&dst)[NVT], int& ydst) {
+    const int r0 = slot_tile * TR;
     const int rows = min(TR, n - r0);
     const float4* s4 = (const float4*)(A.x + (size_t)r0 * D);
 #pragma unroll
     for (int k = 0; k < NVT; ++k) {
       const int i = tid + k * kPassThreads;
-      pre[k] = (i < NV && i < rows * (D / 4)) ? s4[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+      dst[k] = (i < NV && i < rows * (D / 4)) ? s4[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     }
-    ypre = (tid < TR && tid < rows) ? A.y[r0 + tid] : 0;
+    ydst = (tid < TR && tid < rows) ? A.y[r0 + tid] : 0;
   };
+#pragma unroll
+  for (int k = 0; k < PD; ++k) {
+    const int t = blockIdx.x + k * G;
+    if (t < tiles) prefetch(t, pre[k], ypre[k]);
+  }
 
-  int tile = blockIdx.x;
-  if (tile < tiles) prefetch(tile);
-  for (; tile < tiles; tile += G) {
+  for (int base = blockIdx.x; base < tiles; base += PD * G) {
+#pragma unroll
+  for (int slot = 0; slot < PD; ++slot) {
+    const int tile = base + slot * G;
+    if (tile >= tiles) break;
     const int rows = min(TR, n - tile * TR);
 #pragma unroll
     for (int k = 0; k < NVT; ++k) {
       const int i = tid + k * kPassThreads;
       if (i < NV) {
         const int e = 4 * i;
-        *(float4*)(xs + (e / D) * DS + e % D) = pre[k];
+        *(float4*)(xs + (e / D) * DS + e % D) = pre[slot][k];
       }
     }
-    if (tid < TR) ys[tid] = ypre;
+    if (tid < TR) ys[tid] = ypre[slot];
     __syncthreads();
-    if (tile + G < tiles) prefetch(tile + G);
+    if (tile + PD * G < tiles) prefetch(tile + PD * G, pre[slot], ypre[slot]);
     // phase B: t = x . theta_c + b_c
 #pragma unroll
     for (int m = 0; m < NB; ++m) {
@@ -378,8 +415,11 @@ __global__ __launch_bounds__(kPassThreads) void k_hlr_pass_t(PassArgs A) {
     }
     __syncthreads();
   }
-  // block partials -> part[value][block], value = c * (D + 2) + j
+  }
+  // block partials -> part[block][value], value = c * (D + 2) + j: one
+  // contiguous row per block
   constexpr int VS = D + 2;
+  double* pb = A.part + (size_t)blockIdx.x * CP * VS;
   if constexpr (RS > 1) {
 #pragma unroll
     for (int k = 0; k < 4; ++k) red2[((tid / NP) * NP + tid % NP) * 4 + k] = gacc[0][k];
@@ -390,7 +430,7 @@ __global__ __launch_bounds__(kPassThreads) void k_hlr_pass_t(PassArgs A) {
       for (int k = 0; k < 4; ++k) {
         double s = 0.0;
         for (int rs = 0; rs < RS; ++rs) s += red2[(rs * NP + tid) * 4 + k];
-        A.part[(size_t)(c * VS + 4 * jq + k) * G + blockIdx.x] = s;
+        pb[c * VS + 4 * jq + k] = s;
       }
     }
   } else {
@@ -399,7 +439,7 @@ __global__ __launch_bounds__(kPassThreads) void k_hlr_pass_t(PassArgs A) {
       const int pair = tid + s * kPassThreads;
       const int c = pair / (D / 4), jq = pair % (D / 4);
 #pragma unroll
-      for (int k = 0; k < 4; ++k) A.part[(size_t)(c * VS + 4 * jq + k) * G + blockIdx.x] = gacc[s][k];
+      for (int k = 0; k < 4; ++k) pb[c * VS + 4 * jq + k] = gacc[s][k];
     }
   }
   red[tid] = gbacc;
@@ -411,8 +451,8 @@ __global__ __launch_bounds__(kPassThreads) void k_hlr_pass_t(PassArgs A) {
       sb += red[i];
       sl += red[kPassThreads + i];
     }
-    A.part[(size_t)(tid * VS + D) * G + blockIdx.x] = sb;
-    A.part[(size_t)(tid * VS + D + 1) * G + blockIdx.x] = sl;
+    pb[tid * VS + D] = sb;
+    pb[tid * VS + D + 1] = sl;
   }
   (void)flag;
 }
@@ -427,6 +467,9 @@ size_t pass_t_smem_bytes(int d, int cp) {
 }
 
 typedef void (*PassKernel)(PassArgs);
+
+// Tiles the specialised pass keeps in flight per block.
+inline int pass_ring(int d) { return d <= 64 ? 3 : 2; }
 
 // Specialised pass for (d, cp) when one is compiled, else null.
 PassKernel pick_pass(int d, int cp) {
@@ -679,7 +722,7 @@ struct HlrSampler final : augur_sampler {
     }
     AUGUR_CHECK_LAUNCH();
     prof_end();
-    k_hlr_finish<<<(cp * (d + 2) + 3) / 4, 256, 0, stream>>>(A);
+    k_hlr_finish<<<(cp * (d + 2) + kFinVals - 1) / kFinVals, 1024, 0, stream>>>(A);
     AUGUR_CHECK_LAUNCH();
   }
 
@@ -885,14 +928,23 @@ struct HlrSampler final : augur_sampler {
     n = n_; d = d_; chains = chains_;
     cp = next_pow2(chains);
     const int tiles = (n + kTileRows - 1) / kTileRows;
-    // about 256 rows per block, between one and four blocks per CU (measured
-    // at N = 100k, D = 64: 256 blocks 19.6 us, 512 16.0 us, 1024 18.8 us)
-    int want = 256;
-    while (want < 1024 && (size_t)want * 256 < (size_t)n) want *= 2;
-    if (const char* e = getenv("AUGUR_HLR_BLOCKS")) want = std::max(1, atoi(e));
-    G = std::max(1, std::min(tiles, want));
     const char* gen = getenv("AUGUR_HLR_GENERIC");
     fast_pass = (gen && atoi(gen)) ? nullptr : pick_pass(d, cp);
+    int want;
+    if (fast_pass) {
+      // every block's tiles fit in the specialised pass's prefetch ring when
+      // the grid allows (one memory round per pass), and every block is
+      // resident at once (a second round of blocks costs ~35% at N = 100k)
+      int cus = 0;
+      AUGUR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+      want = std::min(kPassBlocksPerCU * std::max(cus, 1), (tiles + pass_ring(d) - 1) / pass_ring(d));
+    } else {
+      // about 256 rows per block, between one and four blocks per CU
+      want = 256;
+      while (want < 1024 && (size_t)want * 256 < (size_t)n) want *= 2;
+    }
+    if (const char* e = getenv("AUGUR_HLR_BLOCKS")) want = std::max(1, atoi(e));
+    G = std::max(1, std::min(tiles, want));
     const size_t P = (size_t)d + 2, ds = (size_t)d + 1;
     x.alloc((size_t)n * d);
     y.alloc(std::max(n, 1));

@@ -67,6 +67,140 @@ def run_hlr(steps, warmup, seed=2024):
     }
 
 
+def _min_ess(draws, keep):
+    from augurv2_amd import diagnostics
+    summ = diagnostics.summarize(draws[:, :, keep])
+    return float(np.min(summ["ess_bulk"])), float(np.max(summ["rhat"]))
+
+
+def run_surrogate(iters=5000, warmup=1000, train=2000, seed=77):
+    """HLR N = 1M, D = 128, 8 chains: plain HMC and surrogate-gradient HMC
+    (linear least squares + MLP 128-256-256-128 on its residual, trained on
+    the pairs of `train` exact iterations after warmup), both with 30
+    leapfrog steps of 0.005 (step jittered as the reference does) and the
+    exact FP64 Metropolis test; `iters` timed iterations each."""
+    from augurv2_amd import runtime, workloads
+    from augurv2_amd._lib import HMC_BLOCK_THETA, HMC_BLOCK_V
+    N, D, C, L, EPS = 1_000_000, 128, 8, 30, 0.005
+    x, y, _, _ = workloads.hlr_data(N, D, seed=0)
+    blocks = [(HMC_BLOCK_V, 1.0, 0.05, 0), (HMC_BLOCK_THETA, L * EPS, EPS, -L)]
+    keep = list(range(D)) + [D + 1]
+    out = {}
+    for mode in ("plain", "surrogate"):
+        s = runtime.HlrSampler(x, y, blocks, chains=C, seed=seed, ref_compat=False)
+        for c, st in enumerate(workloads.hlr_init(D, C, seed=11)):
+            s.set("state", st, chain=c)
+        log(f"surrogate leg ({mode}): warmup {warmup}")
+        s.step(warmup)
+        t_fit = 0.0
+        if mode == "surrogate":
+            s.enable_surrogate(1, hidden=256, train_sweeps=train, max_pairs=train * C * L,
+                               epochs=20, batch=512, lr=1e-3, min_accept=0.6, window=100, seed=5)
+            t0 = time.perf_counter()
+            s.step(train)  # exact HMC collecting pairs, then training
+            t_fit = time.perf_counter() - t0
+        s.trace("state", iters)
+        log(f"surrogate leg ({mode}): timed {iters}")
+        t0 = time.perf_counter()
+        s.step(iters)
+        el = time.perf_counter() - t0
+        ess, rhat = _min_ess(s.read_trace("state"), keep)
+        r = {"elapsed_s": el, "ms_per_iter": 1e3 * el / iters, "min_ess": ess, "rhat_max": rhat,
+             "min_ess_per_s": ess / el, "accept": float(np.nanmean(s.acceptance()[1]))}
+        if mode == "surrogate":
+            info = s.surrogate_info()
+            r.update({"collect_and_train_s": t_fit, "train_ms": info["train_ms"],
+                      "pairs": info["pairs"], "loss": info["loss"], "resid_frac": info["resid_frac"],
+                      "phase": info["phase"], "fallbacks": info["fallbacks"],
+                      "min_ess_per_s_incl_training": ess / (el + t_fit)})
+        s.close()
+        out[mode] = r
+        log(f"surrogate leg ({mode}): {r}")
+    out["ratio"] = out["surrogate"]["min_ess_per_s"] / out["plain"]["min_ess_per_s"]
+    out["config"] = (f"HLR N={N} D={D}, {C} chains, {L} leapfrog steps of {EPS} (step jittered), "
+                     f"warmup {warmup}, surrogate trained on {train} exact iterations, {iters} timed")
+    return out
+
+
+def run_batched(sweeps=1000, warmup=500, chains=4, seed=31):
+    """256 HLR datasets (N_i ~ U[1k, 200k], D_i in {16, 32, 64}) in one
+    batched sampler (BF16 X, autotuned rows per block, hipGraph sweep), 20
+    leapfrog steps of 0.005 with the step jittered.  Beside it the plain
+    sampler's time for the same sweeps on each dataset alone, measured on
+    eight datasets and interpolated linearly in N_i D_i."""
+    from augurv2_amd import runtime, workloads
+    from augurv2_amd._lib import HMC_BLOCK_THETA, HMC_BLOCK_V
+    L, EPS = 20, 0.005
+    xs, ys = workloads.batched_hlr_data()
+    s = runtime.BhlrSampler(xs, ys, chains=chains, seed=seed, v_block=(1.0, 0.05, 0),
+                            theta_block=(L * EPS, EPS, -L), rows_per_block=0, use_graph=True)
+    for i, x in enumerate(xs):
+        for c, st in enumerate(workloads.hlr_init(x.shape[1], chains, seed=100 + i)):
+            s.set(f"state:{i}", st, chain=c)
+    rows, cand = s.tuning()
+    log(f"batched: rows/block {rows}, candidates {cand}; warmup {warmup}")
+    s.step(warmup)
+    s.trace("state", sweeps)
+    t0 = time.perf_counter()
+    s.step(sweeps)
+    el = time.perf_counter() - t0
+    draws = s.read_trace("state")  # (sweeps, C, sum(D_i + 2))
+    acc = s.acceptance()
+    s.close()
+    ess_i, rhat_i = [], []
+    for i, x in enumerate(xs):
+        o, d = int(s.offsets[i]), x.shape[1]
+        e, r = _min_ess(draws[:, :, o:o + d + 2], list(range(d)) + [d + 1])
+        ess_i.append(e)
+        rhat_i.append(r)
+    ess_i = np.array(ess_i)
+    # plain sampler, one dataset at a time
+    work = np.array([x.shape[0] * x.shape[1] for x in xs], np.float64)
+    pick = np.argsort(work)[np.linspace(0, len(xs) - 1, 8).astype(int)]
+    t_plain = []
+    for i in pick:
+        p = runtime.HlrSampler(xs[i], ys[i], [(HMC_BLOCK_V, 1.0, 0.05, 0),
+                                              (HMC_BLOCK_THETA, L * EPS, EPS, -L)],
+                               chains=chains, seed=seed, ref_compat=False)
+        p.step(20)
+        t1 = time.perf_counter()
+        p.step(100)
+        t_plain.append((time.perf_counter() - t1) / 100)
+        p.close()
+    a, b = np.polyfit(work[pick], np.array(t_plain), 1)[::-1]
+    plain_total = float(np.sum(a + b * work)) * sweeps
+    return {"datasets": len(xs), "chains": chains, "rows_total": int(sum(x.shape[0] for x in xs)),
+            "rows_per_block": rows, "autotune_us": cand, "sweeps": sweeps, "elapsed_s": el,
+            "ms_per_sweep": 1e3 * el / sweeps,
+            "sum_dataset_min_ess_per_s": float(ess_i.sum() / el),
+            "min_dataset_ess": float(ess_i.min()), "median_dataset_ess": float(np.median(ess_i)),
+            "rhat_max": float(np.max(rhat_i)), "accept_theta": float(np.nanmean(acc[1])),
+            "plain_sequential_s_est": plain_total, "speedup_vs_plain_sequential": plain_total / el}
+
+
+def run_gmm(sweeps=500, warmup=20, chains=4, seed=5):
+    """Multivariate GMM Gibbs, N = 1M, D = 16, K = 32, 4 chains: sweeps/s."""
+    from augurv2_amd import runtime, workloads
+    N, D, K = 1_000_000, 16, 32
+    y, z_true, _, _, _ = workloads.gmm_data(N, D, K, seed=0)
+    s = runtime.GmmSampler(y, np.ones(K), np.zeros(D), 10.0 * np.eye(D), D + 2, np.eye(D),
+                           chains=chains, seed=seed)
+    rng = np.random.default_rng(1)
+    for c in range(chains):
+        s.set("pis", np.full(K, 1.0 / K), chain=c)
+        s.set("mu", y[rng.choice(N, K, replace=False)].astype(np.float64), chain=c)
+        s.set("cov", np.tile(np.eye(D), (K, 1, 1)), chain=c)
+        s.set("z", rng.integers(0, K, N).astype(np.int32), chain=c)
+    s.step(warmup)
+    t0 = time.perf_counter()
+    s.step(sweeps)
+    el = time.perf_counter() - t0
+    ll = s.log_density()
+    s.close()
+    return {"sweeps": sweeps, "elapsed_s": el, "ms_per_sweep": 1e3 * el / sweeps,
+            "points_per_s": N * chains * sweeps / el, "final_log_density": [float(v) for v in ll]}
+
+
 def run_lda(sweeps=100, warmup=10):
     from augurv2_amd import runtime, workloads
     doc_off, w, alpha, beta, _ = workloads.lda_corpus(K=10, V=1000, D=200, mean_len=100, seed=0)
@@ -115,18 +249,30 @@ def cpu_baseline(hlr, min_ess_per_iter, budget_s=12.0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
+    # 8000 timed sweeps reach R-hat < 1.01 on this workload (2000 reach ~1.04)
+    ap.add_argument("--steps", type=int, default=8000)
     ap.add_argument("--warmup", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--only", default="", help="comma list of legs to run besides the headline "
+                    "HLR leg: lda, surrogate, batched, gmm (default: all)")
     args = ap.parse_args()
+    legs = set(args.only.split(",")) if args.only else {"lda", "surrogate", "batched", "gmm"}
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the sampler "
               "runtime runs on a single MI355X)", file=sys.stderr)
         sys.exit(2)
     hlr = run_hlr(args.steps, args.warmup)
     log(f"hlr: {hlr['elapsed']:.3f} s, min ESS {hlr['min_ess']:.1f}, R-hat {hlr['rhat_max']:.4f}")
-    lda = run_lda()
-    log(f"lda: {lda['tokens_per_s']:.3e} tokens/s")
+    lda = run_lda() if "lda" in legs else None
+    if lda:
+        log(f"lda: {lda['tokens_per_s']:.3e} tokens/s")
+    surr = run_surrogate() if "surrogate" in legs else None
+    bat = run_batched() if "batched" in legs else None
+    if bat:
+        log(f"batched: {bat}")
+    gmm = run_gmm() if "gmm" in legs else None
+    if gmm:
+        log(f"gmm: {gmm}")
     value = hlr["min_ess"] / hlr["elapsed"]
     N, D = hlr["N"], hlr["D"]
     bytes_per_pass = N * D * 4 + N * 4  # X (FP32) + y (int32): the algorithmic bytes
@@ -155,8 +301,12 @@ def main():
         "min_ess": hlr["min_ess"],
         "min_ess_tail": hlr["min_ess_tail"],
         "accept_rate": hlr["accept"],
-        "lda_tokens_per_s": lda["tokens_per_s"],
-        "lda": {k: lda[k] for k in ("tokens", "sweeps", "sweep_us", "zstep_avg_us")},
+        "rhat_ok": hlr["rhat_max"] < 1.01,
+        "lda_tokens_per_s": lda["tokens_per_s"] if lda else None,
+        "lda": {k: lda[k] for k in ("tokens", "sweeps", "sweep_us", "zstep_avg_us")} if lda else None,
+        "surrogate": surr,
+        "batched": bat,
+        "gmm": gmm,
         "roofline": {"kernel": "k_hlr_pass", "bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None, "avg_us": hlr["pass_avg_us"],
