@@ -15,6 +15,7 @@ AUGUR_OK = 0
 STATUS_NAMES = {1: "invalid argument", 2: "HIP error", 3: "invalid state", 4: "unsupported"}
 
 HMC_BLOCK_V, HMC_BLOCK_B, HMC_BLOCK_THETA, HMC_BLOCK_B_THETA = 0, 1, 2, 3
+MWG_BLOCK_V = 4  # proposal Normal(v, step_size): step_size is the variance
 
 
 class AugurError(RuntimeError):

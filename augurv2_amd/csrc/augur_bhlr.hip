@@ -12,9 +12,9 @@
 // chosen by the autotuner at creation); a block stages 64-row tiles in LDS
 // (BF16 -> FP32), computes t = x . theta for all chains of its dataset,
 // a = y - sigmoid(t), log sigmoid(+-t) and a x (FP32 arithmetic, FP64
-// accumulation), and writes its partials; the last block of each dataset sums
-// that dataset's partials in block order (deterministic) and applies the
-// leapfrog update of its chains.  One sweep (v block, momentum draw, L
+// accumulation), and writes its partials; a finish kernel (one wave per
+// value) sums each dataset's partials in block order (deterministic) and
+// applies the leapfrog update of its chains.  One sweep (v block, momentum draw, L
 // passes, Metropolis step, sweep counter) can be captured in a hipGraph and
 // replayed: every kernel reads the sweep counter from device memory.
 //
@@ -62,8 +62,9 @@ struct BArgs {
   double step;       // theta block: step size (jittered per chain and sweep
   int steps;         // when step_jitter) and fixed count
   int step_jitter;
-  double v_sim_len, v_eps;
+  double v_sim_len, v_eps;  // v block: HMC settings, or the MWG proposal variance
   int v_fixed;
+  int v_mwg;
 };
 
 enum { BMODE_CACHE = 0, BMODE_LEAPFROG = 1 };
@@ -237,9 +238,6 @@ __device__ void bhlr_body(const BArgs& A, const BDs& ds, int di, unsigned char* 
   (void)di;
 }
 
-// One wave per value (dataset i, chain c, coordinate j; j = d_i + 1: the
-// log-density): sums the dataset's block partials in block order (fixed
-// butterfly), then applies the leapfrog update of that coordinate.
 // The theta block's step size of chain key gc this sweep: eps, or with
 // step_jitter eps U(0.8, 1.2) from the block's jitter stream, as the
 // reference draws it (cbits/augur_mcmclib.h:401-402).
@@ -249,6 +247,9 @@ __device__ inline double theta_step(const BArgs& A, uint32_t gc) {
   return (0.8 + 0.4 * u01(js.draw(0, 0).x)) * A.step;
 }
 
+// One wave per value (dataset i, chain c, coordinate j; j = d_i + 1: the
+// log-density): sums the dataset's block partials in block order (fixed
+// butterfly), then applies the leapfrog update of that coordinate.
 __global__ __launch_bounds__(256) void k_bhlr_finish(BArgs A) {
   const int lane = threadIdx.x & 63;
   const int v = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -392,29 +393,11 @@ __global__ __launch_bounds__(64) void k_bhmc_vblock(BArgs A) {
   double S = 0.0;
   for (int j = lane; j < D; j += 64) S += st[j] * st[j];
   S = wave_sum(S);
-  auto ll_of = [&](double v) {
-    return normal_sum_ll(D, S, v, A.ref_compat) + v_prior_ll0(A.prior_kind, A.prior_a, A.prior_b, v);
-  };
-  auto grad_of = [&](double v) {
-    return normal_sum_dv(D, S, v, A.ref_compat) + v_prior_dot0(A.prior_kind, A.prior_a, A.prior_b, v);
-  };
-  double step;
-  int steps;
-  hmc_jitter(Stream{A.seed, gc, sw, TAG_HMC_BASE + 1u}, A.v_sim_len, A.v_eps, A.v_fixed, &step, &steps);
-  const double v0 = st[D + 1];
-  const double p0 = normal_at(Stream{A.seed, gc, sw, TAG_HMC_BASE + 0u}, 0, 0);
-  double v = v0;
-  double p = p0 + 0.5 * step * grad_of(v);
-  for (int l = 0; l < steps; ++l) {
-    v += step * p;
-    if (l != steps - 1) p += step * grad_of(v);
-  }
-  p += 0.5 * step * grad_of(v);
-  const double ll0 = ll_of(v0), ll1 = ll_of(v);
-  const double u = u01(Stream{A.seed, gc, sw, TAG_HMC_BASE + 2u}.draw(0, 0).x);
-  const bool acc = log(u) < (ll1 - ll0) + (0.5 * p0 * p0 - 0.5 * p * p);
+  double v;
+  const bool acc = v_update(D, S, st[D + 1], A.ref_compat, A.prior_kind, A.prior_a, A.prior_b, A.v_mwg,
+                            A.v_sim_len, A.v_eps, A.v_fixed, A.seed, gc, sw, TAG_HMC_BASE, &v);
   if (lane == 0) {
-    if (acc) st[D + 1] = v;
+    st[D + 1] = v;
     double* s2 = A.stats + ((size_t)c * A.n_ds + di) * 2;
     s2[0] += acc ? 1.0 : 0.0;
     s2[1] += 1.0;
@@ -675,7 +658,10 @@ augur_status augur_bhlr_create(const augur_bhlr_desc* D, augur_sampler** out) {
     if (D->theta_block.fixed_steps == 0 || D->theta_block.fixed_steps > 100000 ||
         D->theta_block.fixed_steps < -100000 || !(D->theta_block.step_size > 0.0))
       fail(AUGUR_ERR_ARG, "the batched theta block needs fixed_steps != 0 and step_size > 0");
-    if (!(D->v_block.step_size > 0.0) || (D->v_block.fixed_steps == 0 && !(D->v_block.sim_len > 0.0)))
+    if (D->v_block.block != AUGUR_HMC_BLOCK_V && D->v_block.block != AUGUR_MWG_BLOCK_V)
+      fail(AUGUR_ERR_ARG, "the batched v block is HMC or MWG");
+    if (!(D->v_block.step_size > 0.0) ||
+        (D->v_block.block == AUGUR_HMC_BLOCK_V && D->v_block.fixed_steps == 0 && !(D->v_block.sim_len > 0.0)))
       fail(AUGUR_ERR_ARG, "bad v block settings");
     if (D->prior_kind != 0 && D->prior_kind != 1) fail(AUGUR_ERR_ARG, "prior_kind must be 0 or 1");
     for (int i = 0; i < D->n_datasets; ++i) {
@@ -754,6 +740,7 @@ augur_status augur_bhlr_create(const augur_bhlr_desc* D, augur_sampler** out) {
       A.step = D->theta_block.step_size; A.steps = std::abs(D->theta_block.fixed_steps);
       A.step_jitter = D->theta_block.fixed_steps < 0;
       A.v_sim_len = D->v_block.sim_len; A.v_eps = D->v_block.step_size; A.v_fixed = D->v_block.fixed_steps;
+      A.v_mwg = D->v_block.block == AUGUR_MWG_BLOCK_V;
       s->pass = pick_bpass(s->next_cp());
       // state starts at theta = 0, v = 1
       std::vector<double> st(C * s->SW, 0.0);

@@ -15,11 +15,10 @@
 //                 X tile (64 rows) staged in LDS and shared by every chain,
 //                 t = x.theta + b, a = y - sigmoid(t), ll += log sigmoid(+-t),
 //                 grad += a x; FP32 arithmetic, FP64 accumulation across
-//                 tiles and blocks; block partials are reduced in two levels
-//                 in a fixed order (deterministic): the last block of each
-//                 group of 16 sums its group, the last group sums the groups
-//                 and applies the leapfrog update of every chain (momentum,
-//                 position, FP32 copy of the next evaluation point).
+//                 tiles; each block writes one row of partials
+//   k_hlr_finish  sums the partials in a fixed order (deterministic) and
+//                 applies the leapfrog update of every chain (momentum,
+//                 position, FP32 copy of the next evaluation point)
 //   k_hmc_end     kinetic energies, FP64 Metropolis test, accept / reject.
 //
 // The data log-density and gradient at the current point are cached (they
@@ -46,7 +45,7 @@ struct PassArgs {
   const int32_t* __restrict__ y;
   int n, d, cp, c_real, has_b;
   float* beta32;   // cp x (d + 1): evaluation point (theta, b) per chain
-  double* part;    // [cp * (d + 2)][G]
+  double* part;    // [G][cp * (d + 2)]
   int G;           // blocks of the pass kernel
   int mode;
   int l;           // leapfrog pass index (1-based)
@@ -70,9 +69,9 @@ struct PassArgs {
 // Reduction of the block partials and the leapfrog epilogue for values
 // (chain c, coordinate j; j = d: intercept, j = d + 1: log-density); the
 // partials are part[block][value].  A 1024-thread block takes kFinVals
-// consecutive values: thread (group g, value i) sums rows g, g + 32, ... of
-// its column (each group reads contiguous row segments), then thread i adds
-// the 32 group sums in order: the result is deterministic.  The update of
+// consecutive values: thread (group g, value i) sums rows g, g + kFinGroups,
+// ... of its column (each group reads contiguous row segments), then thread
+// i adds the group sums in order: the result is deterministic.  The update of
 // coordinate j needs only value j, so every value finishes independently.
 // (A separate kernel instead of a last-block epilogue: the device-scope
 // fence each pass block would need writes back the L2 and serialises across
@@ -582,8 +581,9 @@ __global__ __launch_bounds__(64) void k_hmc_end(HmcArgs A) {
 struct VBlockArgs {
   int d, has_b, ref_compat, prior_kind;
   double prior_a, prior_b;
-  double sim_len, eps;
+  double sim_len, eps;  // MWG: eps is the proposal variance
   int fixed_steps;
+  int mwg;              // 0: HMC, 1: Metropolis-within-Gibbs
   uint32_t seed, sweep, tag;
   double* state;  // C x P
   double* stats;  // [chain][2]
@@ -592,12 +592,9 @@ struct VBlockArgs {
 __device__ inline double v_prior_ll(const VBlockArgs& A, double v) {
   return v_prior_ll0(A.prior_kind, A.prior_a, A.prior_b, v);
 }
-__device__ inline double v_prior_dot(const VBlockArgs& A, double v) {
-  return v_prior_dot0(A.prior_kind, A.prior_a, A.prior_b, v);
-}
 
-// HMC on the scalar variance v (the full conditional involves only theta
-// and b): one wave per chain runs the whole trajectory.
+// HMC or Metropolis-within-Gibbs on the scalar variance v (the full
+// conditional involves only theta and b): one wave per chain.
 __global__ __launch_bounds__(64) void k_hmc_vblock(VBlockArgs A) {
   const int c = blockIdx.x, lane = threadIdx.x;
   const int d = A.d, P = d + 2;
@@ -606,26 +603,11 @@ __global__ __launch_bounds__(64) void k_hmc_vblock(VBlockArgs A) {
   double S = 0.0;
   for (int j = lane; j < nterm; j += 64) S += st[j] * st[j];
   S = wave_sum(S);
-  auto ll_of = [&](double v) { return normal_sum_ll(nterm, S, v, A.ref_compat) + v_prior_ll(A, v); };
-  auto grad_of = [&](double v) { return normal_sum_dv(nterm, S, v, A.ref_compat) + v_prior_dot(A, v); };
-  double step;
-  int steps;
-  hmc_jitter(Stream{A.seed, (uint32_t)c, A.sweep, A.tag + 1}, A.sim_len, A.eps, A.fixed_steps, &step,
-             &steps);
-  const double v0 = st[d + 1];
-  const double p0 = normal_at(Stream{A.seed, (uint32_t)c, A.sweep, A.tag + 0}, 0, 0);
-  double v = v0;
-  double p = p0 + 0.5 * step * grad_of(v);
-  for (int l = 0; l < steps; ++l) {
-    v += step * p;
-    if (l != steps - 1) p += step * grad_of(v);
-  }
-  p += 0.5 * step * grad_of(v);
-  const double ll0 = ll_of(v0), ll1 = ll_of(v);
-  const double u = u01(Stream{A.seed, (uint32_t)c, A.sweep, A.tag + 2}.draw(0, 0).x);
-  const bool acc = log(u) < (ll1 - ll0) + (0.5 * p0 * p0 - 0.5 * p * p);
+  double v;
+  const bool acc = v_update(nterm, S, st[d + 1], A.ref_compat, A.prior_kind, A.prior_a, A.prior_b, A.mwg,
+                            A.sim_len, A.eps, A.fixed_steps, A.seed, (uint32_t)c, A.sweep, A.tag, &v);
   if (lane == 0) {
-    if (acc) st[d + 1] = v;
+    st[d + 1] = v;
     A.stats[2 * c] += acc ? 1.0 : 0.0;
     A.stats[2 * c + 1] += 1.0;
   }
@@ -886,7 +868,7 @@ struct HlrSampler final : augur_sampler {
     VBlockArgs V{};
     V.d = d; V.has_b = has_b; V.ref_compat = ref_compat; V.prior_kind = prior_kind;
     V.prior_a = prior_a; V.prior_b = prior_b; V.sim_len = blk.sim_len; V.eps = blk.step_size;
-    V.fixed_steps = blk.fixed_steps; V.seed = seed; V.sweep = sweep;
+    V.fixed_steps = blk.fixed_steps; V.mwg = blk.block == AUGUR_MWG_BLOCK_V; V.seed = seed; V.sweep = sweep;
     V.tag = TAG_HMC_BASE + 4u * (uint32_t)bi; V.state = state.p;
     V.stats = stats_buf.p + (size_t)bi * chains * 2;
     return V;
@@ -896,7 +878,7 @@ struct HlrSampler final : augur_sampler {
     for (int i = 0; i < nsw; ++i) {
       for (int bi = 0; bi < (int)sched.size(); ++bi) {
         const augur_hmc_block& blk = sched[bi];
-        if (blk.block == AUGUR_HMC_BLOCK_V) {
+        if (blk.block == AUGUR_HMC_BLOCK_V || blk.block == AUGUR_MWG_BLOCK_V) {
           k_hmc_vblock<<<chains, 64, 0, stream>>>(vargs(bi, blk));
           AUGUR_CHECK_LAUNCH();
         } else {
@@ -984,9 +966,13 @@ augur_status augur_hlr_create(const augur_hlr_desc* D, augur_sampler** out) {
     if (D->n_blocks < 1 || D->n_blocks > 4) fail(AUGUR_ERR_ARG, "1..4 HMC blocks");
     for (int i = 0; i < D->n_blocks; ++i) {
       const augur_hmc_block& b = D->blocks[i];
-      if (b.block < 0 || b.block > 3) fail(AUGUR_ERR_ARG, "bad HMC block kind %d", b.block);
+      if (b.block < 0 || b.block > AUGUR_MWG_BLOCK_V) fail(AUGUR_ERR_ARG, "bad block kind %d", b.block);
       if ((b.block == AUGUR_HMC_BLOCK_B || b.block == AUGUR_HMC_BLOCK_B_THETA) && !D->has_intercept)
         fail(AUGUR_ERR_ARG, "HMC block over b without an intercept");
+      if (b.block == AUGUR_MWG_BLOCK_V) {
+        if (!(b.step_size > 0.0)) fail(AUGUR_ERR_ARG, "MWG block %d needs a proposal variance > 0", i);
+        continue;
+      }
       if (!(b.step_size > 0.0) || (b.fixed_steps == 0 && !(b.sim_len > 0.0)) || b.fixed_steps > 100000 ||
           b.fixed_steps < -100000)
         fail(AUGUR_ERR_ARG, "HMC block %d needs step_size > 0 and sim_len > 0 or fixed_steps", i);

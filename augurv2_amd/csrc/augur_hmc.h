@@ -70,6 +70,49 @@ __device__ inline double v_prior_dot0(int kind, double a, double b, double v) {
   return (a > 0.0 && v > 0.0) ? -a : 0.0;
 }
 
+// One update of the variance v given K terms with S = sum x_j^2, for the
+// block whose streams are (seed, chain, sweep, tag + 0 | 1 | 2):
+//   mwg = 0: HMC (cbits/augur_mcmclib.h:389-424), jitter from tag + 1,
+//            momentum tag + 0, accept uniform tag + 2;
+//   mwg = 1: Metropolis-within-Gibbs (cbits/augur_mcmclib.h:178-193) with the
+//            proposal v' ~ Normal(v, eps) (eps is the proposal variance, as in
+//            the schedule's `MWG [v] ~ Normal(v, eps)`), normal from tag + 0,
+//            accept uniform tag + 2.  The Hastings term of the reference's
+//            generated proposal (Core/CgUserProp.hs:130-141) evaluates both
+//            proposal densities with the current point's parameters:
+//            ll(v | v, eps) - ll(v' | v, eps) = (v' - v)^2 / (2 eps), kept with
+//            ref_compat; otherwise 0, the correct term of a symmetric proposal.
+// Returns whether the move was accepted; *v_out is the new value.
+__device__ inline bool v_update(int K, double S, double v0, int ref_compat, int prior_kind, double prior_a,
+                                double prior_b, int mwg, double sim_len, double eps, int fixed_steps,
+                                uint32_t seed, uint32_t chain, uint32_t sweep, uint32_t tag, double* v_out) {
+  auto ll_of = [&](double v) { return normal_sum_ll(K, S, v, ref_compat) + v_prior_ll0(prior_kind, prior_a, prior_b, v); };
+  const double u = u01(Stream{seed, chain, sweep, tag + 2}.draw(0, 0).x);
+  double v = v0, extra;
+  if (mwg) {
+    v = v0 + sqrt(eps) * normal_at(Stream{seed, chain, sweep, tag + 0}, 0, 0);
+    extra = ref_compat ? 0.5 * (v - v0) * (v - v0) / eps : 0.0;
+  } else {
+    auto grad_of = [&](double w) {
+      return normal_sum_dv(K, S, w, ref_compat) + v_prior_dot0(prior_kind, prior_a, prior_b, w);
+    };
+    double step;
+    int steps;
+    hmc_jitter(Stream{seed, chain, sweep, tag + 1}, sim_len, eps, fixed_steps, &step, &steps);
+    const double p0 = normal_at(Stream{seed, chain, sweep, tag + 0}, 0, 0);
+    double p = p0 + 0.5 * step * grad_of(v);
+    for (int l = 0; l < steps; ++l) {
+      v += step * p;
+      if (l != steps - 1) p += step * grad_of(v);
+    }
+    p += 0.5 * step * grad_of(v);
+    extra = 0.5 * p0 * p0 - 0.5 * p * p;
+  }
+  const bool acc = log(u) < (ll_of(v) - ll_of(v0)) + extra;
+  *v_out = acc ? v : v0;
+  return acc;
+}
+
 // Data term of one row: e = exp(-|t|); sigmoid(t) = 1/(1+e) (t >= 0) or
 // e/(1+e); a = y - sigmoid(t) (= bernoulli_dotarg1 * expit_grad_0,
 // cbits/augur_dist.c:68-71, cbits/augur_math.c:56-59); returns

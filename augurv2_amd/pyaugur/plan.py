@@ -14,7 +14,7 @@ The reference's compiler would instead generate code for any model
 import numpy as np
 
 from .. import runtime
-from .._lib import HMC_BLOCK_B, HMC_BLOCK_B_THETA, HMC_BLOCK_THETA, HMC_BLOCK_V
+from .._lib import HMC_BLOCK_B, HMC_BLOCK_B_THETA, HMC_BLOCK_THETA, HMC_BLOCK_V, MWG_BLOCK_V
 from . import augurdistlib as adl
 from .rvparse import Kern
 
@@ -215,8 +215,16 @@ class HlrPlan:
             sched = [Kern("HMC", [d.name]) for d in model.params]
         self.blocks = []
         for k in sched:
+            if k.kind == "MWG" and k.vars == [v_name]:
+                # MWG [v] ~ Normal(v, s): random walk on v, s the proposal variance
+                prop = k.proposal[0] if k.proposal else None
+                pargs = _dist(prop, "Normal") if prop else None
+                if pargs is None or _var(pargs[0]) != v_name or pargs[1][0] != "lit":
+                    _fail(f"HLR: MWG [{v_name}] needs the proposal Normal({v_name}, <variance>)")
+                self.blocks.append((MWG_BLOCK_V, 0.0, float(pargs[1][1])))
+                continue
             if k.kind != "HMC":
-                _fail(f"HLR: only HMC kernels run on this runtime (got {k.kind})")
+                _fail(f"HLR: only HMC kernels and MWG [{v_name}] run on this runtime (got {k.kind})")
             vs = set(k.vars)
             if vs == {v_name}:
                 kind = HMC_BLOCK_V

@@ -189,7 +189,11 @@ class BhlrSampler(Sampler):
         desc.chains, desc.seed = chains, seed & 0xFFFFFFFF
         desc.prior_kind, desc.prior_a, desc.prior_b = prior_kind, prior_a, prior_b
         desc.ref_compat = int(ref_compat)
-        desc.v_block = L.HmcBlock(L.HMC_BLOCK_V, *v_block[:2], int(v_block[2]))
+        # v_block: (sim_len, eps, fixed_steps) for HMC, or ("mwg", variance)
+        if v_block[0] == "mwg":
+            desc.v_block = L.HmcBlock(L.MWG_BLOCK_V, 0.0, float(v_block[1]), 0)
+        else:
+            desc.v_block = L.HmcBlock(L.HMC_BLOCK_V, *v_block[:2], int(v_block[2]))
         desc.theta_block = L.HmcBlock(L.HMC_BLOCK_THETA, *theta_block[:2], int(theta_block[2]))
         desc.rows_per_block, desc.use_graph = int(rows_per_block), int(use_graph)
         h = C.c_void_p()

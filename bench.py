@@ -80,10 +80,12 @@ def run_surrogate(iters=5000, warmup=1000, train=2000, seed=77):
     leapfrog steps of 0.005 (step jittered as the reference does) and the
     exact FP64 Metropolis test; `iters` timed iterations each."""
     from augurv2_amd import runtime, workloads
-    from augurv2_amd._lib import HMC_BLOCK_THETA, HMC_BLOCK_V
+    from augurv2_amd._lib import HMC_BLOCK_THETA, MWG_BLOCK_V
     N, D, C, L, EPS = 1_000_000, 128, 8, 30, 0.005
     x, y, _, _ = workloads.hlr_data(N, D, seed=0)
-    blocks = [(HMC_BLOCK_V, 1.0, 0.05, 0), (HMC_BLOCK_THETA, L * EPS, EPS, -L)]
+    # MWG [v] ~ Normal(v, 1.0) (examples/hlr.py:57): HMC on v stalls for good
+    # at small v here (its curvature ~ S / v^3 leaves the step unstable)
+    blocks = [(MWG_BLOCK_V, 0.0, 1.0, 0), (HMC_BLOCK_THETA, L * EPS, EPS, -L)]
     keep = list(range(D)) + [D + 1]
     out = {}
     for mode in ("plain", "surrogate"):
@@ -95,7 +97,7 @@ def run_surrogate(iters=5000, warmup=1000, train=2000, seed=77):
         t_fit = 0.0
         if mode == "surrogate":
             s.enable_surrogate(1, hidden=256, train_sweeps=train, max_pairs=train * C * L,
-                               epochs=20, batch=512, lr=1e-3, min_accept=0.6, window=100, seed=5)
+                               epochs=10, batch=4096, lr=2e-3, min_accept=0.6, window=100, seed=5)
             t0 = time.perf_counter()
             s.step(train)  # exact HMC collecting pairs, then training
             t_fit = time.perf_counter() - t0
@@ -125,14 +127,15 @@ def run_surrogate(iters=5000, warmup=1000, train=2000, seed=77):
 def run_batched(sweeps=1000, warmup=500, chains=4, seed=31):
     """256 HLR datasets (N_i ~ U[1k, 200k], D_i in {16, 32, 64}) in one
     batched sampler (BF16 X, autotuned rows per block, hipGraph sweep), 20
-    leapfrog steps of 0.005 with the step jittered.  Beside it the plain
+    leapfrog steps of 0.005 with the step jittered, MWG [v] ~ Normal(v, 1.0).
+    Beside it the plain
     sampler's time for the same sweeps on each dataset alone, measured on
     eight datasets and interpolated linearly in N_i D_i."""
     from augurv2_amd import runtime, workloads
-    from augurv2_amd._lib import HMC_BLOCK_THETA, HMC_BLOCK_V
+    from augurv2_amd._lib import HMC_BLOCK_THETA, MWG_BLOCK_V
     L, EPS = 20, 0.005
     xs, ys = workloads.batched_hlr_data()
-    s = runtime.BhlrSampler(xs, ys, chains=chains, seed=seed, v_block=(1.0, 0.05, 0),
+    s = runtime.BhlrSampler(xs, ys, chains=chains, seed=seed, v_block=("mwg", 1.0),
                             theta_block=(L * EPS, EPS, -L), rows_per_block=0, use_graph=True)
     for i, x in enumerate(xs):
         for c, st in enumerate(workloads.hlr_init(x.shape[1], chains, seed=100 + i)):
@@ -159,7 +162,7 @@ def run_batched(sweeps=1000, warmup=500, chains=4, seed=31):
     pick = np.argsort(work)[np.linspace(0, len(xs) - 1, 8).astype(int)]
     t_plain = []
     for i in pick:
-        p = runtime.HlrSampler(xs[i], ys[i], [(HMC_BLOCK_V, 1.0, 0.05, 0),
+        p = runtime.HlrSampler(xs[i], ys[i], [(MWG_BLOCK_V, 0.0, 1.0, 0),
                                               (HMC_BLOCK_THETA, L * EPS, EPS, -L)],
                                chains=chains, seed=seed, ref_compat=False)
         p.step(20)

@@ -76,8 +76,13 @@ augur_status augur_lda_create(const augur_lda_desc* desc, augur_sampler** out);
 #define AUGUR_HMC_BLOCK_B 1
 #define AUGUR_HMC_BLOCK_THETA 2
 #define AUGUR_HMC_BLOCK_B_THETA 3
+/* Metropolis-within-Gibbs on v with the proposal v' ~ Normal(v, step_size)
+ * (step_size is the proposal variance): the schedule item
+ * `MWG [v] ~ Normal(v, s)` (cbits/augur_mcmclib.h:178-193,
+ * Core/CgUserProp.hs:130-141); sim_len and fixed_steps are ignored. */
+#define AUGUR_MWG_BLOCK_V 4
 typedef struct {
-  int block;        /* AUGUR_HMC_BLOCK_* */
+  int block;        /* AUGUR_HMC_BLOCK_* or AUGUR_MWG_BLOCK_V */
   double sim_len;   /* reference simLen (Core/KernSyn.hs:64-75 default 1.0) */
   double step_size; /* reference eps (default 0.05) */
   int fixed_steps;  /* > 0: fixed step size and this many leapfrog steps;

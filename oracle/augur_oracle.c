@@ -635,6 +635,40 @@ int or_hmc_step(const or_hlr_model* m, double* state, int block, int block_id,
   return acc;
 }
 
+/* Metropolis-within-Gibbs on v (cbits/augur_mcmclib.h:178-193 with the
+ * accept rule of :89-105): v' = v + sqrt(var) z, z from the block's tag + 0
+ * stream, the uniform from tag + 2.  The Hastings term follows the generated
+ * proposal of Core/CgUserProp.hs:130-141, which evaluates both proposal
+ * densities at the current point's parameters: log N(v; v, var) -
+ * log N(v'; v, var) = (v' - v)^2 / (2 var) with ref_compat; without it the
+ * correct term of the symmetric random walk, 0.  out_info (4): v', ll0, ll1,
+ * accepted. */
+int or_mwg_v_step(const or_hlr_model* m, double* state, int block_id, double var, uint32_t seed,
+                  uint32_t chain, uint32_t sweep, double* out_info) {
+  const int sd = m->d + 2;
+  const uint32_t tag = OR_TAG_HMC_BASE + 4u * (uint32_t)block_id;
+  double* prop = (double*)malloc(sizeof(double) * sd);
+  memcpy(prop, state, sizeof(double) * sd);
+  const double v0 = state[m->d + 1];
+  const double v1 = v0 + sqrt(var) * or_normal(seed, chain, 0, 0, sweep, tag + 0);
+  prop[m->d + 1] = v1;
+  const double extra = m->ref_compat ? 0.5 * (v1 - v0) * (v1 - v0) / var : 0.0;
+  const double ll0 = or_hlr_block_ll(m, state, 0);
+  const double ll1 = or_hlr_block_ll(m, prop, 0);
+  uint32_t w[4];
+  or_draw4(seed, chain, 0, 0, sweep, tag + 2, w);
+  const int acc = log(or_u01(w[0])) < ll1 - ll0 + extra;
+  if (acc) state[m->d + 1] = v1;
+  if (out_info) {
+    out_info[0] = v1;
+    out_info[1] = ll0;
+    out_info[2] = ll1;
+    out_info[3] = acc;
+  }
+  free(prop);
+  return acc;
+}
+
 /* ------------------------------------------------------------------------ */
 /* GMM: hmvgmm.rv, schedule ConjGibbs pis, mu, cov, DiscGibbs z.             */
 /* ------------------------------------------------------------------------ */

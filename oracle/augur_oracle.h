@@ -124,6 +124,9 @@ void or_hlr_block_grad(const or_hlr_model* m, const double* state, int block, do
 int or_hmc_step(const or_hlr_model* m, double* state, int block, int block_id,
                 double sim_len, double eps, int fixed_steps, uint32_t seed,
                 uint32_t chain, uint32_t sweep, double* out_info);
+/* Metropolis-within-Gibbs on v, proposal Normal(v, var); out_info (4). */
+int or_mwg_v_step(const or_hlr_model* m, double* state, int block_id, double var, uint32_t seed,
+                  uint32_t chain, uint32_t sweep, double* out_info);
 
 /* ---- GMM (hmvgmm.rv / mvgmm.rv) ---- */
 typedef struct {

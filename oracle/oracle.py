@@ -91,6 +91,8 @@ def lib():
             "or_hlr_block_grad": (None, [C.POINTER(HlrModel), P_D, C.c_int, P_D]),
             "or_hmc_step": (C.c_int, [C.POINTER(HlrModel), P_D, C.c_int, C.c_int, C.c_double,
                                       C.c_double, C.c_int, U32, U32, U32, P_D]),
+            "or_mwg_v_step": (C.c_int, [C.POINTER(HlrModel), P_D, C.c_int, C.c_double, U32, U32,
+                                        U32, P_D]),
             "or_gmm_sweep": (None, [C.POINTER(GmmModel), P_D, P_D, P_D, P_I, U32, U32, U32]),
             "or_gmm_zstep": (None, [C.POINTER(GmmModel), P_D, P_D, P_D, P_I, U32, U32, U32]),
         }
@@ -245,6 +247,12 @@ class Hlr:
         info = np.zeros(7)
         lib().or_hmc_step(C.byref(self.m), state, block, block_id, sim_len, eps,
                           int(fixed_steps), seed, chain, sweep, info)
+        return info
+
+    def mwg_v_step(self, state, block_id, var, seed, chain, sweep):
+        """Metropolis-within-Gibbs on v in place; returns info[4]."""
+        info = np.zeros(4)
+        lib().or_mwg_v_step(C.byref(self.m), state, block_id, var, seed, chain, sweep, info)
         return info
 
 

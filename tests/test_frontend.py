@@ -113,8 +113,16 @@ def test_plan_matching():
     assert p.blocks[1][0] == 3 and p.names["b"] == "b"
     p = plan.match(rvparse.parse_model(HMVGMM), None)
     assert p.family == "gmm" and p.sample_pi and p.sample_cov
-    with pytest.raises(plan.UnsupportedModel):
-        plan.match(rvparse.parse_model(HLR), rvparse.parse_schedule("MWG [v] ~ Normal(v, 1.0) (*) HMC [theta]"))
+    # examples/hlr.py:57's random-walk MWG on the variance
+    p = plan.match(rvparse.parse_model(HLR_CFG),
+                   rvparse.parse_schedule("MWG [v] ~ Normal(v, 1.0) (*) HMC [beta] [0.1, 0.005]"))
+    assert p.blocks == [(4, 0.0, 1.0), (2, 0.1, 0.005)]
+    with pytest.raises(plan.UnsupportedModel):  # MWG on the intercept needs a data pass
+        plan.match(rvparse.parse_model(HLR), rvparse.parse_schedule(
+            "MWG [v] ~ Normal(v, 1.0) (*) MWG [b] ~ Normal(b, 1.0) (*) HMC [theta]"))
+    with pytest.raises(plan.UnsupportedModel):  # only the random walk centred at v
+        plan.match(rvparse.parse_model(HLR_CFG), rvparse.parse_schedule(
+            "MWG [v] ~ Normal(1.0, 1.0) (*) HMC [beta]"))
     with pytest.raises(plan.UnsupportedModel):
         plan.match(rvparse.parse_model(HMVGMM), rvparse.parse_schedule(
             "ConjGibbs [pis] (*) ESlice [mu] (*) ConjGibbs [cov] (*) DiscGibbs [z]"))

@@ -70,6 +70,30 @@ def test_batched_matches_oracle(rt, oracle, L):
     s.close()
 
 
+def test_batched_mwg_v_matches_oracle(rt, oracle):
+    """v block by Metropolis-within-Gibbs, proposal Normal(v, 0.5), with the
+    step-jittered theta block, graph replay on."""
+    rng = np.random.default_rng(5)
+    xs, ys = _datasets(rng, SHAPES)
+    C, L, EPS = 2, -6, 0.01
+    s = rt.BhlrSampler(xs, ys, chains=C, seed=17, v_block=("mwg", 0.5), theta_block=(0.1, EPS, L),
+                       rows_per_block=512, use_graph=True)
+    starts = _init(s, rng, C)
+    models = [oracle.Hlr(bf16_round(x).astype(np.float64), y, prior_kind=0, prior_a=1.0,
+                         prior_b=1.0, ref_compat=False) for x, y in zip(xs, ys)]
+    for sweep in range(3):
+        s.step(1)
+        for i in range(len(SHAPES)):
+            for c in range(C):
+                st = starts[i, c]
+                gc = i * C + c
+                models[i].mwg_v_step(st, 0, 0.5, 17, gc, sweep)
+                models[i].hmc_step(st, 2, 1, 0.1, EPS, L, 17, gc, sweep)
+                np.testing.assert_allclose(s.get(f"state:{i}", c), st, rtol=2e-5, atol=1e-7)
+    assert s.acceptance()[0].mean() > 0.05
+    s.close()
+
+
 def test_graph_replay_equals_direct_launches(rt):
     rng = np.random.default_rng(1)
     xs, ys = _datasets(rng, SHAPES)

@@ -105,6 +105,35 @@ def test_hlr_iterations_match_oracle(rt, oracle, fixed, intercept):
     s.close()
 
 
+@pytest.mark.parametrize("ref_compat", [False, True])
+def test_hlr_mwg_v_matches_oracle(rt, oracle, ref_compat):
+    """MWG [v] ~ Normal(v, 0.3) (*) HMC [theta]: every sweep's state against
+    the oracle, starting from v values where the proposal often goes below 0."""
+    from augurv2_amd._lib import HMC_BLOCK_THETA, MWG_BLOCK_V
+    rng = np.random.default_rng(21)
+    n, d, C = 2000, 8, 4
+    x, y = _hlr_data(rng, n, d)
+    blocks = [(MWG_BLOCK_V, 0.0, 0.3, 0), (HMC_BLOCK_THETA, 0.1, 0.01, 0)]
+    s = rt.HlrSampler(x, y, blocks, chains=C, seed=99, ref_compat=ref_compat)
+    m = oracle.Hlr(x.astype(np.float64), y, ref_compat=ref_compat)
+    states = []
+    for c in range(C):
+        st = np.concatenate([rng.standard_normal(d) * 0.3, [0.0], [0.2 + 0.3 * c]])
+        s.set("state", st, chain=c)
+        states.append(st)
+    moved = 0
+    for sweep in range(6):
+        s.step(1)
+        for c in range(C):
+            v_before = states[c][d + 1]
+            m.mwg_v_step(states[c], 0, 0.3, 99, c, sweep)
+            moved += states[c][d + 1] != v_before
+            m.hmc_step(states[c], 2, 1, 0.1, 0.01, 0, 99, c, sweep)
+            np.testing.assert_allclose(s.get("state", c), states[c], rtol=2e-5, atol=1e-7)
+    assert moved > 0
+    s.close()
+
+
 def test_hlr_curr_ll(rt, oracle):
     from augurv2_amd._lib import HMC_BLOCK_V, HMC_BLOCK_THETA
     rng = np.random.default_rng(4)
