@@ -5,7 +5,9 @@
 // Each dataset i follows hlr.rv without intercept: v_i ~ InvGamma(a, b) |
 // Exponential(a); theta_i[k] ~ Normal(0, v_i); y_i[n] ~ Bernoulli(sigmoid(
 // dotprod(theta_i, x_i[n]))), with C chains per dataset and the schedule
-// HMC [v] (*) HMC [theta] (fixed leapfrog count for theta).
+// HMC [v] | MWG [v] (*) HMC [theta] (fixed leapfrog count for theta, step
+// optionally jittered).  Up to 4 chains per dataset the pass is vector FP32
+// (below); from 5 chains on it is a BF16 matrix-core GEMM (bhlr_body_mfma).
 //
 // Layout in HBM: every dataset's X stored BF16, row-major, back to back; y as
 // uint8.  The pass kernel gives each dataset ceil(N_i / R) blocks of R rows (R
@@ -306,6 +308,192 @@ __global__ __launch_bounds__(kBThreads) void k_bhlr_pass(BArgs A) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Matrix-core form for 5..16 chains per dataset: the data pass is two GEMMs
+// per 16-row tile, T = X B^T (rows x chains) and G += X^T A (features x
+// chains), on V_MFMA with BF16 operands and FP32 accumulation.
+//   - X is BF16 in memory, so its operands are exact.
+//   - theta (FP32) enters as three BF16 parts hi + mid + lo (24 significant
+//     bits), so t carries FP32-level error like the vector form.
+//   - the adjoint a = y - sigmoid(t) enters as three parts as well, so the
+//     gradient matches the vector form's FP32 accuracy (with two parts, 16
+//     bits, trajectories of a 130-row, 128-feature dataset drifted from the
+//     FP64 restatement by 3 % within two sweeps).
+// Each wave takes 16-row tiles.  Lane l holds T[rows 4 (l / 16) + r][chain
+// l % 16] after phase B, which is exactly the B-operand layout of the
+// 16x16x16 product of phase C, so the adjoints stay in registers; X^T comes
+// back from a per-wave LDS copy of the tile.  Padded chains (>= C) get
+// theta = 0 and a = 0.
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+__device__ inline uint16_t bf16_rne(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+constexpr int kMfmaCols = 16;
+
+template <int D>
+constexpr int mfma_slab_stride() { return D + 8; }  // BF16 elements per LDS row
+
+// Phase-B chunking: 32 features per V_MFMA_F32_16X16X32_BF16 when D allows,
+// else 16 per V_MFMA_F32_16X16X16_BF16; lane (i, kg) supplies features
+// KB s + (KB / 4) kg + e of chunk s.
+template <int KB> struct MfmaB;
+template <> struct MfmaB<32> {
+  typedef bf16x8_t vec;
+  typedef uint4 raw;
+  static __device__ f32x4_t mma(vec a, vec b, f32x4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+  }
+};
+template <> struct MfmaB<16> {
+  typedef bf16x4_t vec;
+  typedef uint2 raw;
+  static __device__ f32x4_t mma(vec a, vec b, f32x4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+  }
+};
+
+template <int D>
+__device__ void bhlr_body_mfma(const BArgs& A, const BDs& ds, unsigned char* smem) {
+  constexpr int KB = (D % 32 == 0) ? 32 : 16;  // features per phase-B product
+  constexpr int KE = KB / 4;                   // features per lane per product
+  constexpr int KC = D / KB;                   // phase-B chunks
+  constexpr int FB = D / 16;                   // 16-feature blocks of phase C
+  constexpr int XS = mfma_slab_stride<D>();
+  typedef typename MfmaB<KB>::vec bvec;
+  typedef typename MfmaB<KB>::raw braw;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int li = lane & 15, kg = lane >> 4;
+  uint16_t* slab = (uint16_t*)smem + (size_t)wv * 16 * XS;
+  float* gred = (float*)(smem + (size_t)4 * 16 * XS * sizeof(uint16_t));  // [4][D][16]
+  double* lred = (double*)(gred + 4 * D * kMfmaCols);                     // [4][16]
+
+  // theta of chain li, features KB s + KE kg + e, as hi / mid / lo parts
+  const bool live = li < A.C;
+  bvec bh[KC], bm[KC], bl[KC];
+#pragma unroll
+  for (int s = 0; s < KC; ++s)
+#pragma unroll
+    for (int e = 0; e < KE; ++e) {
+      const float b = live ? A.beta32[(size_t)li * A.GW + ds.goff + KB * s + KE * kg + e] : 0.0f;
+      const uint16_t h = bf16_rne(b);
+      const float r1 = b - __uint_as_float((uint32_t)h << 16);
+      const uint16_t m = bf16_rne(r1);
+      const float r2 = r1 - __uint_as_float((uint32_t)m << 16);
+      bh[s][e] = __builtin_bit_cast(__bf16, h);
+      bm[s][e] = __builtin_bit_cast(__bf16, m);
+      bl[s][e] = __builtin_bit_cast(__bf16, bf16_rne(r2));
+    }
+  f32x4_t g[FB];
+#pragma unroll
+  for (int f = 0; f < FB; ++f) g[f] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+  double ll = 0.0;
+
+  const int local = blockIdx.x - ds.blk0;
+  const int rbeg = local * A.rows_per_blk;
+  const int rend = min(ds.n, rbeg + A.rows_per_blk);
+  for (int r0 = rbeg + 16 * wv; r0 < rend; r0 += 64) {
+    // phase B: A operand = row r0 + li, features KB s + KE kg .. + KE - 1
+    const int row = r0 + li;
+    bvec xa[KC];
+#pragma unroll
+    for (int s = 0; s < KC; ++s) {
+      braw q = {};
+      if (row < rend) q = *(const braw*)(ds.x + (size_t)row * D + KB * s + KE * kg);
+      *(braw*)(slab + li * XS + KB * s + KE * kg) = q;
+      xa[s] = __builtin_bit_cast(bvec, q);
+    }
+    f32x4_t t = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int s = 0; s < KC; ++s) {
+      t = MfmaB<KB>::mma(xa[s], bh[s], t);
+      t = MfmaB<KB>::mma(xa[s], bm[s], t);
+      t = MfmaB<KB>::mma(xa[s], bl[s], t);
+    }
+    // rows r0 + 4 kg + r, chain li
+    bf16x4_t ah, am, al;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int rr = r0 + 4 * kg + r;
+      const bool ok = rr < rend && live;
+      float a;
+      const float lr = log_sigmoid_pm(t[r], rr < rend ? (int)ds.y[rr] : 0, &a);
+      if (ok) ll += (double)lr;
+      else a = 0.0f;
+      const uint16_t h = bf16_rne(a);
+      const float r1 = a - __uint_as_float((uint32_t)h << 16);
+      const uint16_t m = bf16_rne(r1);
+      ah[r] = __builtin_bit_cast(__bf16, h);
+      am[r] = __builtin_bit_cast(__bf16, m);
+      al[r] = __builtin_bit_cast(__bf16, bf16_rne(r1 - __uint_as_float((uint32_t)m << 16)));
+    }
+    // the wave's tile copy is complete before any lane reads it
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // phase C: A operand = X^T[feature 16 f + li][rows 4 kg + e]
+#pragma unroll
+    for (int f = 0; f < FB; ++f) {
+      bf16x4_t xt;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) xt[e] = __builtin_bit_cast(__bf16, slab[(4 * kg + e) * XS + 16 * f + li]);
+      g[f] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(xt, ah, g[f], 0, 0, 0);
+      g[f] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(xt, am, g[f], 0, 0, 0);
+      g[f] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(xt, al, g[f], 0, 0, 0);
+    }
+    // the next tile's copy overwrites the slab only after these reads
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  // g[f][r] = G[feature 16 f + 4 kg + r][chain li]; ll per (kg, chain li)
+  ll += __shfl_xor(ll, 16);
+  ll += __shfl_xor(ll, 32);
+#pragma unroll
+  for (int f = 0; f < FB; ++f)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) gred[((size_t)wv * D + 16 * f + 4 * kg + r) * kMfmaCols + li] = g[f][r];
+  if (lane < 16) lred[wv * kMfmaCols + lane] = ll;
+  __syncthreads();
+  // partials of this block, waves summed in order: part[poff + value * nblk + local]
+  constexpr int VS = D + 2;
+  double* part = A.part + ds.poff;
+  for (int e = tid; e < A.C * D; e += kBThreads) {
+    const int c = e / D, j = e % D;
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) s += (double)gred[((size_t)w * D + j) * kMfmaCols + c];
+    part[(size_t)(c * VS + j) * ds.nblk + local] = s;
+  }
+  if (tid < A.C) {
+    part[(size_t)(tid * VS + D) * ds.nblk + local] = 0.0;  // no intercept
+    part[(size_t)(tid * VS + D + 1) * ds.nblk + local] =
+        ((lred[tid] + lred[kMfmaCols + tid]) + lred[2 * kMfmaCols + tid]) + lred[3 * kMfmaCols + tid];
+  }
+}
+
+__global__ __launch_bounds__(kBThreads) void k_bhlr_pass_mfma(BArgs A) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const BDs ds = A.ds[A.blk_ds[blockIdx.x]];
+  switch (ds.d) {
+    case 16: bhlr_body_mfma<16>(A, ds, smem); break;
+    case 32: bhlr_body_mfma<32>(A, ds, smem); break;
+    case 64: bhlr_body_mfma<64>(A, ds, smem); break;
+    default: bhlr_body_mfma<128>(A, ds, smem); break;
+  }
+}
+
+size_t bhlr_mfma_smem_bytes(int dmax) {
+  return sizeof(uint16_t) * 4 * 16 * (size_t)(dmax + 8) + sizeof(float) * 4 * (size_t)dmax * kMfmaCols +
+         sizeof(double) * 4 * kMfmaCols;
+}
+
 size_t bhlr_smem_bytes(int dmax, int cp) {
   size_t b = sizeof(float) * ((size_t)kBTile * (dmax + 4) + (size_t)cp * (dmax + 4) + (size_t)kBTile * cp) +
              sizeof(int) * kBTile;
@@ -447,6 +635,8 @@ struct BhlrSampler final : augur_sampler {
   DevBuf<uint32_t> sweep_dev;
   BArgs base{};
   BPassKernel pass = nullptr;
+  size_t pass_smem = 0;
+  bool mfma = false;
   bool cache_valid = false, use_graph = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -520,7 +710,7 @@ struct BhlrSampler final : augur_sampler {
     A.mode = mode;
     A.l = l;
     prof_begin();
-    pass<<<nblk_total, kBThreads, bhlr_smem_bytes(dmax, next_cp()), stream>>>(A);
+    pass<<<nblk_total, kBThreads, pass_smem, stream>>>(A);
     AUGUR_CHECK_LAUNCH();
     prof_end();
     k_bhlr_finish<<<(A.nval + 3) / 4, 256, 0, stream>>>(A);
@@ -558,10 +748,10 @@ struct BhlrSampler final : augur_sampler {
       }
       BArgs A = base;
       A.mode = BMODE_CACHE;
-      pass<<<nblk_total, kBThreads, bhlr_smem_bytes(dmax, next_cp()), stream>>>(A);  // warm
+      pass<<<nblk_total, kBThreads, pass_smem, stream>>>(A);  // warm
       AUGUR_CHECK_LAUNCH();
       AUGUR_HIP(hipEventRecord(e0, stream));
-      for (int r = 0; r < 3; ++r) pass<<<nblk_total, kBThreads, bhlr_smem_bytes(dmax, next_cp()), stream>>>(A);
+      for (int r = 0; r < 3; ++r) pass<<<nblk_total, kBThreads, pass_smem, stream>>>(A);
       AUGUR_CHECK_LAUNCH();
       AUGUR_HIP(hipEventRecord(e1, stream));
       AUGUR_HIP(hipEventSynchronize(e1));
@@ -741,7 +931,13 @@ augur_status augur_bhlr_create(const augur_bhlr_desc* D, augur_sampler** out) {
       A.step_jitter = D->theta_block.fixed_steps < 0;
       A.v_sim_len = D->v_block.sim_len; A.v_eps = D->v_block.step_size; A.v_fixed = D->v_block.fixed_steps;
       A.v_mwg = D->v_block.block == AUGUR_MWG_BLOCK_V;
-      s->pass = pick_bpass(s->next_cp());
+      // matrix cores from 5 chains on (the 16 chain columns of a product are
+      // then at least a third used); AUGUR_BHLR_VECTOR=1 keeps the vector form
+      const char* vec = getenv("AUGUR_BHLR_VECTOR");
+      s->mfma = s->next_cp() >= 8 && !(vec && atoi(vec));
+      s->pass = s->mfma ? k_bhlr_pass_mfma : pick_bpass(s->next_cp());
+      s->pass_smem = s->mfma ? bhlr_mfma_smem_bytes(s->dmax) : bhlr_smem_bytes(s->dmax, s->next_cp());
+      allow_lds(s->pass, s->pass_smem);
       // state starts at theta = 0, v = 1
       std::vector<double> st(C * s->SW, 0.0);
       for (size_t c = 0; c < C; ++c)

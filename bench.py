@@ -19,6 +19,25 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per launch of the headline pass kernel: rocprofv3 --pmc FETCH_SIZE
+# and WRITE_SIZE (KiB per dispatch) on this workload, each scaled by what the
+# counter reads for a plain 256 MiB device copy (`--copy-mib 256`: fetch x2.0,
+# write x1.0); profiles/k_hlr_pass_t_hlr_n100k_d64_c4.md
+PASS_FETCH_KIB, PASS_WRITE_KIB = 12771.3, 1074.6
+FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
+
+
+def copy_probe(mib):
+    """A plain device copy of `mib` MiB, five times: the calibration kernel
+    for the traffic counters."""
+    import torch
+    a = torch.ones(mib * 1024 * 1024 // 4, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"copied 5 x {mib} MiB")
+
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
@@ -258,7 +277,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--only", default="", help="comma list of legs to run besides the headline "
                     "HLR leg: lda, surrogate, batched, gmm (default: all)")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
+    if args.copy_mib:
+        copy_probe(args.copy_mib)
+        return
     legs = set(args.only.split(",")) if args.only else {"lda", "surrogate", "batched", "gmm"}
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the sampler "
@@ -312,7 +336,8 @@ def main():
         "gmm": gmm,
         "roofline": {"kernel": "k_hlr_pass", "bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "avg_us": hlr["pass_avg_us"],
+                     "traffic": 1024.0 * (PASS_FETCH_KIB * FETCH_SCALE + PASS_WRITE_KIB * WRITE_SCALE),
+                     "avg_us": hlr["pass_avg_us"],
                      "launches_timed": hlr["pass_launches_timed"]},
         "cpu_baseline": cpu,
     }

@@ -70,6 +70,31 @@ def test_batched_matches_oracle(rt, oracle, L):
     s.close()
 
 
+def test_batched_matrix_core_pass_matches_oracle(rt, oracle):
+    """12 chains per dataset (padded to 16 matrix-core columns): BF16 GEMM
+    form of the data pass for D = 16, 32, 64, 128, against the oracle."""
+    rng = np.random.default_rng(8)
+    xs, ys = _datasets(rng, SHAPES)
+    C, L, EPS = 12, -6, 0.01
+    s = rt.BhlrSampler(xs, ys, chains=C, seed=23, v_block=(1.0, 0.05, 0), theta_block=(0.1, EPS, L),
+                       rows_per_block=512, use_graph=False)
+    starts = _init(s, rng, C)
+    models = [oracle.Hlr(bf16_round(x).astype(np.float64), y, prior_kind=0, prior_a=1.0,
+                         prior_b=1.0, ref_compat=False) for x, y in zip(xs, ys)]
+    for sweep in range(2):
+        s.step(1)
+        for i in range(len(SHAPES)):
+            for c in range(C):
+                st = starts[i, c]
+                gc = i * C + c
+                models[i].hmc_step(st, 0, 0, 1.0, 0.05, 0, 23, gc, sweep)
+                models[i].hmc_step(st, 2, 1, 0.1, EPS, L, 23, gc, sweep)
+                np.testing.assert_allclose(s.get(f"state:{i}", c), st, rtol=2e-5, atol=1e-7,
+                                           err_msg=f"dataset {i} chain {c} sweep {sweep}")
+    assert s.acceptance()[1].mean() > 0.3
+    s.close()
+
+
 def test_batched_mwg_v_matches_oracle(rt, oracle):
     """v block by Metropolis-within-Gibbs, proposal Normal(v, 0.5), with the
     step-jittered theta block, graph replay on."""
