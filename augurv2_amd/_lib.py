@@ -71,6 +71,7 @@ _P = C.c_void_p
 _U = C.c_uint32
 _I = C.c_int
 _S = C.c_size_t
+_D = C.c_double
 
 # name -> argtypes (all return augur_status unless listed in _RESTYPE)
 SIGNATURES = {
@@ -86,6 +87,8 @@ SIGNATURES = {
     "augur_op_mlp_loss_grad": [_I, _I, _I, _P, _P, _P, _P, _P, _P],
     "augur_op_mlp_train": [_I, _I, _I, _P, _P, _I, _I, C.c_double, _U, _P, _P],
     "augur_bhlr_tuning": [_H, _P, _P, _P, _P, _I],
+    "augur_bhlr_adapt": [_H, _I, _D],
+    "augur_bhlr_step_sizes": [_H, _P, _P, _I],
     "augur_set_pt": [_H, C.c_char_p, _I, _P, _S],
     "augur_cpy": [_H, C.c_char_p, _I, _P, _S],
     "augur_step": [_H, _I],

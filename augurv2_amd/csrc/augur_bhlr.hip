@@ -67,6 +67,13 @@ struct BArgs {
   double v_sim_len, v_eps;  // v block: HMC settings, or the MWG proposal variance
   int v_fixed;
   int v_mwg;
+  // per dataset: theta step size and v proposal variance (A.step / A.v_eps
+  // until warmup adaptation changes them), and the last sweep's acceptance
+  // probabilities [C][n_ds] of the theta and v moves
+  double* eps_ds;
+  double* vvar_ds;
+  double* acc_theta;
+  double* acc_v;
 };
 
 enum { BMODE_CACHE = 0, BMODE_LEAPFROG = 1 };
@@ -240,13 +247,14 @@ __device__ void bhlr_body(const BArgs& A, const BDs& ds, int di, unsigned char* 
   (void)di;
 }
 
-// The theta block's step size of chain key gc this sweep: eps, or with
-// step_jitter eps U(0.8, 1.2) from the block's jitter stream, as the
-// reference draws it (cbits/augur_mcmclib.h:401-402).
+// The theta block's step size of chain key gc (dataset gc / C) this sweep:
+// the dataset's eps, or with step_jitter eps U(0.8, 1.2) from the block's
+// jitter stream, as the reference draws it (cbits/augur_mcmclib.h:401-402).
 __device__ inline double theta_step(const BArgs& A, uint32_t gc) {
-  if (!A.step_jitter) return A.step;
+  const double eps = A.eps_ds[gc / (uint32_t)A.C];
+  if (!A.step_jitter) return eps;
   const Stream js{A.seed, gc, *A.sweep, TAG_HMC_BASE + 4u + 1u};
-  return (0.8 + 0.4 * u01(js.draw(0, 0).x)) * A.step;
+  return (0.8 + 0.4 * u01(js.draw(0, 0).x)) * eps;
 }
 
 // One wave per value (dataset i, chain c, coordinate j; j = d_i + 1: the
@@ -398,17 +406,33 @@ __device__ void bhlr_body_mfma(const BArgs& A, const BDs& ds, unsigned char* sme
   const int local = blockIdx.x - ds.blk0;
   const int rbeg = local * A.rows_per_blk;
   const int rend = min(ds.n, rbeg + A.rows_per_blk);
-  for (int r0 = rbeg + 16 * wv; r0 < rend; r0 += 64) {
-    // phase B: A operand = row r0 + li, features KB s + KE kg .. + KE - 1
-    const int row = r0 + li;
-    bvec xa[KC];
+  // the next tile's operands and labels are loaded while this one computes
+  braw qn[KC];
+  int yn[4];
+  auto load_tile = [&](int t0) {
+    const int row = t0 + li;
 #pragma unroll
     for (int s = 0; s < KC; ++s) {
-      braw q = {};
-      if (row < rend) q = *(const braw*)(ds.x + (size_t)row * D + KB * s + KE * kg);
+      qn[s] = braw{};
+      if (row < rend) qn[s] = *(const braw*)(ds.x + (size_t)row * D + KB * s + KE * kg);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) yn[r] = t0 + 4 * kg + r < rend ? (int)ds.y[t0 + 4 * kg + r] : 0;
+  };
+  if (rbeg + 16 * wv < rend) load_tile(rbeg + 16 * wv);
+  for (int r0 = rbeg + 16 * wv; r0 < rend; r0 += 64) {
+    // phase B: A operand = row r0 + li, features KB s + KE kg .. + KE - 1
+    bvec xa[KC];
+    int yv[4];
+#pragma unroll
+    for (int s = 0; s < KC; ++s) {
+      const braw q = qn[s];
       *(braw*)(slab + li * XS + KB * s + KE * kg) = q;
       xa[s] = __builtin_bit_cast(bvec, q);
     }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) yv[r] = yn[r];
+    if (r0 + 64 < rend) load_tile(r0 + 64);
     f32x4_t t = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int s = 0; s < KC; ++s) {
@@ -423,7 +447,7 @@ __device__ void bhlr_body_mfma(const BArgs& A, const BDs& ds, unsigned char* sme
       const int rr = r0 + 4 * kg + r;
       const bool ok = rr < rend && live;
       float a;
-      const float lr = log_sigmoid_pm(t[r], rr < rend ? (int)ds.y[rr] : 0, &a);
+      const float lr = log_sigmoid_pm(t[r], yv[r], &a);
       if (ok) ll += (double)lr;
       else a = 0.0f;
       const uint16_t h = bf16_rne(a);
@@ -555,7 +579,9 @@ __global__ __launch_bounds__(64) void k_bhmc_end(BArgs A) {
   const double ll0 = A.cache_ll[ci] + lp0, ll1 = A.llprop[ci] + lp1;
   const Stream acc_s{A.seed, gc, *A.sweep, TAG_HMC_BASE + 4u + 2u};
   const double u = u01(acc_s.draw(0, 0).x);
-  const bool acc = log(u) < (ll1 - ll0) + (A.k0[ci] - 0.5 * kin);
+  const double lr = (ll1 - ll0) + (A.k0[ci] - 0.5 * kin);
+  const bool acc = log(u) < lr;
+  if (lane == 0) A.acc_theta[ci] = lr >= 0.0 ? 1.0 : (lr < 0.0 ? exp(lr) : 0.0);  // NaN: 0
   if (acc) {
     for (int j = lane; j < D; j += 64) {
       A.state[si + j] = A.prop[si + j];
@@ -582,10 +608,12 @@ __global__ __launch_bounds__(64) void k_bhmc_vblock(BArgs A) {
   for (int j = lane; j < D; j += 64) S += st[j] * st[j];
   S = wave_sum(S);
   double v;
+  const double eps = A.v_mwg ? A.vvar_ds[di] : A.v_eps;
   const bool acc = v_update(D, S, st[D + 1], A.ref_compat, A.prior_kind, A.prior_a, A.prior_b, A.v_mwg,
-                            A.v_sim_len, A.v_eps, A.v_fixed, A.seed, gc, sw, TAG_HMC_BASE, &v);
+                            A.v_sim_len, eps, A.v_fixed, A.seed, gc, sw, TAG_HMC_BASE, &v);
   if (lane == 0) {
     st[D + 1] = v;
+    A.acc_v[(size_t)c * A.n_ds + di] = acc ? 1.0 : 0.0;
     double* s2 = A.stats + ((size_t)c * A.n_ds + di) * 2;
     s2[0] += acc ? 1.0 : 0.0;
     s2[1] += 1.0;
@@ -593,6 +621,41 @@ __global__ __launch_bounds__(64) void k_bhmc_vblock(BArgs A) {
 }
 
 __global__ void k_bump_sweep(uint32_t* sweep) { *sweep += 1u; }
+
+// Warmup adaptation after each sweep, one thread per dataset:
+//  - theta step: dual averaging of log eps towards a mean acceptance
+//    probability `target` over the dataset's chains (Hoffman & Gelman 2014,
+//    Algorithm 5: gamma = 0.05, t0 = 10, kappa = 0.75, mu = log(10 eps0)),
+//    kept within [eps0 e^-12, eps0 e^3];
+//  - MWG proposal variance: Robbins-Monro, log var += (acceptance - 0.44) m^-0.6.
+// ad[6 di ..] = log eps, log eps_bar, H_bar, mu, m, log eps0.  finalize sets
+// eps to exp(log eps_bar), the value sampling keeps.
+__global__ void k_badapt(BArgs A, double* __restrict__ ad, double target, int finalize) {
+  const int di = blockIdx.x * blockDim.x + threadIdx.x;
+  if (di >= A.n_ds) return;
+  double* s = ad + 6 * (size_t)di;
+  if (finalize) {
+    A.eps_ds[di] = exp(s[1]);
+    return;
+  }
+  double at = 0.0, av = 0.0;
+  for (int c = 0; c < A.C; ++c) {
+    at += A.acc_theta[(size_t)c * A.n_ds + di];
+    av += A.acc_v[(size_t)c * A.n_ds + di];
+  }
+  at /= A.C;
+  av /= A.C;
+  const double m = s[4] + 1.0;
+  s[4] = m;
+  const double w = 1.0 / (m + 10.0);
+  s[2] = (1.0 - w) * s[2] + w * (target - at);
+  s[0] = fmin(fmax(s[3] - sqrt(m) / 0.05 * s[2], s[5] - 12.0), s[5] + 3.0);
+  const double k = pow(m, -0.75);
+  s[1] = k * s[0] + (1.0 - k) * s[1];
+  A.eps_ds[di] = exp(s[0]);
+  if (A.v_mwg)
+    A.vvar_ds[di] = fmin(fmax(exp(log(A.vvar_ds[di]) + (av - 0.44) * pow(m, -0.6)), 1e-10), 1e10);
+}
 
 __global__ void k_bstate_to_beta32(BArgs A) {
   const int di = blockIdx.x, c = blockIdx.y;
@@ -631,6 +694,7 @@ struct BhlrSampler final : augur_sampler {
   DevBuf<int> blk_ds;
   DevBuf<float> beta32;
   DevBuf<double> part, cache_g, cache_ll, gprop, llprop, prop, p, p0, k0, state, stats_buf;
+  DevBuf<double> eps_ds, vvar_ds, acc_theta, acc_v;
   DevBuf<int> val_ds;
   DevBuf<uint32_t> sweep_dev;
   BArgs base{};
@@ -799,6 +863,38 @@ struct BhlrSampler final : augur_sampler {
     }
   }
 
+  // Warmup sweeps that adapt each dataset's theta step size (and the MWG
+  // proposal variance of v) after every sweep, then freeze them.  Runs
+  // without the graph; the graph reads the frozen values from device memory.
+  void adapt(int nsw, double target) {
+    if (nsw <= 0) return;
+    if (!cache_valid) refresh_cache();
+    std::vector<double> e(n_ds), ad(6 * (size_t)n_ds);
+    eps_ds.download(e.data(), e.size(), stream);
+    AUGUR_HIP(hipStreamSynchronize(stream));
+    for (int i = 0; i < n_ds; ++i) {
+      const double l0 = log(e[i]);
+      ad[6 * (size_t)i + 0] = l0;
+      ad[6 * (size_t)i + 1] = 0.0;
+      ad[6 * (size_t)i + 2] = 0.0;
+      ad[6 * (size_t)i + 3] = log(10.0) + l0;
+      ad[6 * (size_t)i + 4] = 0.0;
+      ad[6 * (size_t)i + 5] = l0;
+    }
+    DevBuf<double> dad(ad.size());
+    dad.upload(ad.data(), ad.size(), stream);
+    for (int i = 0; i < nsw; ++i) {
+      enqueue_sweep();
+      k_badapt<<<(n_ds + 63) / 64, 64, 0, stream>>>(base, dad.p, target, 0);
+      AUGUR_CHECK_LAUNCH();
+      ++sweep;
+      record_traces();
+    }
+    k_badapt<<<(n_ds + 63) / 64, 64, 0, stream>>>(base, dad.p, target, 1);
+    AUGUR_CHECK_LAUNCH();
+    AUGUR_HIP(hipStreamSynchronize(stream));  // dad is released on return
+  }
+
   void log_density(double* out) override {
     if (!cache_valid) refresh_cache();
     std::vector<double> ll((size_t)C * n_ds), st((size_t)C * SW);
@@ -931,6 +1027,19 @@ augur_status augur_bhlr_create(const augur_bhlr_desc* D, augur_sampler** out) {
       A.step_jitter = D->theta_block.fixed_steps < 0;
       A.v_sim_len = D->v_block.sim_len; A.v_eps = D->v_block.step_size; A.v_fixed = D->v_block.fixed_steps;
       A.v_mwg = D->v_block.block == AUGUR_MWG_BLOCK_V;
+      {
+        const std::vector<double> e(s->n_ds, D->theta_block.step_size), vv(s->n_ds, D->v_block.step_size);
+        s->eps_ds.alloc(s->n_ds);
+        s->vvar_ds.alloc(s->n_ds);
+        s->eps_ds.upload(e.data(), e.size(), s->stream);
+        s->vvar_ds.upload(vv.data(), vv.size(), s->stream);
+        s->acc_theta.alloc((size_t)C * s->n_ds);
+        s->acc_v.alloc((size_t)C * s->n_ds);
+        s->acc_theta.zero(s->stream);
+        s->acc_v.zero(s->stream);
+        AUGUR_HIP(hipStreamSynchronize(s->stream));  // e and vv are released at the block's end
+        A.eps_ds = s->eps_ds.p; A.vvar_ds = s->vvar_ds.p; A.acc_theta = s->acc_theta.p; A.acc_v = s->acc_v.p;
+      }
       // matrix cores from 5 chains on (the 16 chain columns of a product are
       // then at least a third used); AUGUR_BHLR_VECTOR=1 keeps the vector form
       const char* vec = getenv("AUGUR_BHLR_VECTOR");
@@ -965,6 +1074,29 @@ augur_status augur_bhlr_tuning(augur_sampler* s, int* rows_per_block, int* n_can
       if (cand_rows) cand_rows[i] = b->tune_rows[i];
       if (cand_us) cand_us[i] = b->tune_us[i];
     }
+  });
+}
+
+augur_status augur_bhlr_adapt(augur_sampler* s, int n_sweeps, double target_accept) {
+  return guard([&] {
+    auto* b = dynamic_cast<BhlrSampler*>(s);
+    if (!b) fail(AUGUR_ERR_ARG, "not a batched HLR sampler");
+    if (n_sweeps < 0 || !(target_accept > 0.0 && target_accept < 1.0))
+      fail(AUGUR_ERR_ARG, "need n_sweeps >= 0 and 0 < target_accept < 1");
+    AUGUR_HIP(hipSetDevice(b->device));
+    b->adapt(n_sweeps, target_accept);
+  });
+}
+
+augur_status augur_bhlr_step_sizes(augur_sampler* s, double* eps, double* v_var, int n) {
+  return guard([&] {
+    auto* b = dynamic_cast<BhlrSampler*>(s);
+    if (!b) fail(AUGUR_ERR_ARG, "not a batched HLR sampler");
+    if (n < b->n_ds) fail(AUGUR_ERR_ARG, "need room for %d datasets", b->n_ds);
+    AUGUR_HIP(hipSetDevice(b->device));
+    if (eps) b->eps_ds.download(eps, b->n_ds, b->stream);
+    if (v_var) b->vvar_ds.download(v_var, b->n_ds, b->stream);
+    AUGUR_HIP(hipStreamSynchronize(b->stream));
   });
 }
 

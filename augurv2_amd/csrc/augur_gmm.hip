@@ -53,77 +53,116 @@ struct GmmDev {
 };
 
 // ---- small FP64 linear algebra (cbits/augur_matop.c:312-406), row-major ----
+// Wave-parallel: one 64-lane block holds the matrices in LDS and every
+// element is computed with the serial definition's operations in the same
+// order (terms the serial in-place loops multiply by an exact zero are left
+// out), so the results equal the serial forms bit for bit.  Callers keep
+// the whole block in the call; `tmp` is d x d scratch.
+constexpr int kConjLanes = 64;
+
+// Cholesky factor in the lower triangle; the upper triangle is left as is.
 __device__ void chol(double* a, int n) {
-  for (int i = 0; i < n; ++i)
-    for (int j = 0; j <= i; ++j) {
+  const int t = threadIdx.x;
+  for (int j = 0; j < n; ++j) {
+    if (t == 0) {
+      double s = 0.0;
+      for (int k = 0; k < j; ++k) s += a[j * n + k] * a[j * n + k];
+      a[j * n + j] = sqrt(a[j * n + j] - s);
+    }
+    __syncthreads();
+    for (int i = j + 1 + t; i < n; i += kConjLanes) {
       double s = 0.0;
       for (int k = 0; k < j; ++k) s += a[i * n + k] * a[j * n + k];
-      if (i == j) a[i * n + i] = sqrt(a[i * n + i] - s);
-      else a[i * n + j] = (a[i * n + j] - s) / a[j * n + j];
+      a[i * n + j] = (a[i * n + j] - s) / a[j * n + j];
     }
+    __syncthreads();
+  }
 }
-__device__ void lower_trmi(double* l, int n) {
-  for (int c = 0; c < n; ++c)
+// Inverse of the lower triangle in place (upper set to zero): column c by
+// forward substitution, one lane per column.
+__device__ void lower_trmi(double* l, int n, double* tmp) {
+  for (int c = threadIdx.x; c < n; c += kConjLanes)
     for (int m = 0; m < n; ++m) {
+      if (m < c) {
+        tmp[m * n + c] = 0.0;
+        continue;
+      }
       double b = (c == m) ? 1.0 : 0.0;
-      for (int i = 0; i < m; ++i) b -= l[m * n + i] * l[i * n + c];
-      l[m * n + c] = b / l[m * n + m];
+      for (int i = c; i < m; ++i) b -= l[m * n + i] * tmp[i * n + c];
+      tmp[m * n + c] = b / l[m * n + m];
     }
+  __syncthreads();
+  for (int e = threadIdx.x; e < n * n; e += kConjLanes) l[e] = tmp[e];
+  __syncthreads();
 }
-__device__ void chol_invert(double* a, int n) {
-  lower_trmi(a, n);
-  for (int c = 0; c < n; ++c)
-    for (int r = c; r < n; ++r) {
-      double acc = 0.0;
-      for (int k = c; k < n; ++k) acc += a[k * n + r] * a[k * n + c];
-      a[r * n + c] = acc;
-    }
-  for (int c = 0; c < n; ++c)
-    for (int r = c + 1; r < n; ++r) a[c * n + r] = a[r * n + c];
+// From a Cholesky factor in the lower triangle to the full inverse matrix.
+__device__ void chol_invert(double* a, int n, double* tmp) {
+  lower_trmi(a, n, tmp);
+  for (int e = threadIdx.x; e < n * n; e += kConjLanes) {
+    const int r = e / n, c = e % n;
+    if (r < c) continue;
+    double acc = 0.0;
+    for (int k = c; k < n; ++k) acc += a[k * n + r] * a[k * n + c];
+    tmp[r * n + c] = acc;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < n * n; e += kConjLanes) {
+    const int r = e / n, c = e % n;
+    a[e] = r >= c ? tmp[r * n + c] : tmp[c * n + r];
+  }
+  __syncthreads();
 }
-__device__ void trtrmm(const double* l1, double* l2, int n) {
-  for (int i = n - 1; i >= 0; --i)
-    for (int j = 0; j < n; ++j) {
-      double t = 0.0;
-      for (int k = 0; k < i; ++k) t += l1[i * n + k] * l2[k * n + j];
-      t += l1[i * n + i] * l2[i * n + j];
-      l2[i * n + j] = t;
-    }
+// l2 = l1 l2 for lower triangular l1 (cbits/augur_matop.c:381-392).
+__device__ void trtrmm(const double* l1, double* l2, int n, double* tmp) {
+  for (int e = threadIdx.x; e < n * n; e += kConjLanes) {
+    const int i = e / n, j = e % n;
+    double t = 0.0;
+    for (int k = 0; k < i; ++k) t += l1[i * n + k] * l2[k * n + j];
+    t += l1[i * n + i] * l2[i * n + j];
+    tmp[e] = t;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < n * n; e += kConjLanes) l2[e] = tmp[e];
+  __syncthreads();
 }
 
-// z-step table of one component from (pis_k, mu_k, cov_k).
-__device__ void build_table(const GmmDev& m, int c, int k, double logpi_unused, double* work) {
+// z-step table of one component from (pis_k, mu_k, cov_k); work and tmp are
+// d x d LDS scratch of the calling 64-lane block.
+__device__ void build_table(const GmmDev& m, int c, int k, double* work, double* tmp) {
   const int d = m.d, DM = m.DM;
   const double* cov = m.cov64 + ((size_t)c * m.k + k) * d * d;
-  for (int i = 0; i < d * d; ++i) work[i] = cov[i];
+  for (int i = threadIdx.x; i < d * d; i += kConjLanes) work[i] = cov[i];
+  __syncthreads();
   chol(work, d);
   double ldet = 0.0;
   for (int i = 0; i < d; ++i) ldet += log(work[i * d + i]);
-  lower_trmi(work, d);
+  lower_trmi(work, d, tmp);
   float* t = m.tab + ((size_t)c * m.k + k) * m.TS;
-  for (int i = 0; i < DM; ++i)
-    for (int j = 0; j <= i; ++j) t[tri(i) + j] = (i < d && j < d) ? (float)work[i * d + j] : 0.0f;
+  for (int e = threadIdx.x; e < DM * DM; e += kConjLanes) {
+    const int i = e / DM, j = e % DM;
+    if (j <= i) t[tri(i) + j] = (i < d && j < d) ? (float)work[i * d + j] : 0.0f;
+  }
   const double* mu = m.mu64 + ((size_t)c * m.k + k) * d;
-  for (int i = 0; i < DM; ++i) t[tri(DM) + i] = i < d ? (float)mu[i] : 0.0f;
-  t[tri(DM) + DM] = (float)(-ldet);  // -0.5 log det cov
+  for (int i = threadIdx.x; i < DM; i += kConjLanes) t[tri(DM) + i] = i < d ? (float)mu[i] : 0.0f;
+  if (threadIdx.x == 0) t[tri(DM) + DM] = (float)(-ldet);  // -0.5 log det cov
 }
 
-// One block per (component k, chain c).
-__global__ __launch_bounds__(kGmmThreads) void k_gmm_conj(GmmDev m, uint32_t seed, uint32_t sweep,
-                                                          int use_stats) {
+// One 64-lane block per (component k, chain c): statistics summed over the z
+// blocks, then pi_k (Gamma), mu_k | cov_k (MvNormal conjugate) and cov_k |
+// mu_k (InvWishart) and the new z-step table.
+__global__ __launch_bounds__(kConjLanes) void k_gmm_conj(GmmDev m, uint32_t seed, uint32_t sweep,
+                                                         int use_stats) {
   const int k = blockIdx.x, c = blockIdx.y, tid = threadIdx.x;
   const int d = m.d, S = m.S;
   __shared__ double st[1 + kMaxDim + kMaxDim * kMaxDim];
-  __shared__ double work[5 * kMaxDim * kMaxDim + 4 * kMaxDim];
-  if (use_stats) {
-    for (int i = tid; i < S; i += kGmmThreads) {
-      double s = 0.0;
-      for (int g = 0; g < m.G; ++g) s += (double)m.part[(((size_t)c * m.G + g) * m.k + k) * S + i];
-      st[i] = s;
-    }
+  __shared__ double work[6 * kMaxDim * kMaxDim + 4 * kMaxDim];
+  if (!use_stats) return;
+  for (int i = tid; i < S; i += kConjLanes) {
+    double s = 0.0;
+    for (int g = 0; g < m.G; ++g) s += (double)m.part[(((size_t)c * m.G + g) * m.k + k) * S + i];
+    st[i] = s;
   }
   __syncthreads();
-  if (tid != 0 || !use_stats) return;
   // unpack: st[0] count, st[1..d] sum (y - centre), packed lower S after
   const int cnt = (int)llrint(st[0]);
   double* sc = work;                     // d
@@ -135,97 +174,108 @@ __global__ __launch_bounds__(kGmmThreads) void k_gmm_conj(GmmDev m, uint32_t see
   double* mean = v + kMaxDim;            // d
   double* munew = mean + kMaxDim;        // d
   double* B = munew + kMaxDim;           // d x d
-  for (int i = 0; i < d; ++i) sc[i] = st[1 + i];
-  for (int i = 0; i < d; ++i)
-    for (int j = 0; j <= i; ++j) {
-      const double val = st[1 + d + tri(i) + j];
-      Sc[i * d + j] = val;
-      Sc[j * d + i] = val;
-    }
+  double* tmp = B + kMaxDim * kMaxDim;   // d x d
+  for (int i = tid; i < d; i += kConjLanes) sc[i] = st[1 + i];
+  for (int e = tid; e < d * d; e += kConjLanes) {
+    const int i = e / d, j = e % d;
+    Sc[e] = j <= i ? st[1 + d + tri(i) + j] : st[1 + d + tri(j) + i];
+  }
   double* mu = m.mu64 + ((size_t)c * m.k + k) * d;  // current mu = centre
   double* cov = m.cov64 + ((size_t)c * m.k + k) * d * d;
-  if (m.sample_pi) {
+  if (m.sample_pi && tid == 0) {
     const Stream sp{seed, (uint32_t)c, sweep, TAG_GMM_PI};
     m.gpi[(size_t)c * m.k + k] = gamma_at(sp, (uint32_t)k, m.alpha[k] + (double)cnt);
   }
   // ---- mu_k | cov_k: cbits/augur_dist.c:633-669 ----
-  for (int i = 0; i < d * d; ++i) L0[i] = m.cov0[i];
+  for (int e = tid; e < d * d; e += kConjLanes) {
+    L0[e] = m.cov0[e];
+    L[e] = cov[e];
+  }
+  __syncthreads();
   chol(L0, d);
-  chol_invert(L0, d);  // cov0^-1
-  for (int i = 0; i < d * d; ++i) L[i] = cov[i];
+  chol_invert(L0, d, tmp);  // cov0^-1
   chol(L, d);
-  chol_invert(L, d);
-  for (int i = 0; i < d * d; ++i) L[i] *= (double)cnt;  // n cov^-1
-  for (int i = 0; i < d; ++i) {
+  chol_invert(L, d, tmp);
+  for (int e = tid; e < d * d; e += kConjLanes) L[e] *= (double)cnt;  // n cov^-1
+  // the z step centred its statistics at the FP32 table copy of mu: the mean
+  // of the assigned points is centre + s / n (the sum itself when n = 0)
+  const double inv = cnt == 0 ? 1.0 : 1.0 / (double)cnt;
+  for (int i = tid; i < d; i += kConjLanes) {
     double t = 0.0;
     for (int j = 0; j < d; ++j) t += L0[i * d + j] * m.mu0[j];
     v[i] = t;
+    mean[i] = ((double)cnt * (double)(float)mu[i] + sc[i]) * inv;
   }
-  // the z step centred its statistics at the FP32 table copy of mu
-  double cen[kMaxDim];
-  for (int i = 0; i < d; ++i) cen[i] = (double)(float)mu[i];
-  // mean of the assigned points: centre + s / n (the sum itself when n = 0)
-  const double inv = cnt == 0 ? 1.0 : 1.0 / (double)cnt;
-  for (int i = 0; i < d; ++i) mean[i] = ((double)cnt * cen[i] + sc[i]) * inv;
-  for (int i = 0; i < d; ++i) {
+  __syncthreads();
+  for (int i = tid; i < d; i += kConjLanes) {
     double t = 0.0;
     for (int j = 0; j < d; ++j) t += L[i * d + j] * mean[j];
     v[i] += t;
   }
-  for (int i = 0; i < d * d; ++i) L0[i] += L[i];
+  __syncthreads();
+  for (int e = tid; e < d * d; e += kConjLanes) L0[e] += L[e];
+  __syncthreads();
   chol(L0, d);
-  chol_invert(L0, d);  // A = (cov0^-1 + n cov^-1)^-1
-  for (int i = 0; i < d; ++i) {
-    double t = 0.0;
-    for (int j = 0; j < d; ++j) t += L0[i * d + j] * v[j];
-    mean[i] = t;
-  }
+  chol_invert(L0, d, tmp);  // A = (cov0^-1 + n cov^-1)^-1
   {
     const Stream sm{seed, (uint32_t)c, sweep, TAG_GMM_MU};
-    for (int i = 0; i < d; ++i) munew[i] = normal_at(sm, (uint32_t)(k * d + i), 0);
-    for (int i = 0; i < d * d; ++i) L[i] = L0[i];
+    for (int i = tid; i < d; i += kConjLanes) {
+      double t = 0.0;
+      for (int j = 0; j < d; ++j) t += L0[i * d + j] * v[j];
+      mean[i] = t;
+      munew[i] = normal_at(sm, (uint32_t)(k * d + i), 0);
+    }
+    for (int e = tid; e < d * d; e += kConjLanes) L[e] = L0[e];
+    __syncthreads();
     chol(L, d);
-    for (int i = d - 1; i >= 0; --i) {  // trvm
+    for (int i = tid; i < d; i += kConjLanes) {  // trvm
       double t = 0.0;
       for (int j = 0; j <= i; ++j) t += L[i * d + j] * munew[j];
-      munew[i] = t;
+      tmp[i] = t + mean[i];
     }
-    for (int i = 0; i < d; ++i) munew[i] += mean[i];
+    __syncthreads();
+    for (int i = tid; i < d; i += kConjLanes) munew[i] = tmp[i];
+    __syncthreads();
   }
   // ---- cov_k | mu_k: IW(df + n, scale0 + sum (y - mu)(y - mu)^T) ----
   if (m.sample_cov) {
     double* dl = v;  // d = centre - mu_new
-    for (int i = 0; i < d; ++i) dl[i] = cen[i] - munew[i];
-    for (int i = 0; i < d; ++i)
-      for (int j = 0; j < d; ++j)
-        A[i * d + j] = Sc[i * d + j] + sc[i] * dl[j] + dl[i] * sc[j] + (double)cnt * dl[i] * dl[j] +
-                       m.scale0[i * d + j];
+    for (int i = tid; i < d; i += kConjLanes) dl[i] = (double)(float)mu[i] - munew[i];
+    __syncthreads();
+    for (int e = tid; e < d * d; e += kConjLanes) {
+      const int i = e / d, j = e % d;
+      A[e] = Sc[e] + sc[i] * dl[j] + dl[i] * sc[j] + (double)cnt * dl[i] * dl[j] + m.scale0[e];
+    }
+    __syncthreads();
     // augur_invwishart_sample (cbits/augur_dist.c:740-749)
     chol(A, d);
-    chol_invert(A, d);
+    chol_invert(A, d, tmp);
     chol(A, d);
     const Stream sc_s{seed, (uint32_t)c, sweep, TAG_GMM_COV};
     const int dfn = m.df + cnt;
-    for (int i = 0; i < d; ++i)
-      for (int j = 0; j < d; ++j) {
-        const uint32_t site = (uint32_t)(k * d * d + i * d + j);
-        if (j < i) B[i * d + j] = normal_at(sc_s, site, 0);
-        else if (j > i) B[i * d + j] = 0.0;
-        else B[i * d + j] = sqrt(2.0 * gamma_at(sc_s, site, 0.5 * (double)(dfn - i)));
-      }
-    trtrmm(A, B, d);
-    chol_invert(B, d);
-    for (int i = 0; i < d * d; ++i) cov[i] = B[i];
+    for (int e = tid; e < d * d; e += kConjLanes) {
+      const int i = e / d, j = e % d;
+      const uint32_t site = (uint32_t)(k * d * d + e);
+      if (j < i) B[e] = normal_at(sc_s, site, 0);
+      else if (j > i) B[e] = 0.0;
+      else B[e] = sqrt(2.0 * gamma_at(sc_s, site, 0.5 * (double)(dfn - i)));
+    }
+    __syncthreads();
+    trtrmm(A, B, d, tmp);
+    chol_invert(B, d, tmp);
+    for (int e = tid; e < d * d; e += kConjLanes) cov[e] = B[e];
   }
-  for (int i = 0; i < d; ++i) mu[i] = munew[i];
-  build_table(m, c, k, 0.0, L);
+  __syncthreads();
+  for (int i = tid; i < d; i += kConjLanes) mu[i] = munew[i];
+  __syncthreads();
+  build_table(m, c, k, L, tmp);
 }
 
 // Tables from the current parameters (after set_pt).
-__global__ void k_gmm_tables(GmmDev m) {
+__global__ __launch_bounds__(kConjLanes) void k_gmm_tables(GmmDev m) {
   const int k = blockIdx.x, c = blockIdx.y;
-  __shared__ double work[kMaxDim * kMaxDim];
-  if (threadIdx.x == 0) build_table(m, c, k, 0.0, work);
+  __shared__ double work[kMaxDim * kMaxDim], tmp[kMaxDim * kMaxDim];
+  build_table(m, c, k, work, tmp);
 }
 
 // DM: padded dimension, KM: maximum number of components held in registers.
@@ -422,7 +472,7 @@ struct GmmSampler final : augur_sampler {
   }
 
   void refresh_tables() {
-    k_gmm_tables<<<dim3(m.k, chains), 64, 0, stream>>>(m);
+    k_gmm_tables<<<dim3(m.k, chains), kConjLanes, 0, stream>>>(m);
     AUGUR_CHECK_LAUNCH();
   }
 
@@ -450,7 +500,7 @@ struct GmmSampler final : augur_sampler {
     const ZKernel zk = pick_z(m.DM, pad_comp(m.k));
     for (int i = 0; i < nsw; ++i) {
       if (!stats_valid) refresh_stats();
-      k_gmm_conj<<<dim3(m.k, chains), kGmmThreads, 0, stream>>>(m, seed, sweep, 1);
+      k_gmm_conj<<<dim3(m.k, chains), kConjLanes, 0, stream>>>(m, seed, sweep, 1);
       AUGUR_CHECK_LAUNCH();
       prof_begin();
       zk<<<dim3(m.G, chains), kGmmThreads, z_smem(), stream>>>(m, seed, sweep, 0u, 1);
@@ -619,7 +669,7 @@ augur_status augur_op_gmm_conj(int d, int k, const double* alpha, const double* 
         }
     }
     s.part.upload(st.data(), st.size(), s.stream);
-    k_gmm_conj<<<dim3(k, 1), kGmmThreads, 0, s.stream>>>(m, seed, sweep, 1);
+    k_gmm_conj<<<dim3(k, 1), kConjLanes, 0, s.stream>>>(m, seed, sweep, 1);
     AUGUR_CHECK_LAUNCH();
     std::vector<double> g(k);
     s.gpi.download(g.data(), k, s.stream);

@@ -70,43 +70,61 @@ struct PassArgs {
 // (chain c, coordinate j; j = d: intercept, j = d + 1: log-density); the
 // partials are part[block][value].  A 1024-thread block takes kFinVals
 // consecutive values: thread (group g, value i) sums rows g, g + kFinGroups,
-// ... of its column (each group reads contiguous row segments), then thread
-// i adds the group sums in order: the result is deterministic.  The update of
-// coordinate j needs only value j, so every value finishes independently.
-// (A separate kernel instead of a last-block epilogue: the device-scope
-// fence each pass block would need writes back the L2 and serialises across
-// blocks.)
-constexpr int kFinVals = 16;
-constexpr int kFinGroups = 64;
+// ... of its column (each group reads contiguous row segments); the groups
+// are then summed by a butterfly within each wave and the waves in order:
+// the result is deterministic.  The update of coordinate j needs only value
+// j, so every value finishes independently.  (A separate kernel instead of
+// a last-block epilogue: the device-scope fence each pass block would need
+// writes back the L2 and serialises across blocks.)
+constexpr int kFinVals = 8;
+constexpr int kFinGroups = 1024 / kFinVals;
+constexpr int kFinLoads = 8;  // loads in flight per thread: one round up to G = 1024
 
 __global__ __launch_bounds__(1024) void k_hlr_finish(PassArgs A) {
-  __shared__ double red[kFinGroups][kFinVals + 1];
+  __shared__ double red[1024 / 64][kFinVals];
   const int d = A.d, ds = d + 1, vs = d + 2, P = d + 2;
   const int nval = A.cp * vs;
   const int vi = threadIdx.x % kFinVals, grp = threadIdx.x / kFinVals;
   const int v0 = blockIdx.x * kFinVals;
+  const int v = v0 + vi;
+  const int c = v / vs, j = v % vs;
+  // the update's operands are requested with the partials: one memory round
+  const bool upd = grp == 0 && v < nval && A.mode != MODE_CACHE && c < A.c_real && j <= d;
+  double var = 1.0, xv = 0.0, p_old = 0.0;
+  if (upd) {
+    var = A.state[c * P + d + 1];
+    xv = A.prop[c * P + j];
+    p_old = A.p[c * ds + j];
+  }
   double acc = 0.0;
-  if (v0 + vi < nval) {
-    // eight independent loads in flight per round, summed in a fixed order
-    const double* col = A.part + v0 + vi;
-    for (int g0 = grp; g0 < A.G; g0 += 8 * kFinGroups) {
-      double t[8];
+  if (v < nval) {
+    // kFinLoads independent loads per round, summed in a fixed order
+    const double* col = A.part + v;
+    for (int g0 = grp; g0 < A.G; g0 += kFinLoads * kFinGroups) {
+      double t[kFinLoads];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) {
+      for (int k = 0; k < kFinLoads; ++k) {
         const int g = g0 + k * kFinGroups;
         t[k] = g < A.G ? col[(size_t)g * nval] : 0.0;
       }
-      acc += ((t[0] + t[1]) + (t[2] + t[3])) + ((t[4] + t[5]) + (t[6] + t[7]));
+#pragma unroll
+      for (int w = kFinLoads / 2; w >= 1; w >>= 1)
+#pragma unroll
+        for (int k = 0; k < w; ++k) t[k] += t[k + w];
+      acc += t[0];
     }
   }
-  red[grp][vi] = acc;
+  // lanes l and l ^ (kFinVals 2^k) hold the same value
+#pragma unroll
+  for (int off = 32; off >= kFinVals; off >>= 1) acc += __shfl_xor(acc, off);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane < kFinVals) red[wv][lane] = acc;
   __syncthreads();
   if (threadIdx.x >= kFinVals) return;
-  const int v = v0 + threadIdx.x;
   if (v >= nval) return;
   double s = 0.0;
-  for (int k = 0; k < kFinGroups; ++k) s += red[k][threadIdx.x];
-  const int c = v / vs, j = v % vs;
+#pragma unroll
+  for (int w = 0; w < 1024 / 64; ++w) s += red[w][threadIdx.x];
   if (A.mode == MODE_CACHE) {
     if (j <= d) A.cache_g[c * ds + j] = s;
     else A.cache_ll[c] = s;
@@ -129,11 +147,9 @@ __global__ __launch_bounds__(1024) void k_hlr_finish(PassArgs A) {
   const bool in_blk = (j < d) ? A.blk_theta : (A.blk_b && A.has_b);
   if (!in_blk) return;
   const double step = A.hs.step[c];
-  const double var = A.state[c * P + d + 1];
-  double xv = A.prop[c * P + j];
   const double g = s + (-xv / var);
   const double coef = (A.l == L) ? 0.5 * step : step;
-  const double pj = A.p[q] + coef * g;
+  const double pj = p_old + coef * g;
   A.p[q] = pj;
   if (A.l < L) {
     xv += step * pj;
@@ -301,27 +317,11 @@ __global__ __launch_bounds__(kPassThreads, kPassBlocksPerCU) void k_hlr_pass_t(P
   const int G = gridDim.x;
   const int n = A.n;
   const int tiles = (n + TR - 1) / TR;
-  for (int i = tid; i < CP * (D + 1); i += kPassThreads) bs[(i / (D + 1)) * DS + i % (D + 1)] = A.beta32[i];
-  __syncthreads();
-
-  const int my_c = tid % CP;  // phase-B chain of this thread
-  float breg[BREG ? D : 1];
-  if constexpr (BREG) {
-#pragma unroll
-    for (int j = 0; j < D; ++j) breg[j] = bs[my_c * DS + j];
-  }
-  const float b_c = A.has_b ? bs[my_c * DS + D] : 0.0f;
-
-  double gacc[NS][4];
-#pragma unroll
-  for (int s = 0; s < NS; ++s)
-#pragma unroll
-    for (int k = 0; k < 4; ++k) gacc[s][k] = 0.0;
-  double llacc = 0.0, gbacc = 0.0;
 
   // a ring of PD tiles in flight: this block's tiles blockIdx.x + k G are
   // loaded PD ahead of their use (all of them up front when the block has
-  // at most PD), so the pass waits on memory about once, not once per tile
+  // at most PD), so the pass waits on memory about once, not once per tile.
+  // The first tiles are requested before theta, so the two reads overlap.
   float4 pre[PD][NVT];
   int ypre[PD];
   auto prefetch = [&](int slot_tile, float4 (&dst)[NVT], int& ydst) {
@@ -340,6 +340,24 @@ __global__ __launch_bounds__(kPassThreads, kPassBlocksPerCU) void k_hlr_pass_t(P
     const int t = blockIdx.x + k * G;
     if (t < tiles) prefetch(t, pre[k], ypre[k]);
   }
+
+  for (int i = tid; i < CP * (D + 1); i += kPassThreads) bs[(i / (D + 1)) * DS + i % (D + 1)] = A.beta32[i];
+  __syncthreads();
+
+  const int my_c = tid % CP;  // phase-B chain of this thread
+  float breg[BREG ? D : 1];
+  if constexpr (BREG) {
+#pragma unroll
+    for (int j = 0; j < D; ++j) breg[j] = bs[my_c * DS + j];
+  }
+  const float b_c = A.has_b ? bs[my_c * DS + D] : 0.0f;
+
+  double gacc[NS][4];
+#pragma unroll
+  for (int s = 0; s < NS; ++s)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) gacc[s][k] = 0.0;
+  double llacc = 0.0, gbacc = 0.0;
 
   for (int base = blockIdx.x; base < tiles; base += PD * G) {
 #pragma unroll
@@ -441,20 +459,233 @@ __global__ __launch_bounds__(kPassThreads, kPassBlocksPerCU) void k_hlr_pass_t(P
       for (int k = 0; k < 4; ++k) pb[c * VS + 4 * jq + k] = gacc[s][k];
     }
   }
-  red[tid] = gbacc;
-  red[kPassThreads + tid] = llacc;
+  // intercept gradient and ll per chain: lanes l = c (mod CP) of each wave
+  // summed by a butterfly, then the four waves in order
+#pragma unroll
+  for (int off = 32; off >= CP; off >>= 1) {
+    gbacc += __shfl_xor(gbacc, off);
+    llacc += __shfl_xor(llacc, off);
+  }
+  const int lane = tid & 63, wv = tid >> 6;
+  if (lane < CP) {
+    red[wv * CP + lane] = gbacc;
+    red[kPassThreads + wv * CP + lane] = llacc;
+  }
   __syncthreads();
   if (tid < CP) {
+    constexpr int NW = kPassThreads / 64;
     double sb = 0.0, sl = 0.0;
-    for (int i = tid; i < kPassThreads; i += CP) {
-      sb += red[i];
-      sl += red[kPassThreads + i];
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      sb += red[w * CP + tid];
+      sl += red[kPassThreads + w * CP + tid];
     }
     pb[tid * VS + D] = sb;
     pb[tid * VS + D + 1] = sl;
   }
   (void)flag;
 }
+
+// Register-resident form of the data pass for D = 16, 32, 64 and up to
+// D / 8 chains: no LDS staging of X and no block barrier in the row loop.
+// Lane l owns 8 consecutive features (8 (l % LPR) ..) of RPT rows per tile,
+// LPR = D / 8 lanes per row, so one tile is a contiguous 16 KB of X read as
+// 32 bytes per lane.
+//  - phase B: each lane forms its 8-feature share of x . theta_c for its rows
+//    and chains; a butterfly over the row's lanes (DPP: quad permutations and
+//    the half-row mirror) completes every dot product in every lane;
+//  - the RPT * CP (row, chain) adjoints of a row group are computed once
+//    each: lane q takes pairs q, q + LPR, ... (its chain is q % CP) and the
+//    row's lanes gather them back with ds_bpermute;
+//  - phase C: a * x accumulates into the lane's CP x 8 gradient share in FP32
+//    registers; the block sums the lanes in FP64 at the end (LDS, fixed order).
+// theta is requested before the first tiles, so waiting for it never waits on
+// X, and each wave starts on its first tile while the later ones arrive.
+constexpr int kRowFeat = 8;  // features per lane in the register form
+// rows per lane per tile and tiles in flight: 32 KB of X per block either way
+__host__ __device__ constexpr int row_rpt(int cp) { return cp >= 4 ? 1 : 2; }
+__host__ __device__ constexpr int row_ring(int cp) { return cp >= 4 ? 4 : 2; }
+
+__device__ inline float f4c(const float4& v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w; }
+
+template <int CTRL>
+__device__ inline float dpp_f(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+// sum over the LPR lanes of a row (LPR = 2, 4 or 8, aligned groups)
+template <int LPR>
+__device__ inline float row_sum(float v) {
+  v += dpp_f<0xB1>(v);                     // lanes i ^ 1 (quad_perm 1,0,3,2)
+  if constexpr (LPR >= 4) v += dpp_f<0x4E>(v);   // lanes i ^ 2 (quad_perm 2,3,0,1)
+  if constexpr (LPR >= 8) v += dpp_f<0x141>(v);  // half-row mirror: quad 0 <-> quad 1
+  return v;
+}
+
+template <int D, int CP>
+__global__ __launch_bounds__(kPassThreads, kPassBlocksPerCU) void k_hlr_pass_r(PassArgs A) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int F = kRowFeat, RPT = row_rpt(CP);
+  constexpr int LPR = D / F;                      // lanes per row
+  constexpr int RG = kPassThreads / LPR;          // rows per block per row slot
+  constexpr int TRr = RG * RPT;                   // rows per tile
+  constexpr int NC = RPT * CP;                    // (row slot, chain) pairs
+  constexpr int SPL = (NC + LPR - 1) / LPR;       // pairs per lane
+  constexpr int PD = row_ring(CP);                // tiles in flight
+  constexpr int GS = CP * F + 1;                  // LDS floats per lane (padded)
+  static_assert(D % F == 0 && LPR >= 2 && LPR <= 8 && LPR % CP == 0, "shape");
+
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int q = tid % LPR;          // feature group: features F q .. F q + F - 1
+  const int rsub = tid / LPR;       // row within a row slot
+  const int gbase = lane - q;       // first lane of this row's group
+  const int my_c = q % CP;          // chain of every pair this lane evaluates
+  const int G = gridDim.x, n = A.n;
+  const int tiles = (n + TRr - 1) / TRr;
+
+  // theta share of this lane (features F q .., every chain) and b of my_c
+  float th[CP][F];
+#pragma unroll
+  for (int c = 0; c < CP; ++c)
+#pragma unroll
+    for (int j = 0; j < F; ++j) th[c][j] = A.beta32[c * (D + 1) + F * q + j];
+  const float b_c = A.has_b ? A.beta32[my_c * (D + 1) + D] : 0.0f;
+
+  float4 pre[PD][RPT][2];
+  int ypre[PD][RPT];
+  auto prefetch = [&](int tile, float4 (&dst)[RPT][2], int (&yd)[RPT]) {
+#pragma unroll
+    for (int i = 0; i < RPT; ++i) {
+      const int row = tile * TRr + i * RG + rsub;
+      if (row < n) {
+        const float4* s4 = (const float4*)(A.x + (size_t)row * D + F * q);
+        dst[i][0] = s4[0];
+        dst[i][1] = s4[1];
+        yd[i] = A.y[row];
+      } else {
+        dst[i][0] = dst[i][1] = make_float4(0.f, 0.f, 0.f, 0.f);
+        yd[i] = 0;
+      }
+    }
+  };
+#pragma unroll
+  for (int k = 0; k < PD; ++k) {
+    const int t = blockIdx.x + k * G;
+    if (t < tiles) prefetch(t, pre[k], ypre[k]);
+  }
+
+  float g[CP][F];
+#pragma unroll
+  for (int c = 0; c < CP; ++c)
+#pragma unroll
+    for (int j = 0; j < F; ++j) g[c][j] = 0.0f;
+  double llacc = 0.0, gbacc = 0.0;
+
+  for (int base = blockIdx.x; base < tiles; base += PD * G) {
+#pragma unroll
+    for (int slot = 0; slot < PD; ++slot) {
+      const int tile = base + slot * G;
+      if (tile >= tiles) break;
+      // the slot is refilled after this tile (the other slots stay in flight)
+      auto xr = [&](int i, int j) { return f4c(pre[slot][i][j >> 2], j & 3); };
+      const int* yv = ypre[slot];
+      // phase B: shares of t[i][c] = x_i . theta_c, completed over the row's lanes
+      float tp[NC];
+#pragma unroll
+      for (int i = 0; i < RPT; ++i)
+#pragma unroll
+        for (int c = 0; c < CP; ++c) {
+          float s0 = xr(i, 0) * th[c][0], s1 = xr(i, 1) * th[c][1];
+          s0 = fmaf(xr(i, 2), th[c][2], s0);
+          s1 = fmaf(xr(i, 3), th[c][3], s1);
+          s0 = fmaf(xr(i, 4), th[c][4], s0);
+          s1 = fmaf(xr(i, 5), th[c][5], s1);
+          s0 = fmaf(xr(i, 6), th[c][6], s0);
+          s1 = fmaf(xr(i, 7), th[c][7], s1);
+          tp[i * CP + c] = row_sum<LPR>(s0 + s1);
+        }
+      // adjoints: pair m = q + s LPR is row slot m / CP, chain m % CP = my_c
+      float aown[SPL];
+#pragma unroll
+      for (int s = 0; s < SPL; ++s) {
+        const int m = q + s * LPR;
+        float tv = tp[0];
+#pragma unroll
+        for (int k = 1; k < NC; ++k) tv = (m == k) ? tp[k] : tv;
+        const int i = m / CP;
+        int yy = yv[0];
+#pragma unroll
+        for (int k = 1; k < RPT; ++k) yy = (i == k) ? yv[k] : yy;
+        const bool ok = m < NC && tile * TRr + i * RG + rsub < n;
+        float a;
+        const float lr = log_sigmoid_pm(tv + b_c, yy, &a);
+        if (ok) {
+          llacc += (double)lr;
+          gbacc += (double)a;
+        } else {
+          a = 0.0f;
+        }
+        aown[s] = a;
+      }
+      // phase C: g[c] += a[i][c] x_i over the lane's features
+#pragma unroll
+      for (int m = 0; m < NC; ++m) {
+        const float av = __shfl(aown[m / LPR], gbase + m % LPR);
+        const int i = m / CP, c = m % CP;
+#pragma unroll
+        for (int j = 0; j < F; ++j) g[c][j] = fmaf(av, xr(i, j), g[c][j]);
+      }
+      if (tile + PD * G < tiles) prefetch(tile + PD * G, pre[slot], ypre[slot]);
+    }
+  }
+
+  // block partials -> part[block][value], value = c (D + 2) + j
+  constexpr int VS = D + 2;
+  float* gl = (float*)smem;                                      // [256][GS]
+  double* red = (double*)(smem + ((sizeof(float) * kPassThreads * GS + 15) & ~(size_t)15));  // [2][4][CP]
+#pragma unroll
+  for (int c = 0; c < CP; ++c)
+#pragma unroll
+    for (int j = 0; j < F; ++j) gl[tid * GS + c * F + j] = g[c][j];
+  // ll and intercept gradient: lanes with the same chain (lane = c mod CP)
+#pragma unroll
+  for (int off = 32; off >= CP; off >>= 1) {
+    gbacc += __shfl_xor(gbacc, off);
+    llacc += __shfl_xor(llacc, off);
+  }
+  if (lane < CP) {
+    red[wv * CP + lane] = gbacc;
+    red[4 * CP + wv * CP + lane] = llacc;
+  }
+  __syncthreads();
+  double* pb = A.part + (size_t)blockIdx.x * CP * VS;
+  for (int v = tid; v < CP * D; v += kPassThreads) {
+    const int c = v / D, j = v % D, qq = j / F, jj = j % F;
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < kPassThreads / 64; ++w)
+#pragma unroll
+      for (int k = 0; k < 64 / LPR; ++k) s += (double)gl[(w * 64 + k * LPR + qq) * GS + c * F + jj];
+    pb[c * VS + j] = s;
+  }
+  if (tid < CP) {
+    double sb = 0.0, sl = 0.0;
+#pragma unroll
+    for (int w = 0; w < kPassThreads / 64; ++w) {
+      sb += red[w * CP + tid];
+      sl += red[4 * CP + w * CP + tid];
+    }
+    pb[tid * VS + D] = sb;
+    pb[tid * VS + D + 1] = sl;
+  }
+}
+
+size_t pass_r_smem_bytes(int cp) {
+  return ((sizeof(float) * kPassThreads * (cp * kRowFeat + 1) + 15) & ~(size_t)15) + sizeof(double) * 8 * cp;
+}
+
+// rows per tile of the register form
+inline int pass_r_rows(int d, int cp) { return kPassThreads / (d / kRowFeat) * row_rpt(cp); }
 
 size_t pass_t_smem_bytes(int d, int cp) {
   size_t b = sizeof(float) * ((size_t)kTileRows * (d + 4) + (size_t)cp * (d + 4) + (size_t)kTileRows * cp) +
@@ -477,6 +708,16 @@ PassKernel pick_pass(int d, int cp) {
   AUGUR_PASS(32, 1) AUGUR_PASS(32, 2) AUGUR_PASS(32, 4) AUGUR_PASS(32, 8) AUGUR_PASS(32, 16)
   AUGUR_PASS(64, 1) AUGUR_PASS(64, 2) AUGUR_PASS(64, 4) AUGUR_PASS(64, 8) AUGUR_PASS(64, 16)
   AUGUR_PASS(128, 1) AUGUR_PASS(128, 2) AUGUR_PASS(128, 4) AUGUR_PASS(128, 8) AUGUR_PASS(128, 16)
+#undef AUGUR_PASS
+  return nullptr;
+}
+
+// Register form of the pass for (d, cp), if compiled.
+PassKernel pick_pass_r(int d, int cp) {
+#define AUGUR_PASS(D_, C_) if (d == D_ && cp == C_) return k_hlr_pass_r<D_, C_>;
+  AUGUR_PASS(16, 1) AUGUR_PASS(16, 2)
+  AUGUR_PASS(32, 1) AUGUR_PASS(32, 2) AUGUR_PASS(32, 4)
+  AUGUR_PASS(64, 1) AUGUR_PASS(64, 2) AUGUR_PASS(64, 4)
 #undef AUGUR_PASS
   return nullptr;
 }
@@ -650,6 +891,7 @@ struct HlrSampler final : augur_sampler {
   double prior_a = 1.0, prior_b = 1.0;
   int G = 1;
   PassKernel fast_pass = nullptr;  // specialised pass for (d, cp), if compiled
+  size_t fast_smem = 0;            // its dynamic LDS bytes
   std::vector<augur_hmc_block> sched;
   DevBuf<float> x, beta32;
   DevBuf<int32_t> y;
@@ -697,7 +939,7 @@ struct HlrSampler final : augur_sampler {
   void launch_pass(const PassArgs& A) {
     prof_begin();
     if (fast_pass) {
-      fast_pass<<<G, kPassThreads, pass_t_smem_bytes(d, cp), stream>>>(A);
+      fast_pass<<<G, kPassThreads, fast_smem, stream>>>(A);
     } else {
       allow_lds(k_hlr_pass, pass_smem_bytes(d, cp));
       k_hlr_pass<<<G, kPassThreads, pass_smem_bytes(d, cp), stream>>>(A);
@@ -909,9 +1151,20 @@ struct HlrSampler final : augur_sampler {
   void alloc(int n_, int d_, int chains_) {
     n = n_; d = d_; chains = chains_;
     cp = next_pow2(chains);
-    const int tiles = (n + kTileRows - 1) / kTileRows;
+    int tiles = (n + kTileRows - 1) / kTileRows;
     const char* gen = getenv("AUGUR_HLR_GENERIC");
+    const char* lds = getenv("AUGUR_HLR_PASS_LDS");
     fast_pass = (gen && atoi(gen)) ? nullptr : pick_pass(d, cp);
+    fast_smem = pass_t_smem_bytes(d, cp);
+    int ring = pass_ring(d);
+    if (fast_pass && !(lds && atoi(lds))) {
+      if (PassKernel r = pick_pass_r(d, cp)) {
+        fast_pass = r;
+        fast_smem = pass_r_smem_bytes(cp);
+        tiles = (n + pass_r_rows(d, cp) - 1) / pass_r_rows(d, cp);
+        ring = row_ring(cp);
+      }
+    }
     int want;
     if (fast_pass) {
       // every block's tiles fit in the specialised pass's prefetch ring when
@@ -919,7 +1172,7 @@ struct HlrSampler final : augur_sampler {
       // resident at once (a second round of blocks costs ~35% at N = 100k)
       int cus = 0;
       AUGUR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
-      want = std::min(kPassBlocksPerCU * std::max(cus, 1), (tiles + pass_ring(d) - 1) / pass_ring(d));
+      want = std::min(kPassBlocksPerCU * std::max(cus, 1), (tiles + ring - 1) / ring);
     } else {
       // about 256 rows per block, between one and four blocks per CU
       want = 256;

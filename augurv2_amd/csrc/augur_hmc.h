@@ -113,17 +113,26 @@ __device__ inline bool v_update(int K, double S, double v0, int ref_compat, int 
   return acc;
 }
 
-// Data term of one row: e = exp(-|t|); sigmoid(t) = 1/(1+e) (t >= 0) or
-// e/(1+e); a = y - sigmoid(t) (= bernoulli_dotarg1 * expit_grad_0,
-// cbits/augur_dist.c:68-71, cbits/augur_math.c:56-59); returns
-// log sigmoid(+-t) = min(+-t, 0) - log1p(e).
+// Data term of one row: e = exp(-|t|), u = 1 + e; sigmoid(|t|) = 1/u and
+// sigmoid(-|t|) = e/u; a = y - sigmoid(t) (= bernoulli_dotarg1 *
+// expit_grad_0, cbits/augur_dist.c:68-71, cbits/augur_math.c:56-59), taken
+// as sigmoid(-t) for y = 1 and -sigmoid(t) for y = 0 so that it carries no
+// cancellation; returns log sigmoid(+-t) = min(+-t, 0) - log1p(e).
+// FP32 on the hardware exp2 / log2 / reciprocal (about one ulp each);
+// log1p(e) = e log(u) / (u - 1) (Goldberg 1991, Thm. 4: u - 1 is exact, and
+// the quotient cancels the rounding of u), e itself when u rounds to 1.
 __device__ inline float log_sigmoid_pm(float t, int y, float* a_out) {
-  const float e = expf(-fabsf(t));
-  const float inv = 1.0f / (1.0f + e);
-  const float sig = t >= 0.0f ? inv : e * inv;
-  *a_out = (float)y - sig;
+  const float e = __builtin_amdgcn_exp2f(-fabsf(t) * 1.44269504088896341f);
+  const float u = 1.0f + e;
+  const float inv = __builtin_amdgcn_rcpf(u);
+  const float se = e * inv;
+  const bool pos = t >= 0.0f;
+  *a_out = y ? (pos ? se : inv) : -(pos ? inv : se);
+  const float um1 = u - 1.0f;
+  const float l1p = um1 == 0.0f ? e
+                                : e * (__builtin_amdgcn_logf(u) * 0.693147180559945309f) * __builtin_amdgcn_rcpf(um1);
   const float st = y ? t : -t;
-  return fminf(st, 0.0f) - log1pf(e);
+  return fminf(st, 0.0f) - l1p;
 }
 
 __device__ inline double wave_sum(double v) {

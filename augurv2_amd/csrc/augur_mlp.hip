@@ -95,12 +95,31 @@ void gemm_f32(hipStream_t st, int M, int N, int K, const float* A, long sam, lon
 }
 
 // out[n] = sum over rows of x[m][n]
-__global__ void k_colsum(const float* __restrict__ x, int rows, int cols, float* __restrict__ out) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= cols) return;
-  float s = 0.0f;
-  for (int m = 0; m < rows; ++m) s += x[(size_t)m * cols + n];
-  out[n] = s;
+// 1024-thread block per 64 columns: thread (group g, column c) sums rows
+// g, g + 16, ... with four independent accumulators, then the 16 group sums
+// are added in order (deterministic).
+__global__ __launch_bounds__(1024) void k_colsum(const float* __restrict__ x, int rows, int cols,
+                                                 float* __restrict__ out) {
+  __shared__ float red[16][65];
+  const int c = blockIdx.x * 64 + (threadIdx.x & 63), g = threadIdx.x >> 6;
+  float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
+  if (c < cols) {
+    int m = g;
+    for (; m + 48 < rows; m += 64) {
+      s0 += x[(size_t)m * cols + c];
+      s1 += x[(size_t)(m + 16) * cols + c];
+      s2 += x[(size_t)(m + 32) * cols + c];
+      s3 += x[(size_t)(m + 48) * cols + c];
+    }
+    for (; m < rows; m += 16) s0 += x[(size_t)m * cols + c];
+  }
+  red[g][threadIdx.x & 63] = (s0 + s1) + (s2 + s3);
+  __syncthreads();
+  if (threadIdx.x < 64 && c < cols) {
+    float s = 0.0f;
+    for (int k = 0; k < 16; ++k) s += red[k][threadIdx.x];
+    out[c] = s;
+  }
 }
 
 __global__ void k_gelu_bwd(const float* __restrict__ dh, const float* __restrict__ z, float* __restrict__ dz,
@@ -109,23 +128,32 @@ __global__ void k_gelu_bwd(const float* __restrict__ dh, const float* __restrict
     dz[i] = dh[i] * gelu_grad_f(z[i]);
 }
 
-// dout = 2 (out - t) / n and loss = mean (out - t)^2 (one block, fixed order).
-__global__ __launch_bounds__(1024) void k_mse(const float* __restrict__ out, const float* __restrict__ t,
-                                              float* __restrict__ dout, size_t n, double* loss) {
-  __shared__ double red[1024];
+// dout = 2 (out - t) / n and loss = mean (out - t)^2: kMseBlocks blocks write
+// partial sums (fixed tree in each), k_mse_fin adds them in block order.
+constexpr int kMseBlocks = 128;
+
+__global__ __launch_bounds__(256) void k_mse(const float* __restrict__ out, const float* __restrict__ t,
+                                             float* __restrict__ dout, size_t n, double* __restrict__ part) {
+  __shared__ double red[256];
   double s = 0.0;
-  for (size_t i = threadIdx.x; i < n; i += blockDim.x) {
+  for (size_t i = blockIdx.x * (size_t)256 + threadIdx.x; i < n; i += (size_t)kMseBlocks * 256) {
     const float df = out[i] - t[i];
     dout[i] = 2.0f * df / (float)n;
     s += (double)df * (double)df;
   }
   red[threadIdx.x] = s;
   __syncthreads();
-  for (int o = 512; o > 0; o >>= 1) {
+  for (int o = 128; o > 0; o >>= 1) {
     if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
     __syncthreads();
   }
-  if (threadIdx.x == 0) *loss = red[0] / (double)n;
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void k_mse_fin(const double* __restrict__ part, size_t n, double* loss) {
+  double s = 0.0;
+  for (int k = 0; k < kMseBlocks; ++k) s += part[k];
+  *loss = s / (double)n;
 }
 
 __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
@@ -151,7 +179,7 @@ void Mlp::init(int d, int h, int batch_rows, uint32_t seed, hipStream_t st) {
   z2.alloc((size_t)batch * h); h2.alloc((size_t)batch * h);
   out.alloc((size_t)batch * d); dout.alloc((size_t)batch * d);
   dh.alloc((size_t)batch * h); dz.alloc((size_t)batch * h);
-  loss.alloc(1);
+  loss.alloc(1 + kMseBlocks);  // [0] the loss, then the blocks' partial sums
   // Uniform(-1/sqrt(fan_in), 1/sqrt(fan_in)) weights from Philox, zero biases
   std::vector<float> hp(P, 0.0f);
   auto fill = [&](size_t off, int rows, int fan_in, uint32_t tag) {
@@ -183,21 +211,22 @@ void Mlp::loss_grad(const float* x, const float* t, int rows, hipStream_t st) {
   forward(x, rows, st);
   const float* P = params.p;
   float* Gp = grads.p;
-  k_mse<<<1, 1024, 0, st>>>(out.p, t, dout.p, (size_t)rows * d, loss.p);
+  k_mse<<<kMseBlocks, 256, 0, st>>>(out.p, t, dout.p, (size_t)rows * d, loss.p + 1);
+  k_mse_fin<<<1, 1, 0, st>>>(loss.p + 1, (size_t)rows * d, loss.p);
   AUGUR_CHECK_LAUNCH();
   // layer 3: dW3 = dout^T h2, db3 = colsum(dout), dh2 = dout W3
   gemm(st, d, h, rows, dout.p, 1, d, h2.p, h, 1, Gp + s.w3(), h);
-  k_colsum<<<(d + 255) / 256, 256, 0, st>>>(dout.p, rows, d, Gp + s.b3());
+  k_colsum<<<(d + 63) / 64, 1024, 0, st>>>(dout.p, rows, d, Gp + s.b3());
   gemm(st, rows, h, d, dout.p, d, 1, P + s.w3(), h, 1, dh.p, h);
   k_gelu_bwd<<<256, 256, 0, st>>>(dh.p, z2.p, dz.p, (size_t)rows * h);
   // layer 2
   gemm(st, h, h, rows, dz.p, 1, h, h1.p, h, 1, Gp + s.w2(), h);
-  k_colsum<<<(h + 255) / 256, 256, 0, st>>>(dz.p, rows, h, Gp + s.b2());
+  k_colsum<<<(h + 63) / 64, 1024, 0, st>>>(dz.p, rows, h, Gp + s.b2());
   gemm(st, rows, h, h, dz.p, h, 1, P + s.w2(), h, 1, dh.p, h);
   k_gelu_bwd<<<256, 256, 0, st>>>(dh.p, z1.p, dz.p, (size_t)rows * h);
   // layer 1
   gemm(st, h, d, rows, dz.p, 1, h, x, d, 1, Gp + s.w1(), d);
-  k_colsum<<<(h + 255) / 256, 256, 0, st>>>(dz.p, rows, h, Gp + s.b1());
+  k_colsum<<<(h + 63) / 64, 1024, 0, st>>>(dz.p, rows, h, Gp + s.b1());
   AUGUR_CHECK_LAUNCH();
 }
 
@@ -230,19 +259,50 @@ __global__ void k_gather_std(const float* __restrict__ x, const float* __restric
 }
 
 // Column means and inverse standard deviations of x and t (n x d each):
-// stats = [mean x | 1/sd x | mean t | 1/sd t].  One thread per column, FP64.
-__global__ void k_colstats(const float* __restrict__ x, const float* __restrict__ t, int n, int d,
-                           float* __restrict__ stats) {
+// stats = [mean x | 1/sd x | mean t | 1/sd t], FP64 sums.  Block (column
+// group, row split): 256 threads = 4 row groups x 64 columns over the rows of
+// its split; partial sums per split, added in split order by the finish.
+constexpr int kColSplits = 64;
+
+__global__ __launch_bounds__(256) void k_colstats_part(const float* __restrict__ x, const float* __restrict__ t,
+                                                       int n, int d, double* __restrict__ part) {
+  __shared__ double red[4][64][2];
+  const int j = blockIdx.x * 64 + (threadIdx.x & 63), g = threadIdx.x >> 6;
+  const int per = (n + kColSplits - 1) / kColSplits;
+  const int r0 = blockIdx.y * per, r1 = min(n, r0 + per);
+  double s = 0.0, s2 = 0.0;
+  if (j < 2 * d) {
+    const float* a = j < d ? x : t;
+    const int col = j % d;
+    for (int i = r0 + g; i < r1; i += 4) {
+      const double v = a[(size_t)i * d + col];
+      s += v;
+      s2 += v * v;
+    }
+  }
+  red[g][threadIdx.x & 63][0] = s;
+  red[g][threadIdx.x & 63][1] = s2;
+  __syncthreads();
+  if (threadIdx.x < 64 && j < 2 * d) {
+    double a = 0.0, b = 0.0;
+    for (int k = 0; k < 4; ++k) {
+      a += red[k][threadIdx.x][0];
+      b += red[k][threadIdx.x][1];
+    }
+    part[((size_t)blockIdx.y * 2 * d + j) * 2] = a;
+    part[((size_t)blockIdx.y * 2 * d + j) * 2 + 1] = b;
+  }
+}
+
+__global__ void k_colstats_fin(const double* __restrict__ part, int n, int d, float* __restrict__ stats) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= 2 * d) return;
-  const float* a = j < d ? x : t;
-  const int col = j % d;
   double s = 0.0, s2 = 0.0;
-  for (int i = 0; i < n; ++i) {
-    const double v = a[(size_t)i * d + col];
-    s += v;
-    s2 += v * v;
+  for (int k = 0; k < kColSplits; ++k) {
+    s += part[((size_t)k * 2 * d + j) * 2];
+    s2 += part[((size_t)k * 2 * d + j) * 2 + 1];
   }
+  const int col = j % d;
   const double mean = s / n;
   const double var = fmax(s2 / n - mean * mean, 1e-30);
   const int base = j < d ? 0 : 2 * d;
@@ -251,8 +311,12 @@ __global__ void k_colstats(const float* __restrict__ x, const float* __restrict_
 }
 
 void mlp_colstats(const float* x, const float* t, int n, int d, float* stats, hipStream_t st) {
-  k_colstats<<<(2 * d + 63) / 64, 64, 0, st>>>(x, t, n, d, stats);
+  DevBuf<double> part((size_t)kColSplits * 2 * d * 2);
+  k_colstats_part<<<dim3((2 * d + 63) / 64, kColSplits), 256, 0, st>>>(x, t, n, d, part.p);
   AUGUR_CHECK_LAUNCH();
+  k_colstats_fin<<<(2 * d + 63) / 64, 64, 0, st>>>(part.p, n, d, stats);
+  AUGUR_CHECK_LAUNCH();
+  AUGUR_HIP(hipStreamSynchronize(st));  // part is released on return
 }
 
 void mlp_gather(const float* x, const float* t, int n, int d, const float* stats, int rows, uint32_t seed,

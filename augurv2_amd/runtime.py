@@ -204,6 +204,18 @@ class BhlrSampler(Sampler):
             shapes[f"state:{i}"] = ((int(d[i]) + 2,), np.float64)
         super().__init__(h, chains, shapes)
 
+    def adapt(self, n_sweeps, target_accept=0.8):
+        """Warmup sweeps that adapt each dataset's theta step size (and the
+        MWG proposal variance of v), then freeze them."""
+        L.call("augur_bhlr_adapt", self._h, int(n_sweeps), float(target_accept))
+
+    def step_sizes(self):
+        """(theta step size, v proposal variance) per dataset."""
+        k = len(self.xs)
+        eps, vv = np.zeros(k), np.zeros(k)
+        L.call("augur_bhlr_step_sizes", self._h, L.ptr(eps), L.ptr(vv), k)
+        return eps, vv
+
     def tuning(self):
         rows = C.c_int(0)
         n = C.c_int(0)

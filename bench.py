@@ -160,8 +160,14 @@ def run_batched(sweeps=1000, warmup=500, chains=4, seed=31):
         for c, st in enumerate(workloads.hlr_init(x.shape[1], chains, seed=100 + i)):
             s.set(f"state:{i}", st, chain=c)
     rows, cand = s.tuning()
-    log(f"batched: rows/block {rows}, candidates {cand}; warmup {warmup}")
-    s.step(warmup)
+    log(f"batched: rows/block {rows}, candidates {cand}; adapting warmup {warmup}")
+    # per-dataset step sizes (dual averaging) and v proposal variances adapt
+    # during warmup, then stay fixed: one step size cannot suit datasets of
+    # 1k and 200k rows with variances from 0.1 to 100
+    t0 = time.perf_counter()
+    s.adapt(warmup, 0.8)
+    t_warm = time.perf_counter() - t0
+    eps_i, vvar_i = s.step_sizes()
     s.trace("state", sweeps)
     t0 = time.perf_counter()
     s.step(sweeps)
@@ -193,6 +199,9 @@ def run_batched(sweeps=1000, warmup=500, chains=4, seed=31):
     plain_total = float(np.sum(a + b * work)) * sweeps
     return {"datasets": len(xs), "chains": chains, "rows_total": int(sum(x.shape[0] for x in xs)),
             "rows_per_block": rows, "autotune_us": cand, "sweeps": sweeps, "elapsed_s": el,
+            "warmup_s": t_warm, "step_size_range": [float(eps_i.min()), float(eps_i.max())],
+            "v_proposal_var_range": [float(vvar_i.min()), float(vvar_i.max())],
+            "n_rhat_below_1_01": int(np.sum(np.array(rhat_i) < 1.01)),
             "ms_per_sweep": 1e3 * el / sweeps,
             "sum_dataset_min_ess_per_s": float(ess_i.sum() / el),
             "min_dataset_ess": float(ess_i.min()), "median_dataset_ess": float(np.median(ess_i)),

@@ -184,6 +184,16 @@ augur_status augur_bhlr_create(const augur_bhlr_desc* desc, augur_sampler** out)
  * timings of its candidates (microseconds per data pass, up to `cap`). */
 augur_status augur_bhlr_tuning(augur_sampler* s, int* rows_per_block, int* n_candidates,
                                int* cand_rows, float* cand_us, int cap);
+/* No counterpart in the reference, which does not adapt.  Warmup of n_sweeps
+ * sweeps; after each, every dataset's theta step size follows dual averaging
+ * towards a mean acceptance probability target_accept over its chains
+ * (Hoffman & Gelman 2014, Algorithm 5), and with an MWG v block its proposal
+ * variance follows a Robbins-Monro rule towards acceptance 0.44.  At the end
+ * the step sizes take their averaged values; later sweeps keep both fixed. */
+augur_status augur_bhlr_adapt(augur_sampler* s, int n_sweeps, double target_accept);
+/* Per-dataset theta step sizes and v proposal variances (n >= n_datasets;
+ * either pointer may be null). */
+augur_status augur_bhlr_step_sizes(augur_sampler* s, double* eps, double* v_var, int n);
 
 /* hmvgmm.rv: pis ~ Dirichlet(alpha); mu[k] ~ MvNormal(mu0, cov0);
  * cov[k] ~ InvWishart(df, scale0); z[n] ~ Categorical(pis);

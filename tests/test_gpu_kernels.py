@@ -108,9 +108,17 @@ def test_lda_zstep_vs_fp64_reference_form(ops, oracle):
     assert mism <= 2
 
 
+@pytest.mark.parametrize("lds", [False, True])
 @pytest.mark.parametrize("n,d,chains,intercept", [(5000, 16, 4, False), (4099, 64, 4, True),
-                                                  (1000, 128, 8, False), (777, 5, 3, True)])
-def test_hlr_ll_grad_vs_fp64(ops, oracle, n, d, chains, intercept):
+                                                  (1000, 128, 8, False), (777, 5, 3, True),
+                                                  (100003, 64, 4, False), (3001, 32, 3, True),
+                                                  (2500, 16, 2, False), (129, 64, 1, True),
+                                                  (7, 32, 4, False), (20000, 64, 2, False)])
+def test_hlr_ll_grad_vs_fp64(ops, oracle, monkeypatch, n, d, chains, intercept, lds):
+    """Register form of the pass (D <= 64, chains <= D / 8, padded to a
+    power of two) and the LDS-staged form forced on the same shapes;
+    partial tiles, a block of fewer rows than lanes, the intercept."""
+    monkeypatch.setenv("AUGUR_HLR_PASS_LDS", "1" if lds else "0")
     rng = np.random.default_rng(n + d)
     x = rng.standard_normal((n, d)).astype(np.float32)
     beta_true = rng.standard_normal(d) * 0.3
