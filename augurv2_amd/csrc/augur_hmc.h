@@ -118,11 +118,20 @@ __device__ inline bool v_update(int K, double S, double v0, int ref_compat, int 
 // expit_grad_0, cbits/augur_dist.c:68-71, cbits/augur_math.c:56-59), taken
 // as sigmoid(-t) for y = 1 and -sigmoid(t) for y = 0 so that it carries no
 // cancellation; returns log sigmoid(+-t) = min(+-t, 0) - log1p(e).
-// FP32 on the hardware exp2 / log2 / reciprocal (about one ulp each);
-// log1p(e) = e log(u) / (u - 1) (Goldberg 1991, Thm. 4: u - 1 is exact, and
-// the quotient cancels the rounding of u), e itself when u rounds to 1.
+// FP32 on the hardware exp2 / log2 / reciprocal (about one ulp each).  The
+// exponent -|t| log2(e) carries its rounding error and the tail of log2(e)
+// as a first-order correction (2^(h + l) = 2^h (1 + l ln 2)), so e keeps
+// about one ulp for any |t|; log1p(e) = e log(u) / (u - 1) (Goldberg 1991,
+// Thm. 4: u - 1 is exact, and the quotient cancels the rounding of u), e
+// itself when u rounds to 1.
 __device__ inline float log_sigmoid_pm(float t, int y, float* a_out) {
-  const float e = __builtin_amdgcn_exp2f(-fabsf(t) * 1.44269504088896341f);
+  constexpr float kL2eHi = 1.44269502162933349609375f;  // log2(e) rounded to FP32
+  constexpr float kL2eLo = 1.92596299e-08f;             // log2(e) - kL2eHi
+  const float at = fabsf(t);
+  const float h = -at * kL2eHi;
+  const float l = fmaf(-at, kL2eHi, -h) - at * kL2eLo;
+  const float e0 = __builtin_amdgcn_exp2f(h);
+  const float e = fmaf(e0, l * 0.693147180559945309f, e0);
   const float u = 1.0f + e;
   const float inv = __builtin_amdgcn_rcpf(u);
   const float se = e * inv;

@@ -47,11 +47,15 @@ def _init(s, rng, C):
 @pytest.mark.parametrize("L", [8, -8])
 def test_batched_matches_oracle(rt, oracle, L):
     """L > 0: fixed step and count; L < 0: the reference's step jitter with
-    |L| steps (every chain runs the same number of passes)."""
+    |L| steps (every chain runs the same number of passes).  HMC on v with
+    steps of 0.01: at 0.05 its leapfrog is unstable on dataset 3 and the
+    oracle alone turns a 1e-7 nudge of theta into 28 times this test's
+    tolerance within three sweeps; at 0.01 the same nudge stays below a
+    quarter of it on every dataset."""
     rng = np.random.default_rng(0)
     xs, ys = _datasets(rng, SHAPES)
     C, EPS = 2, 0.01
-    s = rt.BhlrSampler(xs, ys, chains=C, seed=7, v_block=(1.0, 0.05, 0), theta_block=(0.1, EPS, L),
+    s = rt.BhlrSampler(xs, ys, chains=C, seed=7, v_block=(0.2, 0.01, 0), theta_block=(0.1, EPS, L),
                        rows_per_block=512, use_graph=True)
     starts = _init(s, rng, C)
     models = [oracle.Hlr(bf16_round(x).astype(np.float64), y, prior_kind=0, prior_a=1.0,
@@ -62,7 +66,7 @@ def test_batched_matches_oracle(rt, oracle, L):
             for c in range(C):
                 st = starts[i, c]
                 gc = i * C + c
-                models[i].hmc_step(st, 0, 0, 1.0, 0.05, 0, 7, gc, sweep)
+                models[i].hmc_step(st, 0, 0, 0.2, 0.01, 0, 7, gc, sweep)
                 models[i].hmc_step(st, 2, 1, 0.1, EPS, L, 7, gc, sweep)
                 np.testing.assert_allclose(s.get(f"state:{i}", c), st, rtol=2e-5, atol=1e-7)
     acc = s.acceptance()
@@ -157,7 +161,11 @@ def test_batched_warmup_adaptation(rt, oracle):
             gc = i * C + c
             models[i].mwg_v_step(st, 0, vv[i], 41, gc, sweep)
             models[i].hmc_step(st, 2, 1, 0.1, eps[i], L, 41, gc, sweep)
-            np.testing.assert_allclose(s.get(f"state:{i}", c), st, rtol=2e-5, atol=1e-7)
+            # absolute 1e-6: the FP32 gradient over N rows is exact to about
+            # 1e-7 sum|a x| (~1e-4 at N = 2500), and ten adapted steps move
+            # theta by eps^2 L times that, ~3e-7 at eps = 0.02
+            np.testing.assert_allclose(s.get(f"state:{i}", c), st, rtol=2e-5, atol=1e-6,
+                                       err_msg=f"dataset {i} chain {c}")
     s.close()
 
 

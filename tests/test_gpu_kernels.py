@@ -143,6 +143,38 @@ def test_hlr_ll_grad_vs_fp64(ops, oracle, monkeypatch, n, d, chains, intercept, 
             assert abs(g[c, d] - gb_ref) <= 1e-5 * a.sum() + 1e-9
 
 
+@pytest.mark.parametrize("chains", [16, 2])
+@pytest.mark.parametrize("y", [0, 1])
+def test_hlr_row_term_per_value(ops, chains, y):
+    """The FP32 row term on its own: one row x = e_0, so chain c's
+    log-density is log sigmoid(+-t_c) and its gradient the adjoint
+    y - sigmoid(t_c) at t_c = theta_c0 (exact in FP32).  Against the stable
+    FP64 forms: both within 8 ulp relative (LDS form at 16 chains, register
+    form at 2), over |t| up to 100 and near 0."""
+    d = 16
+    x = np.zeros((1, d), np.float32)
+    x[0, 0] = 1.0
+    ts = np.concatenate([np.linspace(-30, 30, 61), [0.0, 1e-8, -1e-8, 1e-3, -1e-3, 0.37, -0.37, 17.0,
+                                                   -17.0, 40.0, -40.0, 88.0, -88.0, 100.0, -100.0]])
+    ts = ts.astype(np.float32).astype(np.float64)
+    for k in range(0, len(ts), chains):
+        tc = ts[k:k + chains]
+        beta = np.zeros((len(tc), d + 1))
+        beta[:, 0] = tc
+        ll, g = ops.hlr_ll_grad(x, np.array([y], np.int32), beta)
+        st = tc if y else -tc  # log sigmoid(st) = min(st, 0) - log1p(exp(-|st|))
+        ll_ref = np.minimum(st, 0.0) - np.log1p(np.exp(-np.abs(st)))
+        # a = y - sigmoid(t): sigmoid(-t) for y = 1, -sigmoid(t) for y = 0
+        def sig(s):
+            return np.where(s >= 0, 1 / (1 + np.exp(-np.abs(s))),
+                            np.exp(-np.abs(s)) / (1 + np.exp(-np.abs(s))))
+        a_ref = sig(-tc) if y else -sig(tc)
+        # 8 ulp relative; values below FP32's normal range (|t| near 100) absolutely
+        tol, tiny = 8 * 2.0**-24, 2.0**-126
+        np.testing.assert_allclose(ll, ll_ref, rtol=tol, atol=tiny, err_msg=f"t {tc}")
+        np.testing.assert_allclose(g[:, 0], a_ref, rtol=tol, atol=tiny, err_msg=f"t {tc}")
+
+
 @pytest.mark.parametrize("n,d,chains,intercept", [(5000, 16, 4, False), (4099, 64, 4, False),
                                                   (1000, 128, 8, False), (2003, 128, 16, False),
                                                   (15, 32, 2, False), (777, 5, 3, True),
