@@ -22,8 +22,8 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the headline pass kernel: rocprofv3 --pmc FETCH_SIZE
 # and WRITE_SIZE (KiB per dispatch) on this workload, each scaled by what the
 # counter reads for a plain 256 MiB device copy (`--copy-mib 256`: fetch x2.0,
-# write x1.0); profiles/k_hlr_pass_t_hlr_n100k_d64_c4.md
-PASS_FETCH_KIB, PASS_WRITE_KIB = 12771.3, 1074.6
+# write x1.0); profiles/k_hlr_pass_r_hlr_n100k_d64_c4.md
+PASS_FETCH_KIB, PASS_WRITE_KIB = 12754.8, 1584.0
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 
 
@@ -343,7 +343,7 @@ def main():
         "surrogate": surr,
         "batched": bat,
         "gmm": gmm,
-        "roofline": {"kernel": "k_hlr_pass", "bound": "hbm", "achieved": achieved,
+        "roofline": {"kernel": "k_hlr_pass_r<64,4>", "bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": 1024.0 * (PASS_FETCH_KIB * FETCH_SCALE + PASS_WRITE_KIB * WRITE_SCALE),
                      "avg_us": hlr["pass_avg_us"],
