@@ -13,11 +13,13 @@
 //               triangle), mu_k and -sum log diag L_k, stored FP32.
 //   k_gmm_z     point-parallel categorical draw (Core/CgDisc.hs:122-144) in
 //               log space with max subtraction: logit_k = log pis_k
-//               - 0.5 log det cov_k - 0.5 |L_k^-1 (y - mu_k)|^2, tables in
-//               LDS; fused with the statistics of the new z for the next
-//               sweep, centred at the current mu_k: count, sum (y - mu_k) and
-//               sum (y - mu_k)(y - mu_k)^T (LDS atomics, FP32 per block, FP64
-//               across blocks).  The next sweep's scatter around the new mu is
+//               - 0.5 log det cov_k - 0.5 |L_k^-1 (y - mu_k)|^2, tables read
+//               through the scalar cache; fused with the statistics of the
+//               new z for the next sweep, centred at the current mu_k: count,
+//               sum (y - mu_k) and sum (y - mu_k)(y - mu_k)^T, as a one-hot
+//               product per wave of 64 points on the FP32 matrix core (no
+//               atomics: FP32 per block in a fixed order, FP64 across
+//               blocks).  The next sweep's scatter around the new mu is
 //               S + s d^T + d s^T + n d d^T with d = mu_old - mu_new, so the
 //               data are read once per sweep.
 //
@@ -33,6 +35,8 @@ constexpr int kMaxDim = 16;
 constexpr int kMaxComp = 64;
 
 __host__ __device__ constexpr int tri(int d) { return d * (d + 1) / 2; }
+
+typedef float gmm_f32x4 __attribute__((ext_vector_type(4)));
 
 struct GmmDev {
   int n, d, k, G, S;       // S = 1 + d + d(d+1)/2 statistics per component
@@ -281,16 +285,12 @@ __global__ __launch_bounds__(kConjLanes) void k_gmm_tables(GmmDev m) {
 // DM: padded dimension, KM: maximum number of components held in registers.
 template <int DM, int KM>
 __global__ __launch_bounds__(kGmmThreads) void k_gmm_z(GmmDev m, uint32_t seed, uint32_t sweep,
-                                                      uint32_t chain0, int with_stats) {
-  extern __shared__ __align__(16) unsigned char smem[];
+                                                      uint32_t chain0) {
+  __shared__ float lpi[KM];
   constexpr int TS = tri(DM) + DM + 1;
-  const int K = m.k, d = m.d, S = m.S, c = blockIdx.y, tid = threadIdx.x;
-  float* tabs = (float*)smem;        // K x TS
-  float* acc = tabs + K * TS;        // K x S statistics
-  float* lpi = acc + K * S;          // K
-  const float* gtab = m.tab + (size_t)c * K * m.TS;
-  for (int i = tid; i < K * TS; i += kGmmThreads) tabs[i] = gtab[i];
-  for (int i = tid; i < K * S; i += kGmmThreads) acc[i] = 0.0f;
+  const int K = m.k, d = m.d, c = blockIdx.y, tid = threadIdx.x;
+  // the table is uniform over the block: read through the scalar cache
+  const float* __restrict__ gtab = m.tab + (size_t)c * K * TS;
   if (tid == 0) {
     const double* g = m.gpi + (size_t)c * K;
     double tot = 0.0;
@@ -321,7 +321,7 @@ __global__ __launch_bounds__(kGmmThreads) void k_gmm_z(GmmDev m, uint32_t seed, 
 #pragma unroll
     for (int kk = 0; kk < KM; ++kk) {
       if (kk < K) {
-        const float* t = tabs + kk * TS;
+        const float* t = gtab + kk * TS;
         float dv[DM];
 #pragma unroll
         for (int i = 0; i < DM; ++i) dv[i] = yv[i] - t[tri(DM) + i];
@@ -356,48 +356,137 @@ __global__ __launch_bounds__(kGmmThreads) void k_gmm_z(GmmDev m, uint32_t seed, 
         if (zk == K - 1 && tt < run) zk = kk;
       }
     m.z[(size_t)c * m.n + n] = zk;
-    if (with_stats) {
-      const float* t = tabs + zk * TS;
-      float dv[DM];
-#pragma unroll
-      for (int i = 0; i < DM; ++i) dv[i] = yv[i] - t[tri(DM) + i];
-      float* a = acc + zk * S;
-      atomicAdd(a, 1.0f);
-      for (int i = 0; i < d; ++i) {
-        atomicAdd(a + 1 + i, dv[i]);
-        for (int j = 0; j <= i; ++j) atomicAdd(a + 1 + d + tri(i) + j, dv[i] * dv[j]);
-      }
-    }
   }
-  if (!with_stats) return;
-  __syncthreads();
-  float* dst = m.part + ((size_t)c * m.G + blockIdx.x) * K * S;
-  for (int i = tid; i < K * S; i += kGmmThreads) dst[i] = acc[i];
 }
 
-// Statistics of the current z around the current mu (after set_pt).
-__global__ void k_gmm_stats(GmmDev m) {
-  const int c = blockIdx.y, tid = threadIdx.x;
+// Sufficient statistics of z around the table's mu (the mu of the z step):
+// per (chain, block) and component, count, sum e and sum e e^T (packed lower)
+// with e = y - mu_z.  Each wave takes 64 points and forms
+// S[k][q] += sum_p [z_p = k] F_p[q], F_p = (1, e_p, e_p e_p^T), as a one-hot
+// product on the FP32 matrix core (V_MFMA_F32_16X16X4_F32: A = [z_p = k],
+// 16 components x 4 points; B = F, 4 points x 16 entries) from a per-wave LDS
+// slab of the residuals.  No atomics: FP32 per block in a fixed order, FP64
+// across blocks in k_gmm_conj.
+template <int DM, int KM>
+__global__ __launch_bounds__(kGmmThreads) void k_gmm_stats_t(GmmDev m) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int K = m.k, d = m.d, S = m.S;
-  float* acc = (float*)smem;
-  for (int i = tid; i < K * S; i += blockDim.x) acc[i] = 0.0f;
-  __syncthreads();
-  for (int n = blockIdx.x * blockDim.x + tid; n < m.n; n += gridDim.x * blockDim.x) {
-    const int zk = m.z[(size_t)c * m.n + n];
-    const double* mu = m.mu64 + ((size_t)c * K + zk) * d;
-    float dv[kMaxDim];
-    for (int i = 0; i < d; ++i) dv[i] = m.y[(size_t)n * d + i] - (float)mu[i];
-    float* a = acc + zk * S;
-    atomicAdd(a, 1.0f);
-    for (int i = 0; i < d; ++i) {
-      atomicAdd(a + 1 + i, dv[i]);
-      for (int j = 0; j <= i; ++j) atomicAdd(a + 1 + d + tri(i) + j, dv[i] * dv[j]);
+  constexpr int TS = tri(DM) + DM + 1;
+  constexpr int CB = (KM + 15) / 16;                // 16-component blocks
+  constexpr int EB = (1 + DM + tri(DM) + 15) / 16;  // 16-entry blocks
+  constexpr int XS = DM + 3;                        // slab row: e (DM), 1, 0, pad
+  const int K = m.k, d = m.d, S = m.S, c = blockIdx.y, tid = threadIdx.x;
+  const int lane = tid & 63, wv = tid >> 6;
+  float* slab = (float*)smem + (size_t)wv * 64 * XS;            // per wave: 64 x XS
+  int* zsl = (int*)((float*)smem + 4 * 64 * XS) + wv * 64;      // per wave: 64
+  const float* __restrict__ gtab = m.tab + (size_t)c * K * TS;
+  // entry q = 16 eb + (lane % 16) this lane supplies: q = 0 -> 1 * 1,
+  // q = 1 .. d -> e[q - 1] * 1, then e[a] e[b] at q = 1 + d + tri(a) + b
+  // (b <= a), beyond S -> 0 (slab column DM + 1)
+  int ia[EB], ib[EB];
+#pragma unroll
+  for (int eb = 0; eb < EB; ++eb) {
+    const int q = 16 * eb + (lane & 15);
+    if (q == 0) {
+      ia[eb] = DM; ib[eb] = DM;
+    } else if (q <= d) {
+      ia[eb] = q - 1; ib[eb] = DM;
+    } else if (q < S) {
+      const int r = q - 1 - d;
+      int a = 0;
+      while (tri(a + 1) <= r) ++a;
+      ia[eb] = a; ib[eb] = r - tri(a);
+    } else {
+      ia[eb] = DM + 1; ib[eb] = DM + 1;
     }
   }
+  gmm_f32x4 acc[CB][EB];
+#pragma unroll
+  for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+    for (int eb = 0; eb < EB; ++eb) acc[cb][eb] = gmm_f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  const int stride = gridDim.x * kGmmThreads;
+  // a wave takes 64 consecutive points per step: the trip count is uniform
+  // over the wave, so every lane reaches the product
+  for (int n0 = blockIdx.x * kGmmThreads + wv * 64; n0 < m.n; n0 += stride) {
+    const int n = n0 + lane;
+    const bool live = n < m.n;
+    const int zk = live ? m.z[(size_t)c * m.n + n] : -1;
+    float* row = slab + lane * XS;
+    if (live) {
+      const float* yr = m.y + (size_t)n * d;
+      const float* mu = m.tab + ((size_t)c * K + zk) * TS + tri(DM);
+      if (d == DM && (DM % 4) == 0) {
+#pragma unroll
+        for (int i = 0; i < DM; i += 4) {
+          const float4 q = *(const float4*)(yr + i);
+          row[i] = q.x - mu[i];
+          row[i + 1] = q.y - mu[i + 1];
+          row[i + 2] = q.z - mu[i + 2];
+          row[i + 3] = q.w - mu[i + 3];
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < DM; ++i) row[i] = i < d ? yr[i] - mu[i] : 0.0f;
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < DM; ++i) row[i] = 0.0f;
+    }
+    row[DM] = 1.0f;
+    row[DM + 1] = 0.0f;
+    zsl[lane] = zk;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll 4
+    for (int s = 0; s < 16; ++s) {
+      const int p = 4 * s + (lane >> 4);
+      const int zp = zsl[p];
+      const float* rp = slab + p * XS;
+      float a[CB];
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb) a[cb] = zp == 16 * cb + (lane & 15) ? 1.0f : 0.0f;
+#pragma unroll
+      for (int eb = 0; eb < EB; ++eb) {
+        const float b = rp[ia[eb]] * rp[ib[eb]];
+#pragma unroll
+        for (int cb = 0; cb < CB; ++cb)
+          acc[cb][eb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[cb], b, acc[cb][eb], 0, 0, 0);
+      }
+    }
+    // the slab is rewritten only after every lane has read it
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  // lane l holds S[16 cb + 4 (l / 16) + r][16 eb + l % 16]; the waves are
+  // added in order (deterministic), one row per block
   __syncthreads();
+  float* red = (float*)smem;
+  for (int w = 0; w < kGmmThreads / 64; ++w) {
+    if (wv == w) {
+#pragma unroll
+      for (int cb = 0; cb < CB; ++cb)
+#pragma unroll
+        for (int eb = 0; eb < EB; ++eb)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int comp = 16 * cb + 4 * (lane >> 4) + r, q = 16 * eb + (lane & 15);
+            if (comp < K && q < S) red[comp * S + q] = (w == 0 ? 0.0f : red[comp * S + q]) + acc[cb][eb][r];
+          }
+    }
+    __syncthreads();
+  }
   float* dst = m.part + ((size_t)c * m.G + blockIdx.x) * K * S;
-  for (int i = tid; i < K * S; i += blockDim.x) dst[i] = acc[i];
+  for (int i = tid; i < K * S; i += kGmmThreads) dst[i] = red[i];
+}
+
+// Dynamic LDS of k_gmm_stats_t: the per-wave slabs, reused for the block's
+// statistics at the end.
+inline size_t gmm_stats_smem_bytes(int DM, int KM, int S) {
+  const size_t slabs = (size_t)4 * 64 * (DM + 3) + 4 * 64;
+  const size_t stats = (size_t)KM * S;
+  return sizeof(float) * (slabs > stats ? slabs : stats);
 }
 
 // Data part of the log joint: sum_n log pis_z + mvnormal_ll(y_n; mu_z, cov_z),
@@ -435,7 +524,8 @@ __global__ __launch_bounds__(kGmmThreads) void k_gmm_ll(GmmDev m, double* out) {
 static int pad_dim(int d) { return d <= 4 ? 4 : (d <= 8 ? 8 : 16); }
 static int pad_comp(int k) { return k <= 8 ? 8 : (k <= 16 ? 16 : (k <= 32 ? 32 : 64)); }
 
-typedef void (*ZKernel)(GmmDev, uint32_t, uint32_t, uint32_t, int);
+typedef void (*ZKernel)(GmmDev, uint32_t, uint32_t, uint32_t);
+typedef void (*StatsKernel)(GmmDev);
 
 static ZKernel pick_z(int DM, int KM) {
 #define AUGUR_Z(D_, K_) if (DM == D_ && KM == K_) return k_gmm_z<D_, K_>;
@@ -444,6 +534,15 @@ static ZKernel pick_z(int DM, int KM) {
   AUGUR_Z(16, 8) AUGUR_Z(16, 16) AUGUR_Z(16, 32) AUGUR_Z(16, 64)
 #undef AUGUR_Z
   fail(AUGUR_ERR_UNSUPPORTED, "no z kernel for padded dim %d, components %d", DM, KM);
+}
+
+static StatsKernel pick_stats(int DM, int KM) {
+#define AUGUR_S(D_, K_) if (DM == D_ && KM == K_) return k_gmm_stats_t<D_, K_>;
+  AUGUR_S(4, 8) AUGUR_S(4, 16) AUGUR_S(4, 32) AUGUR_S(4, 64)
+  AUGUR_S(8, 8) AUGUR_S(8, 16) AUGUR_S(8, 32) AUGUR_S(8, 64)
+  AUGUR_S(16, 8) AUGUR_S(16, 16) AUGUR_S(16, 32) AUGUR_S(16, 64)
+#undef AUGUR_S
+  fail(AUGUR_ERR_UNSUPPORTED, "no statistics kernel for padded dim %d, components %d", DM, KM);
 }
 
 struct GmmSampler final : augur_sampler {
@@ -466,19 +565,21 @@ struct GmmSampler final : augur_sampler {
     return v;
   }
 
-  size_t z_smem() const {
-    const int TS = m.TS;
-    return sizeof(float) * ((size_t)m.k * TS + (size_t)m.k * m.S + m.k);
-  }
-
   void refresh_tables() {
     k_gmm_tables<<<dim3(m.k, chains), kConjLanes, 0, stream>>>(m);
     AUGUR_CHECK_LAUNCH();
   }
 
-  void refresh_stats() {
-    k_gmm_stats<<<dim3(m.G, chains), kGmmThreads, sizeof(float) * m.k * m.S, stream>>>(m);
+  // statistics of the current z around the tables' mu
+  void launch_stats() {
+    pick_stats(m.DM, pad_comp(m.k))<<<dim3(m.G, chains), kGmmThreads,
+                                      gmm_stats_smem_bytes(m.DM, pad_comp(m.k), m.S), stream>>>(m);
     AUGUR_CHECK_LAUNCH();
+  }
+
+  void refresh_stats() {
+    refresh_tables();  // the statistics are centred at the tables' (current) mu
+    launch_stats();
     stats_valid = true;
   }
 
@@ -503,9 +604,10 @@ struct GmmSampler final : augur_sampler {
       k_gmm_conj<<<dim3(m.k, chains), kConjLanes, 0, stream>>>(m, seed, sweep, 1);
       AUGUR_CHECK_LAUNCH();
       prof_begin();
-      zk<<<dim3(m.G, chains), kGmmThreads, z_smem(), stream>>>(m, seed, sweep, 0u, 1);
+      zk<<<dim3(m.G, chains), kGmmThreads, 0, stream>>>(m, seed, sweep, 0u);
       AUGUR_CHECK_LAUNCH();
       prof_end();
+      launch_stats();  // for the next sweep's conjugate draws
       ++sweep;
       record_traces();
     }
@@ -622,7 +724,7 @@ augur_status augur_op_gmm_zstep(int n, int d, int k, const float* y, const doubl
     s.cov64.upload(cov, (size_t)k * d * d, s.stream);
     s.refresh_tables();
     const ZKernel zk = pick_z(s.m.DM, pad_comp(k));
-    zk<<<dim3(s.m.G, 1), kGmmThreads, s.z_smem(), s.stream>>>(s.m, seed, sweep, chain, 0);
+    zk<<<dim3(s.m.G, 1), kGmmThreads, 0, s.stream>>>(s.m, seed, sweep, chain);
     AUGUR_CHECK_LAUNCH();
     s.z.download(z, n, s.stream);
     AUGUR_HIP(hipStreamSynchronize(s.stream));

@@ -503,8 +503,8 @@ __global__ __launch_bounds__(kPassThreads, kPassBlocksPerCU) void k_hlr_pass_t(P
 // X, and each wave starts on its first tile while the later ones arrive.
 constexpr int kRowFeat = 8;  // features per lane in the register form
 // rows per lane per tile and tiles in flight: 32 KB of X per block either way
-__host__ __device__ constexpr int row_rpt(int cp) { return cp >= 4 ? 1 : 2; }
-__host__ __device__ constexpr int row_ring(int cp) { return cp >= 4 ? 4 : 2; }
+__host__ __device__ constexpr int row_rpt(int cp) { return 2; }
+__host__ __device__ constexpr int row_ring(int cp) { return 2; }
 
 __device__ inline float f4c(const float4& v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w; }
 
