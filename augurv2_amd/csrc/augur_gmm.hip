@@ -490,14 +490,15 @@ inline size_t gmm_stats_smem_bytes(int DM, int KM, int S) {
 }
 
 // Data part of the log joint: sum_n log pis_z + mvnormal_ll(y_n; mu_z, cov_z),
-// FP32 per point from the tables, FP64 accumulation.  One block per chain.
+// FP32 per point from the tables, FP64 accumulation.  Grid (G, chains): one
+// partial per block in part[c G + g], summed in order by k_gmm_ll_sum.
 template <int DM>
-__global__ __launch_bounds__(kGmmThreads) void k_gmm_ll(GmmDev m, double* out) {
+__global__ __launch_bounds__(kGmmThreads) void k_gmm_ll(GmmDev m, double* part) {
   constexpr int TS = tri(DM) + DM + 1;
-  const int c = blockIdx.x, tid = threadIdx.x, d = m.d, K = m.k;
+  const int c = blockIdx.y, tid = threadIdx.x, d = m.d, K = m.k;
   __shared__ double red[kGmmThreads];
   double accum = 0.0;
-  for (int n = tid; n < m.n; n += kGmmThreads) {
+  for (int n = blockIdx.x * kGmmThreads + tid; n < m.n; n += gridDim.x * kGmmThreads) {
     const int zk = m.z[(size_t)c * m.n + n];
     const float* t = m.tab + ((size_t)c * K + zk) * m.TS;
     float dv[DM];
@@ -516,9 +517,90 @@ __global__ __launch_bounds__(kGmmThreads) void k_gmm_ll(GmmDev m, double* out) {
   if (tid == 0) {
     double s = 0.0;
     for (int i = 0; i < kGmmThreads; ++i) s += red[i];
-    out[c] = s;
+    part[(size_t)c * gridDim.x + blockIdx.x] = s;
   }
   (void)TS;
+}
+
+// Prior part of the log density per (component k, chain c), FP64:
+// MvNormal(mu_k | mu0, cov0) (cbits/augur_dist.c:548-567) and, when cov is
+// sampled, InvWishart(cov_k | df, scale0) (cbits/augur_dist.c:679-711, with
+// the reference's log Gamma_d, cbits/augur_math.c:80-86).  out[c K + k].
+__global__ __launch_bounds__(kConjLanes) void k_gmm_prior_ll(GmmDev m, double* out) {
+  const int k = blockIdx.x, c = blockIdx.y, tid = threadIdx.x, d = m.d;
+  __shared__ double a[kMaxDim * kMaxDim], tmp[kMaxDim * kMaxDim], v[kMaxDim];
+  __shared__ double det_s;
+  for (int i = tid; i < d * d; i += kConjLanes) a[i] = m.cov0[i];
+  for (int i = tid; i < d; i += kConjLanes) v[i] = m.mu64[((size_t)c * m.k + k) * d + i] - m.mu0[i];
+  __syncthreads();
+  chol(a, d);
+  if (tid == 0) {
+    double det = 1.0;
+    for (int i = 0; i < d; ++i) det *= a[i * d + i];
+    det_s = det * det;
+  }
+  lower_trmi(a, d, tmp);
+  __syncthreads();
+  double ll = 0.0;
+  if (tid == 0) {
+    double q = 0.0;
+    for (int i = 0; i < d; ++i) {
+      double w = 0.0;
+      for (int j = 0; j <= i; ++j) w += a[i * d + j] * v[j];
+      q += w * w;
+    }
+    ll = -0.5 * (double)d * log(2.0 * 3.14159265358979323846) - 0.5 * log(det_s) - 0.5 * q;
+  }
+  if (m.sample_cov) {
+    __syncthreads();
+    for (int i = tid; i < d * d; i += kConjLanes) a[i] = m.scale0[i];
+    __syncthreads();
+    chol(a, d);
+    double det_psi = 1.0;
+    if (tid == 0) {
+      for (int i = 0; i < d; ++i) det_psi *= a[i * d + i];
+      det_psi *= det_psi;
+    }
+    __syncthreads();
+    const double* cv = m.cov64 + ((size_t)c * m.k + k) * d * d;
+    for (int i = tid; i < d * d; i += kConjLanes) a[i] = cv[i];
+    __syncthreads();
+    chol(a, d);
+    double det_x = 1.0;
+    if (tid == 0) {
+      for (int i = 0; i < d; ++i) det_x *= a[i * d + i];
+      det_x *= det_x;
+    }
+    chol_invert(a, d, tmp);
+    __syncthreads();
+    if (tid == 0) {
+      double tr = 0.0;
+      for (int i = 0; i < d; ++i) {
+        double t = 0.0;
+        for (int j = 0; j < d; ++j) t += m.scale0[i * d + j] * a[j * d + i];
+        tr += t;
+      }
+      double mlg = (d * (d - 1) * 0.25) * log(3.1415);
+      for (int i = 1; i < d + 1; ++i) mlg += lgamma(m.df / 2.0 - (i - 1.0) / 2.0);
+      const double cst = -m.df * d / 2.0 * log(2.0) - mlg;
+      ll += cst - (m.df + d + 1.0) / 2.0 * log(det_x) + m.df / 2.0 * log(det_psi) - 0.5 * tr;
+    }
+  }
+  if (tid == 0) out[(size_t)c * m.k + k] = ll;
+}
+
+// out[c] = data partials (in order) + component priors + Dirichlet(pis |
+// alpha) without its normaliser (cbits/augur_dist.c:488-496) when pis is
+// sampled.
+__global__ void k_gmm_ll_sum(GmmDev m, const double* part, const double* prior, double* out) {
+  if (threadIdx.x != 0) return;
+  const int c = blockIdx.x;
+  double s = 0.0;
+  for (int g = 0; g < m.G; ++g) s += part[(size_t)c * m.G + g];
+  for (int k = 0; k < m.k; ++k) s += prior[(size_t)c * m.k + k];
+  if (m.sample_pi)
+    for (int k = 0; k < m.k; ++k) s += (m.alpha[k] - 1.0) * log(m.pis64[(size_t)c * m.k + k]);
+  out[c] = s;
 }
 
 static int pad_dim(int d) { return d <= 4 ? 4 : (d <= 8 ? 8 : 16); }
@@ -614,11 +696,19 @@ struct GmmSampler final : augur_sampler {
   }
 
   void log_density(double* out) override {
-    // data part on the device, priors (reference ll_mod terms) on the host
+    // the model's log density (the reference's ll_mod): data terms per point
+    // block, prior terms per component, summed per chain in a fixed order
     const int DM = m.DM;
-    if (DM == 4) k_gmm_ll<4><<<chains, kGmmThreads, 0, stream>>>(m, ll.p);
-    else if (DM == 8) k_gmm_ll<8><<<chains, kGmmThreads, 0, stream>>>(m, ll.p);
-    else k_gmm_ll<16><<<chains, kGmmThreads, 0, stream>>>(m, ll.p);
+    const dim3 grid(m.G, chains);
+    double* lp = ll.p + chains;                  // chains x G data partials
+    double* pr = lp + (size_t)chains * m.G;      // chains x K component priors
+    if (DM == 4) k_gmm_ll<4><<<grid, kGmmThreads, 0, stream>>>(m, lp);
+    else if (DM == 8) k_gmm_ll<8><<<grid, kGmmThreads, 0, stream>>>(m, lp);
+    else k_gmm_ll<16><<<grid, kGmmThreads, 0, stream>>>(m, lp);
+    AUGUR_CHECK_LAUNCH();
+    k_gmm_prior_ll<<<dim3(m.k, chains), kConjLanes, 0, stream>>>(m, pr);
+    AUGUR_CHECK_LAUNCH();
+    k_gmm_ll_sum<<<chains, 64, 0, stream>>>(m, lp, pr, ll.p);
     AUGUR_CHECK_LAUNCH();
     ll.download(out, chains, stream);
     AUGUR_HIP(hipStreamSynchronize(stream));
@@ -655,7 +745,7 @@ static void gmm_setup(GmmSampler* s, int n, int d, int k, int chains) {
   s->tab.alloc((size_t)chains * k * m.TS);
   s->part.alloc((size_t)chains * m.G * k * m.S);
   s->z.alloc((size_t)chains * n);
-  s->ll.alloc(chains);
+  s->ll.alloc((size_t)chains * (1 + m.G + k));  // per-chain sums, data partials, priors
   m.y = s->y.p; m.alpha = s->alpha.p; m.mu0 = s->mu0.p; m.cov0 = s->cov0.p; m.scale0 = s->scale0.p;
   m.gpi = s->gpi.p; m.pis64 = s->pis64.p; m.mu64 = s->mu64.p; m.cov64 = s->cov64.p;
   m.tab = s->tab.p; m.part = s->part.p; m.z = s->z.p;

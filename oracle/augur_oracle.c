@@ -312,6 +312,69 @@ double or_mvnormal_ll(const double* pt, const double* mean, const double* cov, i
   return c - 0.5 * log(det) - 0.5 * q;
 }
 
+/* cbits/augur_math.c:80-86: log of the multivariate Gamma function, with
+ * the reference's constant log(3.1415) in place of log(pi). */
+double or_multi_lgamma(int p, double x) {
+  double r = (p * (p - 1) * 0.25) * log(3.1415);
+  for (int i = 1; i < p + 1; i++) r += lgamma(x - (i - 1.0) / 2.0);
+  return r;
+}
+
+/* cbits/augur_dist.c:488-496: sum (alpha_i - 1) log pt_i (the reference
+ * leaves out the normalising constant). */
+double or_dirichlet_ll(const double* pt, const double* alpha, int k) {
+  double acc = 0.0;
+  for (int i = 0; i < k; i++) acc += (alpha[i] - 1.0) * log(pt[i]);
+  return acc;
+}
+
+/* cbits/augur_dist.c:679-711: determinants from the Cholesky factors'
+ * diagonals, tr(scale X^-1) through the Cholesky inverse. */
+double or_invwishart_ll(const double* pt, int df, const double* scale, int d) {
+  double l[32 * 32], w[32 * 32];
+  memcpy(l, scale, sizeof(double) * d * d);
+  or_mat_cholesky(l, d);
+  double det_psi = 1.0;
+  for (int i = 0; i < d; i++) det_psi *= l[i * d + i];
+  det_psi *= det_psi;
+  memcpy(w, pt, sizeof(double) * d * d);
+  or_mat_cholesky(w, d);
+  double det_x = 1.0;
+  for (int i = 0; i < d; i++) det_x *= w[i * d + i];
+  det_x *= det_x;
+  or_mat_cholesky_invert(w, d);
+  double tr = 0.0;
+  for (int i = 0; i < d; i++) {
+    double t = 0.0;
+    for (int j = 0; j < d; j++) t += scale[i * d + j] * w[j * d + i];
+    tr += t;
+  }
+  const double c = -df * d / 2.0 * log(2.0) - or_multi_lgamma(d, df / 2.0);
+  return c - (df + d + 1.0) / 2.0 * log(det_x) + df / 2.0 * log(det_psi) - 0.5 * tr;
+}
+
+/* The model's log density (the generated ll_mod behind augur_get_curr_ll,
+ * CudaC/CgStInit.hs runCgCallLLMod) for hmvgmm.rv: Dirichlet(pis | alpha)
+ * (sample_pi), MvNormal(mu_k | mu0, cov0), InvWishart(cov_k | df, scale0)
+ * (sample_cov), Categorical(z_n | pis), MvNormal(y_n | mu_z, cov_z).  With
+ * sample_pi / sample_cov = 0 (mvgmm.rv) pis and cov are hyperparameters and
+ * carry no term. */
+double or_gmm_ll_mod(const or_gmm_model* m, const double* pis, const double* mu, const double* cov,
+                     const int* z) {
+  const int d = m->d, k = m->k;
+  double ll = 0.0;
+  if (m->sample_pi) ll += or_dirichlet_ll(pis, m->alpha, k);
+  for (int c = 0; c < k; c++) {
+    ll += or_mvnormal_ll(mu + c * d, m->mu0, m->cov0, d);
+    if (m->sample_cov) ll += or_invwishart_ll(cov + c * d * d, m->df, m->scale0, d);
+  }
+  for (int n = 0; n < m->n; n++) {
+    const int zn = z[n];
+    ll += log(pis[zn]) + or_mvnormal_ll(m->y + n * d, mu + zn * d, cov + zn * d * d, d);
+  }
+  return ll;
+}
+
 /* cbits/augur_dist.c:616-630 with the standard normals from sites site0 + i. */
 static void or_mvnormal_sample(uint32_t seed, uint32_t chain, uint32_t sweep, uint32_t tag,
                                uint32_t site0, const double* mean, const double* cov,

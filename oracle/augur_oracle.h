@@ -144,6 +144,11 @@ void or_gmm_sweep(const or_gmm_model* m, double* pis, double* mu, double* cov, i
                   uint32_t seed, uint32_t chain, uint32_t sweep);
 void or_gmm_zstep(const or_gmm_model* m, const double* pis, const double* mu,
                   const double* cov, int* z, uint32_t seed, uint32_t chain, uint32_t sweep);
+double or_multi_lgamma(int p, double x);
+double or_dirichlet_ll(const double* pt, const double* alpha, int k);
+double or_invwishart_ll(const double* pt, int df, const double* scale, int d);
+double or_gmm_ll_mod(const or_gmm_model* m, const double* pis, const double* mu, const double* cov,
+                     const int* z);
 
 #ifdef __cplusplus
 }

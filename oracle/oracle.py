@@ -95,6 +95,10 @@ def lib():
                                         U32, P_D]),
             "or_gmm_sweep": (None, [C.POINTER(GmmModel), P_D, P_D, P_D, P_I, U32, U32, U32]),
             "or_gmm_zstep": (None, [C.POINTER(GmmModel), P_D, P_D, P_D, P_I, U32, U32, U32]),
+            "or_multi_lgamma": (C.c_double, [C.c_int, C.c_double]),
+            "or_dirichlet_ll": (C.c_double, [P_D, P_D, C.c_int]),
+            "or_invwishart_ll": (C.c_double, [P_D, C.c_int, P_D, C.c_int]),
+            "or_gmm_ll_mod": (C.c_double, [C.POINTER(GmmModel), P_D, P_D, P_D, P_I]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -168,6 +172,18 @@ def trvm(l, v):
 
 def mvnormal_ll(pt, mean, cov):
     return lib().or_mvnormal_ll(_sq(pt), _sq(mean), _sq(cov), len(pt))
+
+
+def multi_lgamma(p, x):
+    return lib().or_multi_lgamma(int(p), float(x))
+
+
+def dirichlet_ll(pt, alpha):
+    return lib().or_dirichlet_ll(_sq(pt), _sq(alpha), len(alpha))
+
+
+def invwishart_ll(pt, df, scale):
+    return lib().or_invwishart_ll(_sq(pt), int(df), _sq(scale), len(scale))
 
 
 def mvnormal_conj(seed, chain, sweep, site0, mu0, cov0, s, cnt, cov):
@@ -276,3 +292,8 @@ class Gmm:
         z = np.zeros(self.m.n, np.int32)
         lib().or_gmm_zstep(C.byref(self.m), _sq(pis), _sq(mu), _sq(cov), z, seed, chain, sweep)
         return z
+
+    def ll_mod(self, pis, mu, cov, z):
+        """The model's log density at (pis, mu, cov, z) (augur_get_curr_ll)."""
+        return lib().or_gmm_ll_mod(C.byref(self.m), _sq(pis), _sq(mu), _sq(cov),
+                                   np.ascontiguousarray(z, np.int32))

@@ -157,6 +157,33 @@ def _gmm_setup(rng, n, d, k):
 
 
 @pytest.mark.parametrize("sample_pi,sample_cov", [(True, True), (False, False)])
+def test_gmm_log_density_matches_oracle(rt, oracle, sample_pi, sample_cov):
+    """augur_get_curr_ll is the model's log density (the reference's ll_mod):
+    priors of pis / mu / cov and the categorical and data terms, at a set
+    state and after sweeps.  Data terms in FP32 per point (2e-6 relative)."""
+    from scipy.stats import invwishart
+    rng = np.random.default_rng(21)
+    n, d, k, C = 5000, 5, 7, 2
+    y, _ = _gmm_setup(rng, n, d, k)
+    alpha, mu0, cov0, df, scale0 = rng.uniform(0.5, 2, k), rng.standard_normal(d), 6 * np.eye(d), d + 3, np.eye(d)
+    s = rt.GmmSampler(y, alpha, mu0, cov0, df, scale0, chains=C, seed=4, sample_pi=sample_pi,
+                      sample_cov=sample_cov)
+    g = oracle.Gmm(y.astype(np.float64), alpha, mu0, cov0, df, scale0, sample_pi, sample_cov)
+    for c in range(C):
+        s.set("pis", rng.dirichlet(np.ones(k)), chain=c)
+        s.set("mu", rng.standard_normal((k, d)) * 2, chain=c)
+        s.set("cov", np.stack([invwishart.rvs(df, scale0, random_state=rng) for _ in range(k)]), chain=c)
+        s.set("z", rng.integers(0, k, n).astype(np.int32), chain=c)
+    for step in (0, 2):
+        s.step(step)
+        ll = s.log_density()
+        for c in range(C):
+            ref = g.ll_mod(s.get("pis", c), s.get("mu", c), s.get("cov", c), s.get("z", c))
+            assert abs(ll[c] - ref) <= 2e-6 * abs(ref), (step, c, ll[c], ref)
+    s.close()
+
+
+@pytest.mark.parametrize("sample_pi,sample_cov", [(True, True), (False, False)])
 def test_gmm_sweeps_match_oracle(rt, oracle, sample_pi, sample_cov):
     rng = np.random.default_rng(8)
     n, d, k, C = 3000, 4, 3, 2

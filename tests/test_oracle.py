@@ -367,3 +367,61 @@ def test_gmm_sweep_recovers_clusters(oracle):
         g.sweep(pis, mu, cov, z, 1, 0, s)
     found = sorted(map(tuple, np.round(mu, 0)))
     assert found == sorted(map(tuple, centers))
+
+
+def test_multi_lgamma_reference_constant(oracle):
+    """cbits/augur_math.c:80-86 is log Gamma_p(x) with log(3.1415) for
+    log(pi): scipy's multigammaln plus that constant's difference."""
+    from scipy.special import multigammaln
+    for p, x in [(1, 0.7), (2, 3.0), (5, 4.5), (16, 9.0)]:
+        quirk = p * (p - 1) * 0.25 * (np.log(3.1415) - np.log(np.pi))
+        assert abs(oracle.multi_lgamma(p, x) - (multigammaln(x, p) + quirk)) < 1e-11 * max(1.0, abs(x * p))
+
+
+def test_dirichlet_ll_without_normaliser(oracle):
+    """cbits/augur_dist.c:488-496: scipy's log density minus its normalising
+    constant log Gamma(sum a) - sum log Gamma(a_i)."""
+    from scipy.special import gammaln
+    rng = np.random.default_rng(4)
+    for k in (2, 5, 32):
+        alpha = rng.uniform(0.3, 3.0, k)
+        pt = rng.dirichlet(alpha)
+        ref = sps.dirichlet.logpdf(pt, alpha) - (gammaln(alpha.sum()) - gammaln(alpha).sum())
+        assert abs(oracle.dirichlet_ll(pt, alpha) - ref) < 1e-10 * (1 + abs(ref))
+
+
+@pytest.mark.parametrize("d,df", [(1, 3), (2, 5), (6, 9), (16, 18)])
+def test_invwishart_ll_against_scipy(oracle, d, df):
+    """cbits/augur_dist.c:679-711 against scipy.stats.invwishart.logpdf; the
+    only difference is the log(3.1415) inside the multivariate Gamma."""
+    rng = np.random.default_rng(d + df)
+    a = rng.standard_normal((d, d))
+    scale = a @ a.T + d * np.eye(d)
+    x = sps.invwishart.rvs(df, scale, random_state=rng).reshape(d, d)
+    quirk = d * (d - 1) * 0.25 * (np.log(3.1415) - np.log(np.pi))
+    ref = sps.invwishart.logpdf(x, df, scale) - quirk
+    assert abs(oracle.invwishart_ll(x, df, scale) - ref) < 1e-9 * (1 + abs(ref))
+
+
+@pytest.mark.parametrize("sample_pi,sample_cov", [(True, True), (False, False)])
+def test_gmm_ll_mod_terms(oracle, sample_pi, sample_cov):
+    """The GMM model log density is the sum of its variables' log densities
+    (hmvgmm.rv; mvgmm.rv without the pis and cov terms), each from scipy."""
+    from scipy.special import gammaln
+    rng = np.random.default_rng(12)
+    n, d, k, df = 200, 3, 4, 6
+    y = rng.standard_normal((n, d)) * 2
+    alpha, mu0, cov0, scale0 = rng.uniform(0.5, 2, k), rng.standard_normal(d), 4 * np.eye(d), np.eye(d)
+    g = oracle.Gmm(y, alpha, mu0, cov0, df, scale0, sample_pi=sample_pi, sample_cov=sample_cov)
+    pis = rng.dirichlet(np.ones(k))
+    mu = rng.standard_normal((k, d))
+    cov = np.stack([sps.invwishart.rvs(df, scale0, random_state=rng) for _ in range(k)])
+    z = rng.integers(0, k, n).astype(np.int32)
+    ref = sum(np.log(pis[z[i]]) + sps.multivariate_normal.logpdf(y[i], mu[z[i]], cov[z[i]]) for i in range(n))
+    ref += sum(sps.multivariate_normal.logpdf(mu[c], mu0, cov0) for c in range(k))
+    if sample_pi:
+        ref += sps.dirichlet.logpdf(pis, alpha) - (gammaln(alpha.sum()) - gammaln(alpha).sum())
+    if sample_cov:
+        quirk = d * (d - 1) * 0.25 * (np.log(3.1415) - np.log(np.pi))
+        ref += sum(sps.invwishart.logpdf(cov[c], df, scale0) - quirk for c in range(k))
+    assert abs(g.ll_mod(pis, mu, cov, z) - ref) < 1e-9 * abs(ref)
