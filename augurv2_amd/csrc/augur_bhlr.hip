@@ -21,6 +21,8 @@
 // replayed: every kernel reads the sweep counter from device memory.
 //
 // Philox chain key of (dataset i, chain c) is i * C + c.
+#include <type_traits>
+
 #include "augur_hmc.h"
 
 namespace augur {
@@ -301,6 +303,197 @@ __global__ __launch_bounds__(256) void k_bhlr_finish(BArgs A) {
     A.prop[si + j] = xv;
     A.beta32[gi] = (float)xv;
   }
+}
+
+// Register form of the pass for up to 4 chains per dataset (the form of the
+// single-dataset sampler's k_hlr_pass_r, on BF16 rows): lane l owns F
+// consecutive features of RPT rows per tile, LPR = max(CP, D / 8) lanes per
+// row and F = D / LPR (8, or 4 for D = 16 with 4 chains), kept as packed
+// BF16 until use.  Phase B: the lane's share of x . theta_c, completed over
+// the row's lanes by a DPP butterfly; each (row, chain) adjoint is formed
+// once (lane q takes pairs q, q + LPR, ..., all of chain q % CP) and
+// gathered back by ds_bpermute; phase C accumulates a x into the lane's
+// CP x F share in FP32; the block sums lanes in FP64 (LDS, fixed order).  A
+// ring of PD tiles per lane streams the block's rows without barriers.
+constexpr int kBrRpt = 2;   // rows per lane per tile
+constexpr int kBrRing = 4;  // tiles in flight per lane
+
+__device__ inline uint32_t xword_at(const uint4& v, int k) {
+  return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w;
+}
+__device__ inline uint32_t xword_at(const uint2& v, int k) { return k == 0 ? v.x : v.y; }
+
+template <int D, int CP>
+__device__ void bhlr_body_r(const BArgs& A, const BDs& ds, unsigned char* smem) {
+  constexpr int LPR = (CP > D / 8) ? CP : D / 8;
+  constexpr int F = D / LPR;
+  constexpr int FW = F / 2;  // 32-bit words of BF16 per lane and row
+  constexpr int RPT = kBrRpt, PD = kBrRing;
+  constexpr int RG = kBThreads / LPR;  // rows per block per row slot
+  constexpr int TRr = RG * RPT;        // rows per tile
+  constexpr int NC = RPT * CP;
+  constexpr int SPL = (NC + LPR - 1) / LPR;
+  constexpr int GS = CP * F + 1;
+  constexpr int VS = D + 2;
+  static_assert(LPR >= 2 && LPR <= 16 && LPR % CP == 0 && (F == 4 || F == 8), "shape");
+  typedef typename std::conditional<FW == 4, uint4, uint2>::type xword;
+
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int q = tid % LPR, rsub = tid / LPR, gbase = lane - q, my_c = q % CP;
+  const int local = blockIdx.x - ds.blk0;
+  const int rbeg = local * A.rows_per_blk;
+  const int rend = min(ds.n, rbeg + A.rows_per_blk);
+  const int ntile = rend > rbeg ? (rend - rbeg + TRr - 1) / TRr : 0;
+
+  float th[CP][F];
+#pragma unroll
+  for (int c = 0; c < CP; ++c)
+#pragma unroll
+    for (int j = 0; j < F; ++j)
+      th[c][j] = c < A.C ? A.beta32[(size_t)c * A.GW + ds.goff + F * q + j] : 0.0f;
+
+  xword pre[PD][RPT];
+  int ypre[PD][RPT];
+  auto prefetch = [&](int t, xword (&dst)[RPT], int (&yd)[RPT]) {
+#pragma unroll
+    for (int i = 0; i < RPT; ++i) {
+      const int row = rbeg + t * TRr + i * RG + rsub;
+      if (row < rend) {
+        dst[i] = *(const xword*)(ds.x + (size_t)row * D + F * q);
+        yd[i] = ds.y[row];
+      } else {
+        dst[i] = xword{};
+        yd[i] = 0;
+      }
+    }
+  };
+#pragma unroll
+  for (int k = 0; k < PD; ++k)
+    if (k < ntile) prefetch(k, pre[k], ypre[k]);
+
+  float g[CP][F];
+#pragma unroll
+  for (int c = 0; c < CP; ++c)
+#pragma unroll
+    for (int j = 0; j < F; ++j) g[c][j] = 0.0f;
+  double llacc = 0.0, gbacc = 0.0;
+
+  for (int base = 0; base < ntile; base += PD) {
+#pragma unroll
+    for (int slot = 0; slot < PD; ++slot) {
+      const int t = base + slot;
+      if (t >= ntile) break;
+      float xr[RPT][F];
+#pragma unroll
+      for (int i = 0; i < RPT; ++i)
+#pragma unroll
+        for (int k = 0; k < FW; ++k) {
+          const uint32_t w = xword_at(pre[slot][i], k);
+          xr[i][2 * k] = __uint_as_float(w << 16);
+          xr[i][2 * k + 1] = __uint_as_float(w & 0xffff0000u);
+        }
+      const int* yv = ypre[slot];
+      float tp[NC];
+#pragma unroll
+      for (int i = 0; i < RPT; ++i)
+#pragma unroll
+        for (int c = 0; c < CP; ++c) {
+          float s0 = xr[i][0] * th[c][0], s1 = xr[i][1] * th[c][1];
+#pragma unroll
+          for (int j = 2; j < F; j += 2) {
+            s0 = fmaf(xr[i][j], th[c][j], s0);
+            s1 = fmaf(xr[i][j + 1], th[c][j + 1], s1);
+          }
+          tp[i * CP + c] = row_sum<LPR>(s0 + s1);
+        }
+      float aown[SPL];
+#pragma unroll
+      for (int s = 0; s < SPL; ++s) {
+        const int m = q + s * LPR;
+        float tv = tp[0];
+#pragma unroll
+        for (int k = 1; k < NC; ++k) tv = (m == k) ? tp[k] : tv;
+        const int i = m / CP;
+        int yy = yv[0];
+#pragma unroll
+        for (int k = 1; k < RPT; ++k) yy = (i == k) ? yv[k] : yy;
+        const bool ok = m < NC && my_c < A.C && rbeg + t * TRr + i * RG + rsub < rend;
+        float a;
+        const float lr = log_sigmoid_pm(tv, yy, &a);
+        if (ok) {
+          llacc += (double)lr;
+          gbacc += (double)a;
+        } else {
+          a = 0.0f;
+        }
+        aown[s] = a;
+      }
+#pragma unroll
+      for (int m = 0; m < NC; ++m) {
+        const float av = __shfl(aown[m / LPR], gbase + m % LPR);
+        const int i = m / CP, c = m % CP;
+#pragma unroll
+        for (int j = 0; j < F; ++j) g[c][j] = fmaf(av, xr[i][j], g[c][j]);
+      }
+      if (t + PD < ntile) prefetch(t + PD, pre[slot], ypre[slot]);
+    }
+  }
+
+  // partials of this block: part[poff + value * nblk + local], real chains only
+  float* gl = (float*)smem;  // [256][GS]
+  double* red = (double*)(smem + ((sizeof(float) * kBThreads * GS + 15) & ~(size_t)15));  // [2][4][CP]
+#pragma unroll
+  for (int c = 0; c < CP; ++c)
+#pragma unroll
+    for (int j = 0; j < F; ++j) gl[tid * GS + c * F + j] = g[c][j];
+#pragma unroll
+  for (int off = 32; off >= CP; off >>= 1) {
+    gbacc += __shfl_xor(gbacc, off);
+    llacc += __shfl_xor(llacc, off);
+  }
+  if (lane < CP) {
+    red[wv * CP + lane] = gbacc;
+    red[4 * CP + wv * CP + lane] = llacc;
+  }
+  __syncthreads();
+  double* part = A.part + ds.poff;
+  for (int v = tid; v < A.C * D; v += kBThreads) {
+    const int c = v / D, j = v % D, qq = j / F, jj = j % F;
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < kBThreads / 64; ++w)
+#pragma unroll
+      for (int k = 0; k < 64 / LPR; ++k) s += (double)gl[(w * 64 + k * LPR + qq) * GS + c * F + jj];
+    part[(size_t)(c * VS + j) * ds.nblk + local] = s;
+  }
+  if (tid < A.C) {
+    double sb = 0.0, sl = 0.0;
+#pragma unroll
+    for (int w = 0; w < kBThreads / 64; ++w) {
+      sb += red[w * CP + tid];
+      sl += red[4 * CP + w * CP + tid];
+    }
+    part[(size_t)(tid * VS + D) * ds.nblk + local] = sb;
+    part[(size_t)(tid * VS + D + 1) * ds.nblk + local] = sl;
+  }
+}
+
+template <int CP>
+__global__ __launch_bounds__(kBThreads) void k_bhlr_pass_r(BArgs A) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const BDs ds = A.ds[A.blk_ds[blockIdx.x]];
+  switch (ds.d) {
+    case 16: bhlr_body_r<16, CP>(A, ds, smem); break;
+    case 32: bhlr_body_r<32, CP>(A, ds, smem); break;
+    case 64: bhlr_body_r<64, CP>(A, ds, smem); break;
+    default: bhlr_body_r<128, CP>(A, ds, smem); break;
+  }
+}
+
+// LDS of the register form: the lanes' gradient shares, then the chains' sums.
+size_t bhlr_r_smem_bytes(int cp) {
+  int f = 8;  // F * CP is largest at F = 8
+  return ((sizeof(float) * kBThreads * (cp * f + 1) + 15) & ~(size_t)15) + sizeof(double) * 8 * cp;
 }
 
 template <int CP>
@@ -684,6 +877,14 @@ static BPassKernel pick_bpass(int cp) {
   }
 }
 
+static BPassKernel pick_bpass_r(int cp) {
+  switch (cp) {
+    case 1: return k_bhlr_pass_r<1>;
+    case 2: return k_bhlr_pass_r<2>;
+    default: return k_bhlr_pass_r<4>;
+  }
+}
+
 struct BhlrSampler final : augur_sampler {
   int n_ds = 0, C = 1, SW = 0, GW = 0, dmax = 16, R = 4096, nblk_total = 0;
   std::vector<BDs> hds;
@@ -1043,9 +1244,20 @@ augur_status augur_bhlr_create(const augur_bhlr_desc* D, augur_sampler** out) {
       // matrix cores from 5 chains on (the 16 chain columns of a product are
       // then at least a third used); AUGUR_BHLR_VECTOR=1 keeps the vector form
       const char* vec = getenv("AUGUR_BHLR_VECTOR");
-      s->mfma = s->next_cp() >= 8 && !(vec && atoi(vec));
-      s->pass = s->mfma ? k_bhlr_pass_mfma : pick_bpass(s->next_cp());
-      s->pass_smem = s->mfma ? bhlr_mfma_smem_bytes(s->dmax) : bhlr_smem_bytes(s->dmax, s->next_cp());
+      // up to 4 chains the register form, AUGUR_BHLR_PASS_LDS=1 the LDS-staged one
+      const char* lds = getenv("AUGUR_BHLR_PASS_LDS");
+      const int cpn = s->next_cp();
+      s->mfma = cpn >= 8 && !(vec && atoi(vec));
+      if (s->mfma) {
+        s->pass = k_bhlr_pass_mfma;
+        s->pass_smem = bhlr_mfma_smem_bytes(s->dmax);
+      } else if (cpn <= 4 && !(lds && atoi(lds))) {
+        s->pass = pick_bpass_r(cpn);
+        s->pass_smem = bhlr_r_smem_bytes(cpn);
+      } else {
+        s->pass = pick_bpass(cpn);
+        s->pass_smem = bhlr_smem_bytes(s->dmax, cpn);
+      }
       allow_lds(s->pass, s->pass_smem);
       // state starts at theta = 0, v = 1
       std::vector<double> st(C * s->SW, 0.0);

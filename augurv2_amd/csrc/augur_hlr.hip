@@ -508,20 +508,6 @@ __host__ __device__ constexpr int row_ring(int cp) { return 2; }
 
 __device__ inline float f4c(const float4& v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w; }
 
-template <int CTRL>
-__device__ inline float dpp_f(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
-}
-
-// sum over the LPR lanes of a row (LPR = 2, 4 or 8, aligned groups)
-template <int LPR>
-__device__ inline float row_sum(float v) {
-  v += dpp_f<0xB1>(v);                     // lanes i ^ 1 (quad_perm 1,0,3,2)
-  if constexpr (LPR >= 4) v += dpp_f<0x4E>(v);   // lanes i ^ 2 (quad_perm 2,3,0,1)
-  if constexpr (LPR >= 8) v += dpp_f<0x141>(v);  // half-row mirror: quad 0 <-> quad 1
-  return v;
-}
-
 template <int D, int CP>
 __global__ __launch_bounds__(kPassThreads, kPassBlocksPerCU) void k_hlr_pass_r(PassArgs A) {
   extern __shared__ __align__(16) unsigned char smem[];

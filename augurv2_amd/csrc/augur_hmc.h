@@ -144,6 +144,24 @@ __device__ inline float log_sigmoid_pm(float t, int y, float* a_out) {
   return fminf(st, 0.0f) - l1p;
 }
 
+template <int CTRL>
+__device__ inline float dpp_f(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+// Sum over the LPR lanes of an aligned group (LPR = 2, 4, 8 or 16), every
+// lane of the group receiving it: DPP quad permutations for lanes i ^ 1 and
+// i ^ 2, then the half-row mirror (quad 0 <-> quad 1 within 8 lanes) and the
+// row mirror (the two halves of a 16-lane row).
+template <int LPR>
+__device__ inline float row_sum(float v) {
+  v += dpp_f<0xB1>(v);                            // quad_perm 1,0,3,2
+  if constexpr (LPR >= 4) v += dpp_f<0x4E>(v);    // quad_perm 2,3,0,1
+  if constexpr (LPR >= 8) v += dpp_f<0x141>(v);   // row_half_mirror
+  if constexpr (LPR >= 16) v += dpp_f<0x140>(v);  // row_mirror
+  return v;
+}
+
 __device__ inline double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
   return v;

@@ -44,14 +44,17 @@ def _init(s, rng, C):
     return starts
 
 
+@pytest.mark.parametrize("lds", [False, True])
 @pytest.mark.parametrize("L", [8, -8])
-def test_batched_matches_oracle(rt, oracle, L):
+def test_batched_matches_oracle(rt, oracle, monkeypatch, L, lds):
     """L > 0: fixed step and count; L < 0: the reference's step jitter with
-    |L| steps (every chain runs the same number of passes).  HMC on v with
+    |L| steps (every chain runs the same number of passes).  The register
+    form of the pass, and the LDS-staged form forced.  HMC on v with
     steps of 0.01: at 0.05 its leapfrog is unstable on dataset 3 and the
     oracle alone turns a 1e-7 nudge of theta into 28 times this test's
     tolerance within three sweeps; at 0.01 the same nudge stays below a
     quarter of it on every dataset."""
+    monkeypatch.setenv("AUGUR_BHLR_PASS_LDS", "1" if lds else "0")
     rng = np.random.default_rng(0)
     xs, ys = _datasets(rng, SHAPES)
     C, EPS = 2, 0.01
