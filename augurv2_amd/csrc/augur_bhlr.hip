@@ -537,6 +537,7 @@ __device__ inline uint16_t bf16_rne(float f) {
 }
 
 constexpr int kMfmaCols = 16;
+constexpr int kMfRing = 4;  // 16-row tiles in flight per wave in the matrix-core form
 
 template <int D>
 constexpr int mfma_slab_stride() { return D + 8; }  // BF16 elements per LDS row
@@ -599,33 +600,45 @@ __device__ void bhlr_body_mfma(const BArgs& A, const BDs& ds, unsigned char* sme
   const int local = blockIdx.x - ds.blk0;
   const int rbeg = local * A.rows_per_blk;
   const int rend = min(ds.n, rbeg + A.rows_per_blk);
-  // the next tile's operands and labels are loaded while this one computes
-  braw qn[KC];
-  int yn[4];
-  auto load_tile = [&](int t0) {
+  // this wave's tiles are rows w0 + 64 t; a ring of kMfRing tiles stays in
+  // flight, the slot refilled as soon as its tile is in registers and LDS
+  constexpr int PD = D >= 128 ? 2 : kMfRing;  // 128 features: half the ring for registers
+  const int w0 = rbeg + 16 * wv;
+  const int ntw = w0 < rend ? (rend - w0 + 63) / 64 : 0;
+  braw qn[PD][KC];
+  int yn[PD][4];
+  auto load_tile = [&](int t, braw (&qd)[KC], int (&yd)[4]) {
+    const int t0 = w0 + 64 * t;
     const int row = t0 + li;
 #pragma unroll
     for (int s = 0; s < KC; ++s) {
-      qn[s] = braw{};
-      if (row < rend) qn[s] = *(const braw*)(ds.x + (size_t)row * D + KB * s + KE * kg);
+      qd[s] = braw{};
+      if (row < rend) qd[s] = *(const braw*)(ds.x + (size_t)row * D + KB * s + KE * kg);
     }
 #pragma unroll
-    for (int r = 0; r < 4; ++r) yn[r] = t0 + 4 * kg + r < rend ? (int)ds.y[t0 + 4 * kg + r] : 0;
+    for (int r = 0; r < 4; ++r) yd[r] = t0 + 4 * kg + r < rend ? (int)ds.y[t0 + 4 * kg + r] : 0;
   };
-  if (rbeg + 16 * wv < rend) load_tile(rbeg + 16 * wv);
-  for (int r0 = rbeg + 16 * wv; r0 < rend; r0 += 64) {
+#pragma unroll
+  for (int k = 0; k < PD; ++k)
+    if (k < ntw) load_tile(k, qn[k], yn[k]);
+  for (int base = 0; base < ntw; base += PD) {
+#pragma unroll
+  for (int slot = 0; slot < PD; ++slot) {
+    const int tt = base + slot;
+    if (tt >= ntw) break;
+    const int r0 = w0 + 64 * tt;
     // phase B: A operand = row r0 + li, features KB s + KE kg .. + KE - 1
     bvec xa[KC];
     int yv[4];
 #pragma unroll
     for (int s = 0; s < KC; ++s) {
-      const braw q = qn[s];
+      const braw q = qn[slot][s];
       *(braw*)(slab + li * XS + KB * s + KE * kg) = q;
       xa[s] = __builtin_bit_cast(bvec, q);
     }
 #pragma unroll
-    for (int r = 0; r < 4; ++r) yv[r] = yn[r];
-    if (r0 + 64 < rend) load_tile(r0 + 64);
+    for (int r = 0; r < 4; ++r) yv[r] = yn[slot][r];
+    if (tt + PD < ntw) load_tile(tt + PD, qn[slot], yn[slot]);
     f32x4_t t = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int s = 0; s < KC; ++s) {
@@ -668,6 +681,7 @@ __device__ void bhlr_body_mfma(const BArgs& A, const BDs& ds, unsigned char* sme
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
   }
   // g[f][r] = G[feature 16 f + 4 kg + r][chain li]; ll per (kg, chain li)
   ll += __shfl_xor(ll, 16);

@@ -147,9 +147,10 @@ def run_batched(sweeps=1000, warmup=500, chains=4, seed=31):
     """256 HLR datasets (N_i ~ U[1k, 200k], D_i in {16, 32, 64}) in one
     batched sampler (BF16 X, autotuned rows per block, hipGraph sweep), 20
     leapfrog steps of 0.005 with the step jittered, MWG [v] ~ Normal(v, 1.0).
-    Beside it the plain
-    sampler's time for the same sweeps on each dataset alone, measured on
-    eight datasets and interpolated linearly in N_i D_i."""
+    Up to 4 chains per dataset the data pass runs on the vector units, from
+    5 on the matrix cores.  Beside it the plain sampler's time for the same
+    sweeps and chains on each dataset alone, measured on eight datasets and
+    interpolated linearly in N_i D_i."""
     from augurv2_amd import runtime, workloads
     from augurv2_amd._lib import HMC_BLOCK_THETA, MWG_BLOCK_V
     L, EPS = 20, 0.005
@@ -198,6 +199,8 @@ def run_batched(sweeps=1000, warmup=500, chains=4, seed=31):
     a, b = np.polyfit(work[pick], np.array(t_plain), 1)[::-1]
     plain_total = float(np.sum(a + b * work)) * sweeps
     return {"datasets": len(xs), "chains": chains, "rows_total": int(sum(x.shape[0] for x in xs)),
+            "data_pass": ("matrix cores: BF16 V_MFMA, FP32 accumulation" if chains >= 5
+                          else "vector units: register form, BF16 rows, FP32 arithmetic"),
             "rows_per_block": rows, "autotune_us": cand, "sweeps": sweeps, "elapsed_s": el,
             "warmup_s": t_warm, "step_size_range": [float(eps_i.min()), float(eps_i.max())],
             "v_proposal_var_range": [float(vvar_i.min()), float(vvar_i.max())],
@@ -306,6 +309,10 @@ def main():
     bat = run_batched() if "batched" in legs else None
     if bat:
         log(f"batched: {bat}")
+    # 16 chains per dataset: the chains-by-rows products on the matrix cores
+    bat16 = run_batched(chains=16) if "batched" in legs else None
+    if bat16:
+        log(f"batched (16 chains): {bat16}")
     gmm = run_gmm() if "gmm" in legs else None
     if gmm:
         log(f"gmm: {gmm}")
@@ -342,6 +349,7 @@ def main():
         "lda": {k: lda[k] for k in ("tokens", "sweeps", "sweep_us", "zstep_avg_us")} if lda else None,
         "surrogate": surr,
         "batched": bat,
+        "batched_16_chains": bat16,
         "gmm": gmm,
         "roofline": {"kernel": "k_hlr_pass_r<64,4>", "bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
