@@ -63,6 +63,17 @@ augur_sampler::~augur_sampler() {
   if (stream) (void)hipStreamDestroy(stream);
 }
 
+std::pair<hipEvent_t, hipEvent_t>* augur_sampler::prof_slot() {
+  if (!prof_on()) return nullptr;
+  if (prof_used == prof_events.size()) {
+    std::pair<hipEvent_t, hipEvent_t> e;
+    AUGUR_HIP(hipEventCreate(&e.first));
+    AUGUR_HIP(hipEventCreate(&e.second));
+    prof_events.push_back(e);
+  }
+  return &prof_events[prof_used++];
+}
+
 void augur_sampler::prof_begin() {
   if (!prof_on()) return;
   if (prof_used == prof_events.size()) {

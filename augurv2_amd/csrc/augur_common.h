@@ -1,5 +1,6 @@
 // augur_common.h -- shared host/device plumbing of the sampler runtime.
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -145,11 +146,15 @@ struct augur_sampler {
   void record_traces();
 
   // Event timing of the family's dominant kernel: every prof_period-th
-  // sweep, each launch of it is bracketed by an event pair on `stream`.
+  // sweep, each launch of it carries an event pair on `stream`.  prof_slot()
+  // hands out the pair for hipExtLaunchKernelGGL, which records the events
+  // with the kernel's own dispatch (kernel time, no gap to the previous
+  // kernel); prof_begin / prof_end bracket a launch with separate records.
   int prof_period = 0;
   size_t prof_used = 0;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_events;
   bool prof_on() const { return prof_period > 0 && (sweep % (uint32_t)prof_period) == 0; }
+  std::pair<hipEvent_t, hipEvent_t>* prof_slot();
   void prof_begin();
   void prof_end();
 };

@@ -685,10 +685,12 @@ struct GmmSampler final : augur_sampler {
       if (!stats_valid) refresh_stats();
       k_gmm_conj<<<dim3(m.k, chains), kConjLanes, 0, stream>>>(m, seed, sweep, 1);
       AUGUR_CHECK_LAUNCH();
-      prof_begin();
-      zk<<<dim3(m.G, chains), kGmmThreads, 0, stream>>>(m, seed, sweep, 0u);
+      if (auto* ev = prof_slot())
+        hipExtLaunchKernelGGL(zk, dim3(m.G, chains), dim3(kGmmThreads), 0, stream, ev->first, ev->second, 0,
+                              m, seed, sweep, 0u);
+      else
+        zk<<<dim3(m.G, chains), kGmmThreads, 0, stream>>>(m, seed, sweep, 0u);
       AUGUR_CHECK_LAUNCH();
-      prof_end();
       launch_stats();  // for the next sweep's conjugate draws
       ++sweep;
       record_traces();

@@ -923,15 +923,19 @@ struct HlrSampler final : augur_sampler {
   }
 
   void launch_pass(const PassArgs& A) {
-    prof_begin();
-    if (fast_pass) {
-      fast_pass<<<G, kPassThreads, fast_smem, stream>>>(A);
+    PassKernel k = fast_pass;
+    size_t smem = fast_smem;
+    if (!k) {
+      k = k_hlr_pass;
+      smem = pass_smem_bytes(d, cp);
+      allow_lds(k_hlr_pass, smem);
+    }
+    if (auto* ev = prof_slot()) {
+      hipExtLaunchKernelGGL(k, dim3(G), dim3(kPassThreads), smem, stream, ev->first, ev->second, 0, A);
     } else {
-      allow_lds(k_hlr_pass, pass_smem_bytes(d, cp));
-      k_hlr_pass<<<G, kPassThreads, pass_smem_bytes(d, cp), stream>>>(A);
+      k<<<G, kPassThreads, smem, stream>>>(A);
     }
     AUGUR_CHECK_LAUNCH();
-    prof_end();
     k_hlr_finish<<<(cp * (d + 2) + kFinVals - 1) / kFinVals, 1024, 0, stream>>>(A);
     AUGUR_CHECK_LAUNCH();
   }

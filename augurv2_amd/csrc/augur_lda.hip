@@ -249,10 +249,12 @@ struct LdaSampler final : augur_sampler {
     for (int i = 0; i < n; ++i) {
       k_lda_dirichlet<<<gd, 1024, 0, stream>>>(m, seed, sweep);
       AUGUR_CHECK_LAUNCH();
-      prof_begin();
-      k_lda_z<<<dim3(zblocks, chains), 256, 0, stream>>>(m, seed, sweep, 0u);
+      if (auto* ev = prof_slot())
+        hipExtLaunchKernelGGL(k_lda_z, dim3(zblocks, chains), dim3(256), 0, stream, ev->first, ev->second, 0,
+                              m, seed, sweep, 0u);
+      else
+        k_lda_z<<<dim3(zblocks, chains), 256, 0, stream>>>(m, seed, sweep, 0u);
       AUGUR_CHECK_LAUNCH();
-      prof_end();
       ++sweep;
       record_traces();
     }
