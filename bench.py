@@ -289,6 +289,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--only", default="", help="comma list of legs to run besides the headline "
                     "HLR leg: lda, surrogate, batched, gmm (default: all)")
+    ap.add_argument("--batched-sweeps", type=int, default=1000,
+                    help="timed sweeps of the batched legs (warmup: half as many)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -306,11 +308,12 @@ def main():
     if lda:
         log(f"lda: {lda['tokens_per_s']:.3e} tokens/s")
     surr = run_surrogate() if "surrogate" in legs else None
-    bat = run_batched() if "batched" in legs else None
+    bs = max(1, args.batched_sweeps)
+    bat = run_batched(sweeps=bs, warmup=max(1, bs // 2)) if "batched" in legs else None
     if bat:
         log(f"batched: {bat}")
     # 16 chains per dataset: the chains-by-rows products on the matrix cores
-    bat16 = run_batched(chains=16) if "batched" in legs else None
+    bat16 = run_batched(sweeps=bs, warmup=max(1, bs // 2), chains=16) if "batched" in legs else None
     if bat16:
         log(f"batched (16 chains): {bat16}")
     gmm = run_gmm() if "gmm" in legs else None
