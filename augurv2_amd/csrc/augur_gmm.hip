@@ -4,9 +4,10 @@
 // Sweep, schedule ConjGibbs [pis] (*) ConjGibbs [mu] (*) ConjGibbs [cov] (*)
 // DiscGibbs [z] (experiments/experiment.py:127, order CudaC/CgKern.hs:192-238):
 //
-//   k_gmm_conj  one block per (component, chain): reduces the sufficient
-//               statistics that the previous z step left per block, draws
-//               pis_k's Gamma (Core/CgConj.hs:171-183), mu_k | cov_k
+//   k_gmm_stats_sum  the statistics of every (chain, component) summed
+//               over the blocks of the statistics kernel, FP64, fixed order.
+//   k_gmm_conj  one block per (component, chain): draws pis_k's Gamma
+//               (Core/CgConj.hs:171-183), mu_k | cov_k
 //               (cbits/augur_dist.c:633-669), cov_k | mu_k
 //               (cbits/augur_dist.c:740-757, Bartlett), all in FP64, and builds
 //               the component's table for the z step: L_k^-1 (packed lower
@@ -14,14 +15,13 @@
 //   k_gmm_z     point-parallel categorical draw (Core/CgDisc.hs:122-144) in
 //               log space with max subtraction: logit_k = log pis_k
 //               - 0.5 log det cov_k - 0.5 |L_k^-1 (y - mu_k)|^2, tables read
-//               through the scalar cache; fused with the statistics of the
-//               new z for the next sweep, centred at the current mu_k: count,
-//               sum (y - mu_k) and sum (y - mu_k)(y - mu_k)^T, as a one-hot
-//               product per wave of 64 points on the FP32 matrix core (no
-//               atomics: FP32 per block in a fixed order, FP64 across
-//               blocks).  The next sweep's scatter around the new mu is
-//               S + s d^T + d s^T + n d d^T with d = mu_old - mu_new, so the
-//               data are read once per sweep.
+//               through the scalar cache.
+//   k_gmm_stats_t  the statistics of the new z for the next sweep, centred
+//               at the current mu_k: count, sum (y - mu_k) and
+//               sum (y - mu_k)(y - mu_k)^T, as a one-hot product per wave of
+//               64 points on the FP32 matrix core (no atomics: FP32 per block
+//               in a fixed order).  The next sweep's scatter around the new
+//               mu is S + s d^T + d s^T + n d d^T with d = mu_old - mu_new.
 //
 // The reference multiplies raw pdfs in linear space (cbits/augur_dist.c:
 // 798-813), which underflows at D = 16; the log-space draw targets the same
@@ -151,21 +151,44 @@ __device__ void build_table(const GmmDev& m, int c, int k, double* work, double*
   if (threadIdx.x == 0) t[tri(DM) + DM] = (float)(-ldet);  // -0.5 log det cov
 }
 
-// One 64-lane block per (component k, chain c): statistics summed over the z
-// blocks, then pi_k (Gamma), mu_k | cov_k (MvNormal conjugate) and cov_k |
-// mu_k (InvWishart) and the new z-step table.
-__global__ __launch_bounds__(kConjLanes) void k_gmm_conj(GmmDev m, uint32_t seed, uint32_t sweep,
-                                                         int use_stats) {
+// Per (chain, component), the statistics summed over the blocks of the
+// statistics kernel in FP64: grid (values / 64, K, chains), thread (value
+// vi, slice sl) adds rows g = sl, sl + 4, ... (loads coalesced over the
+// values of a row), then the four slices in order.  sum[c][k][S].
+constexpr int kSumSlices = 4;
+__global__ __launch_bounds__(64 * kSumSlices) void k_gmm_stats_sum(GmmDev m, double* __restrict__ sum) {
+  __shared__ double red[kSumSlices][64];
+  const int vi = threadIdx.x & 63, sl = threadIdx.x >> 6;
+  const int i = blockIdx.x * 64 + vi, k = blockIdx.y, c = blockIdx.z, S = m.S;
+  double s = 0.0;
+  if (i < S) {
+    const float* col = m.part + ((size_t)c * m.G * m.k + k) * S + i;
+    const size_t row = (size_t)m.k * S;
+    int g = sl;
+    for (; g + 7 * kSumSlices < m.G; g += 8 * kSumSlices) {
+      float t[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) t[u] = col[(size_t)(g + u * kSumSlices) * row];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) s += (double)t[u];
+    }
+    for (; g < m.G; g += kSumSlices) s += (double)col[(size_t)g * row];
+  }
+  red[sl][vi] = s;
+  __syncthreads();
+  if (sl == 0 && i < S) sum[((size_t)c * m.k + k) * S + i] = ((red[0][vi] + red[1][vi]) + red[2][vi]) + red[3][vi];
+}
+
+// One 64-lane block per (component k, chain c): the summed statistics, then
+// pi_k (Gamma), mu_k | cov_k (MvNormal conjugate) and cov_k | mu_k
+// (InvWishart) and the new z-step table.
+__global__ __launch_bounds__(kConjLanes) void k_gmm_conj(GmmDev m, const double* __restrict__ ssum,
+                                                         uint32_t seed, uint32_t sweep) {
   const int k = blockIdx.x, c = blockIdx.y, tid = threadIdx.x;
   const int d = m.d, S = m.S;
   __shared__ double st[1 + kMaxDim + kMaxDim * kMaxDim];
   __shared__ double work[6 * kMaxDim * kMaxDim + 4 * kMaxDim];
-  if (!use_stats) return;
-  for (int i = tid; i < S; i += kConjLanes) {
-    double s = 0.0;
-    for (int g = 0; g < m.G; ++g) s += (double)m.part[(((size_t)c * m.G + g) * m.k + k) * S + i];
-    st[i] = s;
-  }
+  for (int i = tid; i < S; i += kConjLanes) st[i] = ssum[((size_t)c * m.k + k) * S + i];
   __syncthreads();
   // unpack: st[0] count, st[1..d] sum (y - centre), packed lower S after
   const int cnt = (int)llrint(st[0]);
@@ -630,7 +653,7 @@ static StatsKernel pick_stats(int DM, int KM) {
 struct GmmSampler final : augur_sampler {
   GmmDev m{};
   DevBuf<float> y, tab, part;
-  DevBuf<double> alpha, mu0, cov0, scale0, gpi, pis64, mu64, cov64, ll;
+  DevBuf<double> alpha, mu0, cov0, scale0, gpi, pis64, mu64, cov64, ll, ssum;
   DevBuf<int32_t> z;
   std::vector<double> h_alpha, h_mu0, h_cov0, h_scale0;
   bool stats_valid = false;
@@ -659,6 +682,15 @@ struct GmmSampler final : augur_sampler {
     AUGUR_CHECK_LAUNCH();
   }
 
+  // the statistics summed over the statistics kernel's blocks, then the
+  // conjugate draws of every (component, chain)
+  void launch_conj() {
+    k_gmm_stats_sum<<<dim3((m.S + 63) / 64, m.k, chains), 64 * kSumSlices, 0, stream>>>(m, ssum.p);
+    AUGUR_CHECK_LAUNCH();
+    k_gmm_conj<<<dim3(m.k, chains), kConjLanes, 0, stream>>>(m, ssum.p, seed, sweep);
+    AUGUR_CHECK_LAUNCH();
+  }
+
   void refresh_stats() {
     refresh_tables();  // the statistics are centred at the tables' (current) mu
     launch_stats();
@@ -683,7 +715,7 @@ struct GmmSampler final : augur_sampler {
     const ZKernel zk = pick_z(m.DM, pad_comp(m.k));
     for (int i = 0; i < nsw; ++i) {
       if (!stats_valid) refresh_stats();
-      k_gmm_conj<<<dim3(m.k, chains), kConjLanes, 0, stream>>>(m, seed, sweep, 1);
+      launch_conj();
       AUGUR_CHECK_LAUNCH();
       if (auto* ev = prof_slot())
         hipExtLaunchKernelGGL(zk, dim3(m.G, chains), dim3(kGmmThreads), 0, stream, ev->first, ev->second, 0,
@@ -746,6 +778,7 @@ static void gmm_setup(GmmSampler* s, int n, int d, int k, int chains) {
   s->mu64.alloc((size_t)chains * k * d); s->cov64.alloc((size_t)chains * k * d * d);
   s->tab.alloc((size_t)chains * k * m.TS);
   s->part.alloc((size_t)chains * m.G * k * m.S);
+  s->ssum.alloc((size_t)chains * k * m.S);
   s->z.alloc((size_t)chains * n);
   s->ll.alloc((size_t)chains * (1 + m.G + k));  // per-chain sums, data partials, priors
   m.y = s->y.p; m.alpha = s->alpha.p; m.mu0 = s->mu0.p; m.cov0 = s->cov0.p; m.scale0 = s->scale0.p;
@@ -863,7 +896,9 @@ augur_status augur_op_gmm_conj(int d, int k, const double* alpha, const double* 
         }
     }
     s.part.upload(st.data(), st.size(), s.stream);
-    k_gmm_conj<<<dim3(k, 1), kConjLanes, 0, s.stream>>>(m, seed, sweep, 1);
+    s.seed = seed;
+    s.sweep = sweep;
+    s.launch_conj();
     AUGUR_CHECK_LAUNCH();
     std::vector<double> g(k);
     s.gpi.download(g.data(), k, s.stream);
