@@ -1147,12 +1147,17 @@ struct HlrSampler final : augur_sampler {
     fast_pass = (gen && atoi(gen)) ? nullptr : pick_pass(d, cp);
     fast_smem = pass_t_smem_bytes(d, cp);
     int ring = pass_ring(d);
+    int per_cu = kPassBlocksPerCU;
     if (fast_pass && !(lds && atoi(lds))) {
       if (PassKernel r = pick_pass_r(d, cp)) {
         fast_pass = r;
         fast_smem = pass_r_smem_bytes(cp);
         tiles = (n + pass_r_rows(d, cp) - 1) / pass_r_rows(d, cp);
         ring = row_ring(cp);
+        // two blocks per CU: fewer, longer blocks amortise the epilogue and
+        // leave fewer partial rows (measured at N = 100k, D = 64, 4 chains:
+        // 7.4 us pass + 4.1 us finish at 512 blocks, 8.2 + 4.3 at 768)
+        per_cu = 2;
       }
     }
     int want;
@@ -1162,7 +1167,7 @@ struct HlrSampler final : augur_sampler {
       // resident at once (a second round of blocks costs ~35% at N = 100k)
       int cus = 0;
       AUGUR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
-      want = std::min(kPassBlocksPerCU * std::max(cus, 1), (tiles + ring - 1) / ring);
+      want = std::min(per_cu * std::max(cus, 1), (tiles + ring - 1) / ring);
     } else {
       // about 256 rows per block, between one and four blocks per CU
       want = 256;
