@@ -47,6 +47,29 @@ def test_lda_two_sweeps_match_oracle(rt, oracle):
     s.close()
 
 
+def test_lda_log_density_is_the_model_density(rt, oracle):
+    """augur_get_curr_ll for lda.rv is the reference's ll_mod: Dirichlet
+    terms of theta and phi without normaliser (the oracle's dirichlet_ll,
+    pinned against scipy) plus log theta[d, z] + log phi[z, w] per token."""
+    rng = np.random.default_rng(6)
+    K, D, V, C = 7, 25, 90, 2
+    doc_off, w = _corpus(rng, D, V, 50)
+    alpha, beta = rng.uniform(0.05, 0.5, K), rng.uniform(0.01, 0.2, V)
+    s = rt.LdaSampler(K, doc_off, w, alpha, beta, chains=C, seed=5)
+    for c in range(C):
+        s.set("z", rng.integers(0, K, len(w)).astype(np.int32), chain=c)
+    s.step(2)
+    ll = s.log_density()
+    doc = np.repeat(np.arange(D), np.diff(doc_off))
+    for c in range(C):
+        theta, phi, z = s.get("theta", c), s.get("phi", c), s.get("z", c)
+        ref = sum(oracle.dirichlet_ll(theta[d], alpha) for d in range(D))
+        ref += sum(oracle.dirichlet_ll(phi[k], beta) for k in range(K))
+        ref += np.sum(np.log(theta[doc, z])) + np.sum(np.log(phi[z, w]))
+        assert abs(ll[c] - ref) <= 1e-10 * abs(ref), (c, ll[c], ref)
+    s.close()
+
+
 def test_lda_counts_exact_after_set(rt, oracle):
     """Counts are recomputed from a set z; the Dirichlet draws then consume
     them, so equal theta/phi to the oracle imply equal count tables."""
