@@ -10,9 +10,10 @@
 //                     122-144 -> cbits/augur_dist.c:798-813), fused with the
 //                     count accumulation of the new z for the next sweep.
 // Layout in HBM (per chain): w, z int32 in CSR token order; theta FP32 D x K
-// (doc-major, one row per wave); phi_t FP32 V x K (word-major, so the K
+// (doc-major); phi_t FP32 V x K (word-major, so the K
 // probabilities of a token are contiguous); FP64 copies of theta, phi for
-// the caller; cnt_theta D x K, cnt_phi K x V int32.
+// the caller; cnt_theta D x K, cnt_phi K x V int32; the chunk list
+// (document, first token) of the z kernel.
 #include "augur_common.h"
 
 namespace augur {
@@ -35,9 +36,46 @@ __device__ __noinline__ int draw_cat_f32(const float* __restrict__ a, const floa
   return k - 1;
 }
 
+// The same draw for k <= kCatRegs with both operand rows held in registers
+// (loaded up front instead of once per step of the search); every sum is
+// formed in the order above, so the result is identical.
+constexpr int kCatRegs = 16;
+__device__ inline int draw_cat_f32_regs(const float* __restrict__ a, const float* __restrict__ b, int k,
+                                        float u) {
+#pragma clang fp contract(off)
+  float pa[kCatRegs], pb[kCatRegs];
+#pragma unroll
+  for (int i = 0; i < kCatRegs; ++i) {
+    pa[i] = i < k ? a[i] : 0.0f;
+    pb[i] = i < k ? b[i] : 0.0f;
+  }
+  float c = 0.0f;
+#pragma unroll
+  for (int i = 0; i < kCatRegs; ++i)
+    if (i < k) c = c + pa[i] * pb[i];
+  const float t = u * c;
+  float run = 0.0f;
+  int z = k - 1;
+  bool found = false;
+#pragma unroll
+  for (int i = 0; i < kCatRegs - 1; ++i) {
+    if (i < k - 1) {
+      run = run + pa[i] * pb[i];
+      if (!found && t < run) {
+        z = i;
+        found = true;
+      }
+    }
+  }
+  return z;
+}
+
+constexpr int kZThreads = 128;
+
 struct LdaDev {
-  int K, D, V, T;
+  int K, D, V, T, n_chunks;
   const int32_t* doc_off;
+  const int32_t* chunks;  // n_chunks x (document, first token, end token, 0)
   const int32_t* w;
   const double* alpha;
   const double* beta;
@@ -48,6 +86,8 @@ struct LdaDev {
   float* theta32;      // C x D x K
   double* phi64;       // C x K x V
   float* phit32;       // C x V x K
+  double* phi_part;    // C x K x ceil(V / 256) partial row sums
+  uint32_t* phi_done;  // C x K blocks of the row finished (0 between launches)
 };
 
 __device__ inline double block_sum(double v, double* red) {
@@ -62,35 +102,57 @@ __device__ inline double block_sum(double v, double* red) {
   return s;
 }
 
-// blocks [0, K): phi rows; blocks [K, K + ceil(D / waves)): theta rows, one wave each.
-// 1024-thread blocks: one Gamma draw per thread for a row of up to 1024 words.
-__global__ __launch_bounds__(1024) void k_lda_dirichlet(LdaDev m, uint32_t seed, uint32_t sweep) {
-  __shared__ double red[16];
+constexpr int kDirThreads = 256;
+
+__host__ __device__ inline int phi_blocks_per_row(int V) { return (V + kDirThreads - 1) / kDirThreads; }
+
+// 256-thread blocks, one Gamma draw per thread, spread over the chip:
+//   blocks [0, K * pb): phi row k = b / pb, words [256 j, 256 j + 256) with
+//     j = b % pb (pb = ceil(V / 256)).  Each block leaves its partial sum in
+//     phi_part[k][j]; the last block of the row to finish (a counter per row,
+//     reset by that block) sums the partials in order j = 0 .. pb - 1 and
+//     normalises the whole row into phi64 and the word-major FP32 table.
+//   blocks [K * pb, K * pb + ceil(D / 4)): theta rows, one wave per document.
+__global__ __launch_bounds__(kDirThreads) void k_lda_dirichlet(LdaDev m, uint32_t seed, uint32_t sweep) {
+  __shared__ double red[kDirThreads / 64];
+  __shared__ int last;
   const int c = blockIdx.y;
   const int b = blockIdx.x;
-  if (b < m.K) {
-    const int k = b;
+  const int pb = phi_blocks_per_row(m.V);
+  if (b < m.K * pb) {
+    const int k = b / pb, j = b - k * pb;
     const Stream s{seed, (uint32_t)c, sweep, TAG_LDA_PHI};
-    int32_t* cnt = m.cnt_phi + ((size_t)c * m.K + k) * m.V;
-    double* row = m.phi64 + ((size_t)c * m.K + k) * m.V;
-    double part = 0.0;
-    for (int v = threadIdx.x; v < m.V; v += blockDim.x) {
-      const double g = gamma_at(s, (uint32_t)(k * m.V + v), m.beta[v] + (double)cnt[v]);
+    const size_t r = (size_t)c * m.K + k;
+    int32_t* cnt = m.cnt_phi + r * m.V;
+    double* row = m.phi64 + r * m.V;
+    const int v = j * kDirThreads + threadIdx.x;
+    double g = 0.0;
+    if (v < m.V) {
+      g = gamma_at(s, (uint32_t)(k * m.V + v), m.beta[v] + (double)cnt[v]);
       cnt[v] = 0;
       row[v] = g;
-      part += g;
     }
-    const double tot = block_sum(part, red);
+    const double part = block_sum(g, red);
+    if (threadIdx.x == 0) {
+      m.phi_part[r * pb + j] = part;
+      __threadfence();
+      last = atomicAdd(m.phi_done + r, 1u) == (uint32_t)(pb - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();  // the other blocks' gammas and partials are visible from here
+    if (threadIdx.x == 0) m.phi_done[r] = 0u;
+    double tot = 0.0;
+    for (int i = 0; i < pb; ++i) tot += m.phi_part[r * pb + i];
     float* pt = m.phit32 + (size_t)c * m.V * m.K;
-    for (int v = threadIdx.x; v < m.V; v += blockDim.x) {
-      const double p = row[v] / tot;
-      row[v] = p;
-      pt[(size_t)v * m.K + k] = (float)p;
+    for (int u = threadIdx.x; u < m.V; u += kDirThreads) {
+      const double p = row[u] / tot;
+      row[u] = p;
+      pt[(size_t)u * m.K + k] = (float)p;
     }
     return;
   }
-  const int waves = blockDim.x >> 6;
-  const int d = (b - m.K) * waves + (threadIdx.x >> 6);
+  const int d = (b - m.K * pb) * (kDirThreads >> 6) + (threadIdx.x >> 6);
   if (d >= m.D) return;
   const int lane = threadIdx.x & 63;
   const Stream s{seed, (uint32_t)c, sweep, TAG_LDA_THETA};
@@ -110,13 +172,20 @@ __global__ __launch_bounds__(1024) void k_lda_dirichlet(LdaDev m, uint32_t seed,
   }
 }
 
-// One wave per document (grid-stride).  Lane j takes token j of each
-// 64-token chunk.  Topic counts of the document are formed with ballots
-// (lane k owns topic k) and stored once; word-topic counts use global
-// atomics (order-independent integer adds, so the table is exact).
-// The Philox key of table row c is chain0 + c.
-__global__ __launch_bounds__(256) void k_lda_z(LdaDev m, uint32_t seed, uint32_t sweep,
-                                               uint32_t chain0) {
+static dim3 dirichlet_grid(const LdaDev& m, int chains) {
+  return dim3(m.K * phi_blocks_per_row(m.V) + (m.D + 3) / 4, chains);
+}
+
+// One wave per chunk of up to 64 tokens of one document (grid-stride over
+// the chunk list, so a corpus of T tokens runs on ~T / 64 waves whatever
+// the document lengths).  Lane j takes token j of the chunk.  Topic counts
+// of the chunk are formed with ballots (lane k owns topic k) and added with
+// one atomic per topic present; word-topic counts use one atomic per token.
+// Integer adds commute, so both tables are exact whatever the order.  The
+// count tables start the sweep at zero (cleared by the Dirichlet kernel that
+// read them).  The Philox key of table row c is chain0 + c.
+__global__ __launch_bounds__(kZThreads) void k_lda_z(LdaDev m, uint32_t seed, uint32_t sweep,
+                                                     uint32_t chain0) {
   const int c = blockIdx.y;
   const int lane = threadIdx.x & 63;
   const int waves = blockDim.x >> 6;
@@ -124,33 +193,53 @@ __global__ __launch_bounds__(256) void k_lda_z(LdaDev m, uint32_t seed, uint32_t
   const float* phit = m.phit32 + (size_t)c * m.V * m.K;
   int32_t* cphi = m.cnt_phi + (size_t)c * m.K * m.V;
   int32_t* zc = m.z + (size_t)c * m.T;
-  for (int d = blockIdx.x * waves + (threadIdx.x >> 6); d < m.D; d += gridDim.x * waves) {
-    const int du = __builtin_amdgcn_readfirstlane(d);
+  for (int q = blockIdx.x * waves + (threadIdx.x >> 6); q < m.n_chunks; q += gridDim.x * waves) {
+    const int qu = __builtin_amdgcn_readfirstlane(q);
+    const int4 ch = *reinterpret_cast<const int4*>(m.chunks + 4 * (size_t)qu);
+    const int du = ch.x, t0 = ch.y, t1 = ch.z;
     const float* th = m.theta32 + ((size_t)c * m.D + du) * m.K;
-    const int t0 = m.doc_off[du], t1 = m.doc_off[du + 1];
-    int my_count = 0;  // count of topic `lane` in this document (K <= 64)
-    for (int base = t0; base < t1; base += 64) {
-      const int t = base + lane;
-      const bool valid = t < t1;
-      int znew = -1;
-      if (valid) {
-        const int wd = m.w[t];
-        const float u = u01f(s.draw((uint32_t)t, 0).x);
-        znew = draw_cat_f32(th, phit + (size_t)wd * m.K, m.K, u);
-        zc[t] = znew;
-        atomicAdd(cphi + (size_t)znew * m.V + wd, 1);
-      }
-      if (m.K <= 64) {
-        for (int k = 0; k < m.K; ++k) {
-          const unsigned long long bal = __ballot(valid && znew == k);
-          if (lane == k) my_count += __popcll(bal);
-        }
-      } else if (valid) {
-        atomicAdd(m.cnt_theta + ((size_t)c * m.D + du) * m.K + znew, 1);
-      }
+    int32_t* ct = m.cnt_theta + ((size_t)c * m.D + du) * m.K;
+    const int t = t0 + lane;
+    const bool valid = t < t1;
+    int znew = -1;
+    if (valid) {
+      const int wd = m.w[t];
+      const float u = u01f(s.draw((uint32_t)t, 0).x);
+      znew = m.K <= kCatRegs ? draw_cat_f32_regs(th, phit + (size_t)wd * m.K, m.K, u)
+                             : draw_cat_f32(th, phit + (size_t)wd * m.K, m.K, u);
+      zc[t] = znew;
+      atomicAdd(cphi + (size_t)znew * m.V + wd, 1);
     }
-    if (m.K <= 64 && lane < m.K) m.cnt_theta[((size_t)c * m.D + du) * m.K + lane] = my_count;
+    if (m.K <= 64) {
+      int my_count = 0;  // tokens of the chunk on topic `lane`
+      for (int k = 0; k < m.K; ++k) {
+        const unsigned long long bal = __ballot(valid && znew == k);
+        if (lane == k) my_count = __popcll(bal);
+      }
+      if (my_count) atomicAdd(ct + lane, my_count);
+    } else if (valid) {
+      atomicAdd(ct + znew, 1);
+    }
   }
+}
+
+// Chunk list of a CSR corpus: (document, first token, end token, 0) of each
+// run of up to 64 consecutive tokens of one document; empty documents have
+// no chunk.
+static std::vector<int32_t> lda_chunks(const int32_t* doc_off, int D) {
+  std::vector<int32_t> ch;
+  for (int d = 0; d < D; ++d)
+    for (int t = doc_off[d]; t < doc_off[d + 1]; t += 64) {
+      ch.push_back(d);
+      ch.push_back(t);
+      ch.push_back(std::min(t + 64, doc_off[d + 1]));
+      ch.push_back(0);
+    }
+  return ch;
+}
+
+static int z_blocks(int n_chunks) {
+  return std::max(1, std::min((n_chunks + kZThreads / 64 - 1) / (kZThreads / 64), 4096));
 }
 
 // Count tables of the current z (after set_pt / at creation).
@@ -202,8 +291,9 @@ __global__ __launch_bounds__(256) void k_lda_ll(LdaDev m, double* out) {
 
 struct LdaSampler final : augur_sampler {
   LdaDev m{};
-  DevBuf<int32_t> doc_off, w, z, cnt_theta, cnt_phi;
-  DevBuf<double> alpha, beta, theta64, phi64, ll;
+  DevBuf<int32_t> doc_off, chunks, w, z, cnt_theta, cnt_phi;
+  DevBuf<uint32_t> phi_done;
+  DevBuf<double> alpha, beta, theta64, phi64, phi_part, ll;
   DevBuf<float> theta32, phit32;
 
   const char* family() const override { return "lda"; }
@@ -242,18 +332,16 @@ struct LdaSampler final : augur_sampler {
   }
 
   void run_sweeps(int n) override {
-    const int dwaves = 16;
-    const dim3 gd(m.K + (m.D + dwaves - 1) / dwaves, chains);
-    const int waves = 4;
-    const int zblocks = std::max(1, std::min((m.D + waves - 1) / waves, 2048));
+    const dim3 gd = dirichlet_grid(m, chains);
+    const dim3 gz(z_blocks(m.n_chunks), chains);
     for (int i = 0; i < n; ++i) {
-      k_lda_dirichlet<<<gd, 1024, 0, stream>>>(m, seed, sweep);
+      k_lda_dirichlet<<<gd, kDirThreads, 0, stream>>>(m, seed, sweep);
       AUGUR_CHECK_LAUNCH();
       if (auto* ev = prof_slot())
-        hipExtLaunchKernelGGL(k_lda_z, dim3(zblocks, chains), dim3(256), 0, stream, ev->first, ev->second, 0,
-                              m, seed, sweep, 0u);
+        hipExtLaunchKernelGGL(k_lda_z, gz, dim3(kZThreads), 0, stream, ev->first, ev->second, 0, m, seed,
+                              sweep, 0u);
       else
-        k_lda_z<<<dim3(zblocks, chains), 256, 0, stream>>>(m, seed, sweep, 0u);
+        k_lda_z<<<gz, kZThreads, 0, stream>>>(m, seed, sweep, 0u);
       AUGUR_CHECK_LAUNCH();
       ++sweep;
       record_traces();
@@ -301,21 +389,22 @@ static void lda_zstep_op(int n_docs, int k, int v, const int32_t* doc_off, const
   const int T = doc_off[n_docs];
   for (int t = 0; t < T; ++t)
     if (w[t] < 0 || w[t] >= v) fail(AUGUR_ERR_ARG, "word id out of range");
-  DevBuf<int32_t> ddo(n_docs + 1), dw(std::max(T, 1)), dz(std::max(T, 1)), dct((size_t)n_docs * k),
-      dcp((size_t)k * v);
+  const std::vector<int32_t> ch = lda_chunks(doc_off, n_docs);
+  DevBuf<int32_t> ddo(n_docs + 1), dch(std::max<size_t>(ch.size(), 4)), dw(std::max(T, 1)),
+      dz(std::max(T, 1)), dct((size_t)n_docs * k), dcp((size_t)k * v);
   DevBuf<float> dth((size_t)n_docs * k), dph((size_t)v * k);
   ddo.upload(doc_off, n_docs + 1, 0);
+  dch.upload(ch.data(), ch.size(), 0);
   dw.upload(w, T, 0);
   dth.upload(theta, (size_t)n_docs * k, 0);
   dph.upload(phi_t, (size_t)v * k, 0);
   dct.zero(0);
   dcp.zero(0);
   LdaDev m{};
-  m.K = k; m.D = n_docs; m.V = v; m.T = T;
-  m.doc_off = ddo.p; m.w = dw.p; m.z = dz.p; m.cnt_theta = dct.p; m.cnt_phi = dcp.p;
+  m.K = k; m.D = n_docs; m.V = v; m.T = T; m.n_chunks = (int)(ch.size() / 4);
+  m.doc_off = ddo.p; m.chunks = dch.p; m.w = dw.p; m.z = dz.p; m.cnt_theta = dct.p; m.cnt_phi = dcp.p;
   m.theta32 = dth.p; m.phit32 = dph.p;
-  const int zblocks = std::max(1, std::min((n_docs + 3) / 4, 2048));
-  k_lda_z<<<dim3(zblocks, 1), 256>>>(m, seed, sweep, chain);
+  k_lda_z<<<dim3(z_blocks(m.n_chunks), 1), kZThreads>>>(m, seed, sweep, chain);
   AUGUR_CHECK_LAUNCH();
   dz.download(z, T, 0);
   if (cnt_theta) dct.download(cnt_theta, (size_t)n_docs * k, 0);
@@ -343,7 +432,11 @@ augur_status augur_lda_create(const augur_lda_desc* d, augur_sampler** out) {
       s->seed = d->seed;
       const int K = d->n_topics, D = d->n_docs, V = d->n_vocab, C = d->chains;
       const int T = d->doc_off[D];
+      const std::vector<int32_t> ch = lda_chunks(d->doc_off, D);
       s->doc_off.alloc(D + 1);
+      s->chunks.alloc(std::max<size_t>(ch.size(), 4));
+      s->phi_part.alloc((size_t)C * K * phi_blocks_per_row(V));
+      s->phi_done.alloc((size_t)C * K);
       s->w.alloc(std::max(T, 1));
       s->alpha.alloc(K);
       s->beta.alloc(V);
@@ -356,6 +449,8 @@ augur_status augur_lda_create(const augur_lda_desc* d, augur_sampler** out) {
       s->phit32.alloc((size_t)C * V * K);
       s->ll.alloc(C);
       s->doc_off.upload(d->doc_off, D + 1, s->stream);
+      s->chunks.upload(ch.data(), ch.size(), s->stream);
+      s->phi_done.zero(s->stream);
       s->w.upload(d->w, T, s->stream);
       s->alpha.upload(d->alpha, K, s->stream);
       s->beta.upload(d->beta, V, s->stream);
@@ -365,11 +460,11 @@ augur_status augur_lda_create(const augur_lda_desc* d, augur_sampler** out) {
       s->phi64.zero(s->stream);
       s->phit32.zero(s->stream);
       LdaDev& m = s->m;
-      m.K = K; m.D = D; m.V = V; m.T = T;
-      m.doc_off = s->doc_off.p; m.w = s->w.p; m.alpha = s->alpha.p; m.beta = s->beta.p;
+      m.K = K; m.D = D; m.V = V; m.T = T; m.n_chunks = (int)(ch.size() / 4);
+      m.doc_off = s->doc_off.p; m.chunks = s->chunks.p; m.w = s->w.p; m.alpha = s->alpha.p; m.beta = s->beta.p;
       m.z = s->z.p; m.cnt_theta = s->cnt_theta.p; m.cnt_phi = s->cnt_phi.p;
       m.theta64 = s->theta64.p; m.theta32 = s->theta32.p; m.phi64 = s->phi64.p;
-      m.phit32 = s->phit32.p;
+      m.phit32 = s->phit32.p; m.phi_part = s->phi_part.p; m.phi_done = s->phi_done.p;
       s->recount();
       AUGUR_HIP(hipStreamSynchronize(s->stream));
     } catch (...) {
