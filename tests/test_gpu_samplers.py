@@ -47,6 +47,35 @@ def test_lda_two_sweeps_match_oracle(rt, oracle):
     s.close()
 
 
+def test_lda_ragged_documents_match_oracle(rt, oracle):
+    """Documents of 0, 1, 63, 64, 65, 128, 129 and 300 tokens (the z step
+    takes them in 64-token chunks) and V = 512, two full 256-word blocks of
+    each phi row: three sweeps of two chains against the oracle, each sweep
+    continuing from the GPU's assignment, so the count tables the chunks add
+    up are checked through the next sweep's theta and phi."""
+    rng = np.random.default_rng(21)
+    lens = np.array([0, 1, 63, 64, 65, 128, 129, 300, 0, 5, 64, 0])
+    K, D, V, C = 12, len(lens), 512, 2
+    doc_off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    w = rng.integers(0, V, int(doc_off[-1])).astype(np.int32)
+    alpha, beta = np.full(K, 0.3), np.full(V, 0.02)
+    s = rt.LdaSampler(K, doc_off, w, alpha, beta, chains=C, seed=5)
+    z0 = [rng.integers(0, K, len(w)).astype(np.int32) for _ in range(C)]
+    for c in range(C):
+        s.set("z", z0[c], chain=c)
+    for sweep in range(3):
+        s.step(1)
+        for c in range(C):
+            theta, phi, z = np.zeros((D, K)), np.zeros((K, V)), z0[c].copy()
+            oracle.lda_sweep(doc_off, w, alpha, beta, 5, c, sweep, theta, phi, z)
+            np.testing.assert_allclose(s.get("theta", c), theta, rtol=1e-9, atol=1e-300)
+            np.testing.assert_allclose(s.get("phi", c), phi, rtol=1e-9, atol=1e-300)
+            zg = s.get("z", c)
+            assert (zg == z).mean() >= 0.995
+            z0[c] = zg
+    s.close()
+
+
 def test_lda_log_density_is_the_model_density(rt, oracle):
     """augur_get_curr_ll for lda.rv is the reference's ll_mod: Dirichlet
     terms of theta and phi without normaliser (the oracle's dirichlet_ll,
