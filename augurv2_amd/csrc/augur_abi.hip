@@ -25,15 +25,16 @@ __global__ void k_philox(const uint32_t* __restrict__ ctr, const uint32_t* __res
   out[4 * i + 3] = r.w;
 }
 
-__global__ void k_gamma(Stream s, const double* __restrict__ shape, int n, double* __restrict__ out) {
+__global__ __launch_bounds__(256) void k_gamma(Stream s, const double* __restrict__ shape, int n, double* __restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   out[i] = gamma_at(s, (uint32_t)i, shape[i]);
 }
 
 // One wave per row: Gamma(alpha_i + cnt_i) at site r * k + i, then normalise
-// (cbits/augur_dist.c:526-538).  Shared with the LDA sampler.
-__global__ void k_dirichlet_rows(Stream s, const double* __restrict__ alpha,
+// (cbits/augur_dist.c:526-538).  The Dirichlet operator entry point; the LDA
+// sampler draws the same values in its own kernel.
+__global__ __launch_bounds__(256) void k_dirichlet_rows(Stream s, const double* __restrict__ alpha,
                                  const int32_t* __restrict__ cnt, int rows, int k,
                                  double* __restrict__ out) {
   const int lane = threadIdx.x & 63;

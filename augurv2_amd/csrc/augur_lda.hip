@@ -70,7 +70,9 @@ __device__ inline int draw_cat_f32_regs(const float* __restrict__ a, const float
   return z;
 }
 
-constexpr int kZThreads = 128;
+constexpr int kZThreads = 128;      // z step with word-topic counts added in HBM
+constexpr int kZLdsThreads = 1024;  // z step with a per-block word-topic table in LDS
+constexpr int kZLdsCells = 32768;   // largest K * V of the LDS table (128 KB)
 
 struct LdaDev {
   int K, D, V, T, n_chunks;
@@ -180,12 +182,20 @@ static dim3 dirichlet_grid(const LdaDev& m, int chains) {
 // the chunk list, so a corpus of T tokens runs on ~T / 64 waves whatever
 // the document lengths).  Lane j takes token j of the chunk.  Topic counts
 // of the chunk are formed with ballots (lane k owns topic k) and added with
-// one atomic per topic present; word-topic counts use one atomic per token.
-// Integer adds commute, so both tables are exact whatever the order.  The
-// count tables start the sweep at zero (cleared by the Dirichlet kernel that
-// read them).  The Philox key of table row c is chain0 + c.
-__global__ __launch_bounds__(kZThreads) void k_lda_z(LdaDev m, uint32_t seed, uint32_t sweep,
-                                                     uint32_t chain0) {
+// one atomic per topic present.  Word-topic counts: with kLdsTable each block
+// counts its tokens in a K x V table in LDS and adds the non-zero cells to
+// HBM at the end, so a frequent (topic, word) cell takes one device-scope
+// atomic per block instead of one per token (device-scope atomics on one
+// address are serialised, and LDA's sparse topics concentrate tokens on few
+// cells); otherwise one atomic per token.  Integer adds commute, so both
+// tables are exact whatever the order.  The count tables start the sweep at
+// zero (cleared by the Dirichlet kernel that read them).  The Philox key of
+// table row c is chain0 + c.
+template <bool kLdsTable>
+__global__ __launch_bounds__(kLdsTable ? kZLdsThreads : kZThreads) void k_lda_z(LdaDev m, uint32_t seed,
+                                                                              uint32_t sweep, uint32_t chain0) {
+  extern __shared__ __align__(16) unsigned char zsm[];
+  int32_t* tab = reinterpret_cast<int32_t*>(zsm);  // K x V, kLdsTable only
   const int c = blockIdx.y;
   const int lane = threadIdx.x & 63;
   const int waves = blockDim.x >> 6;
@@ -193,6 +203,11 @@ __global__ __launch_bounds__(kZThreads) void k_lda_z(LdaDev m, uint32_t seed, ui
   const float* phit = m.phit32 + (size_t)c * m.V * m.K;
   int32_t* cphi = m.cnt_phi + (size_t)c * m.K * m.V;
   int32_t* zc = m.z + (size_t)c * m.T;
+  const int cells = m.K * m.V;
+  if (kLdsTable) {
+    for (int i = threadIdx.x; i < cells; i += blockDim.x) tab[i] = 0;
+    __syncthreads();
+  }
   for (int q = blockIdx.x * waves + (threadIdx.x >> 6); q < m.n_chunks; q += gridDim.x * waves) {
     const int qu = __builtin_amdgcn_readfirstlane(q);
     const int4 ch = *reinterpret_cast<const int4*>(m.chunks + 4 * (size_t)qu);
@@ -208,7 +223,8 @@ __global__ __launch_bounds__(kZThreads) void k_lda_z(LdaDev m, uint32_t seed, ui
       znew = m.K <= kCatRegs ? draw_cat_f32_regs(th, phit + (size_t)wd * m.K, m.K, u)
                              : draw_cat_f32(th, phit + (size_t)wd * m.K, m.K, u);
       zc[t] = znew;
-      atomicAdd(cphi + (size_t)znew * m.V + wd, 1);
+      if (kLdsTable) atomicAdd(tab + (size_t)znew * m.V + wd, 1);
+      else atomicAdd(cphi + (size_t)znew * m.V + wd, 1);
     }
     if (m.K <= 64) {
       int my_count = 0;  // tokens of the chunk on topic `lane`
@@ -219,6 +235,13 @@ __global__ __launch_bounds__(kZThreads) void k_lda_z(LdaDev m, uint32_t seed, ui
       if (my_count) atomicAdd(ct + lane, my_count);
     } else if (valid) {
       atomicAdd(ct + znew, 1);
+    }
+  }
+  if (kLdsTable) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < cells; i += blockDim.x) {
+      const int n = tab[i];
+      if (n) atomicAdd(cphi + i, n);
     }
   }
 }
@@ -238,8 +261,37 @@ static std::vector<int32_t> lda_chunks(const int32_t* doc_off, int D) {
   return ch;
 }
 
-static int z_blocks(int n_chunks) {
-  return std::max(1, std::min((n_chunks + kZThreads / 64 - 1) / (kZThreads / 64), 4096));
+// The z step over all chains of `m`: the LDS-table form when the K x V table
+// fits (one 1024-thread block per CU at most), else the HBM-atomic form.
+static int device_cus() {
+  int dev = 0, cus = 0;
+  AUGUR_HIP(hipGetDevice(&dev));
+  AUGUR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  return std::max(cus, 1);
+}
+
+static void launch_lda_z(const LdaDev& m, int chains, uint32_t seed, uint32_t sweep, uint32_t chain0, int cus,
+                         hipStream_t stream, const std::pair<hipEvent_t, hipEvent_t>* ev) {
+  if ((size_t)m.K * m.V <= (size_t)kZLdsCells) {
+    const int per = kZLdsThreads / 64;
+    const dim3 g(std::max(1, std::min((m.n_chunks + per - 1) / per, cus)), chains);
+    const size_t smem = (size_t)m.K * m.V * sizeof(int32_t);
+    allow_lds(k_lda_z<true>, smem);
+    if (ev)
+      hipExtLaunchKernelGGL(k_lda_z<true>, g, dim3(kZLdsThreads), smem, stream, ev->first, ev->second, 0, m,
+                            seed, sweep, chain0);
+    else
+      k_lda_z<true><<<g, kZLdsThreads, smem, stream>>>(m, seed, sweep, chain0);
+  } else {
+    const int per = kZThreads / 64;
+    const dim3 g(std::max(1, std::min((m.n_chunks + per - 1) / per, 4096)), chains);
+    if (ev)
+      hipExtLaunchKernelGGL(k_lda_z<false>, g, dim3(kZThreads), 0, stream, ev->first, ev->second, 0, m, seed,
+                            sweep, chain0);
+    else
+      k_lda_z<false><<<g, kZThreads, 0, stream>>>(m, seed, sweep, chain0);
+  }
+  AUGUR_CHECK_LAUNCH();
 }
 
 // Count tables of the current z (after set_pt / at creation).
@@ -295,6 +347,7 @@ struct LdaSampler final : augur_sampler {
   DevBuf<uint32_t> phi_done;
   DevBuf<double> alpha, beta, theta64, phi64, phi_part, ll;
   DevBuf<float> theta32, phit32;
+  int cus = 1;  // compute units of the device (grid of the z step)
 
   const char* family() const override { return "lda"; }
 
@@ -333,16 +386,10 @@ struct LdaSampler final : augur_sampler {
 
   void run_sweeps(int n) override {
     const dim3 gd = dirichlet_grid(m, chains);
-    const dim3 gz(z_blocks(m.n_chunks), chains);
     for (int i = 0; i < n; ++i) {
       k_lda_dirichlet<<<gd, kDirThreads, 0, stream>>>(m, seed, sweep);
       AUGUR_CHECK_LAUNCH();
-      if (auto* ev = prof_slot())
-        hipExtLaunchKernelGGL(k_lda_z, gz, dim3(kZThreads), 0, stream, ev->first, ev->second, 0, m, seed,
-                              sweep, 0u);
-      else
-        k_lda_z<<<gz, kZThreads, 0, stream>>>(m, seed, sweep, 0u);
-      AUGUR_CHECK_LAUNCH();
+      launch_lda_z(m, chains, seed, sweep, 0u, cus, stream, prof_slot());
       ++sweep;
       record_traces();
     }
@@ -404,8 +451,7 @@ static void lda_zstep_op(int n_docs, int k, int v, const int32_t* doc_off, const
   m.K = k; m.D = n_docs; m.V = v; m.T = T; m.n_chunks = (int)(ch.size() / 4);
   m.doc_off = ddo.p; m.chunks = dch.p; m.w = dw.p; m.z = dz.p; m.cnt_theta = dct.p; m.cnt_phi = dcp.p;
   m.theta32 = dth.p; m.phit32 = dph.p;
-  k_lda_z<<<dim3(z_blocks(m.n_chunks), 1), kZThreads>>>(m, seed, sweep, chain);
-  AUGUR_CHECK_LAUNCH();
+  launch_lda_z(m, 1, seed, sweep, chain, device_cus(), 0, nullptr);
   dz.download(z, T, 0);
   if (cnt_theta) dct.download(cnt_theta, (size_t)n_docs * k, 0);
   if (cnt_phi) dcp.download(cnt_phi, (size_t)k * v, 0);
@@ -428,6 +474,7 @@ augur_status augur_lda_create(const augur_lda_desc* d, augur_sampler** out) {
     try {
       AUGUR_HIP(hipGetDevice(&s->device));
       AUGUR_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+      s->cus = device_cus();
       s->chains = d->chains;
       s->seed = d->seed;
       const int K = d->n_topics, D = d->n_docs, V = d->n_vocab, C = d->chains;

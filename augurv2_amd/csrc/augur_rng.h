@@ -85,24 +85,42 @@ __host__ __device__ inline double normal_at(const Stream& s, uint32_t site, uint
   return normal_from(w.x, w.y);
 }
 
+// One Marsaglia-Tsang attempt (counter sub = j): the candidate d v, the
+// word that feeds the shape < 1 boost, and whether it is accepted.
+__host__ __device__ inline bool mt_attempt(const Stream& s, uint32_t site, uint32_t j, double d, double c,
+                                           double* x, uint32_t* boost_word) {
+  const u32x4 w = s.draw(site, j);
+  const double z = normal_from(w.x, w.y);
+  const double u = u01(w.z);
+  const double t = 1.0 + c * z;
+  const double v = t * t * t;
+  *x = d * v;
+  *boost_word = w.w;
+  return z > -1.0 / c && log(u) < 0.5 * z * z + d - d * v + d * log(v);
+}
+
 // Gamma(shape, 1) by Marsaglia-Tsang (cbits/augur_dist.c:287-312); attempt j
-// at a site uses counter sub = j.  Returns NaN for an invalid shape or after
-// kGammaMaxAttempts rejections (acceptance is >= 95 % per attempt, so the cap
-// only bounds the loop for non-finite input).
+// at a site uses counter sub = j and the first accepted attempt is returned.
+// Attempts are evaluated in pairs (2i, 2i + 1): the two are independent, so
+// a wave with one thread per site overlaps their latencies, and in a wave of
+// 64 sites some thread almost always needs the second one.  The result is
+// the same as taking the attempts one at a time.  Returns NaN for an invalid
+// shape or after kGammaMaxAttempts rejections (acceptance is >= 95 % per
+// attempt, so the cap only bounds the loop for non-finite input).
 __host__ __device__ inline double gamma_at(const Stream& s, uint32_t site, double shape) {
+  static_assert(kGammaMaxAttempts % 2 == 0, "attempts are taken in pairs");
   if (!(shape > 0.0) || !(shape < 1.0e300)) return __builtin_nan("");
   const double a = shape < 1.0 ? shape + 1.0 : shape;
   const double d = a - 1.0 / 3.0;
   const double c = 1.0 / sqrt(9.0 * d);
-  for (int j = 0; j < kGammaMaxAttempts; ++j) {
-    const u32x4 w = s.draw(site, (uint32_t)j);
-    const double z = normal_from(w.x, w.y);
-    const double u = u01(w.z);
-    const double t = 1.0 + c * z;
-    const double v = t * t * t;
-    if (z > -1.0 / c && log(u) < 0.5 * z * z + d - d * v + d * log(v)) {
-      double x = d * v;
-      if (shape < 1.0) x *= pow(u01(w.w), 1.0 / shape);
+  for (int j = 0; j < kGammaMaxAttempts; j += 2) {
+    double x0, x1;
+    uint32_t b0, b1;
+    const bool ok0 = mt_attempt(s, site, (uint32_t)j, d, c, &x0, &b0);
+    const bool ok1 = mt_attempt(s, site, (uint32_t)j + 1u, d, c, &x1, &b1);
+    if (ok0 || ok1) {
+      double x = ok0 ? x0 : x1;
+      if (shape < 1.0) x *= pow(u01(ok0 ? b0 : b1), 1.0 / shape);
       return x;
     }
   }

@@ -69,8 +69,13 @@ def _corpus(rng, D, K, V, mean_len):
     return doc_off, w
 
 
-@pytest.mark.parametrize("K", [4, 10, 64, 100])
+@pytest.mark.parametrize("K", [4, 10, 64, 100, 120])
 def test_lda_zstep_bit_exact(ops, oracle, K):
+    """z bit-exact with the oracle's FP32 draw and both count tables exact.
+    Up to K * V = 32768 the kernel counts word-topic pairs in an LDS table
+    per block (K <= 100 here); K = 120 (36,000 cells) takes the form that
+    adds every token in HBM; K = 100 also takes the per-token topic atomics
+    (K > 64)."""
     rng = np.random.default_rng(K)
     D, V = 57, 300
     doc_off, w = _corpus(rng, D, K, V, 150)
