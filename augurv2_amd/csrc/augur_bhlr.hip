@@ -529,11 +529,15 @@ typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x4_t __attribute__((ext_vector_type(4)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 
-__device__ inline uint16_t bf16_rne(float f) {
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
+// a = h + m + l: each part rounded to BF16 to nearest even by the hardware
+// conversion (V_CVT_PK_BF16_F32; identical to the integer round-to-nearest-
+// even for every non-NaN float); the residuals are exact in FP32, so the
+// three parts carry FP32's 24 significand bits.
+__device__ inline void split_bf16x3(float a, __bf16& h, __bf16& m, __bf16& l) {
+  h = (__bf16)a;
+  const float r1 = a - (float)h;
+  m = (__bf16)r1;
+  l = (__bf16)(r1 - (float)m);
 }
 
 constexpr int kMfmaCols = 16;
@@ -584,13 +588,11 @@ __device__ void bhlr_body_mfma(const BArgs& A, const BDs& ds, unsigned char* sme
 #pragma unroll
     for (int e = 0; e < KE; ++e) {
       const float b = live ? A.beta32[(size_t)li * A.GW + ds.goff + KB * s + KE * kg + e] : 0.0f;
-      const uint16_t h = bf16_rne(b);
-      const float r1 = b - __uint_as_float((uint32_t)h << 16);
-      const uint16_t m = bf16_rne(r1);
-      const float r2 = r1 - __uint_as_float((uint32_t)m << 16);
-      bh[s][e] = __builtin_bit_cast(__bf16, h);
-      bm[s][e] = __builtin_bit_cast(__bf16, m);
-      bl[s][e] = __builtin_bit_cast(__bf16, bf16_rne(r2));
+      __bf16 h, m, l;
+      split_bf16x3(b, h, m, l);
+      bh[s][e] = h;
+      bm[s][e] = m;
+      bl[s][e] = l;
     }
   f32x4_t g[FB];
 #pragma unroll
@@ -656,12 +658,11 @@ __device__ void bhlr_body_mfma(const BArgs& A, const BDs& ds, unsigned char* sme
       const float lr = log_sigmoid_pm(t[r], yv[r], &a);
       if (ok) ll += (double)lr;
       else a = 0.0f;
-      const uint16_t h = bf16_rne(a);
-      const float r1 = a - __uint_as_float((uint32_t)h << 16);
-      const uint16_t m = bf16_rne(r1);
-      ah[r] = __builtin_bit_cast(__bf16, h);
-      am[r] = __builtin_bit_cast(__bf16, m);
-      al[r] = __builtin_bit_cast(__bf16, bf16_rne(r1 - __uint_as_float((uint32_t)m << 16)));
+      __bf16 h, m, l;
+      split_bf16x3(a, h, m, l);
+      ah[r] = h;
+      am[r] = m;
+      al[r] = l;
     }
     // the wave's tile copy is complete before any lane reads it
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

@@ -12,11 +12,9 @@ __device__ inline float bf16_lo(uint32_t w) { return __uint_as_float(w << 16); }
 __device__ inline float bf16_hi(uint32_t w) { return __uint_as_float(w & 0xffff0000u); }
 __device__ inline float bf16_to_f(uint16_t u) { return __uint_as_float((uint32_t)u << 16); }
 
-__device__ inline uint16_t f_to_bf16(float f) {  // round to nearest even
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
+__device__ inline uint16_t f_to_bf16(float f) {
+  // round to nearest even by the hardware conversion (V_CVT_PK_BF16_F32)
+  return __builtin_bit_cast(uint16_t, (__bf16)f);
 }
 
 __device__ inline float gelu_s(float z) {
