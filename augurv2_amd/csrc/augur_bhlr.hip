@@ -41,6 +41,14 @@ struct BDs {
   const uint8_t* y;
 };
 
+// Rows of every dataset are stored padded with zero rows (X and y) to a
+// multiple of kRowPad: any tile of the register or matrix-core pass that
+// starts below n then lies wholly in the buffers, so their loads carry no
+// bounds checks (rows at or past the block's end are masked in the row term),
+// and each dataset's X and y start 512-byte and 256-byte aligned.
+constexpr int kRowPad = 256;
+inline int padded_rows(int n) { return (n + kRowPad - 1) / kRowPad * kRowPad; }
+
 struct BArgs {
   const BDs* ds;
   const int* blk_ds;
@@ -354,17 +362,16 @@ __device__ void bhlr_body_r(const BArgs& A, const BDs& ds, unsigned char* smem) 
 
   xword pre[PD][RPT];
   int ypre[PD][RPT];
+  // tiles start at multiples of TRr below rend <= n, inside the dataset's
+  // rows padded to kRowPad: no bounds checks on the loads (rows at or past
+  // rend are masked in the row term)
+  static_assert(kRowPad % TRr == 0, "tiles within the padding");
   auto prefetch = [&](int t, xword (&dst)[RPT], int (&yd)[RPT]) {
 #pragma unroll
     for (int i = 0; i < RPT; ++i) {
       const int row = rbeg + t * TRr + i * RG + rsub;
-      if (row < rend) {
-        dst[i] = *(const xword*)(ds.x + (size_t)row * D + F * q);
-        yd[i] = ds.y[row];
-      } else {
-        dst[i] = xword{};
-        yd[i] = 0;
-      }
+      dst[i] = *(const xword*)(ds.x + (size_t)row * D + F * q);
+      yd[i] = ds.y[row];
     }
   };
 #pragma unroll
@@ -608,17 +615,17 @@ __device__ void bhlr_body_mfma(const BArgs& A, const BDs& ds, unsigned char* sme
   const int w0 = rbeg + 16 * wv;
   const int ntw = w0 < rend ? (rend - w0 + 63) / 64 : 0;
   braw qn[PD][KC];
-  int yn[PD][4];
-  auto load_tile = [&](int t, braw (&qd)[KC], int (&yd)[4]) {
+  uint32_t yn[PD];
+  // A tile starts at a multiple of 16 rows below rend <= n, inside the
+  // dataset's rows padded to kRowPad: X and y load without bounds checks, y
+  // as the 4 bytes of rows t0 + 4 kg .. + 3 in one aligned word.
+  static_assert(kRowPad % 16 == 0, "16-row tiles within the padding");
+  auto load_tile = [&](int t, braw (&qd)[KC], uint32_t& yd) {
     const int t0 = w0 + 64 * t;
-    const int row = t0 + li;
+    const uint16_t* xr = ds.x + (size_t)(t0 + li) * D + KE * kg;
 #pragma unroll
-    for (int s = 0; s < KC; ++s) {
-      qd[s] = braw{};
-      if (row < rend) qd[s] = *(const braw*)(ds.x + (size_t)row * D + KB * s + KE * kg);
-    }
-#pragma unroll
-    for (int r = 0; r < 4; ++r) yd[r] = t0 + 4 * kg + r < rend ? (int)ds.y[t0 + 4 * kg + r] : 0;
+    for (int s = 0; s < KC; ++s) qd[s] = *(const braw*)(xr + KB * s);
+    yd = *(const uint32_t*)(ds.y + t0 + 4 * kg);
   };
 #pragma unroll
   for (int k = 0; k < PD; ++k)
@@ -630,24 +637,19 @@ __device__ void bhlr_body_mfma(const BArgs& A, const BDs& ds, unsigned char* sme
     if (tt >= ntw) break;
     const int r0 = w0 + 64 * tt;
     // phase B: A operand = row r0 + li, features KB s + KE kg .. + KE - 1
-    bvec xa[KC];
-    int yv[4];
 #pragma unroll
-    for (int s = 0; s < KC; ++s) {
-      const braw q = qn[slot][s];
-      *(braw*)(slab + li * XS + KB * s + KE * kg) = q;
-      xa[s] = __builtin_bit_cast(bvec, q);
-    }
-#pragma unroll
-    for (int r = 0; r < 4; ++r) yv[r] = yn[slot][r];
-    if (tt + PD < ntw) load_tile(tt + PD, qn[slot], yn[slot]);
+    for (int s = 0; s < KC; ++s) *(braw*)(slab + li * XS + KB * s + KE * kg) = qn[slot][s];
+    const uint32_t yw = yn[slot];
     f32x4_t t = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int s = 0; s < KC; ++s) {
-      t = MfmaB<KB>::mma(xa[s], bh[s], t);
-      t = MfmaB<KB>::mma(xa[s], bm[s], t);
-      t = MfmaB<KB>::mma(xa[s], bl[s], t);
+      const bvec xa = __builtin_bit_cast(bvec, qn[slot][s]);
+      t = MfmaB<KB>::mma(xa, bh[s], t);
+      t = MfmaB<KB>::mma(xa, bm[s], t);
+      t = MfmaB<KB>::mma(xa, bl[s], t);
     }
+    // the slot's registers are free once the products have read them
+    if (tt + PD < ntw) load_tile(tt + PD, qn[slot], yn[slot]);
     // rows r0 + 4 kg + r, chain li
     bf16x4_t ah, am, al;
 #pragma unroll
@@ -655,7 +657,7 @@ __device__ void bhlr_body_mfma(const BArgs& A, const BDs& ds, unsigned char* sme
       const int rr = r0 + 4 * kg + r;
       const bool ok = rr < rend && live;
       float a;
-      const float lr = log_sigmoid_pm(t[r], yv[r], &a);
+      const float lr = log_sigmoid_pm(t[r], (int)((yw >> (8 * r)) & 0xffu), &a);
       if (ok) ll += (double)lr;
       else a = 0.0f;
       __bf16 h, m, l;
@@ -954,6 +956,10 @@ struct BhlrSampler final : augur_sampler {
 
   // Blocks of R rows per dataset; partial offsets follow.
   void layout(int rows) {
+    // blocks start at multiples of kRowPad rows, so the last block's tiles
+    // end inside its dataset's padding
+    if (rows < kRowPad || rows % kRowPad != 0 || rows % kBTile != 0)
+      fail(AUGUR_ERR_ARG, "rows per block %d is not a positive multiple of %d", rows, kRowPad);
     R = rows;
     int blk = 0, poff = 0, voff = 0;
     std::vector<int> table, vtable;
@@ -1195,8 +1201,8 @@ augur_status augur_bhlr_create(const augur_bhlr_desc* D, augur_sampler** out) {
         soff += b.d + 2;
         goff += b.d + 1;
         s->dmax = std::max(s->dmax, b.d);
-        xtot += (size_t)b.n * b.d;
-        ytot += (size_t)b.n;
+        xtot += (size_t)padded_rows(b.n) * b.d;
+        ytot += (size_t)padded_rows(b.n);
       }
       s->SW = soff;
       s->GW = goff;
@@ -1208,17 +1214,17 @@ augur_status augur_bhlr_create(const augur_bhlr_desc* D, augur_sampler** out) {
         size_t xo = 0, yo = 0;
         for (int i = 0; i < s->n_ds; ++i) {
           BDs& b = s->hds[i];
-          const size_t ne = (size_t)b.n * b.d;
-          hx.resize(ne);
+          const size_t ne = (size_t)b.n * b.d, np = (size_t)padded_rows(b.n);
+          hx.assign(np * b.d, 0);
           for (size_t e = 0; e < ne; ++e) hx[e] = to_bf16(D->x[i][e]);
-          hy.resize(b.n);
+          hy.assign(np, 0);
           for (int r = 0; r < b.n; ++r) hy[r] = (uint8_t)D->y[i][r];
-          AUGUR_HIP(hipMemcpy(s->x.p + xo, hx.data(), ne * 2, hipMemcpyHostToDevice));
-          AUGUR_HIP(hipMemcpy(s->y.p + yo, hy.data(), b.n, hipMemcpyHostToDevice));
+          AUGUR_HIP(hipMemcpy(s->x.p + xo, hx.data(), np * b.d * 2, hipMemcpyHostToDevice));
+          AUGUR_HIP(hipMemcpy(s->y.p + yo, hy.data(), np, hipMemcpyHostToDevice));
           b.x = s->x.p + xo;
           b.y = s->y.p + yo;
-          xo += ne;
-          yo += b.n;
+          xo += np * b.d;
+          yo += np;
         }
       }
       const size_t C = s->C;
@@ -1279,7 +1285,7 @@ augur_status augur_bhlr_create(const augur_bhlr_desc* D, augur_sampler** out) {
       for (size_t c = 0; c < C; ++c)
         for (int i = 0; i < s->n_ds; ++i) st[c * s->SW + s->hds[i].soff + s->hds[i].d + 1] = 1.0;
       s->state.upload(st.data(), st.size(), s->stream);
-      if (D->rows_per_block > 0) s->layout(((D->rows_per_block + kBTile - 1) / kBTile) * kBTile);
+      if (D->rows_per_block > 0) s->layout(padded_rows(D->rows_per_block));
       else s->autotune();
       AUGUR_HIP(hipStreamSynchronize(s->stream));
     } catch (...) {

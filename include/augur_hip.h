@@ -176,7 +176,7 @@ typedef struct {
   int ref_compat;
   augur_hmc_block v_block;          /* HMC [v] */
   augur_hmc_block theta_block;      /* HMC [theta]: fixed_steps >= 1 required */
-  int rows_per_block;               /* 0: autotune */
+  int rows_per_block;               /* 0: autotune; else rounded up to a multiple of 256 */
   int use_graph;
 } augur_bhlr_desc;
 augur_status augur_bhlr_create(const augur_bhlr_desc* desc, augur_sampler** out);
