@@ -553,6 +553,9 @@ constexpr int kMfRing = 4;  // 16-row tiles in flight per wave in the matrix-cor
 template <int D>
 constexpr int mfma_slab_stride() { return D + 8; }  // BF16 elements per LDS row
 
+typedef short i16x4_t __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) i16x4_t lds_i16x4;  // 4 x 16 bits in LDS
+
 // Phase-B chunking: 32 features per V_MFMA_F32_16X16X32_BF16 when D allows,
 // else 16 per V_MFMA_F32_16X16X16_BF16; lane (i, kg) supplies features
 // KB s + (KB / 4) kg + e of chunk s.
@@ -579,6 +582,7 @@ __device__ void bhlr_body_mfma(const BArgs& A, const BDs& ds, unsigned char* sme
   constexpr int KC = D / KB;                   // phase-B chunks
   constexpr int FB = D / 16;                   // 16-feature blocks of phase C
   constexpr int XS = mfma_slab_stride<D>();
+  static_assert(XS % 4 == 0, "8-byte rows for the transposing read");
   typedef typename MfmaB<KB>::vec bvec;
   typedef typename MfmaB<KB>::raw braw;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -670,12 +674,15 @@ __device__ void bhlr_body_mfma(const BArgs& A, const BDs& ds, unsigned char* sme
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // phase C: A operand = X^T[feature 16 f + li][rows 4 kg + e]
+    // phase C: A operand = X^T[feature 16 f + li][rows 4 kg + e], e = 0..3, by
+    // one transposing LDS read: in each 16-lane group, lane i receives as
+    // element e the (i % 4)-th 16-bit value at the address of lane 4 e + i / 4
+    // of its group, so lane (li, kg) supplies row 4 kg + li / 4, columns
+    // 16 f + 4 (li % 4) .. + 3 (8-byte aligned: XS is a multiple of 4)
+    lds_i16x4* trow = (lds_i16x4*)(slab + (4 * kg + (li >> 2)) * XS + 4 * (li & 3));
 #pragma unroll
     for (int f = 0; f < FB; ++f) {
-      bf16x4_t xt;
-#pragma unroll
-      for (int e = 0; e < 4; ++e) xt[e] = __builtin_bit_cast(__bf16, slab[(4 * kg + e) * XS + 16 * f + li]);
+      const bf16x4_t xt = __builtin_bit_cast(bf16x4_t, __builtin_amdgcn_ds_read_tr16_b64_v4i16(trow + 4 * f));
       g[f] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(xt, ah, g[f], 0, 0, 0);
       g[f] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(xt, am, g[f], 0, 0, 0);
       g[f] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(xt, al, g[f], 0, 0, 0);
