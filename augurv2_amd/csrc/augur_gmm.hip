@@ -306,6 +306,8 @@ __global__ __launch_bounds__(kConjLanes) void k_gmm_tables(GmmDev m) {
 }
 
 // DM: padded dimension, KM: maximum number of components held in registers.
+typedef __attribute__((address_space(4))) float kconst_f32;  // constant address space
+
 template <int DM, int KM>
 __global__ __launch_bounds__(kGmmThreads) void k_gmm_z(GmmDev m, uint32_t seed, uint32_t sweep,
                                                       uint32_t chain0) {
@@ -341,10 +343,20 @@ __global__ __launch_bounds__(kGmmThreads) void k_gmm_z(GmmDev m, uint32_t seed, 
     }
     float lg[KM];
     float mx = -INFINITY;
+    // The table is the same for every point, so the compiler would hoist its
+    // per-component addresses and values out of the point loop; they do not
+    // fit in scalar registers across it and were spilled to vector lanes
+    // (about 600 v_writelane / v_readlane per point).  An empty asm statement
+    // makes the pointer opaque once per point, so the table is read through
+    // the scalar cache inside the loop instead; the pointer is in the
+    // constant address space (the table is not written during the kernel),
+    // which keeps those reads scalar loads.
+    const kconst_f32* gt = (const kconst_f32*)gtab;
+    asm volatile("" : "+s"(gt));
 #pragma unroll
     for (int kk = 0; kk < KM; ++kk) {
       if (kk < K) {
-        const float* t = gtab + kk * TS;
+        const kconst_f32* t = gt + kk * TS;
         float dv[DM];
 #pragma unroll
         for (int i = 0; i < DM; ++i) dv[i] = yv[i] - t[tri(DM) + i];
