@@ -136,6 +136,23 @@ __device__ inline float dot_slice(const uint32_t (&w)[NP], const uint16_t* act) 
   return a;
 }
 
+// dot_slice with the weights in LDS, pair q at w[1024 q] (one column per
+// thread, so the reads of a wave are consecutive words).
+template <int NP>
+__device__ inline float dot_slice_lds(const uint32_t* w, const uint16_t* act) {
+  const uint4* src = reinterpret_cast<const uint4*>(act);
+  float a = 0.0f;
+#pragma unroll
+  for (int q = 0; q < NP / 4; ++q) {
+    const uint4 u = src[q];
+    a = dot2_bf16(w[(4 * q) * 1024], u.x, a);
+    a = dot2_bf16(w[(4 * q + 1) * 1024], u.y, a);
+    a = dot2_bf16(w[(4 * q + 2) * 1024], u.z, a);
+    a = dot2_bf16(w[(4 * q + 3) * 1024], u.w, a);
+  }
+  return a;
+}
+
 template <int D, int H>
 __global__ __launch_bounds__(1024) void k_surr_traj_t(SurrTrajArgs A) {
   constexpr int P1 = 1024 / H, I1 = D / P1;  // layer 1: H outputs, D inputs
@@ -151,8 +168,9 @@ __global__ __launch_bounds__(1024) void k_surr_traj_t(SurrTrajArgs A) {
   const int tid = threadIdx.x, c = blockIdx.x;
   const int o1 = tid % H, p1 = tid / H;
   const int o3 = tid % D, p3 = tid / D;
-  uint32_t w1[I1 / 2], w2[I2 / 2], w3[I3 / 2];
+  uint32_t w1[I1 / 2], w3[I3 / 2];
   float wl[IL];
+  extern __shared__ uint32_t w2s[];  // layer 2 weights: pair q of thread t at w2s[q * 1024 + t]
 #pragma unroll
   for (int q = 0; q < I1 / 2; ++q) {
     const int i = p1 * I1 + 2 * q;
@@ -161,7 +179,7 @@ __global__ __launch_bounds__(1024) void k_surr_traj_t(SurrTrajArgs A) {
 #pragma unroll
   for (int q = 0; q < I2 / 2; ++q) {
     const int i = p1 * I2 + 2 * q;
-    w2[q] = (uint32_t)A.w2t[(size_t)i * H + o1] | ((uint32_t)A.w2t[(size_t)(i + 1) * H + o1] << 16);
+    w2s[q * 1024 + tid] = (uint32_t)A.w2t[(size_t)i * H + o1] | ((uint32_t)A.w2t[(size_t)(i + 1) * H + o1] << 16);
   }
 #pragma unroll
   for (int q = 0; q < I3 / 2; ++q) {
@@ -205,7 +223,7 @@ __global__ __launch_bounds__(1024) void k_surr_traj_t(SurrTrajArgs A) {
       h1[tid] = f_to_bf16(gelu_s(z));
     }
     __syncthreads();
-    r2[p1][o1] = dot_slice(w2, h1 + p1 * I2);
+    r2[p1][o1] = dot_slice_lds<I2 / 2>(w2s + tid, h1 + p1 * I2);
     __syncthreads();
     if (tid < H) {
       float z = b2v;
@@ -244,11 +262,18 @@ __global__ __launch_bounds__(1024) void k_surr_traj_t(SurrTrajArgs A) {
   }
 }
 
+template <int D, int H>
+static void launch_traj_t(const SurrTrajArgs& a, int chains, hipStream_t st) {
+  const size_t smem = sizeof(uint32_t) * 1024 * (size_t)(H / (1024 / H) / 2);  // layer 2 weights
+  allow_lds(k_surr_traj_t<D, H>, smem);
+  k_surr_traj_t<D, H><<<chains, 1024, smem, st>>>(a);
+}
+
 void launch_surr_traj(const SurrTrajArgs& a, int chains, hipStream_t st) {
   if (a.d > 256 || a.h > 256) fail(AUGUR_ERR_UNSUPPORTED, "surrogate needs d, hidden <= 256");
-  if (a.d == 128 && a.h == 256) k_surr_traj_t<128, 256><<<chains, 1024, 0, st>>>(a);
-  else if (a.d == 64 && a.h == 256) k_surr_traj_t<64, 256><<<chains, 1024, 0, st>>>(a);
-  else if (a.d == 64 && a.h == 128) k_surr_traj_t<64, 128><<<chains, 1024, 0, st>>>(a);
+  if (a.d == 128 && a.h == 256) launch_traj_t<128, 256>(a, chains, st);
+  else if (a.d == 64 && a.h == 256) launch_traj_t<64, 256>(a, chains, st);
+  else if (a.d == 64 && a.h == 128) launch_traj_t<64, 128>(a, chains, st);
   else k_surr_traj<<<chains, 256, 0, st>>>(a);
   AUGUR_CHECK_LAUNCH();
 }
@@ -338,12 +363,22 @@ __global__ __launch_bounds__(256) void k_ll64_mfma(Ll64Args A) {
       acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)(sg * xv.z), bq[q][2], acc, 0, 0, 0);
       acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)(sg * xv.w), bq[q][3], acc, 0, 0, 0);
     }
+    if (A.cp <= 8) {
+      // columns 8..15 hold no chain: lane i + 8 takes rows 2, 3 of lane i's
+      // column, so each lane evaluates two FP64 row terms instead of four
+      const double u2 = __shfl_xor(acc[2], 8), u3 = __shfl_xor(acc[3], 8);
+      const double a0 = i < 8 ? acc[0] : u2, a1 = i < 8 ? acc[1] : u3;
+      lsum += fmin(a0, 0.0) - log1p(exp(-fabs(a0)));
+      lsum += fmin(a1, 0.0) - log1p(exp(-fabs(a1)));
+    } else {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const double u = acc[k];
-      lsum += fmin(u, 0.0) - log1p(exp(-fabs(u)));
+      for (int k = 0; k < 4; ++k) {
+        const double u = acc[k];
+        lsum += fmin(u, 0.0) - log1p(exp(-fabs(u)));
+      }
     }
   }
+  if (A.cp <= 8) lsum += __shfl_xor(lsum, 8);  // chain i: lanes i and i + 8
   lsum += __shfl_xor(lsum, 16);
   lsum += __shfl_xor(lsum, 32);
   if (lane < 16) red[wv][lane] = lsum;
