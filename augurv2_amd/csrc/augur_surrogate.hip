@@ -350,14 +350,29 @@ __global__ __launch_bounds__(256) void k_ll64_mfma(Ll64Args A) {
     for (int e = 0; e < 4; ++e) bq[q][e] = live ? A.point[(size_t)i * A.P + 4 * (4 * q + kb) + e] : 0.0;
   double lsum = 0.0;
   const int tiles = A.n / 16;
-  for (int t = blockIdx.x * 4 + wv; t < tiles; t += gridDim.x * 4) {
-    const int r = t * 16 + i;
-    const float sg = A.y[r] ? 1.0f : -1.0f;
-    const float4* xr = reinterpret_cast<const float4*>(A.x + (size_t)r * D);
+  // tile t: rows 16 t + i, this lane's features 16 q + 4 kb .. + 3; the next
+  // tile of the wave is requested before this one's products (one in flight)
+  const int tstep = gridDim.x * 4;
+  float4 xn[Q];
+  int yn = 0;
+  auto load = [&](int t) {
+    const float4* xr = reinterpret_cast<const float4*>(A.x + (size_t)(t * 16 + i) * D);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) xn[q] = xr[4 * q + kb];
+    yn = A.y[t * 16 + i];
+  };
+  int t = blockIdx.x * 4 + wv;
+  if (t < tiles) load(t);
+  for (; t < tiles; t += tstep) {
+    float4 xc[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) xc[q] = xn[q];
+    const float sg = yn ? 1.0f : -1.0f;
+    if (t + tstep < tiles) load(t + tstep);
     f64x4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-      const float4 xv = xr[4 * q + kb];
+      const float4 xv = xc[q];
       acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)(sg * xv.x), bq[q][0], acc, 0, 0, 0);
       acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)(sg * xv.y), bq[q][1], acc, 0, 0, 0);
       acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)(sg * xv.z), bq[q][2], acc, 0, 0, 0);
