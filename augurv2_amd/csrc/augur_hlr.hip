@@ -508,6 +508,10 @@ __host__ __device__ constexpr int row_ring(int cp) { return 2; }
 
 __device__ inline float f4c(const float4& v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w; }
 
+// X and y are stored with zero rows up to a multiple of kHlrRowPad, so a tile
+// of the register-form pass that starts below n lies wholly in the buffers.
+constexpr int kHlrRowPad = 256;
+
 template <int D, int CP>
 __global__ __launch_bounds__(kPassThreads, kPassBlocksPerCU) void k_hlr_pass_r(PassArgs A) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -520,6 +524,7 @@ __global__ __launch_bounds__(kPassThreads, kPassBlocksPerCU) void k_hlr_pass_r(P
   constexpr int PD = row_ring(CP);                // tiles in flight
   constexpr int GS = CP * F + 1;                  // LDS floats per lane (padded)
   static_assert(D % F == 0 && LPR >= 2 && LPR <= 8 && LPR % CP == 0, "shape");
+  static_assert(kHlrRowPad % TRr == 0, "tiles end inside the row padding");
 
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int q = tid % LPR;          // feature group: features F q .. F q + F - 1
@@ -537,21 +542,22 @@ __global__ __launch_bounds__(kPassThreads, kPassBlocksPerCU) void k_hlr_pass_r(P
     for (int j = 0; j < F; ++j) th[c][j] = A.beta32[c * (D + 1) + F * q + j];
   const float b_c = A.has_b ? A.beta32[my_c * (D + 1) + D] : 0.0f;
 
-  float4 pre[PD][RPT][2];
-  int ypre[PD][RPT];
+  // PD tiles in flight in NB = PD + 1 register buffers: the refill of a slot
+  // goes to the buffer the previous slot consumed, so the loads can be issued
+  // before this slot's work without copying registers around the loop
+  constexpr int NB = PD + 1;
+  float4 pre[NB][RPT][2];
+  int ypre[NB][RPT];
   auto prefetch = [&](int tile, float4 (&dst)[RPT][2], int (&yd)[RPT]) {
 #pragma unroll
     for (int i = 0; i < RPT; ++i) {
+      // rows of a tile lie below the row count padded to kHlrRowPad (zero
+      // rows): no bounds check; rows at or past n are masked in the row term
       const int row = tile * TRr + i * RG + rsub;
-      if (row < n) {
-        const float4* s4 = (const float4*)(A.x + (size_t)row * D + F * q);
-        dst[i][0] = s4[0];
-        dst[i][1] = s4[1];
-        yd[i] = A.y[row];
-      } else {
-        dst[i][0] = dst[i][1] = make_float4(0.f, 0.f, 0.f, 0.f);
-        yd[i] = 0;
-      }
+      const float4* s4 = (const float4*)(A.x + (size_t)row * D + F * q);
+      dst[i][0] = s4[0];
+      dst[i][1] = s4[1];
+      yd[i] = A.y[row];
     }
   };
 #pragma unroll
@@ -567,12 +573,12 @@ __global__ __launch_bounds__(kPassThreads, kPassBlocksPerCU) void k_hlr_pass_r(P
     for (int j = 0; j < F; ++j) g[c][j] = 0.0f;
   double llacc = 0.0, gbacc = 0.0;
 
-  for (int base = blockIdx.x; base < tiles; base += PD * G) {
+  for (int base = blockIdx.x; base < tiles; base += NB * G) {
 #pragma unroll
-    for (int slot = 0; slot < PD; ++slot) {
+    for (int slot = 0; slot < NB; ++slot) {
       const int tile = base + slot * G;
       if (tile >= tiles) break;
-      // the slot is refilled after this tile (the other slots stay in flight)
+      if (tile + PD * G < tiles) prefetch(tile + PD * G, pre[(slot + PD) % NB], ypre[(slot + PD) % NB]);
       auto xr = [&](int i, int j) { return f4c(pre[slot][i][j >> 2], j & 3); };
       const int* yv = ypre[slot];
       // phase B: shares of t[i][c] = x_i . theta_c, completed over the row's lanes
@@ -621,7 +627,6 @@ __global__ __launch_bounds__(kPassThreads, kPassBlocksPerCU) void k_hlr_pass_r(P
 #pragma unroll
         for (int j = 0; j < F; ++j) g[c][j] = fmaf(av, xr(i, j), g[c][j]);
       }
-      if (tile + PD * G < tiles) prefetch(tile + PD * G, pre[slot], ypre[slot]);
     }
   }
 
@@ -1176,8 +1181,13 @@ struct HlrSampler final : augur_sampler {
     if (const char* e = getenv("AUGUR_HLR_BLOCKS")) want = std::max(1, atoi(e));
     G = std::max(1, std::min(tiles, want));
     const size_t P = (size_t)d + 2, ds = (size_t)d + 1;
-    x.alloc((size_t)n * d);
-    y.alloc(std::max(n, 1));
+    // X and y padded with zero rows to a multiple of kHlrRowPad (the data
+    // overwrite the first n rows)
+    const size_t np = ((size_t)n + kHlrRowPad - 1) / kHlrRowPad * kHlrRowPad;
+    x.alloc(np * d);
+    y.alloc(std::max<size_t>(np, 1));
+    x.zero(stream);
+    y.zero(stream);
     beta32.alloc(cp * ds);
     state.alloc(cp * P); prop.alloc(cp * P);
     p0.alloc(cp * ds); p.alloc(cp * ds); k0.alloc(cp);
