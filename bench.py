@@ -23,7 +23,7 @@ HBM_PEAK_GBS = 8000.0
 # and WRITE_SIZE (KiB per dispatch) on this workload, each scaled by what the
 # counter reads for a plain 256 MiB device copy (`--copy-mib 256`: fetch x2.0,
 # write x1.0); profiles/k_hlr_pass_r_hlr_n100k_d64_c4.md
-PASS_FETCH_KIB, PASS_WRITE_KIB = 12752.8, 1056.0
+PASS_FETCH_KIB, PASS_WRITE_KIB = 12762.9, 1056.0
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 
 
