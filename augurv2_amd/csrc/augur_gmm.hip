@@ -394,6 +394,168 @@ __global__ __launch_bounds__(kGmmThreads) void k_gmm_z(GmmDev m, uint32_t seed, 
   }
 }
 
+typedef __bf16 gmm_bf16x8 __attribute__((ext_vector_type(8)));
+
+// x = h + m + l, each part rounded to BF16 to nearest even by the hardware
+// conversion; the residuals are exact in FP32, so the parts carry x's 24
+// significand bits.
+__device__ inline void gmm_split3(float x, __bf16& h, __bf16& m, __bf16& l) {
+  h = (__bf16)x;
+  const float r1 = x - (float)h;
+  m = (__bf16)r1;
+  l = (__bf16)(r1 - (float)m);
+}
+
+constexpr size_t gmm_zm_smem_bytes(int KM) {
+  return sizeof(float) * 17 * (size_t)KM + sizeof(uint4) * 2 * 64 * (size_t)KM;  // KM * 17 floats: 16-byte multiple
+}
+
+// The z step of k_gmm_z for a padded dimension of 16, with the triangular
+// products on the matrix cores.  For point x and component k,
+// L^-1 (x - mu_k) = L^-1 x - c_k with c_k = L^-1 mu_k (FP64 from the table,
+// rounded once).  L^-1 x for 16 points at a time is a 16 x 16 x 16 product:
+// L^-1 and x are each split into three BF16 parts, and the six cross
+// products above 2^-16 relative (hh, hm, mh, hl, lh, mm) are summed in FP32
+// by three V_MFMA_F32_16X16X32_BF16, each pairing two parts along its
+// 32-deep K (A = [L_h | L_m] with B = [x_h ; x_h], A = [L_h | L_l] with
+// B = [x_m ; x_h], A = [L_h | L_m] with B = [x_l ; x_m]).  The result keeps
+// FP32 accuracy in L^-1 x; the subtraction of c_k adds a rounding of
+// |L^-1 x| (the FP32 form subtracts mu before the product instead).  The
+// log-sum-exp and the draw are those of k_gmm_z.
+// Per wave: 64 points (4 tiles of 16) at a time; lane (n, kg) supplies
+// point n's features 8 (kg & 1) .. + 7 to every tile's B operands, receives
+// output rows 4 kg .. + 3 of point n, and owns point n of tile kg.
+template <int KM>
+__global__ __launch_bounds__(kGmmThreads) void k_gmm_z_mfma(GmmDev m, uint32_t seed, uint32_t sweep,
+                                                           uint32_t chain0) {
+  constexpr int DM = 16, TS = tri(DM) + DM + 1;
+  extern __shared__ __align__(16) unsigned char smem[];
+  // the small arrays first: LDS reads take a 16-bit immediate offset, so this
+  // keeps almost every per-component address a constant offset from one base
+  float* cvec = reinterpret_cast<float*>(smem);         // [KM][16] c_k
+  float* cst = cvec + KM * 16;                          // [KM] log pi_k + log-det term
+  uint4* aop = reinterpret_cast<uint4*>(cst + KM);      // [KM][2][64] A operands per lane
+  const int K = m.k, d = m.d, c = blockIdx.y, tid = threadIdx.x;
+  const int lane = tid & 63, wv = tid >> 6, n16 = lane & 15, kg = lane >> 4;
+  const float* gtab = m.tab + (size_t)c * K * TS;
+  if (tid == 0) {
+    const double* g = m.gpi + (size_t)c * K;
+    double tot = 0.0;
+    for (int i = 0; i < K; ++i) tot += g[i];
+    for (int i = 0; i < K; ++i) {
+      cst[i] = (float)log(g[i] / tot) + gtab[i * TS + tri(DM) + DM];
+      if (blockIdx.x == 0) m.pis64[(size_t)c * K + i] = g[i] / tot;
+    }
+  }
+  // A operand `op` of component k for lane l = (i, g): 8 BF16 at K index
+  // 8 g + e; index j < 16 is the high part of L^-1[i][j], index 16 + j the
+  // middle (op 0) or low (op 1) part
+  for (int e2 = tid; e2 < K * 128; e2 += kGmmThreads) {
+    const int k = e2 >> 7, op = (e2 >> 6) & 1, l = e2 & 63;
+    const int i = l & 15, g = l >> 4;
+    const float* t = gtab + k * TS;
+    gmm_bf16x8 v;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int j = (8 * g + e) & 15;
+      const float a = j <= i ? t[tri(i) + j] : 0.0f;
+      __bf16 h, mm, lo;
+      gmm_split3(a, h, mm, lo);
+      v[e] = g < 2 ? h : (op == 0 ? mm : lo);
+    }
+    aop[(k * 2 + op) * 64 + l] = __builtin_bit_cast(uint4, v);
+  }
+  for (int e2 = tid; e2 < K * 16; e2 += kGmmThreads) {
+    const int k = e2 >> 4, i = e2 & 15;
+    const float* t = gtab + k * TS;
+    double sum = 0.0;
+    for (int j = 0; j <= i; ++j) sum += (double)t[tri(i) + j] * (double)t[tri(DM) + j];
+    cvec[e2] = (float)sum;
+  }
+  __syncthreads();
+
+  const Stream zs{seed, chain0 + (uint32_t)c, sweep, TAG_GMM_Z};
+  const int half = kg & 1;
+  const int groups = (m.n + 63) / 64;
+  for (int grp = blockIdx.x * (kGmmThreads / 64) + wv; grp < groups; grp += gridDim.x * (kGmmThreads / 64)) {
+    gmm_bf16x8 b1[4], b2[4], b3[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int p = 64 * grp + 16 * j + n16;
+      float xv[8];
+      if (p < m.n && d == DM) {
+        const float4* yr = reinterpret_cast<const float4*>(m.y + (size_t)p * DM + 8 * half);
+        const float4 u0 = yr[0], u1 = yr[1];
+        xv[0] = u0.x; xv[1] = u0.y; xv[2] = u0.z; xv[3] = u0.w;
+        xv[4] = u1.x; xv[5] = u1.y; xv[6] = u1.z; xv[7] = u1.w;
+      } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) xv[e] = (p < m.n && 8 * half + e < d) ? m.y[(size_t)p * d + 8 * half + e] : 0.0f;
+      }
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        __bf16 h, mm, lo;
+        gmm_split3(xv[e], h, mm, lo);
+        b1[j][e] = h;
+        b2[j][e] = kg < 2 ? mm : h;
+        b3[j][e] = kg < 2 ? lo : mm;
+      }
+    }
+    float lg[KM];
+    float mx = -INFINITY;
+#pragma unroll
+    for (int kk = 0; kk < KM; ++kk) {
+      if (kk < K) {
+        const gmm_bf16x8 a1 = __builtin_bit_cast(gmm_bf16x8, aop[(kk * 2) * 64 + lane]);
+        const gmm_bf16x8 a2 = __builtin_bit_cast(gmm_bf16x8, aop[(kk * 2 + 1) * 64 + lane]);
+        const float4 cv = *reinterpret_cast<const float4*>(cvec + kk * 16 + 4 * kg);
+        // qp[j]: this lane's 4 rows of ||L^-1 x - c_k||^2 for point n of tile j
+        float qp[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          gmm_f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1[j], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, b2[j], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b3[j], acc, 0, 0, 0);
+          const float d0 = acc[0] - cv.x, d1 = acc[1] - cv.y, d2 = acc[2] - cv.z, d3 = acc[3] - cv.w;
+          qp[j] = fmaf(d0, d0, fmaf(d1, d1, fmaf(d2, d2, d3 * d3)));
+        }
+        // transpose-reduce over the 4 lanes of point n (kg = 0..3): after the
+        // xor-16 exchange a lane holds tiles kg & 1 and 2 + (kg & 1) summed
+        // over its lane pair, after the xor-32 exchange tile kg over all four
+        const bool b0 = kg & 1, b1s = kg >> 1;
+        float k0 = b0 ? qp[1] : qp[0], k1 = b0 ? qp[3] : qp[2];
+        k0 += __shfl_xor(b0 ? qp[0] : qp[1], 16);
+        k1 += __shfl_xor(b0 ? qp[2] : qp[3], 16);
+        const float q = (b1s ? k1 : k0) + __shfl_xor(b1s ? k0 : k1, 32);
+        const float l = cst[kk] - 0.5f * q;
+        lg[kk] = l;
+        mx = fmaxf(mx, l);
+      }
+    }
+    const int n = 64 * grp + 16 * kg + n16;  // the point this lane owns
+    if (n >= m.n) continue;
+    float tot = 0.0f;
+#pragma unroll
+    for (int kk = 0; kk < KM; ++kk)
+      if (kk < K) {
+        lg[kk] = expf(lg[kk] - mx);
+        tot += lg[kk];
+      }
+    const float u = u01f(zs.draw((uint32_t)n, 0).x);
+    const float tt = u * tot;
+    int zk = K - 1;
+    float run = 0.0f;
+#pragma unroll
+    for (int kk = 0; kk < KM; ++kk)
+      if (kk < K - 1) {
+        run += lg[kk];
+        if (zk == K - 1 && tt < run) zk = kk;
+      }
+    m.z[(size_t)c * m.n + n] = zk;
+  }
+}
+
 // Sufficient statistics of z around the table's mu (the mu of the z step):
 // per (chain, block) and component, count, sum e and sum e e^T (packed lower)
 // with e = y - mu_z.  Each wave takes 64 points and forms
@@ -653,6 +815,33 @@ static ZKernel pick_z(int DM, int KM) {
   fail(AUGUR_ERR_UNSUPPORTED, "no z kernel for padded dim %d, components %d", DM, KM);
 }
 
+// The z step of every chain of `m`: the matrix-core form for a padded
+// dimension of 16 (AUGUR_GMM_Z_VALU=1 keeps the FP32 vector form), else
+// k_gmm_z.
+static void launch_gmm_z(const GmmDev& m, int chains, uint32_t seed, uint32_t sweep, uint32_t chain0,
+                         hipStream_t stream, const std::pair<hipEvent_t, hipEvent_t>* ev) {
+  const int KM = pad_comp(m.k);
+  ZKernel k = nullptr;
+  size_t smem = 0;
+  const char* env = getenv("AUGUR_GMM_Z_VALU");
+  if (m.DM == 16 && !(env && env[0] == '1')) {
+    if (KM == 8) k = k_gmm_z_mfma<8>;
+    else if (KM == 16) k = k_gmm_z_mfma<16>;
+    else if (KM == 32) k = k_gmm_z_mfma<32>;
+    else k = k_gmm_z_mfma<64>;
+    smem = gmm_zm_smem_bytes(KM);
+    allow_lds(k, smem);
+  } else {
+    k = pick_z(m.DM, KM);
+  }
+  if (ev)
+    hipExtLaunchKernelGGL(k, dim3(m.G, chains), dim3(kGmmThreads), smem, stream, ev->first, ev->second, 0, m,
+                          seed, sweep, chain0);
+  else
+    k<<<dim3(m.G, chains), kGmmThreads, smem, stream>>>(m, seed, sweep, chain0);
+  AUGUR_CHECK_LAUNCH();
+}
+
 static StatsKernel pick_stats(int DM, int KM) {
 #define AUGUR_S(D_, K_) if (DM == D_ && KM == K_) return k_gmm_stats_t<D_, K_>;
   AUGUR_S(4, 8) AUGUR_S(4, 16) AUGUR_S(4, 32) AUGUR_S(4, 64)
@@ -724,17 +913,11 @@ struct GmmSampler final : augur_sampler {
   }
 
   void run_sweeps(int nsw) override {
-    const ZKernel zk = pick_z(m.DM, pad_comp(m.k));
     for (int i = 0; i < nsw; ++i) {
       if (!stats_valid) refresh_stats();
       launch_conj();
       AUGUR_CHECK_LAUNCH();
-      if (auto* ev = prof_slot())
-        hipExtLaunchKernelGGL(zk, dim3(m.G, chains), dim3(kGmmThreads), 0, stream, ev->first, ev->second, 0,
-                              m, seed, sweep, 0u);
-      else
-        zk<<<dim3(m.G, chains), kGmmThreads, 0, stream>>>(m, seed, sweep, 0u);
-      AUGUR_CHECK_LAUNCH();
+      launch_gmm_z(m, chains, seed, sweep, 0u, stream, prof_slot());
       launch_stats();  // for the next sweep's conjugate draws
       ++sweep;
       record_traces();
@@ -860,9 +1043,7 @@ augur_status augur_op_gmm_zstep(int n, int d, int k, const float* y, const doubl
     s.mu64.upload(mu, (size_t)k * d, s.stream);
     s.cov64.upload(cov, (size_t)k * d * d, s.stream);
     s.refresh_tables();
-    const ZKernel zk = pick_z(s.m.DM, pad_comp(k));
-    zk<<<dim3(s.m.G, 1), kGmmThreads, 0, s.stream>>>(s.m, seed, sweep, chain);
-    AUGUR_CHECK_LAUNCH();
+    launch_gmm_z(s.m, 1, seed, sweep, chain, s.stream, nullptr);
     s.z.download(z, n, s.stream);
     AUGUR_HIP(hipStreamSynchronize(s.stream));
   });

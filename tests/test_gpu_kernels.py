@@ -212,8 +212,15 @@ def _gmm_params(rng, k, d):
     return pis, mu, cov
 
 
-@pytest.mark.parametrize("d,k", [(16, 8), (2, 3), (7, 33)])
-def test_gmm_zstep_vs_oracle(ops, oracle, d, k):
+@pytest.mark.parametrize("valu", [False, True])
+@pytest.mark.parametrize("d,k", [(16, 8), (2, 3), (7, 33), (12, 20), (16, 40)])
+def test_gmm_zstep_vs_oracle(ops, oracle, monkeypatch, d, k, valu):
+    """z from the GPU against the oracle's FP64 form with the same uniforms:
+    at least 99.9 % equal (the rest within rounding of a bin edge).  A padded
+    dimension of 16 (d = 12, 16) runs on the matrix cores unless
+    AUGUR_GMM_Z_VALU=1 selects the FP32 vector form; 4000 points leave a
+    partial 64-point group."""
+    monkeypatch.setenv("AUGUR_GMM_Z_VALU", "1" if valu else "0")
     rng = np.random.default_rng(d * k)
     pis, mu, cov = _gmm_params(rng, k, d)
     n = 4000
