@@ -52,6 +52,7 @@ struct GmmDev {
   double* cov64;   // C x k x d x d
   float* tab;      // C x k x TS   (TS = tri(DM) + DM + 1, DM = padded dim)
   int TS, DM;
+  int cus;         // compute units of the device
   float* part;     // C x G x k x S
   int32_t* z;      // C x n
 };
@@ -407,7 +408,7 @@ __device__ inline void gmm_split3(float x, __bf16& h, __bf16& m, __bf16& l) {
 }
 
 constexpr size_t gmm_zm_smem_bytes(int KM) {
-  return sizeof(float) * 17 * (size_t)KM + sizeof(uint4) * 2 * 64 * (size_t)KM;  // KM * 17 floats: 16-byte multiple
+  return sizeof(float) * 17 * (size_t)KM + sizeof(uint4) * 96 * (size_t)KM;  // KM * 17 floats: 16-byte multiple
 }
 
 // The z step of k_gmm_z for a padded dimension of 16, with the triangular
@@ -434,7 +435,7 @@ __global__ __launch_bounds__(kGmmThreads) void k_gmm_z_mfma(GmmDev m, uint32_t s
   // keeps almost every per-component address a constant offset from one base
   float* cvec = reinterpret_cast<float*>(smem);         // [KM][16] c_k
   float* cst = cvec + KM * 16;                          // [KM] log pi_k + log-det term
-  uint4* aop = reinterpret_cast<uint4*>(cst + KM);      // [KM][2][64] A operands per lane
+  uint4* aop = reinterpret_cast<uint4*>(cst + KM);      // [KM][3][32] parts of L^-1 per lane
   const int K = m.k, d = m.d, c = blockIdx.y, tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6, n16 = lane & 15, kg = lane >> 4;
   const float* gtab = m.tab + (size_t)c * K * TS;
@@ -447,23 +448,24 @@ __global__ __launch_bounds__(kGmmThreads) void k_gmm_z_mfma(GmmDev m, uint32_t s
       if (blockIdx.x == 0) m.pis64[(size_t)c * K + i] = g[i] / tot;
     }
   }
-  // A operand `op` of component k for lane l = (i, g): 8 BF16 at K index
-  // 8 g + e; index j < 16 is the high part of L^-1[i][j], index 16 + j the
-  // middle (op 0) or low (op 1) part
-  for (int e2 = tid; e2 < K * 128; e2 += kGmmThreads) {
-    const int k = e2 >> 7, op = (e2 >> 6) & 1, l = e2 & 63;
+  // A operands of component k: lane (i, g) supplies 8 BF16 at K index
+  // 8 g + e, the high part of L^-1[i][8 (g & 1) + e] for g < 2 and the middle
+  // (first operand) or low (second operand) part for g >= 2.  Stored once
+  // per part: aop[k][part][l'] for l' = i + 16 (g & 1) (32 entries per part).
+  for (int e2 = tid; e2 < K * 96; e2 += kGmmThreads) {
+    const int k = e2 / 96, part = (e2 >> 5) % 3, l = e2 & 31;
     const int i = l & 15, g = l >> 4;
     const float* t = gtab + k * TS;
     gmm_bf16x8 v;
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
-      const int j = (8 * g + e) & 15;
+      const int j = 8 * g + e;
       const float a = j <= i ? t[tri(i) + j] : 0.0f;
       __bf16 h, mm, lo;
       gmm_split3(a, h, mm, lo);
-      v[e] = g < 2 ? h : (op == 0 ? mm : lo);
+      v[e] = part == 0 ? h : (part == 1 ? mm : lo);
     }
-    aop[(k * 2 + op) * 64 + l] = __builtin_bit_cast(uint4, v);
+    aop[e2] = __builtin_bit_cast(uint4, v);
   }
   for (int e2 = tid; e2 < K * 16; e2 += kGmmThreads) {
     const int k = e2 >> 4, i = e2 & 15;
@@ -476,6 +478,8 @@ __global__ __launch_bounds__(kGmmThreads) void k_gmm_z_mfma(GmmDev m, uint32_t s
 
   const Stream zs{seed, chain0 + (uint32_t)c, sweep, TAG_GMM_Z};
   const int half = kg & 1;
+  const int la = (kg < 2 ? 0 : 32) + (lane & 31);       // part 0 for g < 2, part 1 for g >= 2
+  const int lb = (kg < 2 ? 0 : 64) + (lane & 31);       // part 0 for g < 2, part 2 for g >= 2
   const int groups = (m.n + 63) / 64;
   for (int grp = blockIdx.x * (kGmmThreads / 64) + wv; grp < groups; grp += gridDim.x * (kGmmThreads / 64)) {
     gmm_bf16x8 b1[4], b2[4], b3[4];
@@ -506,8 +510,8 @@ __global__ __launch_bounds__(kGmmThreads) void k_gmm_z_mfma(GmmDev m, uint32_t s
 #pragma unroll
     for (int kk = 0; kk < KM; ++kk) {
       if (kk < K) {
-        const gmm_bf16x8 a1 = __builtin_bit_cast(gmm_bf16x8, aop[(kk * 2) * 64 + lane]);
-        const gmm_bf16x8 a2 = __builtin_bit_cast(gmm_bf16x8, aop[(kk * 2 + 1) * 64 + lane]);
+        const gmm_bf16x8 a1 = __builtin_bit_cast(gmm_bf16x8, aop[kk * 96 + la]);
+        const gmm_bf16x8 a2 = __builtin_bit_cast(gmm_bf16x8, aop[kk * 96 + lb]);
         const float4 cv = *reinterpret_cast<const float4*>(cvec + kk * 16 + 4 * kg);
         // qp[j]: this lane's 4 rows of ||L^-1 x - c_k||^2 for point n of tile j
         float qp[4];
@@ -824,6 +828,7 @@ static void launch_gmm_z(const GmmDev& m, int chains, uint32_t seed, uint32_t sw
   ZKernel k = nullptr;
   size_t smem = 0;
   const char* env = getenv("AUGUR_GMM_Z_VALU");
+  int gx = m.G;
   if (m.DM == 16 && !(env && env[0] == '1')) {
     if (KM == 8) k = k_gmm_z_mfma<8>;
     else if (KM == 16) k = k_gmm_z_mfma<16>;
@@ -831,14 +836,18 @@ static void launch_gmm_z(const GmmDev& m, int chains, uint32_t seed, uint32_t sw
     else k = k_gmm_z_mfma<64>;
     smem = gmm_zm_smem_bytes(KM);
     allow_lds(k, smem);
+    // every block builds the component tables in LDS first: one resident
+    // round of blocks (three per CU over all chains) pays for it once
+    gx = std::max(1, std::min(m.G, (3 * m.cus + chains - 1) / chains));
+    if (const char* e = getenv("AUGUR_GMM_ZBLOCKS")) gx = std::max(1, atoi(e));
   } else {
     k = pick_z(m.DM, KM);
   }
   if (ev)
-    hipExtLaunchKernelGGL(k, dim3(m.G, chains), dim3(kGmmThreads), smem, stream, ev->first, ev->second, 0, m,
+    hipExtLaunchKernelGGL(k, dim3(gx, chains), dim3(kGmmThreads), smem, stream, ev->first, ev->second, 0, m,
                           seed, sweep, chain0);
   else
-    k<<<dim3(m.G, chains), kGmmThreads, smem, stream>>>(m, seed, sweep, chain0);
+    k<<<dim3(gx, chains), kGmmThreads, smem, stream>>>(m, seed, sweep, chain0);
   AUGUR_CHECK_LAUNCH();
 }
 
@@ -966,6 +975,12 @@ static void gmm_setup(GmmSampler* s, int n, int d, int k, int chains) {
   m.S = 1 + d + tri(d);
   const int want = (n + kGmmThreads - 1) / kGmmThreads;
   m.G = std::max(1, std::min(want, std::max(1, 1024 / chains)));
+  {
+    int dev = 0;
+    AUGUR_HIP(hipGetDevice(&dev));
+    AUGUR_HIP(hipDeviceGetAttribute(&m.cus, hipDeviceAttributeMultiprocessorCount, dev));
+    m.cus = std::max(m.cus, 1);
+  }
   s->chains = chains;
   s->y.alloc((size_t)n * d);
   s->alpha.alloc(k); s->mu0.alloc(d); s->cov0.alloc((size_t)d * d); s->scale0.alloc((size_t)d * d);
