@@ -639,20 +639,39 @@ __global__ __launch_bounds__(kGmmThreads) void k_gmm_stats_t(GmmDev m) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-#pragma unroll 4
-    for (int s = 0; s < 16; ++s) {
-      const int p = 4 * s + (lane >> 4);
-      const int zp = zsl[p];
-      const float* rp = slab + p * XS;
-      float a[CB];
+    // 32 points per product (V_MFMA_F32_16X16X32_BF16): lane (l % 16, g =
+    // l / 16) supplies points 32 h + 8 g .. + 7 of one component row (one-hot,
+    // exact in BF16) and of one entry column, the entry split into three
+    // BF16 parts whose products with the one-hot row are exact, so the sums
+    // are FP32 sums of the FP32 entries
 #pragma unroll
-      for (int cb = 0; cb < CB; ++cb) a[cb] = zp == 16 * cb + (lane & 15) ? 1.0f : 0.0f;
+    for (int h = 0; h < 2; ++h) {
+      const int p0 = 32 * h + 8 * (lane >> 4);
+      gmm_bf16x8 a[CB];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int zp = zsl[p0 + e];
+#pragma unroll
+        for (int cb = 0; cb < CB; ++cb) a[cb][e] = zp == 16 * cb + (lane & 15) ? (__bf16)1.0f : (__bf16)0.0f;
+      }
 #pragma unroll
       for (int eb = 0; eb < EB; ++eb) {
-        const float b = rp[ia[eb]] * rp[ib[eb]];
+        gmm_bf16x8 bh, bm, bl;
 #pragma unroll
-        for (int cb = 0; cb < CB; ++cb)
-          acc[cb][eb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[cb], b, acc[cb][eb], 0, 0, 0);
+        for (int e = 0; e < 8; ++e) {
+          const float* rp = slab + (p0 + e) * XS;
+          __bf16 x0, x1, x2;
+          gmm_split3(rp[ia[eb]] * rp[ib[eb]], x0, x1, x2);
+          bh[e] = x0;
+          bm[e] = x1;
+          bl[e] = x2;
+        }
+#pragma unroll
+        for (int cb = 0; cb < CB; ++cb) {
+          acc[cb][eb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[cb], bh, acc[cb][eb], 0, 0, 0);
+          acc[cb][eb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[cb], bm, acc[cb][eb], 0, 0, 0);
+          acc[cb][eb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[cb], bl, acc[cb][eb], 0, 0, 0);
+        }
       }
     }
     // the slab is rewritten only after every lane has read it
