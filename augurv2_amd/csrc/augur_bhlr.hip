@@ -14,7 +14,7 @@
 // chosen by the autotuner at creation); a block stages 64-row tiles in LDS
 // (BF16 -> FP32), computes t = x . theta for all chains of its dataset,
 // a = y - sigmoid(t), log sigmoid(+-t) and a x (FP32 arithmetic, FP64
-// accumulation), and writes its partials; a finish kernel (one wave per
+// accumulation), and writes its partials; a finish kernel (one thread per
 // value) sums each dataset's partials in block order (deterministic) and
 // applies the leapfrog update of its chains.  One sweep (v block, momentum draw, L
 // passes, Metropolis step, sweep counter) can be captured in a hipGraph and
@@ -271,8 +271,10 @@ __device__ inline double theta_step(const BArgs& A, uint32_t gc) {
 // log-density): sums the dataset's block partials in block order (fixed
 // butterfly), then applies the leapfrog update of that coordinate.
 __global__ __launch_bounds__(256) void k_bhlr_finish(BArgs A) {
-  const int lane = threadIdx.x & 63;
-  const int v = blockIdx.x * 4 + (threadIdx.x >> 6);
+  // one thread per value: its dataset's block partials summed in block order
+  // (a dataset has a few dozen blocks at most, so a wave per value would
+  // spend its time being launched)
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= A.nval) return;
   const int di = A.val_ds[v];
   const BDs ds = A.ds[di];
@@ -280,9 +282,7 @@ __global__ __launch_bounds__(256) void k_bhlr_finish(BArgs A) {
   const int lv = v - ds.voff;
   const double* pv = A.part + ds.poff + (size_t)lv * ds.nblk;
   double s = 0.0;
-  for (int b = lane; b < ds.nblk; b += 64) s += pv[b];
-  s = wave_sum(s);
-  if (lane != 0) return;
+  for (int b = 0; b < ds.nblk; ++b) s += pv[b];
   const int c = lv / VS, j = lv % VS;
   const size_t ci = (size_t)c * A.n_ds + di;
   if (j == D) return;  // no intercept in the batched model
@@ -1006,7 +1006,7 @@ struct BhlrSampler final : augur_sampler {
     pass<<<nblk_total, kBThreads, pass_smem, stream>>>(A);
     AUGUR_CHECK_LAUNCH();
     prof_end();
-    k_bhlr_finish<<<(A.nval + 3) / 4, 256, 0, stream>>>(A);
+    k_bhlr_finish<<<(A.nval + 255) / 256, 256, 0, stream>>>(A);
     AUGUR_CHECK_LAUNCH();
   }
 
