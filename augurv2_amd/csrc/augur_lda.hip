@@ -5,7 +5,10 @@
 //   k_lda_dirichlet : theta_d ~ Dir(alpha + cnt_theta[d]), phi_k ~ Dir(beta +
 //                     cnt_phi[k]) (Core/CgConj.hs:171-183, 278-306 ->
 //                     cbits/augur_dist.c:526-538); each count is read once and
-//                     cleared by the thread that reads it.
+//                     cleared by the thread that reads it.  With K <= 16 the
+//                     phi rows are left unnormalised (raw gammas and block
+//                     sums) and the z kernel divides, as the normalising
+//                     block would, so no block waits on another here.
 //   k_lda_z         : z[d, n] ~ Cat(theta[d][k] * phi[k][w]) (Core/CgDisc.hs:
 //                     122-144 -> cbits/augur_dist.c:798-813), fused with the
 //                     count accumulation of the new z for the next sweep.
@@ -90,6 +93,7 @@ struct LdaDev {
   float* phit32;       // C x V x K
   double* phi_part;    // C x K x ceil(V / 256) partial row sums
   uint32_t* phi_done;  // C x K blocks of the row finished (0 between launches)
+  double* gam64;       // C x K x V raw phi gammas (folded normalisation, K <= kCatRegs)
 };
 
 __device__ inline double block_sum(double v, double* red) {
@@ -115,6 +119,9 @@ __host__ __device__ inline int phi_blocks_per_row(int V) { return (V + kDirThrea
 //     reset by that block) sums the partials in order j = 0 .. pb - 1 and
 //     normalises the whole row into phi64 and the word-major FP32 table.
 //   blocks [K * pb, K * pb + ceil(D / 4)): theta rows, one wave per document.
+// kFold: the phi blocks leave the raw gammas in gam64 and their partial
+// sums, and the z kernel normalises (no handshake between blocks here).
+template <bool kFold>
 __global__ __launch_bounds__(kDirThreads) void k_lda_dirichlet(LdaDev m, uint32_t seed, uint32_t sweep) {
   __shared__ double red[kDirThreads / 64];
   __shared__ int last;
@@ -126,7 +133,7 @@ __global__ __launch_bounds__(kDirThreads) void k_lda_dirichlet(LdaDev m, uint32_
     const Stream s{seed, (uint32_t)c, sweep, TAG_LDA_PHI};
     const size_t r = (size_t)c * m.K + k;
     int32_t* cnt = m.cnt_phi + r * m.V;
-    double* row = m.phi64 + r * m.V;
+    double* row = (kFold ? m.gam64 : m.phi64) + r * m.V;
     const int v = j * kDirThreads + threadIdx.x;
     double g = 0.0;
     if (v < m.V) {
@@ -135,6 +142,10 @@ __global__ __launch_bounds__(kDirThreads) void k_lda_dirichlet(LdaDev m, uint32_
       row[v] = g;
     }
     const double part = block_sum(g, red);
+    if (kFold) {
+      if (threadIdx.x == 0) m.phi_part[r * pb + j] = part;
+      return;
+    }
     if (threadIdx.x == 0) {
       m.phi_part[r * pb + j] = part;
       __threadfence();
@@ -191,11 +202,21 @@ static dim3 dirichlet_grid(const LdaDev& m, int chains) {
 // tables are exact whatever the order.  The count tables start the sweep at
 // zero (cleared by the Dirichlet kernel that read them).  The Philox key of
 // table row c is chain0 + c.
-template <bool kLdsTable>
+//
+// kFold (K <= kCatRegs): phi arrives as the raw gammas of k_lda_dirichlet<true>
+// and the partial sums of its row blocks.  Every block sums a row's partials
+// in block order (as the normalising block of the unfolded form does) and
+// forms a token's K probabilities as (float)(gamma / sum), the unfolded
+// form's FP64 division, so the draws are the same; the blocks also write
+// their slice of the normalised phi (FP64 K x V and FP32 V x K) for the
+// caller.  This saves the Dirichlet kernel's cross-block handshake.
+template <bool kLdsTable, bool kFold>
 __global__ __launch_bounds__(kLdsTable ? kZLdsThreads : kZThreads) void k_lda_z(LdaDev m, uint32_t seed,
                                                                               uint32_t sweep, uint32_t chain0) {
   extern __shared__ __align__(16) unsigned char zsm[];
   int32_t* tab = reinterpret_cast<int32_t*>(zsm);  // K x V, kLdsTable only
+  __shared__ double rsum[kFold ? kCatRegs : 1];    // phi row sums, kFold only
+
   const int c = blockIdx.y;
   const int lane = threadIdx.x & 63;
   const int waves = blockDim.x >> 6;
@@ -204,6 +225,26 @@ __global__ __launch_bounds__(kLdsTable ? kZLdsThreads : kZThreads) void k_lda_z(
   int32_t* cphi = m.cnt_phi + (size_t)c * m.K * m.V;
   int32_t* zc = m.z + (size_t)c * m.T;
   const int cells = m.K * m.V;
+  const double* gam = m.gam64 + (size_t)c * m.K * m.V;
+  if (kFold) {
+    const int pb = phi_blocks_per_row(m.V);
+    if (threadIdx.x < m.K) {
+      const double* pp = m.phi_part + ((size_t)c * m.K + threadIdx.x) * pb;
+      double tot = 0.0;
+      for (int i = 0; i < pb; ++i) tot += pp[i];
+      rsum[threadIdx.x] = tot;
+    }
+    __syncthreads();
+    // this block's slice of the normalised phi for the caller
+    float* pt = m.phit32 + (size_t)c * m.V * m.K;
+    double* p64 = m.phi64 + (size_t)c * m.K * m.V;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < cells; i += gridDim.x * blockDim.x) {
+      const int k = i / m.V, v = i - k * m.V;
+      const double p = gam[i] / rsum[k];
+      p64[i] = p;
+      pt[(size_t)v * m.K + k] = (float)p;
+    }
+  }
   if (kLdsTable) {
     for (int i = threadIdx.x; i < cells; i += blockDim.x) tab[i] = 0;
     __syncthreads();
@@ -220,8 +261,15 @@ __global__ __launch_bounds__(kLdsTable ? kZLdsThreads : kZThreads) void k_lda_z(
     if (valid) {
       const int wd = m.w[t];
       const float u = u01f(s.draw((uint32_t)t, 0).x);
-      znew = m.K <= kCatRegs ? draw_cat_f32_regs(th, phit + (size_t)wd * m.K, m.K, u)
-                             : draw_cat_f32(th, phit + (size_t)wd * m.K, m.K, u);
+      if (kFold) {
+        float pw[kCatRegs];
+#pragma unroll
+        for (int k = 0; k < kCatRegs; ++k) pw[k] = k < m.K ? (float)(gam[(size_t)k * m.V + wd] / rsum[k]) : 0.0f;
+        znew = draw_cat_f32_regs(th, pw, m.K, u);
+      } else {
+        znew = m.K <= kCatRegs ? draw_cat_f32_regs(th, phit + (size_t)wd * m.K, m.K, u)
+                               : draw_cat_f32(th, phit + (size_t)wd * m.K, m.K, u);
+      }
       zc[t] = znew;
       if (kLdsTable) atomicAdd(tab + (size_t)znew * m.V + wd, 1);
       else atomicAdd(cphi + (size_t)znew * m.V + wd, 1);
@@ -271,25 +319,26 @@ static int device_cus() {
 }
 
 static void launch_lda_z(const LdaDev& m, int chains, uint32_t seed, uint32_t sweep, uint32_t chain0, int cus,
-                         hipStream_t stream, const std::pair<hipEvent_t, hipEvent_t>* ev) {
+                         hipStream_t stream, const std::pair<hipEvent_t, hipEvent_t>* ev, bool fold = false) {
   if ((size_t)m.K * m.V <= (size_t)kZLdsCells) {
     const int per = kZLdsThreads / 64;
     const dim3 g(std::max(1, std::min((m.n_chunks + per - 1) / per, cus)), chains);
     const size_t smem = (size_t)m.K * m.V * sizeof(int32_t);
-    allow_lds(k_lda_z<true>, smem);
+    auto k = fold ? k_lda_z<true, true> : k_lda_z<true, false>;
+    allow_lds(k, smem);
     if (ev)
-      hipExtLaunchKernelGGL(k_lda_z<true>, g, dim3(kZLdsThreads), smem, stream, ev->first, ev->second, 0, m,
-                            seed, sweep, chain0);
+      hipExtLaunchKernelGGL(k, g, dim3(kZLdsThreads), smem, stream, ev->first, ev->second, 0, m, seed, sweep,
+                            chain0);
     else
-      k_lda_z<true><<<g, kZLdsThreads, smem, stream>>>(m, seed, sweep, chain0);
+      k<<<g, kZLdsThreads, smem, stream>>>(m, seed, sweep, chain0);
   } else {
     const int per = kZThreads / 64;
     const dim3 g(std::max(1, std::min((m.n_chunks + per - 1) / per, 4096)), chains);
+    auto k = fold ? k_lda_z<false, true> : k_lda_z<false, false>;
     if (ev)
-      hipExtLaunchKernelGGL(k_lda_z<false>, g, dim3(kZThreads), 0, stream, ev->first, ev->second, 0, m, seed,
-                            sweep, chain0);
+      hipExtLaunchKernelGGL(k, g, dim3(kZThreads), 0, stream, ev->first, ev->second, 0, m, seed, sweep, chain0);
     else
-      k_lda_z<false><<<g, kZThreads, 0, stream>>>(m, seed, sweep, chain0);
+      k<<<g, kZThreads, 0, stream>>>(m, seed, sweep, chain0);
   }
   AUGUR_CHECK_LAUNCH();
 }
@@ -345,7 +394,7 @@ struct LdaSampler final : augur_sampler {
   LdaDev m{};
   DevBuf<int32_t> doc_off, chunks, w, z, cnt_theta, cnt_phi;
   DevBuf<uint32_t> phi_done;
-  DevBuf<double> alpha, beta, theta64, phi64, phi_part, ll;
+  DevBuf<double> alpha, beta, theta64, phi64, phi_part, gam64, ll;
   DevBuf<float> theta32, phit32;
   int cus = 1;  // compute units of the device (grid of the z step)
 
@@ -386,10 +435,12 @@ struct LdaSampler final : augur_sampler {
 
   void run_sweeps(int n) override {
     const dim3 gd = dirichlet_grid(m, chains);
+    const bool fold = m.gam64 != nullptr;  // K <= kCatRegs: normalisation folded into the z step
     for (int i = 0; i < n; ++i) {
-      k_lda_dirichlet<<<gd, kDirThreads, 0, stream>>>(m, seed, sweep);
+      if (fold) k_lda_dirichlet<true><<<gd, kDirThreads, 0, stream>>>(m, seed, sweep);
+      else k_lda_dirichlet<false><<<gd, kDirThreads, 0, stream>>>(m, seed, sweep);
       AUGUR_CHECK_LAUNCH();
-      launch_lda_z(m, chains, seed, sweep, 0u, cus, stream, prof_slot());
+      launch_lda_z(m, chains, seed, sweep, 0u, cus, stream, prof_slot(), fold);
       ++sweep;
       record_traces();
     }
@@ -493,6 +544,7 @@ augur_status augur_lda_create(const augur_lda_desc* d, augur_sampler** out) {
       s->theta64.alloc((size_t)C * D * K);
       s->theta32.alloc((size_t)C * D * K);
       s->phi64.alloc((size_t)C * K * V);
+      if (K <= kCatRegs) s->gam64.alloc((size_t)C * K * V);
       s->phit32.alloc((size_t)C * V * K);
       s->ll.alloc(C);
       s->doc_off.upload(d->doc_off, D + 1, s->stream);
@@ -512,6 +564,7 @@ augur_status augur_lda_create(const augur_lda_desc* d, augur_sampler** out) {
       m.z = s->z.p; m.cnt_theta = s->cnt_theta.p; m.cnt_phi = s->cnt_phi.p;
       m.theta64 = s->theta64.p; m.theta32 = s->theta32.p; m.phi64 = s->phi64.p;
       m.phit32 = s->phit32.p; m.phi_part = s->phi_part.p; m.phi_done = s->phi_done.p;
+      m.gam64 = s->gam64.p;  // null unless K <= kCatRegs
       s->recount();
       AUGUR_HIP(hipStreamSynchronize(s->stream));
     } catch (...) {

@@ -47,15 +47,18 @@ def test_lda_two_sweeps_match_oracle(rt, oracle):
     s.close()
 
 
-def test_lda_ragged_documents_match_oracle(rt, oracle):
+@pytest.mark.parametrize("K", [12, 20])
+def test_lda_ragged_documents_match_oracle(rt, oracle, K):
     """Documents of 0, 1, 63, 64, 65, 128, 129 and 300 tokens (the z step
     takes them in 64-token chunks) and V = 512, two full 256-word blocks of
     each phi row: three sweeps of two chains against the oracle, each sweep
     continuing from the GPU's assignment, so the count tables the chunks add
-    up are checked through the next sweep's theta and phi."""
+    up are checked through the next sweep's theta and phi.  K = 12 takes the
+    phi normalisation inside the z step, K = 20 the Dirichlet kernel's
+    last-block normalisation."""
     rng = np.random.default_rng(21)
     lens = np.array([0, 1, 63, 64, 65, 128, 129, 300, 0, 5, 64, 0])
-    K, D, V, C = 12, len(lens), 512, 2
+    D, V, C = len(lens), 512, 2
     doc_off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
     w = rng.integers(0, V, int(doc_off[-1])).astype(np.int32)
     alpha, beta = np.full(K, 0.3), np.full(V, 0.02)
