@@ -93,7 +93,7 @@ struct LdaDev {
   float* phit32;       // C x V x K
   double* phi_part;    // C x K x ceil(V / 256) partial row sums
   uint32_t* phi_done;  // C x K blocks of the row finished (0 between launches)
-  double* gam64;       // C x K x V raw phi gammas (folded normalisation, K <= kCatRegs)
+  double* gam64;       // C x V x K raw phi gammas, word-major (folded normalisation, K <= kCatRegs)
 };
 
 __device__ inline double block_sum(double v, double* red) {
@@ -133,13 +133,14 @@ __global__ __launch_bounds__(kDirThreads) void k_lda_dirichlet(LdaDev m, uint32_
     const Stream s{seed, (uint32_t)c, sweep, TAG_LDA_PHI};
     const size_t r = (size_t)c * m.K + k;
     int32_t* cnt = m.cnt_phi + r * m.V;
-    double* row = (kFold ? m.gam64 : m.phi64) + r * m.V;
+    double* row = m.phi64 + r * m.V;
     const int v = j * kDirThreads + threadIdx.x;
     double g = 0.0;
     if (v < m.V) {
       g = gamma_at(s, (uint32_t)(k * m.V + v), m.beta[v] + (double)cnt[v]);
       cnt[v] = 0;
-      row[v] = g;
+      if (kFold) m.gam64[((size_t)c * m.V + v) * m.K + k] = g;  // a word's K gammas contiguous
+      else row[v] = g;
     }
     const double part = block_sum(g, red);
     if (kFold) {
@@ -207,9 +208,9 @@ static dim3 dirichlet_grid(const LdaDev& m, int chains) {
 // and the partial sums of its row blocks.  Every block sums a row's partials
 // in block order (as the normalising block of the unfolded form does) and
 // forms a token's K probabilities as (float)(gamma / sum), the unfolded
-// form's FP64 division, so the draws are the same; the blocks also write
-// their slice of the normalised phi (FP64 K x V and FP32 V x K) for the
-// caller.  This saves the Dirichlet kernel's cross-block handshake.
+// form's FP64 division, so the draws are the same.  The normalised phi the
+// caller reads is written by k_lda_phi_norm after the sweeps.  This saves
+// the Dirichlet kernel's cross-block handshake.
 template <bool kLdsTable, bool kFold>
 __global__ __launch_bounds__(kLdsTable ? kZLdsThreads : kZThreads) void k_lda_z(LdaDev m, uint32_t seed,
                                                                               uint32_t sweep, uint32_t chain0) {
@@ -225,7 +226,7 @@ __global__ __launch_bounds__(kLdsTable ? kZLdsThreads : kZThreads) void k_lda_z(
   int32_t* cphi = m.cnt_phi + (size_t)c * m.K * m.V;
   int32_t* zc = m.z + (size_t)c * m.T;
   const int cells = m.K * m.V;
-  const double* gam = m.gam64 + (size_t)c * m.K * m.V;
+  const double* gam = m.gam64 + (size_t)c * m.V * m.K;
   if (kFold) {
     const int pb = phi_blocks_per_row(m.V);
     if (threadIdx.x < m.K) {
@@ -235,15 +236,6 @@ __global__ __launch_bounds__(kLdsTable ? kZLdsThreads : kZThreads) void k_lda_z(
       rsum[threadIdx.x] = tot;
     }
     __syncthreads();
-    // this block's slice of the normalised phi for the caller
-    float* pt = m.phit32 + (size_t)c * m.V * m.K;
-    double* p64 = m.phi64 + (size_t)c * m.K * m.V;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < cells; i += gridDim.x * blockDim.x) {
-      const int k = i / m.V, v = i - k * m.V;
-      const double p = gam[i] / rsum[k];
-      p64[i] = p;
-      pt[(size_t)v * m.K + k] = (float)p;
-    }
   }
   if (kLdsTable) {
     for (int i = threadIdx.x; i < cells; i += blockDim.x) tab[i] = 0;
@@ -264,7 +256,7 @@ __global__ __launch_bounds__(kLdsTable ? kZLdsThreads : kZThreads) void k_lda_z(
       if (kFold) {
         float pw[kCatRegs];
 #pragma unroll
-        for (int k = 0; k < kCatRegs; ++k) pw[k] = k < m.K ? (float)(gam[(size_t)k * m.V + wd] / rsum[k]) : 0.0f;
+        for (int k = 0; k < kCatRegs; ++k) pw[k] = k < m.K ? (float)(gam[(size_t)wd * m.K + k] / rsum[k]) : 0.0f;
         znew = draw_cat_f32_regs(th, pw, m.K, u);
       } else {
         znew = m.K <= kCatRegs ? draw_cat_f32_regs(th, phit + (size_t)wd * m.K, m.K, u)
@@ -291,6 +283,25 @@ __global__ __launch_bounds__(kLdsTable ? kZLdsThreads : kZThreads) void k_lda_z(
       const int n = tab[i];
       if (n) atomicAdd(cphi + i, n);
     }
+  }
+}
+
+// The normalised phi of the folded form (FP64 K x V for the caller, FP32
+// V x K): one block per (row k, chain), the row's partials summed in block
+// order and each gamma divided as in the z step.
+__global__ __launch_bounds__(256) void k_lda_phi_norm(LdaDev m) {
+  const int k = blockIdx.x, c = blockIdx.y;
+  const int pb = phi_blocks_per_row(m.V);
+  const size_t r = (size_t)c * m.K + k;
+  double tot = 0.0;
+  for (int i = 0; i < pb; ++i) tot += m.phi_part[r * pb + i];
+  const double* g = m.gam64 + (size_t)c * m.V * m.K;
+  double* p64 = m.phi64 + r * m.V;
+  float* pt = m.phit32 + (size_t)c * m.V * m.K;
+  for (int v = threadIdx.x; v < m.V; v += 256) {
+    const double p = g[(size_t)v * m.K + k] / tot;
+    p64[v] = p;
+    pt[(size_t)v * m.K + k] = (float)p;
   }
 }
 
@@ -442,6 +453,12 @@ struct LdaSampler final : augur_sampler {
       AUGUR_CHECK_LAUNCH();
       launch_lda_z(m, chains, seed, sweep, 0u, cus, stream, prof_slot(), fold);
       ++sweep;
+      // the folded form writes the caller's phi when it is read: after every
+      // sweep while traces record, else after the last one
+      if (fold && (!traces.empty() || i == n - 1)) {
+        k_lda_phi_norm<<<dim3(m.K, chains), 256, 0, stream>>>(m);
+        AUGUR_CHECK_LAUNCH();
+      }
       record_traces();
     }
   }
