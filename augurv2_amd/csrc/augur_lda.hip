@@ -227,6 +227,16 @@ __global__ __launch_bounds__(kLdsTable ? kZLdsThreads : kZThreads) void k_lda_z(
   int32_t* zc = m.z + (size_t)c * m.T;
   const int cells = m.K * m.V;
   const double* gam = m.gam64 + (size_t)c * m.V * m.K;
+  // the wave's first chunk (descriptor and word ids) is requested before the
+  // barriers below, so that memory round overlaps them
+  const int qstep = gridDim.x * waves;
+  int q = blockIdx.x * waves + (threadIdx.x >> 6);
+  int4 ch = make_int4(0, 0, 0, 0);
+  int wd = 0;
+  if (q < m.n_chunks) {
+    ch = *reinterpret_cast<const int4*>(m.chunks + 4 * (size_t)__builtin_amdgcn_readfirstlane(q));
+    if (ch.y + lane < ch.z) wd = m.w[ch.y + lane];
+  }
   if (kFold) {
     const int pb = phi_blocks_per_row(m.V);
     if (threadIdx.x < m.K) {
@@ -241,17 +251,14 @@ __global__ __launch_bounds__(kLdsTable ? kZLdsThreads : kZThreads) void k_lda_z(
     for (int i = threadIdx.x; i < cells; i += blockDim.x) tab[i] = 0;
     __syncthreads();
   }
-  for (int q = blockIdx.x * waves + (threadIdx.x >> 6); q < m.n_chunks; q += gridDim.x * waves) {
-    const int qu = __builtin_amdgcn_readfirstlane(q);
-    const int4 ch = *reinterpret_cast<const int4*>(m.chunks + 4 * (size_t)qu);
-    const int du = ch.x, t0 = ch.y, t1 = ch.z;
+  for (; q < m.n_chunks; q += qstep) {
+    const int du = __builtin_amdgcn_readfirstlane(ch.x), t0 = ch.y, t1 = ch.z;
     const float* th = m.theta32 + ((size_t)c * m.D + du) * m.K;
     int32_t* ct = m.cnt_theta + ((size_t)c * m.D + du) * m.K;
     const int t = t0 + lane;
     const bool valid = t < t1;
     int znew = -1;
     if (valid) {
-      const int wd = m.w[t];
       const float u = u01f(s.draw((uint32_t)t, 0).x);
       if (kFold) {
         float pw[kCatRegs];
@@ -275,6 +282,10 @@ __global__ __launch_bounds__(kLdsTable ? kZLdsThreads : kZThreads) void k_lda_z(
       if (my_count) atomicAdd(ct + lane, my_count);
     } else if (valid) {
       atomicAdd(ct + znew, 1);
+    }
+    if (q + qstep < m.n_chunks) {  // the next chunk of this wave
+      ch = *reinterpret_cast<const int4*>(m.chunks + 4 * (size_t)__builtin_amdgcn_readfirstlane(q + qstep));
+      wd = ch.y + lane < ch.z ? m.w[ch.y + lane] : 0;
     }
   }
   if (kLdsTable) {
