@@ -64,7 +64,9 @@ def run_hlr(steps, warmup, seed=2024):
     log(f"hlr: warmup {warmup} sweeps")
     s.step(warmup)
     s.trace("state", steps)
-    s.profile(10)
+    # kernel-time events on every pass launch of one sweep in fifty: about
+    # 3,700 timed passes in the default run, few sweeps perturbed
+    s.profile(50)
     log(f"hlr: timed {steps} sweeps")
     t0 = time.perf_counter()
     s.step(steps)  # returns after the stream is synchronised
