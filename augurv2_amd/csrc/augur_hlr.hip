@@ -744,7 +744,7 @@ struct HmcArgs {
 
 
 // One wave per chain: momentum draw, half step, first position step.
-__global__ __launch_bounds__(64) void k_hmc_begin(HmcArgs A) {
+__device__ inline void hmc_begin_body(const HmcArgs& A) {
   const int c = blockIdx.x, lane = threadIdx.x;
   const int d = A.d, ds = d + 1, P = d + 2;
   const Stream mom{A.seed, (uint32_t)c, A.sweep, A.tag + 0};
@@ -773,6 +773,8 @@ __global__ __launch_bounds__(64) void k_hmc_begin(HmcArgs A) {
   kin = wave_sum(kin);
   if (lane == 0) A.k0[c] = 0.5 * kin;
 }
+
+__global__ __launch_bounds__(64) void k_hmc_begin(HmcArgs A) { hmc_begin_body(A); }
 
 // One wave per chain: energies, FP64 Metropolis test (cbits/augur_mcmclib.h:
 // 70-83, 410-423), accept or reject.
@@ -827,7 +829,7 @@ __device__ inline double v_prior_ll(const VBlockArgs& A, double v) {
 
 // HMC or Metropolis-within-Gibbs on the scalar variance v (the full
 // conditional involves only theta and b): one wave per chain.
-__global__ __launch_bounds__(64) void k_hmc_vblock(VBlockArgs A) {
+__device__ inline void hmc_vblock_body(const VBlockArgs& A) {
   const int c = blockIdx.x, lane = threadIdx.x;
   const int d = A.d, P = d + 2;
   double* st = A.state + (size_t)c * P;
@@ -843,6 +845,17 @@ __global__ __launch_bounds__(64) void k_hmc_vblock(VBlockArgs A) {
     A.stats[2 * c] += acc ? 1.0 : 0.0;
     A.stats[2 * c + 1] += 1.0;
   }
+}
+
+__global__ __launch_bounds__(64) void k_hmc_vblock(VBlockArgs A) { hmc_vblock_body(A); }
+
+// A v block followed by a data block in one launch: the v block's update of
+// v, a barrier that publishes it to the block's lanes, then the data block's
+// momentum draw and first steps, which read it.  Saves a launch per sweep.
+__global__ __launch_bounds__(64) void k_hmc_vblock_begin(VBlockArgs V, HmcArgs A) {
+  hmc_vblock_body(V);
+  __syncthreads();
+  hmc_begin_body(A);
 }
 
 __global__ void k_state_to_beta32(const double* __restrict__ state, float* __restrict__ beta32, int d,
@@ -1055,9 +1068,15 @@ struct HlrSampler final : augur_sampler {
     }
   }
 
-  void data_block(int bi, const augur_hmc_block& blk) {
+  // fuse_v: the v block scheduled just before, run in the same launch as
+  // this block's first kernel
+  void data_block(int bi, const augur_hmc_block& blk, const VBlockArgs* fuse_v = nullptr) {
     const bool surr_blk = surr_phase != SURR_OFF && bi == scfg.block_index;
     if (surr_blk && surr_phase == SURR_ACTIVE) {
+      if (fuse_v) {
+        k_hmc_vblock<<<chains, 64, 0, stream>>>(*fuse_v);
+        AUGUR_CHECK_LAUNCH();
+      }
       surrogate_iteration(bi, blk);
       return;
     }
@@ -1078,7 +1097,8 @@ struct HlrSampler final : augur_sampler {
                  &H.hs.step[c], &H.hs.steps[c]);
       lmax = std::max(lmax, H.hs.steps[c]);
     }
-    k_hmc_begin<<<chains, 64, 0, stream>>>(H);
+    if (fuse_v) k_hmc_vblock_begin<<<chains, 64, 0, stream>>>(*fuse_v, H);
+    else k_hmc_begin<<<chains, 64, 0, stream>>>(H);
     AUGUR_CHECK_LAUNCH();
     PassArgs A = pass_args(MODE_LEAPFROG, 0);
     A.blk_theta = H.blk_theta;
@@ -1112,12 +1132,21 @@ struct HlrSampler final : augur_sampler {
   }
 
   void run_sweeps(int nsw) override {
+    auto is_v = [](const augur_hmc_block& b) {
+      return b.block == AUGUR_HMC_BLOCK_V || b.block == AUGUR_MWG_BLOCK_V;
+    };
     for (int i = 0; i < nsw; ++i) {
       for (int bi = 0; bi < (int)sched.size(); ++bi) {
         const augur_hmc_block& blk = sched[bi];
-        if (blk.block == AUGUR_HMC_BLOCK_V || blk.block == AUGUR_MWG_BLOCK_V) {
-          k_hmc_vblock<<<chains, 64, 0, stream>>>(vargs(bi, blk));
-          AUGUR_CHECK_LAUNCH();
+        if (is_v(blk)) {
+          const VBlockArgs va = vargs(bi, blk);
+          if (bi + 1 < (int)sched.size() && !is_v(sched[bi + 1])) {
+            data_block(bi + 1, sched[bi + 1], &va);  // v block fused into the next block's first launch
+            ++bi;
+          } else {
+            k_hmc_vblock<<<chains, 64, 0, stream>>>(va);
+            AUGUR_CHECK_LAUNCH();
+          }
         } else {
           data_block(bi, blk);
         }
