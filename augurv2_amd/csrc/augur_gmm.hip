@@ -15,11 +15,15 @@
 //   k_gmm_z     point-parallel categorical draw (Core/CgDisc.hs:122-144) in
 //               log space with max subtraction: logit_k = log pis_k
 //               - 0.5 log det cov_k - 0.5 |L_k^-1 (y - mu_k)|^2, tables read
-//               through the scalar cache.
+//               through the scalar cache (FP32 vector form).
+//   k_gmm_z_mfma  the same draw for a padded dimension of 16 with
+//               L_k^-1 y on the matrix cores (BF16-split 16x16x32 products)
+//               minus c_k = L_k^-1 mu_k.
 //   k_gmm_stats_t  the statistics of the new z for the next sweep, centred
 //               at the current mu_k: count, sum (y - mu_k) and
-//               sum (y - mu_k)(y - mu_k)^T, as a one-hot product per wave of
-//               64 points on the FP32 matrix core (no atomics: FP32 per block
+//               sum (y - mu_k)(y - mu_k)^T, as one-hot products per wave of
+//               64 points on the matrix cores (one-hot rows exact in BF16,
+//               entries split in three BF16 parts; no atomics: FP32 per block
 //               in a fixed order).  The next sweep's scatter around the new
 //               mu is S + s d^T + d s^T + n d d^T with d = mu_old - mu_new.
 //

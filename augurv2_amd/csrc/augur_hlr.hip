@@ -11,8 +11,12 @@
 // 343-424.  Here the whole HMC iteration stays on the device:
 //
 //   k_hmc_begin   momentum draw, first half step, first position step
-//   k_hlr_pass    one data pass per leapfrog step for all chains at once:
-//                 X tile (64 rows) staged in LDS and shared by every chain,
+//                 (k_hmc_vblock_begin: after the v block's update in the
+//                 same launch when a v block precedes the data block)
+//   k_hlr_pass    one data pass per leapfrog step for all chains at once
+//                 (k_hlr_pass_r: the register form, X rows in registers and
+//                 x.theta completed by DPP; k_hlr_pass_t / k_hlr_pass: the
+//                 LDS-staged forms): X tile staged in LDS and shared by every chain,
 //                 t = x.theta + b, a = y - sigmoid(t), ll += log sigmoid(+-t),
 //                 grad += a x; FP32 arithmetic, FP64 accumulation across
 //                 tiles; each block writes one row of partials

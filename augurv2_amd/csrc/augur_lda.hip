@@ -8,7 +8,9 @@
 //                     cleared by the thread that reads it.  With K <= 16 the
 //                     phi rows are left unnormalised (raw gammas and block
 //                     sums) and the z kernel divides, as the normalising
-//                     block would, so no block waits on another here.
+//                     block would, so no block waits on another here;
+//                     k_lda_phi_norm writes the normalised phi the caller
+//                     reads once per step (every sweep while tracing).
 //   k_lda_z         : z[d, n] ~ Cat(theta[d][k] * phi[k][w]) (Core/CgDisc.hs:
 //                     122-144 -> cbits/augur_dist.c:798-813), fused with the
 //                     count accumulation of the new z for the next sweep.
