@@ -79,6 +79,31 @@ def test_lda_ragged_documents_match_oracle(rt, oracle, K):
     s.close()
 
 
+def test_lda_traced_phi_is_normalised_every_sweep(rt, oracle):
+    """With K <= 16 the normalised phi is written after the sweeps of a step,
+    and after every sweep while a trace records.  The first traced phi equals
+    the oracle's sweep from the same z; every traced phi is a set of
+    distributions and differs from the sweep before (no stale copy); the
+    last equals what get returns."""
+    rng = np.random.default_rng(8)
+    K, D, V = 9, 30, 200
+    doc_off, w = _corpus(rng, D, V, 70)
+    alpha, beta = np.full(K, 0.2), np.full(V, 0.03)
+    s = rt.LdaSampler(K, doc_off, w, alpha, beta, chains=1, seed=4)
+    z = rng.integers(0, K, len(w)).astype(np.int32)
+    s.set("z", z)
+    s.trace("phi", 3)
+    s.step(3)
+    tr = s.read_trace("phi").reshape(3, K, V)
+    theta, phi = np.zeros((D, K)), np.zeros((K, V))
+    oracle.lda_sweep(doc_off, w, alpha, beta, 4, 0, 0, theta, phi, z.copy())
+    np.testing.assert_allclose(tr[0], phi, rtol=1e-9, atol=1e-300)
+    np.testing.assert_allclose(tr.sum(axis=2), 1.0, rtol=1e-12)
+    assert not np.array_equal(tr[0], tr[1]) and not np.array_equal(tr[1], tr[2])
+    np.testing.assert_array_equal(s.get("phi").reshape(K, V), tr[2])
+    s.close()
+
+
 def test_lda_log_density_is_the_model_density(rt, oracle):
     """augur_get_curr_ll for lda.rv is the reference's ll_mod: Dirichlet
     terms of theta and phi without normaliser (the oracle's dirichlet_ll,
@@ -157,6 +182,30 @@ def test_hlr_iterations_match_oracle(rt, oracle, fixed, intercept):
             np.testing.assert_allclose(s.get("state", c), states[c], rtol=2e-5, atol=1e-7)
     acc = s.acceptance()
     assert acc.shape == (len(blocks), C)
+    s.close()
+
+
+def test_hlr_v_block_last_matches_oracle(rt, oracle):
+    """HMC [theta] (*) HMC [v]: a v block with no data block after it runs as
+    its own launch instead of sharing the next block's first launch."""
+    from augurv2_amd._lib import HMC_BLOCK_V, HMC_BLOCK_THETA
+    rng = np.random.default_rng(31)
+    n, d, C = 2500, 8, 4
+    x, y = _hlr_data(rng, n, d)
+    blocks = [(HMC_BLOCK_THETA, 0.1, 0.01, 0), (HMC_BLOCK_V, 1.0, 0.05, 0)]
+    s = rt.HlrSampler(x, y, blocks, chains=C, seed=77)
+    m = oracle.Hlr(x.astype(np.float64), y)
+    states = []
+    for c in range(C):
+        st = np.concatenate([rng.standard_normal(d) * 0.3, [0.0], [0.7 + 0.1 * c]])
+        s.set("state", st, chain=c)
+        states.append(st)
+    for sweep in range(3):
+        s.step(1)
+        for c in range(C):
+            m.hmc_step(states[c], 2, 0, 0.1, 0.01, 0, 77, c, sweep)
+            m.hmc_step(states[c], 0, 1, 1.0, 0.05, 0, 77, c, sweep)
+            np.testing.assert_allclose(s.get("state", c), states[c], rtol=2e-5, atol=1e-7)
     s.close()
 
 
