@@ -65,8 +65,10 @@ def run_hlr(steps, warmup, seed=2024):
     s.step(warmup)
     s.trace("state", steps)
     # kernel-time events on every pass launch of one sweep in fifty: about
-    # 3,700 timed passes in the default run, few sweeps perturbed
-    s.profile(50)
+    # 3,700 timed passes in the default run, few sweeps perturbed.  Sweeps
+    # whose global index is a multiple of the period are timed, so a period
+    # of at most `steps` always catches one.
+    s.profile(min(50, steps))
     log(f"hlr: timed {steps} sweeps")
     t0 = time.perf_counter()
     s.step(steps)  # returns after the stream is synchronised
@@ -327,6 +329,8 @@ def main():
     value = hlr["min_ess"] / hlr["elapsed"]
     N, D = hlr["N"], hlr["D"]
     bytes_per_pass = N * D * 4 + N * 4  # X (FP32) + y (int32): the algorithmic bytes
+    if hlr["pass_launches_timed"] == 0:
+        raise SystemExit("bench: no pass launch was timed (steps must be at least 1)")
     achieved = bytes_per_pass / (hlr["pass_avg_us"] * 1e-6) / 1e9
     cpu = None
     if not args.no_cpu:
