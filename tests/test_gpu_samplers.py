@@ -287,6 +287,34 @@ def test_gmm_log_density_matches_oracle(rt, oracle, sample_pi, sample_cov):
     s.close()
 
 
+def test_gmm_draws_do_not_depend_on_the_z_grid(rt, monkeypatch):
+    """The matrix-core z step (D = 16) run with 1, 7 and the default number of
+    blocks per chain: every draw comes from its own counter, so four sweeps
+    leave bit-identical states."""
+    rng = np.random.default_rng(12)
+    n, d, k, C = 20000, 16, 8, 2
+    y, _ = _gmm_setup(rng, n, d, k)
+    init = [(rng.standard_normal((k, d)) * 2, rng.integers(0, k, n).astype(np.int32)) for _ in range(C)]
+    out = []
+    for blocks in ("1", "7", None):
+        if blocks is None:
+            monkeypatch.delenv("AUGUR_GMM_ZBLOCKS", raising=False)
+        else:
+            monkeypatch.setenv("AUGUR_GMM_ZBLOCKS", blocks)
+        s = rt.GmmSampler(y, np.ones(k), np.zeros(d), 10 * np.eye(d), d + 2, np.eye(d), chains=C,
+                          seed=6)
+        for c, (mu, z) in enumerate(init):
+            s.set("mu", mu, chain=c)
+            s.set("z", z, chain=c)
+        s.step(4)
+        out.append([[s.get(name, c) for name in ("pis", "mu", "cov", "z")] for c in range(C)])
+        s.close()
+    for other in out[1:]:
+        for c in range(C):
+            for a, b in zip(out[0][c], other[c]):
+                np.testing.assert_array_equal(a, b)
+
+
 @pytest.mark.parametrize("sample_pi,sample_cov", [(True, True), (False, False)])
 def test_gmm_sweeps_match_oracle(rt, oracle, sample_pi, sample_cov):
     rng = np.random.default_rng(8)
