@@ -55,7 +55,8 @@ class SurrogateCfg(C.Structure):
     _fields_ = [("block_index", C.c_int), ("hidden", C.c_int), ("train_sweeps", C.c_int),
                 ("max_pairs", C.c_int), ("epochs", C.c_int), ("batch", C.c_int),
                 ("lr", C.c_double), ("steps", C.c_int), ("step_size", C.c_double),
-                ("min_accept", C.c_double), ("window", C.c_int), ("seed", C.c_uint32)]
+                ("min_accept", C.c_double), ("window", C.c_int), ("seed", C.c_uint32),
+                ("linear_only", C.c_int)]
 
 
 class GmmDesc(C.Structure):

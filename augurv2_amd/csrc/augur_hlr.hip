@@ -918,6 +918,10 @@ struct HlrSampler final : augur_sampler {
   SurrInfer inf;
   double resid_frac = -1.0;  // residual variance of the linear term / target variance
   DevBuf<double> ll64_cur, ll64_prop, ll64_part;
+  // ll64_cur no longer holds the FP64 data log-density of the current state:
+  // set_pt wrote theta / b, or another block moved them since the last
+  // surrogate iteration
+  bool ll64_dirty = false;
 
   const char* family() const override { return "hlr"; }
 
@@ -932,7 +936,10 @@ struct HlrSampler final : augur_sampler {
   }
 
   void on_set(const std::string& name) override {
-    if (name != "v") cache_valid = false;
+    if (name != "v") {
+      cache_valid = false;
+      ll64_dirty = true;
+    }
   }
 
   PassArgs pass_args(int mode, int l) {
@@ -994,7 +1001,13 @@ struct HlrSampler final : augur_sampler {
     surr_standardise(cap_x.p, cap_y.p, n_pairs, d, sstats.p, stream);
     surr_linear_fit(cap_x.p, cap_y.p, n_pairs, d, 1e-6, lin_b.p, stream);
     mlp_colstats(cap_x.p, cap_y.p, n_pairs, d, sstats2.p, stream);
-    const long steps = std::max<long>(1, (long)scfg.epochs * n_pairs / scfg.batch);
+    const long steps = scfg.linear_only ? 0 : std::max<long>(1, (long)scfg.epochs * n_pairs / scfg.batch);
+    if (scfg.linear_only) {
+      // no network: its output layer is zero, so the fitted gradient is the
+      // linear term alone
+      const MlpShape sh{d, scfg.hidden};
+      AUGUR_HIP(hipMemsetAsync(mlp.params.p + sh.w3(), 0, sizeof(float) * (sh.count() - sh.w3()), stream));
+    }
     double lsum = 0.0;
     int lcount = 0;
     for (long s = 0; s < steps; ++s) {
@@ -1012,6 +1025,7 @@ struct HlrSampler final : augur_sampler {
     surr_loss = lcount ? lsum / lcount : 0.0;
     surr_fold(d, scfg.hidden, sstats.p, sstats2.p, lin_b.p, mlp.params.p, inf, stream);
     launch_ll64(ll64_args(state.p, ll64_cur.p), stream);
+    ll64_dirty = false;
     std::vector<double> st((size_t)sched.size() * chains * 2);
     stats_buf.download(st.data(), st.size(), stream);
     std::vector<float> s2(4 * (size_t)d);
@@ -1048,6 +1062,10 @@ struct HlrSampler final : augur_sampler {
     S.w3t = inf.w3t.p; S.b3 = inf.b3.p;
     S.in_mean = inf.in_mean.p; S.in_scale = inf.in_scale.p; S.lin_mean = inf.lin_mean.p; S.lin_t = inf.lin_t.p;
     S.state = state.p; S.prop = prop.p; S.p = p.p; S.k0 = k0.p; S.beta32 = beta32.p;
+    if (ll64_dirty) {
+      launch_ll64(ll64_args(state.p, ll64_cur.p), stream);
+      ll64_dirty = false;
+    }
     launch_surr_traj(S, chains, stream);
     launch_ll64(ll64_args(prop.p, ll64_prop.p), stream);
     k_hmc_end<<<chains, 64, 0, stream>>>(end_args_surrogate(bi));
@@ -1122,6 +1140,7 @@ struct HlrSampler final : augur_sampler {
     }
     k_hmc_end<<<chains, 64, 0, stream>>>(H);
     AUGUR_CHECK_LAUNCH();
+    if (H.blk_theta || H.blk_b) ll64_dirty = true;  // this block may have moved theta or b
     if (capture && ++surr_sweeps_collected >= scfg.train_sweeps && n_pairs > 0) train_surrogate();
   }
 
@@ -1362,7 +1381,7 @@ augur_status augur_hlr_surrogate(augur_sampler* s, const augur_surrogate_cfg* cf
       fail(AUGUR_ERR_ARG, "block_index must name an HMC [theta] block");
     if (h->has_b) fail(AUGUR_ERR_UNSUPPORTED, "the surrogate covers models without an intercept");
     if (h->d > 256 || c.hidden < 1 || c.hidden > 256) fail(AUGUR_ERR_ARG, "need d <= 256 and 1 <= hidden <= 256");
-    if (c.train_sweeps < 1 || c.epochs < 1 || c.batch < 1 || c.batch > 65536 || !(c.lr > 0.0) ||
+    if (c.train_sweeps < 1 || (c.epochs < 1 && !c.linear_only) || c.batch < 1 || c.batch > 65536 || !(c.lr > 0.0) ||
         c.window < 1 || c.steps < 0 || (c.steps > 0 && !(c.step_size > 0.0)) || c.max_pairs < h->chains)
       fail(AUGUR_ERR_ARG, "bad surrogate settings");
     if (h->surr_phase != HlrSampler::SURR_OFF) fail(AUGUR_ERR_STATE, "surrogate already configured");

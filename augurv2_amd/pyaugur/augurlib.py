@@ -33,11 +33,12 @@ log = logging.getLogger("augurv2_amd")
 
 class AugurOpt:
     """cached: accepted for compatibility (nothing is generated or cached);
-    target: 'gpu' runs on the MI355X; 'cpu' is not provided by this runtime
-    (the reference's libcpu); initStrat: initialisation strategy; paramScale:
-    accepted for compatibility (unused by the reference's compiler too)."""
+    target: 'gpu' runs on the MI355X; 'cpu' (the reference's default, its
+    libcpu) is not provided by this runtime and compile raises for it;
+    initStrat: initialisation strategy; paramScale: accepted for
+    compatibility (unused by the reference's compiler too)."""
 
-    def __init__(self, cached=False, target="gpu", initStrat=None, paramScale=None, chains=1,
+    def __init__(self, cached=False, target="cpu", initStrat=None, paramScale=None, chains=1,
                  seed=0, hmc_jitter=True, ref_compat=True):
         assert target in ("cpu", "gpu")
         self.cached = cached
@@ -153,20 +154,26 @@ class AugurInfer:
 
     def samplen(self, burnIn=0, numSamples=1, thin=1, log_like=False):
         """Returns {name: [value per kept sample]} (a list of per-chain values
-        per sample when chains > 1)."""
+        per sample when chains > 1).  As the reference (augurlib.py:195-222):
+        numSamples * thin sweeps after burnIn, keeping the state after sweeps
+        1, thin + 1, 2 thin + 1, ... (sweep index i kept when i % thin == 0)."""
         assert self.sampler is not None
+        assert thin >= 1
         if burnIn:
             self.sampler.step(burnIn)
         samples = {name: [] for name in self.param_names}
-        for i in range(numSamples):
-            self.sampler.step(thin)
-            if i % 100 == 0:
-                log.info("Doing iter: %d / %d", i, numSamples)
+        for j in range(numSamples):
+            # sweep i = j * thin is the kept one; the thin - 1 after it follow
+            self.sampler.step(1)
+            if (j * thin) % 100 == 0:
+                log.info("Doing iter: %d / %d", j * thin, numSamples)
             pt = self._point()
             if log_like:
-                print("log-like[%d]: %s" % (i, self.curr_ll()))
+                print("log-like[%d]: %s" % (j, self.curr_ll()))
             for name in self.param_names:
                 samples[name].append(pt[name])
+            if thin > 1:
+                self.sampler.step(thin - 1)
         return samples
 
 

@@ -142,12 +142,13 @@ class HlrSampler(Sampler):
 
     def enable_surrogate(self, block_index, hidden=256, train_sweeps=2000, max_pairs=500_000,
                          epochs=20, batch=512, lr=1e-3, steps=0, step_size=0.0,
-                         min_accept=0.6, window=50, seed=0):
+                         min_accept=0.6, window=50, seed=0, linear_only=False):
         """Surrogate-gradient HMC for block `block_index` (an HMC [theta] block).
         steps = 0: surrogate trajectories follow the block's own (jittered)
-        schedule; steps > 0: that many leapfrog steps of step_size."""
+        schedule; steps > 0: that many leapfrog steps of step_size.
+        linear_only: the least-squares linear term without the network."""
         cfg = L.SurrogateCfg(block_index, hidden, train_sweeps, max_pairs, epochs, batch, lr, steps,
-                             step_size, min_accept, window, seed & 0xFFFFFFFF)
+                             step_size, min_accept, window, seed & 0xFFFFFFFF, int(linear_only))
         L.call("augur_hlr_surrogate", self._h, C.byref(cfg))
 
     def surrogate_info(self):

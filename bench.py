@@ -1,14 +1,22 @@
 """Benchmark: min-ESS/sec of the HMC sampler for hierarchical logistic
 regression (N = 100k rows, D = 64, 4 chains, HMC [v] (*) HMC [beta] with 20
 leapfrog steps of 0.005) on one MI355X, at R-hat < 1.01; beside it LDA
-tokens/sec (K = 10, V = 1000, D = 200, ~20k tokens, 100 sweeps, 1 chain).
+tokens/sec (K = 10, V = 1000, D = 200, ~20k tokens, 100 sweeps, 1 chain),
+the GMM (N = 1M, D = 16, K = 32, 4 chains, 500 sweeps), the batched HLR
+sampler (256 datasets) and surrogate-gradient HMC against the plain GPU
+sampler (N = 1M, D = 128, 8 chains).
 
-A step is one sweep of the schedule (one HMC iteration of every block on
-every chain).  Prints one JSON line.
+A step of the headline is a block of SWEEPS_PER_STEP sweeps of the schedule
+(one HMC iteration of every block on every chain per sweep), so that
+`--steps 20 --warmup 5` times 10,000 sweeps after 2,500 warmup sweeps: the
+sweeps R-hat < 1.01 needs on this workload.  `value` is null unless every
+monitored coordinate has R-hat < 1.01.  Prints one JSON line.
 """
 import argparse
+import contextlib
 import json
 import os
+import statistics
 import sys
 import time
 
@@ -18,6 +26,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+SWEEPS_PER_STEP = 500
+RHAT_MAX = 1.01
 
 # HBM bytes per launch of the headline pass kernel: rocprofv3 --pmc FETCH_SIZE
 # and WRITE_SIZE (KiB per dispatch) on this workload, each scaled by what the
@@ -43,8 +53,60 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+# ---------------------------------------------------------------- CPU side
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+@contextlib.contextmanager
+def one_core():
+    """Pins this process to one core for the duration (the `taskset -c`
+    protocol of the CPU baseline), then restores the affinity."""
+    old = os.sched_getaffinity(0)
+    core = min(old)
+    os.sched_setaffinity(0, {core})
+    try:
+        yield core
+    finally:
+        os.sched_setaffinity(0, old)
+
+
+def median_time(fn, runs=3):
+    """Median wall time of `runs` calls of fn() on one pinned core."""
+    times = []
+    with one_core() as core:
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            fn()
+            times.append(time.perf_counter() - t0)
+    return statistics.median(times), times, core
+
+
+def cpu_record(value, unit, sample, times, core):
+    return {"value": value, "unit": unit, "cores": 1, "kind": "port", "sample": sample,
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(), "pinned_core": core,
+            "runs_s": [round(t, 4) for t in times], "statistic": "median of 3"}
+
+
+# ---------------------------------------------------------------- legs
+
+def _summary(draws, keep=None):
+    from augurv2_amd import diagnostics
+    d = draws if keep is None else draws[:, :, keep]
+    summ = diagnostics.summarize(d)
+    return float(np.min(summ["ess_bulk"])), float(np.max(summ["rhat"])), summ
+
+
 def run_hlr(steps, warmup, seed=2024):
-    from augurv2_amd import diagnostics, runtime, workloads
+    from augurv2_amd import runtime, workloads
     from augurv2_amd._lib import HMC_BLOCK_THETA, HMC_BLOCK_V
     N, D, C, L, EPS = 100_000, 64, 4, 20, 0.005
     x, y, v_true, beta_true = workloads.hlr_data(N, D, seed=0)
@@ -61,47 +123,185 @@ def run_hlr(steps, warmup, seed=2024):
                            prior_b=1.0, ref_compat=False)
     for c, st in enumerate(workloads.hlr_init(D, C)):
         s.set("state", st, chain=c)
-    log(f"hlr: warmup {warmup} sweeps")
-    s.step(warmup)
-    s.trace("state", steps)
-    # kernel-time events on every pass launch of one sweep in fifty: about
-    # 3,700 timed passes in the default run, few sweeps perturbed.  Sweeps
-    # whose global index is a multiple of the period are timed, so a period
-    # of at most `steps` always catches one.
-    s.profile(min(50, steps))
-    log(f"hlr: timed {steps} sweeps")
+    n_warm, n_timed = warmup * SWEEPS_PER_STEP, steps * SWEEPS_PER_STEP
+    log(f"hlr: warmup {warmup} steps ({n_warm} sweeps)")
+    s.step(n_warm)
+    s.trace("state", n_timed)
+    # kernel-time events on every pass launch of one sweep in fifty
+    s.profile(50)
+    log(f"hlr: timed {steps} steps ({n_timed} sweeps)")
     t0 = time.perf_counter()
-    s.step(steps)  # returns after the stream is synchronised
+    for _ in range(steps):
+        s.step(SWEEPS_PER_STEP)  # returns after the stream is synchronised
     t1 = time.perf_counter()
-    avg_us, n_timed = s.profile_read()
-    draws = s.read_trace("state")  # (steps, C, D + 2)
+    avg_us, n_prof = s.profile_read()
+    draws = s.read_trace("state")  # (sweeps, C, D + 2)
     keep = list(range(D)) + [D + 1]  # theta and v (no intercept in this model)
-    summ = diagnostics.summarize(draws[:, :, keep])
+    min_ess, rhat, summ = _summary(draws, keep)
     acc = s.acceptance()
     s.close()
-    elapsed = t1 - t0
-    min_ess = float(np.min(summ["ess_bulk"]))
     return {
-        "elapsed": elapsed, "min_ess": min_ess, "rhat_max": float(np.max(summ["rhat"])),
-        "min_ess_tail": float(np.min(summ["ess_tail"])),
+        "elapsed": t1 - t0, "min_ess": min_ess, "rhat_max": rhat,
+        "min_ess_tail": float(np.min(summ["ess_tail"])), "sweeps": n_timed,
         "accept": [float(a) for a in acc.mean(axis=1)], "pass_avg_us": avg_us,
-        "pass_launches_timed": n_timed, "N": N, "D": D, "C": C, "L": L, "eps": EPS,
+        "pass_launches_timed": n_prof, "N": N, "D": D, "C": C, "L": L, "eps": EPS,
         "x": x, "y": y, "blocks": blocks, "post_mean_v": float(summ["mean"][-1]),
     }
 
 
-def _min_ess(draws, keep):
-    from augurv2_amd import diagnostics
-    summ = diagnostics.summarize(draws[:, :, keep])
-    return float(np.min(summ["ess_bulk"])), float(np.max(summ["rhat"]))
+def cpu_hlr(hlr, sweeps_per_run=6):
+    """The FP64 oracle (a restatement of the reference's libcpu sampler: same
+    schedule, same Philox streams, L + 3 data passes per HMC iteration), one
+    pinned core, on the same data: median of 3 runs of a truncated number of
+    sweeps of all chains, extrapolated linearly per sweep (the protocol for
+    long configurations).  The oracle's chain follows the GPU's draw for
+    draw (same algorithm and streams), so its min-ESS per sweep is the GPU
+    run's."""
+    from augurv2_amd import workloads
+    from oracle import oracle as orc
+    x64 = hlr["x"].astype(np.float64)
+    m = orc.Hlr(x64, hlr["y"], prior_kind=0, prior_a=1.0, prior_b=1.0, ref_compat=False)
+    kinds = {0: 0, 2: 2}
+    start = [st.copy() for st in workloads.hlr_init(hlr["D"], hlr["C"])]
+
+    def run():
+        states = [st.copy() for st in start]
+        for it in range(sweeps_per_run):
+            for c in range(hlr["C"]):
+                for bi, (kind, sl, eps, fx) in enumerate(hlr["blocks"]):
+                    m.hmc_step(states[c], kinds[kind], bi, sl, eps, fx, 2024, c, it)
+
+    med, times, core = median_time(run)
+    s_per_sweep = med / sweeps_per_run
+    ok = hlr["rhat_max"] < RHAT_MAX
+    ess_per_sweep = hlr["min_ess"] / hlr["sweeps"]
+    return cpu_record(ess_per_sweep / s_per_sweep if ok else None, "min-ESS/s",
+                      f"{sweeps_per_run} HMC sweeps x {hlr['C']} chains of the same workload per run "
+                      f"({s_per_sweep:.3f} s/sweep), extrapolated to the GPU run's min-ESS per sweep "
+                      f"({ess_per_sweep:.4f}; same algorithm and Philox streams)"
+                      + ("" if ok else "; null: the GPU run did not reach R-hat < 1.01"),
+                      times, core)
 
 
-def run_surrogate(iters=5000, warmup=1000, train=2000, seed=77):
-    """HLR N = 1M, D = 128, 8 chains: plain HMC and surrogate-gradient HMC
-    (linear least squares + MLP 128-256-256-128 on its residual, trained on
-    the pairs of `train` exact iterations after warmup), both with 30
-    leapfrog steps of 0.005 (step jittered as the reference does) and the
-    exact FP64 Metropolis test; `iters` timed iterations each."""
+def run_lda(sweeps=100, warmup=10):
+    from augurv2_amd import runtime, workloads
+    doc_off, w, alpha, beta, _ = workloads.lda_corpus(K=10, V=1000, D=200, mean_len=100, seed=0)
+    s = runtime.LdaSampler(10, doc_off, w, alpha, beta, chains=1, seed=7)
+    rng = np.random.default_rng(3)
+    z0 = rng.integers(0, 10, len(w)).astype(np.int32)
+    s.set("z", z0)
+    s.step(warmup)
+    t0 = time.perf_counter()
+    s.step(sweeps)
+    t1 = time.perf_counter()
+    # the z step's own duration from kernel-time events, in sweeps after the
+    # timed ones (an event pair on every launch would slow the timed sweeps)
+    s.profile(1)
+    s.step(20)
+    zavg, _ = s.profile_read()
+    s.close()
+    T = int(doc_off[-1])
+    # CPU: the oracle's sweep (the reference's libcpu order: theta, phi, z)
+    from oracle import oracle as orc
+    th, ph = np.zeros((200, 10)), np.zeros((10, 1000))
+
+    def cpu_run():
+        z = z0.copy()
+        for sw in range(sweeps):
+            orc.lda_sweep(doc_off, w, alpha, beta, 7, 0, sw, th, ph, z)
+
+    med, times, core = median_time(cpu_run)
+    # the z step's algorithmic bytes: w read + z write per token
+    zbytes = 8.0 * T
+    return {"tokens": T, "sweeps": sweeps, "tokens_per_s": T * sweeps / (t1 - t0),
+            "sweep_us": 1e6 * (t1 - t0) / sweeps, "zstep_avg_us": zavg,
+            "roofline": {"kernel": "k_lda_z", "bound": "hbm", "achieved": zbytes / (zavg * 1e-6) / 1e9,
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": zbytes / (zavg * 1e-6) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                         "note": "8 B per token over 19.9k tokens: latency-bound (dependent gathers), "
+                                 "not bandwidth-bound, at this size"},
+            "cpu_baseline": cpu_record(T * sweeps / med, "tokens/s",
+                                       f"the oracle's LDA sweep, {sweeps} sweeps of the same corpus per run",
+                                       times, core)}
+
+
+def gmm_summaries(pis, mu, cov):
+    """Label-invariant scalars per (sweep, chain): the sorted weights, the
+    sorted norms of the means and the sorted traces of the covariances."""
+    return np.concatenate([np.sort(pis, axis=-1), np.sort(np.linalg.norm(mu, axis=-1), axis=-1),
+                           np.sort(np.trace(cov, axis1=-2, axis2=-1), axis=-1)], axis=-1)
+
+
+def run_gmm(sweeps=500, warmup=100, chains=4, seed=5, cpu_points=10_000):
+    """Multivariate GMM Gibbs, N = 1M, D = 16, K = 32, 4 chains, schedule
+    ConjGibbs [pis] (*) ConjGibbs [mu] (*) ConjGibbs [cov] (*) DiscGibbs [z]
+    (experiments/experiment.py:127).  Every chain starts from the
+    data-generating assignment z (Gibbs from a random start ends in different
+    local modes: the data hold components of a few dozen points); the chains
+    then run on their own Philox streams.  min-ESS and R-hat are taken over
+    label-invariant summaries (sorted weights, mean norms, covariance
+    traces)."""
+    from augurv2_amd import runtime, workloads
+    N, D, K = 1_000_000, 16, 32
+    y, z_true, _, _, _ = workloads.gmm_data(N, D, K, seed=0)
+    s = runtime.GmmSampler(y, np.ones(K), np.zeros(D), 10.0 * np.eye(D), D + 2, np.eye(D),
+                           chains=chains, seed=seed)
+    for c in range(chains):
+        s.set("pis", np.full(K, 1.0 / K), chain=c)
+        s.set("mu", np.zeros((K, D)), chain=c)
+        s.set("cov", np.tile(np.eye(D), (K, 1, 1)), chain=c)
+        s.set("z", z_true, chain=c)
+    s.step(warmup)
+    for name in ("pis", "mu", "cov"):
+        s.trace(name, sweeps)
+    s.profile(25)
+    t0 = time.perf_counter()
+    s.step(sweeps)
+    el = time.perf_counter() - t0
+    zavg, _ = s.profile_read()
+    summ = gmm_summaries(s.read_trace("pis"), s.read_trace("mu"), s.read_trace("cov"))
+    ll = s.log_density()
+    s.close()
+    min_ess, rhat, _ = _summary(summ)
+    ok = rhat < RHAT_MAX
+    # CPU: the oracle's sweep with the reference's per-(point, component)
+    # Cholesky in the categorical step (cbits/augur_dist.c:548-568), on the
+    # first cpu_points points, extrapolated linearly in N
+    from oracle import oracle as orc
+    ys = y[:cpu_points].astype(np.float64)
+    g = orc.Gmm(ys, np.ones(K), np.zeros(D), 10.0 * np.eye(D), D + 2, np.eye(D))
+    z0 = z_true[:cpu_points].copy()
+
+    def cpu_run():
+        pis, mu, cov, z = np.full(K, 1.0 / K), np.zeros((K, D)), np.tile(np.eye(D), (K, 1, 1)), z0.copy()
+        g.sweep(pis, mu, cov, z, seed, 0, 0)
+
+    med, times, core = median_time(cpu_run)
+    s_sweep_chain = med * N / cpu_points
+    ess_per_sweep = min_ess / sweeps
+    cpu = cpu_record(ess_per_sweep / (s_sweep_chain * chains) if ok else None, "min-ESS/s",
+                     f"one sweep of one chain over the first {cpu_points} points per run "
+                     f"({med:.3f} s), extrapolated linearly to N = {N} and {chains} chains "
+                     f"({s_sweep_chain * chains:.1f} s per sweep of all chains) at the GPU run's "
+                     "min-ESS per sweep", times, core)
+    return {"sweeps": sweeps, "warmup": warmup, "chains": chains, "elapsed_s": el,
+            "ms_per_sweep": 1e3 * el / sweeps, "points_per_s": N * chains * sweeps / el,
+            "min_ess": min_ess, "rhat_max": rhat, "rhat_ok": ok,
+            "min_ess_per_s": min_ess / el if ok else None,
+            "summary": "sorted pis, sorted |mu_k|, sorted tr(cov_k): 96 label-invariant scalars",
+            "zstep_avg_us": zavg, "final_log_density": [float(v) for v in ll],
+            "cpu_baseline": cpu}
+
+
+def run_surrogate(iters=5000, burn=1000, train=1000, seed=77, ablation=True):
+    """HLR N = 1M, D = 128, 8 chains, MWG [v] (*) HMC [theta] with 30
+    leapfrog steps of 0.005 (step jittered as the reference does), the exact
+    FP64 Metropolis test.  Both samplers run burn + train exact iterations
+    first (the first 2,000 exact-HMC iterations of the configuration; the
+    surrogate records its training pairs during the last `train`), then the
+    timed region: `iters` iterations of plain HMC, or the surrogate's training
+    followed by `iters` surrogate iterations.  min-ESS over the timed
+    iterations."""
     from augurv2_amd import runtime, workloads
     from augurv2_amd._lib import HMC_BLOCK_THETA, MWG_BLOCK_V
     N, D, C, L, EPS = 1_000_000, 128, 8, 30, 0.005
@@ -111,40 +311,68 @@ def run_surrogate(iters=5000, warmup=1000, train=2000, seed=77):
     blocks = [(MWG_BLOCK_V, 0.0, 1.0, 0), (HMC_BLOCK_THETA, L * EPS, EPS, -L)]
     keep = list(range(D)) + [D + 1]
     out = {}
-    for mode in ("plain", "surrogate"):
+    modes = ("plain", "surrogate", "linear_only") if ablation else ("plain", "surrogate")
+    for mode in modes:
         s = runtime.HlrSampler(x, y, blocks, chains=C, seed=seed, ref_compat=False)
         for c, st in enumerate(workloads.hlr_init(D, C, seed=11)):
             s.set("state", st, chain=c)
-        log(f"surrogate leg ({mode}): warmup {warmup}")
-        s.step(warmup)
-        t_fit = 0.0
-        if mode == "surrogate":
+        log(f"surrogate leg ({mode}): {burn + train} exact iterations")
+        t_pre = time.perf_counter()
+        s.step(burn)
+        if mode != "plain":
             s.enable_surrogate(1, hidden=256, train_sweeps=train, max_pairs=train * C * L,
-                               epochs=10, batch=4096, lr=2e-3, min_accept=0.6, window=100, seed=5)
-            t0 = time.perf_counter()
-            s.step(train)  # exact HMC collecting pairs, then training
-            t_fit = time.perf_counter() - t0
+                               epochs=10, batch=4096, lr=2e-3, min_accept=0.6, window=100, seed=5,
+                               linear_only=(mode == "linear_only"))
+            s.step(train - 1)  # the last collection sweep (and training) is timed
+            extra = 1
+        else:
+            s.step(train)
+            extra = 0
+        t_pre = time.perf_counter() - t_pre
         s.trace("state", iters)
         log(f"surrogate leg ({mode}): timed {iters}")
         t0 = time.perf_counter()
-        s.step(iters)
+        s.step(iters + extra)
         el = time.perf_counter() - t0
-        ess, rhat = _min_ess(s.read_trace("state"), keep)
+        draws = s.read_trace("state")
+        ess, rhat, summ = _summary(draws, keep)
+        ok = rhat < RHAT_MAX
         r = {"elapsed_s": el, "ms_per_iter": 1e3 * el / iters, "min_ess": ess, "rhat_max": rhat,
-             "min_ess_per_s": ess / el, "accept": float(np.nanmean(s.acceptance()[1]))}
-        if mode == "surrogate":
+             "rhat_ok": ok, "min_ess_per_s": ess / el if ok else None,
+             "accept": float(np.nanmean(s.acceptance()[1])), "exact_phase_s": t_pre,
+             "whole_job_s": t_pre + el, "post_mean": summ["mean"], "post_mcse": summ["mcse"],
+             "post_sd": draws[:, :, keep].reshape(-1, len(keep)).std(axis=0)}
+        if mode != "plain":
             info = s.surrogate_info()
-            r.update({"collect_and_train_s": t_fit, "train_ms": info["train_ms"],
-                      "pairs": info["pairs"], "loss": info["loss"], "resid_frac": info["resid_frac"],
-                      "phase": info["phase"], "fallbacks": info["fallbacks"],
-                      "min_ess_per_s_incl_training": ess / (el + t_fit)})
+            r.update({"train_ms": info["train_ms"], "pairs": info["pairs"], "loss": info["loss"],
+                      "resid_frac": info["resid_frac"], "phase": info["phase"],
+                      "fallbacks": info["fallbacks"],
+                      "min_ess_per_s_excl_training": ess / (el - 1e-3 * info["train_ms"]) if ok else None})
         s.close()
         out[mode] = r
-        log(f"surrogate leg ({mode}): {r}")
-    out["ratio"] = out["surrogate"]["min_ess_per_s"] / out["plain"]["min_ess_per_s"]
-    out["config"] = (f"HLR N={N} D={D}, {C} chains, {L} leapfrog steps of {EPS} (step jittered), "
-                     f"warmup {warmup}, surrogate trained on {train} exact iterations, {iters} timed")
-    return out
+        log(f"surrogate leg ({mode}): " + str({k: v for k, v in r.items()
+                                               if not k.startswith("post_")}))
+    p, q = out["plain"], out["surrogate"]
+    # posterior moments of the surrogate sampler against the FP64 plain
+    # sampler: means within 3 combined MCSE (family-wise count beyond 3)
+    z = np.abs(q["post_mean"] - p["post_mean"]) / np.hypot(p["post_mcse"], q["post_mcse"])
+    res = {"config": (f"HLR N={N} D={D}, {C} chains, {L} leapfrog steps of {EPS} (step jittered), "
+                      f"{burn + train} exact iterations first, {iters} timed"),
+           "moments_vs_plain": {"max_z": float(z.max()), "n_beyond_3_mcse": int((z > 3).sum()),
+                                "n_params": int(z.size),
+                                "max_rel_sd_diff": float(np.max(np.abs(q["post_sd"] / p["post_sd"] - 1)))}}
+    if p["min_ess_per_s"] and q["min_ess_per_s"]:
+        res["ratio_timed_incl_training"] = q["min_ess_per_s"] / p["min_ess_per_s"]
+        res["ratio_excl_training"] = q["min_ess_per_s_excl_training"] / p["min_ess_per_s"]
+        res["ratio_whole_job"] = (q["min_ess"] / q["whole_job_s"]) / (p["min_ess"] / p["whole_job_s"])
+    if "linear_only" in out:
+        lo = out["linear_only"]
+        res["ablation"] = {"linear_only_accept": lo["accept"], "mlp_accept": q["accept"],
+                           "linear_only_min_ess_per_s": lo["min_ess_per_s"],
+                           "mlp_min_ess_per_s": q["min_ess_per_s"]}
+    for mode, r in out.items():
+        res[mode] = {k: v for k, v in r.items() if not k.startswith("post_")}
+    return res
 
 
 def run_batched(sweeps=1000, warmup=500, chains=4, seed=31):
@@ -183,10 +411,11 @@ def run_batched(sweeps=1000, warmup=500, chains=4, seed=31):
     ess_i, rhat_i = [], []
     for i, x in enumerate(xs):
         o, d = int(s.offsets[i]), x.shape[1]
-        e, r = _min_ess(draws[:, :, o:o + d + 2], list(range(d)) + [d + 1])
+        e, r, _ = _summary(draws[:, :, o:o + d + 2], list(range(d)) + [d + 1])
         ess_i.append(e)
         rhat_i.append(r)
-    ess_i = np.array(ess_i)
+    ess_i, rhat_i = np.array(ess_i), np.array(rhat_i)
+    conv = rhat_i < RHAT_MAX
     # plain sampler, one dataset at a time
     work = np.array([x.shape[0] * x.shape[1] for x in xs], np.float64)
     pick = np.argsort(work)[np.linspace(0, len(xs) - 1, 8).astype(int)]
@@ -208,92 +437,22 @@ def run_batched(sweeps=1000, warmup=500, chains=4, seed=31):
             "rows_per_block": rows, "autotune_us": cand, "sweeps": sweeps, "elapsed_s": el,
             "warmup_s": t_warm, "step_size_range": [float(eps_i.min()), float(eps_i.max())],
             "v_proposal_var_range": [float(vvar_i.min()), float(vvar_i.max())],
-            "n_rhat_below_1_01": int(np.sum(np.array(rhat_i) < 1.01)),
+            "n_rhat_below_1_01": int(conv.sum()),
             "ms_per_sweep": 1e3 * el / sweeps,
-            "sum_dataset_min_ess_per_s": float(ess_i.sum() / el),
+            "sum_converged_dataset_min_ess_per_s": float(ess_i[conv].sum() / el),
             "min_dataset_ess": float(ess_i.min()), "median_dataset_ess": float(np.median(ess_i)),
             "rhat_max": float(np.max(rhat_i)), "accept_theta": float(np.nanmean(acc[1])),
-            "plain_sequential_s_est": plain_total, "speedup_vs_plain_sequential": plain_total / el}
-
-
-def run_gmm(sweeps=500, warmup=20, chains=4, seed=5):
-    """Multivariate GMM Gibbs, N = 1M, D = 16, K = 32, 4 chains: sweeps/s."""
-    from augurv2_amd import runtime, workloads
-    N, D, K = 1_000_000, 16, 32
-    y, z_true, _, _, _ = workloads.gmm_data(N, D, K, seed=0)
-    s = runtime.GmmSampler(y, np.ones(K), np.zeros(D), 10.0 * np.eye(D), D + 2, np.eye(D),
-                           chains=chains, seed=seed)
-    rng = np.random.default_rng(1)
-    for c in range(chains):
-        s.set("pis", np.full(K, 1.0 / K), chain=c)
-        s.set("mu", y[rng.choice(N, K, replace=False)].astype(np.float64), chain=c)
-        s.set("cov", np.tile(np.eye(D), (K, 1, 1)), chain=c)
-        s.set("z", rng.integers(0, K, N).astype(np.int32), chain=c)
-    s.step(warmup)
-    t0 = time.perf_counter()
-    s.step(sweeps)
-    el = time.perf_counter() - t0
-    ll = s.log_density()
-    s.close()
-    return {"sweeps": sweeps, "elapsed_s": el, "ms_per_sweep": 1e3 * el / sweeps,
-            "points_per_s": N * chains * sweeps / el, "final_log_density": [float(v) for v in ll]}
-
-
-def run_lda(sweeps=100, warmup=10):
-    from augurv2_amd import runtime, workloads
-    doc_off, w, alpha, beta, _ = workloads.lda_corpus(K=10, V=1000, D=200, mean_len=100, seed=0)
-    s = runtime.LdaSampler(10, doc_off, w, alpha, beta, chains=1, seed=7)
-    rng = np.random.default_rng(3)
-    s.set("z", rng.integers(0, 10, len(w)).astype(np.int32))
-    s.step(warmup)
-    t0 = time.perf_counter()
-    s.step(sweeps)
-    t1 = time.perf_counter()
-    # the z step's own duration from kernel-time events, in sweeps after the
-    # timed ones (an event pair on every launch would slow the timed sweeps)
-    s.profile(1)
-    s.step(20)
-    zavg, _ = s.profile_read()
-    s.close()
-    T = int(doc_off[-1])
-    return {"tokens": T, "sweeps": sweeps, "tokens_per_s": T * sweeps / (t1 - t0),
-            "sweep_us": 1e6 * (t1 - t0) / sweeps, "zstep_avg_us": zavg}
-
-
-def cpu_baseline(hlr, min_ess_per_iter, budget_s=12.0):
-    """The FP64 oracle (a restatement of the reference's libcpu sampler: same
-    schedule, same RNG contract, L + 3 data passes per HMC iteration), one
-    thread, on the same data; timed over whole iterations of all chains
-    until the budget is spent."""
-    from oracle import oracle as orc
-    x64 = hlr["x"].astype(np.float64)
-    m = orc.Hlr(x64, hlr["y"], prior_kind=0, prior_a=1.0, prior_b=1.0, ref_compat=False)
-    from augurv2_amd import workloads
-    states = [st.copy() for st in workloads.hlr_init(hlr["D"], hlr["C"])]
-    kinds = {0: 0, 2: 2}
-    it = 0
-    t0 = time.perf_counter()
-    while True:
-        for c in range(hlr["C"]):
-            for bi, (kind, sl, eps, fx) in enumerate(hlr["blocks"]):
-                m.hmc_step(states[c], kinds[kind], bi, sl, eps, fx, 2024, c, it)
-        it += 1
-        if time.perf_counter() - t0 >= budget_s:
-            break
-    el = time.perf_counter() - t0
-    return {"value": min_ess_per_iter * it / el, "unit": "min-ESS/s", "cores": 1, "kind": "port",
-            "sample": f"{it} HMC sweeps x {hlr['C']} chains of the same workload on the FP64 "
-                      f"oracle in {el:.1f} s ({el / it:.2f} s/sweep); min-ESS per sweep taken "
-                      "from the GPU run (same algorithm and RNG streams)"}
+            "plain_sequential_s_est": plain_total,
+            "sweep_time_ratio_vs_plain_sequential": plain_total / el}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    # 8000 timed sweeps reach R-hat < 1.01 on this workload (2000 reach ~1.04)
-    ap.add_argument("--steps", type=int, default=8000)
-    ap.add_argument("--warmup", type=int, default=1000)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--steps", type=int, default=20,
+                    help=f"timed steps of the headline, {SWEEPS_PER_STEP} sweeps each")
+    ap.add_argument("--warmup", type=int, default=5, help="untimed warmup steps of the headline")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--only", default="", help="comma list of legs to run besides the headline "
                     "HLR leg: lda, surrogate, batched, gmm (default: all)")
     ap.add_argument("--batched-sweeps", type=int, default=1000,
@@ -304,16 +463,21 @@ def main():
     if args.copy_mib:
         copy_probe(args.copy_mib)
         return
-    legs = set(args.only.split(",")) if args.only else {"lda", "surrogate", "batched", "gmm"}
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the sampler "
               "runtime runs on a single MI355X)", file=sys.stderr)
         sys.exit(2)
+    if args.steps < 1:
+        raise SystemExit("bench: --steps must be at least 1")
+    legs = set(args.only.split(",")) if args.only else {"lda", "surrogate", "batched", "gmm"}
     hlr = run_hlr(args.steps, args.warmup)
     log(f"hlr: {hlr['elapsed']:.3f} s, min ESS {hlr['min_ess']:.1f}, R-hat {hlr['rhat_max']:.4f}")
     lda = run_lda() if "lda" in legs else None
     if lda:
         log(f"lda: {lda['tokens_per_s']:.3e} tokens/s")
+    gmm = run_gmm() if "gmm" in legs else None
+    if gmm:
+        log(f"gmm: {gmm}")
     surr = run_surrogate() if "surrogate" in legs else None
     bs = max(1, args.batched_sweeps)
     bat = run_batched(sweeps=bs, warmup=max(1, bs // 2)) if "batched" in legs else None
@@ -323,22 +487,26 @@ def main():
     bat16 = run_batched(sweeps=bs, warmup=max(1, bs // 2), chains=16) if "batched" in legs else None
     if bat16:
         log(f"batched (16 chains): {bat16}")
-    gmm = run_gmm() if "gmm" in legs else None
-    if gmm:
-        log(f"gmm: {gmm}")
-    value = hlr["min_ess"] / hlr["elapsed"]
+    ok = hlr["rhat_max"] < RHAT_MAX
+    raw = hlr["min_ess"] / hlr["elapsed"]
     N, D = hlr["N"], hlr["D"]
     bytes_per_pass = N * D * 4 + N * 4  # X (FP32) + y (int32): the algorithmic bytes
     if hlr["pass_launches_timed"] == 0:
-        raise SystemExit("bench: no pass launch was timed (steps must be at least 1)")
+        raise SystemExit("bench: no pass launch was timed")
     achieved = bytes_per_pass / (hlr["pass_avg_us"] * 1e-6) / 1e9
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(hlr, hlr["min_ess"] / args.steps)
+        cpu = cpu_hlr(hlr)
         log(f"cpu: {cpu['sample']}")
+    vs_plain = None
+    if surr:
+        vs_plain = {k: surr.get(k) for k in ("ratio_timed_incl_training", "ratio_excl_training",
+                                             "ratio_whole_job", "config")}
+        vs_plain["accelerated_min_ess_per_s"] = surr["surrogate"]["min_ess_per_s"]
+        vs_plain["plain_min_ess_per_s"] = surr["plain"]["min_ess_per_s"]
     out = {
-        "metric": "min-ESS/sec on 1 GPU (HLR HMC, R-hat<1.01); LDA tokens/sec",
-        "value": value,
+        "metric": "min-ESS/sec on 1 GPU (HLR HMC, R-hat<1.01) vs plain GPU sampler; LDA tokens/sec",
+        "value": raw if ok else None,
         "unit": "min-ESS/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -350,27 +518,33 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "HLR HMC: N=100000 rows, D=64, 4 chains, HMC [v] (*) HMC [beta] "
-                               "with 20 leapfrog steps of 0.005 (jittered as the reference), FP32 X",
-                   "global_batch": N, "chains": 4, "parallelism": "single GPU"},
+                               "with 20 leapfrog steps of 0.005 (jittered as the reference), FP32 X; "
+                               f"one step = {SWEEPS_PER_STEP} sweeps",
+                   "global_batch": N, "chains": 4, "sweeps_per_step": SWEEPS_PER_STEP,
+                   "parallelism": "single GPU"},
         "rhat_max": hlr["rhat_max"],
+        "rhat_ok": ok,
         "min_ess": hlr["min_ess"],
         "min_ess_tail": hlr["min_ess_tail"],
+        "min_ess_per_s_unchecked": raw,
+        "timed_sweeps": hlr["sweeps"],
         "accept_rate": hlr["accept"],
-        "rhat_ok": hlr["rhat_max"] < 1.01,
+        "vs_plain_gpu_sampler": vs_plain,
         "lda_tokens_per_s": lda["tokens_per_s"] if lda else None,
-        "lda": {k: lda[k] for k in ("tokens", "sweeps", "sweep_us", "zstep_avg_us")} if lda else None,
+        "lda": lda,
+        "gmm": gmm,
         "surrogate": surr,
         "batched": bat,
         "batched_16_chains": bat16,
-        "gmm": gmm,
         "roofline": {"kernel": "k_hlr_pass_r<64,4>", "bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": 1024.0 * (PASS_FETCH_KIB * FETCH_SCALE + PASS_WRITE_KIB * WRITE_SCALE),
-                     "avg_us": hlr["pass_avg_us"],
-                     "launches_timed": hlr["pass_launches_timed"]},
+                     "avg_us": hlr["pass_avg_us"], "launches_timed": hlr["pass_launches_timed"],
+                     "note": "X (25.6 MB) stays in the 256 MB Infinity Cache between passes: the "
+                             "pass is latency-bound (launch + one memory round), not HBM-bound"},
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
+    print(json.dumps(out, default=float), flush=True)
 
 
 if __name__ == "__main__":

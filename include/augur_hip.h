@@ -133,6 +133,8 @@ typedef struct {
   double min_accept;
   int window;
   uint32_t seed;     /* network initialisation and minibatch draws */
+  int linear_only;   /* 1: fit the least-squares linear term alone (no network,
+                        epochs ignored): the ablation of the network */
 } augur_surrogate_cfg;
 augur_status augur_hlr_surrogate(augur_sampler* s, const augur_surrogate_cfg* cfg);
 /* info[8] (info[9] when bytes allow): phase (0 off, 1 collecting,

@@ -139,6 +139,52 @@ def test_augur_infer_errors_without_gpu_work():
             infer.compile(2)()
 
 
+class _CountingSampler:
+    """Stands in for a runtime sampler: counts sweeps."""
+
+    def __init__(self):
+        self.sweeps = 0
+
+    def step(self, n=1):
+        self.sweeps += n
+
+    def close(self):
+        pass
+
+
+class _CountingPlan:
+    param_names = ["x"]
+
+    def __init__(self, sampler):
+        self.sampler = sampler
+
+    def get_point(self, chain):
+        return {"x": self.sampler.sweeps}
+
+
+def test_samplen_keeps_the_reference_sweeps():
+    # pyaugur/augurlib.py:195-222: numSamples * thin sweeps after burnIn, the
+    # state kept after sweep i when i % thin == 0
+    for burn, num, thin in ((2, 3, 4), (0, 5, 1), (1, 2, 3)):
+        with AugurInfer(None, LDA) as infer:
+            infer.aopt = AugurOpt(target="gpu")
+            infer.sampler = _CountingSampler()
+            infer.plan = _CountingPlan(infer.sampler)
+            infer.param_names = ["x"]
+            out = infer.samplen(burnIn=burn, numSamples=num, thin=thin)
+            ref = [burn + i + 1 for i in range(num * thin) if i % thin == 0]
+            assert out["x"] == ref
+            assert infer.sampler.sweeps == burn + num * thin
+            infer.sampler = None
+
+
+def test_augur_opt_defaults_match_the_reference():
+    # pyaugur/augurlib.py:39: AugurOpt(cached=False, target='cpu', initStrat=AugurRandom(), paramScale=None)
+    opt = AugurOpt()
+    assert opt.target == "cpu" and opt.cached is False and opt.paramScale is None
+    assert type(opt.initStrat).__name__ == "AugurRandom"
+
+
 # ---- the C-ABI library loads and exports every declared symbol ----
 
 def _declared_symbols():

@@ -99,14 +99,18 @@ def test_mvgmm_example():
     centers = [np.array([0.0, 2.0]), np.array([2.0, -2.0]), np.array([-2.0, -2.0])]
     y = np.array([rng.multivariate_normal(centers[n % K], covs) for n in range(N)])
     with AugurInfer("config.yml", MVGMM) as infer:
-        infer.set_compile_opt(AugurOpt(target="gpu", seed=4))
+        infer.set_compile_opt(AugurOpt(target="gpu", seed=4, chains=4))
         infer.set_user_sched("ConjGibbs [mu] (*) DiscGibbs [z]")
         infer.compile(K, N, mu0, covs0, pis, covs)(y)
         s = infer.samplen(burnIn=50, numSamples=20)
-        mus = np.stack(s["mu"][-1])
+        ll = infer.curr_ll()
+        # Gibbs from a random start can sit in a local mode (two components on
+        # one cluster); the chain of highest log density holds the global one,
+        # which puts one centre on each cluster (up to label order)
+        best = int(np.argmax(ll))
+        mus = np.stack(s["mu"][-1][best])
         found = sorted(map(tuple, np.round(mus)))
-        # a well-mixed start finds the three centres (up to label order)
-        assert found == sorted(map(tuple, np.round(np.array(centers)))) or len(set(found)) >= 2
+        assert found == sorted(map(tuple, np.round(np.array(centers)))), (found, ll)
 
 
 def test_multichain_and_set_pt():

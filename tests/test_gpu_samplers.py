@@ -315,10 +315,22 @@ def test_gmm_draws_do_not_depend_on_the_z_grid(rt, monkeypatch):
                 np.testing.assert_array_equal(a, b)
 
 
-@pytest.mark.parametrize("sample_pi,sample_cov", [(True, True), (False, False)])
-def test_gmm_sweeps_match_oracle(rt, oracle, sample_pi, sample_cov):
-    rng = np.random.default_rng(8)
-    n, d, k, C = 3000, 4, 3, 2
+@pytest.mark.parametrize("d,k,n,sample_pi,sample_cov", [
+    (4, 3, 3000, True, True), (4, 3, 3000, False, False),
+    # the benchmark's shape (padded dimension 16, 32 components: the
+    # matrix-core statistics kernel with two 16-component blocks, the 16 x 16
+    # FP64 conjugate draws and the matrix-core z step)
+    (16, 32, 6000, True, True),
+    (16, 20, 4000, True, True),   # 20 of 32 padded components live
+    (8, 40, 4000, True, True),    # padded dimension 8, 64 padded components
+])
+def test_gmm_sweeps_match_oracle(rt, oracle, d, k, n, sample_pi, sample_cov):
+    """Whole sweeps (statistics, conjugate draws of pis / mu / cov, z step)
+    against the oracle's sweep (cbits/augur_dist.c:526-538, 633-669, 740-757;
+    Core/CgConj.hs:171-246) on the same Philox draws: pis to 1e-9 relative
+    pins the counts, mu and cov the sums and scatter matrices."""
+    rng = np.random.default_rng(8 + d + k)
+    C = 2
     y, zt = _gmm_setup(rng, n, d, k)
     alpha, mu0, cov0, df, scale0 = np.ones(k), np.zeros(d), 10 * np.eye(d), d + 2, np.eye(d)
     s = rt.GmmSampler(y, alpha, mu0, cov0, df, scale0, chains=C, seed=9, sample_pi=sample_pi,

@@ -85,9 +85,9 @@ def _hlr_data(rng, n, d):
     return x, y
 
 
-def _run(rt, x, y, surrogate, seed, n_sweeps, train=300, hidden=64, fallback_cfg=None):
+def _run(rt, x, y, surrogate, seed, n_sweeps, train=300, hidden=64, fallback_cfg=None, eps=0.015):
     from augurv2_amd._lib import HMC_BLOCK_THETA, HMC_BLOCK_V
-    C, SIM, EPS = 4, 0.15, 0.015  # about 10 jittered leapfrog steps
+    C, SIM, EPS = 4, 0.15, eps  # 0.15 / eps jittered leapfrog steps
     d = x.shape[1]
     s = rt.HlrSampler(x, y, [(HMC_BLOCK_V, 1.0, 0.05, 0), (HMC_BLOCK_THETA, SIM, EPS, 0)],
                       chains=C, seed=seed, ref_compat=False)
@@ -131,26 +131,30 @@ def _compare(exact, surr, d, n):
     return ess_e, ess_s
 
 
-@pytest.mark.parametrize("d,hidden", [(16, 64), (64, 128)])
-def test_surrogate_hmc_is_exact_and_pays_off(gpu, d, hidden):
+@pytest.mark.parametrize("d,hidden,n_rows,eps", [(16, 64, 20000, 0.015), (64, 128, 20000, 0.015),
+                                                  (128, 256, 30000, 0.0075)])
+def test_surrogate_hmc_is_exact_and_pays_off(gpu, d, hidden, n_rows, eps):
     """d = 16: the generic trajectory kernel and FP64 log-density; d = 64,
     hidden = 128: the register-resident BF16-dot trajectory kernel and the
-    FP64 matrix-core log-density."""
+    FP64 matrix-core log-density; d = 128, hidden = 256: the trajectory
+    kernel of the N = 1M, D = 128 benchmark (layer 2 in LDS), with half the
+    step size (at 0.015 exact HMC itself accepts under half its proposals
+    there)."""
     _, rt = gpu
     rng = np.random.default_rng(2)
-    x, y = _hlr_data(rng, 20000, d)
+    x, y = _hlr_data(rng, n_rows, d)
     n = 2000
-    exact, _, acc_e = _run(rt, x, y, False, 11, n)
-    surr, info, acc_s = _run(rt, x, y, True, 12, n, hidden=hidden)
+    exact, _, acc_e = _run(rt, x, y, False, 11, n, eps=eps)
+    surr, info, acc_s = _run(rt, x, y, True, 12, n, hidden=hidden, eps=eps)
     assert info["phase"] == 2 and info["fallbacks"] == 0, (info, acc_e, acc_s)
     assert info["window_accept"] >= 0.6, info
     # near the posterior the logistic gradient is close to linear in theta
     assert 0.0 < info["resid_frac"] < 0.2, info
     ess_e, ess_s = _compare(exact, surr, d, n)
     # the gain: effective draws per exact data pass.  Exact HMC spends about
-    # 10 passes per sweep (the jittered leapfrog count); surrogate HMC one
-    # (the log-density of the proposal).
-    assert ess_s >= 3.0 * ess_e / 10, (ess_s, ess_e, info)
+    # 0.15 / eps passes per sweep (the jittered leapfrog count); surrogate HMC
+    # one (the log-density of the proposal).
+    assert ess_s >= 3.0 * ess_e * eps / 0.15, (ess_s, ess_e, info)
 
 
 def test_failing_surrogate_trajectories_fall_back_to_exact_hmc(gpu):
@@ -165,3 +169,46 @@ def test_failing_surrogate_trajectories_fall_back_to_exact_hmc(gpu):
     assert info["window_accept"] < 0.6
     assert np.isfinite(draws).all()
     assert acc[1].mean() > 0.3  # exact HMC took over
+
+
+def test_set_state_after_training_refreshes_the_exact_density(gpu):
+    """The surrogate block's Metropolis test uses the FP64 data log-density of
+    the current state.  After set_pt moves every chain 3 posterior standard
+    deviations off the mean (a state of far lower density), proposals back
+    toward the mean must be accepted: judged against the density of the
+    state before the move, they would all be rejected."""
+    from augurv2_amd import _lib
+    _, rt = gpu
+    rng = np.random.default_rng(5)
+    d = 16
+    x, y = _hlr_data(rng, 5000, d)
+    from augurv2_amd._lib import HMC_BLOCK_THETA, HMC_BLOCK_V
+    C = 4
+    s = rt.HlrSampler(x, y, [(HMC_BLOCK_V, 1.0, 0.05, 0), (HMC_BLOCK_THETA, 0.15, 0.015, 0)],
+                      chains=C, seed=21, ref_compat=False)
+    s.set("state", np.concatenate([np.zeros(d), [0.0], [1.0]]))
+    s.step(200)
+    s.enable_surrogate(1, hidden=64, train_sweeps=200, max_pairs=100_000, epochs=20, batch=256,
+                       lr=2e-3, min_accept=0.0, window=1000, seed=3)
+    s.step(200)
+    s.trace("state", 300)
+    s.step(300)
+    draws = s.read_trace("state")[:, :, :d].reshape(-1, d)
+    assert s.surrogate_info()["phase"] == 2
+    mean, sd = draws.mean(axis=0), draws.std(axis=0)
+    for c in range(C):
+        st = s.get("state", c)
+        st[:d] = mean + 3.0 * sd
+        s.set("state", st, chain=c)
+
+    def counts():
+        out = np.zeros((2, C, 2))
+        _lib.call("augur_stats", s._h, _lib.ptr(out), out.nbytes)
+        return out[1]
+
+    before = counts()
+    s.step(5)
+    acc = counts() - before
+    s.close()
+    assert acc[:, 1].sum() == 5 * C
+    assert acc[:, 0].sum() >= 0.5 * 5 * C, acc
