@@ -85,11 +85,14 @@ def _hlr_data(rng, n, d):
     return x, y
 
 
-def _run(rt, x, y, surrogate, seed, n_sweeps, train=300, hidden=64, fallback_cfg=None, eps=0.015):
-    from augurv2_amd._lib import HMC_BLOCK_THETA, HMC_BLOCK_V
+def _run(rt, x, y, surrogate, seed, n_sweeps, train=300, hidden=64, fallback_cfg=None, eps=0.015,
+         v_mwg=0.0):
+    from augurv2_amd._lib import HMC_BLOCK_THETA, HMC_BLOCK_V, MWG_BLOCK_V
     C, SIM, EPS = 4, 0.15, eps  # 0.15 / eps jittered leapfrog steps
     d = x.shape[1]
-    s = rt.HlrSampler(x, y, [(HMC_BLOCK_V, 1.0, 0.05, 0), (HMC_BLOCK_THETA, SIM, EPS, 0)],
+    # v by HMC, or by MWG [v] ~ Normal(v, v_mwg) when v_mwg > 0
+    vblk = (MWG_BLOCK_V, 0.0, v_mwg, 0) if v_mwg > 0 else (HMC_BLOCK_V, 1.0, 0.05, 0)
+    s = rt.HlrSampler(x, y, [vblk, (HMC_BLOCK_THETA, SIM, EPS, 0)],
                       chains=C, seed=seed, ref_compat=False)
     for c in range(C):
         s.set("state", np.concatenate([np.zeros(d), [0.0], [1.0]]), chain=c)
@@ -131,21 +134,23 @@ def _compare(exact, surr, d, n):
     return ess_e, ess_s
 
 
-@pytest.mark.parametrize("d,hidden,n_rows,eps", [(16, 64, 20000, 0.015), (64, 128, 20000, 0.015),
-                                                  (128, 256, 30000, 0.0075)])
-def test_surrogate_hmc_is_exact_and_pays_off(gpu, d, hidden, n_rows, eps):
+@pytest.mark.parametrize("d,hidden,n_rows,eps,v_mwg", [(16, 64, 20000, 0.015, 0.0),
+                                                        (64, 128, 20000, 0.015, 0.0),
+                                                        (128, 256, 30000, 0.0075, 0.002)])
+def test_surrogate_hmc_is_exact_and_pays_off(gpu, d, hidden, n_rows, eps, v_mwg):
     """d = 16: the generic trajectory kernel and FP64 log-density; d = 64,
     hidden = 128: the register-resident BF16-dot trajectory kernel and the
     FP64 matrix-core log-density; d = 128, hidden = 256: the trajectory
     kernel of the N = 1M, D = 128 benchmark (layer 2 in LDS), with half the
     step size (at 0.015 exact HMC itself accepts under half its proposals
-    there)."""
+    there) and v by MWG (HMC on v accepts ~0.2% there and v does not mix in
+    2000 sweeps)."""
     _, rt = gpu
     rng = np.random.default_rng(2)
     x, y = _hlr_data(rng, n_rows, d)
     n = 2000
-    exact, _, acc_e = _run(rt, x, y, False, 11, n, eps=eps)
-    surr, info, acc_s = _run(rt, x, y, True, 12, n, hidden=hidden, eps=eps)
+    exact, _, acc_e = _run(rt, x, y, False, 11, n, eps=eps, v_mwg=v_mwg)
+    surr, info, acc_s = _run(rt, x, y, True, 12, n, hidden=hidden, eps=eps, v_mwg=v_mwg)
     assert info["phase"] == 2 and info["fallbacks"] == 0, (info, acc_e, acc_s)
     assert info["window_accept"] >= 0.6, info
     # near the posterior the logistic gradient is close to linear in theta
