@@ -107,6 +107,7 @@ SIGNATURES = {
     "augur_op_lda_zstep": [_I, _I, _I, _P, _P, _P, _P, _U, _U, _U, _P, _P, _P],
     "augur_op_hlr_ll_grad": [_I, _I, _I, _P, _P, _P, _I, _P, _P],
     "augur_op_hlr_ll64": [_I, _I, _I, _P, _P, _P, _I, _P],
+    "augur_op_hlr_ll_grad_noalloc": [_I, _I, _I, _P, _P, _P, _I, _P, _P, _P],
     "augur_op_gmm_zstep": [_I, _I, _I, _P, _P, _P, _P, _U, _U, _U, _P],
     "augur_op_gmm_conj": [_I, _I, _P, _P, _P, _I, _P, _P, _P, _P, _I, _I, _U, _U, _U, _P, _P,
                           _P],

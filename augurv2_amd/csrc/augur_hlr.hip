@@ -30,6 +30,7 @@
 // iteration of L leapfrog steps costs L data passes instead of the
 // reference's L + 3 (L + 1 gradients and 2 log-densities).
 #include <chrono>
+#include <mutex>
 
 #include "augur_hmc.h"
 #include "augur_mlp.h"
@@ -1260,6 +1261,58 @@ static void validate_shapes(int n, int d, int chains) {
     fail(AUGUR_ERR_ARG, "d * chains (padded to a power of two) must be <= %d", kMaxGradSlots * kPassThreads);
 }
 
+// ---- the data term without allocation (augur_op_hlr_ll_grad_noalloc) ----
+// Its scratch: the FP32 evaluation points (cp x (d + 1)), the per-chain sums
+// (cp x (d + 1) gradients, cp log-densities) and the pass's block partials
+// (G x cp x (d + 2)), at 256-byte aligned offsets.  One copy per device,
+// held until the process ends; host code reaches it only through the
+// address hipGetSymbolAddress returns for the current device.
+constexpr size_t kNoAllocScratchBytes = size_t(4) << 20;
+constexpr int kNoAllocMaxBlocks = 512;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+static size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+// The scratch's device address on the current device, looked up once per
+// device with that device current.
+static unsigned char* noalloc_scratch() {
+  static std::mutex mu;
+  static std::map<int, unsigned char*> ptrs;
+  int dev = 0;
+  AUGUR_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = ptrs.find(dev);
+  if (it != ptrs.end()) return it->second;
+  void* ptr = nullptr;
+  AUGUR_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  if (!ptr) fail(AUGUR_ERR_HIP, "no device address for the scratch array");
+  ptrs[dev] = (unsigned char*)ptr;
+  return (unsigned char*)ptr;
+}
+
+// beta (chains x (d + 1), the caller's layout) -> FP32 points cp x (d + 1),
+// zero for padded chains and for b without an intercept
+__global__ void k_beta_to_beta32(const double* __restrict__ beta, int d, int chains, int cp, int has_b,
+                                 float* __restrict__ beta32) {
+  const int ds = d + 1;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < cp * ds; q += gridDim.x * blockDim.x) {
+    const int c = q / ds, j = q % ds;
+    float v = 0.0f;
+    if (c < chains && (j < d || has_b)) v = (float)beta[q];
+    beta32[q] = v;
+  }
+}
+
+// per-chain sums (cp rows) -> the caller's ll (chains) and grad (chains x (d + 1))
+__global__ void k_llgrad_out(const double* __restrict__ cache_g, const double* __restrict__ cache_ll, int d,
+                             int chains, double* __restrict__ ll, double* __restrict__ grad) {
+  const int ds = d + 1;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < chains * ds; q += gridDim.x * blockDim.x) {
+    grad[q] = cache_g[q];
+    if (q % ds == 0) ll[q / ds] = cache_ll[q / ds];
+  }
+}
+
 }  // namespace augur
 
 using namespace augur;
@@ -1341,6 +1394,42 @@ augur_status augur_op_hlr_ll_grad(int n_rows, int d, int chains, const float* x,
       ll[c] = l[c];
       for (int j = 0; j <= d; ++j) grad[(size_t)c * (d + 1) + j] = g[(size_t)c * (d + 1) + j];
     }
+  });
+}
+
+augur_status augur_op_hlr_ll_grad_noalloc(int n_rows, int d, int chains, const float* x, const int32_t* y,
+                                          const double* beta, int has_intercept, double* ll, double* grad,
+                                          void* stream) {
+  return guard([&] {
+    validate_shapes(n_rows, d, chains);
+    if (!x || !y || !beta || !ll || !grad) fail(AUGUR_ERR_ARG, "null argument");
+    if ((reinterpret_cast<uintptr_t>(x) & 15) != 0) fail(AUGUR_ERR_ARG, "x must be 16-byte aligned");
+    const int cp = next_pow2(chains), ds = d + 1;
+    const size_t o_g = align256(sizeof(float) * cp * ds);
+    const size_t o_ll = o_g + align256(sizeof(double) * cp * ds);
+    const size_t o_part = o_ll + align256(sizeof(double) * cp);
+    const size_t per_block = sizeof(double) * cp * (d + 2);
+    if (o_part + per_block > kNoAllocScratchBytes)
+      fail(AUGUR_ERR_ARG, "d = %d, chains = %d need more than the %zu-byte scratch", d, chains,
+           kNoAllocScratchBytes);
+    unsigned char* base = noalloc_scratch();
+    const int tiles = (n_rows + kTileRows - 1) / kTileRows;
+    const int G = (int)std::min<size_t>(std::min(tiles, kNoAllocMaxBlocks), (kNoAllocScratchBytes - o_part) / per_block);
+    PassArgs A{};
+    A.x = x; A.y = y; A.n = n_rows; A.d = d; A.cp = cp; A.c_real = chains; A.has_b = has_intercept ? 1 : 0;
+    A.beta32 = (float*)base; A.cache_g = (double*)(base + o_g); A.cache_ll = (double*)(base + o_ll);
+    A.part = (double*)(base + o_part); A.G = G; A.mode = MODE_CACHE;
+    hipStream_t st = (hipStream_t)stream;
+    k_beta_to_beta32<<<8, 256, 0, st>>>(beta, d, chains, cp, A.has_b, A.beta32);
+    AUGUR_CHECK_LAUNCH();
+    const size_t smem = pass_smem_bytes(d, cp);
+    allow_lds(k_hlr_pass, smem);
+    k_hlr_pass<<<G, kPassThreads, smem, st>>>(A);
+    AUGUR_CHECK_LAUNCH();
+    k_hlr_finish<<<(cp * (d + 2) + kFinVals - 1) / kFinVals, 1024, 0, st>>>(A);
+    AUGUR_CHECK_LAUNCH();
+    k_llgrad_out<<<8, 256, 0, st>>>(A.cache_g, A.cache_ll, d, chains, ll, grad);
+    AUGUR_CHECK_LAUNCH();
   });
 }
 

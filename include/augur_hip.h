@@ -287,6 +287,24 @@ augur_status augur_op_lda_zstep(int n_docs, int k, int v, const int32_t* doc_off
 augur_status augur_op_hlr_ll_grad(int n_rows, int d, int chains, const float* x,
                                   const int32_t* y, const double* beta, int has_intercept,
                                   double* ll, double* grad);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the data term of augur_op_hlr_ll_grad on buffers the
+ * caller owns in device memory, queued on `stream` (a hipStream_t; null: the
+ * default stream) of the current device, with no allocation.  x (N x D FP32,
+ * 16-byte aligned), y (N labels in {0, 1}), beta (chains x (D + 1) f64) in,
+ * ll (chains) and grad (chains x (D + 1)) out, as augur_op_hlr_ll_grad.  The
+ * kernels' scratch (FP32 evaluation points, block partials, per-chain sums)
+ * is a library-owned static device array of 4 MiB per device, kept until
+ * the process ends; every entry is written before it is read, so no call
+ * depends on an earlier one.  Calls that overlap on one device (other
+ * streams, other threads) race on that array.  Shapes as
+ * augur_op_hlr_ll_grad (D <= 256, chains <= 16); a shape whose scratch would
+ * exceed the array is rejected with AUGUR_ERR_ARG.  Returns once the kernels
+ * are queued; the results are ready when the stream reaches them. */
+augur_status augur_op_hlr_ll_grad_noalloc(int n_rows, int d, int chains, const float* x,
+                                          const int32_t* y, const double* beta, int has_intercept,
+                                          double* ll, double* grad, void* stream);
 /* The same data log-density in FP64 row arithmetic (FP64 matrix cores for
  * D in {16, 32, 64, 128} without intercept): the Metropolis term of
  * surrogate HMC.  Same arguments as augur_op_hlr_ll_grad, no gradient. */
