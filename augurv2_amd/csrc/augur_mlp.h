@@ -3,7 +3,9 @@
 //
 // Parameters live in one FP32 vector: W1 (H x D), b1 (H), W2 (H x H), b2 (H),
 // W3 (D x H), b3 (D).  Inputs and targets are standardised per column; the
-// network predicts the standardised target.
+// network predicts the standardised target.  Training products take BF16
+// operands (rounded to nearest even) with FP32 accumulation; biases,
+// activations, losses and the Adam state stay FP32.
 #pragma once
 #include "augur_common.h"
 
@@ -26,6 +28,7 @@ struct Mlp {
   int batch = 0;
   DevBuf<float> params, grads, m1, m2;
   DevBuf<float> z1, h1, z2, h2, out, dout, dh, dz;
+  DevBuf<float> kpart;  // split-k partial tiles of the training GEMMs
   DevBuf<double> loss;  // one value per launch of the loss kernel
   int adam_t = 0;
 
@@ -35,6 +38,10 @@ struct Mlp {
   // forward + MSE loss against t + gradients into `grads`; loss in loss.p[0]
   void loss_grad(const float* x, const float* t, int rows, hipStream_t st);
   void adam(double lr, hipStream_t st);
+  // C = A B (+ bias, GELU) with BF16 operands and FP32 accumulation on the
+  // matrix cores; strides as gemm_f32
+  void mm(hipStream_t st, int M, int N, int K, const float* A, long sam, long sak, const float* B, long sbk,
+          long sbn, float* C, int ldc, const float* bias = nullptr, float* zout = nullptr, int act = 0);
 };
 
 // C (M x N, row stride ldc) = A B with A(m, k) = A[m sam + k sak] and

@@ -1,6 +1,8 @@
 // augur_mlp.hip -- training and batched evaluation of the gradient-surrogate
-// MLP: a strided tiled FP32 GEMM with fused bias / GELU epilogues, GELU
-// backward, column sums, MSE loss and Adam, all hand-written.
+// MLP: a BF16 matrix-core GEMM (FP32 accumulation) with fused bias / GELU
+// epilogues for the network, a strided tiled FP32 GEMM for the linear
+// term's residual, GELU backward, column sums, MSE loss and Adam, all
+// hand-written.
 #include "augur_mlp.h"
 
 namespace augur {
@@ -87,6 +89,142 @@ static void gemm(hipStream_t st, int M, int N, int K, const float* A, long sam, 
   GemmArgs g{M, N, K, A, sam, sak, B, sbk, sbn, Cm, ldc, bias, zout, act};
   k_gemm<<<dim3((N + 63) / 64, (M + 63) / 64), 256, 0, st>>>(g);
   AUGUR_CHECK_LAUNCH();
+}
+
+// ---------------------------------------------------------------------------
+// The network's training GEMMs on the matrix cores: C = A B (+ bias, GELU)
+// with both operands rounded to BF16 (nearest even) as they are staged and
+// FP32 accumulation in V_MFMA_F32_16X16X32_BF16.  A 256-thread block owns a
+// 64 x 64 tile of C, each wave a 32 x 32 quarter (2 x 2 products of 16 x 16);
+// k advances 32 at a time through LDS.  Operand layouts of the product (lane
+// l, i = l % 16, g = l / 16): A supplies row i, k = 8 g .. 8 g + 7; B column
+// i, the same k; the result is C[4 g + r][i] in register r.  Long-k products
+// with few tiles (the weight gradients, k = batch rows) split k over
+// blockIdx.z: each slice writes its partial tile and k_gemm_splitk_sum adds
+// the slices in order, so the result does not depend on scheduling.
+typedef __bf16 mlp_bf16x8 __attribute__((ext_vector_type(8)));
+typedef float mlp_f32x4 __attribute__((ext_vector_type(4)));
+constexpr int kMmTile = 64, kMmK = 32, kMmLd = kMmK + 8;  // LDS row: 40 BF16 = 80 bytes
+
+__device__ inline float gemm_epilogue(const GemmArgs& g, int m, int n, float v) {
+  v += g.bias ? g.bias[n] : 0.0f;
+  if (g.act) {
+    g.zout[(size_t)m * g.ldc + n] = v;
+    v = gelu_f(v);
+  }
+  return v;
+}
+
+__global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g, int kper, float* __restrict__ part) {
+  __shared__ __align__(16) __bf16 As[kMmTile][kMmLd];  // [m][k]
+  __shared__ __align__(16) __bf16 Bs[kMmTile][kMmLd];  // [n][k]
+  const int tm = blockIdx.y * kMmTile, tn = blockIdx.x * kMmTile;
+  const int k_beg = blockIdx.z * kper, k_end = min(g.K, k_beg + kper);
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int li = lane & 15, kg = lane >> 4;
+  const int wm = (wv >> 1) * 32, wn = (wv & 1) * 32;
+  // staging maps: along k when k is the contiguous index, else along m / n
+  const bool a_k = g.sak == 1 || g.sam != 1;
+  const bool b_k = g.sbk == 1 || g.sbn != 1;
+  mlp_f32x4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = mlp_f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int k0 = k_beg; k0 < k_end; k0 += kMmK) {
+    if (a_k) {  // thread: row m, k = k0 + 8 q .. + 7
+      const int m = tid >> 2, q = (tid & 3) * 8;
+      mlp_bf16x8 v;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int k = k0 + q + e;
+        v[e] = (__bf16)((tm + m < g.M && k < k_end) ? g.A[(size_t)(tm + m) * g.sam + (size_t)k * g.sak] : 0.0f);
+      }
+      *(mlp_bf16x8*)&As[m][q] = v;
+    } else {  // thread: k = k0 + kk, rows m0 .. m0 + 7 (contiguous in memory)
+      const int kk = tid >> 3, m0 = (tid & 7) * 8, k = k0 + kk;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int m = tm + m0 + e;
+        As[m0 + e][kk] = (__bf16)((m < g.M && k < k_end) ? g.A[(size_t)m * g.sam + (size_t)k * g.sak] : 0.0f);
+      }
+    }
+    if (b_k) {  // thread: column n, k = k0 + 8 q .. + 7
+      const int n = tid >> 2, q = (tid & 3) * 8;
+      mlp_bf16x8 v;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int k = k0 + q + e;
+        v[e] = (__bf16)((tn + n < g.N && k < k_end) ? g.B[(size_t)k * g.sbk + (size_t)(tn + n) * g.sbn] : 0.0f);
+      }
+      *(mlp_bf16x8*)&Bs[n][q] = v;
+    } else {  // thread: k = k0 + kk, columns n0 .. n0 + 7 (contiguous in memory)
+      const int kk = tid >> 3, n0 = (tid & 7) * 8, k = k0 + kk;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int n = tn + n0 + e;
+        Bs[n0 + e][kk] = (__bf16)((n < g.N && k < k_end) ? g.B[(size_t)k * g.sbk + (size_t)n * g.sbn] : 0.0f);
+      }
+    }
+    __syncthreads();
+    mlp_bf16x8 a[2], b[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) a[i] = *(const mlp_bf16x8*)&As[wm + 16 * i + li][8 * kg];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) b[j] = *(const mlp_bf16x8*)&Bs[wn + 16 * j + li][8 * kg];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    __syncthreads();
+  }
+  const bool split = gridDim.z > 1;
+  float* pz = split ? part + (size_t)blockIdx.z * g.M * g.N : nullptr;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = tm + wm + 16 * i + 4 * kg + r, n = tn + wn + 16 * j + li;
+        if (m >= g.M || n >= g.N) continue;
+        if (split) pz[(size_t)m * g.N + n] = acc[i][j][r];
+        else g.C[(size_t)m * g.ldc + n] = gemm_epilogue(g, m, n, acc[i][j][r]);
+      }
+}
+
+__global__ void k_gemm_splitk_sum(GemmArgs g, int slices, const float* __restrict__ part) {
+  const size_t mn = (size_t)g.M * g.N;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < mn; e += (size_t)gridDim.x * blockDim.x) {
+    float s = 0.0f;
+    for (int z = 0; z < slices; ++z) s += part[(size_t)z * mn + e];
+    const int m = (int)(e / g.N), n = (int)(e % g.N);
+    g.C[(size_t)m * g.ldc + n] = gemm_epilogue(g, m, n, s);
+  }
+}
+
+// Slices of k for a product of M x N x K: enough blocks to cover the chip
+// when the tiles alone do not, at least 256 of k per slice, and the partial
+// tiles within `cap` floats.
+static int splitk_slices(int M, int N, int K, size_t cap) {
+  const int tiles = ((M + kMmTile - 1) / kMmTile) * ((N + kMmTile - 1) / kMmTile);
+  int s = 1;
+  while (tiles * s < 256 && K / (2 * s) >= 256 && (size_t)(2 * s) * M * N <= cap) s *= 2;
+  return s;
+}
+
+void Mlp::mm(hipStream_t st, int M, int N, int K, const float* A, long sam, long sak, const float* B, long sbk,
+             long sbn, float* Cm, int ldc, const float* bias, float* zout, int act) {
+  GemmArgs g{M, N, K, A, sam, sak, B, sbk, sbn, Cm, ldc, bias, zout, act};
+  const int slices = splitk_slices(M, N, K, kpart.n);
+  const int kper = ((K + slices - 1) / slices + kMmK - 1) / kMmK * kMmK;
+  const dim3 grid((N + kMmTile - 1) / kMmTile, (M + kMmTile - 1) / kMmTile, slices);
+  k_gemm_bf16<<<grid, 256, 0, st>>>(g, kper, kpart.p);
+  AUGUR_CHECK_LAUNCH();
+  if (slices > 1) {
+    k_gemm_splitk_sum<<<std::min<size_t>(1024, ((size_t)M * N + 255) / 256), 256, 0, st>>>(g, slices, kpart.p);
+    AUGUR_CHECK_LAUNCH();
+  }
 }
 
 void gemm_f32(hipStream_t st, int M, int N, int K, const float* A, long sam, long sak, const float* B,
@@ -179,6 +317,9 @@ void Mlp::init(int d, int h, int batch_rows, uint32_t seed, hipStream_t st) {
   z2.alloc((size_t)batch * h); h2.alloc((size_t)batch * h);
   out.alloc((size_t)batch * d); dout.alloc((size_t)batch * d);
   dh.alloc((size_t)batch * h); dz.alloc((size_t)batch * h);
+  // split-k partial tiles of the weight gradients: up to 16 slices of the
+  // largest of them (H x H, D x H)
+  kpart.alloc(16 * (size_t)std::max(h * h, d * h));
   loss.alloc(1 + kMseBlocks);  // [0] the loss, then the blocks' partial sums
   // Uniform(-1/sqrt(fan_in), 1/sqrt(fan_in)) weights from Philox, zero biases
   std::vector<float> hp(P, 0.0f);
@@ -201,9 +342,9 @@ void Mlp::init(int d, int h, int batch_rows, uint32_t seed, hipStream_t st) {
 void Mlp::forward(const float* x, int rows, hipStream_t st) {
   const int d = s.d, h = s.h;
   const float* P = params.p;
-  gemm(st, rows, h, d, x, d, 1, P + s.w1(), 1, d, h1.p, h, P + s.b1(), z1.p, 1);
-  gemm(st, rows, h, h, h1.p, h, 1, P + s.w2(), 1, h, h2.p, h, P + s.b2(), z2.p, 1);
-  gemm(st, rows, d, h, h2.p, h, 1, P + s.w3(), 1, h, out.p, d, P + s.b3());
+  mm(st, rows, h, d, x, d, 1, P + s.w1(), 1, d, h1.p, h, P + s.b1(), z1.p, 1);
+  mm(st, rows, h, h, h1.p, h, 1, P + s.w2(), 1, h, h2.p, h, P + s.b2(), z2.p, 1);
+  mm(st, rows, d, h, h2.p, h, 1, P + s.w3(), 1, h, out.p, d, P + s.b3());
 }
 
 void Mlp::loss_grad(const float* x, const float* t, int rows, hipStream_t st) {
@@ -215,17 +356,17 @@ void Mlp::loss_grad(const float* x, const float* t, int rows, hipStream_t st) {
   k_mse_fin<<<1, 1, 0, st>>>(loss.p + 1, (size_t)rows * d, loss.p);
   AUGUR_CHECK_LAUNCH();
   // layer 3: dW3 = dout^T h2, db3 = colsum(dout), dh2 = dout W3
-  gemm(st, d, h, rows, dout.p, 1, d, h2.p, h, 1, Gp + s.w3(), h);
+  mm(st, d, h, rows, dout.p, 1, d, h2.p, h, 1, Gp + s.w3(), h);
   k_colsum<<<(d + 63) / 64, 1024, 0, st>>>(dout.p, rows, d, Gp + s.b3());
-  gemm(st, rows, h, d, dout.p, d, 1, P + s.w3(), h, 1, dh.p, h);
+  mm(st, rows, h, d, dout.p, d, 1, P + s.w3(), h, 1, dh.p, h);
   k_gelu_bwd<<<256, 256, 0, st>>>(dh.p, z2.p, dz.p, (size_t)rows * h);
   // layer 2
-  gemm(st, h, h, rows, dz.p, 1, h, h1.p, h, 1, Gp + s.w2(), h);
+  mm(st, h, h, rows, dz.p, 1, h, h1.p, h, 1, Gp + s.w2(), h);
   k_colsum<<<(h + 63) / 64, 1024, 0, st>>>(dz.p, rows, h, Gp + s.b2());
-  gemm(st, rows, h, h, dz.p, h, 1, P + s.w2(), h, 1, dh.p, h);
+  mm(st, rows, h, h, dz.p, h, 1, P + s.w2(), h, 1, dh.p, h);
   k_gelu_bwd<<<256, 256, 0, st>>>(dh.p, z1.p, dz.p, (size_t)rows * h);
   // layer 1
-  gemm(st, h, d, rows, dz.p, 1, h, x, d, 1, Gp + s.w1(), d);
+  mm(st, h, d, rows, dz.p, 1, h, x, d, 1, Gp + s.w1(), d);
   k_colsum<<<(h + 63) / 64, 1024, 0, st>>>(dz.p, rows, h, Gp + s.b1());
   AUGUR_CHECK_LAUNCH();
 }

@@ -24,29 +24,49 @@ def gelu_grad(z):
     return 0.5 * (1.0 + t) + 0.5 * z * (1.0 - t * t) * du
 
 
-def forward(p, x):
-    z1 = x @ p["W1"].T + p["b1"]
+def bf16_round(a):
+    """Each value rounded to the nearest BF16 (8 significand bits), ties to
+    even, as FP64: the rounding of IEEE 754 round-to-nearest-even applied to
+    the FP32 value, done on its bit pattern (add 0x7FFF plus the lowest kept
+    bit, clear the low 16 bits)."""
+    f = np.ascontiguousarray(a, dtype=np.float32)
+    u = f.view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000).astype(np.uint32)
+    return u.view(np.float32).astype(np.float64)
+
+
+def _ident(a):
+    return a
+
+
+def forward(p, x, rnd=None):
+    """rnd: applied to both operands of every product (None: exact), e.g.
+    bf16_round for the device's matrix-core training products."""
+    r = rnd or _ident
+    z1 = r(x) @ r(p["W1"]).T + p["b1"]
     h1 = gelu(z1)
-    z2 = h1 @ p["W2"].T + p["b2"]
+    z2 = r(h1) @ r(p["W2"]).T + p["b2"]
     h2 = gelu(z2)
-    out = h2 @ p["W3"].T + p["b3"]
+    out = r(h2) @ r(p["W3"]).T + p["b3"]
     return out, (x, z1, h1, z2, h2)
 
 
-def loss_and_grads(p, x, t):
-    """MSE over all B x D outputs and its gradient w.r.t. every parameter."""
-    out, (x, z1, h1, z2, h2) = forward(p, x)
+def loss_and_grads(p, x, t, rnd=None):
+    """MSE over all B x D outputs and its gradient w.r.t. every parameter;
+    rnd as forward (the bias sums are not products and stay exact)."""
+    r = rnd or _ident
+    out, (x, z1, h1, z2, h2) = forward(p, x, rnd)
     diff = out - t
     loss = np.mean(diff**2)
     dout = 2.0 * diff / diff.size
-    g = {"W3": dout.T @ h2, "b3": dout.sum(0)}
-    dh2 = dout @ p["W3"]
+    g = {"W3": r(dout).T @ r(h2), "b3": dout.sum(0)}
+    dh2 = r(dout) @ r(p["W3"])
     dz2 = dh2 * gelu_grad(z2)
-    g["W2"] = dz2.T @ h1
+    g["W2"] = r(dz2).T @ r(h1)
     g["b2"] = dz2.sum(0)
-    dh1 = dz2 @ p["W2"]
+    dh1 = r(dz2) @ r(p["W2"])
     dz1 = dh1 * gelu_grad(z1)
-    g["W1"] = dz1.T @ x
+    g["W1"] = r(dz1).T @ r(x)
     g["b1"] = dz1.sum(0)
     return loss, g
 

@@ -34,10 +34,12 @@ def _unflat(v, d, h):
     return out
 
 
-def test_mlp_loss_and_gradient_match_oracle(gpu):
+@pytest.mark.parametrize("d,h,rows", [(16, 48, 300), (128, 256, 4096)])
+def test_mlp_loss_and_gradient_match_oracle(gpu, d, h, rows):
+    """Ragged tiles (16, 48, 300) and the benchmark's network on one
+    training batch (128, 256, 4096 rows: the weight gradients split k)."""
     ops, _ = gpu
     rng = np.random.default_rng(0)
-    d, h, rows = 16, 48, 300
     p = omlp.init_params(d, h, rng)
     for k in ("b1", "b2", "b3"):
         p[k] = 0.1 * rng.standard_normal(p[k].shape)
@@ -45,14 +47,26 @@ def test_mlp_loss_and_gradient_match_oracle(gpu):
     t = rng.standard_normal((rows, d)).astype(np.float32)
     p32 = _unflat(_flat(p), d, h)  # the FP32-rounded parameters the GPU sees
     out, loss, g = ops.mlp_loss_grad(_flat(p), x, t, d, h)
-    ref_out, _ = omlp.forward(p32, x.astype(np.float64))
-    ref_loss, ref_g = omlp.loss_and_grads(p32, x.astype(np.float64), t.astype(np.float64))
-    np.testing.assert_allclose(out, ref_out, rtol=1e-4, atol=1e-5)
-    assert abs(loss - ref_loss) <= 1e-5 * ref_loss
+    # the device's products take BF16 operands with FP32 accumulation: the
+    # restatement rounds the same operands.  What remains is FP32 rounding
+    # (accumulation order, FP32 activations), plus the rare operand whose FP32
+    # and FP64 values round to neighbouring BF16 values (one BF16 step, 2^-8,
+    # in one term of a sum): hence 2e-3 of each tensor's scale.
+    rnd = omlp.bf16_round
+    ref_out, _ = omlp.forward(p32, x.astype(np.float64), rnd)
+    ref_loss, ref_g = omlp.loss_and_grads(p32, x.astype(np.float64), t.astype(np.float64), rnd)
+    assert np.abs(out - ref_out).max() <= 2e-3 * np.abs(ref_out).max()
+    assert abs(loss - ref_loss) <= 1e-4 * ref_loss
     gg = _unflat(g, d, h)
     for k in ref_g:
         scale = np.abs(ref_g[k]).max()
-        assert np.abs(gg[k] - ref_g[k]).max() <= 1e-4 * scale + 1e-7, k
+        assert np.abs(gg[k] - ref_g[k]).max() <= 2e-3 * scale + 1e-7, k
+    # against the exact (unrounded) network the BF16 products are a
+    # percent-level approximation, no more
+    ex_loss, ex_g = omlp.loss_and_grads(p32, x.astype(np.float64), t.astype(np.float64))
+    assert abs(loss - ex_loss) <= 2e-2 * ex_loss
+    for k in ex_g:
+        assert np.abs(gg[k] - ex_g[k]).max() <= 5e-2 * np.abs(ex_g[k]).max(), k
 
 
 def test_mlp_adam_step_matches_oracle_and_training_converges(gpu, oracle):
@@ -68,8 +82,9 @@ def test_mlp_adam_step_matches_oracle_and_training_converges(gpu, oracle):
     # (seed, chain 0) at counter (row, 0, step 0, tag 900)
     idx = [int((int(oracle.draw4(5, 0, r, 0, 0, 900)[0]) * n) >> 32) for r in range(batch)]
     q = _unflat(_flat(p0), d, h)
-    loss, g = omlp.loss_and_grads(q, x[idx].astype(np.float64), t[idx].astype(np.float64))
-    assert abs(hist[0] - loss) <= 1e-5 * loss
+    loss, g = omlp.loss_and_grads(q, x[idx].astype(np.float64), t[idx].astype(np.float64),
+                                  omlp.bf16_round)
+    assert abs(hist[0] - loss) <= 1e-4 * loss
     m = {k: np.zeros_like(v) for k, v in q.items()}
     v = {k: np.zeros_like(v) for k, v in q.items()}
     omlp.adam_step(q, g, m, v, 1, lr)
