@@ -102,6 +102,7 @@ SIGNATURES = {
     "augur_profile_read": [_H, _P, _P],
     "augur_destroy": [_H],
     "augur_op_philox": [_P, _P, _I, _P],
+    "augur_op_stream_bandwidth": [_S, _I, _P, _P],
     "augur_op_gamma": [_U, _U, _U, _U, _P, _I, _P],
     "augur_op_dirichlet_conj": [_U, _U, _U, _U, _P, _P, _I, _I, _P],
     "augur_op_lda_zstep": [_I, _I, _I, _P, _P, _P, _P, _U, _U, _U, _P, _P, _P],

@@ -259,6 +259,84 @@ augur_status augur_destroy(augur_sampler* s) {
   });
 }
 
+// Attainable-bandwidth probes: a grid-stride read of float4 with four
+// independent loads in flight per thread (one float per block out, so the
+// loads are not dead), and a grid-stride float4 copy.
+__global__ __launch_bounds__(256) void k_stream_read(const float4* __restrict__ a, size_t n4, float* __restrict__ out) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+  for (; i + 3 * stride < n4; i += 4 * stride) {
+    const float4 v0 = a[i], v1 = a[i + stride], v2 = a[i + 2 * stride], v3 = a[i + 3 * stride];
+    s0 += (v0.x + v0.y) + (v0.z + v0.w);
+    s1 += (v1.x + v1.y) + (v1.z + v1.w);
+    s2 += (v2.x + v2.y) + (v2.z + v2.w);
+    s3 += (v3.x + v3.y) + (v3.z + v3.w);
+  }
+  for (; i < n4; i += stride) {
+    const float4 v = a[i];
+    s0 += (v.x + v.y) + (v.z + v.w);
+  }
+  __shared__ float red[256];
+  red[threadIdx.x] = (s0 + s1) + (s2 + s3);
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_stream_copy(const float4* __restrict__ a, float4* __restrict__ b, size_t n4) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n4; i += stride) b[i] = a[i];
+}
+
+__global__ void k_stream_fill(float4* __restrict__ a, size_t n4) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n4; i += stride)
+    a[i] = make_float4(1.f, 1.f, 1.f, 1.f);
+}
+
+augur_status augur_op_stream_bandwidth(size_t bytes, int reps, double* read_gbs, double* copy_gbs) {
+  return guard([&] {
+    require_device();
+    if (bytes < 16 || reps < 1 || !read_gbs || !copy_gbs) fail(AUGUR_ERR_ARG, "bad arguments");
+    const size_t n4 = bytes / 16;
+    int cus = 0;
+    AUGUR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0));
+    const int grid = 16 * std::max(cus, 1);
+    DevBuf<float4> a(n4), b(n4);
+    DevBuf<float> out(grid);
+    k_stream_fill<<<grid, 256>>>(a.p, n4);
+    k_stream_fill<<<grid, 256>>>(b.p, n4);
+    AUGUR_CHECK_LAUNCH();
+    hipEvent_t e0, e1;
+    AUGUR_HIP(hipEventCreate(&e0));
+    AUGUR_HIP(hipEventCreate(&e1));
+    auto timed = [&](bool copy) {
+      // one untimed launch, then reps timed back to back
+      if (copy) k_stream_copy<<<grid, 256>>>(a.p, b.p, n4);
+      else k_stream_read<<<grid, 256>>>(a.p, n4, out.p);
+      AUGUR_HIP(hipEventRecord(e0, 0));
+      for (int r = 0; r < reps; ++r) {
+        if (copy) k_stream_copy<<<grid, 256>>>(a.p, b.p, n4);
+        else k_stream_read<<<grid, 256>>>(a.p, n4, out.p);
+      }
+      AUGUR_HIP(hipEventRecord(e1, 0));
+      AUGUR_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      AUGUR_HIP(hipEventElapsedTime(&ms, e0, e1));
+      return (double)ms / reps;
+    };
+    const double t_read = timed(false), t_copy = timed(true);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *read_gbs = (double)(n4 * 16) / (t_read * 1e-3) / 1e9;
+    *copy_gbs = 2.0 * (double)(n4 * 16) / (t_copy * 1e-3) / 1e9;
+  });
+}
+
 augur_status augur_op_philox(const uint32_t* ctr, const uint32_t* key, int n, uint32_t* out) {
   return guard([&] {
     require_device();

@@ -517,8 +517,11 @@ __device__ inline float f4c(const float4& v, int k) { return k == 0 ? v.x : k ==
 // of the register-form pass that starts below n lies wholly in the buffers.
 constexpr int kHlrRowPad = 256;
 
+// Blocks per CU the register form is compiled for: at D * CP > 256 its
+// theta and gradient shares (2 CP F floats) and tile ring need up to 256
+// registers a lane, so two blocks (two waves per SIMD) instead of three.
 template <int D, int CP>
-__global__ __launch_bounds__(kPassThreads, kPassBlocksPerCU) void k_hlr_pass_r(PassArgs A) {
+__global__ __launch_bounds__(kPassThreads, (D * CP > 256 ? 2 : kPassBlocksPerCU)) void k_hlr_pass_r(PassArgs A) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int F = kRowFeat, RPT = row_rpt(CP);
   constexpr int LPR = D / F;                      // lanes per row
@@ -528,7 +531,7 @@ __global__ __launch_bounds__(kPassThreads, kPassBlocksPerCU) void k_hlr_pass_r(P
   constexpr int SPL = (NC + LPR - 1) / LPR;       // pairs per lane
   constexpr int PD = row_ring(CP);                // tiles in flight
   constexpr int GS = CP * F + 1;                  // LDS floats per lane (padded)
-  static_assert(D % F == 0 && LPR >= 2 && LPR <= 8 && LPR % CP == 0, "shape");
+  static_assert(D % F == 0 && LPR >= 2 && LPR <= 16 && LPR % CP == 0, "shape");
   static_assert(kHlrRowPad % TRr == 0, "tiles end inside the row padding");
 
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -714,6 +717,7 @@ PassKernel pick_pass_r(int d, int cp) {
   AUGUR_PASS(16, 1) AUGUR_PASS(16, 2)
   AUGUR_PASS(32, 1) AUGUR_PASS(32, 2) AUGUR_PASS(32, 4)
   AUGUR_PASS(64, 1) AUGUR_PASS(64, 2) AUGUR_PASS(64, 4)
+  AUGUR_PASS(128, 1) AUGUR_PASS(128, 2) AUGUR_PASS(128, 4) AUGUR_PASS(128, 8)
 #undef AUGUR_PASS
   return nullptr;
 }

@@ -49,6 +49,16 @@ def copy_probe(mib):
     log(f"copied 5 x {mib} MiB")
 
 
+def stream_bandwidth(mib=1024, reps=20):
+    """Attainable HBM bandwidth on this device: a read and a copy of a
+    buffer four times the 256 MB Infinity Cache (GB/s)."""
+    import ctypes as C
+    from augurv2_amd import _lib
+    rd, cp = C.c_double(0.0), C.c_double(0.0)
+    _lib.call("augur_op_stream_bandwidth", mib << 20, reps, C.byref(rd), C.byref(cp))
+    return {"read_gbs": rd.value, "copy_gbs": cp.value, "buffer_mib": mib}
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -183,7 +193,7 @@ def cpu_hlr(hlr, sweeps_per_run=6):
                       times, core)
 
 
-def run_lda(sweeps=100, warmup=10):
+def run_lda(sweeps=100, warmup=10, cpu=True):
     from augurv2_amd import runtime, workloads
     doc_off, w, alpha, beta, _ = workloads.lda_corpus(K=10, V=1000, D=200, mean_len=100, seed=0)
     s = runtime.LdaSampler(10, doc_off, w, alpha, beta, chains=1, seed=7)
@@ -201,18 +211,23 @@ def run_lda(sweeps=100, warmup=10):
     zavg, _ = s.profile_read()
     s.close()
     T = int(doc_off[-1])
-    # CPU: the oracle's sweep (the reference's libcpu order: theta, phi, z)
-    from oracle import oracle as orc
-    th, ph = np.zeros((200, 10)), np.zeros((10, 1000))
-
-    def cpu_run():
-        z = z0.copy()
-        for sw in range(sweeps):
-            orc.lda_sweep(doc_off, w, alpha, beta, 7, 0, sw, th, ph, z)
-
-    med, times, core = median_time(cpu_run)
     # the z step's algorithmic bytes: w read + z write per token
     zbytes = 8.0 * T
+    cpu_rec = None
+    if cpu:
+        # CPU: the oracle's sweep (the reference's libcpu order: theta, phi, z)
+        from oracle import oracle as orc
+        th, ph = np.zeros((200, 10)), np.zeros((10, 1000))
+
+        def cpu_run():
+            z = z0.copy()
+            for sw in range(sweeps):
+                orc.lda_sweep(doc_off, w, alpha, beta, 7, 0, sw, th, ph, z)
+
+        med, times, core = median_time(cpu_run)
+        cpu_rec = cpu_record(T * sweeps / med, "tokens/s",
+                             f"the oracle's LDA sweep, {sweeps} sweeps of the same corpus per run",
+                             times, core)
     return {"tokens": T, "sweeps": sweeps, "tokens_per_s": T * sweeps / (t1 - t0),
             "sweep_us": 1e6 * (t1 - t0) / sweeps, "zstep_avg_us": zavg,
             "roofline": {"kernel": "k_lda_z", "bound": "hbm", "achieved": zbytes / (zavg * 1e-6) / 1e9,
@@ -220,9 +235,7 @@ def run_lda(sweeps=100, warmup=10):
                          "frac": zbytes / (zavg * 1e-6) / 1e9 / HBM_PEAK_GBS, "traffic": None,
                          "note": "8 B per token over 19.9k tokens: latency-bound (dependent gathers), "
                                  "not bandwidth-bound, at this size"},
-            "cpu_baseline": cpu_record(T * sweeps / med, "tokens/s",
-                                       f"the oracle's LDA sweep, {sweeps} sweeps of the same corpus per run",
-                                       times, core)}
+            "cpu_baseline": cpu_rec}
 
 
 def gmm_summaries(pis, mu, cov):
@@ -232,7 +245,7 @@ def gmm_summaries(pis, mu, cov):
                            np.sort(np.trace(cov, axis1=-2, axis2=-1), axis=-1)], axis=-1)
 
 
-def run_gmm(sweeps=500, warmup=100, chains=4, seed=5, cpu_points=10_000):
+def run_gmm(sweeps=500, warmup=100, chains=4, seed=5, cpu_points=10_000, cpu=True):
     """Multivariate GMM Gibbs, N = 1M, D = 16, K = 32, 4 chains, schedule
     ConjGibbs [pis] (*) ConjGibbs [mu] (*) ConjGibbs [cov] (*) DiscGibbs [z]
     (experiments/experiment.py:127).  Every chain starts from the
@@ -264,33 +277,35 @@ def run_gmm(sweeps=500, warmup=100, chains=4, seed=5, cpu_points=10_000):
     s.close()
     min_ess, rhat, _ = _summary(summ)
     ok = rhat < RHAT_MAX
-    # CPU: the oracle's sweep with the reference's per-(point, component)
-    # Cholesky in the categorical step (cbits/augur_dist.c:548-568), on the
-    # first cpu_points points, extrapolated linearly in N
-    from oracle import oracle as orc
-    ys = y[:cpu_points].astype(np.float64)
-    g = orc.Gmm(ys, np.ones(K), np.zeros(D), 10.0 * np.eye(D), D + 2, np.eye(D))
-    z0 = z_true[:cpu_points].copy()
+    cpu_rec = None
+    if cpu:
+        # CPU: the oracle's sweep with the reference's per-(point, component)
+        # Cholesky in the categorical step (cbits/augur_dist.c:548-568), on the
+        # first cpu_points points, extrapolated linearly in N
+        from oracle import oracle as orc
+        ys = y[:cpu_points].astype(np.float64)
+        g = orc.Gmm(ys, np.ones(K), np.zeros(D), 10.0 * np.eye(D), D + 2, np.eye(D))
+        z0 = z_true[:cpu_points].copy()
 
-    def cpu_run():
-        pis, mu, cov, z = np.full(K, 1.0 / K), np.zeros((K, D)), np.tile(np.eye(D), (K, 1, 1)), z0.copy()
-        g.sweep(pis, mu, cov, z, seed, 0, 0)
+        def cpu_run():
+            pis, mu, cov, z = np.full(K, 1.0 / K), np.zeros((K, D)), np.tile(np.eye(D), (K, 1, 1)), z0.copy()
+            g.sweep(pis, mu, cov, z, seed, 0, 0)
 
-    med, times, core = median_time(cpu_run)
-    s_sweep_chain = med * N / cpu_points
-    ess_per_sweep = min_ess / sweeps
-    cpu = cpu_record(ess_per_sweep / (s_sweep_chain * chains) if ok else None, "min-ESS/s",
-                     f"one sweep of one chain over the first {cpu_points} points per run "
-                     f"({med:.3f} s), extrapolated linearly to N = {N} and {chains} chains "
-                     f"({s_sweep_chain * chains:.1f} s per sweep of all chains) at the GPU run's "
-                     "min-ESS per sweep", times, core)
+        med, times, core = median_time(cpu_run)
+        s_sweep_chain = med * N / cpu_points
+        ess_per_sweep = min_ess / sweeps
+        cpu_rec = cpu_record(ess_per_sweep / (s_sweep_chain * chains) if ok else None, "min-ESS/s",
+                             f"one sweep of one chain over the first {cpu_points} points per run "
+                             f"({med:.3f} s), extrapolated linearly to N = {N} and {chains} chains "
+                             f"({s_sweep_chain * chains:.1f} s per sweep of all chains) at the GPU "
+                             "run's min-ESS per sweep", times, core)
     return {"sweeps": sweeps, "warmup": warmup, "chains": chains, "elapsed_s": el,
             "ms_per_sweep": 1e3 * el / sweeps, "points_per_s": N * chains * sweeps / el,
             "min_ess": min_ess, "rhat_max": rhat, "rhat_ok": ok,
             "min_ess_per_s": min_ess / el if ok else None,
             "summary": "sorted pis, sorted |mu_k|, sorted tr(cov_k): 96 label-invariant scalars",
             "zstep_avg_us": zavg, "final_log_density": [float(v) for v in ll],
-            "cpu_baseline": cpu}
+            "cpu_baseline": cpu_rec}
 
 
 def run_surrogate(iters=5000, burn=1000, train=1000, seed=77, ablation=True):
@@ -375,7 +390,7 @@ def run_surrogate(iters=5000, burn=1000, train=1000, seed=77, ablation=True):
     return res
 
 
-def run_batched(sweeps=1000, warmup=500, chains=4, seed=31):
+def run_batched(sweeps=1000, warmup=500, chains=4, seed=31, plain=True):
     """256 HLR datasets (N_i ~ U[1k, 200k], D_i in {16, 32, 64}) in one
     batched sampler (BF16 X, autotuned rows per block, hipGraph sweep), 20
     leapfrog steps of 0.005 with the step jittered, MWG [v] ~ Normal(v, 1.0).
@@ -416,6 +431,9 @@ def run_batched(sweeps=1000, warmup=500, chains=4, seed=31):
         rhat_i.append(r)
     ess_i, rhat_i = np.array(ess_i), np.array(rhat_i)
     conv = rhat_i < RHAT_MAX
+    out = {"datasets": len(xs), "chains": chains}
+    if not plain:
+        return out
     # plain sampler, one dataset at a time
     work = np.array([x.shape[0] * x.shape[1] for x in xs], np.float64)
     pick = np.argsort(work)[np.linspace(0, len(xs) - 1, 8).astype(int)]
@@ -457,6 +475,13 @@ def main():
                     "HLR leg: lda, surrogate, batched, gmm (default: all)")
     ap.add_argument("--batched-sweeps", type=int, default=1000,
                     help="timed sweeps of the batched legs (warmup: half as many)")
+    ap.add_argument("--surrogate-iters", type=int, default=5000,
+                    help="timed iterations of each surrogate-leg sampler (shorter for profiling)")
+    ap.add_argument("--surrogate-exact", type=int, default=2000,
+                    help="exact iterations before the timed ones (half burn-in, half collection)")
+    ap.add_argument("--gmm-sweeps", type=int, default=500, help="timed sweeps of the GMM leg")
+    ap.add_argument("--skip-headline", action="store_true",
+                    help="run only the legs named by --only (profiling runs; no bench line)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -470,15 +495,31 @@ def main():
     if args.steps < 1:
         raise SystemExit("bench: --steps must be at least 1")
     legs = set(args.only.split(",")) if args.only else {"lda", "surrogate", "batched", "gmm"}
+    if args.skip_headline:
+        for leg, fn in (("lda", lambda: run_lda(cpu=False)),
+                        ("gmm", lambda: run_gmm(sweeps=args.gmm_sweeps, cpu=False)),
+                        ("surrogate", lambda: run_surrogate(iters=args.surrogate_iters,
+                                                            burn=max(1, args.surrogate_exact // 2),
+                                                            train=max(1, args.surrogate_exact // 2),
+                                                            ablation=False)),
+                        ("batched", lambda: run_batched(sweeps=args.batched_sweeps,
+                                                        warmup=max(1, args.batched_sweeps // 2),
+                                                        chains=16, plain=False)),
+                        ("stream", lambda: log(str(stream_bandwidth(256, 3))))):
+            if leg in legs:
+                log(f"{leg}: done ({type(fn()).__name__})")
+        return
     hlr = run_hlr(args.steps, args.warmup)
     log(f"hlr: {hlr['elapsed']:.3f} s, min ESS {hlr['min_ess']:.1f}, R-hat {hlr['rhat_max']:.4f}")
-    lda = run_lda() if "lda" in legs else None
+    lda = run_lda(cpu=not args.no_cpu) if "lda" in legs else None
     if lda:
         log(f"lda: {lda['tokens_per_s']:.3e} tokens/s")
-    gmm = run_gmm() if "gmm" in legs else None
+    gmm = run_gmm(sweeps=args.gmm_sweeps, cpu=not args.no_cpu) if "gmm" in legs else None
     if gmm:
         log(f"gmm: {gmm}")
-    surr = run_surrogate() if "surrogate" in legs else None
+    half = max(1, args.surrogate_exact // 2)
+    surr = (run_surrogate(iters=args.surrogate_iters, burn=half, train=half)
+            if "surrogate" in legs else None)
     bs = max(1, args.batched_sweeps)
     bat = run_batched(sweeps=bs, warmup=max(1, bs // 2)) if "batched" in legs else None
     if bat:
@@ -494,6 +535,8 @@ def main():
     if hlr["pass_launches_timed"] == 0:
         raise SystemExit("bench: no pass launch was timed")
     achieved = bytes_per_pass / (hlr["pass_avg_us"] * 1e-6) / 1e9
+    stream = stream_bandwidth()
+    log(f"stream: {stream}")
     cpu = None
     if not args.no_cpu:
         cpu = cpu_hlr(hlr)
@@ -543,6 +586,7 @@ def main():
                      "note": "X (25.6 MB) stays in the 256 MB Infinity Cache between passes: the "
                              "pass is latency-bound (launch + one memory round), not HBM-bound"},
         "cpu_baseline": cpu,
+        "attainable_bandwidth": stream,
     }
     print(json.dumps(out, default=float), flush=True)
 

@@ -263,6 +263,11 @@ augur_status augur_destroy(augur_sampler* s);
  * checked one by one against the CPU restatement.
  * ---------------------------------------------------------------------- */
 
+/* No counterpart in the reference.  Attainable device bandwidth on device
+ * 0: a grid-stride read and a copy of `bytes` (rounded down to 16), each
+ * launched reps times back to back after one untimed launch, timed with
+ * HIP events; copy counts read + write bytes.  GB/s out. */
+augur_status augur_op_stream_bandwidth(size_t bytes, int reps, double* read_gbs, double* copy_gbs);
 /* n Philox4x32-10 blocks: ctr[4 n], key[2 n] -> out[4 n]. */
 augur_status augur_op_philox(const uint32_t* ctr, const uint32_t* key, int n, uint32_t* out);
 /* out[i] = Gamma(shape[i], 1) at site i (cbits/augur_dist.c:302-312). */

@@ -118,11 +118,14 @@ def test_lda_zstep_vs_fp64_reference_form(ops, oracle):
                                                   (1000, 128, 8, False), (777, 5, 3, True),
                                                   (100003, 64, 4, False), (3001, 32, 3, True),
                                                   (2500, 16, 2, False), (129, 64, 1, True),
-                                                  (7, 32, 4, False), (20000, 64, 2, False)])
+                                                  (7, 32, 4, False), (20000, 64, 2, False),
+                                                  (50001, 128, 8, False), (3000, 128, 3, True),
+                                                  (130, 128, 1, False), (9000, 128, 2, True)])
 def test_hlr_ll_grad_vs_fp64(ops, oracle, monkeypatch, n, d, chains, intercept, lds):
-    """Register form of the pass (D <= 64, chains <= D / 8, padded to a
-    power of two) and the LDS-staged form forced on the same shapes;
-    partial tiles, a block of fewer rows than lanes, the intercept."""
+    """Register form of the pass (D <= 64 with chains <= D / 8, and D = 128
+    with up to 8 chains, padded to a power of two) and the LDS-staged form
+    forced on the same shapes; partial tiles, a block of fewer rows than
+    lanes, the intercept."""
     monkeypatch.setenv("AUGUR_HLR_PASS_LDS", "1" if lds else "0")
     rng = np.random.default_rng(n + d)
     x = rng.standard_normal((n, d)).astype(np.float32)
