@@ -64,7 +64,12 @@ class GmmDesc(C.Structure):
                 ("chains", C.c_int), ("seed", C.c_uint32), ("y", C.c_void_p),
                 ("alpha", C.c_void_p), ("mu0", C.c_void_p), ("cov0", C.c_void_p),
                 ("df", C.c_int), ("scale0", C.c_void_p), ("sample_pi", C.c_int),
-                ("sample_cov", C.c_int)]
+                ("sample_cov", C.c_int), ("mu_kernel", C.c_int), ("mu_sim_len", C.c_double),
+                ("mu_step_size", C.c_double), ("mu_fixed_steps", C.c_int),
+                ("mu_prop_cov", C.c_void_p), ("ref_compat", C.c_int)]
+
+
+GMM_MU_CONJ, GMM_MU_ESLICE, GMM_MU_HMC, GMM_MU_MWG, GMM_MU_RSLICE = 0, 1, 2, 3, 4
 
 
 _H = C.c_void_p

@@ -31,6 +31,7 @@
 // 798-813), which underflows at D = 16; the log-space draw targets the same
 // distribution.
 #include "augur_common.h"
+#include "augur_hmc.h"
 
 namespace augur {
 
@@ -59,7 +60,20 @@ struct GmmDev {
   int cus;         // compute units of the device
   float* part;     // C x G x k x S
   int32_t* z;      // C x n
+  // the mu block's move when it is not ConjGibbs (GMM_MU_*)
+  int mu_mode, ref_compat, mu_fixed_steps;
+  double mu_sim_len, mu_step;
+  const double* cov0_inv;  // d x d
+  const double* chol0;     // d x d lower, cov0 = L L^T
+  const double* cholp;     // d x d lower, MWG proposal covariance
+  double* mcent;  // C x k x d      centre of the statistics (the FP32 table mu)
+  double* mprec;  // C x k x d x d  cov_k^-1
+  double* mlin;   // C x k x d      cov_k^-1 s, s = sum over the component of (y - centre)
+  double* mcnt;   // C x k          points in the component
+  double* macc;   // C x 2          accepted / proposed moves of the mu block
 };
+
+enum { GMM_MU_CONJ = 0, GMM_MU_ESLICE = 1, GMM_MU_HMC = 2, GMM_MU_MWG = 3, GMM_MU_RSLICE = 4 };
 
 // ---- small FP64 linear algebra (cbits/augur_matop.c:312-406), row-major ----
 // Wave-parallel: one 64-lane block holds the matrices in LDS and every
@@ -218,6 +232,17 @@ __global__ __launch_bounds__(kConjLanes) void k_gmm_conj(GmmDev m, const double*
     const Stream sp{seed, (uint32_t)c, sweep, TAG_GMM_PI};
     m.gpi[(size_t)c * m.k + k] = gamma_at(sp, (uint32_t)k, m.alpha[k] + (double)cnt);
   }
+  // the statistics' centre: the FP32 table copy of the mu they were taken at
+  // (saved by k_gmm_mu_prep when another kernel has moved mu since)
+  double* cen = mean;
+  const bool mu_moved = m.mu_mode != GMM_MU_CONJ;
+  for (int i = tid; i < d; i += kConjLanes)
+    cen[i] = mu_moved ? m.mcent[((size_t)c * m.k + k) * d + i] : (double)(float)mu[i];
+  __syncthreads();
+  if (mu_moved) {
+    for (int i = tid; i < d; i += kConjLanes) munew[i] = mu[i];
+    __syncthreads();
+  } else {
   // ---- mu_k | cov_k: cbits/augur_dist.c:633-669 ----
   for (int e = tid; e < d * d; e += kConjLanes) {
     L0[e] = m.cov0[e];
@@ -236,7 +261,7 @@ __global__ __launch_bounds__(kConjLanes) void k_gmm_conj(GmmDev m, const double*
     double t = 0.0;
     for (int j = 0; j < d; ++j) t += L0[i * d + j] * m.mu0[j];
     v[i] = t;
-    mean[i] = ((double)cnt * (double)(float)mu[i] + sc[i]) * inv;
+    mean[i] = ((double)cnt * cen[i] + sc[i]) * inv;
   }
   __syncthreads();
   for (int i = tid; i < d; i += kConjLanes) {
@@ -269,10 +294,15 @@ __global__ __launch_bounds__(kConjLanes) void k_gmm_conj(GmmDev m, const double*
     for (int i = tid; i < d; i += kConjLanes) munew[i] = tmp[i];
     __syncthreads();
   }
+  // the centre again (mean held the conjugate posterior mean meanwhile)
+  for (int i = tid; i < d; i += kConjLanes)
+    cen[i] = mu_moved ? m.mcent[((size_t)c * m.k + k) * d + i] : (double)(float)mu[i];
+  __syncthreads();
+  }
   // ---- cov_k | mu_k: IW(df + n, scale0 + sum (y - mu)(y - mu)^T) ----
   if (m.sample_cov) {
     double* dl = v;  // d = centre - mu_new
-    for (int i = tid; i < d; i += kConjLanes) dl[i] = (double)(float)mu[i] - munew[i];
+    for (int i = tid; i < d; i += kConjLanes) dl[i] = cen[i] - munew[i];
     __syncthreads();
     for (int e = tid; e < d * d; e += kConjLanes) {
       const int i = e / d, j = e % d;
@@ -301,6 +331,243 @@ __global__ __launch_bounds__(kConjLanes) void k_gmm_conj(GmmDev m, const double*
   for (int i = tid; i < d; i += kConjLanes) mu[i] = munew[i];
   __syncthreads();
   build_table(m, c, k, L, tmp);
+}
+
+// ---- mu moves other than ConjGibbs --------------------------------------
+// Given z and cov_k, the data term of mu_k is a quadratic in mu_k fixed by the
+// component's statistics around the centre c_k (count n, s = sum (y - c),
+// S = sum (y - c)(y - c)^T):
+//   sum_n log N(y_n | mu, cov) = -1/2 tr(P S) + d^T P s - n/2 d^T P d + const,
+//   d = mu - c, P = cov^-1,
+// with gradient P s - n P d.  The moves below evaluate it from the
+// statistics (O(D^2) per evaluation) where the reference loops over the
+// data on every evaluation; the constant -1/2 tr(P S) - n/2 log det cov cancels
+// in every comparison the kernels make (slice levels, Metropolis ratios).
+
+// Per (component, chain): P_k, P_k s_k, n_k and c_k for k_gmm_mu_move.
+__global__ __launch_bounds__(kConjLanes) void k_gmm_mu_prep(GmmDev m, const double* __restrict__ ssum) {
+  const int k = blockIdx.x, c = blockIdx.y, tid = threadIdx.x, d = m.d, S = m.S;
+  __shared__ double L[kMaxDim * kMaxDim], tmp[kMaxDim * kMaxDim], sc[kMaxDim];
+  const size_t ck = (size_t)c * m.k + k;
+  const double* cov = m.cov64 + ck * d * d;
+  for (int e = tid; e < d * d; e += kConjLanes) L[e] = cov[e];
+  for (int i = tid; i < d; i += kConjLanes) sc[i] = ssum[ck * S + 1 + i];
+  __syncthreads();
+  chol(L, d);
+  chol_invert(L, d, tmp);
+  for (int e = tid; e < d * d; e += kConjLanes) m.mprec[ck * d * d + e] = L[e];
+  for (int i = tid; i < d; i += kConjLanes) {
+    double t = 0.0;
+    for (int j = 0; j < d; ++j) t += L[i * d + j] * sc[j];
+    m.mlin[ck * d + i] = t;
+    m.mcent[ck * d + i] = (double)(float)m.mu64[ck * d + i];
+  }
+  if (tid == 0) m.mcnt[ck] = ssum[ck * S];
+}
+
+constexpr int kMuThreads = 1024;
+constexpr int kMuWaves = kMuThreads / 64;
+constexpr int kESliceMaxShrinks = 200;
+constexpr double kTwoPi = 6.283185307179586476925;
+
+// Data term of component k at x (lane i < d holds x_i) minus its constant;
+// with_prior adds the MvNormal(mu0, cov0) prior term, also without its
+// constant.  Every lane returns the same sum (xor butterfly).
+__device__ double mu_comp_ll(const GmmDev& m, int c, int k, double x, bool with_prior) {
+  const int lane = threadIdx.x & 63, d = m.d;
+  const size_t ck = (size_t)c * m.k + k;
+  const bool on = lane < d;
+  const double dl = on ? x - m.mcent[ck * d + lane] : 0.0;
+  const double dm = on ? x - m.mu0[lane] : 0.0;
+  double pd = 0.0, q = 0.0;
+  for (int j = 0; j < d; ++j) {
+    const double dj = __shfl(dl, j), mj = __shfl(dm, j);
+    if (on) {
+      pd += m.mprec[(ck * d + lane) * d + j] * dj;
+      q += m.cov0_inv[lane * d + j] * mj;
+    }
+  }
+  double t = on ? dl * m.mlin[ck * d + lane] - 0.5 * m.mcnt[ck] * dl * pd : 0.0;
+  if (with_prior && on) t -= 0.5 * dm * q;
+  for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
+  return t;
+}
+
+// Lane i < d: (L z)_i for the lower triangular L (d x d) and z_i in lane i.
+__device__ double mu_lower_times(const double* L, int d, double z) {
+  const int lane = threadIdx.x & 63;
+  double r = 0.0;
+  for (int j = 0; j < d; ++j) {
+    const double zj = __shfl(z, j);
+    if (lane < d && j <= lane) r += L[lane * d + j] * zj;
+  }
+  return r;
+}
+
+// Block-wide FP64 sum (every thread gets it), waves in a fixed order.
+__device__ double mu_block_sum(double v, double* red) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wv] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int w = 0; w < kMuWaves; ++w) s += red[w];
+  return s;
+}
+
+// Gradient of the mu block's log full conditional at th (all coordinates in
+// LDS, thread t = k d + i owns coordinate i of component k) and this
+// thread's share of the log density (prior and data, without constants).
+__device__ void mu_block_eval(const GmmDev& m, int c, const double* th, double* g, double* lt) {
+  const int t = threadIdx.x, d = m.d;
+  *g = 0.0;
+  *lt = 0.0;
+  if (t >= m.k * d) return;
+  const int k = t / d, i = t % d;
+  const size_t ck = (size_t)c * m.k + k;
+  double pd = 0.0, q = 0.0;
+  for (int j = 0; j < d; ++j) {
+    pd += m.mprec[(ck * d + i) * d + j] * (th[k * d + j] - m.mcent[ck * d + j]);
+    q += m.cov0_inv[i * d + j] * (th[k * d + j] - m.mu0[j]);
+  }
+  const double n = m.mcnt[ck], lin = m.mlin[ck * d + i];
+  const double dl = th[t] - m.mcent[ck * d + i], dm = th[t] - m.mu0[i];
+  *g = lin - n * pd - q;
+  *lt = dl * lin - 0.5 * n * dl * pd - 0.5 * dm * q;
+}
+
+// One block of kMuThreads per chain: the schedule's move of the mu block.
+//   ESlice [mu]  per component (cbits/augur_mcmclib.h:439-485, Core/CgESlice.hs):
+//                nu ~ MvNormal(mu0, cov0), log y = like(mu_k) + log u, angle
+//                bracket shrunk until like(prop) > log y; one wave per component.
+//   MWG [mu] ~ MvNormal(mu[k], P) per component (:178-193,
+//                Core/CgUserProp.hs): mu' = mu + chol(P) z, accepted on the
+//                full conditional; with ref_compat the reference's Hastings
+//                term 1/2 z.z (both proposal densities taken at mu).
+//   HMC [mu]     one trajectory over all components (:343-424), the
+//                reference's jitter, one Metropolis test.
+//   RSlice [mu]  reflective slice over all components (:500-543): slice level
+//                log p(mu) - Exp(1), round(sim_len / eps) steps of eps, the
+//                momentum reflected on the gradient outside the slice, the end
+//                point kept if it lies in the slice.
+// Draws: Philox (seed, chain) at tag TAG_GMM_MU_MOVE (normals: site k D + i),
+// + 1 (HMC jitter), + 2 (uniforms: site k, sub 1 the slice level / accept,
+// sub 2 + j the j-th angle; block moves site 0).
+__global__ __launch_bounds__(kMuThreads) void k_gmm_mu_move(GmmDev m, uint32_t seed, uint32_t sweep) {
+  const int c = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6, d = m.d, K = m.k;
+  __shared__ double th[kMaxComp * kMaxDim], red[kMuWaves];
+  __shared__ int acc_s;
+  double* mu = m.mu64 + (size_t)c * K * d;
+  const Stream sn{seed, (uint32_t)c, sweep, TAG_GMM_MU_MOVE};
+  const Stream su{seed, (uint32_t)c, sweep, TAG_GMM_MU_MOVE + 2};
+  if (t == 0) acc_s = 0;
+  __syncthreads();
+  if (m.mu_mode == GMM_MU_ESLICE || m.mu_mode == GMM_MU_MWG) {
+    int acc = 0;
+    for (int k = wv; k < K; k += kMuWaves) {
+      const bool on = lane < d;
+      const double x = on ? mu[k * d + lane] : 0.0;
+      const double z = on ? normal_at(sn, (uint32_t)(k * d + lane), 0) : 0.0;
+      if (m.mu_mode == GMM_MU_ESLICE) {
+        const double nu = mu_lower_times(m.chol0, d, z) + (on ? m.mu0[lane] : 0.0);
+        const double mu0 = on ? m.mu0[lane] : 0.0;
+        const double log_y = mu_comp_ll(m, c, k, x, false) + log(u01(su.draw((uint32_t)k, 1).x));
+        double ang = kTwoPi * u01(su.draw((uint32_t)k, 2).x);
+        double amin = ang - kTwoPi, amax = ang;
+        double out = x;
+        for (int j = 0; j < kESliceMaxShrinks; ++j) {
+          if (amax - amin < 1e-8) break;  // keep the current point
+          const double pr = (x - mu0) * cos(ang) + (nu - mu0) * sin(ang) + mu0;
+          if (mu_comp_ll(m, c, k, pr, false) > log_y) {
+            out = pr;
+            ++acc;
+            break;
+          }
+          if (ang < 0.0) amin = ang;
+          else amax = ang;
+          ang = amin + (amax - amin) * u01(su.draw((uint32_t)k, (uint32_t)(3 + j)).x);
+        }
+        if (on) mu[k * d + lane] = out;
+      } else {
+        const double pr = x + mu_lower_times(m.cholp, d, z);
+        double corr = 0.0;
+        if (m.ref_compat) {
+          double zz = z * z;
+          for (int o = 32; o > 0; o >>= 1) zz += __shfl_xor(zz, o);
+          corr = 0.5 * zz;
+        }
+        const double lu = log(u01(su.draw((uint32_t)k, 1).x));
+        const double diff = mu_comp_ll(m, c, k, pr, true) - mu_comp_ll(m, c, k, x, true);
+        if (lu < diff + corr) {
+          if (on) mu[k * d + lane] = pr;
+          ++acc;
+        }
+      }
+    }
+    if (lane == 0) atomicAdd(&acc_s, acc);
+    __syncthreads();
+    if (t == 0) {
+      m.macc[2 * c] += (double)acc_s;
+      m.macc[2 * c + 1] += (double)K;
+    }
+    return;
+  }
+  // ---- block moves: HMC, RSlice ----
+  const bool own = t < K * d;
+  const double x0 = own ? mu[t] : 0.0;
+  double x = x0;
+  if (own) th[t] = x0;
+  const double p0 = own ? normal_at(sn, (uint32_t)t, 0) : 0.0;
+  double g, lt;
+  __syncthreads();
+  mu_block_eval(m, c, th, &g, &lt);
+  const double ll0 = mu_block_sum(lt, red);
+  bool accept = false;
+  if (m.mu_mode == GMM_MU_HMC) {
+    double step;
+    int steps;
+    hmc_jitter(Stream{seed, (uint32_t)c, sweep, TAG_GMM_MU_MOVE + 1}, m.mu_sim_len, m.mu_step, m.mu_fixed_steps,
+               &step, &steps);
+    double p = p0 + 0.5 * step * g;
+    for (int l = 0; l < steps; ++l) {
+      x += step * p;
+      __syncthreads();
+      if (own) th[t] = x;
+      __syncthreads();
+      mu_block_eval(m, c, th, &g, &lt);
+      if (l != steps - 1) p += step * g;
+    }
+    p += 0.5 * step * g;
+    const double ll1 = mu_block_sum(lt, red);
+    const double kin = mu_block_sum(0.5 * p0 * p0 - 0.5 * p * p, red);
+    const double lu = log(u01(su.draw(0, 1).x));
+    accept = lu < (ll1 - ll0) + kin;
+  } else {  // GMM_MU_RSLICE
+    const double level = ll0 + log(u01(su.draw(0, 1).x));  // ll - Exp(1)
+    const double rs = round(m.mu_sim_len / m.mu_step);
+    const int steps = rs < 1.0 ? 1 : (int)rs;
+    double p = p0;
+    bool out_of_slice = false;
+    for (int l = 0; l < steps; ++l) {
+      x += m.mu_step * p;
+      __syncthreads();
+      if (own) th[t] = x;
+      __syncthreads();
+      mu_block_eval(m, c, th, &g, &lt);
+      out_of_slice = mu_block_sum(lt, red) < level;
+      if (out_of_slice) {
+        const double pg = mu_block_sum(p * g, red), gg = mu_block_sum(g * g, red);
+        p -= 2.0 * (pg / gg) * g;
+      }
+    }
+    accept = !out_of_slice;
+  }
+  if (own && accept) mu[t] = x;
+  if (t == 0) {
+    m.macc[2 * c] += accept ? 1.0 : 0.0;
+    m.macc[2 * c + 1] += 1.0;
+  }
 }
 
 // Tables from the current parameters (after set_pt).
@@ -887,6 +1154,7 @@ struct GmmSampler final : augur_sampler {
   GmmDev m{};
   DevBuf<float> y, tab, part;
   DevBuf<double> alpha, mu0, cov0, scale0, gpi, pis64, mu64, cov64, ll, ssum;
+  DevBuf<double> cov0_inv, chol0, cholp, mcent, mprec, mlin, mcnt, macc;
   DevBuf<int32_t> z;
   std::vector<double> h_alpha, h_mu0, h_cov0, h_scale0;
   bool stats_valid = false;
@@ -920,6 +1188,14 @@ struct GmmSampler final : augur_sampler {
   void launch_conj() {
     k_gmm_stats_sum<<<dim3((m.S + 63) / 64, m.k, chains), 64 * kSumSlices, 0, stream>>>(m, ssum.p);
     AUGUR_CHECK_LAUNCH();
+    if (m.mu_mode != GMM_MU_CONJ) {
+      // the schedule's mu move from the statistics, before pis / cov's draws
+      // use the new mu (pis does not depend on mu; cov follows mu)
+      k_gmm_mu_prep<<<dim3(m.k, chains), kConjLanes, 0, stream>>>(m, ssum.p);
+      AUGUR_CHECK_LAUNCH();
+      k_gmm_mu_move<<<chains, kMuThreads, 0, stream>>>(m, seed, sweep);
+      AUGUR_CHECK_LAUNCH();
+    }
     k_gmm_conj<<<dim3(m.k, chains), kConjLanes, 0, stream>>>(m, ssum.p, seed, sweep);
     AUGUR_CHECK_LAUNCH();
   }
@@ -978,6 +1254,16 @@ struct GmmSampler final : augur_sampler {
   void stats(double* out, size_t bytes) override {
     if (bytes < sizeof(double)) fail(AUGUR_ERR_ARG, "stats buffer too small");
     out[0] = (double)sweep;
+    // then, when the buffer holds them, the mu block's accepted and proposed
+    // moves per chain (non-conjugate mu kernels)
+    if (bytes >= sizeof(double) * (1 + 2 * (size_t)chains)) {
+      if (macc.p) {
+        macc.download(out + 1, 2 * (size_t)chains, stream);
+        AUGUR_HIP(hipStreamSynchronize(stream));
+      } else {
+        for (int i = 0; i < 2 * chains; ++i) out[1 + i] = 0.0;
+      }
+    }
   }
 };
 
@@ -988,6 +1274,51 @@ static void gmm_validate(const augur_gmm_desc* D) {
     fail(AUGUR_ERR_ARG, "need N >= 1, 1 <= D <= %d, 1 <= K <= %d, chains >= 1", kMaxDim, kMaxComp);
   if (!D->y || !D->alpha || !D->mu0 || !D->cov0 || !D->scale0) fail(AUGUR_ERR_ARG, "null array");
   if (D->sample_cov && D->df <= D->dim - 1) fail(AUGUR_ERR_ARG, "InvWishart needs df > D - 1");
+  if (D->mu_kernel < AUGUR_GMM_MU_CONJ || D->mu_kernel > AUGUR_GMM_MU_RSLICE)
+    fail(AUGUR_ERR_ARG, "unknown mu kernel %d", D->mu_kernel);
+  if (D->mu_kernel == AUGUR_GMM_MU_HMC || D->mu_kernel == AUGUR_GMM_MU_RSLICE) {
+    if (!(D->mu_step_size > 0.0) || (D->mu_fixed_steps == 0 && !(D->mu_sim_len > 0.0)) ||
+        D->mu_sim_len / D->mu_step_size > 1.0e5 || D->mu_fixed_steps > 100000 || D->mu_fixed_steps < -100000)
+      fail(AUGUR_ERR_ARG, "mu kernel needs step_size > 0 and sim_len > 0 (or fixed_steps)");
+  }
+  if (D->mu_kernel == AUGUR_GMM_MU_MWG && !D->mu_prop_cov) fail(AUGUR_ERR_ARG, "MWG [mu] needs a proposal covariance");
+}
+
+// Lower Cholesky factor of a symmetric positive definite d x d matrix on the
+// host (FP64), for the hyper-parameter matrices of the mu kernels.
+static std::vector<double> host_chol(const double* a, int d, const char* what) {
+  std::vector<double> l((size_t)d * d, 0.0);
+  for (int i = 0; i < d; ++i)
+    for (int j = 0; j <= i; ++j) {
+      double s = a[i * d + j];
+      for (int k = 0; k < j; ++k) s -= l[i * d + k] * l[j * d + k];
+      if (i == j) {
+        if (!(s > 0.0)) fail(AUGUR_ERR_ARG, "%s is not positive definite", what);
+        l[i * d + i] = sqrt(s);
+      } else {
+        l[i * d + j] = s / l[j * d + j];
+      }
+    }
+  return l;
+}
+
+// Inverse of a d x d matrix from its lower Cholesky factor (host, FP64).
+static std::vector<double> host_chol_inverse(const std::vector<double>& l, int d) {
+  std::vector<double> inv((size_t)d * d, 0.0), e(d), y(d);
+  for (int c = 0; c < d; ++c) {
+    for (int i = 0; i < d; ++i) e[i] = i == c ? 1.0 : 0.0;
+    for (int i = 0; i < d; ++i) {  // L y = e
+      double s = e[i];
+      for (int k = 0; k < i; ++k) s -= l[i * d + k] * y[k];
+      y[i] = s / l[i * d + i];
+    }
+    for (int i = d - 1; i >= 0; --i) {  // L^T x = y
+      double s = y[i];
+      for (int k = i + 1; k < d; ++k) s -= l[k * d + i] * inv[k * d + c];
+      inv[i * d + c] = s / l[i * d + i];
+    }
+  }
+  return inv;
 }
 
 static void gmm_setup(GmmSampler* s, int n, int d, int k, int chains) {
@@ -1040,6 +1371,27 @@ augur_status augur_gmm_create(const augur_gmm_desc* D, augur_sampler** out) {
       gmm_setup(s, n, d, k, C);
       GmmDev& m = s->m;
       m.df = D->df; m.sample_pi = D->sample_pi; m.sample_cov = D->sample_cov;
+      m.mu_mode = D->mu_kernel;
+      m.ref_compat = D->ref_compat;
+      m.mu_sim_len = D->mu_sim_len;
+      m.mu_step = D->mu_step_size;
+      m.mu_fixed_steps = D->mu_fixed_steps;
+      if (m.mu_mode != GMM_MU_CONJ) {
+        const std::vector<double> l0 = host_chol(D->cov0, d, "cov0");
+        const std::vector<double> i0 = host_chol_inverse(l0, d);
+        const std::vector<double> lp =
+            m.mu_mode == GMM_MU_MWG ? host_chol(D->mu_prop_cov, d, "the MWG proposal covariance") : l0;
+        s->cov0_inv.alloc((size_t)d * d); s->chol0.alloc((size_t)d * d); s->cholp.alloc((size_t)d * d);
+        s->cov0_inv.upload(i0.data(), i0.size(), s->stream);
+        s->chol0.upload(l0.data(), l0.size(), s->stream);
+        s->cholp.upload(lp.data(), lp.size(), s->stream);
+        s->mcent.alloc((size_t)C * k * d); s->mprec.alloc((size_t)C * k * d * d);
+        s->mlin.alloc((size_t)C * k * d); s->mcnt.alloc((size_t)C * k); s->macc.alloc(2 * (size_t)C);
+        s->macc.zero(s->stream);
+        m.cov0_inv = s->cov0_inv.p; m.chol0 = s->chol0.p; m.cholp = s->cholp.p;
+        m.mcent = s->mcent.p; m.mprec = s->mprec.p; m.mlin = s->mlin.p; m.mcnt = s->mcnt.p; m.macc = s->macc.p;
+        AUGUR_HIP(hipStreamSynchronize(s->stream));  // the host vectors are released on return
+      }
       s->y.upload(D->y, (size_t)n * d, s->stream);
       s->alpha.upload(D->alpha, k, s->stream);
       s->mu0.upload(D->mu0, d, s->stream);

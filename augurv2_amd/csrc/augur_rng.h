@@ -28,6 +28,7 @@ enum : uint32_t {
   TAG_GMM_MU = 5,
   TAG_GMM_COV = 6,
   TAG_GMM_Z = 7,
+  TAG_GMM_MU_MOVE = 8,  // ESlice / HMC / MWG / RSlice on mu: 8 normals, 9 jitter, 10 uniforms
   TAG_HMC_BASE = 16,  // HMC block b: 16 + 4 b + {0 momentum, 1 jitter, 2 accept}
 };
 

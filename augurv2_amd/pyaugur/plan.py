@@ -7,14 +7,16 @@ Families (each recognised up to variable names):
   HLR  -- compiler/augur/test/hlr.rv family (InvGamma or Exponential prior on
           the variance, optional intercept) with HMC blocks
   GMM  -- compiler/augur/test/hmvgmm.rv (and mvgmm.rv: fixed pis and covs)
-          with ConjGibbs pis, mu, cov, DiscGibbs z
+          with ConjGibbs pis, ConjGibbs | ESlice | HMC | MWG | RSlice mu,
+          ConjGibbs cov, DiscGibbs z
 The reference's compiler would instead generate code for any model
 (Compile.hs:311-345); other models raise UnsupportedModel.
 """
 import numpy as np
 
 from .. import runtime
-from .._lib import HMC_BLOCK_B, HMC_BLOCK_B_THETA, HMC_BLOCK_THETA, HMC_BLOCK_V, MWG_BLOCK_V
+from .._lib import (GMM_MU_CONJ, GMM_MU_ESLICE, GMM_MU_HMC, GMM_MU_MWG, GMM_MU_RSLICE, HMC_BLOCK_B,
+                    HMC_BLOCK_B_THETA, HMC_BLOCK_THETA, HMC_BLOCK_V, MWG_BLOCK_V)
 from . import augurdistlib as adl
 from .rvparse import Kern
 
@@ -346,9 +348,30 @@ class GmmPlan:
         want.append(("DiscGibbs", z_name))
         if sched is None:
             sched = [Kern(k, [v]) for k, v in want]
-        got = [(k.kind, k.vars[0]) for k in sched]
-        if got != want:
-            _fail("GMM: supported schedule is " + " (*) ".join(f"{k} [{v}]" for k, v in want))
+        # the kernel for mu may be ConjGibbs, ESlice, HMC, MWG or RSlice
+        # (compiler/augur/test/hmvgmm1-3.infer, mvgmm1-5.infer)
+        mu_kerns = {"ConjGibbs", "ESlice", "HMC", "MWG", "RSlice"}
+        got = [(k.kind, k.vars[0]) if len(k.vars) == 1 else (k.kind, tuple(k.vars)) for k in sched]
+        norm = [("ConjGibbs", v) if (v == mu_name and kd in mu_kerns) else (kd, v) for kd, v in got]
+        if norm != want:
+            _fail("GMM: supported schedule is " + " (*) ".join(
+                f"{'ConjGibbs|ESlice|HMC|MWG|RSlice' if v == mu_name else k} [{v}]" for k, v in want))
+        mk = next(k for k in sched if k.vars == [mu_name])
+        self.mu_kernel = {"ConjGibbs": GMM_MU_CONJ, "ESlice": GMM_MU_ESLICE, "HMC": GMM_MU_HMC,
+                          "MWG": GMM_MU_MWG, "RSlice": GMM_MU_RSLICE}[mk.kind]
+        # Core/KernSyn.hs:64-75, :105-117: HMC and RSlice default to (1.0, 0.05)
+        self.mu_settings = tuple(mk.settings) if mk.settings else (1.0, 0.05)
+        self.mu_prop_cov_name = None
+        if mk.kind == "MWG":
+            # MWG [mu] ~ MvNormal(mu[k], P) for k <- 0 until K
+            prop, grids = mk.proposal
+            pargs = _dist(prop, "MvNormal")
+            pname, pidx = _proj(pargs[0]) if pargs else (None, None)
+            if (pargs is None or pname != mu_name or len(grids) != 1 or not pidx
+                    or _var(pidx[0]) != grids[0][0] or _var(pargs[1]) is None):
+                _fail(f"GMM: MWG [{mu_name}] needs the proposal MvNormal({mu_name}[k], <cov hyper>) "
+                      f"for k <- 0 until K")
+            self.mu_prop_cov_name = _var(pargs[1])
 
     def build(self, hypers, data, opt):
         y = np.asarray(np.stack([np.asarray(r, np.float64) for r in data[self.names["y"]]]),
@@ -369,9 +392,16 @@ class GmmPlan:
         else:
             df, scale0 = self.Dm + 2, np.eye(self.Dm)
         self.hyp = dict(alpha=alpha, mu0=mu0, cov0=cov0, df=df, scale0=scale0)
+        sim_len, eps = self.mu_settings
+        prop_cov = (np.asarray(hypers[self.mu_prop_cov_name], np.float64)
+                    if self.mu_prop_cov_name else None)
+        fixed = 0 if opt.hmc_jitter else max(1, int(round(sim_len / eps)))
         s = runtime.GmmSampler(y.astype(np.float32), alpha, mu0, cov0, df, scale0,
                                chains=opt.chains, seed=opt.seed, sample_pi=self.sample_pi,
-                               sample_cov=self.sample_cov)
+                               sample_cov=self.sample_cov, mu_kernel=self.mu_kernel,
+                               mu_sim_len=sim_len, mu_step_size=eps,
+                               mu_fixed_steps=fixed if self.mu_kernel == GMM_MU_HMC else 0,
+                               mu_prop_cov=prop_cov, ref_compat=opt.ref_compat)
         if not self.sample_pi:
             s.set("pis", pis)
         if not self.sample_cov:

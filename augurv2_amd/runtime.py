@@ -238,17 +238,33 @@ class GmmSampler(Sampler):
     family = "gmm"
 
     def __init__(self, y, alpha, mu0, cov0, df, scale0, chains=1, seed=0, sample_pi=True,
-                 sample_cov=True):
+                 sample_cov=True, mu_kernel=0, mu_sim_len=1.0, mu_step_size=0.05,
+                 mu_fixed_steps=0, mu_prop_cov=None, ref_compat=True):
+        """mu_kernel: L.GMM_MU_* (ConjGibbs, ESlice, HMC, MWG, RSlice)."""
         L.require_gpu()
         self.y = L.f32(y)
         n, dim = self.y.shape
         self.alpha, self.mu0 = L.f64(alpha), L.f64(mu0)
         self.cov0, self.scale0 = L.f64(cov0), L.f64(scale0)
+        self.prop_cov = L.f64(mu_prop_cov) if mu_prop_cov is not None else None
         k = len(self.alpha)
         desc = L.GmmDesc(n, dim, k, chains, seed & 0xFFFFFFFF, L.ptr(self.y).value,
                          L.ptr(self.alpha).value, L.ptr(self.mu0).value, L.ptr(self.cov0).value,
-                         int(df), L.ptr(self.scale0).value, int(sample_pi), int(sample_cov))
+                         int(df), L.ptr(self.scale0).value, int(sample_pi), int(sample_cov),
+                         int(mu_kernel), float(mu_sim_len), float(mu_step_size),
+                         int(mu_fixed_steps),
+                         L.ptr(self.prop_cov).value if self.prop_cov is not None else None,
+                         int(ref_compat))
         h = C.c_void_p()
         L.call("augur_gmm_create", C.byref(desc), C.byref(h))
         super().__init__(h, chains, {"pis": ((k,), np.float64), "mu": ((k, dim), np.float64),
                                      "cov": ((k, dim, dim), np.float64), "z": ((n,), np.int32)})
+
+    def mu_acceptance(self):
+        """Per chain: accepted / proposed moves of a non-conjugate mu kernel
+        (ESlice counts a move when the slice returned a new point)."""
+        out = np.zeros(1 + 2 * self.chains)
+        L.call("augur_stats", self._h, L.ptr(out), out.nbytes)
+        a = out[1:].reshape(self.chains, 2)
+        with np.errstate(invalid="ignore"):
+            return a[:, 0] / a[:, 1]

@@ -200,7 +200,18 @@ augur_status augur_bhlr_step_sizes(augur_sampler* s, double* eps, double* v_var,
 /* hmvgmm.rv: pis ~ Dirichlet(alpha); mu[k] ~ MvNormal(mu0, cov0);
  * cov[k] ~ InvWishart(df, scale0); z[n] ~ Categorical(pis);
  * y[n] ~ MvNormal(mu[z[n]], cov[z[n]]).  sample_pi / sample_cov = 0 keep
- * pis / cov fixed at their set_pt values (mvgmm.rv). */
+ * pis / cov fixed at their set_pt values (mvgmm.rv).  mu_kernel names the
+ * schedule's kernel for mu (compiler/augur/test/hmvgmm1-3.infer,
+ * mvgmm1-5.infer); the others stay ConjGibbs [pis], ConjGibbs [cov] and
+ * DiscGibbs [z], in the order pis, mu, cov, z. */
+#define AUGUR_GMM_MU_CONJ 0   /* ConjGibbs [mu] (Core/CgConj.hs:186-206) */
+#define AUGUR_GMM_MU_ESLICE 1 /* ESlice [mu], per component (cbits/augur_mcmclib.h:439-485) */
+#define AUGUR_GMM_MU_HMC 2    /* HMC [mu] [sim_len, step_size], all components in one block
+                                 (cbits/augur_mcmclib.h:389-424) */
+#define AUGUR_GMM_MU_MWG 3    /* MWG [mu] ~ MvNormal(mu[k], mu_prop_cov) for k, per component
+                                 (cbits/augur_mcmclib.h:178-193, Core/CgUserProp.hs) */
+#define AUGUR_GMM_MU_RSLICE 4 /* RSlice [mu] [sim_len, step_size], one block
+                                 (cbits/augur_mcmclib.h:500-543) */
 typedef struct {
   int n_points;      /* N */
   int dim;           /* D, 1..16 */
@@ -215,6 +226,13 @@ typedef struct {
   const double* scale0;  /* D x D */
   int sample_pi;
   int sample_cov;
+  int mu_kernel;             /* AUGUR_GMM_MU_* */
+  double mu_sim_len;         /* HMC / RSlice (Core/KernSyn.hs defaults 1.0, 0.05) */
+  double mu_step_size;
+  int mu_fixed_steps;        /* HMC: as augur_hmc_block.fixed_steps */
+  const double* mu_prop_cov; /* MWG: D x D proposal covariance */
+  int ref_compat;            /* MWG: keep the reference's Hastings term (both proposal
+                                densities evaluated at the current point) */
 } augur_gmm_desc;
 augur_status augur_gmm_create(const augur_gmm_desc* desc, augur_sampler** out);
 
@@ -242,7 +260,8 @@ augur_status augur_trace_read(augur_sampler* s, const char* name, void* dst, siz
                               int* n_recorded);
 /* Per-chain sampler statistics: HMC blocks report accepted proposals and
  * proposals for each block ([block][chain][2] as double); Gibbs samplers
- * report the sweep count. */
+ * report the sweep count (the GMM then, when bytes allow, [chain][2]
+ * accepted and proposed moves of a non-conjugate mu kernel). */
 augur_status augur_stats(augur_sampler* s, double* out, size_t bytes);
 /* Number of sweeps run so far (the Philox sweep counter). */
 augur_status augur_sweep_count(augur_sampler* s, uint32_t* out);

@@ -753,6 +753,223 @@ void or_gmm_zstep(const or_gmm_model* m, const double* pis, const double* mu,
   free(lg);
 }
 
+/* ---- non-conjugate moves of the GMM means ---- */
+
+double or_gmm_comp_like(const or_gmm_model* m, const int* z, int k, const double* mu_k,
+                        const double* cov_k) {
+  double s = 0.0;
+  for (int n = 0; n < m->n; n++)
+    if (z[n] == k) s += or_mvnormal_ll(m->y + (size_t)n * m->d, mu_k, cov_k, m->d);
+  return s;
+}
+
+static double or_gmm_comp_fc(const or_gmm_model* m, const int* z, int k, const double* mu_k,
+                             const double* cov_k) {
+  return or_mvnormal_ll(mu_k, m->mu0, m->cov0, m->d) + or_gmm_comp_like(m, z, k, mu_k, cov_k);
+}
+
+double or_gmm_mu_fc(const or_gmm_model* m, const int* z, const double* mu, const double* cov) {
+  double s = 0.0;
+  for (int k = 0; k < m->k; k++)
+    s += or_gmm_comp_fc(m, z, k, mu + (size_t)k * m->d, cov + (size_t)k * m->d * m->d);
+  return s;
+}
+
+/* d/dmu_k: -cov0^-1 (mu_k - mu0) + sum_{z_n = k} cov_k^-1 (y_n - mu_k)
+ * (the gradients of augur_mvnormal_ll in its point and mean) */
+void or_gmm_mu_grad(const or_gmm_model* m, const int* z, const double* mu, const double* cov,
+                    double* g) {
+  const int d = m->d;
+  double p0[32 * 32], pk[32 * 32], r[32];
+  memcpy(p0, m->cov0, sizeof(double) * d * d);
+  or_mat_cholesky(p0, d);
+  or_mat_cholesky_invert(p0, d);
+  for (int k = 0; k < m->k; k++) {
+    const double* mk = mu + (size_t)k * d;
+    memcpy(pk, cov + (size_t)k * d * d, sizeof(double) * d * d);
+    or_mat_cholesky(pk, d);
+    or_mat_cholesky_invert(pk, d);
+    for (int i = 0; i < d; i++) r[i] = 0.0;
+    for (int n = 0; n < m->n; n++) {
+      if (z[n] != k) continue;
+      for (int i = 0; i < d; i++) r[i] += m->y[(size_t)n * d + i] - mk[i];
+    }
+    for (int i = 0; i < d; i++) {
+      double a = 0.0, b = 0.0;
+      for (int j = 0; j < d; j++) {
+        a += pk[i * d + j] * r[j];
+        b += p0[i * d + j] * (mk[j] - m->mu0[j]);
+      }
+      g[(size_t)k * d + i] = a - b;
+    }
+  }
+}
+
+/* cbits/augur_mcmclib.h:439-485 (augur_mcmc_eslice) for each component in
+ * turn (Core/CgESlice.hs): prior MvNormal(mu0, cov0), objective the
+ * component's likelihood. */
+static int or_gmm_mu_eslice(const or_gmm_model* m, double* mu, const double* cov, const int* z,
+                            uint32_t seed, uint32_t chain, uint32_t sweep) {
+  const int d = m->d;
+  const uint32_t tu = OR_TAG_GMM_MU_MOVE + 2;
+  int acc = 0;
+  double nu[32], prop[32];
+  for (int k = 0; k < m->k; k++) {
+    double* mk = mu + (size_t)k * d;
+    const double* ck = cov + (size_t)k * d * d;
+    or_mvnormal_sample(seed, chain, sweep, OR_TAG_GMM_MU_MOVE, (uint32_t)(k * d), m->mu0, m->cov0, d, nu);
+    uint32_t w[4];
+    or_draw4(seed, chain, (uint32_t)k, 1, sweep, tu, w);
+    const double log_y = or_gmm_comp_like(m, z, k, mk, ck) + log(or_u01(w[0]));
+    or_draw4(seed, chain, (uint32_t)k, 2, sweep, tu, w);
+    double theta = 2.0 * OR_PI * or_u01(w[0]);
+    double tmin = theta - 2.0 * OR_PI, tmax = theta;
+    for (int j = 0; j < 200; j++) {
+      if (tmax - tmin < 1e-8) break; /* keep the current point */
+      for (int i = 0; i < d; i++)
+        prop[i] = (mk[i] - m->mu0[i]) * cos(theta) + (nu[i] - m->mu0[i]) * sin(theta) + m->mu0[i];
+      if (or_gmm_comp_like(m, z, k, prop, ck) > log_y) {
+        memcpy(mk, prop, sizeof(double) * d);
+        acc++;
+        break;
+      }
+      if (theta < 0) tmin = theta;
+      else tmax = theta;
+      or_draw4(seed, chain, (uint32_t)k, (uint32_t)(3 + j), sweep, tu, w);
+      theta = tmin + (tmax - tmin) * or_u01(w[0]);
+    }
+  }
+  return acc;
+}
+
+/* cbits/augur_mcmclib.h:178-193 with the proposal of Core/CgUserProp.hs:
+ * mu'_k ~ MvNormal(mu_k, P); llCorrect = ll(mu_k | mu_k, P) - ll(mu'_k | mu_k, P)
+ * (ref_compat; 0 otherwise), accepted by :89-105 on the component's full
+ * conditional. */
+static int or_gmm_mu_mwg(const or_gmm_model* m, double* mu, const double* cov, const int* z,
+                         uint32_t seed, uint32_t chain, uint32_t sweep) {
+  const int d = m->d;
+  int acc = 0;
+  double prop[32];
+  for (int k = 0; k < m->k; k++) {
+    double* mk = mu + (size_t)k * d;
+    const double* ck = cov + (size_t)k * d * d;
+    or_mvnormal_sample(seed, chain, sweep, OR_TAG_GMM_MU_MOVE, (uint32_t)(k * d), mk, m->mu_prop_cov, d, prop);
+    double corr = 0.0;
+    if (m->ref_compat)
+      corr = or_mvnormal_ll(mk, mk, m->mu_prop_cov, d) - or_mvnormal_ll(prop, mk, m->mu_prop_cov, d);
+    uint32_t w[4];
+    or_draw4(seed, chain, (uint32_t)k, 1, sweep, OR_TAG_GMM_MU_MOVE + 2, w);
+    const double diff = or_gmm_comp_fc(m, z, k, prop, ck) - or_gmm_comp_fc(m, z, k, mk, ck);
+    if (log(or_u01(w[0])) < diff + corr) {
+      memcpy(mk, prop, sizeof(double) * d);
+      acc++;
+    }
+  }
+  return acc;
+}
+
+/* cbits/augur_mcmclib.h:389-424 over the block of all means (HMC [mu]). */
+static int or_gmm_mu_hmc(const or_gmm_model* m, double* mu, const double* cov, const int* z,
+                         uint32_t seed, uint32_t chain, uint32_t sweep) {
+  const int dim = m->k * m->d;
+  double* p0 = (double*)malloc(sizeof(double) * dim);
+  double* p = (double*)malloc(sizeof(double) * dim);
+  double* g = (double*)malloc(sizeof(double) * dim);
+  double* x = (double*)malloc(sizeof(double) * dim);
+  for (int i = 0; i < dim; i++) p0[i] = or_normal(seed, chain, (uint32_t)i, 0, sweep, OR_TAG_GMM_MU_MOVE);
+  double step;
+  int steps;
+  uint32_t w[4];
+  or_draw4(seed, chain, 0, 0, sweep, OR_TAG_GMM_MU_MOVE + 1, w);
+  if (m->mu_fixed_steps > 0) {
+    step = m->mu_step_size;
+    steps = m->mu_fixed_steps;
+  } else if (m->mu_fixed_steps < 0) {
+    step = (0.8 + 0.4 * or_u01(w[0])) * m->mu_step_size;
+    steps = -m->mu_fixed_steps;
+  } else {
+    step = (0.8 + 0.4 * or_u01(w[0])) * m->mu_step_size;
+    double r = round((0.9 + 0.2 * or_u01(w[1])) * m->mu_sim_len / step);
+    steps = r < 1.0 ? 1 : (int)r;
+  }
+  memcpy(x, mu, sizeof(double) * dim);
+  or_gmm_mu_grad(m, z, x, cov, g);
+  for (int i = 0; i < dim; i++) p[i] = p0[i] + 0.5 * step * g[i];
+  for (int l = 0; l < steps; l++) {
+    for (int i = 0; i < dim; i++) x[i] += step * p[i];
+    if (l != steps - 1) {
+      or_gmm_mu_grad(m, z, x, cov, g);
+      for (int i = 0; i < dim; i++) p[i] += step * g[i];
+    }
+  }
+  or_gmm_mu_grad(m, z, x, cov, g);
+  for (int i = 0; i < dim; i++) p[i] += 0.5 * step * g[i];
+  double k0 = 0.0, k1 = 0.0;
+  for (int i = 0; i < dim; i++) {
+    k0 += p0[i] * p0[i];
+    k1 += p[i] * p[i];
+  }
+  const double ll0 = or_gmm_mu_fc(m, z, mu, cov), ll1 = or_gmm_mu_fc(m, z, x, cov);
+  or_draw4(seed, chain, 0, 1, sweep, OR_TAG_GMM_MU_MOVE + 2, w);
+  const int acc = log(or_u01(w[0])) < ll1 - ll0 + (0.5 * k0 - 0.5 * k1);
+  if (acc) memcpy(mu, x, sizeof(double) * dim);
+  free(p0);
+  free(p);
+  free(g);
+  free(x);
+  return acc;
+}
+
+/* cbits/augur_mcmclib.h:500-543 (h_augur_mcmc_refl_slice) over the block of
+ * all means: slice level ll - Exp(1), max(1, round(simLen / eps)) steps of
+ * eps, the momentum reflected on the gradient whenever a step leaves the
+ * slice, the end point kept if it lies in the slice. */
+static int or_gmm_mu_rslice(const or_gmm_model* m, double* mu, const double* cov, const int* z,
+                            uint32_t seed, uint32_t chain, uint32_t sweep) {
+  const int dim = m->k * m->d;
+  double* p = (double*)malloc(sizeof(double) * dim);
+  double* g = (double*)malloc(sizeof(double) * dim);
+  double* x = (double*)malloc(sizeof(double) * dim);
+  for (int i = 0; i < dim; i++) p[i] = or_normal(seed, chain, (uint32_t)i, 0, sweep, OR_TAG_GMM_MU_MOVE);
+  uint32_t w[4];
+  or_draw4(seed, chain, 0, 1, sweep, OR_TAG_GMM_MU_MOVE + 2, w);
+  const double level = or_gmm_mu_fc(m, z, mu, cov) - (-log(or_u01(w[0])));
+  const double r = round(m->mu_sim_len / m->mu_step_size);
+  const int steps = r < 1.0 ? 1 : (int)r;
+  memcpy(x, mu, sizeof(double) * dim);
+  int cond = 0;
+  for (int l = 0; l < steps; l++) {
+    for (int i = 0; i < dim; i++) x[i] += m->mu_step_size * p[i];
+    cond = or_gmm_mu_fc(m, z, x, cov) < level;
+    if (cond) {
+      or_gmm_mu_grad(m, z, x, cov, g);
+      double a = 0.0, b = 0.0;
+      for (int i = 0; i < dim; i++) {
+        a += p[i] * g[i];
+        b += g[i] * g[i];
+      }
+      for (int i = 0; i < dim; i++) p[i] += -2.0 * (a / b) * g[i];
+    }
+  }
+  if (!cond) memcpy(mu, x, sizeof(double) * dim);
+  free(p);
+  free(g);
+  free(x);
+  return !cond;
+}
+
+int or_gmm_mu_move(const or_gmm_model* m, double* mu, const double* cov, const int* z,
+                   uint32_t seed, uint32_t chain, uint32_t sweep) {
+  switch (m->mu_kernel) {
+    case 1: return or_gmm_mu_eslice(m, mu, cov, z, seed, chain, sweep);
+    case 2: return or_gmm_mu_hmc(m, mu, cov, z, seed, chain, sweep);
+    case 3: return or_gmm_mu_mwg(m, mu, cov, z, seed, chain, sweep);
+    case 4: return or_gmm_mu_rslice(m, mu, cov, z, seed, chain, sweep);
+    default: return 0;
+  }
+}
+
 void or_gmm_sweep(const or_gmm_model* m, double* pis, double* mu, double* cov, int* z,
                   uint32_t seed, uint32_t chain, uint32_t sweep) {
   const int d = m->d, k = m->k, n = m->n;
@@ -765,11 +982,16 @@ void or_gmm_sweep(const or_gmm_model* m, double* pis, double* mu, double* cov, i
   }
   /* ConjGibbs [pis]: Core/CgConj.hs:171-183 -> augur_dirichlet_categorical_conj */
   if (m->sample_pi) or_dirichlet_conj(seed, chain, sweep, OR_TAG_GMM_PI, 0, m->alpha, cnt, k, pis);
-  /* ConjGibbs [mu]: Core/CgConj.hs:186-206 -> augur_mvnormal_mvnormal_conj */
-  for (int j = 0; j < k; j++)
-    or_mvnormal_mvnormal_conj(seed, chain, sweep, (uint32_t)(j * d), m->mu0, m->cov0,
-                              sum + (size_t)j * d, cnt[j], cov + (size_t)j * d * d, d,
-                              mu + (size_t)j * d);
+  /* ConjGibbs [mu]: Core/CgConj.hs:186-206 -> augur_mvnormal_mvnormal_conj,
+   * or the schedule's other kernel for mu */
+  if (m->mu_kernel == 0) {
+    for (int j = 0; j < k; j++)
+      or_mvnormal_mvnormal_conj(seed, chain, sweep, (uint32_t)(j * d), m->mu0, m->cov0,
+                                sum + (size_t)j * d, cnt[j], cov + (size_t)j * d * d, d,
+                                mu + (size_t)j * d);
+  } else {
+    or_gmm_mu_move(m, mu, cov, z, seed, chain, sweep);
+  }
   /* ConjGibbs [cov]: Core/CgConj.hs:209-246 -> augur_invwishart_mvnormal_conj
    * (cbits/augur_dist.c:751-757): IW(df + cnt, scale0 + sum (y - mu)(y - mu)^T) */
   if (m->sample_cov) {

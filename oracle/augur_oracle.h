@@ -41,6 +41,8 @@ enum {
   OR_TAG_GMM_MU = 5,
   OR_TAG_GMM_COV = 6,
   OR_TAG_GMM_Z = 7,
+  /* non-conjugate moves of the GMM means: 8 normals, 9 HMC jitter, 10 uniforms */
+  OR_TAG_GMM_MU_MOVE = 8,
   /* HMC block b uses tags 16 + 4 b + {0 momentum, 1 jitter, 2 accept} */
   OR_TAG_HMC_BASE = 16
 };
@@ -138,7 +140,28 @@ typedef struct {
   int df;
   const double* scale0; /* d x d */
   int sample_pi, sample_cov;
+  /* the schedule's kernel for mu: 0 ConjGibbs, 1 ESlice, 2 HMC, 3 MWG, 4 RSlice */
+  int mu_kernel;
+  double mu_sim_len, mu_step_size;
+  int mu_fixed_steps;         /* HMC: > 0 fixed, < 0 step jitter only, 0 reference jitter */
+  const double* mu_prop_cov;  /* MWG proposal covariance (d x d) */
+  int ref_compat;             /* MWG: the reference's Hastings term */
 } or_gmm_model;
+
+/* Log-likelihood of component k's mean: the data terms of the points with
+ * z_n = k (the reference's single-index likelihood, Core/CgLike.hs
+ * runLLFnOne), each by or_mvnormal_ll (cbits/augur_dist.c:548-568). */
+double or_gmm_comp_like(const or_gmm_model* m, const int* z, int k, const double* mu_k,
+                        const double* cov_k);
+/* Log full conditional of the mean block: priors MvNormal(mu0, cov0) of
+ * every component plus every data term; and its gradient. */
+double or_gmm_mu_fc(const or_gmm_model* m, const int* z, const double* mu, const double* cov);
+void or_gmm_mu_grad(const or_gmm_model* m, const int* z, const double* mu, const double* cov,
+                    double* g);
+/* The model's mu_kernel move of all means (chain, sweep streams); returns
+ * the number of accepted moves (per component for ESlice / MWG). */
+int or_gmm_mu_move(const or_gmm_model* m, double* mu, const double* cov, const int* z,
+                   uint32_t seed, uint32_t chain, uint32_t sweep);
 
 void or_gmm_sweep(const or_gmm_model* m, double* pis, double* mu, double* cov, int* z,
                   uint32_t seed, uint32_t chain, uint32_t sweep);

@@ -40,7 +40,9 @@ class GmmModel(C.Structure):
     _fields_ = [("n", C.c_int), ("d", C.c_int), ("k", C.c_int), ("y", C.c_void_p),
                 ("alpha", C.c_void_p), ("mu0", C.c_void_p), ("cov0", C.c_void_p),
                 ("df", C.c_int), ("scale0", C.c_void_p), ("sample_pi", C.c_int),
-                ("sample_cov", C.c_int)]
+                ("sample_cov", C.c_int), ("mu_kernel", C.c_int), ("mu_sim_len", C.c_double),
+                ("mu_step_size", C.c_double), ("mu_fixed_steps", C.c_int),
+                ("mu_prop_cov", C.c_void_p), ("ref_compat", C.c_int)]
 
 
 def lib():
@@ -99,6 +101,10 @@ def lib():
             "or_dirichlet_ll": (C.c_double, [P_D, P_D, C.c_int]),
             "or_invwishart_ll": (C.c_double, [P_D, C.c_int, P_D, C.c_int]),
             "or_gmm_ll_mod": (C.c_double, [C.POINTER(GmmModel), P_D, P_D, P_D, P_I]),
+            "or_gmm_comp_like": (C.c_double, [C.POINTER(GmmModel), P_I, C.c_int, P_D, P_D]),
+            "or_gmm_mu_fc": (C.c_double, [C.POINTER(GmmModel), P_I, P_D, P_D]),
+            "or_gmm_mu_grad": (None, [C.POINTER(GmmModel), P_I, P_D, P_D, P_D]),
+            "or_gmm_mu_move": (C.c_int, [C.POINTER(GmmModel), P_D, P_D, P_I, U32, U32, U32]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -273,7 +279,12 @@ class Hlr:
 
 
 class Gmm:
-    def __init__(self, y, alpha, mu0, cov0, df, scale0, sample_pi=True, sample_cov=True):
+    """mu_kernel: 0 ConjGibbs, 1 ESlice, 2 HMC, 3 MWG (proposal MvNormal(mu[k],
+    mu_prop_cov)), 4 RSlice."""
+
+    def __init__(self, y, alpha, mu0, cov0, df, scale0, sample_pi=True, sample_cov=True,
+                 mu_kernel=0, mu_sim_len=1.0, mu_step_size=0.05, mu_fixed_steps=0,
+                 mu_prop_cov=None, ref_compat=True):
         self.y = np.ascontiguousarray(y, np.float64)
         n, d = self.y.shape
         k = len(alpha)
@@ -281,9 +292,25 @@ class Gmm:
         self.mu0 = _sq(mu0)
         self.cov0 = _sq(cov0)
         self.scale0 = _sq(scale0)
+        self.prop_cov = _sq(mu_prop_cov if mu_prop_cov is not None else cov0)
         self.m = GmmModel(n, d, k, self.y.ctypes.data, self.alpha.ctypes.data,
                           self.mu0.ctypes.data, self.cov0.ctypes.data, int(df),
-                          self.scale0.ctypes.data, int(sample_pi), int(sample_cov))
+                          self.scale0.ctypes.data, int(sample_pi), int(sample_cov),
+                          int(mu_kernel), float(mu_sim_len), float(mu_step_size),
+                          int(mu_fixed_steps), self.prop_cov.ctypes.data, int(ref_compat))
+
+    def mu_move(self, mu, cov, z, seed, chain, sweep):
+        """The model's mu kernel applied to mu (k x d, in place)."""
+        return lib().or_gmm_mu_move(C.byref(self.m), mu, _sq(cov), np.ascontiguousarray(z, np.int32),
+                                    seed, chain, sweep)
+
+    def mu_fc(self, mu, cov, z):
+        return lib().or_gmm_mu_fc(C.byref(self.m), np.ascontiguousarray(z, np.int32), _sq(mu), _sq(cov))
+
+    def mu_grad(self, mu, cov, z):
+        g = np.zeros_like(_sq(mu))
+        lib().or_gmm_mu_grad(C.byref(self.m), np.ascontiguousarray(z, np.int32), _sq(mu), _sq(cov), g)
+        return g
 
     def sweep(self, pis, mu, cov, z, seed, chain, sweep):
         lib().or_gmm_sweep(C.byref(self.m), pis, mu, cov, z, seed, chain, sweep)

@@ -101,6 +101,28 @@ def test_reference_models_and_schedules_parse():
         rvparse.parse_schedule(open(f).read())
 
 
+@pytest.mark.skipif(not os.path.isdir(REF_TEST), reason="reference tree not mounted")
+def test_reference_schedules_match_a_plan():
+    """Every model / schedule pair of compiler/augur/test matches a sampler
+    family, except the ones listed as outside this runtime."""
+    pairs = {"lda1": "lda", "hlr1": "hlr", "hlr2": "hlr", "hmvgmm1": "hmvgmm", "hmvgmm2": "hmvgmm",
+             "hmvgmm3": "hmvgmm", "mvgmm1": "mvgmm", "mvgmm2": "mvgmm", "mvgmm3": "mvgmm",
+             "mvgmm4": "mvgmm", "mvgmm5": "mvgmm", "mvgmm6": "mvgmm"}
+    unsupported = {"mvgmm6"}  # NUTS [mu]
+    got = {}
+    for sched, model in pairs.items():
+        m = rvparse.parse_model(open(os.path.join(REF_TEST, model + ".rv")).read())
+        k = rvparse.parse_schedule(open(os.path.join(REF_TEST, sched + ".infer")).read())
+        try:
+            plan.match(m, k)
+            got[sched] = True
+        except plan.UnsupportedModel:
+            got[sched] = False
+    assert {s for s, ok in got.items() if not ok} <= unsupported | {"hlr2"}, got
+    for s in ("hmvgmm2", "hmvgmm3", "mvgmm2", "mvgmm3", "mvgmm4", "mvgmm5"):
+        assert got[s], s
+
+
 def test_plan_matching():
     p = plan.match(rvparse.parse_model(LDA),
                    rvparse.parse_schedule("ConjGibbs [theta] (*) ConjGibbs [phi] (*) DiscGibbs [z]"))
@@ -123,9 +145,15 @@ def test_plan_matching():
     with pytest.raises(plan.UnsupportedModel):  # only the random walk centred at v
         plan.match(rvparse.parse_model(HLR_CFG), rvparse.parse_schedule(
             "MWG [v] ~ Normal(1.0, 1.0) (*) HMC [beta]"))
-    with pytest.raises(plan.UnsupportedModel):
+    p = plan.match(rvparse.parse_model(HMVGMM), rvparse.parse_schedule(
+        "ConjGibbs [pis] (*) ESlice [mu] (*) ConjGibbs [cov] (*) DiscGibbs [z]"))
+    assert p.mu_kernel == 1
+    p = plan.match(rvparse.parse_model(HMVGMM), rvparse.parse_schedule(
+        "ConjGibbs [pis] (*) HMC [mu] [0.5, 0.05] (*) ConjGibbs [cov] (*) DiscGibbs [z]"))
+    assert p.mu_kernel == 2 and p.mu_settings == (0.5, 0.05)
+    with pytest.raises(plan.UnsupportedModel):  # the kernels must keep the order pis, mu, cov, z
         plan.match(rvparse.parse_model(HMVGMM), rvparse.parse_schedule(
-            "ConjGibbs [pis] (*) ESlice [mu] (*) ConjGibbs [cov] (*) DiscGibbs [z]"))
+            "ConjGibbs [pis] (*) ConjGibbs [cov] (*) ESlice [mu] (*) DiscGibbs [z]"))
     with pytest.raises(NameError):  # UnsupportedModel is the reference's NameError
         plan.match(rvparse.parse_model("(a : Real) => { param p ~ Normal(0.0, a) ; }"), None)
 

@@ -90,19 +90,34 @@ def test_hlr_example_schedule():
         assert abs(np.mean(s["b"]) + 0.5) < 0.25
 
 
-def test_mvgmm_example():
-    # examples/mvgmm.py:39-73 (K = 3, D = 2, fixed pis and covs)
+@pytest.mark.parametrize("sched", [
+    "ConjGibbs [mu] (*) DiscGibbs [z]",                                  # examples/mvgmm.py sched1
+    "ESlice [mu] (*) DiscGibbs [z]",                                     # sched2
+    "HMC [mu] [0.2, 0.01] (*) DiscGibbs [z]",                            # sched3
+    "MWG [mu] ~ MvNormal(mu[k], covs0) for k <- 0 until K (*) DiscGibbs [z]",  # sched4
+    "RSlice [mu] [0.05, 0.01] (*) DiscGibbs [z]",                        # mvgmm5.infer
+])
+def test_mvgmm_example(sched):
+    # examples/mvgmm.py:39-73 (K = 3, D = 2, fixed pis and covs) with each of
+    # its schedules
     rng = np.random.default_rng(1)
     K, D, N = 3, 2, 300
     mu0, covs0, pis = np.zeros(D), np.eye(D), np.full(K, 1.0 / K)
     covs = np.array([[0.1, 0.0], [0.0, 0.1]])
     centers = [np.array([0.0, 2.0]), np.array([2.0, -2.0]), np.array([-2.0, -2.0])]
     y = np.array([rng.multivariate_normal(centers[n % K], covs) for n in range(N)])
+    opt = AugurOpt(target="gpu", seed=4, chains=4)
+    if sched.startswith("MWG"):
+        # the proposal MvNormal(mu[k], covs0) has 30x the posterior's spread
+        # and accepts about one move in a thousand: it holds the centres (from
+        # which it starts here) rather than finding them
+        opt = AugurOpt(target="gpu", seed=4, chains=4, initStrat=AugurUser(
+            {"mu": np.array(centers), "z": (np.arange(N) % K).astype(np.int32)}))
     with AugurInfer("config.yml", MVGMM) as infer:
-        infer.set_compile_opt(AugurOpt(target="gpu", seed=4, chains=4))
-        infer.set_user_sched("ConjGibbs [mu] (*) DiscGibbs [z]")
+        infer.set_compile_opt(opt)
+        infer.set_user_sched(sched)
         infer.compile(K, N, mu0, covs0, pis, covs)(y)
-        s = infer.samplen(burnIn=50, numSamples=20)
+        s = infer.samplen(burnIn=150, numSamples=20)
         ll = infer.curr_ll()
         # Gibbs from a random start can sit in a local mode (two components on
         # one cluster); the chain of highest log density holds the global one,

@@ -425,3 +425,103 @@ def test_gmm_ll_mod_terms(oracle, sample_pi, sample_cov):
         quirk = d * (d - 1) * 0.25 * (np.log(3.1415) - np.log(np.pi))
         ref += sum(sps.invwishart.logpdf(cov[c], df, scale0) - quirk for c in range(k))
     assert abs(g.ll_mod(pis, mu, cov, z) - ref) < 1e-9 * abs(ref)
+
+
+# ---- non-conjugate moves of the GMM means (ESlice, HMC, MWG, RSlice) ----
+
+def _mu_problem(rng, n=30, d=2, k=2):
+    mu_true = rng.standard_normal((k, d)) * 2
+    z = (np.arange(n) % k).astype(np.int32)
+    cov = np.stack([np.eye(d) * (0.5 + 0.3 * j) + 0.1 for j in range(k)])
+    y = np.stack([rng.multivariate_normal(mu_true[z[i]], cov[z[i]]) for i in range(n)])
+    mu0, cov0 = rng.standard_normal(d) * 0.5, np.eye(d) * 3.0 + 0.5
+    return y, z, cov, mu0, cov0
+
+
+def _mu_posterior(y, z, cov, mu0, cov0, k):
+    """Conditional of each mean given z and cov (MvNormal prior, MvNormal
+    data): the textbook conjugate posterior."""
+    i0 = np.linalg.inv(cov0)
+    out = []
+    for j in range(k):
+        yj = y[z == j]
+        ik = np.linalg.inv(cov[j])
+        a = np.linalg.inv(i0 + len(yj) * ik)
+        out.append((a @ (i0 @ mu0 + ik @ yj.sum(axis=0)), a))
+    return out
+
+
+def test_gmm_mu_fc_and_gradient(oracle):
+    rng = np.random.default_rng(40)
+    y, z, cov, mu0, cov0 = _mu_problem(rng)
+    k, d = cov.shape[0], y.shape[1]
+    g = oracle.Gmm(y, np.ones(k), mu0, cov0, d + 2, np.eye(d), sample_pi=False, sample_cov=False,
+                   mu_kernel=2)
+    mu = rng.standard_normal((k, d))
+    ref = sum(sps.multivariate_normal.logpdf(mu[j], mu0, cov0) for j in range(k))
+    ref += sum(sps.multivariate_normal.logpdf(y[i], mu[z[i]], cov[z[i]]) for i in range(len(y)))
+    assert abs(g.mu_fc(mu, cov, z) - ref) <= 1e-9 * abs(ref)
+    grad = g.mu_grad(mu, cov, z)
+    h = 1e-6
+    for j in range(k):
+        for i in range(d):
+            e = np.zeros_like(mu)
+            e[j, i] = h
+            fd = (g.mu_fc(mu + e, cov, z) - g.mu_fc(mu - e, cov, z)) / (2 * h)
+            assert abs(grad[j, i] - fd) <= 1e-6 * max(1.0, abs(fd))
+
+
+@pytest.mark.parametrize("kernel,settings", [
+    (1, {}),                                             # ESlice [mu]
+    (2, dict(mu_sim_len=1.0, mu_step_size=0.1)),         # HMC [mu] (reference jitter)
+    (3, dict(ref_compat=False)),                         # MWG [mu] ~ MvNormal(mu[k], P)
+    (4, dict(mu_sim_len=0.6, mu_step_size=0.1)),         # RSlice [mu]
+])
+def test_gmm_mu_kernels_leave_the_conjugate_posterior_invariant(oracle, kernel, settings):
+    """With z and cov fixed the conditional of each mean is the textbook
+    MvNormal posterior: 4000 moves of each kernel (a new sweep counter each)
+    reproduce its mean within 5 MCSE and its variances within 12%."""
+    from augurv2_amd import diagnostics
+    rng = np.random.default_rng(41 + kernel)
+    y, z, cov, mu0, cov0 = _mu_problem(rng)
+    k, d = cov.shape[0], y.shape[1]
+    post = _mu_posterior(y, z, cov, mu0, cov0, k)
+    kw = dict(settings)
+    if kernel == 3:
+        kw["mu_prop_cov"] = 0.6 * post[0][1]
+    g = oracle.Gmm(y, np.ones(k), mu0, cov0, d + 2, np.eye(d), sample_pi=False, sample_cov=False,
+                   mu_kernel=kernel, **kw)
+    mu = np.stack([p[0] for p in post]) + 0.5
+    draws = []
+    acc = 0
+    for sweep in range(4200):
+        acc += g.mu_move(mu, cov, z, 9, 0, sweep)
+        if sweep >= 200:
+            draws.append(mu.copy())
+    draws = np.array(draws)
+    assert acc > 0
+    for j in range(k):
+        m, a = post[j]
+        for i in range(d):
+            x = draws[:, j, i]
+            se = diagnostics.mcse_mean(x[None, :])
+            assert abs(x.mean() - m[i]) <= 5 * se, (j, i, x.mean(), m[i], se)
+            assert abs(x.var() / a[i, i] - 1) <= 0.12, (j, i, x.var(), a[i, i])
+
+
+def test_gmm_mwg_reference_hastings_term(oracle):
+    """ref_compat keeps the reference's llCorrect = ll(mu | mu, P) -
+    ll(mu' | mu, P) (Core/CgUserProp.hs: both proposal densities at the
+    current point), which for a random walk is 1/2 z.z >= 0: it accepts at
+    least as often as the symmetric rule on the same draws."""
+    rng = np.random.default_rng(44)
+    y, z, cov, mu0, cov0 = _mu_problem(rng)
+    k, d = cov.shape[0], y.shape[1]
+    acc = {}
+    for rc in (0, 1):
+        g = oracle.Gmm(y, np.ones(k), mu0, cov0, d + 2, np.eye(d), sample_pi=False,
+                       sample_cov=False, mu_kernel=3, mu_prop_cov=np.eye(d), ref_compat=rc)
+        mu = np.zeros((k, d))
+        acc[rc] = [g.mu_move(mu.copy(), cov, z, 3, 0, s) for s in range(300)]
+    assert all(a1 >= a0 for a0, a1 in zip(acc[0], acc[1]))
+    assert sum(acc[1]) > sum(acc[0])
