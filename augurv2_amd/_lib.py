@@ -16,6 +16,7 @@ STATUS_NAMES = {1: "invalid argument", 2: "HIP error", 3: "invalid state", 4: "u
 
 HMC_BLOCK_V, HMC_BLOCK_B, HMC_BLOCK_THETA, HMC_BLOCK_B_THETA = 0, 1, 2, 3
 MWG_BLOCK_V = 4  # proposal Normal(v, step_size): step_size is the variance
+MWG_BLOCK_B = 5  # proposal Normal(b, step_size) on the intercept
 
 
 class AugurError(RuntimeError):

@@ -96,6 +96,7 @@ void augur_sampler::record_traces() {
   for (auto& kv : traces) {
     Trace& t = kv.second;
     if (t.capacity <= 0 || t.recorded >= t.capacity) continue;
+    prepare(kv.first);
     ParamView v = param(kv.first);
     const size_t per_chain = v.elems * v.elem_size;
     unsigned char* dst = t.buf.p + (size_t)t.recorded * per_chain * chains;
@@ -158,6 +159,7 @@ augur_status augur_cpy(augur_sampler* s, const char* name, int chain, void* dst,
   return guard([&] {
     ParamView v = checked_view(s, name, chain, bytes, false);
     if (!dst) fail(AUGUR_ERR_ARG, "null destination");
+    s->prepare(name);
     AUGUR_HIP(hipMemcpyAsync(dst, (unsigned char*)v.base + v.chain_stride * v.elem_size * chain,
                              bytes, hipMemcpyDeviceToHost, s->stream));
     AUGUR_HIP(hipStreamSynchronize(s->stream));

@@ -142,6 +142,9 @@ struct augur_sampler {
   virtual void stats(double* out, size_t bytes) = 0;
   // Called after a parameter is written through augur_set_pt.
   virtual void on_set(const std::string& name) = 0;
+  // Called before a parameter is read (augur_cpy, traces): computes
+  // deterministic parameters on the device.
+  virtual void prepare(const std::string& name) { (void)name; }
   // Called after every sweep: copies traced parameters.
   void record_traces();
 

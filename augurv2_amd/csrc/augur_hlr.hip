@@ -867,6 +867,87 @@ __global__ __launch_bounds__(64) void k_hmc_vblock_begin(VBlockArgs V, HmcArgs A
   hmc_begin_body(A);
 }
 
+// Metropolis-within-Gibbs on the intercept b, the schedule item
+// `MWG [b] ~ Normal(b, s)` (compiler/augur/test/hlr2.infer;
+// cbits/augur_mcmclib.h:178-193 with Core/CgUserProp.hs's proposal): the
+// proposal b' = b + sqrt(s) z (normal at tag + 0) is written as the pass's
+// FP32 evaluation point; k_mwg_b_end accepts on the full conditional of b,
+// prior Normal(0, v) plus the data term, against the cached data term of the
+// current point (accept uniform at tag + 2).  The Hastings term as for v:
+// (b' - b)^2 / (2 s) with ref_compat, else 0.
+struct MwgBArgs {
+  int d, cp, c_real, ref_compat;
+  double var;
+  uint32_t seed, sweep, tag;
+  double* state;       // C x P
+  double* prop_b;      // C: the proposed b
+  float* beta32;       // cp x (d + 1)
+  double* cache_g;     // cp x (d + 1)
+  double* cache_ll;    // cp
+  const double* gprop; // cp x (d + 1): the pass at the proposal
+  const double* llprop;
+  double* stats;       // [chain][2]
+};
+
+__global__ void k_mwg_b_begin(MwgBArgs A) {
+  const int ds = A.d + 1, P = A.d + 2;
+  for (int q = threadIdx.x; q < A.cp * ds; q += blockDim.x) {
+    const int c = q / ds, j = q % ds;
+    float v = 0.0f;
+    if (c < A.c_real) {
+      const double* st = A.state + (size_t)c * P;
+      if (j < A.d) {
+        v = (float)st[j];
+      } else {
+        const double b1 = st[A.d] + sqrt(A.var) * normal_at(Stream{A.seed, (uint32_t)c, A.sweep, A.tag}, 0, 0);
+        A.prop_b[c] = b1;
+        v = (float)b1;
+      }
+    }
+    A.beta32[q] = v;
+  }
+}
+
+__global__ __launch_bounds__(64) void k_mwg_b_end(MwgBArgs A) {
+  const int c = blockIdx.x, lane = threadIdx.x, d = A.d, ds = d + 1, P = d + 2;
+  if (c >= A.c_real) return;
+  double* st = A.state + (size_t)c * P;
+  const double b0 = st[d], b1 = A.prop_b[c], v = st[d + 1];
+  const double ll0 = normal_ll0(b0, v, A.ref_compat) + A.cache_ll[c];
+  const double ll1 = normal_ll0(b1, v, A.ref_compat) + A.llprop[c];
+  const double extra = A.ref_compat ? 0.5 * (b1 - b0) * (b1 - b0) / A.var : 0.0;
+  const double u = u01(Stream{A.seed, (uint32_t)c, A.sweep, A.tag + 2}.draw(0, 0).x);
+  const bool acc = log(u) < ll1 - ll0 + extra;
+  if (acc) {
+    for (int j = lane; j < ds; j += 64) A.cache_g[c * ds + j] = A.gprop[c * ds + j];
+    if (lane == 0) {
+      st[d] = b1;
+      A.cache_ll[c] = A.llprop[c];
+    }
+  }
+  if (lane == 0) {
+    A.stats[2 * c] += acc ? 1.0 : 0.0;
+    A.stats[2 * c + 1] += 1.0;
+  }
+}
+
+// The deterministic parameter bias[n] = sigmoid(dotprod(theta, x[n]) + b) of
+// compiler/augur/test/dethlr.rv for every chain (Core/CgDet.hs), in FP64
+// from the FP32 rows: out[c][n].
+__global__ void k_hlr_det_bias(const float* __restrict__ x, const double* __restrict__ state, int n, int d,
+                               int chains, int has_b, double* __restrict__ out) {
+  const int P = d + 2;
+  const size_t total = (size_t)chains * n;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const int c = (int)(e / n), r = (int)(e % n);
+    const double* st = state + (size_t)c * P;
+    const float* xr = x + (size_t)r * d;
+    double t = has_b ? st[d] : 0.0;
+    for (int j = 0; j < d; ++j) t += (double)xr[j] * st[j];
+    out[e] = 1.0 / (1.0 + exp(-t));  // cbits/augur_math.c:52-54
+  }
+}
+
 __global__ void k_state_to_beta32(const double* __restrict__ state, float* __restrict__ beta32, int d,
                                   int cp, int c_real, int has_b) {
   const int ds = d + 1, P = d + 2;
@@ -923,6 +1004,8 @@ struct HlrSampler final : augur_sampler {
   SurrInfer inf;
   double resid_frac = -1.0;  // residual variance of the linear term / target variance
   DevBuf<double> ll64_cur, ll64_prop, ll64_part;
+  DevBuf<double> prop_b;  // MWG [b]: the proposals
+  DevBuf<double> det;     // chains x n: the deterministic bias[n] (dethlr.rv), computed on read
   // ll64_cur no longer holds the FP64 data log-density of the current state:
   // set_pt wrote theta / b, or another block moved them since the last
   // surrogate iteration
@@ -937,7 +1020,18 @@ struct HlrSampler final : augur_sampler {
     else if (name == "theta") v = {state.p, sizeof(double), (size_t)d, P, true};
     else if (name == "b" && has_b) v = {state.p + d, sizeof(double), 1, P, true};
     else if (name == "v") v = {state.p + d + 1, sizeof(double), 1, P, true};
+    else if (name == "bias") {
+      if (!det.p) det.alloc((size_t)chains * n);
+      v = {det.p, sizeof(double), (size_t)n, (size_t)n, false};
+    }
     return v;
+  }
+
+  void prepare(const std::string& name) override {
+    if (name != "bias") return;
+    if (!det.p) det.alloc((size_t)chains * n);
+    k_hlr_det_bias<<<1024, 256, 0, stream>>>(x.p, state.p, n, d, chains, has_b, det.p);
+    AUGUR_CHECK_LAUNCH();
   }
 
   void on_set(const std::string& name) override {
@@ -1095,9 +1189,35 @@ struct HlrSampler final : augur_sampler {
     }
   }
 
+  void mwg_b_block(int bi, const augur_hmc_block& blk) {
+    if (!cache_valid) refresh_cache();
+    MwgBArgs M{};
+    M.d = d; M.cp = cp; M.c_real = chains; M.ref_compat = ref_compat; M.var = blk.step_size;
+    M.seed = seed; M.sweep = sweep; M.tag = TAG_HMC_BASE + 4u * (uint32_t)bi;
+    M.state = state.p; M.prop_b = prop_b.p; M.beta32 = beta32.p; M.cache_g = cache_g.p; M.cache_ll = cache_ll.p;
+    M.gprop = gprop.p; M.llprop = llprop.p; M.stats = stats_buf.p + (size_t)bi * chains * 2;
+    k_mwg_b_begin<<<1, 256, 0, stream>>>(M);
+    AUGUR_CHECK_LAUNCH();
+    PassArgs A = pass_args(MODE_CACHE, 0);  // the data term at the proposal, into gprop / llprop
+    A.cache_g = gprop.p;
+    A.cache_ll = llprop.p;
+    launch_pass(A);
+    k_mwg_b_end<<<chains, 64, 0, stream>>>(M);
+    AUGUR_CHECK_LAUNCH();
+    ll64_dirty = true;
+  }
+
   // fuse_v: the v block scheduled just before, run in the same launch as
   // this block's first kernel
   void data_block(int bi, const augur_hmc_block& blk, const VBlockArgs* fuse_v = nullptr) {
+    if (blk.block == AUGUR_MWG_BLOCK_B) {
+      if (fuse_v) {
+        k_hmc_vblock<<<chains, 64, 0, stream>>>(*fuse_v);
+        AUGUR_CHECK_LAUNCH();
+      }
+      mwg_b_block(bi, blk);
+      return;
+    }
     const bool surr_blk = surr_phase != SURR_OFF && bi == scfg.block_index;
     if (surr_blk && surr_phase == SURR_ACTIVE) {
       if (fuse_v) {
@@ -1252,6 +1372,7 @@ struct HlrSampler final : augur_sampler {
     part.alloc((size_t)cp * (d + 2) * G);
     ll_out.alloc(cp);
     stats_buf.alloc(std::max<size_t>(1, 4 * (size_t)chains * 2));
+    prop_b.alloc(cp);
     state.zero(stream); prop.zero(stream); beta32.zero(stream);
     stats_buf.zero(stream); p.zero(stream); p0.zero(stream);
   }
@@ -1333,10 +1454,11 @@ augur_status augur_hlr_create(const augur_hlr_desc* D, augur_sampler** out) {
     if (D->n_blocks < 1 || D->n_blocks > 4) fail(AUGUR_ERR_ARG, "1..4 HMC blocks");
     for (int i = 0; i < D->n_blocks; ++i) {
       const augur_hmc_block& b = D->blocks[i];
-      if (b.block < 0 || b.block > AUGUR_MWG_BLOCK_V) fail(AUGUR_ERR_ARG, "bad block kind %d", b.block);
-      if ((b.block == AUGUR_HMC_BLOCK_B || b.block == AUGUR_HMC_BLOCK_B_THETA) && !D->has_intercept)
-        fail(AUGUR_ERR_ARG, "HMC block over b without an intercept");
-      if (b.block == AUGUR_MWG_BLOCK_V) {
+      if (b.block < 0 || b.block > AUGUR_MWG_BLOCK_B) fail(AUGUR_ERR_ARG, "bad block kind %d", b.block);
+      if ((b.block == AUGUR_HMC_BLOCK_B || b.block == AUGUR_HMC_BLOCK_B_THETA || b.block == AUGUR_MWG_BLOCK_B) &&
+          !D->has_intercept)
+        fail(AUGUR_ERR_ARG, "block over b without an intercept");
+      if (b.block == AUGUR_MWG_BLOCK_V || b.block == AUGUR_MWG_BLOCK_B) {
         if (!(b.step_size > 0.0)) fail(AUGUR_ERR_ARG, "MWG block %d needs a proposal variance > 0", i);
         continue;
       }

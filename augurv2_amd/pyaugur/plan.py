@@ -5,7 +5,9 @@ reference's Python value containers and the C-ABI buffers.
 Families (each recognised up to variable names):
   LDA  -- compiler/augur/test/lda.rv with ConjGibbs theta, phi, DiscGibbs z
   HLR  -- compiler/augur/test/hlr.rv family (InvGamma or Exponential prior on
-          the variance, optional intercept) with HMC blocks
+          the variance, optional intercept, the Bernoulli bias optionally a
+          deterministic parameter as in dethlr.rv) with HMC blocks and MWG on
+          v or b
   GMM  -- compiler/augur/test/hmvgmm.rv (and mvgmm.rv: fixed pis and covs)
           with ConjGibbs pis, ConjGibbs | ESlice | HMC | MWG | RSlice mu,
           ConjGibbs cov, DiscGibbs z
@@ -16,7 +18,7 @@ import numpy as np
 
 from .. import runtime
 from .._lib import (GMM_MU_CONJ, GMM_MU_ESLICE, GMM_MU_HMC, GMM_MU_MWG, GMM_MU_RSLICE, HMC_BLOCK_B,
-                    HMC_BLOCK_B_THETA, HMC_BLOCK_THETA, HMC_BLOCK_V, MWG_BLOCK_V)
+                    HMC_BLOCK_B_THETA, HMC_BLOCK_THETA, HMC_BLOCK_V, MWG_BLOCK_B, MWG_BLOCK_V)
 from . import augurdistlib as adl
 from .rvparse import Kern
 
@@ -173,6 +175,15 @@ class HlrPlan:
             _fail("not an HLR model")
         yd = data[0]
         bargs = _dist(yd.exp, "Bernoulli")
+        self.det_name = None
+        if bargs is not None:
+            # a deterministic parameter as the Bernoulli bias (dethlr.rv):
+            # param bias[n] : Real = sigmoid(...) for n, y[n] ~ Bernoulli(bias[n])
+            dn, _ = _proj(bargs[0])
+            dd = model.decl(dn) if dn else None
+            if dd is not None and dd.kind == "param" and not dd.stochastic:
+                self.det_name = dn
+                bargs = [dd.exp]
         if bargs is None or bargs[0][0] != "prim" or bargs[0][1] != "sigmoid":
             _fail("HLR: data must be y[n] ~ Bernoulli(sigmoid(...))")
         lin = bargs[0][2][0]
@@ -209,21 +220,27 @@ class HlrPlan:
             if bargs2 is None or not _is_lit(bargs2[0], 0.0) or _var(bargs2[1]) != v_name:
                 _fail("HLR: intercept b ~ Normal(0.0, v) expected")
         expected = {v_name, th_name} | ({b_name} if b_name else set())
-        if {d.name for d in model.params} != expected:
-            _fail("HLR: unexpected parameters (deterministic or extra parameters)")
-        self.names = {"v": v_name, "theta": th_name, "b": b_name, "x": x_name, "y": yd.name}
+        stoch = {d.name for d in model.params if d.stochastic}
+        dets = {d.name for d in model.params if not d.stochastic}
+        if stoch != expected or dets != ({self.det_name} if self.det_name else set()):
+            _fail("HLR: unexpected parameters (extra or deterministic parameters)")
+        self.names = {"v": v_name, "theta": th_name, "b": b_name, "x": x_name, "y": yd.name,
+                      "bias": self.det_name}
         self.param_names = [d.name for d in model.params]
         if sched is None:
-            sched = [Kern("HMC", [d.name]) for d in model.params]
+            sched = [Kern("HMC", [d.name]) for d in model.params if d.stochastic]
         self.blocks = []
         for k in sched:
-            if k.kind == "MWG" and k.vars == [v_name]:
-                # MWG [v] ~ Normal(v, s): random walk on v, s the proposal variance
+            if k.kind == "MWG" and (k.vars == [v_name] or (b_name and k.vars == [b_name])):
+                # MWG [v] ~ Normal(v, s) / MWG [b] ~ Normal(b, s): random walks,
+                # s the proposal variance (examples/hlr.py:57, test/hlr2.infer)
+                nm = k.vars[0]
                 prop = k.proposal[0] if k.proposal else None
                 pargs = _dist(prop, "Normal") if prop else None
-                if pargs is None or _var(pargs[0]) != v_name or pargs[1][0] != "lit":
-                    _fail(f"HLR: MWG [{v_name}] needs the proposal Normal({v_name}, <variance>)")
-                self.blocks.append((MWG_BLOCK_V, 0.0, float(pargs[1][1])))
+                if pargs is None or _var(pargs[0]) != nm or pargs[1][0] != "lit":
+                    _fail(f"HLR: MWG [{nm}] needs the proposal Normal({nm}, <variance>)")
+                self.blocks.append((MWG_BLOCK_V if nm == v_name else MWG_BLOCK_B, 0.0,
+                                    float(pargs[1][1])))
                 continue
             if k.kind != "HMC":
                 _fail(f"HLR: only HMC kernels and MWG [{v_name}] run on this runtime (got {k.kind})")
@@ -291,6 +308,8 @@ class HlrPlan:
         pt = {n["v"]: float(st[self.D + 1]), n["theta"]: st[:self.D].copy()}
         if n["b"]:
             pt[n["b"]] = float(st[self.D])
+        if n["bias"]:
+            pt[n["bias"]] = self.sampler.get("bias", chain)
         return pt
 
 

@@ -135,7 +135,9 @@ class HlrSampler(Sampler):
         L.call("augur_hlr_create", C.byref(desc), C.byref(h))
         self.n_blocks = len(blocks)
         shapes = {"state": ((dim + 2,), np.float64), "theta": ((dim,), np.float64),
-                  "v": ((1,), np.float64)}
+                  "v": ((1,), np.float64),
+                  # read-only: sigmoid(x[n].theta + b) per row, computed when read
+                  "bias": ((n,), np.float64)}
         if has_intercept:
             shapes["b"] = ((1,), np.float64)
         super().__init__(h, chains, shapes)

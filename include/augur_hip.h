@@ -15,7 +15,9 @@
  *     Calls on one handle must not run concurrently; different handles may.
  *   - Parameter names are the family's canonical names:
  *       LDA : "theta" (D x K f64), "phi" (K x V f64), "z" (T i32)
- *       HLR : "theta" (D f64), "b" (1 f64), "v" (1 f64), "state" (D + 2 f64)
+ *       HLR : "theta" (D f64), "b" (1 f64), "v" (1 f64), "state" (D + 2 f64),
+ *             "bias" (N f64, read-only: the deterministic sigmoid(x[n].theta + b)
+ *             of compiler/augur/test/dethlr.rv, computed when read)
  *       GMM : "pis" (K f64), "mu" (K x D f64), "cov" (K x D x D f64), "z" (N i32)
  *     `chain` selects one chain's copy.
  *   - Random numbers: Philox4x32-10 keyed by (seed, chain); see augur_rng.h.
@@ -81,6 +83,10 @@ augur_status augur_lda_create(const augur_lda_desc* desc, augur_sampler** out);
  * `MWG [v] ~ Normal(v, s)` (cbits/augur_mcmclib.h:178-193,
  * Core/CgUserProp.hs:130-141); sim_len and fixed_steps are ignored. */
 #define AUGUR_MWG_BLOCK_V 4
+/* The same on the intercept b, `MWG [b] ~ Normal(b, s)`
+ * (compiler/augur/test/hlr2.infer): one data pass per sweep at the proposal,
+ * accepted on b's full conditional. */
+#define AUGUR_MWG_BLOCK_B 5
 typedef struct {
   int block;        /* AUGUR_HMC_BLOCK_* or AUGUR_MWG_BLOCK_V */
   double sim_len;   /* reference simLen (Core/KernSyn.hs:64-75 default 1.0) */

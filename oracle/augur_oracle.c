@@ -732,6 +732,35 @@ int or_mwg_v_step(const or_hlr_model* m, double* state, int block_id, double var
   return acc;
 }
 
+/* MWG [b] ~ Normal(b, var) (compiler/augur/test/hlr2.infer): the same rule
+ * as or_mwg_v_step on the intercept's full conditional (prior Normal(0, v)
+ * plus the data term). */
+int or_mwg_b_step(const or_hlr_model* m, double* state, int block_id, double var, uint32_t seed,
+                  uint32_t chain, uint32_t sweep, double* out_info) {
+  const int sd = m->d + 2;
+  const uint32_t tag = OR_TAG_HMC_BASE + 4u * (uint32_t)block_id;
+  double* prop = (double*)malloc(sizeof(double) * sd);
+  memcpy(prop, state, sizeof(double) * sd);
+  const double b0 = state[m->d];
+  const double b1 = b0 + sqrt(var) * or_normal(seed, chain, 0, 0, sweep, tag + 0);
+  prop[m->d] = b1;
+  const double extra = m->ref_compat ? 0.5 * (b1 - b0) * (b1 - b0) / var : 0.0;
+  const double ll0 = or_hlr_block_ll(m, state, 1);
+  const double ll1 = or_hlr_block_ll(m, prop, 1);
+  uint32_t w[4];
+  or_draw4(seed, chain, 0, 0, sweep, tag + 2, w);
+  const int acc = log(or_u01(w[0])) < ll1 - ll0 + extra;
+  if (acc) state[m->d] = b1;
+  if (out_info) {
+    out_info[0] = b1;
+    out_info[1] = ll0;
+    out_info[2] = ll1;
+    out_info[3] = acc;
+  }
+  free(prop);
+  return acc;
+}
+
 /* ------------------------------------------------------------------------ */
 /* GMM: hmvgmm.rv, schedule ConjGibbs pis, mu, cov, DiscGibbs z.             */
 /* ------------------------------------------------------------------------ */

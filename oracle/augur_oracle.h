@@ -129,6 +129,9 @@ int or_hmc_step(const or_hlr_model* m, double* state, int block, int block_id,
 /* Metropolis-within-Gibbs on v, proposal Normal(v, var); out_info (4). */
 int or_mwg_v_step(const or_hlr_model* m, double* state, int block_id, double var, uint32_t seed,
                   uint32_t chain, uint32_t sweep, double* out_info);
+/* Metropolis-within-Gibbs on the intercept b, proposal Normal(b, var). */
+int or_mwg_b_step(const or_hlr_model* m, double* state, int block_id, double var, uint32_t seed,
+                  uint32_t chain, uint32_t sweep, double* out_info);
 
 /* ---- GMM (hmvgmm.rv / mvgmm.rv) ---- */
 typedef struct {

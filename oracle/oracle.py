@@ -95,6 +95,8 @@ def lib():
                                       C.c_double, C.c_int, U32, U32, U32, P_D]),
             "or_mwg_v_step": (C.c_int, [C.POINTER(HlrModel), P_D, C.c_int, C.c_double, U32, U32,
                                         U32, P_D]),
+            "or_mwg_b_step": (C.c_int, [C.POINTER(HlrModel), P_D, C.c_int, C.c_double, U32, U32,
+                                        U32, P_D]),
             "or_gmm_sweep": (None, [C.POINTER(GmmModel), P_D, P_D, P_D, P_I, U32, U32, U32]),
             "or_gmm_zstep": (None, [C.POINTER(GmmModel), P_D, P_D, P_D, P_I, U32, U32, U32]),
             "or_multi_lgamma": (C.c_double, [C.c_int, C.c_double]),
@@ -275,6 +277,11 @@ class Hlr:
         """Metropolis-within-Gibbs on v in place; returns info[4]."""
         info = np.zeros(4)
         lib().or_mwg_v_step(C.byref(self.m), state, block_id, var, seed, chain, sweep, info)
+        return info
+
+    def mwg_b_step(self, state, block_id, var, seed, chain, sweep):
+        info = np.zeros(4)
+        lib().or_mwg_b_step(C.byref(self.m), state, block_id, var, seed, chain, sweep, info)
         return info
 
 

@@ -109,6 +109,8 @@ def test_reference_schedules_match_a_plan():
              "hmvgmm3": "hmvgmm", "mvgmm1": "mvgmm", "mvgmm2": "mvgmm", "mvgmm3": "mvgmm",
              "mvgmm4": "mvgmm", "mvgmm5": "mvgmm", "mvgmm6": "mvgmm"}
     unsupported = {"mvgmm6"}  # NUTS [mu]
+    dm = rvparse.parse_model(open(os.path.join(REF_TEST, "dethlr.rv")).read())
+    assert plan.match(dm, None).names["bias"] == "bias"  # deterministic parameter
     got = {}
     for sched, model in pairs.items():
         m = rvparse.parse_model(open(os.path.join(REF_TEST, model + ".rv")).read())
@@ -118,7 +120,7 @@ def test_reference_schedules_match_a_plan():
             got[sched] = True
         except plan.UnsupportedModel:
             got[sched] = False
-    assert {s for s, ok in got.items() if not ok} <= unsupported | {"hlr2"}, got
+    assert {s for s, ok in got.items() if not ok} <= unsupported, got
     for s in ("hmvgmm2", "hmvgmm3", "mvgmm2", "mvgmm3", "mvgmm4", "mvgmm5"):
         assert got[s], s
 
@@ -139,9 +141,9 @@ def test_plan_matching():
     p = plan.match(rvparse.parse_model(HLR_CFG),
                    rvparse.parse_schedule("MWG [v] ~ Normal(v, 1.0) (*) HMC [beta] [0.1, 0.005]"))
     assert p.blocks == [(4, 0.0, 1.0), (2, 0.1, 0.005)]
-    with pytest.raises(plan.UnsupportedModel):  # MWG on the intercept needs a data pass
-        plan.match(rvparse.parse_model(HLR), rvparse.parse_schedule(
-            "MWG [v] ~ Normal(v, 1.0) (*) MWG [b] ~ Normal(b, 1.0) (*) HMC [theta]"))
+    p = plan.match(rvparse.parse_model(HLR), rvparse.parse_schedule(  # test/hlr2.infer
+        "MWG [v] ~ Normal(v, 1.0) (*) MWG [b] ~ Normal(b, 1.0) (*) HMC [theta]"))
+    assert p.blocks == [(4, 0.0, 1.0), (5, 0.0, 1.0), (2, 1.0, 0.05)]
     with pytest.raises(plan.UnsupportedModel):  # only the random walk centred at v
         plan.match(rvparse.parse_model(HLR_CFG), rvparse.parse_schedule(
             "MWG [v] ~ Normal(1.0, 1.0) (*) HMC [beta]"))

@@ -90,6 +90,47 @@ def test_hlr_example_schedule():
         assert abs(np.mean(s["b"]) + 0.5) < 0.25
 
 
+DETHLR = """(K : Int, N : Int, lam : Real, x : Vec (Vec Real)) => {
+  param v ~ Exponential(lam) ;
+  param b ~ Normal(0.0, v) ;
+  param theta[k] ~ Normal(0.0, v)
+      for k <- 0 until K ;
+  param bias[n] : Real = sigmoid(dotprod(theta, x[n]) + b)
+      for n <- 0 until N ;
+  data y[n] ~ Bernoulli(bias[n])
+      for n <- 0 until N ;
+}
+"""
+
+
+@pytest.mark.parametrize("sched", [None, "MWG [v] ~ Normal(v, 1.0) (*) MWG [b] ~ Normal(b, 0.01) (*) HMC [theta]"])
+def test_dethlr_deterministic_bias(sched):
+    """compiler/augur/test/dethlr.rv (the Bernoulli bias a deterministic
+    parameter) with the automatic schedule and with hlr2.infer's MWG kernels:
+    the samples hold bias[n] = sigmoid(x[n] . theta + b) at every kept sweep,
+    and b is recovered."""
+    rng = np.random.default_rng(5)
+    K, N = 6, 3000
+    x = rng.standard_normal((N, K))
+    theta, b = rng.standard_normal(K) * 0.5, -0.5
+    y = (rng.random(N) < 1 / (1 + np.exp(-(x @ theta + b)))).astype(np.int32)
+    xs = np.empty(N, dtype=object)
+    for i in range(N):
+        xs[i] = x[i]
+    with AugurInfer("config.yml", DETHLR) as infer:
+        infer.set_compile_opt(AugurOpt(target="gpu", seed=3, chains=1, ref_compat=False,
+                                       initStrat=AugurUser({"v": 1.0, "b": 0.0, "theta": np.zeros(K)})))
+        if sched:
+            infer.set_user_sched(sched)
+        infer.compile(K, N, 1.0, xs)(y)
+        s = infer.samplen(burnIn=300, numSamples=50)
+        assert set(s) == {"v", "b", "theta", "bias"}
+        for i in (0, 49):
+            ref = 1 / (1 + np.exp(-(x.astype(np.float32).astype(np.float64) @ s["theta"][i] + s["b"][i])))
+            np.testing.assert_allclose(s["bias"][i], ref, rtol=1e-12)
+        assert abs(np.mean(s["b"]) - b) < 0.25
+
+
 @pytest.mark.parametrize("sched", [
     "ConjGibbs [mu] (*) DiscGibbs [z]",                                  # examples/mvgmm.py sched1
     "ESlice [mu] (*) DiscGibbs [z]",                                     # sched2

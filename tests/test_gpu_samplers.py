@@ -238,6 +238,45 @@ def test_hlr_mwg_v_matches_oracle(rt, oracle, ref_compat):
     s.close()
 
 
+@pytest.mark.parametrize("ref_compat", [False, True])
+def test_hlr_mwg_b_matches_oracle(rt, oracle, ref_compat):
+    """compiler/augur/test/hlr2.infer: MWG [v] ~ Normal(v, 1.0) (*) MWG [b] ~
+    Normal(b, s) (*) HMC [theta] (with a proposal variance the data accept
+    sometimes), every sweep's state against the oracle."""
+    from augurv2_amd._lib import HMC_BLOCK_THETA, MWG_BLOCK_B, MWG_BLOCK_V
+    rng = np.random.default_rng(22)
+    n, d, C = 2000, 8, 4
+    x, y = _hlr_data(rng, n, d)
+    y = ((x @ (rng.standard_normal(d) * 0.3) + 0.7 + rng.logistic(size=n)) > 0).astype(np.int32)
+    blocks = [(MWG_BLOCK_V, 0.0, 1.0, 0), (MWG_BLOCK_B, 0.0, 0.01, 0), (HMC_BLOCK_THETA, 0.1, 0.01, 0)]
+    s = rt.HlrSampler(x, y, blocks, chains=C, seed=98, has_intercept=True, ref_compat=ref_compat)
+    m = oracle.Hlr(x.astype(np.float64), y, has_intercept=True, ref_compat=ref_compat)
+    states = []
+    for c in range(C):
+        st = np.concatenate([rng.standard_normal(d) * 0.3, [0.2 * c], [1.0 + 0.3 * c]])
+        s.set("state", st, chain=c)
+        states.append(st)
+    moved = 0
+    for sweep in range(8):
+        s.step(1)
+        for c in range(C):
+            m.mwg_v_step(states[c], 0, 1.0, 98, c, sweep)
+            b_before = states[c][d]
+            m.mwg_b_step(states[c], 1, 0.01, 98, c, sweep)
+            moved += states[c][d] != b_before
+            m.hmc_step(states[c], 2, 2, 0.1, 0.01, 0, 98, c, sweep)
+            np.testing.assert_allclose(s.get("state", c), states[c], rtol=2e-5, atol=1e-7)
+    assert moved > 0
+    acc = s.acceptance()
+    assert np.all(acc[1] > 0.0)
+    # the deterministic sigmoid(x . theta + b) of dethlr.rv, read per chain
+    for c in range(C):
+        st = s.get("state", c)
+        ref = 1.0 / (1.0 + np.exp(-(x.astype(np.float64) @ st[:d] + st[d])))
+        np.testing.assert_allclose(s.get("bias", c), ref, rtol=1e-12)
+    s.close()
+
+
 def test_hlr_curr_ll(rt, oracle):
     from augurv2_amd._lib import HMC_BLOCK_V, HMC_BLOCK_THETA
     rng = np.random.default_rng(4)
