@@ -390,14 +390,21 @@ def run_surrogate(iters=5000, burn=1000, train=1000, seed=77, ablation=True):
     return res
 
 
-def run_batched(sweeps=1000, warmup=500, chains=4, seed=31, plain=True):
+def run_batched(sweeps=2000, warmup=1000, chains=4, seed=31, plain=True, plain_sample=6):
     """256 HLR datasets (N_i ~ U[1k, 200k], D_i in {16, 32, 64}) in one
     batched sampler (BF16 X, autotuned rows per block, hipGraph sweep), 20
-    leapfrog steps of 0.005 with the step jittered, MWG [v] ~ Normal(v, 1.0).
+    leapfrog steps of 0.005 with the step jittered, MWG [v] ~ Normal(v, 1.0);
+    step sizes and v proposal variances adapted per dataset in warmup.
     Up to 4 chains per dataset the data pass runs on the vector units, from
-    5 on the matrix cores.  Beside it the plain sampler's time for the same
-    sweeps and chains on each dataset alone, measured on eight datasets and
-    interpolated linearly in N_i D_i."""
+    5 on the matrix cores.  Chains start at v = 1, 1.25, 1.5, ... (a draw of
+    v from its InvGamma(1, 1) prior can be 10^3, from where a chain random-
+    walks down for longer than the warmup).
+
+    Beside it the plain sampler (FP32 X, one dataset at a time) on a
+    stratified sample of datasets, with each dataset's adapted step size and
+    proposal variance: its min-ESS/s per dataset is measured, and the time
+    it would take to reach the batched run's min-ESS on every dataset is
+    extrapolated from the sample in proportion to N_i D_i."""
     from augurv2_amd import runtime, workloads
     from augurv2_amd._lib import HMC_BLOCK_THETA, MWG_BLOCK_V
     L, EPS = 20, 0.005
@@ -406,9 +413,10 @@ def run_batched(sweeps=1000, warmup=500, chains=4, seed=31, plain=True):
                             theta_block=(L * EPS, EPS, -L), rows_per_block=0, use_graph=True)
     for i, x in enumerate(xs):
         for c, st in enumerate(workloads.hlr_init(x.shape[1], chains, seed=100 + i)):
+            st[-1] = 1.0 + 0.25 * c
             s.set(f"state:{i}", st, chain=c)
     rows, cand = s.tuning()
-    log(f"batched: rows/block {rows}, candidates {cand}; adapting warmup {warmup}")
+    log(f"batched ({chains} chains): rows/block {rows}, candidates {cand}; adapting warmup {warmup}")
     # per-dataset step sizes (dual averaging) and v proposal variances adapt
     # during warmup, then stay fixed: one step size cannot suit datasets of
     # 1k and 200k rows with variances from 0.1 to 100
@@ -431,37 +439,58 @@ def run_batched(sweeps=1000, warmup=500, chains=4, seed=31, plain=True):
         rhat_i.append(r)
     ess_i, rhat_i = np.array(ess_i), np.array(rhat_i)
     conv = rhat_i < RHAT_MAX
-    out = {"datasets": len(xs), "chains": chains}
+    out = {"datasets": len(xs), "chains": chains, "rows_total": int(sum(x.shape[0] for x in xs)),
+           "data_pass": ("matrix cores: BF16 V_MFMA, FP32 accumulation" if chains >= 5
+                         else "vector units: register form, BF16 rows, FP32 arithmetic"),
+           "rows_per_block": rows, "autotune_us": cand, "sweeps": sweeps, "warmup": warmup,
+           "elapsed_s": el, "warmup_s": t_warm,
+           "step_size_range": [float(eps_i.min()), float(eps_i.max())],
+           "v_proposal_var_range": [float(vvar_i.min()), float(vvar_i.max())],
+           "n_rhat_below_1_01": int(conv.sum()),
+           "not_converged": [{"dataset": int(i), "rows": int(xs[i].shape[0]), "d": int(xs[i].shape[1]),
+                              "rhat": float(rhat_i[i])} for i in np.nonzero(~conv)[0]],
+           "ms_per_sweep": 1e3 * el / sweeps,
+           "sum_converged_dataset_min_ess_per_s": float(ess_i[conv].sum() / el),
+           "min_converged_dataset_ess": float(ess_i[conv].min()) if conv.any() else None,
+           "median_dataset_ess": float(np.median(ess_i)),
+           "rhat_max": float(np.max(rhat_i)), "accept_theta": float(np.nanmean(acc[1]))}
     if not plain:
         return out
-    # plain sampler, one dataset at a time
+    # the plain sampler, one dataset at a time, on datasets spread over the
+    # range of N_i D_i (converged ones), with the batched run's adapted settings
     work = np.array([x.shape[0] * x.shape[1] for x in xs], np.float64)
-    pick = np.argsort(work)[np.linspace(0, len(xs) - 1, 8).astype(int)]
-    t_plain = []
+    order = [i for i in np.argsort(work) if conv[i]]
+    pick = [order[j] for j in np.linspace(0, len(order) - 1, plain_sample).astype(int)]
+    p_sweeps = max(200, sweeps // 4)
+    rows_p = []
     for i in pick:
-        p = runtime.HlrSampler(xs[i], ys[i], [(MWG_BLOCK_V, 0.0, 1.0, 0),
-                                              (HMC_BLOCK_THETA, L * EPS, EPS, -L)],
-                               chains=chains, seed=seed, ref_compat=False)
-        p.step(20)
-        t1 = time.perf_counter()
+        p = runtime.HlrSampler(xs[i], ys[i], [(MWG_BLOCK_V, 0.0, float(vvar_i[i]), 0),
+                                              (HMC_BLOCK_THETA, L * float(eps_i[i]), float(eps_i[i]), -L)],
+                               chains=chains, seed=seed + 1, ref_compat=False)
+        for c in range(chains):
+            p.set("state", draws[-1, c, int(s.offsets[i]):int(s.offsets[i]) + xs[i].shape[1] + 2], chain=c)
         p.step(100)
-        t_plain.append((time.perf_counter() - t1) / 100)
+        p.trace("state", p_sweeps)
+        t1 = time.perf_counter()
+        p.step(p_sweeps)
+        t_p = time.perf_counter() - t1
+        d = xs[i].shape[1]
+        e_p, r_p, _ = _summary(p.read_trace("state"), list(range(d)) + [d + 1])
         p.close()
-    a, b = np.polyfit(work[pick], np.array(t_plain), 1)[::-1]
-    plain_total = float(np.sum(a + b * work)) * sweeps
-    return {"datasets": len(xs), "chains": chains, "rows_total": int(sum(x.shape[0] for x in xs)),
-            "data_pass": ("matrix cores: BF16 V_MFMA, FP32 accumulation" if chains >= 5
-                          else "vector units: register form, BF16 rows, FP32 arithmetic"),
-            "rows_per_block": rows, "autotune_us": cand, "sweeps": sweeps, "elapsed_s": el,
-            "warmup_s": t_warm, "step_size_range": [float(eps_i.min()), float(eps_i.max())],
-            "v_proposal_var_range": [float(vvar_i.min()), float(vvar_i.max())],
-            "n_rhat_below_1_01": int(conv.sum()),
-            "ms_per_sweep": 1e3 * el / sweeps,
-            "sum_converged_dataset_min_ess_per_s": float(ess_i[conv].sum() / el),
-            "min_dataset_ess": float(ess_i.min()), "median_dataset_ess": float(np.median(ess_i)),
-            "rhat_max": float(np.max(rhat_i)), "accept_theta": float(np.nanmean(acc[1])),
-            "plain_sequential_s_est": plain_total,
-            "sweep_time_ratio_vs_plain_sequential": plain_total / el}
+        rows_p.append({"dataset": int(i), "rows": int(xs[i].shape[0]), "d": int(d),
+                       "plain_min_ess_per_s": e_p / t_p, "plain_rhat": r_p,
+                       "batched_min_ess": float(ess_i[i]),
+                       "plain_s_to_batched_ess": float(ess_i[i]) / (e_p / t_p)})
+    samp_t = sum(r["plain_s_to_batched_ess"] for r in rows_p)
+    seq_est = samp_t * float(work[conv].sum() / work[pick].sum())
+    out.update({"plain_sample": rows_p, "plain_sweeps": p_sweeps,
+                "plain_sequential_s_to_same_ess_est": seq_est,
+                "min_ess_per_s_ratio_vs_plain_sequential": seq_est / el,
+                "plain_comparison": (f"plain sampler on {len(pick)} converged datasets spread over "
+                                     "N_i D_i; time to reach the batched run's min-ESS on each, "
+                                     "extrapolated to all converged datasets in proportion to "
+                                     "N_i D_i")})
+    return out
 
 
 def main():
@@ -473,7 +502,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--only", default="", help="comma list of legs to run besides the headline "
                     "HLR leg: lda, surrogate, batched, gmm (default: all)")
-    ap.add_argument("--batched-sweeps", type=int, default=1000,
+    ap.add_argument("--batched-sweeps", type=int, default=2000,
                     help="timed sweeps of the batched legs (warmup: half as many)")
     ap.add_argument("--surrogate-iters", type=int, default=5000,
                     help="timed iterations of each surrogate-leg sampler (shorter for profiling)")
