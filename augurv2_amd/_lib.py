@@ -70,7 +70,7 @@ class GmmDesc(C.Structure):
                 ("mu_prop_cov", C.c_void_p), ("ref_compat", C.c_int)]
 
 
-GMM_MU_CONJ, GMM_MU_ESLICE, GMM_MU_HMC, GMM_MU_MWG, GMM_MU_RSLICE = 0, 1, 2, 3, 4
+GMM_MU_CONJ, GMM_MU_ESLICE, GMM_MU_HMC, GMM_MU_MWG, GMM_MU_RSLICE, GMM_MU_NUTS = 0, 1, 2, 3, 4, 5
 
 
 _H = C.c_void_p

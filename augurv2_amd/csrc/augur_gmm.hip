@@ -71,9 +71,15 @@ struct GmmDev {
   double* mlin;   // C x k x d      cov_k^-1 s, s = sum over the component of (y - centre)
   double* mcnt;   // C x k          points in the component
   double* macc;   // C x 2          accepted / proposed moves of the mu block
+  double* nuts;   // C x 3 kNutsSlots x 1024: the NUTS stacks, one value per thread
 };
 
-enum { GMM_MU_CONJ = 0, GMM_MU_ESLICE = 1, GMM_MU_HMC = 2, GMM_MU_MWG = 3, GMM_MU_RSLICE = 4 };
+enum { GMM_MU_CONJ = 0, GMM_MU_ESLICE = 1, GMM_MU_HMC = 2, GMM_MU_MWG = 3, GMM_MU_RSLICE = 4, GMM_MU_NUTS = 5 };
+
+// NUTS [mu]: the tree depth is capped at kNutsMaxDepth doublings; its stacks
+// (proposals, theta, momenta) hold kNutsMaxDepth + 2 vectors each.
+constexpr int kNutsMaxDepth = 10;
+constexpr int kNutsSlots = kNutsMaxDepth + 2;
 
 // ---- small FP64 linear algebra (cbits/augur_matop.c:312-406), row-major ----
 // Wave-parallel: one 64-lane block holds the matrices in LDS and every
@@ -437,6 +443,138 @@ __device__ void mu_block_eval(const GmmDev& m, int c, const double* th, double* 
   *lt = dl * lin - 0.5 * n * dl * pd - 0.5 * dm * q;
 }
 
+// NUTS over the block of all means (cbits/augur_mcmclib.h:548-776), the
+// recursion of h_augur_nuts_build_tree unrolled into a stack of frames that
+// every thread keeps alike; thread t owns coordinate t of every vector, its
+// stack entries in global scratch.  Draws: momentum at TAG_GMM_MU_MOVE site
+// t; the slice's Exp(1) from uniform (0, 1) and the i-th Bernoulli draw
+// from uniform (i, 3) at TAG_GMM_MU_MOVE + 2, in the reference's order.
+// Returns whether the last doubling's proposal was kept.
+__device__ bool mu_nuts(const GmmDev& m, int c, double* th, double* red, uint32_t seed, uint32_t sweep, double* x) {
+  const int t = threadIdx.x, dim = m.k * m.d;
+  const bool own = t < dim;
+  const Stream sn{seed, (uint32_t)c, sweep, TAG_GMM_MU_MOVE};
+  const Stream su{seed, (uint32_t)c, sweep, TAG_GMM_MU_MOVE + 2};
+  double* base = m.nuts + (size_t)c * 3 * kNutsSlots * kMuThreads + t;
+  double* pstk = base;                                      // proposals
+  double* tstk = base + (size_t)kNutsSlots * kMuThreads;    // theta
+  double* mstk = base + (size_t)2 * kNutsSlots * kMuThreads; // momenta
+  auto at = [](double* s, int i) -> double& { return s[(size_t)i * kMuThreads]; };
+  uint32_t nb = 0;
+  auto bern = [&](double bias) { return u01(su.draw(nb++, 3).x) < bias; };
+  double g, lt;
+  // grad and log-density share at xv (all threads, block barriers inside)
+  auto eval = [&](double xv) {
+    __syncthreads();
+    if (own) th[t] = xv;
+    __syncthreads();
+    mu_block_eval(m, c, th, &g, &lt);
+  };
+  const double x0 = *x;
+  double tm = x0, tp = x0, mm = own ? normal_at(sn, (uint32_t)t, 0) : 0.0, mp = mm;
+  eval(x0);
+  const double logu = mu_block_sum(lt - 0.5 * mm * mm, red) + log(u01(su.draw(0, 1).x));
+  int n = 1, j = 0, ptop = 0, ttop = 0;
+  bool f_cpy = false;
+  double xnew = x0;
+  int fj[kNutsMaxDepth + 1], fst[kNutsMaxDepth + 1], fln[kNutsMaxDepth + 1], fls[kNutsMaxDepth + 1];
+  while (true) {
+    ttop = 0;
+    const bool minus = bern(0.5);
+    const double step = minus ? -m.mu_step : m.mu_step;
+    double xw = minus ? tm : tp, mw = minus ? mm : mp;
+    int top = 1, rn = 0, rs = 0;
+    fj[0] = j;
+    fst[0] = 0;
+    while (top > 0) {
+      const int f = top - 1;
+      if (fj[f] == 0) {  // leaf: cbits/augur_mcmclib.h:305-325 and :603-615
+        eval(xw);
+        mw += step / 2.0 * g;
+        xw += step * mw;
+        eval(xw);
+        mw += step / 2.0 * g;
+        if (own) at(pstk, ptop) = xw;
+        ++ptop;
+        const double ll = mu_block_sum(lt - 0.5 * mw * mw, red);
+        rn = logu <= ll;
+        rs = logu < 1000.0 + ll;
+        --top;
+        continue;
+      }
+      if (fst[f] == 0) {  // build the left subtree
+        fst[f] = 1;
+        fj[top] = fj[f] - 1;
+        fst[top] = 0;
+        ++top;
+        continue;
+      }
+      if (fst[f] == 1) {  // the left subtree returned (rn, rs)
+        fln[f] = rn;
+        fls[f] = rs;
+        if (rs != 1) {
+          --top;
+          continue;
+        }
+        if (fj[f] == 1) {
+          if (own) {
+            at(tstk, ttop) = xw;
+            at(mstk, ttop) = mw;
+          }
+          ++ttop;
+        }
+        fst[f] = 2;
+        fj[top] = fj[f] - 1;
+        fst[top] = 0;
+        ++top;
+        continue;
+      }
+      // the right subtree returned (rn, rs)
+      const double bias = (double)rn / (double)(fln[f] + rn);
+      if (bern(bias) && own) at(pstk, ptop - 2) = at(pstk, ptop - 1);
+      --ptop;
+      fln[f] += rn;
+      int snew = 0;
+      if (rs == 1) {
+        const double pt = own ? at(tstk, ttop - 1) : 0.0, pm = own ? at(mstk, ttop - 1) : 0.0;
+        const double tpl = minus ? pt : xw, tmi = minus ? xw : pt;
+        const double mpl = minus ? pm : mw, mmi = minus ? mw : pm;
+        const double a = mu_block_sum((tpl - tmi) * mpl, red), b = mu_block_sum((tpl - tmi) * mmi, red);
+        snew = (a >= 0.0 && b >= 0.0) ? 1 : 0;
+      }
+      fls[f] = snew;
+      if (fj[f] > 1) --ttop;
+      rn = fln[f];
+      rs = fls[f];
+      --top;
+    }
+    if (minus) {
+      tm = xw;
+      mm = mw;
+    } else {
+      tp = xw;
+      mp = mw;
+    }
+    --ptop;
+    const double cand = own ? at(pstk, ptop) : 0.0;
+    f_cpy = false;
+    if (rs == 1 && bern(fmin(1.0, (double)rn / (double)n))) {
+      xnew = cand;
+      f_cpy = true;
+    }
+    n += rn;
+    int s = 0;
+    if (rs == 1) {
+      const double a = mu_block_sum((tp - tm) * mp, red), b = mu_block_sum((tp - tm) * mm, red);
+      s = (a >= 0.0 && b >= 0.0) ? 1 : 0;
+    }
+    ++j;
+    if (s != 1 || j >= kNutsMaxDepth) break;
+  }
+  if (f_cpy) *x = xnew;
+  return f_cpy;
+}
+
 // One block of kMuThreads per chain: the schedule's move of the mu block.
 //   ESlice [mu]  per component (cbits/augur_mcmclib.h:439-485, Core/CgESlice.hs):
 //                nu ~ MvNormal(mu0, cov0), log y = like(mu_k) + log u, angle
@@ -524,7 +662,9 @@ __global__ __launch_bounds__(kMuThreads) void k_gmm_mu_move(GmmDev m, uint32_t s
   mu_block_eval(m, c, th, &g, &lt);
   const double ll0 = mu_block_sum(lt, red);
   bool accept = false;
-  if (m.mu_mode == GMM_MU_HMC) {
+  if (m.mu_mode == GMM_MU_NUTS) {
+    accept = mu_nuts(m, c, th, red, seed, sweep, &x);
+  } else if (m.mu_mode == GMM_MU_HMC) {
     double step;
     int steps;
     hmc_jitter(Stream{seed, (uint32_t)c, sweep, TAG_GMM_MU_MOVE + 1}, m.mu_sim_len, m.mu_step, m.mu_fixed_steps,
@@ -1154,7 +1294,7 @@ struct GmmSampler final : augur_sampler {
   GmmDev m{};
   DevBuf<float> y, tab, part;
   DevBuf<double> alpha, mu0, cov0, scale0, gpi, pis64, mu64, cov64, ll, ssum;
-  DevBuf<double> cov0_inv, chol0, cholp, mcent, mprec, mlin, mcnt, macc;
+  DevBuf<double> cov0_inv, chol0, cholp, mcent, mprec, mlin, mcnt, macc, nuts;
   DevBuf<int32_t> z;
   std::vector<double> h_alpha, h_mu0, h_cov0, h_scale0;
   bool stats_valid = false;
@@ -1274,8 +1414,9 @@ static void gmm_validate(const augur_gmm_desc* D) {
     fail(AUGUR_ERR_ARG, "need N >= 1, 1 <= D <= %d, 1 <= K <= %d, chains >= 1", kMaxDim, kMaxComp);
   if (!D->y || !D->alpha || !D->mu0 || !D->cov0 || !D->scale0) fail(AUGUR_ERR_ARG, "null array");
   if (D->sample_cov && D->df <= D->dim - 1) fail(AUGUR_ERR_ARG, "InvWishart needs df > D - 1");
-  if (D->mu_kernel < AUGUR_GMM_MU_CONJ || D->mu_kernel > AUGUR_GMM_MU_RSLICE)
+  if (D->mu_kernel < AUGUR_GMM_MU_CONJ || D->mu_kernel > AUGUR_GMM_MU_NUTS)
     fail(AUGUR_ERR_ARG, "unknown mu kernel %d", D->mu_kernel);
+  if (D->mu_kernel == AUGUR_GMM_MU_NUTS && !(D->mu_step_size > 0.0)) fail(AUGUR_ERR_ARG, "NUTS [mu] needs step_size > 0");
   if (D->mu_kernel == AUGUR_GMM_MU_HMC || D->mu_kernel == AUGUR_GMM_MU_RSLICE) {
     if (!(D->mu_step_size > 0.0) || (D->mu_fixed_steps == 0 && !(D->mu_sim_len > 0.0)) ||
         D->mu_sim_len / D->mu_step_size > 1.0e5 || D->mu_fixed_steps > 100000 || D->mu_fixed_steps < -100000)
@@ -1388,6 +1529,10 @@ augur_status augur_gmm_create(const augur_gmm_desc* D, augur_sampler** out) {
         s->mcent.alloc((size_t)C * k * d); s->mprec.alloc((size_t)C * k * d * d);
         s->mlin.alloc((size_t)C * k * d); s->mcnt.alloc((size_t)C * k); s->macc.alloc(2 * (size_t)C);
         s->macc.zero(s->stream);
+        if (m.mu_mode == GMM_MU_NUTS) {
+          s->nuts.alloc((size_t)C * 3 * kNutsSlots * kMuThreads);
+          m.nuts = s->nuts.p;
+        }
         m.cov0_inv = s->cov0_inv.p; m.chol0 = s->chol0.p; m.cholp = s->cholp.p;
         m.mcent = s->mcent.p; m.mprec = s->mprec.p; m.mlin = s->mlin.p; m.mcnt = s->mcnt.p; m.macc = s->macc.p;
         AUGUR_HIP(hipStreamSynchronize(s->stream));  // the host vectors are released on return

@@ -9,7 +9,7 @@ Families (each recognised up to variable names):
           deterministic parameter as in dethlr.rv) with HMC blocks and MWG on
           v or b
   GMM  -- compiler/augur/test/hmvgmm.rv (and mvgmm.rv: fixed pis and covs)
-          with ConjGibbs pis, ConjGibbs | ESlice | HMC | MWG | RSlice mu,
+          with ConjGibbs pis, ConjGibbs | ESlice | HMC | MWG | RSlice | NUTS mu,
           ConjGibbs cov, DiscGibbs z
 The reference's compiler would instead generate code for any model
 (Compile.hs:311-345); other models raise UnsupportedModel.
@@ -17,7 +17,7 @@ The reference's compiler would instead generate code for any model
 import numpy as np
 
 from .. import runtime
-from .._lib import (GMM_MU_CONJ, GMM_MU_ESLICE, GMM_MU_HMC, GMM_MU_MWG, GMM_MU_RSLICE, HMC_BLOCK_B,
+from .._lib import (GMM_MU_CONJ, GMM_MU_ESLICE, GMM_MU_HMC, GMM_MU_MWG, GMM_MU_NUTS, GMM_MU_RSLICE, HMC_BLOCK_B,
                     HMC_BLOCK_B_THETA, HMC_BLOCK_THETA, HMC_BLOCK_V, MWG_BLOCK_B, MWG_BLOCK_V)
 from . import augurdistlib as adl
 from .rvparse import Kern
@@ -369,17 +369,21 @@ class GmmPlan:
             sched = [Kern(k, [v]) for k, v in want]
         # the kernel for mu may be ConjGibbs, ESlice, HMC, MWG or RSlice
         # (compiler/augur/test/hmvgmm1-3.infer, mvgmm1-5.infer)
-        mu_kerns = {"ConjGibbs", "ESlice", "HMC", "MWG", "RSlice"}
+        mu_kerns = {"ConjGibbs", "ESlice", "HMC", "MWG", "RSlice", "NUTS"}
         got = [(k.kind, k.vars[0]) if len(k.vars) == 1 else (k.kind, tuple(k.vars)) for k in sched]
         norm = [("ConjGibbs", v) if (v == mu_name and kd in mu_kerns) else (kd, v) for kd, v in got]
         if norm != want:
             _fail("GMM: supported schedule is " + " (*) ".join(
-                f"{'ConjGibbs|ESlice|HMC|MWG|RSlice' if v == mu_name else k} [{v}]" for k, v in want))
+                f"{'ConjGibbs|ESlice|HMC|MWG|RSlice|NUTS' if v == mu_name else k} [{v}]" for k, v in want))
         mk = next(k for k in sched if k.vars == [mu_name])
         self.mu_kernel = {"ConjGibbs": GMM_MU_CONJ, "ESlice": GMM_MU_ESLICE, "HMC": GMM_MU_HMC,
-                          "MWG": GMM_MU_MWG, "RSlice": GMM_MU_RSLICE}[mk.kind]
-        # Core/KernSyn.hs:64-75, :105-117: HMC and RSlice default to (1.0, 0.05)
-        self.mu_settings = tuple(mk.settings) if mk.settings else (1.0, 0.05)
+                          "MWG": GMM_MU_MWG, "RSlice": GMM_MU_RSLICE, "NUTS": GMM_MU_NUTS}[mk.kind]
+        # Core/KernSyn.hs:64-75, :94-117: HMC and RSlice default to (1.0, 0.05),
+        # NUTS to the step size 0.05
+        if mk.kind == "NUTS":
+            self.mu_settings = (0.0, float(mk.settings[0]) if mk.settings else 0.05)
+        else:
+            self.mu_settings = tuple(mk.settings) if mk.settings else (1.0, 0.05)
         self.mu_prop_cov_name = None
         if mk.kind == "MWG":
             # MWG [mu] ~ MvNormal(mu[k], P) for k <- 0 until K

@@ -218,6 +218,8 @@ augur_status augur_bhlr_step_sizes(augur_sampler* s, double* eps, double* v_var,
                                  (cbits/augur_mcmclib.h:178-193, Core/CgUserProp.hs) */
 #define AUGUR_GMM_MU_RSLICE 4 /* RSlice [mu] [sim_len, step_size], one block
                                  (cbits/augur_mcmclib.h:500-543) */
+#define AUGUR_GMM_MU_NUTS 5   /* NUTS [mu] [step_size], one block (cbits/augur_mcmclib.h:548-776),
+                                 at most 10 doublings */
 typedef struct {
   int n_points;      /* N */
   int dim;           /* D, 1..16 */
@@ -233,7 +235,7 @@ typedef struct {
   int sample_pi;
   int sample_cov;
   int mu_kernel;             /* AUGUR_GMM_MU_* */
-  double mu_sim_len;         /* HMC / RSlice (Core/KernSyn.hs defaults 1.0, 0.05) */
+  double mu_sim_len;         /* HMC / RSlice (Core/KernSyn.hs defaults 1.0, 0.05; NUTS 0.05) */
   double mu_step_size;
   int mu_fixed_steps;        /* HMC: as augur_hmc_block.fixed_steps */
   const double* mu_prop_cov; /* MWG: D x D proposal covariance */

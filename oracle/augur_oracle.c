@@ -988,9 +988,152 @@ static int or_gmm_mu_rslice(const or_gmm_model* m, double* mu, const double* cov
   return !cond;
 }
 
+/* cbits/augur_mcmclib.h:548-776 (h_augur_mcmc_nuts, h_augur_nuts_build_tree,
+ * h_augur_nuts_update_s) over the block of all means, step size
+ * mu_step_size, with its stacks and its Bernoulli draws in the order the
+ * reference makes them.  Two properties of the reference's algorithm are
+ * kept as they are: inside a subtree of depth >= 2 the u-turn test compares
+ * the current end with the first point of the subtree's right half (the top
+ * of its theta stack), and a proposal accepted in one doubling is replaced
+ * by theta_minus / theta_plus at the next, so only the last doubling's
+ * proposal can be kept.  The tree depth is capped at OR_NUTS_MAX_DEPTH
+ * doublings (the reference has no cap). */
+#define OR_NUTS_MAX_DEPTH 10
+typedef struct {
+  const or_gmm_model* m;
+  const double* cov;
+  const int* z;
+  int dim;
+  double step, logu;
+  double *prop, *th, *mom, *g, *work;  /* stacks of (depth + 2) vectors; scratch */
+  int ptop, ttop;
+  uint32_t seed, chain, sweep, nbern;
+} or_nuts_ctx;
+
+static int or_nuts_bern(or_nuts_ctx* c, double bias) {
+  uint32_t w[4];
+  or_draw4(c->seed, c->chain, c->nbern++, 3, c->sweep, OR_TAG_GMM_MU_MOVE + 2, w);
+  return or_u01(w[0]) < bias;
+}
+
+static double or_dot(const double* a, const double* b, int n) {
+  double s = 0.0;
+  for (int i = 0; i < n; i++) s += a[i] * b[i];
+  return s;
+}
+
+static int or_nuts_update_s(or_nuts_ctx* c, int s, const double* tp, const double* mp, const double* tm,
+                            const double* mm) {
+  if (s == 1) {
+    for (int i = 0; i < c->dim; i++) c->work[i] = tp[i] - tm[i];
+    if (or_dot(c->work, mp, c->dim) >= 0.0 && or_dot(c->work, mm, c->dim) >= 0.0) return 1;
+  }
+  return 0;
+}
+
+static void or_nuts_build(or_nuts_ctx* c, double* theta, double* mom, int j, int minus, int* rn, int* rs) {
+  const int dim = c->dim;
+  if (j == 0) {
+    /* cbits/augur_mcmclib.h:305-325 (leapfrog1) */
+    or_gmm_mu_grad(c->m, c->z, theta, c->cov, c->g);
+    for (int i = 0; i < dim; i++) mom[i] += c->step / 2.0 * c->g[i];
+    for (int i = 0; i < dim; i++) theta[i] += c->step * mom[i];
+    or_gmm_mu_grad(c->m, c->z, theta, c->cov, c->g);
+    for (int i = 0; i < dim; i++) mom[i] += c->step / 2.0 * c->g[i];
+    memcpy(c->prop + (size_t)(c->ptop++) * dim, theta, sizeof(double) * dim);
+    const double ll = or_gmm_mu_fc(c->m, c->z, theta, c->cov) - 0.5 * or_dot(mom, mom, dim);
+    *rn = c->logu <= ll;
+    *rs = c->logu < 1000.0 + ll; /* MCMC.deltaMax */
+    return;
+  }
+  int ln, ls;
+  or_nuts_build(c, theta, mom, j - 1, minus, &ln, &ls);
+  if (ls == 1) {
+    if (j == 1) {
+      memcpy(c->th + (size_t)c->ttop * dim, theta, sizeof(double) * dim);
+      memcpy(c->mom + (size_t)c->ttop * dim, mom, sizeof(double) * dim);
+      c->ttop++;
+    }
+    int r_n, r_s;
+    or_nuts_build(c, theta, mom, j - 1, minus, &r_n, &r_s);
+    const double bias = (double)r_n / (double)(ln + r_n);
+    if (or_nuts_bern(c, bias)) /* keep the right subtree's proposal */
+      memcpy(c->prop + (size_t)(c->ptop - 2) * dim, c->prop + (size_t)(c->ptop - 1) * dim, sizeof(double) * dim);
+    c->ptop--;
+    ln += r_n;
+    const double* pk_t = c->th + (size_t)(c->ttop - 1) * dim;
+    const double* pk_m = c->mom + (size_t)(c->ttop - 1) * dim;
+    if (minus) ls = or_nuts_update_s(c, r_s, pk_t, pk_m, theta, mom);
+    else ls = or_nuts_update_s(c, r_s, theta, mom, pk_t, pk_m);
+    if (j > 1) c->ttop--;
+  }
+  *rn = ln;
+  *rs = ls;
+}
+
+static int or_gmm_mu_nuts(const or_gmm_model* m, double* mu, const double* cov, const int* z,
+                          uint32_t seed, uint32_t chain, uint32_t sweep) {
+  const int dim = m->k * m->d, slots = OR_NUTS_MAX_DEPTH + 2;
+  or_nuts_ctx c;
+  c.m = m; c.cov = cov; c.z = z; c.dim = dim; c.seed = seed; c.chain = chain; c.sweep = sweep; c.nbern = 0;
+  c.prop = (double*)malloc(sizeof(double) * dim * slots);
+  c.th = (double*)malloc(sizeof(double) * dim * slots);
+  c.mom = (double*)malloc(sizeof(double) * dim * slots);
+  c.g = (double*)malloc(sizeof(double) * dim);
+  c.work = (double*)malloc(sizeof(double) * dim);
+  double* tm = (double*)malloc(sizeof(double) * dim);
+  double* tp = (double*)malloc(sizeof(double) * dim);
+  double* mm = (double*)malloc(sizeof(double) * dim);
+  double* mp = (double*)malloc(sizeof(double) * dim);
+  double* th = (double*)malloc(sizeof(double) * dim);
+  memcpy(tm, mu, sizeof(double) * dim);
+  memcpy(tp, mu, sizeof(double) * dim);
+  for (int i = 0; i < dim; i++) mm[i] = or_normal(seed, chain, (uint32_t)i, 0, sweep, OR_TAG_GMM_MU_MOVE);
+  memcpy(mp, mm, sizeof(double) * dim);
+  uint32_t w[4];
+  or_draw4(seed, chain, 0, 1, sweep, OR_TAG_GMM_MU_MOVE + 2, w);
+  c.logu = or_gmm_mu_fc(m, z, mu, cov) - 0.5 * or_dot(mm, mm, dim) - (-log(or_u01(w[0])));
+  c.ptop = 0;
+  int n = 1, j = 0, f_cpy = 0;
+  while (1) {
+    int rn, rs;
+    c.ttop = 0;
+    const int minus = or_nuts_bern(&c, 0.5);
+    if (minus) {
+      c.step = -m->mu_step_size;
+      memcpy(th, tm, sizeof(double) * dim);
+      or_nuts_build(&c, th, mm, j, 1, &rn, &rs);
+      memcpy(tm, th, sizeof(double) * dim);
+    } else {
+      c.step = m->mu_step_size;
+      memcpy(th, tp, sizeof(double) * dim);
+      or_nuts_build(&c, th, mp, j, 0, &rn, &rs);
+      memcpy(tp, th, sizeof(double) * dim);
+    }
+    const double* cand = c.prop + (size_t)(--c.ptop) * dim;
+    f_cpy = 0;
+    if (rs == 1) {
+      const double bias = fmin(1.0, (double)rn / (double)n);
+      if (or_nuts_bern(&c, bias)) {
+        memcpy(th, cand, sizeof(double) * dim);
+        f_cpy = 1;
+      }
+    }
+    n += rn;
+    const int s = or_nuts_update_s(&c, rs, tp, mp, tm, mm);
+    j += 1;
+    if (s != 1 || j >= OR_NUTS_MAX_DEPTH) break;
+  }
+  if (f_cpy) memcpy(mu, th, sizeof(double) * dim);
+  free(c.prop); free(c.th); free(c.mom); free(c.g); free(c.work);
+  free(tm); free(tp); free(mm); free(mp); free(th);
+  return f_cpy;
+}
+
 int or_gmm_mu_move(const or_gmm_model* m, double* mu, const double* cov, const int* z,
                    uint32_t seed, uint32_t chain, uint32_t sweep) {
   switch (m->mu_kernel) {
+    case 5: return or_gmm_mu_nuts(m, mu, cov, z, seed, chain, sweep);
     case 1: return or_gmm_mu_eslice(m, mu, cov, z, seed, chain, sweep);
     case 2: return or_gmm_mu_hmc(m, mu, cov, z, seed, chain, sweep);
     case 3: return or_gmm_mu_mwg(m, mu, cov, z, seed, chain, sweep);

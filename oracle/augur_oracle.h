@@ -143,7 +143,7 @@ typedef struct {
   int df;
   const double* scale0; /* d x d */
   int sample_pi, sample_cov;
-  /* the schedule's kernel for mu: 0 ConjGibbs, 1 ESlice, 2 HMC, 3 MWG, 4 RSlice */
+  /* the schedule's kernel for mu: 0 ConjGibbs, 1 ESlice, 2 HMC, 3 MWG, 4 RSlice, 5 NUTS */
   int mu_kernel;
   double mu_sim_len, mu_step_size;
   int mu_fixed_steps;         /* HMC: > 0 fixed, < 0 step jitter only, 0 reference jitter */

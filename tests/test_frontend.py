@@ -108,7 +108,7 @@ def test_reference_schedules_match_a_plan():
     pairs = {"lda1": "lda", "hlr1": "hlr", "hlr2": "hlr", "hmvgmm1": "hmvgmm", "hmvgmm2": "hmvgmm",
              "hmvgmm3": "hmvgmm", "mvgmm1": "mvgmm", "mvgmm2": "mvgmm", "mvgmm3": "mvgmm",
              "mvgmm4": "mvgmm", "mvgmm5": "mvgmm", "mvgmm6": "mvgmm"}
-    unsupported = {"mvgmm6"}  # NUTS [mu]
+    unsupported = set()
     dm = rvparse.parse_model(open(os.path.join(REF_TEST, "dethlr.rv")).read())
     assert plan.match(dm, None).names["bias"] == "bias"  # deterministic parameter
     got = {}
@@ -121,7 +121,7 @@ def test_reference_schedules_match_a_plan():
         except plan.UnsupportedModel:
             got[sched] = False
     assert {s for s, ok in got.items() if not ok} <= unsupported, got
-    for s in ("hmvgmm2", "hmvgmm3", "mvgmm2", "mvgmm3", "mvgmm4", "mvgmm5"):
+    for s in ("hmvgmm2", "hmvgmm3", "mvgmm2", "mvgmm3", "mvgmm4", "mvgmm5", "mvgmm6", "hlr2"):
         assert got[s], s
 
 

@@ -10,7 +10,7 @@ from augurv2_amd import diagnostics
 
 pytestmark = pytest.mark.gpu
 
-ESLICE, HMC, MWG, RSLICE = 1, 2, 3, 4
+ESLICE, HMC, MWG, RSLICE, NUTS = 1, 2, 3, 4, 5
 
 
 @pytest.fixture(scope="module")
@@ -35,6 +35,8 @@ def _settings(kernel, d):
         return dict(mu_sim_len=0.2, mu_step_size=0.02)
     if kernel == MWG:
         return dict(mu_prop_cov=0.01 * np.eye(d))
+    if kernel == NUTS:
+        return dict(mu_step_size=0.01)
     return {}
 
 
@@ -44,6 +46,7 @@ def _settings(kernel, d):
     (HMC, 16, 64, 6000, False),   # 64 components x 16: every thread of the block owns a mean entry
     (MWG, 4, 3, 3000, True), (MWG, 4, 3, 3000, False),
     (RSLICE, 4, 3, 3000, True), (RSLICE, 8, 5, 3000, False),
+    (NUTS, 4, 3, 3000, True), (NUTS, 4, 3, 3000, False), (NUTS, 16, 8, 2000, False),
 ])
 def test_gmm_mu_kernel_sweeps_match_oracle(rt, oracle, kernel, d, k, n, hier):
     """Two sweeps of ConjGibbs [pis] (*) <kernel> [mu] (*) ConjGibbs [cov] (*)
@@ -87,7 +90,7 @@ def test_gmm_mu_kernel_sweeps_match_oracle(rt, oracle, kernel, d, k, n, hier):
     s.close()
 
 
-@pytest.mark.parametrize("kernel", [ESLICE, HMC, MWG, RSLICE])
+@pytest.mark.parametrize("kernel", [ESLICE, HMC, MWG, RSLICE, NUTS])
 def test_gmm_mu_kernels_sample_the_conjugate_posterior(rt, kernel):
     """examples/mvgmm.py's mixture (K = 3, D = 2, fixed pis and covs), 300
     points: the posterior means of mu under each kernel equal those of
@@ -102,7 +105,8 @@ def test_gmm_mu_kernels_sample_the_conjugate_posterior(rt, kernel):
     out = {}
     for kern in (0, kernel):
         kw = {0: {}, ESLICE: {}, HMC: dict(mu_sim_len=0.1, mu_step_size=0.01),
-              MWG: dict(mu_prop_cov=0.002 * np.eye(D)), RSLICE: dict(mu_sim_len=0.05, mu_step_size=0.01)}[kern]
+              MWG: dict(mu_prop_cov=0.002 * np.eye(D)), RSLICE: dict(mu_sim_len=0.05, mu_step_size=0.01),
+              NUTS: dict(mu_step_size=0.01)}[kern]
         s = rt.GmmSampler(y, np.ones(K), np.zeros(D), np.eye(D), D + 2, np.eye(D), chains=C, seed=21 + kern,
                           sample_pi=False, sample_cov=False, mu_kernel=kern, ref_compat=False, **kw)
         s.set("pis", np.full(K, 1.0 / K))

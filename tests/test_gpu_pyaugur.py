@@ -137,6 +137,7 @@ def test_dethlr_deterministic_bias(sched):
     "HMC [mu] [0.2, 0.01] (*) DiscGibbs [z]",                            # sched3
     "MWG [mu] ~ MvNormal(mu[k], covs0) for k <- 0 until K (*) DiscGibbs [z]",  # sched4
     "RSlice [mu] [0.05, 0.01] (*) DiscGibbs [z]",                        # mvgmm5.infer
+    "NUTS [mu] [0.01] (*) DiscGibbs [z]",                                # mvgmm6.infer
 ])
 def test_mvgmm_example(sched):
     # examples/mvgmm.py:39-73 (K = 3, D = 2, fixed pis and covs) with each of
