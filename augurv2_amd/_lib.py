@@ -96,6 +96,10 @@ SIGNATURES = {
     "augur_bhlr_tuning": [_H, _P, _P, _P, _P, _I],
     "augur_bhlr_adapt": [_H, _I, _D],
     "augur_bhlr_step_sizes": [_H, _P, _P, _I],
+    "augur_lda_predictive_ll": [_H, _I, _P, _P, _P],
+    "augur_gmm_predictive_ll": [_H, _I, _P, _P],
+    "augur_op_lda_predictive_ll": [_I, _I, _I, _P, _P, _P, _P, _P],
+    "augur_op_gmm_predictive_ll": [_I, _I, _P, _P, _P, _I, _P, _P],
     "augur_set_pt": [_H, C.c_char_p, _I, _P, _S],
     "augur_cpy": [_H, C.c_char_p, _I, _P, _S],
     "augur_step": [_H, _I],

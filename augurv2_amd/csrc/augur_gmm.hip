@@ -710,6 +710,58 @@ __global__ __launch_bounds__(kMuThreads) void k_gmm_mu_move(GmmDev m, uint32_t s
   }
 }
 
+// Predictive log-likelihood of test points (experiments/hmvgmm/hmvgmm.py:
+// 143-178): sum_n log sum_k pis_k N(y_n | mu_k, cov_k), FP64.  The first
+// kernel forms each (component, chain)'s L^-1 and log pis_k - log det L -
+// D/2 log 2 pi; the second one log-sum-exp per point, one partial per block.
+__global__ __launch_bounds__(kConjLanes) void k_gmm_pred_prep(GmmDev m, double* __restrict__ pb) {
+  const int k = blockIdx.x, c = blockIdx.y, d = m.d, tid = threadIdx.x;
+  __shared__ double L[kMaxDim * kMaxDim], tmp[kMaxDim * kMaxDim];
+  const size_t ck = (size_t)c * m.k + k;
+  for (int e = tid; e < d * d; e += kConjLanes) L[e] = m.cov64[ck * d * d + e];
+  __syncthreads();
+  chol(L, d);
+  double ldet = 0.0;
+  for (int i = 0; i < d; ++i) ldet += log(L[i * d + i]);
+  lower_trmi(L, d, tmp);
+  double* o = pb + ck * (d * d + 1);
+  for (int e = tid; e < d * d; e += kConjLanes) o[e] = L[e];
+  if (tid == 0) o[d * d] = log(m.pis64[ck]) - ldet - 0.5 * d * log(6.283185307179586476925);
+}
+
+constexpr int kGmmPredBlocks = 256;
+__global__ __launch_bounds__(256) void k_gmm_predll(GmmDev m, const double* __restrict__ yt, int n,
+                                                    const double* __restrict__ pb, double* __restrict__ part) {
+  __shared__ double red[kMuWaves];
+  const int c = blockIdx.y, d = m.d, K = m.k;
+  double acc = 0.0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    const double* y = yt + (size_t)i * d;
+    double mx = -INFINITY, l[kMaxComp];
+    for (int k = 0; k < K; ++k) {
+      const size_t ck = (size_t)c * K + k;
+      const double* Li = pb + ck * (d * d + 1);
+      const double* mu = m.mu64 + ck * d;
+      double q = 0.0;
+      for (int a = 0; a < d; ++a) {
+        double t = 0.0;
+        for (int b = 0; b <= a; ++b) t += Li[a * d + b] * (y[b] - mu[b]);
+        q += t * t;
+      }
+      l[k] = Li[d * d] - 0.5 * q;
+      mx = fmax(mx, l[k]);
+    }
+    double se = 0.0;
+    for (int k = 0; k < K; ++k) se += exp(l[k] - mx);
+    acc += mx + log(se);
+  }
+  // block sum, waves in order
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) part[(size_t)c * gridDim.x + blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+}
+
 // Tables from the current parameters (after set_pt).
 __global__ __launch_bounds__(kConjLanes) void k_gmm_tables(GmmDev m) {
   const int k = blockIdx.x, c = blockIdx.y;
@@ -1560,6 +1612,50 @@ augur_status augur_gmm_create(const augur_gmm_desc* D, augur_sampler** out) {
       throw;
     }
     *out = s;
+  });
+}
+
+augur_status augur_gmm_predictive_ll(augur_sampler* s, int n, const double* y_test, double* out_per_chain) {
+  return guard([&] {
+    auto* h = dynamic_cast<GmmSampler*>(s);
+    if (!h || n < 0 || (n > 0 && !y_test) || !out_per_chain) fail(AUGUR_ERR_ARG, "needs a GMM sampler and arrays");
+    const GmmDev& m = h->m;
+    const int C = h->chains;
+    std::vector<double> part((size_t)C * kGmmPredBlocks, 0.0);
+    if (n > 0) {
+      DevBuf<double> yt((size_t)n * m.d), pb((size_t)C * m.k * (m.d * m.d + 1)), dpart(part.size());
+      yt.upload(y_test, (size_t)n * m.d, h->stream);
+      k_gmm_pred_prep<<<dim3(m.k, C), kConjLanes, 0, h->stream>>>(m, pb.p);
+      AUGUR_CHECK_LAUNCH();
+      k_gmm_predll<<<dim3(kGmmPredBlocks, C), 256, 0, h->stream>>>(m, yt.p, n, pb.p, dpart.p);
+      AUGUR_CHECK_LAUNCH();
+      dpart.download(part.data(), part.size(), h->stream);
+      AUGUR_HIP(hipStreamSynchronize(h->stream));
+    }
+    for (int c = 0; c < C; ++c) {
+      double t = 0.0;
+      for (int b = 0; b < kGmmPredBlocks; ++b) t += part[(size_t)c * kGmmPredBlocks + b];
+      out_per_chain[c] = t;
+    }
+  });
+}
+
+augur_status augur_op_gmm_predictive_ll(int k, int d, const double* pis, const double* mu, const double* cov,
+                                        int n, const double* y_test, double* out) {
+  return guard([&] {
+    if (k < 1 || k > kMaxComp || d < 1 || d > kMaxDim || n < 0 || !pis || !mu || !cov || (n > 0 && !y_test) || !out)
+      fail(AUGUR_ERR_ARG, "bad arguments");
+    require_device();
+    GmmSampler s;
+    AUGUR_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+    gmm_setup(&s, 1, d, k, 1);
+    s.pis64.upload(pis, k, s.stream);
+    s.mu64.upload(mu, (size_t)k * d, s.stream);
+    s.cov64.upload(cov, (size_t)k * d * d, s.stream);
+    double r = 0.0;
+    const augur_status st = augur_gmm_predictive_ll(&s, n, y_test, &r);
+    if (st != AUGUR_OK) fail(st, "%s", augur_last_error());
+    *out = r;
   });
 }
 

@@ -541,9 +541,115 @@ static void lda_zstep_op(int n_docs, int k, int v, const int32_t* doc_off, const
 
 }  // namespace augur
 
+namespace augur {
+
+// Predictive log-likelihood of held-out tokens under each chain's current
+// theta and phi (experiments/lda/lda.py:149-173): sum over tokens of
+// log sum_k theta[d, k] phi[k, w], FP64 log-sum-exp with max subtraction.
+// Grid (blocks, chains); one partial per block, summed in order.
+constexpr int kPredBlocks = 256;
+__global__ __launch_bounds__(256) void k_lda_predll(LdaDev m, const int32_t* __restrict__ doc, const int32_t* __restrict__ w,
+                                                    int T, double* __restrict__ part) {
+  __shared__ double red[4];
+  const int c = blockIdx.y, K = m.K;
+  const double* th = m.theta64 + (size_t)c * m.D * K;
+  const double* ph = m.phi64 + (size_t)c * K * m.V;
+  double acc = 0.0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < T; i += gridDim.x * 256) {
+    const int d = doc[i], v = w[i];
+    double mx = -INFINITY;
+    for (int k = 0; k < K; ++k) mx = fmax(mx, log(th[(size_t)d * K + k]) + log(ph[(size_t)k * m.V + v]));
+    double se = 0.0;
+    for (int k = 0; k < K; ++k) se += exp(log(th[(size_t)d * K + k]) + log(ph[(size_t)k * m.V + v]) - mx);
+    acc += mx + log(se);
+  }
+  const double bs = block_sum(acc, red);
+  if (threadIdx.x == 0) part[(size_t)c * gridDim.x + blockIdx.x] = bs;
+}
+
+}  // namespace augur
+
 using namespace augur;
 
 extern "C" {
+
+augur_status augur_lda_predictive_ll(augur_sampler* s, int n_docs, const int32_t* doc_off, const int32_t* w,
+                                     double* out_per_chain) {
+  return guard([&] {
+    auto* h = dynamic_cast<LdaSampler*>(s);
+    if (!h || !doc_off || !w || !out_per_chain) fail(AUGUR_ERR_ARG, "needs an LDA sampler and arrays");
+    if (n_docs != h->m.D) fail(AUGUR_ERR_ARG, "held-out documents: %d, the model has %d", n_docs, h->m.D);
+    const int T = doc_off[n_docs];
+    if (doc_off[0] != 0 || T < 0) fail(AUGUR_ERR_ARG, "bad held-out offsets");
+    std::vector<int32_t> doc((size_t)std::max(T, 1));
+    for (int d = 0; d < n_docs; ++d) {
+      if (doc_off[d + 1] < doc_off[d]) fail(AUGUR_ERR_ARG, "held-out offsets decrease at %d", d);
+      for (int i = doc_off[d]; i < doc_off[d + 1]; ++i) doc[i] = d;
+    }
+    for (int i = 0; i < T; ++i)
+      if (w[i] < 0 || w[i] >= h->m.V) fail(AUGUR_ERR_ARG, "held-out word %d outside [0, %d)", w[i], h->m.V);
+    const int C = h->chains;
+    std::vector<double> part((size_t)C * kPredBlocks);
+    if (T > 0) {
+      DevBuf<int32_t> ddoc(T), dw(T);
+      DevBuf<double> dpart((size_t)C * kPredBlocks);
+      ddoc.upload(doc.data(), T, h->stream);
+      dw.upload(w, T, h->stream);
+      k_lda_predll<<<dim3(kPredBlocks, C), 256, 0, h->stream>>>(h->m, ddoc.p, dw.p, T, dpart.p);
+      AUGUR_CHECK_LAUNCH();
+      dpart.download(part.data(), part.size(), h->stream);
+      AUGUR_HIP(hipStreamSynchronize(h->stream));
+    }
+    for (int c = 0; c < C; ++c) {
+      double t = 0.0;
+      for (int b = 0; b < kPredBlocks && T > 0; ++b) t += part[(size_t)c * kPredBlocks + b];
+      out_per_chain[c] = t;
+    }
+  });
+}
+
+augur_status augur_op_lda_predictive_ll(int n_docs, int k, int v, const double* theta, const double* phi,
+                                        const int32_t* doc_off, const int32_t* w, double* out) {
+  return guard([&] {
+    if (n_docs < 1 || k < 1 || v < 1 || !theta || !phi || !doc_off || !w || !out) fail(AUGUR_ERR_ARG, "bad arguments");
+    require_device();
+    const int T = doc_off[n_docs];
+    if (doc_off[0] != 0 || T < 0) fail(AUGUR_ERR_ARG, "bad held-out offsets");
+    std::vector<int32_t> doc((size_t)std::max(T, 1));
+    for (int d = 0; d < n_docs; ++d) {
+      if (doc_off[d + 1] < doc_off[d]) fail(AUGUR_ERR_ARG, "held-out offsets decrease at %d", d);
+      for (int i = doc_off[d]; i < doc_off[d + 1]; ++i) doc[i] = d;
+    }
+    for (int i = 0; i < T; ++i)
+      if (w[i] < 0 || w[i] >= v) fail(AUGUR_ERR_ARG, "held-out word %d outside [0, %d)", w[i], v);
+    *out = 0.0;
+    if (T == 0) return;
+    hipStream_t st;
+    AUGUR_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    try {
+      DevBuf<double> th((size_t)n_docs * k), ph((size_t)k * v), part(kPredBlocks);
+      DevBuf<int32_t> ddoc(T), dw(T);
+      th.upload(theta, th.n, st);
+      ph.upload(phi, ph.n, st);
+      ddoc.upload(doc.data(), T, st);
+      dw.upload(w, T, st);
+      LdaDev m{};
+      m.K = k; m.D = n_docs; m.V = v; m.theta64 = th.p; m.phi64 = ph.p;
+      k_lda_predll<<<dim3(kPredBlocks, 1), 256, 0, st>>>(m, ddoc.p, dw.p, T, part.p);
+      AUGUR_CHECK_LAUNCH();
+      std::vector<double> hp(kPredBlocks);
+      part.download(hp.data(), kPredBlocks, st);
+      AUGUR_HIP(hipStreamSynchronize(st));
+      double t = 0.0;
+      for (double x : hp) t += x;
+      *out = t;
+    } catch (...) {
+      (void)hipStreamDestroy(st);
+      throw;
+    }
+    AUGUR_HIP(hipStreamDestroy(st));
+  });
+}
 
 augur_status augur_lda_create(const augur_lda_desc* d, augur_sampler** out) {
   return guard([&] {

@@ -111,6 +111,15 @@ class LdaSampler(Sampler):
                                      "z": ((T,), np.int32)})
 
 
+    def predictive_ll(self, doc_off_ho, w_ho):
+        """Per chain: sum over the held-out tokens (CSR over the model's
+        documents) of log sum_k theta[d, k] phi[k, w] (FP64, on the device)."""
+        off, w = L.i32(doc_off_ho), L.i32(w_ho)
+        out = np.zeros(self.chains)
+        L.call("augur_lda_predictive_ll", self._h, len(off) - 1, L.ptr(off), L.ptr(w), L.ptr(out))
+        return out
+
+
 class HlrSampler(Sampler):
     family = "hlr"
 
@@ -261,6 +270,14 @@ class GmmSampler(Sampler):
         L.call("augur_gmm_create", C.byref(desc), C.byref(h))
         super().__init__(h, chains, {"pis": ((k,), np.float64), "mu": ((k, dim), np.float64),
                                      "cov": ((k, dim, dim), np.float64), "z": ((n,), np.int32)})
+
+    def predictive_ll(self, y_test):
+        """Per chain: sum over test points of log sum_k pis_k N(y | mu_k, cov_k)
+        (FP64, on the device)."""
+        yt = L.f64(y_test)
+        out = np.zeros(self.chains)
+        L.call("augur_gmm_predictive_ll", self._h, yt.shape[0], L.ptr(yt), L.ptr(out))
+        return out
 
     def mu_acceptance(self):
         """Per chain: accepted / proposed moves of a non-conjugate mu kernel

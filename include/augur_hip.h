@@ -244,6 +244,26 @@ typedef struct {
 } augur_gmm_desc;
 augur_status augur_gmm_create(const augur_gmm_desc* desc, augur_sampler** out);
 
+/* No counterpart in the C runtime: the experiments harness's predictive
+ * log-likelihood on the device, per chain at the current state.
+ *   LDA (experiments/lda/lda.py:149-173): held-out tokens of the model's
+ *   n_docs documents (CSR doc_off[n_docs + 1], w), sum over tokens of
+ *   log sum_k theta[d, k] phi[k, w].
+ *   GMM (experiments/hmvgmm/hmvgmm.py:143-178): n test points y_test
+ *   (n x D f64), sum over points of log sum_k pis_k MvNormal(y | mu_k, cov_k).
+ * FP64 throughout. */
+augur_status augur_lda_predictive_ll(augur_sampler* s, int n_docs, const int32_t* doc_off,
+                                     const int32_t* w, double* out_per_chain);
+augur_status augur_gmm_predictive_ll(augur_sampler* s, int n, const double* y_test,
+                                     double* out_per_chain);
+/* The same sums for given parameters (host buffers): theta (n_docs x K),
+ * phi (K x V); pis (K), mu (K x D), cov (K x D x D).  One value out. */
+augur_status augur_op_lda_predictive_ll(int n_docs, int k, int v, const double* theta,
+                                        const double* phi, const int32_t* doc_off, const int32_t* w,
+                                        double* out);
+augur_status augur_op_gmm_predictive_ll(int k, int d, const double* pis, const double* mu,
+                                        const double* cov, int n, const double* y_test, double* out);
+
 /* ------------------------------------------------------------------------
  * Sampler operation (generic over families).
  * ---------------------------------------------------------------------- */

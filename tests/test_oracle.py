@@ -555,3 +555,34 @@ def test_gmm_nuts_reference_variant_keeps_the_means(oracle):
             x = draws[:, j, i]
             assert abs(x.mean() - m[i]) <= 5 * diagnostics.mcse_mean(x[None, :])
             assert 0.9 <= x.var() / a[i, i] <= 1.2
+
+
+# ---- predictive log-likelihood of the experiments harness ----
+
+def test_predll_restatement_reduces_to_closed_forms():
+    """With one topic / one component the log-sum-exp is the single term:
+    log theta + log phi per token, and scipy's MvNormal log density."""
+    from oracle import predll
+    rng = np.random.default_rng(50)
+    theta = np.ones((3, 1))
+    phi = rng.dirichlet(np.ones(7), 1)
+    w = [rng.integers(0, 7, n) for n in (4, 0, 5)]
+    ref = sum(np.log(phi[0][x]) for d in w for x in d)
+    assert abs(predll.lda_term(theta, phi, w) - ref) <= 1e-12 * abs(ref)
+    y = rng.standard_normal((20, 3))
+    mu, cov = rng.standard_normal((1, 3)), np.eye(3)[None] * 2.0
+    ref = sps.multivariate_normal.logpdf(y, mu[0], cov[0]).sum()
+    assert abs(predll.gmm_term(np.ones(1), mu, cov, y) - ref) <= 1e-12 * abs(ref)
+    # two identical components with weights 1/2 each: the same value
+    mu2, cov2 = np.concatenate([mu, mu]), np.concatenate([cov, cov])
+    assert abs(predll.gmm_term(np.full(2, 0.5), mu2, cov2, y) - ref) <= 1e-12 * abs(ref)
+
+
+def test_stats_yml_format(tmp_path):
+    """experiments/util/modelenv.py:73-86 writes compile, sample and predll."""
+    from augurv2_amd.experiments import dump_pred_ll_stats
+    import yaml
+    p = dump_pred_ll_stats(str(tmp_path / "stat"), 1.5, 2.25, [-10.0, -9.5])
+    text = open(p).read()
+    assert text == "compile: 1.5\nsample: 2.25\npredll: [-10.0, -9.5]\n"
+    assert yaml.safe_load(text) == {"compile": 1.5, "sample": 2.25, "predll": [-10.0, -9.5]}
