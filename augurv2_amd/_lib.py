@@ -70,6 +70,16 @@ class GmmDesc(C.Structure):
                 ("mu_prop_cov", C.c_void_p), ("ref_compat", C.c_int)]
 
 
+class GenDesc(C.Structure):
+    _fields_ = [("source", C.c_char_p), ("kernel", C.c_char_p), ("n_params", C.c_int),
+                ("grid", C.c_int), ("chains", C.c_int), ("seed", C.c_uint32),
+                ("reals", C.c_void_p), ("n_reals", C.c_size_t), ("ints", C.c_void_p),
+                ("n_ints", C.c_size_t), ("n_names", C.c_int), ("names", C.c_void_p),
+                ("offsets", C.c_void_p), ("sizes", C.c_void_p), ("n_blocks", C.c_int),
+                ("masks", C.c_void_p), ("sim_len", C.c_void_p), ("step_size", C.c_void_p),
+                ("fixed_steps", C.c_void_p)]
+
+
 GMM_MU_CONJ, GMM_MU_ESLICE, GMM_MU_HMC, GMM_MU_MWG, GMM_MU_RSLICE, GMM_MU_NUTS = 0, 1, 2, 3, 4, 5
 
 
@@ -89,6 +99,8 @@ SIGNATURES = {
     "augur_hlr_create": [_P, _P],
     "augur_gmm_create": [_P, _P],
     "augur_bhlr_create": [_P, _P],
+    "augur_gen_create": [_P, _P],
+    "augur_gen_eval": [_H, _P, _P, _P],
     "augur_hlr_surrogate": [_H, _P],
     "augur_hlr_surrogate_info": [_H, _P, _S],
     "augur_op_mlp_loss_grad": [_I, _I, _I, _P, _P, _P, _P, _P, _P],

@@ -70,3 +70,22 @@ def initInvWishart(strat, df, scale, rng):
     if isinstance(strat, AugurDefault):
         return np.eye(scale.shape[0])
     return np.atleast_2d(sps.invwishart.rvs(df=df, scale=scale, random_state=rng))
+
+
+def initGamma(strat, shape, rate, rng, size=None):
+    if isinstance(strat, AugurDefault):
+        return np.full(size, 0.5) if size is not None else 0.5
+    return rng.gamma(shape, 1.0 / rate, size=size)
+
+
+def initBeta(strat, a, b, rng, size=None):
+    if isinstance(strat, AugurDefault):
+        return np.full(size, 1.0) if size is not None else 1.0
+    return rng.beta(a, b, size=size)
+
+
+def initUniform(strat, lower, upper, rng, size=None):
+    if isinstance(strat, AugurDefault):
+        mid = (np.asarray(lower) + np.asarray(upper)) / 2.0
+        return np.broadcast_to(mid, size).copy() if size is not None else float(mid)
+    return rng.uniform(lower, upper, size=size)

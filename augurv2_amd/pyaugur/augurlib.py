@@ -16,7 +16,8 @@ follow the reference:
 
 Additions (keyword arguments with reference-compatible defaults): chains,
 seed, hmc_jitter (False: fixed leapfrog count round(simLen / eps)),
-ref_compat (the reference's normal_ll / normal_dotarg2 forms).
+ref_compat (the reference's normal_ll / normal_dotarg2 forms), generic
+(True: skip the hand-written families and lower the model generically).
 """
 import logging
 import os
@@ -39,7 +40,7 @@ class AugurOpt:
     compatibility (unused by the reference's compiler too)."""
 
     def __init__(self, cached=False, target="cpu", initStrat=None, paramScale=None, chains=1,
-                 seed=0, hmc_jitter=True, ref_compat=True):
+                 seed=0, hmc_jitter=True, ref_compat=True, generic=False):
         assert target in ("cpu", "gpu")
         self.cached = cached
         self.target = target
@@ -50,6 +51,7 @@ class AugurOpt:
         self.seed = int(seed)
         self.hmc_jitter = bool(hmc_jitter)
         self.ref_compat = bool(ref_compat)
+        self.generic = bool(generic)
 
 
 def _read_config(path):
@@ -120,7 +122,7 @@ class AugurInfer:
                                 f"got {len(data)}")
             hyp = {name: val for (name, _), val in zip(model.hypers, hypers)}
             dat = {d.name: val for d, val in zip(model.data, data)}
-            self.plan = match(model, sched)
+            self.plan = match(model, sched, generic=self.aopt.generic)
             self.sampler = self.plan.build(hyp, dat, self.aopt)
             self.param_names = list(self.plan.param_names)
             self._init_chains()

@@ -1,0 +1,683 @@
+"""Generic lowering of continuous models to a HIP data pass.
+
+This plays the role of the reference's code generators for models that no
+hand-written sampler family covers.  The reference's generators are
+compiler/augur/src/CudaC/CgCuda.hs:371-516 (map / map-reduce kernels), with
+Core/CgLike.hs:90-267 (the log joint) and Core/CgGrad.hs:151-376 (its
+reverse-mode gradient).  Here a model whose parameters are all continuous
+becomes one HIP kernel source.  It is compiled at run time with hipRTC and
+driven by the library's device-resident HMC (augur_gen_create).
+
+The generated kernel ``augur_gen_pass`` runs on a grid of (G, chains)
+blocks of 256 threads.  Each block does the following.
+  - It loads its chain's parameter vector q (P reals) into LDS.
+  - It walks every declaration ``x[i..] ~ Dist(args) for i <- 0 until n``
+    as a grid-stride loop over the flattened grid.  A declaration without a
+    grid runs on the chain's first thread only.
+  - At each point it adds the log density and the reverse-mode
+    contributions to d ll / dq, in FP64.  A contribution to a coordinate
+    known when the kernel is generated goes to a register accumulator.  One
+    whose coordinate depends on the grid index or on data (theta[k],
+    alpha[g[n]]) is an LDS atomic add.
+  - It sums the register accumulators over the block in a fixed order.
+  - It writes one row of P + 1 partials (the gradient, then ll).
+The library sums the G rows of each chain in a fixed order.  The
+result is deterministic whenever no coordinate receives two LDS atomic adds
+from one block.
+
+Layout
+  - The parameters come in declaration order, each flattened row-major.
+    Together they form q; chain c's vector is q[c * P .. c * P + P).
+  - Real data and real hyper-parameter arrays are packed into one FP64
+    buffer R.  Int arrays are packed into one int32 buffer I.  Arrays of
+    rank 2 are stored column-major, so that x[n][k] at consecutive n are
+    adjacent and a wave's loads coalesce.
+  - Scalar hyper-parameters and every size are constants in the source.
+
+Supported
+  - Parameters: Normal, Exponential, Gamma, InvGamma, Beta and Uniform, of
+    rank 0-2 over rectangular grids.
+  - Data: those distributions plus Bernoulli and Poisson.
+  - Expressions: + - * /, sigmoid, dotprod, literals, and projections
+    indexed by grid variables, literals or int data.
+  - Schedules: HMC blocks.
+  - Anything else raises LoweringError.
+"""
+import math
+
+import numpy as np
+
+PARAM_DISTS = {"Normal", "Exponential", "Gamma", "InvGamma", "Beta", "Uniform"}
+DATA_DISTS = PARAM_DISTS | {"Bernoulli", "Poisson"}
+INT_DISTS = {"Bernoulli", "Poisson"}
+ARITY = {"Normal": 2, "Exponential": 1, "Gamma": 2, "InvGamma": 2, "Beta": 2, "Uniform": 2,
+         "Bernoulli": 1, "Poisson": 1}
+THREADS = 256
+MAX_GRID = 512
+MAX_PARAMS = 3000      # q and the gradient row are held in LDS
+MAX_REG_SLOTS = 96     # FP64 register accumulators per thread
+UNROLL_CAP = 128       # longest dotprod whose adjoint loop is unrolled into registers
+
+
+class LoweringError(ValueError):
+    """The model is outside what the generic lowering supports."""
+
+
+def _fail(msg):
+    raise LoweringError(msg)
+
+
+def _lit(v):
+    """C++ literal that parses to exactly the double v."""
+    v = float(v)
+    if math.isnan(v) or math.isinf(v):
+        _fail(f"non-finite constant {v}")
+    return repr(v) if ("e" in repr(v) or "." in repr(v)) else repr(v) + ".0"
+
+
+# ---- the device prelude: densities and derivatives of cbits/augur_dist.c ----
+
+PRELUDE = r"""
+#define GEN_LARGE 1e8
+// cbits/augur_dist.c:372-394 (kRefCompat: the reference's -log var and
+// -0.5 var forms; otherwise the textbook -0.5 log var and -0.5 / var)
+__device__ inline double gen_normal_ll(double pt, double mean, double var) {
+  if (var > 0.0) { const double t = pt - mean; return -(kRefCompat ? 1.0 : 0.5) * log(var) - 0.5 * t * t / var; }
+  return -GEN_LARGE;
+}
+__device__ inline double gen_normal_dotpt(double pt, double mean, double var) { return -(pt - mean) / var; }
+__device__ inline double gen_normal_dotarg1(double pt, double mean, double var) { return (pt - mean) / var; }
+__device__ inline double gen_normal_dotarg2(double pt, double mean, double var) {
+  const double t = pt - mean;
+  return (kRefCompat ? -0.5 * var : -0.5 / var) + 0.5 * (t * t) / (var * var);
+}
+// :59-71
+__device__ inline double gen_bernoulli_ll(int pt, double bias) {
+  if (bias >= 0.0 && bias <= 1.0) return log(pt * bias + (1 - pt) * (1.0 - bias));
+  return -GEN_LARGE;
+}
+__device__ inline double gen_bernoulli_dotarg1(int pt, double bias) {
+  return (double)(pt + pt - 1) / (pt * bias + (1 - pt) * (1.0 - bias));
+}
+// :124-139 (sum of log k for k = 1..pt as lgamma(pt + 1))
+__device__ inline double gen_poisson_ll(int pt, double rate) { return pt * log(rate) - rate - lgamma(pt + 1.0); }
+__device__ inline double gen_poisson_dotarg1(int pt, double rate) { return rate > 0.0 ? pt / rate - 1.0 : -GEN_LARGE; }
+// :231-256
+__device__ inline double gen_exponential_ll(double pt, double rate) {
+  return (rate > 0.0 && pt > 0.0) ? log(rate) - rate * pt : -GEN_LARGE;
+}
+__device__ inline double gen_exponential_dotpt(double pt, double rate) { return (rate > 0.0 && pt > 0.0) ? -rate : 0.0; }
+__device__ inline double gen_exponential_dotarg1(double pt, double rate) {
+  return (rate > 0.0 && pt > 0.0) ? 1.0 / rate - pt : 0.0;
+}
+// digamma: recurrence psi(x) = psi(x + 1) - 1 / x up to x >= 6, then the
+// asymptotic series ln x - 1/(2x) - sum B_2k / (2k x^2k) (Abramowitz and
+// Stegun 6.3.5, 6.3.18), five terms
+__device__ inline double gen_digamma(double x) {
+  double r = 0.0;
+  while (x < 6.0) { r -= 1.0 / x; x += 1.0; }
+  const double f = 1.0 / (x * x);
+  const double s = f * (1.0 / 12 - f * (1.0 / 120 - f * (1.0 / 252 - f * (1.0 / 240 - f * (1.0 / 132)))));
+  return r + log(x) - 0.5 / x - s;
+}
+// :265-285 (the log of the pdf term by term)
+__device__ inline double gen_gamma_ll(double pt, double shape, double rate) {
+  return shape * log(rate) - lgamma(shape) + (shape - 1.0) * log(pt) - rate * pt;
+}
+__device__ inline double gen_gamma_dotpt(double pt, double shape, double rate) { return ((shape - 1.0) / pt) - rate; }
+__device__ inline double gen_gamma_dotarg1(double pt, double shape, double rate) {
+  return -gen_digamma(shape) + log(rate) + log(pt);
+}
+__device__ inline double gen_gamma_dotarg2(double pt, double shape, double rate) { return (shape / rate) - pt; }
+// :345-359
+__device__ inline double gen_invgamma_ll(double pt, double shape, double scale) {
+  return shape * log(scale) - lgamma(shape) - (shape + 1.0) * log(pt) - scale / pt;
+}
+__device__ inline double gen_invgamma_dotpt(double pt, double shape, double scale) {
+  return -(shape + 1.0) / pt + scale / (pt * pt);
+}
+__device__ inline double gen_invgamma_dotarg1(double pt, double shape, double scale) {
+  return log(scale) - gen_digamma(shape) - log(pt);
+}
+__device__ inline double gen_invgamma_dotarg2(double pt, double shape, double scale) { return shape / scale - 1.0 / pt; }
+// :158-177 (the log of the pdf term by term)
+__device__ inline double gen_beta_ll(double pt, double a, double b) {
+  return lgamma(a + b) - lgamma(a) - lgamma(b) + (a - 1.0) * log(pt) + (b - 1.0) * log(1.0 - pt);
+}
+__device__ inline double gen_beta_dotpt(double pt, double a, double b) { return (a - 1.0) / pt - (b - 1.0) / (1.0 - pt); }
+__device__ inline double gen_beta_dotarg1(double pt, double a, double b) {
+  return gen_digamma(a + b) - gen_digamma(a) + log(pt);
+}
+__device__ inline double gen_beta_dotarg2(double pt, double a, double b) {
+  return gen_digamma(a + b) - gen_digamma(b) + log(1.0 - pt);
+}
+// :426-455
+__device__ inline double gen_uniform_ll(double pt, double l, double u) { return (l < pt && pt < u) ? -log(u - l) : -GEN_LARGE; }
+__device__ inline double gen_uniform_dotpt(double pt, double l, double u) { return 0.0; }
+__device__ inline double gen_uniform_dotarg1(double pt, double l, double u) { return (l < pt && pt < u) ? 1.0 / (u - l) : 0.0; }
+__device__ inline double gen_uniform_dotarg2(double pt, double l, double u) { return (l < pt && pt < u) ? -1.0 / (u - l) : 0.0; }
+
+__device__ inline double gen_wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+"""
+
+
+# ---- values of the code generator ----------------------------------------------
+
+class _Val:
+    """A generated value: a scalar (C++ expression ``code``) or a vector
+    view (``elem(k)`` gives element k's expression, ``coord(k)`` its
+    coordinate in q when the vector is a parameter's slice)."""
+
+    def __init__(self, code=None, dep=False, op=None, kids=(), coord=None, length=None, elem=None,
+                 vcoord=None):
+        self.code = code
+        self.dep = dep
+        self.op = op
+        self.kids = kids
+        self.coord = coord        # scalar parameter element: int or C++ int expression
+        self.length = length      # vector views
+        self.elem = elem
+        self.vcoord = vcoord
+
+    @property
+    def is_vec(self):
+        return self.length is not None
+
+
+class _Sym:
+    """What a name denotes inside the model."""
+
+    def __init__(self, kind, shape, is_int=False, value=None, off=None):
+        self.kind = kind          # "hyper" | "param" | "data"
+        self.shape = shape        # () | (n,) | (n, m)
+        self.is_int = is_int
+        self.value = value        # hyper scalars: the constant
+        self.off = off            # offset into q (params) or into R / I (arrays)
+
+
+class Lowering:
+    """Lowers one model (parsed by rvparse) bound to its hyper-parameters and
+    data.  Attributes after construction:
+      source, kernel_name, P, grid, reals (f64), ints (i32),
+      layout {param: (offset, shape)}, param_names (declaration order)."""
+
+    kernel_name = "augur_gen_pass"
+
+    def __init__(self, model, hypers, data, ref_compat=True):
+        self.model = model
+        self.ref_compat = bool(ref_compat)
+        self.syms = {}
+        self._int_values = {}      # int arrays by name (index bounds checks)
+        self._grid_ranges = {}     # grid variable -> extent, in the current term
+        self._reals, self._ints = [], []
+        self._nreal = self._nint = 0
+        self._bind_hypers(hypers)
+        self._bind_params()
+        self._bind_data(data)
+        self.slots = {}            # coordinate -> register slot
+        self.param_slots = {}      # param -> first slot (params held whole in registers)
+        self._tmp = 0
+        self._emit_kernel()
+
+    # ---- binding ----
+    def _pack(self, arr, is_int):
+        arr = np.asarray(arr)
+        if arr.ndim == 2:
+            arr = arr.T  # column-major
+        flat = np.ascontiguousarray(arr.ravel(), np.int32 if is_int else np.float64)
+        if is_int:
+            off = self._nint
+            self._ints.append(flat)
+            self._nint += flat.size
+        else:
+            off = self._nreal
+            self._reals.append(flat)
+            self._nreal += flat.size
+        return off
+
+    @staticmethod
+    def _as_array(v, ty):
+        rank = 0
+        t = ty
+        while t[0] in ("Vec", "Mat"):
+            rank += 1 if t[0] == "Vec" else 2
+            t = t[1]
+        if rank == 0:
+            return None, t
+        if isinstance(v, np.ndarray) and v.dtype != object:
+            a = v
+        else:
+            rows = [np.asarray(r) for r in v]
+            if rank == 2 and len({r.shape for r in rows}) > 1:
+                _fail("ragged arrays are outside the generic lowering")
+            a = np.stack(rows) if rank == 2 else np.asarray(v)
+        if a.ndim != rank:
+            _fail(f"expected an array of rank {rank}, got shape {a.shape}")
+        return a, t
+
+    def _bind_hypers(self, hypers):
+        for name, ty in self.model.hypers:
+            v = hypers[name]
+            a, base = self._as_array(v, ty)
+            is_int = base == ("Int",)
+            if a is None:
+                self.syms[name] = _Sym("hyper", (), is_int, value=int(v) if is_int else float(v))
+            else:
+                self.syms[name] = _Sym("hyper", a.shape, is_int, off=self._pack(a, is_int))
+                if is_int:
+                    self._int_values[name] = np.asarray(a, np.int64)
+
+    def _grid_sizes(self, d):
+        sizes = []
+        for var, lo, hi in d.grids:
+            if self._const(lo) != 0:
+                _fail(f"{d.name}: grids must start at 0")
+            n = self._const(hi)
+            if not isinstance(n, int) or n < 0:
+                _fail(f"{d.name}: grid bound must be a non-negative integer constant")
+            sizes.append(n)
+        return tuple(sizes)
+
+    def _bind_params(self):
+        off = 0
+        self.layout = {}
+        self.param_names = []
+        for d in self.model.params:
+            if not d.stochastic:
+                _fail(f"deterministic parameter {d.name}: outside the generic lowering")
+            if d.exp[0] != "dist" or d.exp[1] not in PARAM_DISTS:
+                _fail(f"parameter {d.name}: the generic lowering samples continuous parameters "
+                      f"({', '.join(sorted(PARAM_DISTS))}) with HMC")
+            if len(d.grids) != len(d.idxs) or len(d.idxs) > 2:
+                _fail(f"parameter {d.name}: rank 0-2 over its own grid")
+            shape = self._grid_sizes(d)
+            size = int(np.prod(shape)) if shape else 1
+            self.syms[d.name] = _Sym("param", shape, off=off)
+            self.layout[d.name] = (off, shape)
+            self.param_names.append(d.name)
+            off += size
+        self.P = off
+        if self.P == 0:
+            _fail("the model has no parameters")
+        if self.P > MAX_PARAMS:
+            _fail(f"{self.P} parameter coordinates: the generic lowering holds at most {MAX_PARAMS}")
+
+    def _bind_data(self, data):
+        for d in self.model.data:
+            if not d.stochastic or d.exp[0] != "dist" or d.exp[1] not in DATA_DISTS:
+                _fail(f"data {d.name}: supported distributions are {', '.join(sorted(DATA_DISTS))}")
+            if len(d.grids) != len(d.idxs) or len(d.idxs) > 2:
+                _fail(f"data {d.name}: rank 0-2 over its own grid")
+            shape = self._grid_sizes(d)
+            is_int = d.exp[1] in INT_DISTS
+            v = data[d.name]
+            if shape:
+                a = np.asarray(v if not (isinstance(v, np.ndarray) and v.dtype == object)
+                               else np.stack([np.asarray(r) for r in v]))
+                if a.shape != shape:
+                    _fail(f"data {d.name}: expected shape {shape}, got {a.shape}")
+                self.syms[d.name] = _Sym("data", shape, is_int, off=self._pack(a, is_int))
+                if is_int:
+                    self._int_values[d.name] = np.asarray(a, np.int64)
+            else:
+                self.syms[d.name] = _Sym("data", (), is_int, value=int(v) if is_int else float(v))
+
+    def _const(self, e):
+        """Compile-time value of an expression over literals and scalar hypers."""
+        k = e[0]
+        if k == "lit":
+            return e[1]
+        if k == "var":
+            s = self.syms.get(e[1])
+            if s is None or s.kind != "hyper" or s.shape != ():
+                _fail(f"{e[1]} is not a scalar hyper-parameter")
+            return s.value
+        if k == "bin":
+            a, b = self._const(e[2]), self._const(e[3])
+            if e[1] == "/" and isinstance(a, int) and isinstance(b, int):
+                return a // b
+            return {"+": a + b, "-": a - b, "*": a * b, "/": a / b}[e[1]]
+        _fail(f"expected a constant expression, found {e}")
+
+    @property
+    def reals(self):
+        return np.concatenate(self._reals) if self._reals else np.zeros(1)
+
+    @property
+    def ints(self):
+        return np.concatenate(self._ints) if self._ints else np.zeros(1, np.int32)
+
+    # ---- code emission helpers ----
+    def _new(self, prefix="v"):
+        self._tmp += 1
+        return f"{prefix}{self._tmp}"
+
+    def _temp(self, lines, expr, prefix="v"):
+        name = self._new(prefix)
+        lines.append(f"const double {name} = {expr};")
+        return name
+
+    def _slot(self, param, coord):
+        """Register slot of a constant coordinate, or None (use LDS atomics)."""
+        if param not in self.param_slots:
+            off, shape = self.layout[param]
+            size = int(np.prod(shape)) if shape else 1
+            used = sum(int(np.prod(self.layout[p][1])) if self.layout[p][1] else 1
+                       for p in self.param_slots)
+            if used + size > MAX_REG_SLOTS:
+                self.param_slots[param] = None
+            else:
+                self.param_slots[param] = used
+                for j in range(size):
+                    self.slots[off + j] = used + j
+        base = self.param_slots[param]
+        if base is None:
+            return None
+        return self.slots[coord]
+
+    def _add(self, lines, param, coord, val):
+        if isinstance(coord, int):
+            s = self._slot(param, coord)
+            if s is not None:
+                lines.append(f"acc[{s}] += {val};")
+                return
+        lines.append(f"atomicAdd(&gs[{coord}], {val});")
+
+    # ---- indices ----
+    def _range(self, e):
+        """Inclusive bounds of an index expression's values."""
+        k = e[0]
+        if k == "lit" and isinstance(e[1], int):
+            return e[1], e[1]
+        if k == "var":
+            if e[1] in self._grid_ranges:
+                return 0, self._grid_ranges[e[1]] - 1
+            s = self.syms.get(e[1])
+            if s is not None and s.shape == () and s.is_int and s.kind != "param":
+                return s.value, s.value
+        if k == "proj" and e[1][0] == "var" and e[1][1] in self._int_values:
+            self._check_proj(e[1][1], e[2])
+            vals = self._int_values[e[1][1]]
+            return (int(vals.min()), int(vals.max())) if vals.size else (0, -1)
+        if k == "bin" and e[1] in "+-*":
+            (al, ah), (bl, bh) = self._range(e[2]), self._range(e[3])
+            if e[1] == "+":
+                return al + bl, ah + bh
+            if e[1] == "-":
+                return al - bh, ah - bl
+            prods = [al * bl, al * bh, ah * bl, ah * bh]
+            return min(prods), max(prods)
+        _fail(f"unsupported index expression {e}")
+
+    def _check_proj(self, name, idx_exprs):
+        """Every index of name[idx...] stays inside its dimension."""
+        s = self.syms[name]
+        for dim, e in zip(s.shape, idx_exprs):
+            lo, hi = self._range(e)
+            if lo < 0 or hi >= dim:
+                _fail(f"an index of {name} ranges over [{lo}, {hi}], outside [0, {dim})")
+
+    def _index(self, e, env):
+        """Integer index expression: a Python int or a C++ int expression."""
+        k = e[0]
+        if k == "lit":
+            if not isinstance(e[1], int):
+                _fail(f"index {e[1]} is not an integer")
+            return e[1]
+        if k == "var":
+            if e[1] in env:
+                return env[e[1]]
+            s = self.syms.get(e[1])
+            if s and s.kind == "hyper" and s.shape == () and s.is_int:
+                return s.value
+            if s and s.kind == "data" and s.shape == () and s.is_int:
+                return s.value
+            _fail(f"{e[1]} cannot index")
+        if k == "proj":
+            name = e[1][1] if e[1][0] == "var" else None
+            s = self.syms.get(name)
+            if s is None or not s.is_int or s.kind == "param" or len(e[2]) != len(s.shape):
+                _fail(f"index {e} must be an element of an int array")
+            return f"I[{self._flat(s, [self._index(i, env) for i in e[2]])}]"
+        if k == "bin" and e[1] in "+-*":
+            a, b = self._index(e[2], env), self._index(e[3], env)
+            if isinstance(a, int) and isinstance(b, int):
+                return {"+": a + b, "-": a - b, "*": a * b}[e[1]]
+            return f"({a} {e[1]} {b})"
+        _fail(f"unsupported index expression {e}")
+
+    @staticmethod
+    def _flat(s, idx):
+        """Flat position of an element of an array symbol (column-major for
+        rank-2 data, row-major for parameters)."""
+        base = s.off
+        if len(idx) == 1:
+            i = idx[0]
+            return base + i if isinstance(i, int) and isinstance(base, int) else f"{base} + ({i})"
+        i, j = idx
+        if s.kind == "param":
+            if isinstance(i, int) and isinstance(j, int):
+                return base + i * s.shape[1] + j
+            return f"{base} + ({i}) * {s.shape[1]} + ({j})"
+        if isinstance(i, int) and isinstance(j, int):
+            return base + j * s.shape[0] + i
+        return f"{base} + ({j}) * {s.shape[0]} + ({i})"
+
+    # ---- forward evaluation ----
+    def _elem_code(self, s, pos):
+        if s.kind == "param":
+            return f"qs[{pos}]"
+        return f"(double)I[{pos}]" if s.is_int else f"R[{pos}]"
+
+    def _sym_val(self, name, s, idx, lines):
+        """Value of name[idx...] (idx may be shorter than the rank: a view)."""
+        rank = len(s.shape)
+        if len(idx) > rank:
+            _fail(f"{name} indexed {len(idx)} times but has rank {rank}")
+        if rank == 0:
+            if s.kind == "hyper":
+                return _Val(_lit(s.value))
+            if s.kind == "data":
+                return _Val(_lit(s.value))
+            return _Val(f"qs[{s.off}]", dep=True, coord=s.off, op=("param", name))
+        if len(idx) == rank:
+            pos = self._flat(s, idx)
+            code = self._temp(lines, self._elem_code(s, pos))
+            if s.kind == "param":
+                return _Val(code, dep=True, coord=pos, op=("param", name))
+            return _Val(code)
+        # a view of length shape[-1]: the whole vector, or one row of a matrix
+        length = s.shape[-1]
+        if rank == 1:
+            def pos_of(k):
+                return self._flat(s, [k])
+        else:
+            row = idx[0]
+
+            def pos_of(k):
+                return self._flat(s, [row, k])
+        dep = s.kind == "param"
+        return _Val(length=length, dep=dep, op=("param", name) if dep else None,
+                    elem=lambda k: self._elem_code(s, pos_of(k)), vcoord=pos_of if dep else None)
+
+    def _fwd(self, e, env, lines):
+        k = e[0]
+        if k == "lit":
+            return _Val(_lit(e[1]))
+        if k == "var":
+            if e[1] in env:
+                return _Val(f"(double)({env[e[1]]})")
+            s = self.syms.get(e[1])
+            if s is None:
+                _fail(f"unknown name {e[1]}")
+            return self._sym_val(e[1], s, [], lines)
+        if k == "proj":
+            if e[1][0] != "var":
+                _fail(f"unsupported projection {e}")
+            name = e[1][1]
+            s = self.syms.get(name)
+            if s is None:
+                _fail(f"unknown name {name}")
+            self._check_proj(name, e[2])
+            return self._sym_val(name, s, [self._index(i, env) for i in e[2]], lines)
+        if k == "bin":
+            a, b = self._fwd(e[2], env, lines), self._fwd(e[3], env, lines)
+            if a.is_vec or b.is_vec:
+                _fail("arithmetic on vectors: only dotprod takes vector arguments")
+            code = self._temp(lines, f"{a.code} {e[1]} {b.code}")
+            return _Val(code, dep=a.dep or b.dep, op=("bin", e[1]), kids=(a, b))
+        if k == "prim" and e[1] == "sigmoid":
+            a = self._fwd(e[2][0], env, lines)
+            if a.is_vec:
+                _fail("sigmoid of a vector")
+            code = self._temp(lines, f"1.0 / (1.0 + exp(-{a.code}))")
+            return _Val(code, dep=a.dep, op=("sigmoid",), kids=(a,))
+        if k == "prim" and e[1] == "dotprod":
+            a, b = self._fwd(e[2][0], env, lines), self._fwd(e[2][1], env, lines)
+            if not (a.is_vec and b.is_vec) or a.length != b.length:
+                _fail("dotprod takes two vectors of one length")
+            name = self._new()
+            lines.append(f"double {name} = 0.0;")
+            lines.append(f"for (int k_ = 0; k_ < {a.length}; ++k_) {name} += {a.elem('k_')} * {b.elem('k_')};")
+            return _Val(name, dep=a.dep or b.dep, op=("dotprod",), kids=(a, b))
+        _fail(f"unsupported expression {e}")
+
+    # ---- reverse-mode adjoints ----
+    def _bwd(self, v, adj, lines):
+        if not v.dep:
+            return
+        op = v.op[0]
+        if op == "param":
+            self._add(lines, v.op[1], v.coord, adj)
+        elif op == "bin":
+            a, b = v.kids
+            o = v.op[1]
+            if o == "+":
+                self._bwd(a, adj, lines)
+                self._bwd(b, adj, lines)
+            elif o == "-":
+                self._bwd(a, adj, lines)
+                if b.dep:
+                    self._bwd(b, self._temp(lines, f"-{adj}", "d"), lines)
+            elif o == "*":
+                if a.dep:
+                    self._bwd(a, self._temp(lines, f"{adj} * {b.code}", "d"), lines)
+                if b.dep:
+                    self._bwd(b, self._temp(lines, f"{adj} * {a.code}", "d"), lines)
+            else:  # d(a / b) = da / b - a db / b^2
+                if a.dep:
+                    self._bwd(a, self._temp(lines, f"{adj} / {b.code}", "d"), lines)
+                if b.dep:
+                    self._bwd(b, self._temp(lines, f"-{adj} * {a.code} / ({b.code} * {b.code})", "d"), lines)
+        elif op == "sigmoid":
+            self._bwd(v.kids[0], self._temp(lines, f"{adj} * {v.code} * (1.0 - {v.code})", "d"), lines)
+        elif op == "dotprod":
+            a, b = v.kids
+            for x, y in ((a, b), (b, a)):
+                if x.dep:
+                    self._vec_adjoint(x, y, adj, lines)
+        else:
+            _fail(f"no adjoint for {op}")
+
+    def _vec_adjoint(self, x, y, adj, lines):
+        """acc[q(x_k)] += adj * y_k for every k of a parameter view x."""
+        param = x.op[1]
+        c0 = x.vcoord(0)
+        if isinstance(c0, int) and x.length <= UNROLL_CAP and self._slot(param, c0) is not None:
+            s0 = self.slots[c0]
+            lines.append("#pragma unroll")
+            lines.append(f"for (int k_ = 0; k_ < {x.length}; ++k_) acc[{s0} + k_] += {adj} * {y.elem('k_')};")
+        else:
+            lines.append(f"for (int k_ = 0; k_ < {x.length}; ++k_) "
+                         f"atomicAdd(&gs[{x.vcoord('k_')}], {adj} * {y.elem('k_')});")
+
+    # ---- declarations ----
+    def _term(self, d):
+        lines = []
+        dist, args_e = d.exp[1], d.exp[2]
+        if len(args_e) != ARITY[dist]:
+            _fail(f"{d.name}: {dist} takes {ARITY[dist]} arguments")
+        sizes = self._grid_sizes(d)
+        self._grid_ranges = {g[0]: n for g, n in zip(d.grids, sizes)}
+        env = {}
+        if sizes:
+            total = int(np.prod(sizes))
+            if len(sizes) == 1:
+                env[d.grids[0][0]] = "f_"
+            else:
+                env[d.grids[0][0]] = f"(f_ / {sizes[1]})"
+                env[d.grids[1][0]] = f"(f_ % {sizes[1]})"
+        s = self.syms[d.name]
+        pt = self._sym_val(d.name, s, [env[g[0]] for g in d.grids], lines)
+        pt_code = f"(int){pt.code}" if dist in INT_DISTS else pt.code
+        args = [self._fwd(a, env, lines) for a in args_e]
+        for a in args:
+            if a.is_vec:
+                _fail(f"{d.name}: distribution arguments must be scalars")
+        fn = "gen_" + dist.lower()
+        call = ", ".join([pt_code] + [a.code for a in args])
+        lines.append(f"llacc += {fn}_ll({call});")
+        if pt.dep:
+            self._bwd(pt, self._temp(lines, f"{fn}_dotpt({call})", "d"), lines)
+        for i, a in enumerate(args):
+            if a.dep:
+                self._bwd(a, self._temp(lines, f"{fn}_dotarg{i + 1}({call})", "d"), lines)
+        body = "\n      ".join(lines)
+        if sizes:
+            total = int(np.prod(sizes))
+            self.max_range = max(self.max_range, total)
+            return (f"  // {d.kind} {d.name} ~ {dist}\n"
+                    f"  for (int f_ = gtid; f_ < {total}; f_ += gstride) {{\n      {body}\n  }}\n")
+        return f"  // {d.kind} {d.name} ~ {dist}\n  if (gtid == 0) {{\n      {body}\n  }}\n"
+
+    def _emit_kernel(self):
+        self.max_range = 1
+        terms = "".join(self._term(d) for d in self.model.decls)
+        ns = max(1, len(self.slots))
+        coord_of = [0] * ns
+        for c, s in self.slots.items():
+            coord_of[s] = c
+        self.n_slots = len(self.slots)
+        self.grid = max(1, min(MAX_GRID, -(-self.max_range // THREADS)))
+        P = self.P
+        table = ", ".join(str(c) for c in coord_of)
+        self.source = (
+            f"constexpr int kRefCompat = {1 if self.ref_compat else 0};\n" + PRELUDE +
+            f"__constant__ int kSlotCoord[{ns}] = {{{table}}};\n"
+            f'extern "C" __global__ __launch_bounds__({THREADS}) void {self.kernel_name}(\n'
+            f"    const double* __restrict__ q, double* __restrict__ part,\n"
+            f"    const double* __restrict__ R, const int* __restrict__ I) {{\n"
+            f"  constexpr int P = {P}, NS = {self.n_slots};\n"
+            f"  __shared__ double qs[P];\n"
+            f"  __shared__ double gs[P + 1];\n"
+            f"  __shared__ double red[{THREADS // 64}][NS + 1];\n"
+            f"  const int c = blockIdx.y, tid = threadIdx.x;\n"
+            f"  const int gtid = blockIdx.x * {THREADS} + tid, gstride = gridDim.x * {THREADS};\n"
+            f"  for (int j = tid; j < P; j += {THREADS}) qs[j] = q[(size_t)c * P + j];\n"
+            f"  for (int j = tid; j <= P; j += {THREADS}) gs[j] = 0.0;\n"
+            f"  __syncthreads();\n"
+            f"  double acc[NS + 1];\n"
+            f"#pragma unroll\n"
+            f"  for (int s = 0; s <= NS; ++s) acc[s] = 0.0;\n"
+            f"  double llacc = 0.0;\n"
+            f"{terms}"
+            f"  acc[NS] = llacc;\n"
+            f"#pragma unroll\n"
+            f"  for (int s = 0; s <= NS; ++s) {{\n"
+            f"    const double w = gen_wave_sum(acc[s]);\n"
+            f"    if ((tid & 63) == 0) red[tid >> 6][s] = w;\n"
+            f"  }}\n"
+            f"  __syncthreads();\n"
+            f"  for (int s = tid; s <= NS; s += {THREADS}) {{\n"
+            f"    double t = red[0][s];\n"
+            f"    for (int w = 1; w < {THREADS // 64}; ++w) t += red[w][s];\n"
+            f"    gs[s < NS ? kSlotCoord[s] : P] += t;\n"
+            f"  }}\n"
+            f"  __syncthreads();\n"
+            f"  double* out = part + ((size_t)c * gridDim.x + blockIdx.x) * (P + 1);\n"
+            f"  for (int j = tid; j <= P; j += {THREADS}) out[j] = gs[j];\n"
+            f"}}\n")

@@ -11,8 +11,12 @@ Families (each recognised up to variable names):
   GMM  -- compiler/augur/test/hmvgmm.rv (and mvgmm.rv: fixed pis and covs)
           with ConjGibbs pis, ConjGibbs | ESlice | HMC | MWG | RSlice | NUTS mu,
           ConjGibbs cov, DiscGibbs z
-The reference's compiler would instead generate code for any model
-(Compile.hs:311-345); other models raise UnsupportedModel.
+  GEN  -- any other model whose parameters are all continuous: the data pass
+          is generated from the model (pyaugur/lower.py, compiled at run
+          time with hipRTC) and sampled with HMC blocks.
+The reference's compiler generates code for any model (Compile.hs:311-345);
+models with discrete or multivariate parameters outside the three families
+raise UnsupportedModel.
 """
 import numpy as np
 
@@ -469,16 +473,143 @@ class GmmPlan:
         return pt
 
 
-def match(model, sched):
-    """Return the plan of the first family that matches, else raise."""
+def _np_eval(e, env):
+    """Value of a prior argument for initialisation (numpy, vectorised over
+    the grid variables bound in env)."""
+    k = e[0]
+    if k == "lit":
+        return e[1]
+    if k == "var":
+        return env[e[1]]
+    if k == "proj":
+        base = np.asarray(_np_eval(e[1], env))
+        idx = tuple(np.asarray(_np_eval(i, env)).astype(np.int64) for i in e[2])
+        return base[idx]
+    if k == "bin":
+        a, b = _np_eval(e[2], env), _np_eval(e[3], env)
+        return {"+": lambda: a + b, "-": lambda: a - b, "*": lambda: a * b, "/": lambda: a / b}[e[1]]()
+    if k == "prim" and e[1] == "sigmoid":
+        return 1.0 / (1.0 + np.exp(-_np_eval(e[2][0], env)))
+    if k == "prim" and e[1] == "dotprod":
+        return np.sum(np.asarray(_np_eval(e[2][0], env)) * np.asarray(_np_eval(e[2][1], env)), axis=-1)
+    _fail(f"cannot evaluate {e} for initialisation")
+
+
+class GenPlan:
+    """Any model whose parameters are all continuous: the data pass is
+    generated from the model (pyaugur.lower) and sampled with HMC blocks.
+    The reference's kernel heuristic gives each such parameter its own HMC
+    block with the defaults simLen 1.0, eps 0.05 (Compile/Compile.hs:143-165,
+    Core/KernSyn.hs:64-75); it would pick ConjGibbs where it detects
+    conjugacy, which the generic path does not analyse (HMC targets the same
+    posterior)."""
+
+    family = "gen"
+
+    def __init__(self, model, sched):
+        from .lower import DATA_DISTS, PARAM_DISTS
+        for d in model.params:
+            if not d.stochastic or d.exp[0] != "dist" or d.exp[1] not in PARAM_DISTS:
+                _fail(f"generic: parameter {d.name} must be continuous "
+                      f"({', '.join(sorted(PARAM_DISTS))})")
+        for d in model.data:
+            if d.exp[0] != "dist" or d.exp[1] not in DATA_DISTS:
+                _fail(f"generic: data {d.name} must follow one of {', '.join(sorted(DATA_DISTS))}")
+        self.model = model
+        self.param_names = [d.name for d in model.params]
+        if sched is None:
+            sched = [Kern("HMC", [n]) for n in self.param_names]
+        self.blocks = []
+        seen = set()
+        for k in sched:
+            if k.kind != "HMC":
+                _fail(f"generic: the generic path runs HMC blocks (got {k.kind})")
+            for v in k.vars:
+                if v not in self.param_names:
+                    _fail(f"generic: {v} is not a parameter")
+                if v in seen:
+                    _fail(f"generic: {v} appears in two blocks")
+                seen.add(v)
+            sim_len, eps = k.settings if k.settings else (1.0, 0.05)
+            self.blocks.append((list(k.vars), float(sim_len), float(eps)))
+        missing = [n for n in self.param_names if n not in seen]
+        if missing:
+            _fail(f"generic: no kernel updates {', '.join(missing)}")
+
+    def build(self, hypers, data, opt):
+        from .lower import Lowering, LoweringError
+        try:
+            self.lowering = Lowering(self.model, hypers, data, ref_compat=opt.ref_compat)
+        except LoweringError as e:
+            _fail(f"generic: {e}")
+        self.hypers = hypers
+        blocks = []
+        for vars_, sim_len, eps in self.blocks:
+            fixed = 0 if opt.hmc_jitter else max(1, int(round(sim_len / eps)))
+            blocks.append((vars_, sim_len, eps, fixed))
+        self.sampler = runtime.GenSampler(self.lowering, blocks, chains=opt.chains, seed=opt.seed)
+        return self.sampler
+
+    def init_point(self, strat, rng):
+        """Initial values from each parameter's prior, in declaration order
+        (augurdistlib's init* per distribution)."""
+        env = {k: (np.asarray(v) if not np.isscalar(v) else v) for k, v in self.hypers.items()}
+        pt = {}
+        for d in self.model.params:
+            shape = self.lowering.layout[d.name][1]
+            g = dict(env)
+            if shape:
+                idx = np.meshgrid(*[np.arange(s) for s in shape], indexing="ij")
+                for (var, _, _), ix in zip(d.grids, idx):
+                    g[var] = ix
+            args = [np.broadcast_to(np.asarray(_np_eval(a, g), np.float64), shape) if shape
+                    else float(_np_eval(a, g)) for a in d.exp[2]]
+            size = shape if shape else None
+            name = d.exp[1]
+            if name == "Normal":
+                v = adl.initNormal(strat, args[0], args[1], rng, size=size)
+            elif name == "Exponential":
+                v = (np.full(shape, 1.0) if isinstance(strat, adl.AugurDefault) else
+                     rng.exponential(1.0 / np.asarray(args[0]), size=size)) if shape else \
+                    adl.initExponential(strat, args[0], rng)
+            elif name == "InvGamma":
+                v = (np.full(shape, 0.5) if isinstance(strat, adl.AugurDefault) else
+                     1.0 / rng.gamma(args[0], 1.0 / np.asarray(args[1]), size=size)) if shape else \
+                    adl.initInverseGamma(strat, args[0], args[1], rng)
+            elif name == "Gamma":
+                v = adl.initGamma(strat, args[0], args[1], rng, size=size)
+            elif name == "Beta":
+                v = adl.initBeta(strat, args[0], args[1], rng, size=size)
+            else:
+                v = adl.initUniform(strat, args[0], args[1], rng, size=size)
+            pt[d.name] = np.asarray(v, np.float64) if shape else float(v)
+            env[d.name] = pt[d.name]
+        return pt
+
+    def set_point(self, pt, chain):
+        for name in self.param_names:
+            if name in pt:
+                self.sampler.set(name, np.asarray(pt[name], np.float64), chain)
+
+    def get_point(self, chain):
+        out = {}
+        for name in self.param_names:
+            v = self.sampler.get(name, chain)
+            out[name] = float(v[0]) if not self.lowering.layout[name][1] else v
+        return out
+
+
+def match(model, sched, generic=False):
+    """Return the plan of the first family that matches, else the generic
+    lowering's plan, else raise.  generic=True skips the families."""
     errors = []
-    for cls in (LdaPlan, HlrPlan, GmmPlan):
+    for cls in (() if generic else (LdaPlan, HlrPlan, GmmPlan)) + (GenPlan,):
         try:
             return cls(model, sched)
         except UnsupportedModel as e:
             errors.append(f"{cls.family}: {e}")
         except (AttributeError, IndexError, TypeError) as e:
             errors.append(f"{cls.family}: {e!r}")
-    raise UnsupportedModel("Error: the model is outside the families this runtime executes on "
-                           "the GPU (LDA, hierarchical logistic regression, Gaussian mixture): "
-                           + "; ".join(errors))
+    raise UnsupportedModel("Error: the model is outside what this runtime executes on the GPU "
+                           "(LDA, hierarchical logistic regression, Gaussian mixture, or a model "
+                           "with continuous parameters only): " + "; ".join(errors))

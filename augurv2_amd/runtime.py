@@ -287,3 +287,63 @@ class GmmSampler(Sampler):
         a = out[1:].reshape(self.chains, 2)
         with np.errstate(invalid="ignore"):
             return a[:, 0] / a[:, 1]
+
+
+class GenSampler(Sampler):
+    """A generic continuous model: the data pass generated by
+    pyaugur.lower.Lowering, compiled with hipRTC, under HMC blocks."""
+
+    family = "gen"
+
+    def __init__(self, lowering, blocks, chains=1, seed=0):
+        """blocks: list of (parameter names, sim_len, step_size, fixed_steps)."""
+        L.require_gpu()
+        lw = lowering
+        self.P = lw.P
+        self.layout = dict(lw.layout)
+        self._src = lw.source.encode()
+        self._kernel = lw.kernel_name.encode()
+        self._reals, self._ints = L.f64(lw.reals), L.i32(lw.ints)
+        names = list(lw.param_names)
+        self._names = [n.encode() for n in names]
+        name_arr = (C.c_char_p * len(names))(*self._names)
+        offs = L.i32([lw.layout[n][0] for n in names])
+        sizes = L.i32([int(np.prod(lw.layout[n][1])) if lw.layout[n][1] else 1 for n in names])
+        masks = np.zeros((max(1, len(blocks)), self.P), np.uint8)
+        sim_len = np.zeros(max(1, len(blocks)))
+        eps = np.zeros(max(1, len(blocks)))
+        fixed = np.zeros(max(1, len(blocks)), np.int32)
+        for b, (vars_, sl, ep, fx) in enumerate(blocks):
+            for v in vars_:
+                off, shape = lw.layout[v]
+                masks[b, off:off + (int(np.prod(shape)) if shape else 1)] = 1
+            sim_len[b], eps[b], fixed[b] = sl, ep, fx
+        self._keep = (name_arr, offs, sizes, masks, sim_len, eps, fixed)
+        desc = L.GenDesc(self._src, self._kernel, self.P, lw.grid, chains, seed & 0xFFFFFFFF,
+                         L.ptr(self._reals).value, self._reals.size, L.ptr(self._ints).value,
+                         self._ints.size, len(names), C.cast(name_arr, C.c_void_p).value,
+                         L.ptr(offs).value, L.ptr(sizes).value, len(blocks), L.ptr(masks).value,
+                         L.ptr(sim_len).value, L.ptr(eps).value, L.ptr(fixed).value)
+        h = C.c_void_p()
+        L.call("augur_gen_create", C.byref(desc), C.byref(h))
+        self.n_blocks = len(blocks)
+        shapes = {"state": ((self.P,), np.float64)}
+        for n in names:
+            shape = lw.layout[n][1]
+            shapes[n] = (shape if shape else (1,), np.float64)
+        super().__init__(h, chains, shapes)
+
+    def eval(self, q):
+        """(ll per chain, gradient chains x P) at the points q (chains x P)."""
+        qa = L.f64(q).reshape(self.chains, self.P)
+        ll = np.zeros(self.chains)
+        g = np.zeros((self.chains, self.P))
+        L.call("augur_gen_eval", self._h, L.ptr(qa), L.ptr(ll), L.ptr(g))
+        return ll, g
+
+    def acceptance(self):
+        """[block][chain] accepted / proposed."""
+        out = np.zeros((self.n_blocks, self.chains, 2))
+        L.call("augur_stats", self._h, L.ptr(out), out.nbytes)
+        with np.errstate(invalid="ignore"):
+            return out[..., 0] / out[..., 1]

@@ -265,6 +265,58 @@ augur_status augur_op_gmm_predictive_ll(int k, int d, const double* pis, const d
                                         const double* cov, int n, const double* y_test, double* out);
 
 /* ------------------------------------------------------------------------
+ * Generic continuous models: a generated data pass under HMC.
+ *
+ * Replaces, for models outside the three families, the reference's
+ * compile-and-load step: hs_compile (compiler/augur/src/FFI/AugurFFI.hs:35,
+ * 78-89) generating the sampler's code (CudaC/CgCuda.hs:371-516 over
+ * Core/CgLike.hs:90-267 and Core/CgGrad.hs:151-376), which pyaugur builds
+ * with gcc / nvcc and dlopens (pyaugur/pyaugur/augurlib.py:263-423).
+ *
+ * `source` is HIP source that defines
+ *   extern "C" __global__ void <kernel>(const double* q, double* part,
+ *                                        const double* R, const int* I)
+ * for a grid of (grid, chains) blocks of 256 threads.  Block (g, c) writes
+ * part[(c * grid + g) * (n_params + 1) + j]: its share of d ll / dq_j for
+ * j < n_params and of ll for j = n_params, at chain c's point
+ * q[c * n_params ..).  R and I are the `reals` and `ints` buffers.  The
+ * library compiles it with hipRTC for the device and runs HMC
+ * (cbits/augur_mcmclib.h:343-424).  Block b moves the coordinates where
+ * masks[b * n_params + j] = 1, with the reference's jitter of step size
+ * and count (fixed_steps as augur_hmc_block.fixed_steps).  Its Philox tags
+ * are 16 + 4b (momentum), + 1 (jitter) and + 2 (accept).
+ * Parameter names address slices of the vector; "state" is the whole
+ * vector.  augur_stats writes (accepted, proposed) per block and chain.
+ * ---------------------------------------------------------------------- */
+typedef struct {
+  const char* source;        /* HIP source of the data pass */
+  const char* kernel;        /* its extern "C" name */
+  int n_params;              /* P: reals per chain */
+  int grid;                  /* blocks of the pass per chain, 1..4096 */
+  int chains;                /* 1..16 */
+  uint32_t seed;
+  const double* reals;       /* R */
+  size_t n_reals;
+  const int32_t* ints;       /* I */
+  size_t n_ints;
+  int n_names;
+  const char* const* names;  /* parameter names */
+  const int* offsets;        /* first coordinate of each */
+  const int* sizes;          /* coordinates of each */
+  int n_blocks;              /* HMC blocks per sweep, 0..16 */
+  const unsigned char* masks;  /* n_blocks x n_params */
+  const double* sim_len;     /* n_blocks */
+  const double* step_size;   /* n_blocks */
+  const int* fixed_steps;    /* n_blocks */
+} augur_gen_desc;
+augur_status augur_gen_create(const augur_gen_desc* desc, augur_sampler** out);
+/* ll (chains) and d ll / dq (chains x n_params) at the points q (chains x
+ * n_params): one pass and its reduction; the sampler's state is unchanged.
+ * (The reference's generated ll and gradient functions,
+ * Core/CgLike.hs:90-136, Core/CgGrad.hs:306-342.) */
+augur_status augur_gen_eval(augur_sampler* s, const double* q, double* ll, double* grad);
+
+/* ------------------------------------------------------------------------
  * Sampler operation (generic over families).
  * ---------------------------------------------------------------------- */
 

@@ -157,7 +157,16 @@ def test_plan_matching():
         plan.match(rvparse.parse_model(HMVGMM), rvparse.parse_schedule(
             "ConjGibbs [pis] (*) ConjGibbs [cov] (*) ESlice [mu] (*) DiscGibbs [z]"))
     with pytest.raises(NameError):  # UnsupportedModel is the reference's NameError
-        plan.match(rvparse.parse_model("(a : Real) => { param p ~ Normal(0.0, a) ; }"), None)
+        plan.match(rvparse.parse_model("(a : Real) => { param p ~ Poisson(a) ; }"), None)
+    # a continuous model outside the families goes to the generic lowering,
+    # one HMC block per parameter with the defaults (1.0, 0.05)
+    p = plan.match(rvparse.parse_model("(a : Real) => { param p ~ Normal(0.0, a) ; }"), None)
+    assert p.family == "gen" and p.blocks == [(["p"], 1.0, 0.05)]
+    with pytest.raises(NameError):  # the generic path runs HMC blocks only
+        plan.match(rvparse.parse_model("(a : Real) => { param p ~ Normal(0.0, a) ; }"),
+                   rvparse.parse_schedule("ESlice [p]"))
+    p = plan.match(rvparse.parse_model(HLR), None, generic=True)
+    assert p.family == "gen" and [b[0] for b in p.blocks] == [["v"], ["b"], ["theta"]]
 
 
 def test_augur_infer_errors_without_gpu_work():
