@@ -56,6 +56,8 @@ struct GmmDev {
   double* mu64;    // C x k x d
   double* cov64;   // C x k x d x d
   float* tab;      // C x k x TS   (TS = tri(DM) + DM + 1, DM = padded dim)
+  double* prec;    // C x k x d x d  cov_k^-1 of the current cov_k (with the table)
+  double* prec0;   // d x d          cov0^-1
   int TS, DM;
   int cus;         // compute units of the device
   float* part;     // C x G x k x S
@@ -155,25 +157,175 @@ __device__ void trtrmm(const double* l1, double* l2, int n, double* tmp) {
   __syncthreads();
 }
 
-// z-step table of one component from (pis_k, mu_k, cov_k); work and tmp are
-// d x d LDS scratch of the calling 64-lane block.
-__device__ void build_table(const GmmDev& m, int c, int k, double* work, double* tmp) {
-  const int d = m.d, DM = m.DM;
-  const double* cov = m.cov64 + ((size_t)c * m.k + k) * d * d;
-  for (int i = threadIdx.x; i < d * d; i += kConjLanes) work[i] = cov[i];
-  __syncthreads();
-  chol(work, d);
-  double ldet = 0.0;
-  for (int i = 0; i < d; ++i) ldet += log(work[i * d + i]);
-  lower_trmi(work, d, tmp);
-  float* t = m.tab + ((size_t)c * m.k + k) * m.TS;
-  for (int e = threadIdx.x; e < DM * DM; e += kConjLanes) {
-    const int i = e / DM, j = e % DM;
-    if (j <= i) t[tri(i) + j] = (i < d && j < d) ? (float)work[i * d + j] : 0.0f;
+// ---- the same algebra with the matrices in registers -----------------------
+// One wave per problem: lane i < d holds row i of a d x d matrix in
+// v[0..d) (the column forms: column i).  Another lane's value is read with
+// v_readlane (the source lane is uniform), so a dependent step costs VALU
+// latency instead of an LDS round trip; each element keeps the serial
+// definition's operation order, and terms that are exact zeros are left
+// out (adding them cannot change a sum that starts at +0).
+__device__ inline double lane_bcast(double x, int src) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)b, src);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
+}
+
+// Cholesky factor in the lower triangle of the rows (above the diagonal
+// unchanged): the order of chol() above.
+__device__ inline void rchol(double (&a)[kMaxDim], int d) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int j = 0; j < kMaxDim; ++j) {
+    if (j < d) {
+      double s = 0.0;
+#pragma unroll
+      for (int k = 0; k < j; ++k) s += a[k] * a[k];
+      const double djj = lane_bcast(sqrt(a[j] - s), j);
+      double t = 0.0;
+#pragma unroll
+      for (int k = 0; k < j; ++k) t += a[k] * lane_bcast(a[k], j);
+      if (lane == j) a[j] = djj;
+      else if (lane > j) a[j] = (a[j] - t) / djj;
+    }
   }
-  const double* mu = m.mu64 + ((size_t)c * m.k + k) * d;
-  for (int i = threadIdx.x; i < DM; i += kConjLanes) t[tri(DM) + i] = i < d ? (float)mu[i] : 0.0f;
-  if (threadIdx.x == 0) t[tri(DM) + DM] = (float)(-ldet);  // -0.5 log det cov
+}
+
+// T = L^-1 of the lower triangle of L (rows): lane c gets column c of T
+// (zeros above the diagonal), the order of lower_trmi() above.
+__device__ inline void rtrmi_cols(const double (&l)[kMaxDim], double (&t)[kMaxDim], int d) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int m = 0; m < kMaxDim; ++m) {
+    t[m] = 0.0;
+    if (m < d) {
+      double b = (lane == m) ? 1.0 : 0.0;
+#pragma unroll
+      for (int i = 0; i < m; ++i) {
+        const double lmi = lane_bcast(l[i], m);
+        if (i >= lane) b -= lmi * t[i];
+      }
+      const double lmm = lane_bcast(l[m], m);
+      if (m >= lane) t[m] = b / lmm;
+    }
+  }
+}
+
+// Rows from columns (lane i gets row i).
+__device__ inline void rcols_to_rows(const double (&t)[kMaxDim], double (&r)[kMaxDim], int d) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int c = 0; c < kMaxDim; ++c) r[c] = 0.0;
+#pragma unroll
+  for (int i = 0; i < kMaxDim; ++i)
+    if (i < d)
+#pragma unroll
+      for (int c = 0; c < kMaxDim; ++c)
+        if (c < d) {
+          const double x = lane_bcast(t[i], c);
+          if (lane == i) r[c] = x;
+        }
+}
+
+// Rows of T^T T from the columns of T (lane c: column c): entry (i, r) is
+// the sum over k >= max(i, r) of T[k][i] T[k][r] (chol_invert() order).
+__device__ inline void rinv_from_cols(const double (&t)[kMaxDim], double (&a)[kMaxDim], int d) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int r = 0; r < kMaxDim; ++r) {
+    a[r] = 0.0;
+    if (r < d) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = r; k < kMaxDim; ++k)
+        if (k < d) {
+          const double tkr = lane_bcast(t[k], r);
+          if (k >= lane) acc += t[k] * tkr;
+        }
+      a[r] = acc;
+    }
+  }
+}
+
+// (L L^T)^-1 from the Cholesky factor L (rows), in rows.
+__device__ inline void rchol_invert(double (&a)[kMaxDim], int d) {
+  double t[kMaxDim];
+  rtrmi_cols(a, t, d);
+  rinv_from_cols(t, a, d);
+}
+
+// l2 = l1 l2 for lower triangular l1 (rows; trtrmm() order: the terms below
+// the diagonal in k order, then the diagonal term).
+__device__ inline void rtrtrmm(const double (&l1)[kMaxDim], double (&l2)[kMaxDim], int d) {
+  const int lane = threadIdx.x & 63;
+  double out[kMaxDim], diag = 0.0;
+#pragma unroll
+  for (int j = 0; j < kMaxDim; ++j) {
+    out[j] = 0.0;
+    if (j == lane) diag = l1[j];
+  }
+#pragma unroll
+  for (int k = 0; k < kMaxDim; ++k)
+    if (k < d)
+#pragma unroll
+      for (int j = 0; j < kMaxDim; ++j)
+        if (j < d) {
+          const double l2kj = lane_bcast(l2[j], k);
+          if (k < lane) out[j] += l1[k] * l2kj;
+        }
+#pragma unroll
+  for (int j = 0; j < kMaxDim; ++j) l2[j] = j < d ? out[j] + diag * l2[j] : 0.0;
+}
+
+// (M x)_i on lane i for M in rows and x_j on lane j.
+__device__ inline double rmatvec(const double (&mrow)[kMaxDim], double x, int d) {
+  double t = 0.0;
+#pragma unroll
+  for (int j = 0; j < kMaxDim; ++j)
+    if (j < d) t += mrow[j] * lane_bcast(x, j);
+  return t;
+}
+
+// Row i (lane i < d) of a d x d row-major matrix in global memory.
+__device__ inline void rload(const double* g, int d, double (&a)[kMaxDim]) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int j = 0; j < kMaxDim; ++j) a[j] = (lane < d && j < d) ? g[lane * d + j] : 0.0;
+}
+
+// z-step table of one component from (mu_k, cov_k) (cov in rows): L_k^-1
+// (packed lower triangle), mu_k and -sum log diag L_k, stored FP32; and
+// cov_k^-1 in FP64.  Called by the whole wave.
+__device__ __forceinline__ void build_table_r(const GmmDev& m, int c, int k, const double (&cov)[kMaxDim]) {
+  const int d = m.d, DM = m.DM, lane = threadIdx.x & 63;
+  double L[kMaxDim], t[kMaxDim], tr[kMaxDim];
+#pragma unroll
+  for (int j = 0; j < kMaxDim; ++j) L[j] = cov[j];
+  rchol(L, d);
+  double ldet = 0.0;
+#pragma unroll
+  for (int i = 0; i < kMaxDim; ++i)
+    if (i < d) ldet += log(lane_bcast(L[i], i));
+  rtrmi_cols(L, t, d);
+  rcols_to_rows(t, tr, d);
+  // cov_k^-1 in FP64 for the next mu_k | cov_k draw (chol_invert() of cov_k)
+  double prec[kMaxDim];
+  rinv_from_cols(t, prec, d);
+  if (lane < d) {
+    double* pg = m.prec + ((size_t)c * m.k + k) * d * d + lane * d;
+#pragma unroll
+    for (int j = 0; j < kMaxDim; ++j)
+      if (j < d) pg[j] = prec[j];
+  }
+  float* tab = m.tab + ((size_t)c * m.k + k) * m.TS;
+  if (lane < DM) {
+#pragma unroll
+    for (int j = 0; j < kMaxDim; ++j)
+      if (j <= lane && j < DM) tab[tri(lane) + j] = (lane < d && j < d) ? (float)tr[j] : 0.0f;
+    const double* mu = m.mu64 + ((size_t)c * m.k + k) * d;
+    tab[tri(DM) + lane] = lane < d ? (float)mu[lane] : 0.0f;
+  }
+  if (lane == 0) tab[tri(DM) + DM] = (float)(-ldet);  // -0.5 log det cov
 }
 
 // Per (chain, component), the statistics summed over the blocks of the
@@ -204,139 +356,107 @@ __global__ __launch_bounds__(64 * kSumSlices) void k_gmm_stats_sum(GmmDev m, dou
   if (sl == 0 && i < S) sum[((size_t)c * m.k + k) * S + i] = ((red[0][vi] + red[1][vi]) + red[2][vi]) + red[3][vi];
 }
 
-// One 64-lane block per (component k, chain c): the summed statistics, then
-// pi_k (Gamma), mu_k | cov_k (MvNormal conjugate) and cov_k | mu_k
-// (InvWishart) and the new z-step table.
+// One wave per (component k, chain c): the summed statistics, then pi_k
+// (Gamma), mu_k | cov_k (MvNormal conjugate) and cov_k | mu_k (InvWishart)
+// and the new z-step table.  Matrices are held in registers (lane i: row i),
+// vectors one element per lane.
 __global__ __launch_bounds__(kConjLanes) void k_gmm_conj(GmmDev m, const double* __restrict__ ssum,
                                                          uint32_t seed, uint32_t sweep) {
-  const int k = blockIdx.x, c = blockIdx.y, tid = threadIdx.x;
+  const int k = blockIdx.x, c = blockIdx.y, lane = threadIdx.x;
   const int d = m.d, S = m.S;
-  __shared__ double st[1 + kMaxDim + kMaxDim * kMaxDim];
-  __shared__ double work[6 * kMaxDim * kMaxDim + 4 * kMaxDim];
-  for (int i = tid; i < S; i += kConjLanes) st[i] = ssum[((size_t)c * m.k + k) * S + i];
-  __syncthreads();
-  // unpack: st[0] count, st[1..d] sum (y - centre), packed lower S after
+  const size_t ck = (size_t)c * m.k + k;
+  const double* st = ssum + ck * S;  // count, sum (y - centre), packed lower scatter
   const int cnt = (int)llrint(st[0]);
-  double* sc = work;                     // d
-  double* Sc = sc + kMaxDim;             // d x d
-  double* L0 = Sc + kMaxDim * kMaxDim;   // d x d
-  double* L = L0 + kMaxDim * kMaxDim;    // d x d
-  double* A = L + kMaxDim * kMaxDim;     // d x d
-  double* v = A + kMaxDim * kMaxDim;     // d
-  double* mean = v + kMaxDim;            // d
-  double* munew = mean + kMaxDim;        // d
-  double* B = munew + kMaxDim;           // d x d
-  double* tmp = B + kMaxDim * kMaxDim;   // d x d
-  for (int i = tid; i < d; i += kConjLanes) sc[i] = st[1 + i];
-  for (int e = tid; e < d * d; e += kConjLanes) {
-    const int i = e / d, j = e % d;
-    Sc[e] = j <= i ? st[1 + d + tri(i) + j] : st[1 + d + tri(j) + i];
-  }
-  double* mu = m.mu64 + ((size_t)c * m.k + k) * d;  // current mu = centre
-  double* cov = m.cov64 + ((size_t)c * m.k + k) * d * d;
-  if (m.sample_pi && tid == 0) {
+  const bool on = lane < d;
+  const double sc = on ? st[1 + lane] : 0.0;
+  double Sc[kMaxDim];
+#pragma unroll
+  for (int j = 0; j < kMaxDim; ++j)
+    Sc[j] = (on && j < d) ? (j <= lane ? st[1 + d + tri(lane) + j] : st[1 + d + tri(j) + lane]) : 0.0;
+  double* mu = m.mu64 + ck * d;  // current mu = centre
+  double* cov = m.cov64 + ck * d * d;
+  if (m.sample_pi && lane == 0) {
     const Stream sp{seed, (uint32_t)c, sweep, TAG_GMM_PI};
-    m.gpi[(size_t)c * m.k + k] = gamma_at(sp, (uint32_t)k, m.alpha[k] + (double)cnt);
+    m.gpi[ck] = gamma_at(sp, (uint32_t)k, m.alpha[k] + (double)cnt);
   }
   // the statistics' centre: the FP32 table copy of the mu they were taken at
   // (saved by k_gmm_mu_prep when another kernel has moved mu since)
-  double* cen = mean;
   const bool mu_moved = m.mu_mode != GMM_MU_CONJ;
-  for (int i = tid; i < d; i += kConjLanes)
-    cen[i] = mu_moved ? m.mcent[((size_t)c * m.k + k) * d + i] : (double)(float)mu[i];
-  __syncthreads();
-  if (mu_moved) {
-    for (int i = tid; i < d; i += kConjLanes) munew[i] = mu[i];
-    __syncthreads();
-  } else {
-  // ---- mu_k | cov_k: cbits/augur_dist.c:633-669 ----
-  for (int e = tid; e < d * d; e += kConjLanes) {
-    L0[e] = m.cov0[e];
-    L[e] = cov[e];
-  }
-  __syncthreads();
-  chol(L0, d);
-  chol_invert(L0, d, tmp);  // cov0^-1
-  chol(L, d);
-  chol_invert(L, d, tmp);
-  for (int e = tid; e < d * d; e += kConjLanes) L[e] *= (double)cnt;  // n cov^-1
-  // the z step centred its statistics at the FP32 table copy of mu: the mean
-  // of the assigned points is centre + s / n (the sum itself when n = 0)
-  const double inv = cnt == 0 ? 1.0 : 1.0 / (double)cnt;
-  for (int i = tid; i < d; i += kConjLanes) {
-    double t = 0.0;
-    for (int j = 0; j < d; ++j) t += L0[i * d + j] * m.mu0[j];
-    v[i] = t;
-    mean[i] = ((double)cnt * cen[i] + sc[i]) * inv;
-  }
-  __syncthreads();
-  for (int i = tid; i < d; i += kConjLanes) {
-    double t = 0.0;
-    for (int j = 0; j < d; ++j) t += L[i * d + j] * mean[j];
-    v[i] += t;
-  }
-  __syncthreads();
-  for (int e = tid; e < d * d; e += kConjLanes) L0[e] += L[e];
-  __syncthreads();
-  chol(L0, d);
-  chol_invert(L0, d, tmp);  // A = (cov0^-1 + n cov^-1)^-1
-  {
+  const double cen = on ? (mu_moved ? m.mcent[ck * d + lane] : (double)(float)mu[lane]) : 0.0;
+  double munew = on ? mu[lane] : 0.0;
+  if (!mu_moved) {
+    // ---- mu_k | cov_k: cbits/augur_dist.c:633-669 ----
+    // cov0^-1 (k_gmm_prec0) and cov_k^-1 (stored with the table of cov_k):
+    // the same operations as a Cholesky factor and its chol_invert here
+    double Q[kMaxDim], P[kMaxDim];
+    rload(m.prec0, d, Q);
+    rload(m.prec + ck * d * d, d, P);
+#pragma unroll
+    for (int j = 0; j < kMaxDim; ++j) P[j] *= (double)cnt;  // n cov^-1
+    // the z step centred its statistics at the FP32 table copy of mu: the
+    // mean of the assigned points is centre + s / n (the sum itself when n = 0)
+    const double inv = cnt == 0 ? 1.0 : 1.0 / (double)cnt;
+    const double mu0 = on ? m.mu0[lane] : 0.0;
+    double v = rmatvec(Q, mu0, d);
+    const double mean = ((double)cnt * cen + sc) * inv;
+    v += rmatvec(P, mean, d);
+#pragma unroll
+    for (int j = 0; j < kMaxDim; ++j) Q[j] += P[j];
+    rchol(Q, d);
+    rchol_invert(Q, d);  // A = (cov0^-1 + n cov^-1)^-1
     const Stream sm{seed, (uint32_t)c, sweep, TAG_GMM_MU};
-    for (int i = tid; i < d; i += kConjLanes) {
-      double t = 0.0;
-      for (int j = 0; j < d; ++j) t += L0[i * d + j] * v[j];
-      mean[i] = t;
-      munew[i] = normal_at(sm, (uint32_t)(k * d + i), 0);
-    }
-    for (int e = tid; e < d * d; e += kConjLanes) L[e] = L0[e];
-    __syncthreads();
-    chol(L, d);
-    for (int i = tid; i < d; i += kConjLanes) {  // trvm
-      double t = 0.0;
-      for (int j = 0; j <= i; ++j) t += L[i * d + j] * munew[j];
-      tmp[i] = t + mean[i];
-    }
-    __syncthreads();
-    for (int i = tid; i < d; i += kConjLanes) munew[i] = tmp[i];
-    __syncthreads();
+    const double pmean = rmatvec(Q, v, d);
+    const double z = on ? normal_at(sm, (uint32_t)(k * d + lane), 0) : 0.0;
+    rchol(Q, d);
+    double t = 0.0;  // trvm: (L z)_i over j <= i
+#pragma unroll
+    for (int j = 0; j < kMaxDim; ++j)
+      if (j < d) {
+        const double zj = lane_bcast(z, j);
+        if (j <= lane) t += Q[j] * zj;
+      }
+    munew = t + pmean;
   }
-  // the centre again (mean held the conjugate posterior mean meanwhile)
-  for (int i = tid; i < d; i += kConjLanes)
-    cen[i] = mu_moved ? m.mcent[((size_t)c * m.k + k) * d + i] : (double)(float)mu[i];
-  __syncthreads();
-  }
-  // ---- cov_k | mu_k: IW(df + n, scale0 + sum (y - mu)(y - mu)^T) ----
+  double cv[kMaxDim];
   if (m.sample_cov) {
-    double* dl = v;  // d = centre - mu_new
-    for (int i = tid; i < d; i += kConjLanes) dl[i] = cen[i] - munew[i];
-    __syncthreads();
-    for (int e = tid; e < d * d; e += kConjLanes) {
-      const int i = e / d, j = e % d;
-      A[e] = Sc[e] + sc[i] * dl[j] + dl[i] * sc[j] + (double)cnt * dl[i] * dl[j] + m.scale0[e];
+    // ---- cov_k | mu_k: IW(df + n, scale0 + sum (y - mu)(y - mu)^T) ----
+    const double dl = cen - munew;  // centre - mu_new
+    double A[kMaxDim];
+#pragma unroll
+    for (int j = 0; j < kMaxDim; ++j) {
+      const double scj = lane_bcast(sc, j), dlj = lane_bcast(dl, j);
+      A[j] = (on && j < d) ? Sc[j] + sc * dlj + dl * scj + (double)cnt * dl * dlj + m.scale0[lane * d + j] : 0.0;
     }
-    __syncthreads();
     // augur_invwishart_sample (cbits/augur_dist.c:740-749)
-    chol(A, d);
-    chol_invert(A, d, tmp);
-    chol(A, d);
+    rchol(A, d);
+    rchol_invert(A, d);
+    rchol(A, d);
     const Stream sc_s{seed, (uint32_t)c, sweep, TAG_GMM_COV};
     const int dfn = m.df + cnt;
-    for (int e = tid; e < d * d; e += kConjLanes) {
-      const int i = e / d, j = e % d;
-      const uint32_t site = (uint32_t)(k * d * d + e);
-      if (j < i) B[e] = normal_at(sc_s, site, 0);
-      else if (j > i) B[e] = 0.0;
-      else B[e] = sqrt(2.0 * gamma_at(sc_s, site, 0.5 * (double)(dfn - i)));
+    // Bartlett factor: normals below the diagonal, sqrt(2 Gamma) on it
+    // (drawn in a rolled loop through LDS, then read into registers)
+    __shared__ double bart[kMaxDim * (kMaxDim + 1)];
+    if (on) {
+      const uint32_t row = (uint32_t)(k * d * d + lane * d);
+      for (int j = 0; j < lane; ++j) bart[lane * (kMaxDim + 1) + j] = normal_at(sc_s, row + j, 0);
+      bart[lane * (kMaxDim + 1) + lane] = sqrt(2.0 * gamma_at(sc_s, row + lane, 0.5 * (double)(dfn - lane)));
     }
     __syncthreads();
-    trtrmm(A, B, d, tmp);
-    chol_invert(B, d, tmp);
-    for (int e = tid; e < d * d; e += kConjLanes) cov[e] = B[e];
+    double B[kMaxDim];
+#pragma unroll
+    for (int j = 0; j < kMaxDim; ++j) B[j] = (on && j <= lane && j < d) ? bart[lane * (kMaxDim + 1) + j] : 0.0;
+    rtrtrmm(A, B, d);
+    rchol_invert(B, d);
+#pragma unroll
+    for (int j = 0; j < kMaxDim; ++j) {
+      if (on && j < d) cov[lane * d + j] = B[j];
+      cv[j] = B[j];
+    }
+  } else {
+    rload(cov, d, cv);
   }
-  __syncthreads();
-  for (int i = tid; i < d; i += kConjLanes) mu[i] = munew[i];
-  __syncthreads();
-  build_table(m, c, k, L, tmp);
+  if (on) mu[lane] = munew;
+  build_table_r(m, c, k, cv);
 }
 
 // ---- mu moves other than ConjGibbs --------------------------------------
@@ -353,14 +473,12 @@ __global__ __launch_bounds__(kConjLanes) void k_gmm_conj(GmmDev m, const double*
 // Per (component, chain): P_k, P_k s_k, n_k and c_k for k_gmm_mu_move.
 __global__ __launch_bounds__(kConjLanes) void k_gmm_mu_prep(GmmDev m, const double* __restrict__ ssum) {
   const int k = blockIdx.x, c = blockIdx.y, tid = threadIdx.x, d = m.d, S = m.S;
-  __shared__ double L[kMaxDim * kMaxDim], tmp[kMaxDim * kMaxDim], sc[kMaxDim];
+  __shared__ double L[kMaxDim * kMaxDim], sc[kMaxDim];
   const size_t ck = (size_t)c * m.k + k;
-  const double* cov = m.cov64 + ck * d * d;
-  for (int e = tid; e < d * d; e += kConjLanes) L[e] = cov[e];
+  // cov_k^-1 as stored with the table of cov_k
+  for (int e = tid; e < d * d; e += kConjLanes) L[e] = m.prec[ck * d * d + e];
   for (int i = tid; i < d; i += kConjLanes) sc[i] = ssum[ck * S + 1 + i];
   __syncthreads();
-  chol(L, d);
-  chol_invert(L, d, tmp);
   for (int e = tid; e < d * d; e += kConjLanes) m.mprec[ck * d * d + e] = L[e];
   for (int i = tid; i < d; i += kConjLanes) {
     double t = 0.0;
@@ -763,10 +881,22 @@ __global__ __launch_bounds__(256) void k_gmm_predll(GmmDev m, const double* __re
 }
 
 // Tables from the current parameters (after set_pt).
+__global__ __launch_bounds__(kConjLanes) void k_gmm_prec0(GmmDev m) {
+  double a[kMaxDim];
+  rload(m.cov0, m.d, a);
+  rchol(a, m.d);
+  rchol_invert(a, m.d);
+  const int lane = threadIdx.x;
+#pragma unroll
+  for (int j = 0; j < kMaxDim; ++j)
+    if (lane < m.d && j < m.d) m.prec0[lane * m.d + j] = a[j];
+}
+
 __global__ __launch_bounds__(kConjLanes) void k_gmm_tables(GmmDev m) {
   const int k = blockIdx.x, c = blockIdx.y;
-  __shared__ double work[kMaxDim * kMaxDim], tmp[kMaxDim * kMaxDim];
-  build_table(m, c, k, work, tmp);
+  double cv[kMaxDim];
+  rload(m.cov64 + ((size_t)c * m.k + k) * m.d * m.d, m.d, cv);
+  build_table_r(m, c, k, cv);
 }
 
 // DM: padded dimension, KM: maximum number of components held in registers.
@@ -1345,7 +1475,7 @@ static StatsKernel pick_stats(int DM, int KM) {
 struct GmmSampler final : augur_sampler {
   GmmDev m{};
   DevBuf<float> y, tab, part;
-  DevBuf<double> alpha, mu0, cov0, scale0, gpi, pis64, mu64, cov64, ll, ssum;
+  DevBuf<double> alpha, mu0, cov0, scale0, gpi, pis64, mu64, cov64, ll, ssum, prec, prec0;
   DevBuf<double> cov0_inv, chol0, cholp, mcent, mprec, mlin, mcnt, macc, nuts;
   DevBuf<int32_t> z;
   std::vector<double> h_alpha, h_mu0, h_cov0, h_scale0;
@@ -1363,7 +1493,11 @@ struct GmmSampler final : augur_sampler {
     return v;
   }
 
+  // z-step tables and FP64 precisions of the current covariances (and
+  // cov0^-1, fixed, recomputed with them)
   void refresh_tables() {
+    k_gmm_prec0<<<1, kConjLanes, 0, stream>>>(m);
+    AUGUR_CHECK_LAUNCH();
     k_gmm_tables<<<dim3(m.k, chains), kConjLanes, 0, stream>>>(m);
     AUGUR_CHECK_LAUNCH();
   }
@@ -1534,6 +1668,8 @@ static void gmm_setup(GmmSampler* s, int n, int d, int k, int chains) {
   s->gpi.alloc((size_t)chains * k); s->pis64.alloc((size_t)chains * k);
   s->mu64.alloc((size_t)chains * k * d); s->cov64.alloc((size_t)chains * k * d * d);
   s->tab.alloc((size_t)chains * k * m.TS);
+  s->prec.alloc((size_t)chains * k * d * d);
+  s->prec0.alloc((size_t)d * d);
   s->part.alloc((size_t)chains * m.G * k * m.S);
   s->ssum.alloc((size_t)chains * k * m.S);
   s->z.alloc((size_t)chains * n);
@@ -1541,6 +1677,7 @@ static void gmm_setup(GmmSampler* s, int n, int d, int k, int chains) {
   m.y = s->y.p; m.alpha = s->alpha.p; m.mu0 = s->mu0.p; m.cov0 = s->cov0.p; m.scale0 = s->scale0.p;
   m.gpi = s->gpi.p; m.pis64 = s->pis64.p; m.mu64 = s->mu64.p; m.cov64 = s->cov64.p;
   m.tab = s->tab.p; m.part = s->part.p; m.z = s->z.p;
+  m.prec = s->prec.p; m.prec0 = s->prec0.p;
 }
 
 }  // namespace augur
@@ -1722,6 +1859,7 @@ augur_status augur_op_gmm_conj(int d, int k, const double* alpha, const double* 
     s.part.upload(st.data(), st.size(), s.stream);
     s.seed = seed;
     s.sweep = sweep;
+    s.refresh_tables();  // cov0^-1 and the precisions of the given covariances
     s.launch_conj();
     AUGUR_CHECK_LAUNCH();
     std::vector<double> g(k);
