@@ -77,7 +77,8 @@ class GenDesc(C.Structure):
                 ("n_ints", C.c_size_t), ("n_names", C.c_int), ("names", C.c_void_p),
                 ("offsets", C.c_void_p), ("sizes", C.c_void_p), ("n_blocks", C.c_int),
                 ("masks", C.c_void_p), ("sim_len", C.c_void_p), ("step_size", C.c_void_p),
-                ("fixed_steps", C.c_void_p)]
+                ("fixed_steps", C.c_void_p), ("block_kernels", C.c_void_p),
+                ("block_grids", C.c_void_p)]
 
 
 GMM_MU_CONJ, GMM_MU_ESLICE, GMM_MU_HMC, GMM_MU_MWG, GMM_MU_RSLICE, GMM_MU_NUTS = 0, 1, 2, 3, 4, 5

@@ -20,9 +20,11 @@
 //                 the leapfrog update of every coordinate that the block
 //                 moves.
 //   k_gen_end     kinetic energies, FP64 Metropolis test, accept / reject.
-// The log density and gradient at the current point are cached: they are
-// what the pass computed when that point was proposed.  An iteration of L
-// leapfrog steps therefore costs L passes.
+// Each block runs the pass of its own full conditional when the source
+// provides one (else the log joint's).  The log density and gradient at the
+// current point are cached: they are what the pass computed when that point
+// was proposed.  An iteration of L leapfrog steps therefore costs L passes,
+// plus one at the block's start when the previous block used another pass.
 #include <hip/hiprtc.h>
 
 #include <algorithm>
@@ -209,18 +211,28 @@ static std::vector<char> rtc_compile(const std::string& src, const std::string& 
 struct GenBlock {
   double sim_len, eps;
   int fixed_steps;
+  int kernel;  // index into GenSampler::passes
+};
+
+// One compiled pass: the log joint (passes[0]) or a block's full conditional.
+struct GenPass {
+  hipFunction_t fn = nullptr;
+  int grid = 1;
 };
 
 struct GenSampler final : augur_sampler {
-  int P = 0, G = 1;
+  int P = 0;
   hipModule_t mod = nullptr;
-  hipFunction_t fpass = nullptr;
+  std::vector<GenPass> passes;
   DevBuf<double> q, prop, p0, p, k0, gcur, llcur, gprop, llprop, part, acc, R;
   DevBuf<int32_t> I;
   DevBuf<unsigned char> masks;
   std::vector<GenBlock> blocks;
   std::map<std::string, std::pair<int, int>> names;  // name -> (offset, size)
-  bool cache_valid = false;
+  // the pass whose ll and gradient at q are in llcur / gcur (-1: none);
+  // a block's HMC keeps its own pass's cache valid, since an accepted
+  // proposal brings the values computed for it
+  int cache_owner = -1;
 
   ~GenSampler() override {
     if (mod) (void)hipModuleUnload(mod);
@@ -235,12 +247,12 @@ struct GenSampler final : augur_sampler {
     return ParamView{q.p + it->second.first, sizeof(double), (size_t)it->second.second, (size_t)P, true};
   }
 
-  void on_set(const std::string&) override { cache_valid = false; }
+  void on_set(const std::string&) override { cache_owner = -1; }
 
-  GenArgs args(int mode) {
+  GenArgs args(int mode, int k) {
     GenArgs A{};
     A.P = P;
-    A.G = G;
+    A.G = passes[k].grid;
     A.mode = mode;
     A.part = part.p;
     A.seed = seed;
@@ -250,14 +262,15 @@ struct GenSampler final : augur_sampler {
     return A;
   }
 
-  void pass(const double* at) {
+  void pass(int k, const double* at) {
     const double* qa = at;
     double* pa = part.p;
     const double* ra = R.p;
     const int32_t* ia = I.p;
     void* kargs[] = {(void*)&qa, (void*)&pa, (void*)&ra, (void*)&ia};
     prof_begin();
-    AUGUR_HIP(hipModuleLaunchKernel(fpass, G, chains, 1, kGenThreads, 1, 1, 0, stream, kargs, nullptr));
+    AUGUR_HIP(hipModuleLaunchKernel(passes[k].fn, passes[k].grid, chains, 1, kGenThreads, 1, 1, 0, stream,
+                                    kargs, nullptr));
     prof_end();
   }
 
@@ -266,17 +279,17 @@ struct GenSampler final : augur_sampler {
     AUGUR_CHECK_LAUNCH();
   }
 
-  void ensure_cache() {
-    if (cache_valid) return;
-    pass(q.p);
-    finish(args(GEN_CACHE));
-    cache_valid = true;
+  void ensure_cache(int k) {
+    if (cache_owner == k) return;
+    pass(k, q.p);
+    finish(args(GEN_CACHE, k));
+    cache_owner = k;
   }
 
   void hmc(int b) {
-    ensure_cache();
     const GenBlock& B = blocks[b];
-    GenArgs A = args(GEN_LEAPFROG);
+    ensure_cache(B.kernel);
+    GenArgs A = args(GEN_LEAPFROG, B.kernel);
     A.tag = 16u + 4u * (uint32_t)b;
     A.mask = masks.p + (size_t)b * P;
     A.stats = acc.p + (size_t)b * 2 * chains;
@@ -289,7 +302,7 @@ struct GenSampler final : augur_sampler {
     k_gen_begin<<<chains, kGenThreads, 0, stream>>>(A);
     AUGUR_CHECK_LAUNCH();
     for (int l = 1; l <= lmax; ++l) {
-      pass(prop.p);
+      pass(B.kernel, prop.p);
       A.l = l;
       finish(A);
     }
@@ -306,9 +319,15 @@ struct GenSampler final : augur_sampler {
     AUGUR_HIP(hipStreamSynchronize(stream));
   }
 
+  // the log joint at the current point
   void log_density(double* out) override {
-    ensure_cache();
-    llcur.download(out, chains, stream);
+    if (cache_owner == 0) {
+      llcur.download(out, chains, stream);
+    } else {
+      pass(0, q.p);
+      finish(args(GEN_EVAL, 0));
+      llprop.download(out, chains, stream);
+    }
     AUGUR_HIP(hipStreamSynchronize(stream));
   }
 
@@ -320,12 +339,12 @@ struct GenSampler final : augur_sampler {
     AUGUR_HIP(hipStreamSynchronize(stream));
   }
 
-  // ll and gradient at caller-given points (the pass and its reduction,
-  // without touching the sampler's state)
+  // log joint and its gradient at caller-given points (the pass and its
+  // reduction, without touching the sampler's state)
   void eval(const double* hq, double* hll, double* hgrad) {
     prop.upload(hq, (size_t)chains * P, stream);
-    pass(prop.p);
-    finish(args(GEN_EVAL));
+    pass(0, prop.p);
+    finish(args(GEN_EVAL, 0));
     llprop.download(hll, chains, stream);
     gprop.download(hgrad, (size_t)chains * P, stream);
     AUGUR_HIP(hipStreamSynchronize(stream));
@@ -345,6 +364,12 @@ static void gen_validate(const augur_gen_desc* D) {
   for (int b = 0; b < D->n_blocks; ++b)
     if (!(D->step_size[b] > 0.0) || !(D->sim_len[b] >= 0.0))
       fail(AUGUR_ERR_ARG, "block %d: step size must be positive and sim_len non-negative", b);
+  if (D->block_kernels) {
+    if (!D->block_grids) fail(AUGUR_ERR_ARG, "block kernels need their grids");
+    for (int b = 0; b < D->n_blocks; ++b)
+      if (!D->block_kernels[b] || D->block_grids[b] < 1 || D->block_grids[b] > kGenMaxGrid)
+        fail(AUGUR_ERR_ARG, "block %d: kernel name and a grid in [1, %d] are required", b, kGenMaxGrid);
+  }
   if (D->n_names < 0 || (D->n_names && (!D->names || !D->offsets || !D->sizes)))
     fail(AUGUR_ERR_ARG, "parameter names need offsets and sizes");
   for (int i = 0; i < D->n_names; ++i)
@@ -375,18 +400,33 @@ augur_status augur_gen_create(const augur_gen_desc* D, augur_sampler** out) {
       const std::vector<char> code = rtc_compile(D->source, arch);
       AUGUR_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
       AUGUR_HIP(hipModuleLoadData(&s->mod, code.data()));
-      AUGUR_HIP(hipModuleGetFunction(&s->fpass, s->mod, D->kernel));
+      GenPass joint;
+      AUGUR_HIP(hipModuleGetFunction(&joint.fn, s->mod, D->kernel));
+      joint.grid = D->grid;
+      s->passes.push_back(joint);
+      int max_grid = D->grid;
+      for (int b = 0; b < D->n_blocks; ++b) {
+        int k = 0;
+        if (D->block_kernels) {
+          GenPass bp;
+          AUGUR_HIP(hipModuleGetFunction(&bp.fn, s->mod, D->block_kernels[b]));
+          bp.grid = D->block_grids[b];
+          max_grid = std::max(max_grid, bp.grid);
+          s->passes.push_back(bp);
+          k = (int)s->passes.size() - 1;
+        }
+        s->blocks.push_back({D->sim_len[b], D->step_size[b], D->fixed_steps[b], k});
+      }
       s->seed = D->seed;
       s->chains = D->chains;
       s->P = D->n_params;
-      s->G = D->grid;
       const size_t C = (size_t)D->chains, P = (size_t)D->n_params;
       for (auto* b : {&s->q, &s->prop, &s->p0, &s->p, &s->gcur, &s->gprop}) {
         b->alloc(C * P);
         b->zero(s->stream);
       }
       for (auto* b : {&s->k0, &s->llcur, &s->llprop}) b->alloc(C);
-      s->part.alloc(C * (size_t)D->grid * (P + 1));
+      s->part.alloc(C * (size_t)max_grid * (P + 1));
       s->acc.alloc(std::max<size_t>(1, 2 * C * (size_t)D->n_blocks));
       s->acc.zero(s->stream);
       s->R.alloc(std::max<size_t>(1, D->n_reals));
@@ -395,8 +435,6 @@ augur_status augur_gen_create(const augur_gen_desc* D, augur_sampler** out) {
       s->I.upload(D->ints, D->n_ints, s->stream);
       s->masks.alloc(std::max<size_t>(1, (size_t)D->n_blocks * P));
       if (D->n_blocks) s->masks.upload(D->masks, (size_t)D->n_blocks * P, s->stream);
-      for (int b = 0; b < D->n_blocks; ++b)
-        s->blocks.push_back({D->sim_len[b], D->step_size[b], D->fixed_steps[b]});
       for (int i = 0; i < D->n_names; ++i) s->names[D->names[i]] = {D->offsets[i], D->sizes[i]};
       AUGUR_HIP(hipStreamSynchronize(s->stream));
     } catch (...) {

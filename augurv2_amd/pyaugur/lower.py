@@ -9,7 +9,13 @@ becomes one HIP kernel source.  It is compiled at run time with hipRTC and
 driven by the library's device-resident HMC (augur_gen_create).
 
 The generated kernel ``augur_gen_pass`` runs on a grid of (G, chains)
-blocks of 256 threads.  Each block does the following.
+blocks of 256 threads, G a multiple of 8 with about four points of the
+longest grid per thread.  The block at dispatch position L takes row tile
+(L / 8 / chains) * 8 + L % 8 of chain (L / 8) % chains: dispatch goes
+round-robin over the 8 XCDs, so the chains of one tile run on one XCD and
+its L2 serves their common data reads (measured on hlr.rv at N = 100k,
+D = 64, 4 chains: 69 us per pass against 248 us with one row per thread
+and blocks in chain-major order).  Each block does the following.
   - It loads its chain's parameter vector q (P reals) into LDS.
   - It walks every declaration ``x[i..] ~ Dist(args) for i <- 0 until n``
     as a grid-stride loop over the flattened grid.  A declaration without a
@@ -21,6 +27,10 @@ blocks of 256 threads.  Each block does the following.
     alpha[g[n]]) is an LDS atomic add.
   - It sums the register accumulators over the block in a fixed order.
   - It writes one row of P + 1 partials (the gradient, then ll).
+Besides the log joint (``augur_gen_pass``, for the log density and the
+pass at given points), each HMC block b gets ``augur_gen_pass_b<b>`` over
+only the declarations that mention its variables: its full conditional, so
+that a block whose variables touch no data never reads the data.
 The library sums the G rows of each chain in a fixed order.  The
 result is deterministic whenever no coordinate receives two LDS atomic adds
 from one block.
@@ -53,7 +63,8 @@ INT_DISTS = {"Bernoulli", "Poisson"}
 ARITY = {"Normal": 2, "Exponential": 1, "Gamma": 2, "InvGamma": 2, "Beta": 2, "Uniform": 2,
          "Bernoulli": 1, "Poisson": 1}
 THREADS = 256
-MAX_GRID = 512
+MAX_GRID = 512         # a multiple of 8
+ROWS_PER_THREAD = 4
 MAX_PARAMS = 3000      # q and the gradient row are held in LDS
 MAX_REG_SLOTS = 96     # FP64 register accumulators per thread
 UNROLL_CAP = 128       # longest dotprod whose adjoint loop is unrolled into registers
@@ -65,6 +76,24 @@ class LoweringError(ValueError):
 
 def _fail(msg):
     raise LoweringError(msg)
+
+
+def _free_vars(e, out):
+    """Names an expression mentions."""
+    k = e[0]
+    if k == "var":
+        out.add(e[1])
+    elif k == "proj":
+        _free_vars(e[1], out)
+        for i in e[2]:
+            _free_vars(i, out)
+    elif k in ("prim", "dist"):
+        for a in e[2]:
+            _free_vars(a, out)
+    elif k == "bin":
+        _free_vars(e[2], out)
+        _free_vars(e[3], out)
+    return out
 
 
 def _lit(v):
@@ -207,7 +236,7 @@ class Lowering:
 
     kernel_name = "augur_gen_pass"
 
-    def __init__(self, model, hypers, data, ref_compat=True):
+    def __init__(self, model, hypers, data, ref_compat=True, blocks=None):
         self.model = model
         self.ref_compat = bool(ref_compat)
         self.syms = {}
@@ -218,10 +247,25 @@ class Lowering:
         self._bind_hypers(hypers)
         self._bind_params()
         self._bind_data(data)
-        self.slots = {}            # coordinate -> register slot
-        self.param_slots = {}      # param -> first slot (params held whole in registers)
         self._tmp = 0
-        self._emit_kernel()
+        # the log joint, then one pass per HMC block over the declarations
+        # whose terms mention the block's variables: the block's full
+        # conditional (Core/RwCore.hs:58-67 fullCond, one per kernel at
+        # Compile/Compile.hs:182-184).  A factor shared by two variables of
+        # one block is counted once (the reference's fullConds' multiplies
+        # the variables' full conditionals, which counts it twice).
+        decls = [d for d in self.model.decls if d.stochastic]
+        kernels = [self._emit_kernel(self.kernel_name, decls)]
+        self.grid, self.n_slots = self._last_grid, self._last_slots
+        self.block_kernels = []
+        for b, vars_ in enumerate(blocks or []):
+            vs = set(vars_)
+            sub = [d for d in decls if d.name in vs or (_free_vars(d.exp, set()) & vs)]
+            name = f"{self.kernel_name}_b{b}"
+            kernels.append(self._emit_kernel(name, sub))
+            self.block_kernels.append((name, self._last_grid))
+        self.source = (f"constexpr int kRefCompat = {1 if self.ref_compat else 0};\n" + PRELUDE
+                       + "".join(kernels))
 
     # ---- binding ----
     def _pack(self, arr, is_int):
@@ -542,6 +586,7 @@ class Lowering:
                 _fail("dotprod takes two vectors of one length")
             name = self._new()
             lines.append(f"double {name} = 0.0;")
+            lines.append("#pragma unroll 16")
             lines.append(f"for (int k_ = 0; k_ < {a.length}; ++k_) {name} += {a.elem('k_')} * {b.elem('k_')};")
             return _Val(name, dep=a.dep or b.dep, op=("dotprod",), kids=(a, b))
         _fail(f"unsupported expression {e}")
@@ -634,29 +679,44 @@ class Lowering:
                     f"  for (int f_ = gtid; f_ < {total}; f_ += gstride) {{\n      {body}\n  }}\n")
         return f"  // {d.kind} {d.name} ~ {dist}\n  if (gtid == 0) {{\n      {body}\n  }}\n"
 
-    def _emit_kernel(self):
+    def _emit_kernel(self, name, decls):
+        """Source of one pass kernel over the terms of `decls`; sets
+        _last_grid and _last_slots."""
+        self.slots = {}            # coordinate -> register slot
+        self.param_slots = {}      # param -> first slot (params held whole in registers)
         self.max_range = 1
-        terms = "".join(self._term(d) for d in self.model.decls)
+        terms = "".join(self._term(d) for d in decls)
         ns = max(1, len(self.slots))
         coord_of = [0] * ns
         for c, s in self.slots.items():
             coord_of[s] = c
-        self.n_slots = len(self.slots)
-        self.grid = max(1, min(MAX_GRID, -(-self.max_range // THREADS)))
+        n_slots = len(self.slots)
+        # about ROWS_PER_THREAD points of the longest grid per thread, in a
+        # multiple of the 8 XCDs (the block mapping below)
+        g = -(-self.max_range // (THREADS * ROWS_PER_THREAD))
+        self._last_grid = max(8, min(MAX_GRID, -(-g // 8) * 8))
+        self._last_slots = n_slots
         P = self.P
         table = ", ".join(str(c) for c in coord_of)
-        self.source = (
-            f"constexpr int kRefCompat = {1 if self.ref_compat else 0};\n" + PRELUDE +
-            f"__constant__ int kSlotCoord[{ns}] = {{{table}}};\n"
-            f'extern "C" __global__ __launch_bounds__({THREADS}) void {self.kernel_name}(\n'
+        return (
+            f"__constant__ int kSlotCoord_{name}[{ns}] = {{{table}}};\n"
+            f'extern "C" __global__ __launch_bounds__({THREADS}) void {name}(\n'
             f"    const double* __restrict__ q, double* __restrict__ part,\n"
             f"    const double* __restrict__ R, const int* __restrict__ I) {{\n"
-            f"  constexpr int P = {P}, NS = {self.n_slots};\n"
+            f"  constexpr int P = {P}, NS = {n_slots};\n"
             f"  __shared__ double qs[P];\n"
             f"  __shared__ double gs[P + 1];\n"
             f"  __shared__ double red[{THREADS // 64}][NS + 1];\n"
-            f"  const int c = blockIdx.y, tid = threadIdx.x;\n"
-            f"  const int gtid = blockIdx.x * {THREADS} + tid, gstride = gridDim.x * {THREADS};\n"
+            f"  // XCD-aware block mapping: workgroups go round-robin over the 8 XCDs\n"
+            f"  // in dispatch order, so the chains of one row tile are placed on one\n"
+            f"  // XCD and share its L2 (identity when the grid is not a multiple of 8)\n"
+            f"  const int lid_ = blockIdx.y * gridDim.x + blockIdx.x;\n"
+            f"  const bool xcd_map_ = (gridDim.x & 7) == 0;\n"
+            f"  const int w_ = lid_ >> 3;\n"
+            f"  const int c = xcd_map_ ? w_ % gridDim.y : blockIdx.y;\n"
+            f"  const int tile = xcd_map_ ? (w_ / gridDim.y) * 8 + (lid_ & 7) : blockIdx.x;\n"
+            f"  const int tid = threadIdx.x;\n"
+            f"  const int gtid = tile * {THREADS} + tid, gstride = gridDim.x * {THREADS};\n"
             f"  for (int j = tid; j < P; j += {THREADS}) qs[j] = q[(size_t)c * P + j];\n"
             f"  for (int j = tid; j <= P; j += {THREADS}) gs[j] = 0.0;\n"
             f"  __syncthreads();\n"
@@ -675,9 +735,9 @@ class Lowering:
             f"  for (int s = tid; s <= NS; s += {THREADS}) {{\n"
             f"    double t = red[0][s];\n"
             f"    for (int w = 1; w < {THREADS // 64}; ++w) t += red[w][s];\n"
-            f"    gs[s < NS ? kSlotCoord[s] : P] += t;\n"
+            f"    gs[s < NS ? kSlotCoord_{name}[s] : P] += t;\n"
             f"  }}\n"
             f"  __syncthreads();\n"
-            f"  double* out = part + ((size_t)c * gridDim.x + blockIdx.x) * (P + 1);\n"
+            f"  double* out = part + ((size_t)c * gridDim.x + tile) * (P + 1);\n"
             f"  for (int j = tid; j <= P; j += {THREADS}) out[j] = gs[j];\n"
             f"}}\n")

@@ -539,7 +539,8 @@ class GenPlan:
     def build(self, hypers, data, opt):
         from .lower import Lowering, LoweringError
         try:
-            self.lowering = Lowering(self.model, hypers, data, ref_compat=opt.ref_compat)
+            self.lowering = Lowering(self.model, hypers, data, ref_compat=opt.ref_compat,
+                                     blocks=[b[0] for b in self.blocks])
         except LoweringError as e:
             _fail(f"generic: {e}")
         self.hypers = hypers

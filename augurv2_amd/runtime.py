@@ -318,12 +318,22 @@ class GenSampler(Sampler):
                 off, shape = lw.layout[v]
                 masks[b, off:off + (int(np.prod(shape)) if shape else 1)] = 1
             sim_len[b], eps[b], fixed[b] = sl, ep, fx
-        self._keep = (name_arr, offs, sizes, masks, sim_len, eps, fixed)
+        # the blocks' full-conditional passes, when the lowering generated
+        # them for exactly these blocks
+        bk = getattr(lw, "block_kernels", [])
+        if blocks and len(bk) == len(blocks):
+            kn = [n.encode() for n, _ in bk]
+            kernel_arr = (C.c_char_p * len(kn))(*kn)
+            grids = L.i32([g for _, g in bk])
+            kptr, gptr = C.cast(kernel_arr, C.c_void_p).value, L.ptr(grids).value
+        else:
+            kn, kernel_arr, grids, kptr, gptr = [], None, None, None, None
+        self._keep = (name_arr, offs, sizes, masks, sim_len, eps, fixed, kn, kernel_arr, grids)
         desc = L.GenDesc(self._src, self._kernel, self.P, lw.grid, chains, seed & 0xFFFFFFFF,
                          L.ptr(self._reals).value, self._reals.size, L.ptr(self._ints).value,
                          self._ints.size, len(names), C.cast(name_arr, C.c_void_p).value,
                          L.ptr(offs).value, L.ptr(sizes).value, len(blocks), L.ptr(masks).value,
-                         L.ptr(sim_len).value, L.ptr(eps).value, L.ptr(fixed).value)
+                         L.ptr(sim_len).value, L.ptr(eps).value, L.ptr(fixed).value, kptr, gptr)
         h = C.c_void_p()
         L.call("augur_gen_create", C.byref(desc), C.byref(h))
         self.n_blocks = len(blocks)

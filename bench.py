@@ -145,6 +145,7 @@ def run_hlr(steps, warmup, seed=2024):
         s.step(SWEEPS_PER_STEP)  # returns after the stream is synchronised
     t1 = time.perf_counter()
     avg_us, n_prof = s.profile_read()
+    final = [s.get("state", c) for c in range(C)]
     draws = s.read_trace("state")  # (sweeps, C, D + 2)
     keep = list(range(D)) + [D + 1]  # theta and v (no intercept in this model)
     min_ess, rhat, summ = _summary(draws, keep)
@@ -155,7 +156,72 @@ def run_hlr(steps, warmup, seed=2024):
         "min_ess_tail": float(np.min(summ["ess_tail"])), "sweeps": n_timed,
         "accept": [float(a) for a in acc.mean(axis=1)], "pass_avg_us": avg_us,
         "pass_launches_timed": n_prof, "N": N, "D": D, "C": C, "L": L, "eps": EPS,
-        "x": x, "y": y, "blocks": blocks, "post_mean_v": float(summ["mean"][-1]),
+        "x": x, "y": y, "blocks": blocks, "post_mean_v": float(summ["mean"][-1]), "final": final,
+    }
+
+
+GEN_HLR = """(K : Int, N : Int, a : Real, b : Real, x : Vec (Vec Real)) => {
+  param v ~ InvGamma(a, b) ;
+  param theta[k] ~ Normal(0.0, v)
+      for k <- 0 until K ;
+  data y[n] ~ Bernoulli( sigmoid(dotprod(theta, x[n])) )
+      for n <- 0 until N ;
+}
+"""
+
+
+def run_generic(hlr, sweeps=5000, warmup=300, seed=2024):
+    """The headline model and schedule through the generic lowering (the
+    data pass generated from the model text, FP64 X, compiled with hipRTC):
+    what a model outside the hand-written families gets.  The chains start
+    from the headline run's final states."""
+    from augurv2_amd import runtime
+    from augurv2_amd.pyaugur.lower import Lowering
+    from augurv2_amd.pyaugur.rvparse import parse_model
+    N, D, C = hlr["N"], hlr["D"], hlr["C"]
+    x = hlr["x"].astype(np.float64)
+    t0 = time.perf_counter()
+    lw = Lowering(parse_model(GEN_HLR), dict(K=D, N=N, a=1.0, b=1.0, x=x), dict(y=hlr["y"]),
+                  ref_compat=False, blocks=[["v"], ["theta"]])
+    s = runtime.GenSampler(lw, [(["v"], 1.0, 0.05, 0), (["theta"], hlr["L"] * hlr["eps"], hlr["eps"], 0)],
+                           chains=C, seed=seed)
+    compile_s = time.perf_counter() - t0
+    for c, st in enumerate(hlr["final"]):
+        # the HLR state is (theta, b, v); the generic vector is (v, theta)
+        s.set("state", np.concatenate([[st[D + 1]], st[:D]]), chain=c)
+    log(f"generic: warmup {warmup} sweeps")
+    s.step(warmup)
+    s.trace("state", sweeps)
+    s.profile(50)
+    t0 = time.perf_counter()
+    s.step(sweeps)
+    elapsed = time.perf_counter() - t0
+    avg_us, n_prof = s.profile_read()
+    draws = s.read_trace("state")
+    min_ess, rhat, _ = _summary(draws)
+    acc = s.acceptance()
+    # the data pass alone: the log joint's pass (the theta block's full
+    # conditional reads the same data) at the final points
+    qf = np.stack([s.get("state", c) for c in range(C)])
+    s.profile(1)
+    for _ in range(30):
+        s.eval(qf)
+    data_us, _ = s.profile_read()
+    s.close()
+    # X (FP64) and y once: the chains of a row tile share them through one
+    # XCD's L2
+    pass_bytes = N * D * 8 + N * 4
+    return {
+        "workload": "the headline model and schedule lowered generically (FP64 X), from the "
+                    "headline chains' final states",
+        "sweeps": sweeps, "warmup": warmup, "elapsed_s": elapsed, "ms_per_sweep": 1e3 * elapsed / sweeps,
+        "compile_s": compile_s, "min_ess": min_ess, "rhat_max": rhat, "rhat_ok": rhat < RHAT_MAX,
+        "min_ess_per_s": min_ess / elapsed if rhat < RHAT_MAX else None,
+        "min_ess_per_s_unchecked": min_ess / elapsed,
+        "accept": [float(a) for a in acc.mean(axis=1)], "grid": lw.grid,
+        "pass_avg_us_all_blocks": avg_us, "pass_launches_timed": n_prof,
+        "data_pass_us": data_us, "data_pass_bytes": pass_bytes,
+        "data_pass_gbs": pass_bytes / (data_us * 1e-6) / 1e9 if data_us else None,
     }
 
 
@@ -501,7 +567,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5, help="untimed warmup steps of the headline")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--only", default="", help="comma list of legs to run besides the headline "
-                    "HLR leg: lda, surrogate, batched, gmm (default: all)")
+                    "HLR leg: lda, surrogate, batched, gmm, generic (default: all)")
     ap.add_argument("--batched-sweeps", type=int, default=2000,
                     help="timed sweeps of the batched legs (warmup: half as many)")
     ap.add_argument("--surrogate-iters", type=int, default=5000,
@@ -523,7 +589,7 @@ def main():
         sys.exit(2)
     if args.steps < 1:
         raise SystemExit("bench: --steps must be at least 1")
-    legs = set(args.only.split(",")) if args.only else {"lda", "surrogate", "batched", "gmm"}
+    legs = set(args.only.split(",")) if args.only else {"lda", "surrogate", "batched", "gmm", "generic"}
     if args.skip_headline:
         for leg, fn in (("lda", lambda: run_lda(cpu=False)),
                         ("gmm", lambda: run_gmm(sweeps=args.gmm_sweeps, cpu=False)),
@@ -540,6 +606,9 @@ def main():
         return
     hlr = run_hlr(args.steps, args.warmup)
     log(f"hlr: {hlr['elapsed']:.3f} s, min ESS {hlr['min_ess']:.1f}, R-hat {hlr['rhat_max']:.4f}")
+    gen = run_generic(hlr) if "generic" in legs else None
+    if gen:
+        log(f"generic: {gen}")
     lda = run_lda(cpu=not args.no_cpu) if "lda" in legs else None
     if lda:
         log(f"lda: {lda['tokens_per_s']:.3e} tokens/s")
@@ -608,6 +677,7 @@ def main():
         "surrogate": surr,
         "batched": bat,
         "batched_16_chains": bat16,
+        "generic_lowering": gen,
         "roofline": {"kernel": "k_hlr_pass_r<64,4>", "bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": 1024.0 * (PASS_FETCH_KIB * FETCH_SCALE + PASS_WRITE_KIB * WRITE_SCALE),

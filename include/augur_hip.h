@@ -276,12 +276,16 @@ augur_status augur_op_gmm_predictive_ll(int k, int d, const double* pis, const d
  * `source` is HIP source that defines
  *   extern "C" __global__ void <kernel>(const double* q, double* part,
  *                                        const double* R, const int* I)
- * for a grid of (grid, chains) blocks of 256 threads.  Block (g, c) writes
- * part[(c * grid + g) * (n_params + 1) + j]: its share of d ll / dq_j for
- * j < n_params and of ll for j = n_params, at chain c's point
- * q[c * n_params ..).  R and I are the `reals` and `ints` buffers.  The
+ * for a grid of (grid, chains) blocks of 256 threads.  For every g < grid
+ * and chain c, one block writes part[(c * grid + g) * (n_params + 1) + j]:
+ * a share of d ll / dq_j for j < n_params and of ll for j = n_params, at
+ * chain c's point q[c * n_params ..); the grid shares of chain c sum to
+ * its ll and gradient.  R and I are the `reals` and `ints` buffers.  The
  * library compiles it with hipRTC for the device and runs HMC
- * (cbits/augur_mcmclib.h:343-424).  Block b moves the coordinates where
+ * (cbits/augur_mcmclib.h:343-424).  Each block may name its own pass of the
+ * same signature over its full conditional (the terms that mention its
+ * variables, Core/RwCore.hs:58-67); `kernel` stays the log joint, used by
+ * augur_get_curr_ll and augur_gen_eval.  Block b moves the coordinates where
  * masks[b * n_params + j] = 1, with the reference's jitter of step size
  * and count (fixed_steps as augur_hmc_block.fixed_steps).  Its Philox tags
  * are 16 + 4b (momentum), + 1 (jitter) and + 2 (accept).
@@ -308,6 +312,9 @@ typedef struct {
   const double* sim_len;     /* n_blocks */
   const double* step_size;   /* n_blocks */
   const int* fixed_steps;    /* n_blocks */
+  const char* const* block_kernels;  /* n_blocks names of the blocks' full-conditional
+                                        passes in `source`, or null: the log joint's */
+  const int* block_grids;    /* their grids */
 } augur_gen_desc;
 augur_status augur_gen_create(const augur_gen_desc* desc, augur_sampler** out);
 /* ll (chains) and d ll / dq (chains x n_params) at the points q (chains x

@@ -26,9 +26,10 @@ for a model whose parameters are all continuous:
     complex steps, so no hand-written adjoint is involved.  Those
     expressions are analytic, and complex steps give their derivatives to
     machine precision.
-  HMC: cbits/augur_mcmclib.h:343-424.  The momentum and leapfrog updates
-    are the reference's.  The step size and step count are jittered.  The
-    Metropolis test is in FP64.  Random numbers come from the same Philox
+  HMC: cbits/augur_mcmclib.h:343-424, each block on its full conditional
+    (the declarations that mention its variables, Core/RwCore.hs:58-67).
+    The momentum and leapfrog updates are the reference's.  The step size
+    and step count are jittered.  The Metropolis test is in FP64.  Random numbers come from the same Philox
     streams as the device: per (seed, chain), with counter
     (site, sub, sweep, tag).  Block b uses tags 16 + 4b + {0: momentum,
     1: jitter, 2: accept}.
@@ -38,8 +39,8 @@ row-major, give one vector q of P reals per chain.
 
 Pinning:
   - the tests check ll against scipy.stats log densities;
-  - they check the gradient against complex-step derivatives of ll (both in
-    the textbook mode);
+  - they check the gradient against central differences of ll (both in the
+    textbook mode);
   - they check the reference-compatible Normal forms against their closed
     forms;
   - they check the HMC restatement against a conjugate posterior.
@@ -261,8 +262,16 @@ class GenModel:
             out.append(float(np.sum(np.broadcast_to(dist_ll(d.exp[1], pt, args, self.ref_compat), (n,)))))
         return out
 
-    def ll(self, q):
-        return float(sum(self.ll_terms(q)))
+    def ll(self, q, block_vars=None):
+        """Log joint, or with block_vars the block's full conditional: the
+        terms of the declarations that mention a block variable, each once
+        (compiler/augur/src/Core/RwCore.hs:58-67)."""
+        terms = self.ll_terms(q)
+        if block_vars is None:
+            return float(sum(terms))
+        vs = set(block_vars)
+        return float(sum(t for d, t in zip(self.terms, terms)
+                         if d.name in vs or (_free_vars(d.exp, set()) & vs)))
 
     def grad(self, q):
         """The reference's gradient of the log joint with respect to q."""
@@ -294,9 +303,11 @@ class GenModel:
         return g
 
     # ---- HMC (cbits/augur_mcmclib.h:343-424) ----
-    def hmc_step(self, q, coords, sim_len, eps, fixed_steps, seed, chain, sweep, block):
-        """One HMC update of the coordinates `coords` (a boolean mask) of q;
-        returns (new q, accepted)."""
+    def hmc_step(self, q, coords, sim_len, eps, fixed_steps, seed, chain, sweep, block, block_vars=None):
+        """One HMC update of the coordinates `coords` (a boolean mask) of q,
+        accepted on the block's full conditional when block_vars names the
+        block's variables (else on the log joint); returns (new q,
+        accepted)."""
         tag = 16 + 4 * block
         step, L = jitter(seed, chain, sweep, tag + 1, sim_len, eps, fixed_steps)
         mask = np.asarray(coords, bool)
@@ -316,7 +327,7 @@ class GenModel:
         k1 = 0.5 * float(np.sum(p[mask] ** 2))
         w = orc.draw4(seed, chain, 0, 0, sweep, tag + 2)
         u = u01(int(w[0]))
-        acc = math.log(u) < (self.ll(x) - self.ll(q)) + (k0 - k1)
+        acc = math.log(u) < (self.ll(x, block_vars) - self.ll(q, block_vars)) + (k0 - k1)
         return (x if acc else q.copy()), acc
 
 

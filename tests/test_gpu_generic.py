@@ -23,6 +23,11 @@ def gpu():
     _lib.require_gpu()
 
 
+def _pair_data(name, n, seed=7):
+    _, hyp, dat = models(np.random.default_rng(seed), n)[name]
+    return hyp, dat
+
+
 def _pair(name, n, ref_compat, seed=7):
     rng = np.random.default_rng(seed)
     src, hyp, dat = models(rng, n)[name]
@@ -49,10 +54,10 @@ def test_pass_matches_oracle(name, n, ref_compat):
 
 
 def test_pass_grid_covers_every_row():
-    # more rows than one pass block per chain holds, and a grid of many
-    # blocks: a dropped or doubled row changes ll
-    lw, om = _pair("hlr", 200_001, False)
-    assert lw.grid > 256
+    # more rows than the largest grid holds at four per thread: the
+    # grid-stride loop takes the rest; a dropped or doubled row changes ll
+    lw, om = _pair("hlr", 600_001, False)
+    assert lw.grid == 512
     rng = np.random.default_rng(12)
     q = point(om, rng)[None]
     with runtime.GenSampler(lw, [], chains=1, seed=1) as s:
@@ -74,6 +79,9 @@ def test_hmc_trajectories_match_oracle(name, blocks):
     cbits/augur_mcmclib.h:343-424 (same jitter, momenta and decisions)."""
     print(f"generic HMC {name}", flush=True)
     lw, om = _pair(name, 400, True)
+    # each block runs the pass of its own full conditional
+    lw = Lowering(lw.model, *_pair_data(name, 400), ref_compat=True, blocks=[b[0] for b in blocks])
+    assert len(lw.block_kernels) == len(blocks)
     rng = np.random.default_rng(13)
     chains, seed = 2, 21
     q0 = np.stack([point(om, rng) for _ in range(chains)])
@@ -90,7 +98,7 @@ def test_hmc_trajectories_match_oracle(name, blocks):
                     for v in vars_:
                         off, shape = om.layout[v]
                         mask[off:off + (int(np.prod(shape)) if shape else 1)] = True
-                    cur[c], acc = om.hmc_step(cur[c], mask, sl, ep, 0, seed, c, sweep, b)
+                    cur[c], acc = om.hmc_step(cur[c], mask, sl, ep, 0, seed, c, sweep, b, block_vars=vars_)
                     accepted += acc
                 np.testing.assert_allclose(s.get("state", c), cur[c], rtol=1e-9, atol=1e-11)
         assert accepted > 0
