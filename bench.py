@@ -170,11 +170,13 @@ GEN_HLR = """(K : Int, N : Int, a : Real, b : Real, x : Vec (Vec Real)) => {
 """
 
 
-def run_generic(hlr, sweeps=5000, warmup=300, seed=2024):
+def run_generic(hlr, sweeps=10_000, warmup=300, seed=77):
     """The headline model and schedule through the generic lowering (the
     data pass generated from the model text, FP64 X, compiled with hipRTC):
     what a model outside the hand-written families gets.  The chains start
-    from the headline run's final states."""
+    from the headline run's final states, with a seed of their own (chains
+    driven by the same Philox streams as the headline's would couple to
+    them)."""
     from augurv2_amd import runtime
     from augurv2_amd.pyaugur.lower import Lowering
     from augurv2_amd.pyaugur.rvparse import parse_model
