@@ -577,6 +577,8 @@ def main():
     ap.add_argument("--surrogate-exact", type=int, default=2000,
                     help="exact iterations before the timed ones (half burn-in, half collection)")
     ap.add_argument("--gmm-sweeps", type=int, default=500, help="timed sweeps of the GMM leg")
+    ap.add_argument("--generic-sweeps", type=int, default=10_000,
+                    help="timed sweeps of the generic-lowering leg")
     ap.add_argument("--skip-headline", action="store_true",
                     help="run only the legs named by --only (profiling runs; no bench line)")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -602,13 +604,15 @@ def main():
                         ("batched", lambda: run_batched(sweeps=args.batched_sweeps,
                                                         warmup=max(1, args.batched_sweeps // 2),
                                                         chains=16, plain=False)),
+                        ("generic", lambda: run_generic(run_hlr(1, 1), sweeps=args.generic_sweeps,
+                                                        warmup=50)),
                         ("stream", lambda: log(str(stream_bandwidth(256, 3))))):
             if leg in legs:
                 log(f"{leg}: done ({type(fn()).__name__})")
         return
     hlr = run_hlr(args.steps, args.warmup)
     log(f"hlr: {hlr['elapsed']:.3f} s, min ESS {hlr['min_ess']:.1f}, R-hat {hlr['rhat_max']:.4f}")
-    gen = run_generic(hlr) if "generic" in legs else None
+    gen = run_generic(hlr, sweeps=args.generic_sweeps) if "generic" in legs else None
     if gen:
         log(f"generic: {gen}")
     lda = run_lda(cpu=not args.no_cpu) if "lda" in legs else None
