@@ -78,7 +78,12 @@ class GenDesc(C.Structure):
                 ("offsets", C.c_void_p), ("sizes", C.c_void_p), ("n_blocks", C.c_int),
                 ("masks", C.c_void_p), ("sim_len", C.c_void_p), ("step_size", C.c_void_p),
                 ("fixed_steps", C.c_void_p), ("block_kernels", C.c_void_p),
-                ("block_grids", C.c_void_p)]
+                ("block_grids", C.c_void_p), ("block_kinds", C.c_void_p), ("n_zints", C.c_int),
+                ("n_znames", C.c_int), ("znames", C.c_void_p), ("zoffsets", C.c_void_p),
+                ("zsizes", C.c_void_p)]
+
+
+GEN_HMC, GEN_DISC = 0, 1
 
 
 GMM_MU_CONJ, GMM_MU_ESLICE, GMM_MU_HMC, GMM_MU_MWG, GMM_MU_RSLICE, GMM_MU_NUTS = 0, 1, 2, 3, 4, 5

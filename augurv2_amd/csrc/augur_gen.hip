@@ -8,7 +8,7 @@
 // for the device's architecture and driven by device-resident HMC
 // (cbits/augur_mcmclib.h:343-424).
 //
-// The pass kernel is augur_gen_pass(q, part, R, I).  It runs on a grid of
+// The pass kernel is augur_gen_pass(q, part, R, I, Z).  It runs on a grid of
 // (G, chains) blocks of 256 threads and evaluates every chain at its
 // vector q[c * P .. + P).  Each block writes one row of P + 1 FP64
 // partials: d ll / dq, then ll.
@@ -20,7 +20,9 @@
 //                 the leapfrog update of every coordinate that the block
 //                 moves.
 //   k_gen_end     kinetic energies, FP64 Metropolis test, accept / reject.
-// Each block runs the pass of its own full conditional when the source
+// A DiscGibbs block runs its generated kernel (q, Z, R, I, seed, sweep, tag),
+// which redraws every point of one discrete parameter in Z.
+// Each HMC block runs the pass of its own full conditional when the source
 // provides one (else the log joint's).  The log density and gradient at the
 // current point are cached: they are what the pass computed when that point
 // was proposed.  An iteration of L leapfrog steps therefore costs L passes,
@@ -212,6 +214,7 @@ struct GenBlock {
   double sim_len, eps;
   int fixed_steps;
   int kernel;  // index into GenSampler::passes
+  int kind;    // AUGUR_GEN_HMC or AUGUR_GEN_DISC
 };
 
 // One compiled pass: the log joint (passes[0]) or a block's full conditional.
@@ -225,10 +228,12 @@ struct GenSampler final : augur_sampler {
   hipModule_t mod = nullptr;
   std::vector<GenPass> passes;
   DevBuf<double> q, prop, p0, p, k0, gcur, llcur, gprop, llprop, part, acc, R;
-  DevBuf<int32_t> I;
+  DevBuf<int32_t> I, Z;
+  int NZ = 1;  // discrete coordinates per chain
   DevBuf<unsigned char> masks;
   std::vector<GenBlock> blocks;
-  std::map<std::string, std::pair<int, int>> names;  // name -> (offset, size)
+  std::map<std::string, std::pair<int, int>> names;   // continuous: name -> (offset, size) in q
+  std::map<std::string, std::pair<int, int>> znames;  // discrete: name -> (offset, size) in Z
   // the pass whose ll and gradient at q are in llcur / gcur (-1: none);
   // a block's HMC keeps its own pass's cache valid, since an accepted
   // proposal brings the values computed for it
@@ -242,9 +247,14 @@ struct GenSampler final : augur_sampler {
   ParamView param(const std::string& name) override {
     ParamView v;
     if (name == "state") return ParamView{q.p, sizeof(double), (size_t)P, (size_t)P, true};
+    if (name == "zstate") return ParamView{Z.p, sizeof(int32_t), (size_t)NZ, (size_t)NZ, true};
     auto it = names.find(name);
-    if (it == names.end()) return v;
-    return ParamView{q.p + it->second.first, sizeof(double), (size_t)it->second.second, (size_t)P, true};
+    if (it != names.end())
+      return ParamView{q.p + it->second.first, sizeof(double), (size_t)it->second.second, (size_t)P, true};
+    auto zt = znames.find(name);
+    if (zt != znames.end())
+      return ParamView{Z.p + zt->second.first, sizeof(int32_t), (size_t)zt->second.second, (size_t)NZ, true};
+    return v;
   }
 
   void on_set(const std::string&) override { cache_owner = -1; }
@@ -267,7 +277,8 @@ struct GenSampler final : augur_sampler {
     double* pa = part.p;
     const double* ra = R.p;
     const int32_t* ia = I.p;
-    void* kargs[] = {(void*)&qa, (void*)&pa, (void*)&ra, (void*)&ia};
+    const int32_t* za = Z.p;
+    void* kargs[] = {(void*)&qa, (void*)&pa, (void*)&ra, (void*)&ia, (void*)&za};
     prof_begin();
     AUGUR_HIP(hipModuleLaunchKernel(passes[k].fn, passes[k].grid, chains, 1, kGenThreads, 1, 1, 0, stream,
                                     kargs, nullptr));
@@ -310,9 +321,28 @@ struct GenSampler final : augur_sampler {
     AUGUR_CHECK_LAUNCH();
   }
 
+  // DiscGibbs: every point of the discrete parameter drawn from its full
+  // conditional by the block's generated kernel; Z changes, so no pass's
+  // cache stays valid
+  void disc(int b) {
+    const GenBlock& B = blocks[b];
+    const double* qa = q.p;
+    int32_t* za = Z.p;
+    const double* ra = R.p;
+    const int32_t* ia = I.p;
+    uint32_t sd = seed, sw = sweep, tg = 16u + 4u * (uint32_t)b;
+    void* kargs[] = {(void*)&qa, (void*)&za, (void*)&ra, (void*)&ia, (void*)&sd, (void*)&sw, (void*)&tg};
+    AUGUR_HIP(hipModuleLaunchKernel(passes[B.kernel].fn, passes[B.kernel].grid, chains, 1, kGenThreads, 1, 1,
+                                    0, stream, kargs, nullptr));
+    cache_owner = -1;
+  }
+
   void run_sweeps(int n) override {
     for (int i = 0; i < n; ++i) {
-      for (int b = 0; b < (int)blocks.size(); ++b) hmc(b);
+      for (int b = 0; b < (int)blocks.size(); ++b) {
+        if (blocks[b].kind == AUGUR_GEN_DISC) disc(b);
+        else hmc(b);
+      }
       ++sweep;
       record_traces();
     }
@@ -364,6 +394,17 @@ static void gen_validate(const augur_gen_desc* D) {
   for (int b = 0; b < D->n_blocks; ++b)
     if (!(D->step_size[b] > 0.0) || !(D->sim_len[b] >= 0.0))
       fail(AUGUR_ERR_ARG, "block %d: step size must be positive and sim_len non-negative", b);
+  for (int b = 0; b < D->n_blocks; ++b)
+    if (D->block_kinds && D->block_kinds[b] != AUGUR_GEN_HMC && D->block_kinds[b] != AUGUR_GEN_DISC)
+      fail(AUGUR_ERR_ARG, "block %d: kind must be AUGUR_GEN_HMC or AUGUR_GEN_DISC", b);
+  for (int b = 0; b < D->n_blocks; ++b)
+    if (D->block_kinds && D->block_kinds[b] == AUGUR_GEN_DISC && !D->block_kernels)
+      fail(AUGUR_ERR_ARG, "block %d: a DiscGibbs block needs its kernel", b);
+  if (D->n_zints < 0 || (D->n_znames && (!D->znames || !D->zoffsets || !D->zsizes)))
+    fail(AUGUR_ERR_ARG, "discrete parameter names need offsets and sizes");
+  for (int i = 0; i < D->n_znames; ++i)
+    if (!D->znames[i] || D->zoffsets[i] < 0 || D->zsizes[i] < 1 || D->zoffsets[i] + D->zsizes[i] > D->n_zints)
+      fail(AUGUR_ERR_ARG, "discrete parameter %d lies outside the vector of %d", i, D->n_zints);
   if (D->block_kernels) {
     if (!D->block_grids) fail(AUGUR_ERR_ARG, "block kernels need their grids");
     for (int b = 0; b < D->n_blocks; ++b)
@@ -415,7 +456,8 @@ augur_status augur_gen_create(const augur_gen_desc* D, augur_sampler** out) {
           s->passes.push_back(bp);
           k = (int)s->passes.size() - 1;
         }
-        s->blocks.push_back({D->sim_len[b], D->step_size[b], D->fixed_steps[b], k});
+        s->blocks.push_back({D->sim_len[b], D->step_size[b], D->fixed_steps[b], k,
+                             D->block_kinds ? D->block_kinds[b] : AUGUR_GEN_HMC});
       }
       s->seed = D->seed;
       s->chains = D->chains;
@@ -436,6 +478,10 @@ augur_status augur_gen_create(const augur_gen_desc* D, augur_sampler** out) {
       s->masks.alloc(std::max<size_t>(1, (size_t)D->n_blocks * P));
       if (D->n_blocks) s->masks.upload(D->masks, (size_t)D->n_blocks * P, s->stream);
       for (int i = 0; i < D->n_names; ++i) s->names[D->names[i]] = {D->offsets[i], D->sizes[i]};
+      s->NZ = std::max(1, D->n_zints);
+      s->Z.alloc(C * (size_t)s->NZ);
+      s->Z.zero(s->stream);
+      for (int i = 0; i < D->n_znames; ++i) s->znames[D->znames[i]] = {D->zoffsets[i], D->zsizes[i]};
       AUGUR_HIP(hipStreamSynchronize(s->stream));
     } catch (...) {
       delete s;

@@ -58,10 +58,12 @@ import math
 import numpy as np
 
 PARAM_DISTS = {"Normal", "Exponential", "Gamma", "InvGamma", "Beta", "Uniform"}
+DISC_DISTS = {"Categorical", "Bernoulli"}   # discrete parameters (DiscGibbs)
 DATA_DISTS = PARAM_DISTS | {"Bernoulli", "Poisson"}
-INT_DISTS = {"Bernoulli", "Poisson"}
+INT_DISTS = {"Bernoulli", "Poisson", "Categorical"}
 ARITY = {"Normal": 2, "Exponential": 1, "Gamma": 2, "InvGamma": 2, "Beta": 2, "Uniform": 2,
-         "Bernoulli": 1, "Poisson": 1}
+         "Bernoulli": 1, "Poisson": 1, "Categorical": 1}
+MAX_SUPPORT = 64       # values of a discrete parameter
 THREADS = 256
 MAX_GRID = 512         # a multiple of 8
 ROWS_PER_THREAD = 4
@@ -186,6 +188,23 @@ __device__ inline double gen_uniform_dotpt(double pt, double l, double u) { retu
 __device__ inline double gen_uniform_dotarg1(double pt, double l, double u) { return (l < pt && pt < u) ? 1.0 / (u - l) : 0.0; }
 __device__ inline double gen_uniform_dotarg2(double pt, double l, double u) { return (l < pt && pt < u) ? -1.0 / (u - l) : 0.0; }
 
+// Philox4x32-10 (Salmon et al., SC'11) with counter (site, sub, sweep, tag)
+// and key (seed, chain), as the library's RNG; u = (w + 0.5) 2^-32
+__device__ inline double gen_u01(unsigned seed, unsigned chain, unsigned site, unsigned sweep,
+                                 unsigned tag) {
+  unsigned c0 = site, c1 = 0u, c2 = sweep, c3 = tag, k0 = seed, k1 = chain;
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+    const unsigned long long p0 = (unsigned long long)0xD2511F53u * c0;
+    const unsigned long long p1 = (unsigned long long)0xCD9E8D57u * c2;
+    const unsigned hi0 = (unsigned)(p0 >> 32), lo0 = (unsigned)p0;
+    const unsigned hi1 = (unsigned)(p1 >> 32), lo1 = (unsigned)p1;
+    const unsigned n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+    c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+  }
+  return ((double)c0 + 0.5) * 2.3283064365386963e-10;
+}
+
 __device__ inline double gen_wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
@@ -221,11 +240,12 @@ class _Sym:
     """What a name denotes inside the model."""
 
     def __init__(self, kind, shape, is_int=False, value=None, off=None):
-        self.kind = kind          # "hyper" | "param" | "data"
+        self.kind = kind          # "hyper" | "param" | "dparam" (discrete) | "data"
         self.shape = shape        # () | (n,) | (n, m)
         self.is_int = is_int
         self.value = value        # hyper scalars: the constant
-        self.off = off            # offset into q (params) or into R / I (arrays)
+        self.off = off            # offset into q (params), Z (discrete) or R / I (arrays)
+        self.support = 0          # discrete parameters: number of values
 
 
 class Lowering:
@@ -237,7 +257,10 @@ class Lowering:
     kernel_name = "augur_gen_pass"
 
     def __init__(self, model, hypers, data, ref_compat=True, blocks=None):
+        """blocks: the schedule, each an HMC block (a list of continuous
+        parameter names) or ("DiscGibbs", [name]) for a discrete parameter."""
         self.model = model
+        self._disc_sub = None      # (name, index codes, candidate) inside a DiscGibbs kernel
         self.ref_compat = bool(ref_compat)
         self.syms = {}
         self._int_values = {}      # int arrays by name (index bounds checks)
@@ -257,13 +280,23 @@ class Lowering:
         decls = [d for d in self.model.decls if d.stochastic]
         kernels = [self._emit_kernel(self.kernel_name, decls)]
         self.grid, self.n_slots = self._last_grid, self._last_slots
-        self.block_kernels = []
-        for b, vars_ in enumerate(blocks or []):
+        self.block_kernels = []   # (kernel name, grid, kind: "HMC" | "DiscGibbs")
+        for b, blk in enumerate(blocks or []):
+            kind, vars_ = ("HMC", blk) if isinstance(blk, list) else blk
             vs = set(vars_)
             sub = [d for d in decls if d.name in vs or (_free_vars(d.exp, set()) & vs)]
-            name = f"{self.kernel_name}_b{b}"
-            kernels.append(self._emit_kernel(name, sub))
-            self.block_kernels.append((name, self._last_grid))
+            if kind == "DiscGibbs":
+                if len(vars_) != 1 or self.syms.get(vars_[0]) is None or self.syms[vars_[0]].kind != "dparam":
+                    _fail(f"DiscGibbs updates one discrete parameter, got {vars_}")
+                name = f"augur_gen_disc_b{b}"
+                kernels.append(self._emit_disc_kernel(name, vars_[0], sub))
+            else:
+                for v in vars_:
+                    if self.syms.get(v) is None or self.syms[v].kind != "param":
+                        _fail(f"HMC moves continuous parameters; {v} is not one")
+                name = f"{self.kernel_name}_b{b}"
+                kernels.append(self._emit_kernel(name, sub))
+            self.block_kernels.append((name, self._last_grid, kind))
         self.source = (f"constexpr int kRefCompat = {1 if self.ref_compat else 0};\n" + PRELUDE
                        + "".join(kernels))
 
@@ -326,27 +359,49 @@ class Lowering:
             sizes.append(n)
         return tuple(sizes)
 
+    def _support(self, d):
+        """Values of a discrete parameter: Bernoulli 2, Categorical(pmf) the
+        length of its pmf (a Real vector hyper-parameter)."""
+        if d.exp[1] == "Bernoulli":
+            return 2
+        a = d.exp[2][0] if len(d.exp[2]) == 1 else None
+        s = self.syms.get(a[1]) if a is not None and a[0] == "var" else None
+        if s is None or s.kind != "hyper" or len(s.shape) != 1 or s.is_int:
+            _fail(f"{d.name} ~ Categorical(pmf): pmf must be a Real vector hyper-parameter")
+        if not 1 <= s.shape[0] <= MAX_SUPPORT:
+            _fail(f"{d.name}: the generic DiscGibbs takes at most {MAX_SUPPORT} values")
+        return s.shape[0]
+
     def _bind_params(self):
-        off = 0
-        self.layout = {}
+        off = zoff = 0
+        self.layout = {}           # continuous: name -> (offset in q, shape)
+        self.zlayout = {}          # discrete: name -> (offset in Z, shape, support)
         self.param_names = []
         for d in self.model.params:
             if not d.stochastic:
                 _fail(f"deterministic parameter {d.name}: outside the generic lowering")
-            if d.exp[0] != "dist" or d.exp[1] not in PARAM_DISTS:
+            if d.exp[0] != "dist" or d.exp[1] not in PARAM_DISTS | DISC_DISTS:
                 _fail(f"parameter {d.name}: the generic lowering samples continuous parameters "
-                      f"({', '.join(sorted(PARAM_DISTS))}) with HMC")
+                      f"({', '.join(sorted(PARAM_DISTS))}) with HMC and discrete ones "
+                      f"({', '.join(sorted(DISC_DISTS))}) with DiscGibbs")
             if len(d.grids) != len(d.idxs) or len(d.idxs) > 2:
                 _fail(f"parameter {d.name}: rank 0-2 over its own grid")
             shape = self._grid_sizes(d)
             size = int(np.prod(shape)) if shape else 1
-            self.syms[d.name] = _Sym("param", shape, off=off)
-            self.layout[d.name] = (off, shape)
+            if d.exp[1] in DISC_DISTS:
+                sym = _Sym("dparam", shape, is_int=True, off=zoff)
+                sym.support = self._support(d)
+                self.syms[d.name] = sym
+                self.zlayout[d.name] = (zoff, shape, sym.support)
+                zoff += size
+            else:
+                self.syms[d.name] = _Sym("param", shape, off=off)
+                self.layout[d.name] = (off, shape)
+                off += size
             self.param_names.append(d.name)
-            off += size
-        self.P = off
-        if self.P == 0:
-            _fail("the model has no parameters")
+        # kernels hold q and Z in arrays of at least one element
+        self.P = max(off, 1)
+        self.NZ = max(zoff, 1)
         if self.P > MAX_PARAMS:
             _fail(f"{self.P} parameter coordinates: the generic lowering holds at most {MAX_PARAMS}")
 
@@ -441,8 +496,13 @@ class Lowering:
             if e[1] in self._grid_ranges:
                 return 0, self._grid_ranges[e[1]] - 1
             s = self.syms.get(e[1])
+            if s is not None and s.shape == () and s.kind == "dparam":
+                return 0, s.support - 1
             if s is not None and s.shape == () and s.is_int and s.kind != "param":
                 return s.value, s.value
+        if k == "proj" and e[1][0] == "var" and e[1][1] in self.zlayout:
+            self._check_proj(e[1][1], e[2])
+            return 0, self.syms[e[1][1]].support - 1
         if k == "proj" and e[1][0] == "var" and e[1][1] in self._int_values:
             self._check_proj(e[1][1], e[2])
             vals = self._int_values[e[1][1]]
@@ -476,6 +536,8 @@ class Lowering:
             if e[1] in env:
                 return env[e[1]]
             s = self.syms.get(e[1])
+            if s and s.kind == "dparam" and s.shape == ():
+                return self._zread(e[1], s, [])
             if s and s.kind == "hyper" and s.shape == () and s.is_int:
                 return s.value
             if s and s.kind == "data" and s.shape == () and s.is_int:
@@ -486,7 +548,10 @@ class Lowering:
             s = self.syms.get(name)
             if s is None or not s.is_int or s.kind == "param" or len(e[2]) != len(s.shape):
                 _fail(f"index {e} must be an element of an int array")
-            return f"I[{self._flat(s, [self._index(i, env) for i in e[2]])}]"
+            idx = [self._index(i, env) for i in e[2]]
+            if s.kind == "dparam":
+                return self._zread(name, s, idx)
+            return f"I[{self._flat(s, idx)}]"
         if k == "bin" and e[1] in "+-*":
             a, b = self._index(e[2], env), self._index(e[3], env)
             if isinstance(a, int) and isinstance(b, int):
@@ -503,7 +568,7 @@ class Lowering:
             i = idx[0]
             return base + i if isinstance(i, int) and isinstance(base, int) else f"{base} + ({i})"
         i, j = idx
-        if s.kind == "param":
+        if s.kind in ("param", "dparam"):
             if isinstance(i, int) and isinstance(j, int):
                 return base + i * s.shape[1] + j
             return f"{base} + ({i}) * {s.shape[1]} + ({j})"
@@ -515,7 +580,20 @@ class Lowering:
     def _elem_code(self, s, pos):
         if s.kind == "param":
             return f"qs[{pos}]"
+        if s.kind == "dparam":
+            return f"(double)Zc[{pos}]"
         return f"(double)I[{pos}]" if s.is_int else f"R[{pos}]"
+
+    def _zread(self, name, s, idx):
+        """Int expression of a discrete parameter's element: its current value
+        (Z), or inside a DiscGibbs kernel the candidate value at the point
+        being drawn."""
+        sub = self._disc_sub
+        if sub is not None and sub[0] == name:
+            if tuple(str(i) for i in idx) != sub[1]:
+                _fail(f"{name} is used at another point than the one being drawn")
+            return sub[2]
+        return f"Zc[{self._flat(s, idx) if idx else s.off}]"
 
     def _sym_val(self, name, s, idx, lines):
         """Value of name[idx...] (idx may be shorter than the rank: a view)."""
@@ -527,7 +605,11 @@ class Lowering:
                 return _Val(_lit(s.value))
             if s.kind == "data":
                 return _Val(_lit(s.value))
+            if s.kind == "dparam":
+                return _Val(self._temp(lines, f"(double){self._zread(name, s, [])}"))
             return _Val(f"qs[{s.off}]", dep=True, coord=s.off, op=("param", name))
+        if len(idx) == rank and s.kind == "dparam":
+            return _Val(self._temp(lines, f"(double){self._zread(name, s, idx)}"))
         if len(idx) == rank:
             pos = self._flat(s, idx)
             code = self._temp(lines, self._elem_code(s, pos))
@@ -641,23 +723,35 @@ class Lowering:
                          f"atomicAdd(&gs[{x.vcoord('k_')}], {adj} * {y.elem('k_')});")
 
     # ---- declarations ----
-    def _term(self, d):
+    def _grid_env(self, d):
+        """Grid extents of a declaration and its grid variables as C++ int
+        expressions of the flattened index f_."""
+        sizes = self._grid_sizes(d)
+        self._grid_ranges = {g[0]: n for g, n in zip(d.grids, sizes)}
+        env = {}
+        if len(sizes) == 1:
+            env[d.grids[0][0]] = "f_"
+        elif len(sizes) == 2:
+            env[d.grids[0][0]] = f"(f_ / {sizes[1]})"
+            env[d.grids[1][0]] = f"(f_ % {sizes[1]})"
+        return sizes, env
+
+    def _term_lines(self, d, env, ll_var, grads):
+        """Code adding the term's log density at the point of env to ll_var,
+        and (grads) its reverse-mode contributions."""
         lines = []
         dist, args_e = d.exp[1], d.exp[2]
         if len(args_e) != ARITY[dist]:
             _fail(f"{d.name}: {dist} takes {ARITY[dist]} arguments")
-        sizes = self._grid_sizes(d)
-        self._grid_ranges = {g[0]: n for g, n in zip(d.grids, sizes)}
-        env = {}
-        if sizes:
-            total = int(np.prod(sizes))
-            if len(sizes) == 1:
-                env[d.grids[0][0]] = "f_"
-            else:
-                env[d.grids[0][0]] = f"(f_ / {sizes[1]})"
-                env[d.grids[1][0]] = f"(f_ % {sizes[1]})"
         s = self.syms[d.name]
         pt = self._sym_val(d.name, s, [env[g[0]] for g in d.grids], lines)
+        if dist == "Categorical":
+            # cbits/augur_dist.c:100-104: log pmf[pt]; the pmf is a hyper-parameter
+            a = self._fwd(args_e[0], env, lines)
+            if not a.is_vec or a.dep:
+                _fail(f"{d.name} ~ Categorical(pmf): pmf must be a Real vector hyper-parameter")
+            lines.append(f"{ll_var} += log({a.elem(f'(int){pt.code}')});")
+            return lines
         pt_code = f"(int){pt.code}" if dist in INT_DISTS else pt.code
         args = [self._fwd(a, env, lines) for a in args_e]
         for a in args:
@@ -665,19 +759,124 @@ class Lowering:
                 _fail(f"{d.name}: distribution arguments must be scalars")
         fn = "gen_" + dist.lower()
         call = ", ".join([pt_code] + [a.code for a in args])
-        lines.append(f"llacc += {fn}_ll({call});")
+        lines.append(f"{ll_var} += {fn}_ll({call});")
+        if not grads:
+            return lines
         if pt.dep:
             self._bwd(pt, self._temp(lines, f"{fn}_dotpt({call})", "d"), lines)
         for i, a in enumerate(args):
             if a.dep:
                 self._bwd(a, self._temp(lines, f"{fn}_dotarg{i + 1}({call})", "d"), lines)
-        body = "\n      ".join(lines)
+        return lines
+
+    def _term(self, d):
+        sizes, env = self._grid_env(d)
+        body = "\n      ".join(self._term_lines(d, env, "llacc", True))
         if sizes:
             total = int(np.prod(sizes))
             self.max_range = max(self.max_range, total)
-            return (f"  // {d.kind} {d.name} ~ {dist}\n"
+            return (f"  // {d.kind} {d.name} ~ {d.exp[1]}\n"
                     f"  for (int f_ = gtid; f_ < {total}; f_ += gstride) {{\n      {body}\n  }}\n")
-        return f"  // {d.kind} {d.name} ~ {dist}\n  if (gtid == 0) {{\n      {body}\n  }}\n"
+        return f"  // {d.kind} {d.name} ~ {d.exp[1]}\n  if (gtid == 0) {{\n      {body}\n  }}\n"
+
+    def _check_aligned(self, zname, d, zsizes):
+        """Every mention of zname in d is zname[d's own grid variables], over
+        the same extents: the full conditional of one point of zname then
+        holds only terms at that point."""
+        gv = [g[0] for g in d.grids]
+
+        def walk(e):
+            k = e[0]
+            if k == "var" and e[1] == zname and zsizes:
+                _fail(f"{d.name} uses {zname} whole; DiscGibbs needs it at the term's own point")
+            if k == "proj" and e[1] == ("var", zname):
+                if [i for i in e[2]] != [("var", v) for v in gv]:
+                    _fail(f"{d.name} uses {zname} at another point than its own")
+            if k == "proj":
+                if e[1] != ("var", zname):
+                    walk(e[1])
+                for i in e[2]:
+                    walk(i)
+            elif k in ("prim", "dist"):
+                for a in e[2]:
+                    walk(a)
+            elif k == "bin":
+                walk(e[2])
+                walk(e[3])
+        walk(d.exp)
+        if d.name != zname and self._grid_sizes(d) != zsizes:
+            _fail(f"{d.name}: DiscGibbs on {zname} needs the terms that use it over its grid")
+
+    def _emit_disc_kernel(self, name, zname, decls):
+        """DiscGibbs on a discrete parameter (Core/CgDisc.hs:120-165): at every
+        point of its grid, the log full conditional of each value (its prior
+        and the terms that use it at that point), then one draw by inverse
+        CDF in log space with the maximum subtracted first (the reference
+        multiplies pdfs in linear space) from the uniform of Philox counter
+        (point, 0, sweep, tag) under key (seed, chain)."""
+        zs = self.syms[zname]
+        zsizes = zs.shape
+        K = zs.support
+        for d in decls:
+            self._check_aligned(zname, d, zsizes)
+        total = int(np.prod(zsizes)) if zsizes else 1
+        codes = tuple(self._grid_env(self.model.decl(zname))[1][g[0]]
+                      for g in self.model.decl(zname).grids)
+        self._disc_sub = (zname, codes, "k_")
+        body = []
+        try:
+            for d in decls:
+                _, env = self._grid_env(d)
+                body.append(f"// {d.kind} {d.name} ~ {d.exp[1]}")
+                body.extend(self._term_lines(d, env, "llk", False))
+        finally:
+            self._disc_sub = None
+        body_code = "\n        ".join(body)
+        zpos = self._flat(zs, list(codes)) if codes else zs.off
+        g = -(-total // (THREADS * ROWS_PER_THREAD))
+        self._last_grid = max(8, min(MAX_GRID, -(-g // 8) * 8))
+        self._last_slots = 0
+        return (
+            f'extern "C" __global__ __launch_bounds__({THREADS}) void {name}(\n'
+            f"    const double* __restrict__ q, int* __restrict__ Z, const double* __restrict__ R,\n"
+            f"    const int* __restrict__ I, unsigned seed, unsigned sweep, unsigned tag) {{\n"
+            f"  constexpr int P = {self.P}, NZ = {self.NZ}, K = {K};\n"
+            f"  __shared__ double qs[P];\n"
+            + self._mapping_code() +
+            f"  for (int j = tid; j < P; j += {THREADS}) qs[j] = q[(size_t)c * P + j];\n"
+            f"  __syncthreads();\n"
+            f"  int* Zc = Z + (size_t)c * NZ;\n"
+            f"  for (int f_ = gtid; f_ < {total}; f_ += gstride) {{\n"
+            f"    double lp[K];\n"
+            f"    double mx = -1.0e308;\n"
+            f"    for (int k_ = 0; k_ < K; ++k_) {{\n"
+            f"        double llk = 0.0;\n"
+            f"        {body_code}\n"
+            f"        lp[k_] = llk;\n"
+            f"        mx = llk > mx ? llk : mx;\n"
+            f"    }}\n"
+            f"    double tot = 0.0;\n"
+            f"    for (int k_ = 0; k_ < K; ++k_) {{ lp[k_] = exp(lp[k_] - mx); tot += lp[k_]; }}\n"
+            f"    const double u = gen_u01(seed, (unsigned)c, (unsigned)f_, sweep, tag) * tot;\n"
+            f"    int pick = K - 1;\n"
+            f"    double cum = 0.0;\n"
+            f"    for (int k_ = 0; k_ < K; ++k_) {{ cum += lp[k_]; if (u < cum) {{ pick = k_; break; }} }}\n"
+            f"    Zc[{zpos}] = pick;\n"
+            f"  }}\n"
+            f"}}\n")
+
+    def _mapping_code(self):
+        return (
+            f"  // XCD-aware block mapping: workgroups go round-robin over the 8 XCDs\n"
+            f"  // in dispatch order, so the chains of one row tile are placed on one\n"
+            f"  // XCD and share its L2 (identity when the grid is not a multiple of 8)\n"
+            f"  const int lid_ = blockIdx.y * gridDim.x + blockIdx.x;\n"
+            f"  const bool xcd_map_ = (gridDim.x & 7) == 0;\n"
+            f"  const int w_ = lid_ >> 3;\n"
+            f"  const int c = xcd_map_ ? w_ % gridDim.y : blockIdx.y;\n"
+            f"  const int tile = xcd_map_ ? (w_ / gridDim.y) * 8 + (lid_ & 7) : blockIdx.x;\n"
+            f"  const int tid = threadIdx.x;\n"
+            f"  const int gtid = tile * {THREADS} + tid, gstride = gridDim.x * {THREADS};\n")
 
     def _emit_kernel(self, name, decls):
         """Source of one pass kernel over the terms of `decls`; sets
@@ -702,21 +901,14 @@ class Lowering:
             f"__constant__ int kSlotCoord_{name}[{ns}] = {{{table}}};\n"
             f'extern "C" __global__ __launch_bounds__({THREADS}) void {name}(\n'
             f"    const double* __restrict__ q, double* __restrict__ part,\n"
-            f"    const double* __restrict__ R, const int* __restrict__ I) {{\n"
-            f"  constexpr int P = {P}, NS = {n_slots};\n"
+            f"    const double* __restrict__ R, const int* __restrict__ I,\n"
+            f"    const int* __restrict__ Z) {{\n"
+            f"  constexpr int P = {P}, NS = {n_slots}, NZ = {self.NZ};\n"
             f"  __shared__ double qs[P];\n"
             f"  __shared__ double gs[P + 1];\n"
             f"  __shared__ double red[{THREADS // 64}][NS + 1];\n"
-            f"  // XCD-aware block mapping: workgroups go round-robin over the 8 XCDs\n"
-            f"  // in dispatch order, so the chains of one row tile are placed on one\n"
-            f"  // XCD and share its L2 (identity when the grid is not a multiple of 8)\n"
-            f"  const int lid_ = blockIdx.y * gridDim.x + blockIdx.x;\n"
-            f"  const bool xcd_map_ = (gridDim.x & 7) == 0;\n"
-            f"  const int w_ = lid_ >> 3;\n"
-            f"  const int c = xcd_map_ ? w_ % gridDim.y : blockIdx.y;\n"
-            f"  const int tile = xcd_map_ ? (w_ / gridDim.y) * 8 + (lid_ & 7) : blockIdx.x;\n"
-            f"  const int tid = threadIdx.x;\n"
-            f"  const int gtid = tile * {THREADS} + tid, gstride = gridDim.x * {THREADS};\n"
+            + self._mapping_code() +
+            f"  const int* Zc = Z + (size_t)c * NZ;\n"
             f"  for (int j = tid; j < P; j += {THREADS}) qs[j] = q[(size_t)c * P + j];\n"
             f"  for (int j = tid; j <= P; j += {THREADS}) gs[j] = 0.0;\n"
             f"  __syncthreads();\n"

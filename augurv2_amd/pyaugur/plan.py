@@ -11,12 +11,13 @@ Families (each recognised up to variable names):
   GMM  -- compiler/augur/test/hmvgmm.rv (and mvgmm.rv: fixed pis and covs)
           with ConjGibbs pis, ConjGibbs | ESlice | HMC | MWG | RSlice | NUTS mu,
           ConjGibbs cov, DiscGibbs z
-  GEN  -- any other model whose parameters are all continuous: the data pass
-          is generated from the model (pyaugur/lower.py, compiled at run
-          time with hipRTC) and sampled with HMC blocks.
+  GEN  -- any other model whose parameters are continuous or finite discrete
+          (Categorical over a fixed pmf, Bernoulli): the passes are generated
+          from the model (pyaugur/lower.py, compiled at run time with hipRTC)
+          and run as HMC and DiscGibbs blocks.
 The reference's compiler generates code for any model (Compile.hs:311-345);
-models with discrete or multivariate parameters outside the three families
-raise UnsupportedModel.
+models with multivariate parameters (Dirichlet, MvNormal, InvWishart) or
+deterministic parameters outside the three families raise UnsupportedModel.
 """
 import numpy as np
 
@@ -496,40 +497,47 @@ def _np_eval(e, env):
 
 
 class GenPlan:
-    """Any model whose parameters are all continuous: the data pass is
-    generated from the model (pyaugur.lower) and sampled with HMC blocks.
-    The reference's kernel heuristic gives each such parameter its own HMC
-    block with the defaults simLen 1.0, eps 0.05 (Compile/Compile.hs:143-165,
-    Core/KernSyn.hs:64-75); it would pick ConjGibbs where it detects
-    conjugacy, which the generic path does not analyse (HMC targets the same
-    posterior)."""
+    """Any model whose parameters are continuous or finite discrete: the
+    passes are generated from the model (pyaugur.lower) and run as HMC
+    blocks on continuous parameters and DiscGibbs on discrete ones.  The
+    reference's kernel heuristic gives each parameter its own kernel:
+    DiscGibbs for a discrete one, else HMC with the defaults simLen 1.0,
+    eps 0.05 (Compile/Compile.hs:143-165, Core/KernSyn.hs:64-75); it would
+    pick ConjGibbs where it detects conjugacy, which the generic path does
+    not analyse (HMC targets the same posterior)."""
 
     family = "gen"
 
     def __init__(self, model, sched):
-        from .lower import DATA_DISTS, PARAM_DISTS
+        from .lower import DATA_DISTS, DISC_DISTS, PARAM_DISTS
         for d in model.params:
-            if not d.stochastic or d.exp[0] != "dist" or d.exp[1] not in PARAM_DISTS:
+            if not d.stochastic or d.exp[0] != "dist" or d.exp[1] not in PARAM_DISTS | DISC_DISTS:
                 _fail(f"generic: parameter {d.name} must be continuous "
-                      f"({', '.join(sorted(PARAM_DISTS))})")
+                      f"({', '.join(sorted(PARAM_DISTS))}) or discrete ({', '.join(sorted(DISC_DISTS))})")
         for d in model.data:
             if d.exp[0] != "dist" or d.exp[1] not in DATA_DISTS:
                 _fail(f"generic: data {d.name} must follow one of {', '.join(sorted(DATA_DISTS))}")
         self.model = model
         self.param_names = [d.name for d in model.params]
+        self.discrete = {d.name for d in model.params if d.exp[1] in DISC_DISTS}
         if sched is None:
-            sched = [Kern("HMC", [n]) for n in self.param_names]
+            sched = [Kern("DiscGibbs" if n in self.discrete else "HMC", [n]) for n in self.param_names]
         self.blocks = []
         seen = set()
         for k in sched:
-            if k.kind != "HMC":
-                _fail(f"generic: the generic path runs HMC blocks (got {k.kind})")
+            if k.kind not in ("HMC", "DiscGibbs"):
+                _fail(f"generic: the generic path runs HMC and DiscGibbs blocks (got {k.kind})")
             for v in k.vars:
                 if v not in self.param_names:
                     _fail(f"generic: {v} is not a parameter")
                 if v in seen:
                     _fail(f"generic: {v} appears in two blocks")
+                if (k.kind == "DiscGibbs") != (v in self.discrete):
+                    _fail(f"generic: {v} needs {'DiscGibbs' if v in self.discrete else 'HMC'}")
                 seen.add(v)
+            if k.kind == "DiscGibbs":
+                self.blocks.append(("DiscGibbs", list(k.vars)))
+                continue
             sim_len, eps = k.settings if k.settings else (1.0, 0.05)
             self.blocks.append((list(k.vars), float(sim_len), float(eps)))
         missing = [n for n in self.param_names if n not in seen]
@@ -538,18 +546,26 @@ class GenPlan:
 
     def build(self, hypers, data, opt):
         from .lower import Lowering, LoweringError
+        lblocks = [b if b[0] == "DiscGibbs" else b[0] for b in self.blocks]
         try:
-            self.lowering = Lowering(self.model, hypers, data, ref_compat=opt.ref_compat,
-                                     blocks=[b[0] for b in self.blocks])
+            self.lowering = Lowering(self.model, hypers, data, ref_compat=opt.ref_compat, blocks=lblocks)
         except LoweringError as e:
             _fail(f"generic: {e}")
         self.hypers = hypers
         blocks = []
-        for vars_, sim_len, eps in self.blocks:
+        for b in self.blocks:
+            if b[0] == "DiscGibbs":
+                blocks.append(("DiscGibbs", b[1][0]))
+                continue
+            vars_, sim_len, eps = b
             fixed = 0 if opt.hmc_jitter else max(1, int(round(sim_len / eps)))
             blocks.append((vars_, sim_len, eps, fixed))
         self.sampler = runtime.GenSampler(self.lowering, blocks, chains=opt.chains, seed=opt.seed)
         return self.sampler
+
+    def _shape(self, name):
+        lw = self.lowering
+        return lw.layout[name][1] if name in lw.layout else lw.zlayout[name][1]
 
     def init_point(self, strat, rng):
         """Initial values from each parameter's prior, in declaration order
@@ -557,16 +573,28 @@ class GenPlan:
         env = {k: (np.asarray(v) if not np.isscalar(v) else v) for k, v in self.hypers.items()}
         pt = {}
         for d in self.model.params:
-            shape = self.lowering.layout[d.name][1]
+            shape = self._shape(d.name)
             g = dict(env)
             if shape:
                 idx = np.meshgrid(*[np.arange(s) for s in shape], indexing="ij")
                 for (var, _, _), ix in zip(d.grids, idx):
                     g[var] = ix
-            args = [np.broadcast_to(np.asarray(_np_eval(a, g), np.float64), shape) if shape
-                    else float(_np_eval(a, g)) for a in d.exp[2]]
             size = shape if shape else None
             name = d.exp[1]
+            if name == "Categorical":
+                pmf = np.asarray(_np_eval(d.exp[2][0], g), np.float64)
+                v = adl.initCategorical(strat, pmf, rng, size=size)
+                pt[d.name] = np.asarray(v, np.int32) if shape else int(v)
+                env[d.name] = pt[d.name]
+                continue
+            args = [np.broadcast_to(np.asarray(_np_eval(a, g), np.float64), shape) if shape
+                    else float(_np_eval(a, g)) for a in d.exp[2]]
+            if name == "Bernoulli":
+                v = (np.zeros(shape, np.int32) if isinstance(strat, adl.AugurDefault) else
+                     (rng.random(size) < np.asarray(args[0])).astype(np.int32))
+                pt[d.name] = np.asarray(v, np.int32) if shape else int(v)
+                env[d.name] = pt[d.name]
+                continue
             if name == "Normal":
                 v = adl.initNormal(strat, args[0], args[1], rng, size=size)
             elif name == "Exponential":
@@ -590,13 +618,17 @@ class GenPlan:
     def set_point(self, pt, chain):
         for name in self.param_names:
             if name in pt:
-                self.sampler.set(name, np.asarray(pt[name], np.float64), chain)
+                dt = np.int32 if name in self.discrete else np.float64
+                self.sampler.set(name, np.asarray(pt[name], dt), chain)
 
     def get_point(self, chain):
         out = {}
         for name in self.param_names:
             v = self.sampler.get(name, chain)
-            out[name] = float(v[0]) if not self.lowering.layout[name][1] else v
+            if not self._shape(name):
+                out[name] = int(v[0]) if name in self.discrete else float(v[0])
+            else:
+                out[name] = v
         return out
 
 

@@ -290,61 +290,86 @@ class GmmSampler(Sampler):
 
 
 class GenSampler(Sampler):
-    """A generic continuous model: the data pass generated by
-    pyaugur.lower.Lowering, compiled with hipRTC, under HMC blocks."""
+    """A generic model: the passes generated by pyaugur.lower.Lowering,
+    compiled with hipRTC, under HMC and DiscGibbs blocks."""
 
     family = "gen"
 
     def __init__(self, lowering, blocks, chains=1, seed=0):
-        """blocks: list of (parameter names, sim_len, step_size, fixed_steps)."""
+        """blocks: list of (parameter names, sim_len, step_size, fixed_steps)
+        for HMC, or ("DiscGibbs", name) for a discrete parameter."""
         L.require_gpu()
         lw = lowering
         self.P = lw.P
+        self.NZ = getattr(lw, "NZ", 1)
         self.layout = dict(lw.layout)
+        self.zlayout = dict(getattr(lw, "zlayout", {}))
         self._src = lw.source.encode()
         self._kernel = lw.kernel_name.encode()
         self._reals, self._ints = L.f64(lw.reals), L.i32(lw.ints)
-        names = list(lw.param_names)
+        names = [n for n in lw.param_names if n in self.layout]
+        znames = [n for n in lw.param_names if n in self.zlayout]
         self._names = [n.encode() for n in names]
-        name_arr = (C.c_char_p * len(names))(*self._names)
-        offs = L.i32([lw.layout[n][0] for n in names])
-        sizes = L.i32([int(np.prod(lw.layout[n][1])) if lw.layout[n][1] else 1 for n in names])
-        masks = np.zeros((max(1, len(blocks)), self.P), np.uint8)
-        sim_len = np.zeros(max(1, len(blocks)))
-        eps = np.zeros(max(1, len(blocks)))
-        fixed = np.zeros(max(1, len(blocks)), np.int32)
-        for b, (vars_, sl, ep, fx) in enumerate(blocks):
+        self._znames = [n.encode() for n in znames]
+        name_arr = (C.c_char_p * max(1, len(names)))(*self._names)
+        zname_arr = (C.c_char_p * max(1, len(znames)))(*self._znames)
+
+        def _size(shape):
+            return int(np.prod(shape)) if shape else 1
+        offs = L.i32([self.layout[n][0] for n in names] or [0])
+        sizes = L.i32([_size(self.layout[n][1]) for n in names] or [1])
+        zoffs = L.i32([self.zlayout[n][0] for n in znames] or [0])
+        zsizes = L.i32([_size(self.zlayout[n][1]) for n in znames] or [1])
+        nb = max(1, len(blocks))
+        masks = np.zeros((nb, self.P), np.uint8)
+        sim_len, eps = np.ones(nb), np.ones(nb)
+        fixed, kinds = np.zeros(nb, np.int32), np.zeros(nb, np.int32)
+        for b, blk in enumerate(blocks):
+            if blk[0] == "DiscGibbs":
+                kinds[b] = L.GEN_DISC
+                continue
+            vars_, sl, ep, fx = blk
             for v in vars_:
-                off, shape = lw.layout[v]
-                masks[b, off:off + (int(np.prod(shape)) if shape else 1)] = 1
+                off, shape = self.layout[v]
+                masks[b, off:off + _size(shape)] = 1
             sim_len[b], eps[b], fixed[b] = sl, ep, fx
-        # the blocks' full-conditional passes, when the lowering generated
-        # them for exactly these blocks
+        # the blocks' own kernels, when the lowering generated them for
+        # exactly these blocks
         bk = getattr(lw, "block_kernels", [])
         if blocks and len(bk) == len(blocks):
-            kn = [n.encode() for n, _ in bk]
+            kn = [k[0].encode() for k in bk]
             kernel_arr = (C.c_char_p * len(kn))(*kn)
-            grids = L.i32([g for _, g in bk])
+            grids = L.i32([k[1] for k in bk])
             kptr, gptr = C.cast(kernel_arr, C.c_void_p).value, L.ptr(grids).value
+        elif any(k == L.GEN_DISC for k in kinds[:len(blocks)]):
+            raise ValueError("DiscGibbs blocks need the lowering's kernels for the same blocks")
         else:
             kn, kernel_arr, grids, kptr, gptr = [], None, None, None, None
-        self._keep = (name_arr, offs, sizes, masks, sim_len, eps, fixed, kn, kernel_arr, grids)
+        self._keep = (name_arr, zname_arr, offs, sizes, zoffs, zsizes, masks, sim_len, eps, fixed, kinds,
+                      kn, kernel_arr, grids)
         desc = L.GenDesc(self._src, self._kernel, self.P, lw.grid, chains, seed & 0xFFFFFFFF,
                          L.ptr(self._reals).value, self._reals.size, L.ptr(self._ints).value,
                          self._ints.size, len(names), C.cast(name_arr, C.c_void_p).value,
                          L.ptr(offs).value, L.ptr(sizes).value, len(blocks), L.ptr(masks).value,
-                         L.ptr(sim_len).value, L.ptr(eps).value, L.ptr(fixed).value, kptr, gptr)
+                         L.ptr(sim_len).value, L.ptr(eps).value, L.ptr(fixed).value, kptr, gptr,
+                         L.ptr(kinds).value, self.NZ, len(znames),
+                         C.cast(zname_arr, C.c_void_p).value, L.ptr(zoffs).value, L.ptr(zsizes).value)
         h = C.c_void_p()
         L.call("augur_gen_create", C.byref(desc), C.byref(h))
         self.n_blocks = len(blocks)
-        shapes = {"state": ((self.P,), np.float64)}
+        self.kinds = kinds[:len(blocks)].tolist()
+        shapes = {"state": ((self.P,), np.float64), "zstate": ((self.NZ,), np.int32)}
         for n in names:
-            shape = lw.layout[n][1]
+            shape = self.layout[n][1]
             shapes[n] = (shape if shape else (1,), np.float64)
+        for n in znames:
+            shape = self.zlayout[n][1]
+            shapes[n] = (shape if shape else (1,), np.int32)
         super().__init__(h, chains, shapes)
 
     def eval(self, q):
-        """(ll per chain, gradient chains x P) at the points q (chains x P)."""
+        """(log joint per chain, gradient chains x P) at the points q (chains x
+        P) and the current discrete values."""
         qa = L.f64(q).reshape(self.chains, self.P)
         ll = np.zeros(self.chains)
         g = np.zeros((self.chains, self.P))
@@ -352,7 +377,7 @@ class GenSampler(Sampler):
         return ll, g
 
     def acceptance(self):
-        """[block][chain] accepted / proposed."""
+        """[block][chain] accepted / proposed (HMC blocks; NaN for DiscGibbs)."""
         out = np.zeros((self.n_blocks, self.chains, 2))
         L.call("augur_stats", self._h, L.ptr(out), out.nbytes)
         with np.errstate(invalid="ignore"):

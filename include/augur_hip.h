@@ -275,12 +275,14 @@ augur_status augur_op_gmm_predictive_ll(int k, int d, const double* pis, const d
  *
  * `source` is HIP source that defines
  *   extern "C" __global__ void <kernel>(const double* q, double* part,
- *                                        const double* R, const int* I)
+ *                                        const double* R, const int* I,
+ *                                        const int* Z)
  * for a grid of (grid, chains) blocks of 256 threads.  For every g < grid
  * and chain c, one block writes part[(c * grid + g) * (n_params + 1) + j]:
  * a share of d ll / dq_j for j < n_params and of ll for j = n_params, at
- * chain c's point q[c * n_params ..); the grid shares of chain c sum to
- * its ll and gradient.  R and I are the `reals` and `ints` buffers.  The
+ * chain c's point q[c * n_params ..) and discrete values Z[c * n_zints ..);
+ * the grid shares of chain c sum to its ll and gradient.  R and I are the
+ * `reals` and `ints` buffers.  The
  * library compiles it with hipRTC for the device and runs HMC
  * (cbits/augur_mcmclib.h:343-424).  Each block may name its own pass of the
  * same signature over its full conditional (the terms that mention its
@@ -315,7 +317,19 @@ typedef struct {
   const char* const* block_kernels;  /* n_blocks names of the blocks' full-conditional
                                         passes in `source`, or null: the log joint's */
   const int* block_grids;    /* their grids */
+  const int* block_kinds;    /* n_blocks AUGUR_GEN_HMC | AUGUR_GEN_DISC, or null: all HMC */
+  int n_zints;               /* discrete coordinates per chain (the int32 vector "zstate") */
+  int n_znames;
+  const char* const* znames; /* discrete parameter names */
+  const int* zoffsets;
+  const int* zsizes;
 } augur_gen_desc;
+#define AUGUR_GEN_HMC 0
+/* DiscGibbs on one discrete parameter (Core/CgDisc.hs): block_kernels[b]
+ * names a kernel (const double* q, int* Z, const double* R, const int* I,
+ * unsigned seed, unsigned sweep, unsigned tag) that redraws every point of
+ * the parameter in Z from its full conditional; tag = 16 + 4b. */
+#define AUGUR_GEN_DISC 1
 augur_status augur_gen_create(const augur_gen_desc* desc, augur_sampler** out);
 /* ll (chains) and d ll / dq (chains x n_params) at the points q (chains x
  * n_params): one pass and its reduction; the sampler's state is unchanged.

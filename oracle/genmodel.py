@@ -26,6 +26,8 @@ for a model whose parameters are all continuous:
     complex steps, so no hand-written adjoint is involved.  Those
     expressions are analytic, and complex steps give their derivatives to
     machine precision.
+  DiscGibbs on finite discrete parameters (Categorical over a fixed pmf,
+    Bernoulli): Core/CgDisc.hs:120-165, in log space (disc_step).
   HMC: cbits/augur_mcmclib.h:343-424, each block on its full conditional
     (the declarations that mention its variables, Core/RwCore.hs:58-67).
     The momentum and leapfrog updates are the reference's.  The step size
@@ -64,6 +66,9 @@ def _where(cond, a, b):
 
 def dist_ll(name, pt, args, ref_compat):
     a = args
+    if name == "Categorical":  # cbits/augur_dist.c:100-104, pmf a fixed vector
+        with np.errstate(divide="ignore"):
+            return np.log(np.asarray(a[0], np.float64)[np.asarray(pt, np.int64)])
     if name == "Normal":
         mean, var = a
         t = pt - mean
@@ -153,7 +158,8 @@ def dist_dot(name, which, pt, args, ref_compat):
     raise ValueError(f"no derivative for {name}")
 
 
-INT_DISTS = {"Bernoulli", "Poisson"}
+INT_DISTS = {"Bernoulli", "Poisson", "Categorical"}
+DISC_DISTS = {"Bernoulli", "Categorical"}
 
 
 # ---- model interpretation -------------------------------------------------------
@@ -190,27 +196,39 @@ class GenModel:
             self.env[name] = np.asarray(v) if not np.isscalar(v) else v
         for d in model.data:
             self.env[d.name] = np.asarray(data[d.name])
-        self.layout = {}
-        off = 0
+        self.layout = {}       # continuous: name -> (offset in q, shape)
+        self.zlayout = {}      # discrete: name -> (offset in zq, shape)
+        off = zoff = 0
         for d in model.params:
             shape = tuple(int(self._eval_const(hi)) for (_, lo, hi) in d.grids)
             size = int(np.prod(shape)) if shape else 1
-            self.layout[d.name] = (off, shape)
-            off += size
-        self.P = off
+            if d.exp[1] in DISC_DISTS:
+                self.zlayout[d.name] = (zoff, shape)
+                zoff += size
+            else:
+                self.layout[d.name] = (off, shape)
+                off += size
+        self.P = max(off, 1)
+        self.NZ = max(zoff, 1)
         self.pnames = set(self.layout)
+        self.zq = np.zeros(self.NZ, np.int64)  # discrete values used by ll / grad / hmc_step
         self.terms = [d for d in model.decls if d.stochastic]
 
     def _eval_const(self, e):
         v = self._ev(e, self.env)
         return v
 
-    def unpack(self, q):
+    def unpack(self, q, zq=None):
         env = dict(self.env)
         for name, (off, shape) in self.layout.items():
             size = int(np.prod(shape)) if shape else 1
             v = q[off:off + size]
             env[name] = v.reshape(shape) if shape else v[0]
+        zq = self.zq if zq is None else zq
+        for name, (off, shape) in self.zlayout.items():
+            size = int(np.prod(shape)) if shape else 1
+            v = np.asarray(zq[off:off + size], np.int64)
+            env[name] = v.reshape(shape) if shape else int(v[0])
         return env
 
     # expressions, vectorised over the grid variables bound in env
@@ -329,6 +347,45 @@ class GenModel:
         u = u01(int(w[0]))
         acc = math.log(u) < (self.ll(x, block_vars) - self.ll(q, block_vars)) + (k0 - k1)
         return (x if acc else q.copy()), acc
+
+
+def disc_step(model, q, zq, name, seed, chain, sweep, block):
+    """DiscGibbs on the discrete parameter `name` (Core/CgDisc.hs:120-165):
+    at each point of its grid the log full conditional of every value (its
+    prior and the terms that use it at that point, all points at once), then
+    the inverse-CDF draw in log space with the maximum subtracted, from the
+    uniform of Philox counter (point, 0, sweep, 16 + 4 block) under key
+    (seed, chain).  Returns the new discrete vector."""
+    off, shape = model.zlayout[name]
+    n = int(np.prod(shape)) if shape else 1
+    d0 = model.model.decl(name)
+    K = 2 if d0.exp[1] == "Bernoulli" else len(np.asarray(model.env[d0.exp[2][0][1]]))
+    terms = [d for d in model.terms if d.name == name or (name in _free_vars(d.exp, set()))]
+    q = np.asarray(q, np.float64)
+    lp = np.zeros((n, K))
+    for k in range(K):
+        zk = np.array(zq, np.int64)
+        zk[off:off + n] = k
+        env = model.unpack(q, zk)
+        for d in terms:
+            _, pt, args, m = model._term_parts(d, env)
+            lp[:, k] += np.broadcast_to(dist_ll(d.exp[1], pt, args, model.ref_compat), (m,))
+    out = np.array(zq, np.int64)
+    tag = 16 + 4 * block
+    for f in range(n):
+        w = np.exp(lp[f] - lp[f].max())
+        tot = 0.0
+        for v in w:
+            tot += v
+        u = u01(int(orc.draw4(seed, chain, f, 0, sweep, tag)[0])) * tot
+        pick, cum = K - 1, 0.0
+        for k in range(K):
+            cum += w[k]
+            if u < cum:
+                pick = k
+                break
+        out[off + f] = pick
+    return out
 
 
 def u01(w):

@@ -81,3 +81,33 @@ def point(m, rng):
         else:
             q[off:off + size] = rng.normal(0.0, 0.7, size)
     return q
+
+
+MIX = """(K : Int, N : Int, pis : Vec Real, s2 : Real) => {
+  param mu[k] ~ Normal(0.0, 10.0)
+      for k <- 0 until K ;
+  param z[n] ~ Categorical(pis)
+      for n <- 0 until N ;
+  data y[n] ~ Normal(mu[z[n]], s2)
+      for n <- 0 until N ;
+}
+"""
+
+SPIKE = """(N : Int) => {
+  param p ~ Beta(1.0, 1.0) ;
+  param z[n] ~ Bernoulli(p)
+      for n <- 0 until N ;
+  data y[n] ~ Normal(z[n] * 3.0, 1.0)
+      for n <- 0 until N ;
+}
+"""
+
+
+def mixture(rng, n=600, centers=(-4.0, 0.0, 5.0), s2=1.0):
+    """A 1-D mixture with fixed weights and variance: hypers, data and the
+    generating assignment."""
+    K = len(centers)
+    pis = np.full(K, 1.0 / K)
+    z = rng.integers(0, K, n)
+    y = np.asarray(centers)[z] + rng.normal(0.0, np.sqrt(s2), n)
+    return dict(K=K, N=n, pis=pis, s2=s2), dict(y=y), z.astype(np.int32)

@@ -189,3 +189,101 @@ def test_poisson_gamma_model_through_the_drop_in():
     np.testing.assert_allclose(lam.mean(0), group_means, rtol=0.03)
     np.testing.assert_allclose(lam.mean(0), rates, rtol=0.12)
     assert np.all(acc > 0.3)
+
+
+# ---- discrete parameters: DiscGibbs --------------------------------------------
+
+def _mixture_pair(n, seed=21, ref_compat=True, blocks=None):
+    from tests.gen_models import MIX, mixture
+    hyp, dat, z = mixture(np.random.default_rng(seed), n)
+    model = parse_model(MIX)
+    lw = Lowering(model, hyp, dat, ref_compat, blocks=blocks)
+    return lw, gm.GenModel(model, hyp, dat, ref_compat), z
+
+
+def test_disc_gibbs_matches_oracle():
+    """Two DiscGibbs sweeps of the mixture's assignments from a given state
+    (two chains): every point's draw equals the oracle's (same Philox
+    uniforms, FP64 log-space inverse CDF)."""
+    print("generic DiscGibbs", flush=True)
+    lw, om, z0 = _mixture_pair(3000, blocks=[("DiscGibbs", ["z"])])
+    mus = [np.array([-3.5, 0.3, 4.2]), np.array([1.0, -1.0, 2.0])]
+    with runtime.GenSampler(lw, [("DiscGibbs", "z")], chains=2, seed=13) as s:
+        cur = []
+        for c in range(2):
+            s.set("mu", mus[c], c)
+            s.set("z", z0, c)
+            cur.append(z0.astype(np.int64))
+        for sweep in range(2):
+            s.step(1)
+            for c in range(2):
+                cur[c] = gm.disc_step(om, mus[c], cur[c], "z", 13, c, sweep, 0)
+                np.testing.assert_array_equal(s.get("z", c), cur[c])
+        assert len(np.unique(cur[0])) == 3
+
+
+def test_mixture_sweeps_match_oracle():
+    """HMC [mu] (*) DiscGibbs [z], three sweeps of two chains: the means after
+    each sweep (rtol 1e-9) and every assignment equal the oracle's, and the
+    log joint at the end matches it."""
+    print("generic mixture sweeps", flush=True)
+    blocks = [["mu"], ("DiscGibbs", ["z"])]
+    lw, om, z0 = _mixture_pair(2000, blocks=blocks)
+    mu0 = np.array([-3.0, 0.5, 4.0])
+    with runtime.GenSampler(lw, [(["mu"], 0.3, 0.03, 0), ("DiscGibbs", "z")], chains=2, seed=17) as s:
+        q = [mu0.copy(), mu0.copy() + 0.2]
+        zq = [z0.astype(np.int64), z0.astype(np.int64)]
+        for c in range(2):
+            s.set("mu", q[c], c)
+            s.set("z", z0, c)
+        for sweep in range(3):
+            s.step(1)
+            for c in range(2):
+                om.zq = zq[c]
+                q[c], _ = om.hmc_step(q[c], np.ones(3, bool), 0.3, 0.03, 0, 17, c, sweep, 0, block_vars=["mu"])
+                zq[c] = gm.disc_step(om, q[c], zq[c], "z", 17, c, sweep, 1)
+                np.testing.assert_allclose(s.get("mu", c), q[c], rtol=1e-9)
+                np.testing.assert_array_equal(s.get("z", c), zq[c])
+        ll = s.log_density()
+        for c in range(2):
+            om.zq = zq[c]
+            assert ll[c] == pytest.approx(om.ll(q[c]), rel=1e-11)
+
+
+def test_mixture_through_the_drop_in():
+    """A 1-D mixture with fixed weights (no family covers it): the default
+    schedule (HMC on the means, DiscGibbs on the assignments) recovers the
+    centres, up to label order."""
+    from tests.gen_models import MIX, mixture
+    centers = np.array([-4.0, 0.0, 5.0])
+    hyp, dat, _ = mixture(np.random.default_rng(22), 900, centers=centers)
+    with AugurInfer(None, MIX) as infer:
+        infer.set_compile_opt(AugurOpt(target="gpu", seed=8, chains=2, ref_compat=False,
+                                       initStrat=AugurUser({"mu": np.array([-1.0, 0.5, 1.0]),
+                                                            "z": np.zeros(900, np.int32)})))
+        infer.set_user_sched("HMC [mu] [0.3, 0.03] (*) DiscGibbs [z]")
+        infer.compile(hyp["K"], hyp["N"], hyp["pis"], hyp["s2"])(dat["y"])
+        assert infer.plan.family == "gen"
+        s = infer.samplen(burnIn=300, numSamples=300)
+    mu = np.sort(np.array(s["mu"]).reshape(-1, 3), axis=1)
+    np.testing.assert_allclose(mu.mean(0), centers, atol=0.25)
+
+
+def test_bernoulli_indicator_model():
+    """z[n] ~ Bernoulli(p) used as a value (y[n] ~ Normal(3 z[n], 1)), p ~
+    Beta(1, 1): the posterior mean of p is near the generating fraction."""
+    from tests.gen_models import SPIKE
+    rng = np.random.default_rng(23)
+    N, frac = 2000, 0.3
+    zt = (rng.random(N) < frac).astype(np.int32)
+    y = 3.0 * zt + rng.standard_normal(N)
+    with AugurInfer(None, SPIKE) as infer:
+        infer.set_compile_opt(AugurOpt(target="gpu", seed=9, chains=2, ref_compat=False,
+                                       initStrat=AugurUser({"p": 0.5, "z": np.zeros(N, np.int32)})))
+        infer.set_user_sched("HMC [p] [0.05, 0.005] (*) DiscGibbs [z]")
+        infer.compile(N)(y)
+        s = infer.samplen(burnIn=200, numSamples=300)
+    p = np.array(s["p"]).ravel()
+    assert abs(p.mean() - zt.mean()) < 0.05
+    z_last = np.array(s["z"][-1][0])
+    assert (z_last == zt).mean() > 0.85

@@ -196,3 +196,55 @@ def test_generated_source_compiles_for_gfx950(name):
         lw = Lowering(parse_model(src), hyp, dat, ref_compat)
         r, log, size = _hiprtc_compile(lw.source)
         assert r == 0 and size > 0, log[-2000:]
+
+
+# ---- discrete parameters (DiscGibbs) ------------------------------------------
+
+def test_disc_step_draws_the_exact_conditional():
+    """One point of the mixture, fixed means: p(z = k | y) is proportional
+    to pis_k N(y | mu_k, s2).  The oracle's DiscGibbs draws over 4000 sweeps
+    (fresh Philox counters) match those probabilities within 4 standard
+    errors; a wrong normalisation, a shifted bin or a dropped term fails."""
+    from tests.gen_models import MIX
+    hyp = dict(K=3, N=1, pis=np.array([0.2, 0.5, 0.3]), s2=2.0)
+    y = np.array([0.7])
+    m = gm.GenModel(parse_model(MIX), hyp, dict(y=y), ref_compat=True)
+    mu = np.array([-1.0, 0.5, 2.0])
+    w = hyp["pis"] * np.exp(-0.5 * (y[0] - mu) ** 2 / hyp["s2"])
+    prob = w / w.sum()
+    counts = np.zeros(3)
+    zq = np.zeros(1, np.int64)
+    for sweep in range(4000):
+        zq = gm.disc_step(m, mu, zq, "z", seed=5, chain=0, sweep=sweep, block=1)
+        counts[zq[0]] += 1
+    freq = counts / counts.sum()
+    se = np.sqrt(prob * (1 - prob) / 4000)
+    assert np.all(np.abs(freq - prob) < 4 * se), (freq, prob)
+
+
+def test_disc_lowering_errors():
+    from tests.gen_models import MIX
+    hyp = dict(K=3, N=4, pis=np.full(3, 1 / 3), s2=1.0)
+    dat = dict(y=np.zeros(4))
+    # a term using z at another point cannot be drawn point by point
+    src = MIX.replace("Normal(mu[z[n]], s2)", "Normal(mu[z[n]] + mu[z[0]], s2)")
+    with pytest.raises(LoweringError, match="another point"):
+        Lowering(parse_model(src), hyp, dat, blocks=[["mu"], ("DiscGibbs", ["z"])])
+    # the pmf of a discrete parameter is a fixed vector
+    src = MIX.replace("Categorical(pis)", "Categorical(mu)")
+    with pytest.raises(LoweringError, match="pmf"):
+        Lowering(parse_model(src), hyp, dat)
+    with pytest.raises(LoweringError, match="DiscGibbs updates"):
+        Lowering(parse_model(MIX), hyp, dat, blocks=[("DiscGibbs", ["mu"])])
+
+
+@pytest.mark.skipif(not os.path.exists(HIPRTC), reason="hipRTC is not installed")
+def test_discrete_models_compile_for_gfx950():
+    from tests.gen_models import MIX, SPIKE, mixture
+    hyp, dat, _ = mixture(np.random.default_rng(3), 200)
+    for src, h, d, blocks in ((MIX, hyp, dat, [["mu"], ("DiscGibbs", ["z"])]),
+                              (SPIKE, dict(N=50), dict(y=np.zeros(50)), [["p"], ("DiscGibbs", ["z"])])):
+        lw = Lowering(parse_model(src), h, d, blocks=blocks)
+        assert [k[2] for k in lw.block_kernels] == ["HMC", "DiscGibbs"]
+        r, log, size = _hiprtc_compile(lw.source)
+        assert r == 0 and size > 0, log[-2000:]
