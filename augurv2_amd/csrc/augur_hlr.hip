@@ -984,6 +984,7 @@ struct HlrSampler final : augur_sampler {
   int n = 0, d = 0, cp = 1, has_b = 0, ref_compat = 1, prior_kind = 0;
   double prior_a = 1.0, prior_b = 1.0;
   int G = 1;
+  int G64 = 1;                     // grid of the FP64 log-density (launch_ll64)
   PassKernel fast_pass = nullptr;  // specialised pass for (d, cp), if compiled
   size_t fast_smem = 0;            // its dynamic LDS bytes
   std::vector<augur_hmc_block> sched;
@@ -1087,7 +1088,7 @@ struct HlrSampler final : augur_sampler {
   }
 
   Ll64Args ll64_args(const double* point, double* out) {
-    return Ll64Args{x.p, y.p, n, d, cp, has_b, d + 2, point, ll64_part.p, G, out};
+    return Ll64Args{x.p, y.p, n, d, cp, has_b, d + 2, point, ll64_part.p, G64, out};
   }
 
   // Fits the surrogate to the captured pairs (standardisation, least-squares
@@ -1357,6 +1358,11 @@ struct HlrSampler final : augur_sampler {
     }
     if (const char* e = getenv("AUGUR_HLR_BLOCKS")) want = std::max(1, atoi(e));
     G = std::max(1, std::min(tiles, want));
+    {
+      int cus = 0;
+      AUGUR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+      G64 = ll64_grid(n, d, cp, has_b, G, cus);
+    }
     const size_t P = (size_t)d + 2, ds = (size_t)d + 1;
     // X and y padded with zero rows to a multiple of kHlrRowPad (the data
     // overwrite the first n rows)
@@ -1501,6 +1507,7 @@ augur_status augur_op_hlr_ll_grad(int n_rows, int d, int chains, const float* x,
     if (!x || !y || !beta || !ll || !grad) fail(AUGUR_ERR_ARG, "null argument");
     require_device();
     HlrSampler s;
+    AUGUR_HIP(hipGetDevice(&s.device));
     AUGUR_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
     s.has_b = has_intercept;
     s.alloc(n_rows, d, chains);
@@ -1566,6 +1573,7 @@ augur_status augur_op_hlr_ll64(int n_rows, int d, int chains, const float* x, co
     if (!x || !y || !beta || !ll) fail(AUGUR_ERR_ARG, "null argument");
     require_device();
     HlrSampler s;
+    AUGUR_HIP(hipGetDevice(&s.device));
     AUGUR_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
     s.has_b = has_intercept;
     s.alloc(n_rows, d, chains);
@@ -1576,7 +1584,7 @@ augur_status augur_op_hlr_ll64(int n_rows, int d, int chains, const float* x, co
     for (int c = 0; c < chains; ++c)
       for (int j = 0; j <= d; ++j) st[(size_t)c * P + j] = (j < d || has_intercept) ? beta[(size_t)c * (d + 1) + j] : 0.0;
     s.state.upload(st.data(), st.size(), s.stream);
-    s.ll64_part.alloc((size_t)s.cp * s.G);
+    s.ll64_part.alloc((size_t)s.cp * s.G64);
     s.ll64_cur.alloc(s.cp);
     launch_ll64(s.ll64_args(s.state.p, s.ll64_cur.p), s.stream);
     std::vector<double> l(s.cp);
@@ -1612,7 +1620,7 @@ augur_status augur_hlr_surrogate(augur_sampler* s, const augur_surrogate_cfg* cf
     h->inf.alloc(h->d, c.hidden);
     h->ll64_cur.alloc(h->cp);
     h->ll64_prop.alloc(h->cp);
-    h->ll64_part.alloc((size_t)h->cp * h->G);
+    h->ll64_part.alloc((size_t)h->cp * h->G64);
     h->mlp.init(h->d, c.hidden, c.batch, c.seed, h->stream);
     AUGUR_HIP(hipStreamSynchronize(h->stream));
     h->surr_phase = HlrSampler::SURR_COLLECT;

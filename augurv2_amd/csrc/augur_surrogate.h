@@ -66,6 +66,10 @@ struct SurrInfer {
 };
 
 void launch_surr_traj(const SurrTrajArgs& a, int chains, hipStream_t st);
+// Grid of launch_ll64 for these shapes (generic_grid when the matrix-core
+// form does not apply); part must hold cp x that many doubles.
+bool ll64_mfma_form(int d, int cp, int has_b);
+int ll64_grid(int n, int d, int cp, int has_b, int generic_grid, int cus);
 void launch_ll64(const Ll64Args& a, hipStream_t st);
 
 // In place: x = (x - mx) * ix, t = (t - mt) * it, stats = [mx | ix | mt | it].

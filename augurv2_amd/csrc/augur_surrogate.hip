@@ -281,6 +281,59 @@ void launch_surr_traj(const SurrTrajArgs& a, int chains, hipStream_t st) {
 // ---------------------------------------------------------------------------
 // Data log-density sum_n log sigmoid(+-(x_n . theta + b)) in FP64 arithmetic.
 
+// log(1 + e^-x) for x >= 0, within 2.1e-16 absolute of the exact value (a
+// 4M-point sweep of [0, 40] against long double).  The terms are summed, so
+// absolute accuracy is what the sum needs; this form takes about 40 FP64
+// operations where exp followed by the library's log1p takes about 100.
+//   e^-x: -x = n ln2 + r with ln2 split in two parts (Cody & Waite), |r| <=
+//     ln2 / 2, e^r by its Taylor series to r^12 (remainder < 2e-16), then
+//     scaled by 2^n.
+//   log1p(t), t in (0, 1]: log(1 + f) = 2 atanh(f / (2 + f)); for t above
+//     sqrt2 - 1 take log 2 + log(1 + f) with f = (t - 1) / 2 instead, so that
+//     |s| = |f / (2 + f)| <= 0.172 and the odd series 2 (s + s^3/3 + ...
+//     + s^19/19) is complete to 1e-17.
+// x above 746 (e^-x below the smallest subnormal) is clamped; NaN passes.
+__device__ inline double log1p_exp_neg(double x) {
+  x = x > 746.0 ? 746.0 : x;
+  constexpr double kLog2e = 1.4426950408889634, kLn2Hi = 6.93147180369123816490e-01,
+                   kLn2Lo = 1.90821492927058770002e-10;
+  const double nn = rint(-x * kLog2e);
+  double r = fma(nn, -kLn2Hi, -x);
+  r = fma(nn, -kLn2Lo, r);
+  double p = 1.0 / 479001600.0;
+  p = fma(p, r, 1.0 / 39916800.0);
+  p = fma(p, r, 1.0 / 3628800.0);
+  p = fma(p, r, 1.0 / 362880.0);
+  p = fma(p, r, 1.0 / 40320.0);
+  p = fma(p, r, 1.0 / 5040.0);
+  p = fma(p, r, 1.0 / 720.0);
+  p = fma(p, r, 1.0 / 120.0);
+  p = fma(p, r, 1.0 / 24.0);
+  p = fma(p, r, 1.0 / 6.0);
+  p = fma(p, r, 0.5);
+  p = fma(p, r, 1.0);
+  p = fma(p, r, 1.0);
+  const double t = ldexp(p, (int)nn);
+  const bool hi = t > 0.41421356237309503;
+  const double f = hi ? 0.5 * (t - 1.0) : t;
+  const double sr = f / (2.0 + f);
+  const double s2 = sr * sr;
+  double q = 1.0 / 19;
+  q = fma(q, s2, 1.0 / 17);
+  q = fma(q, s2, 1.0 / 15);
+  q = fma(q, s2, 1.0 / 13);
+  q = fma(q, s2, 1.0 / 11);
+  q = fma(q, s2, 1.0 / 9);
+  q = fma(q, s2, 1.0 / 7);
+  q = fma(q, s2, 1.0 / 5);
+  q = fma(q, s2, 1.0 / 3);
+  const double l = 2.0 * fma(sr * s2, q, sr);
+  return hi ? l + 0.69314718055994531 : l;
+}
+
+// log sigmoid(u) = min(u, 0) - log(1 + e^-|u|)
+__device__ inline double log_sigmoid64(double u) { return fmin(u, 0.0) - log1p_exp_neg(fabs(u)); }
+
 // Generic form: 64-row tiles of X staged in LDS, one thread per (row, chain).
 __global__ __launch_bounds__(256) void k_ll64(Ll64Args A) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -310,7 +363,7 @@ __global__ __launch_bounds__(256) void k_ll64(Ll64Args A) {
       double t = A.has_b ? pr[d] : 0.0;
       for (int j = 0; j < d; ++j) t = fma((double)xr[j], pr[j], t);
       const double st = ys[r] ? t : -t;
-      acc += fmin(st, 0.0) - log1p(exp(-fabs(t)));
+      acc += log_sigmoid64(st);
     }
   }
   // thread tid's rows all belong to chain tid % cp (cp divides 256)
@@ -325,6 +378,7 @@ __global__ __launch_bounds__(256) void k_ll64(Ll64Args A) {
 }
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef float v4f_t __attribute__((ext_vector_type(4)));
 
 // FP64 matrix-core form for D % 16 == 0 without intercept: each wave takes
 // 16-row tiles and computes the 16 x 16 block (rows x chains) of
@@ -336,60 +390,76 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 // Output column l % 16 is chain l % 16, whichever rows the lane holds, and
 // log sigmoid(s t) is summed per chain.  Rows past the last full tile are
 // left to k_ll64_finish.
+//
+// The chain table sits in LDS (row stride D + 2 doubles: 16-byte aligned
+// pairs, and the 16 chain rows of one read fall on distinct banks), which
+// keeps the kernel near 100 VGPRs and four waves per SIMD; X is read once
+// with nontemporal loads when it is far beyond the Infinity Cache.
 template <int D>
-__global__ __launch_bounds__(256) void k_ll64_mfma(Ll64Args A) {
-  constexpr int Q = D / 16;
+__global__ __launch_bounds__(256) void k_ll64_mfma(Ll64Args A, int x_stream) {
+  constexpr int Q = D / 16, LD = D + 2;
+  __shared__ __align__(16) double th[16 * LD];
   __shared__ double red[4][16];
+  for (int j = threadIdx.x; j < 16 * D; j += 256) {
+    const int c = j / D, k = j - c * D;
+    th[c * LD + k] = c < A.cp ? A.point[(size_t)c * A.P + k] : 0.0;
+  }
+  __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int i = lane & 15, kb = lane >> 4;
-  double bq[Q][4];
-  const bool live = i < A.cp;
-#pragma unroll
-  for (int q = 0; q < Q; ++q)
-#pragma unroll
-    for (int e = 0; e < 4; ++e) bq[q][e] = live ? A.point[(size_t)i * A.P + 4 * (4 * q + kb) + e] : 0.0;
+  const double2* tb = reinterpret_cast<const double2*>(th + i * LD + 4 * kb);
   double lsum = 0.0;
   const int tiles = A.n / 16;
   // tile t: rows 16 t + i, this lane's features 16 q + 4 kb .. + 3; the next
   // tile of the wave is requested before this one's products (one in flight)
   const int tstep = gridDim.x * 4;
-  float4 xn[Q];
+  v4f_t xn[Q];
   int yn = 0;
   auto load = [&](int t) {
-    const float4* xr = reinterpret_cast<const float4*>(A.x + (size_t)(t * 16 + i) * D);
+    const v4f_t* xr = reinterpret_cast<const v4f_t*>(A.x + (size_t)(t * 16 + i) * D) + kb;
+    if (x_stream) {
 #pragma unroll
-    for (int q = 0; q < Q; ++q) xn[q] = xr[4 * q + kb];
+      for (int q = 0; q < Q; ++q) xn[q] = __builtin_nontemporal_load(xr + 4 * q);
+    } else {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) xn[q] = xr[4 * q];
+    }
     yn = A.y[t * 16 + i];
   };
   int t = blockIdx.x * 4 + wv;
   if (t < tiles) load(t);
   for (; t < tiles; t += tstep) {
-    float4 xc[Q];
+    v4f_t xc[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) xc[q] = xn[q];
     const float sg = yn ? 1.0f : -1.0f;
     if (t + tstep < tiles) load(t + tstep);
+    // the table is read again on every tile: an opaque zero offset keeps the
+    // compiler from hoisting the reads into 64 loop-invariant VGPRs
+    int z = 0;
+    asm volatile("" : "+s"(z));
+    const double2* tz = tb + z;
     f64x4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-      const float4 xv = xc[q];
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)(sg * xv.x), bq[q][0], acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)(sg * xv.y), bq[q][1], acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)(sg * xv.z), bq[q][2], acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)(sg * xv.w), bq[q][3], acc, 0, 0, 0);
+      const v4f_t xv = xc[q];
+      const double2 b01 = tz[8 * q], b23 = tz[8 * q + 1];
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)(sg * xv.x), b01.x, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)(sg * xv.y), b01.y, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)(sg * xv.z), b23.x, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64((double)(sg * xv.w), b23.y, acc, 0, 0, 0);
     }
     if (A.cp <= 8) {
       // columns 8..15 hold no chain: lane i + 8 takes rows 2, 3 of lane i's
       // column, so each lane evaluates two FP64 row terms instead of four
       const double u2 = __shfl_xor(acc[2], 8), u3 = __shfl_xor(acc[3], 8);
       const double a0 = i < 8 ? acc[0] : u2, a1 = i < 8 ? acc[1] : u3;
-      lsum += fmin(a0, 0.0) - log1p(exp(-fabs(a0)));
-      lsum += fmin(a1, 0.0) - log1p(exp(-fabs(a1)));
+      lsum += log_sigmoid64(a0);
+      lsum += log_sigmoid64(a1);
     } else {
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        const double u = acc[k];
-        lsum += fmin(u, 0.0) - log1p(exp(-fabs(u)));
+        lsum += log_sigmoid64(acc[k]);
       }
     }
   }
@@ -416,20 +486,34 @@ __global__ void k_ll64_finish(Ll64Args A, int tail) {
     for (int j = lane; j < A.d; j += 64) t = fma((double)A.x[(size_t)r * A.d + j], pt[j], t);
     t = wave_sum(t) + (A.has_b ? pt[A.d] : 0.0);
     const double st = A.y[r] ? t : -t;
-    s += fmin(st, 0.0) - log1p(exp(-fabs(t)));
+    s += log_sigmoid64(st);
   }
   if (lane == 0) A.out[c] = s;
 }
 
+bool ll64_mfma_form(int d, int cp, int has_b) {
+  return !has_b && cp <= 16 && (d == 16 || d == 32 || d == 64 || d == 128);
+}
+
+int ll64_grid(int n, int d, int cp, int has_b, int generic_grid, int cus) {
+  if (!ll64_mfma_form(d, cp, has_b)) return generic_grid;
+  // four blocks of four waves per CU (four waves per SIMD), or one 16-row
+  // tile per wave when there are fewer tiles
+  const int tiles = n / 16;
+  return std::max(1, std::min((tiles + 3) / 4, 4 * std::max(cus, 1)));
+}
+
 void launch_ll64(const Ll64Args& a, hipStream_t st) {
   int tail = a.n;
-  if (!a.has_b && a.cp <= 16 && (a.d == 16 || a.d == 32 || a.d == 64 || a.d == 128)) {
+  if (ll64_mfma_form(a.d, a.cp, a.has_b)) {
     tail = (a.n / 16) * 16;
+    // nontemporal X loads once X passes the 256 MB Infinity Cache
+    const int xs = (size_t)a.n * a.d * sizeof(float) > ((size_t)256 << 20);
     switch (a.d) {
-      case 16: k_ll64_mfma<16><<<a.G, 256, 0, st>>>(a); break;
-      case 32: k_ll64_mfma<32><<<a.G, 256, 0, st>>>(a); break;
-      case 64: k_ll64_mfma<64><<<a.G, 256, 0, st>>>(a); break;
-      default: k_ll64_mfma<128><<<a.G, 256, 0, st>>>(a); break;
+      case 16: k_ll64_mfma<16><<<a.G, 256, 0, st>>>(a, xs); break;
+      case 32: k_ll64_mfma<32><<<a.G, 256, 0, st>>>(a, xs); break;
+      case 64: k_ll64_mfma<64><<<a.G, 256, 0, st>>>(a, xs); break;
+      default: k_ll64_mfma<128><<<a.G, 256, 0, st>>>(a, xs); break;
     }
   } else {
     const int ds = a.d + 1;

@@ -205,6 +205,26 @@ def test_hlr_ll64_vs_fp64(ops, oracle, n, d, chains, intercept):
         assert abs(ll[c] - ll_ref) <= 1e-11 * abs(ll_ref), (c, ll[c], ll_ref)
 
 
+@pytest.mark.parametrize("n,d,chains", [(4099, 16, 4), (2000, 128, 8), (1600, 64, 16)])
+def test_hlr_ll64_terms_over_a_wide_range(ops, oracle, n, d, chains):
+    """Row terms log sigmoid(+-t) with t spread over [-60, 60] (both branches
+    of the FP64 log(1 + e^-x) evaluation, its switch at x = 0.8814, and terms
+    far below one ulp of the sum): each chain's sum within 1e-13 relative of
+    the oracle's libm form."""
+    rng = np.random.default_rng(n + d)
+    x = (rng.standard_normal((n, d)) * 0.01).astype(np.float32)
+    x[:, 0] = np.linspace(-60.0, 60.0, n).astype(np.float32)
+    y = (rng.random(n) < 0.5).astype(np.int32)
+    beta = np.zeros((chains, d + 1))
+    beta[:, 0] = (np.arange(chains) + 1.0) / chains
+    beta[:, 1:d] = rng.standard_normal((chains, d - 1)) * 0.3
+    ll = ops.hlr_ll64(x, y, beta, False)
+    m = oracle.Hlr(x.astype(np.float64), y)
+    for c in range(chains):
+        ll_ref = m.data_ll(np.concatenate([beta[c], [1.0]]))
+        assert abs(ll[c] - ll_ref) <= 1e-13 * abs(ll_ref), (c, ll[c], ll_ref)
+
+
 def _gmm_params(rng, k, d):
     pis = rng.dirichlet(np.ones(k))
     mu = rng.standard_normal((k, d)) * 3
