@@ -60,6 +60,11 @@ class SurrogateCfg(C.Structure):
                 ("linear_only", C.c_int)]
 
 
+class BsurrCfg(C.Structure):
+    _fields_ = [("train_sweeps", C.c_int), ("ridge", C.c_double), ("min_accept", C.c_double),
+                ("window", C.c_int)]
+
+
 class GmmDesc(C.Structure):
     _fields_ = [("n_points", C.c_int), ("dim", C.c_int), ("n_components", C.c_int),
                 ("chains", C.c_int), ("seed", C.c_uint32), ("y", C.c_void_p),
@@ -114,6 +119,11 @@ SIGNATURES = {
     "augur_bhlr_tuning": [_H, _P, _P, _P, _P, _I],
     "augur_bhlr_adapt": [_H, _I, _D],
     "augur_bhlr_step_sizes": [_H, _P, _P, _I],
+    "augur_bhlr_surrogate": [_H, _P],
+    "augur_bhlr_surrogate_info": [_H, _P, _S],
+    "augur_bhlr_surrogate_modes": [_H, _P, _I],
+    "augur_bhlr_surrogate_map": [_H, _I, _P, _P, _P],
+    "augur_bhlr_surrogate_pairs": [_H, _I, _P, _P, _I, _P],
     "augur_lda_predictive_ll": [_H, _I, _P, _P, _P],
     "augur_gmm_predictive_ll": [_H, _I, _P, _P],
     "augur_op_lda_predictive_ll": [_I, _I, _I, _P, _P, _P, _P, _P],

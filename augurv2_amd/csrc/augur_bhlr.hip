@@ -21,6 +21,7 @@
 // replayed: every kernel reads the sweep counter from device memory.
 //
 // Philox chain key of (dataset i, chain c) is i * C + c.
+#include <chrono>
 #include <type_traits>
 
 #include "augur_hmc.h"
@@ -37,6 +38,7 @@ struct BDs {
   int soff;        // offset in the per-chain state vector ([theta, b, v] per dataset)
   int goff;        // offset in the per-chain (d + 1)-wide vectors
   int voff;        // first global value index of this dataset
+  int woff;        // offset of its D x D surrogate map (sum of D_i'^2 before it)
   const uint16_t* x;
   const uint8_t* y;
 };
@@ -85,9 +87,25 @@ struct BArgs {
   double* acc_theta;
   double* acc_v;
   int x_stream;  // nontemporal X loads (all datasets' X is far beyond the Infinity Cache)
+  // linear gradient surrogate (null: off).  smode[i] = 1: dataset i runs
+  // surrogate trajectories and evaluates only the last pass (its proposal).
+  // cap_t / cap_g: this pass's capture slot ([C][GW]) for the fit, or null.
+  // sw: dataset i's map W (D x D FP32 at ds.woff, row = output coordinate);
+  // sm / sg ([GW]): mean theta and mean data gradient of the fitting pairs.
+  const int* smode;
+  float* cap_t;
+  float* cap_g;
+  const float* sw;
+  const double* sm;
+  const double* sg;
 };
 
 enum { BMODE_CACHE = 0, BMODE_LEAPFROG = 1 };
+
+// A surrogate dataset's blocks skip the leapfrog passes before the last.
+__device__ inline bool bsurr_skip(const BArgs& A, int di) {
+  return A.smode && A.mode == BMODE_LEAPFROG && A.l < A.steps && A.smode[di];
+}
 
 // Nontemporal 16- / 8-byte loads of packed BF16 X.
 typedef unsigned v4u_t __attribute__((ext_vector_type(4)));
@@ -292,6 +310,7 @@ __global__ __launch_bounds__(256) void k_bhlr_finish(BArgs A) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= A.nval) return;
   const int di = A.val_ds[v];
+  if (bsurr_skip(A, di)) return;
   const BDs ds = A.ds[di];
   const int D = ds.d, VS = D + 2;
   const int lv = v - ds.voff;
@@ -313,6 +332,13 @@ __global__ __launch_bounds__(256) void k_bhlr_finish(BArgs A) {
   }
   const int L = A.steps;
   if (A.l == L) A.gprop[gi] = s;
+  // a surrogate dataset's last pass only evaluates its proposal: the
+  // trajectory kernel has already moved theta and the momentum
+  if (A.smode && A.smode[di]) return;
+  if (A.cap_t) {  // a fitting pair: the point this pass evaluated, its data gradient
+    A.cap_t[gi] = A.beta32[gi];
+    A.cap_g[gi] = (float)s;
+  }
   const size_t si = (size_t)c * A.SW + ds.soff;
   const double var = A.state[si + D + 1];
   double xv = A.prop[si + j];
@@ -503,7 +529,9 @@ __device__ void bhlr_body_r(const BArgs& A, const BDs& ds, unsigned char* smem) 
 template <int CP>
 __global__ __launch_bounds__(kBThreads) void k_bhlr_pass_r(BArgs A) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const BDs ds = A.ds[A.blk_ds[blockIdx.x]];
+  const int di = A.blk_ds[blockIdx.x];
+  if (bsurr_skip(A, di)) return;
+  const BDs ds = A.ds[di];
   switch (ds.d) {
     case 16: bhlr_body_r<16, CP>(A, ds, smem); break;
     case 32: bhlr_body_r<32, CP>(A, ds, smem); break;
@@ -522,6 +550,7 @@ template <int CP>
 __global__ __launch_bounds__(kBThreads) void k_bhlr_pass(BArgs A) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int di = A.blk_ds[blockIdx.x];
+  if (bsurr_skip(A, di)) return;
   const BDs ds = A.ds[di];
   switch (ds.d) {
     case 16: bhlr_body<16, CP>(A, ds, di, smem); break;
@@ -736,7 +765,9 @@ __device__ void bhlr_body_mfma(const BArgs& A, const BDs& ds, unsigned char* sme
 
 __global__ __launch_bounds__(kBThreads) void k_bhlr_pass_mfma(BArgs A) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const BDs ds = A.ds[A.blk_ds[blockIdx.x]];
+  const int di = A.blk_ds[blockIdx.x];
+  if (bsurr_skip(A, di)) return;
+  const BDs ds = A.ds[di];
   switch (ds.d) {
     case 16: bhlr_body_mfma<16>(A, ds, smem); break;
     case 32: bhlr_body_mfma<32>(A, ds, smem); break;
@@ -762,6 +793,7 @@ size_t bhlr_smem_bytes(int dmax, int cp) {
 // One wave per (dataset, chain): momentum draw, half step, first position step.
 __global__ __launch_bounds__(64) void k_bhmc_begin(BArgs A) {
   const int di = blockIdx.x, c = blockIdx.y, lane = threadIdx.x;
+  if (A.smode && A.smode[di]) return;  // k_bsurr_traj moves this dataset
   const BDs ds = A.ds[di];
   const int D = ds.d;
   const uint32_t gc = (uint32_t)(di * A.C + c);
@@ -896,6 +928,216 @@ __global__ void k_bstate_to_beta32(BArgs A) {
     A.beta32[(size_t)c * A.GW + ds.goff + j] = (float)A.state[(size_t)c * A.SW + ds.soff + j];
 }
 
+// ---------------------------------------------------------------------------
+// Linear gradient surrogate.  Each dataset's data gradient is fitted by least
+// squares on the (theta, gradient) pairs its exact leapfrog passes produced:
+// g(theta) ~ gbar + W (theta - m), m and gbar the pairs' means.  Trajectories
+// then use that gradient plus the exact prior gradient, and one exact pass
+// at the proposal feeds the unchanged FP64 Metropolis test (k_bhmc_end), so
+// the target is the exact one whatever W is.
+
+// One block per (dataset, chain) in surrogate mode: momentum draw (the same
+// Philox stream as k_bhmc_begin), then L leapfrog steps with the surrogate
+// gradient; W sits in LDS (row stride D + 1).  Thread (o, g) with o = tid /
+// segs, g = tid % segs takes row o's columns g * cps .. + cps - 1 of the
+// matrix-vector product; the row's segs lanes are adjacent, summed by a
+// butterfly; lane g = 0 owns theta_o and p_o.
+__global__ __launch_bounds__(256) void k_bsurr_traj(BArgs A) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int di = blockIdx.x, c = blockIdx.y, tid = threadIdx.x;
+  if (!A.smode[di]) return;
+  const BDs ds = A.ds[di];
+  const int D = ds.d, segs = 256 / D, cps = D / segs;
+  const int o = tid / segs, g = tid % segs;
+  float* wl = (float*)smem;            // D x (D + 1)
+  float* df = wl + D * (D + 1);        // D: theta - m, FP32
+  double* kin = (double*)(((uintptr_t)(df + D) + 15) & ~(uintptr_t)15);  // D
+  const float* w = A.sw + ds.woff;
+  for (int e = tid; e < D * D; e += 256) wl[(e / D) * (D + 1) + e % D] = w[e];
+  const uint32_t gc = (uint32_t)(di * A.C + c);
+  const size_t si = (size_t)c * A.SW + ds.soff, gi = (size_t)c * A.GW + ds.goff;
+  const double v = A.state[si + D + 1];
+  const double step = theta_step(A, gc);
+  const Stream mom{A.seed, gc, *A.sweep, TAG_HMC_BASE + 4u + 0u};
+  const double m_o = A.sm[ds.goff + o], gb_o = A.sg[ds.goff + o];
+  double th = A.state[si + o];
+  const double p0 = g == 0 ? normal_at(mom, (uint32_t)o, 0) : 0.0;
+  if (g == 0) kin[o] = p0 * p0;
+  auto grad = [&]() {
+    __syncthreads();  // W loaded / the last product's reads of df done
+    if (g == 0) df[o] = (float)(th - m_o);
+    __syncthreads();
+    float part = 0.0f;
+    const float* wr = wl + o * (D + 1) + g * cps;
+    const float* dr = df + g * cps;
+    for (int k = 0; k < cps; ++k) part = fmaf(wr[k], dr[k], part);
+    for (int off = 1; off < segs; off <<= 1) part += __shfl_xor(part, off);
+    return gb_o + (double)part + (-th / v);
+  };
+  const int L = A.steps;
+  double p = p0 + 0.5 * step * grad();
+  for (int l = 1; l <= L; ++l) {
+    th += step * p;
+    const double gr = grad();
+    p += (l == L ? 0.5 * step : step) * gr;
+  }
+  if (g == 0) {
+    A.prop[si + o] = th;
+    A.p[gi + o] = p;
+    A.beta32[gi + o] = (float)th;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double k0 = 0.0;
+    for (int j = 0; j < D; ++j) k0 += kin[j];
+    A.k0[(size_t)c * A.n_ds + di] = 0.5 * k0;
+    A.prop[si + D] = A.state[si + D];
+    A.prop[si + D + 1] = v;
+  }
+}
+
+size_t bsurr_traj_smem_bytes(int dmax) {
+  size_t b = sizeof(float) * ((size_t)dmax * (dmax + 1) + dmax);
+  return ((b + 15) & ~(size_t)15) + sizeof(double) * dmax;
+}
+
+// Means of the fitting pairs (all chains and slots), one block per dataset:
+// thread j < D sums theta_j, thread D + j sums g_j, in slot and chain order.
+__global__ __launch_bounds__(256) void k_bsurr_mean(BArgs A, const float* __restrict__ ct,
+                                                    const float* __restrict__ cg, int slots,
+                                                    double* __restrict__ sm, double* __restrict__ sg) {
+  const int di = blockIdx.x, tid = threadIdx.x;
+  const BDs ds = A.ds[di];
+  const int D = ds.d;
+  if (tid >= 2 * D) return;
+  const int j = tid % D;
+  const float* src = tid < D ? ct : cg;
+  double acc = 0.0;
+  for (int r = 0; r < slots; ++r)
+    for (int c = 0; c < A.C; ++c) acc += (double)src[((size_t)r * A.C + c) * A.GW + ds.goff + j];
+  const double mean = acc / ((double)slots * A.C);
+  if (tid < D) sm[ds.goff + j] = mean;
+  else sg[ds.goff + j] = mean;
+}
+
+// Centred cross products of the pairs, one dataset per blockIdx.x and 4096
+// entries per blockIdx.y: G[j][e] over e < D is sum (theta_j - m_j)(theta_e
+// - m_e), over D <= e < 2D sum (theta_j - m_j)(g_{e-D} - gbar_{e-D}).  Pairs
+// are staged 16 at a time in LDS as FP64 deviations; every thread keeps 16
+// entries in registers; sums run in slot and chain order.
+constexpr int kGramPairs = 16;
+constexpr int kGramPer = 16;
+__global__ __launch_bounds__(256) void k_bsurr_gram(BArgs A, const float* __restrict__ ct,
+                                                    const float* __restrict__ cg, int slots,
+                                                    const double* __restrict__ sm,
+                                                    const double* __restrict__ sg, double* __restrict__ G,
+                                                    const size_t* __restrict__ goffs) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int di = blockIdx.x, tid = threadIdx.x;
+  const BDs ds = A.ds[di];
+  const int D = ds.d, W2 = 2 * D, ne = D * W2;
+  const int e0 = blockIdx.y * 256 * kGramPer;
+  if (e0 >= ne) return;
+  double* dv = (double*)smem;  // kGramPairs x 2D: theta deviations, then g deviations
+  int ja[kGramPer], ea[kGramPer];
+  double acc[kGramPer];
+#pragma unroll
+  for (int t = 0; t < kGramPer; ++t) {
+    const int e = e0 + t * 256 + tid;
+    ja[t] = e < ne ? e / W2 : 0;
+    ea[t] = e < ne ? e % W2 : 0;
+    acc[t] = 0.0;
+  }
+  const int npairs = slots * A.C;
+  for (int q0 = 0; q0 < npairs; q0 += kGramPairs) {
+    __syncthreads();
+    for (int i = tid; i < kGramPairs * W2; i += 256) {
+      const int q = q0 + i / W2, k = i % W2;
+      double dvi = 0.0;
+      if (q < npairs) {
+        const size_t base = (size_t)q * A.GW + ds.goff;  // pair q = slot * C + chain
+        dvi = k < D ? (double)ct[base + k] - sm[ds.goff + k] : (double)cg[base + k - D] - sg[ds.goff + k - D];
+      }
+      dv[i] = dvi;
+    }
+    __syncthreads();
+    for (int q = 0; q < kGramPairs; ++q) {
+      const double* r = dv + q * W2;
+#pragma unroll
+      for (int t = 0; t < kGramPer; ++t) acc[t] = fma(r[ja[t]], r[ea[t]], acc[t]);
+    }
+  }
+  double* gd = G + goffs[di];
+#pragma unroll
+  for (int t = 0; t < kGramPer; ++t) {
+    const int e = e0 + t * 256 + tid;
+    if (e < ne) gd[e] = acc[t];
+  }
+}
+
+// The least-squares map of one dataset (one block per dataset), on the
+// standardised inputs z_j = (theta_j - m_j) / s_j, s_j^2 = Cxx[j][j] / n:
+// (R + ridge I) B = Cxg / (n s), R the pairs' correlation matrix, by
+// Cholesky in place; then W[o][j] = B[j][o] / s_j (FP32).  A coordinate that
+// never moved (s_j = 0) gets s_j = 1 and a zero column.
+__global__ __launch_bounds__(256) void k_bsurr_solve(BArgs A, double* __restrict__ G,
+                                                     const size_t* __restrict__ goffs, int npairs,
+                                                     double ridge, float* __restrict__ Wout) {
+  __shared__ double sc[128];
+  const int di = blockIdx.x, tid = threadIdx.x;
+  const BDs ds = A.ds[di];
+  const int D = ds.d, W2 = 2 * D;
+  double* g = G + goffs[di];
+  const double n = (double)npairs;
+  for (int j = tid; j < D; j += 256) {
+    const double var = g[(size_t)j * W2 + j] / n;
+    sc[j] = var > 0.0 ? sqrt(var) : 1.0;
+  }
+  __syncthreads();
+  for (int e = tid; e < D * W2; e += 256) {
+    const int j = e / W2, k = e % W2;
+    double val = g[e] / (n * sc[j]);
+    if (k < D) {
+      val /= sc[k];
+      if (k == j) val = (g[e] > 0.0 ? val : 1.0) + ridge;
+    }
+    g[e] = val;
+  }
+  __syncthreads();
+  for (int k = 0; k < D; ++k) {
+    if (tid == 0) {
+      const double a = g[(size_t)k * W2 + k];
+      g[(size_t)k * W2 + k] = sqrt(a > 1e-300 ? a : 1e-300);
+    }
+    __syncthreads();
+    const double lkk = g[(size_t)k * W2 + k];
+    for (int i = k + 1 + tid; i < D; i += 256) g[(size_t)i * W2 + k] /= lkk;
+    __syncthreads();
+    const int m = D - k - 1;
+    for (int e = tid; e < m * m; e += 256) {
+      const int i = k + 1 + e / m, j = k + 1 + e % m;
+      if (j <= i) g[(size_t)i * W2 + j] -= g[(size_t)i * W2 + k] * g[(size_t)j * W2 + k];
+    }
+    __syncthreads();
+  }
+  // column o of the right-hand side: forward then backward substitution in
+  // place (L y = b, L^T x = y)
+  for (int o = tid; o < D; o += 256) {
+    double* b = g + D + o;
+    for (int i = 0; i < D; ++i) {
+      double t = b[(size_t)i * W2];
+      for (int j = 0; j < i; ++j) t -= g[(size_t)i * W2 + j] * b[(size_t)j * W2];
+      b[(size_t)i * W2] = t / g[(size_t)i * W2 + i];
+    }
+    for (int i = D - 1; i >= 0; --i) {
+      double t = b[(size_t)i * W2];
+      for (int j = i + 1; j < D; ++j) t -= g[(size_t)j * W2 + i] * b[(size_t)j * W2];
+      b[(size_t)i * W2] = t / g[(size_t)i * W2 + i];
+    }
+    for (int j = 0; j < D; ++j) Wout[ds.woff + (size_t)o * D + j] = (float)(b[(size_t)j * W2] / sc[j]);
+  }
+}
+
 static uint16_t to_bf16(float f) {  // round to nearest even
   uint32_t u;
   memcpy(&u, &f, 4);
@@ -946,6 +1188,21 @@ struct BhlrSampler final : augur_sampler {
   hipGraphExec_t graph_exec = nullptr;
   std::vector<float> tune_us;
   std::vector<int> tune_rows;
+  size_t WT = 0;  // sum of D_i^2: the surrogate maps
+
+  // ---- linear gradient surrogate ----
+  enum { BS_OFF = 0, BS_COLLECT = 1, BS_ACTIVE = 2 };
+  int bs_phase = BS_OFF;
+  int bs_train = 0, bs_collected = 0, bs_window = 0, bs_since = 0, bs_fallbacks = 0, bs_active = 0;
+  long long bs_sweeps = 0;
+  double bs_ridge = 1e-6, bs_min_accept = 0.0, bs_fit_ms = 0.0;
+  DevBuf<float> cap_t, cap_g, surr_w;
+  DevBuf<double> surr_m, surr_g, gram;
+  DevBuf<size_t> gram_off;
+  DevBuf<int> smode;
+  std::vector<int> hsmode;
+  std::vector<double> win0;  // theta-block (accepted, proposed) [C][n_ds] at the window's start
+  size_t traj_smem = 0;
 
   ~BhlrSampler() override {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
@@ -1020,6 +1277,11 @@ struct BhlrSampler final : augur_sampler {
     BArgs A = base;
     A.mode = mode;
     A.l = l;
+    if (bs_phase == BS_COLLECT && mode == BMODE_LEAPFROG) {
+      const size_t slot = (size_t)bs_collected * base.steps + (size_t)(l - 1);
+      A.cap_t = cap_t.p + slot * C * GW;
+      A.cap_g = cap_g.p + slot * C * GW;
+    }
     prof_begin();
     pass<<<nblk_total, kBThreads, pass_smem, stream>>>(A);
     AUGUR_CHECK_LAUNCH();
@@ -1084,9 +1346,18 @@ struct BhlrSampler final : augur_sampler {
   void enqueue_sweep() {
     k_bhmc_vblock<<<dim3(n_ds, C), 64, 0, stream>>>(base);
     AUGUR_CHECK_LAUNCH();
-    k_bhmc_begin<<<dim3(n_ds, C), 64, 0, stream>>>(base);
-    AUGUR_CHECK_LAUNCH();
-    for (int l = 1; l <= base.steps; ++l) launch_pass(BMODE_LEAPFROG, l);
+    if (bs_phase == BS_ACTIVE && bs_active > 0) {
+      // surrogate datasets: one trajectory kernel and the last pass only
+      k_bsurr_traj<<<dim3(n_ds, C), 256, traj_smem, stream>>>(base);
+      AUGUR_CHECK_LAUNCH();
+    }
+    if (bs_phase == BS_ACTIVE && bs_active == n_ds) {
+      launch_pass(BMODE_LEAPFROG, base.steps);
+    } else {
+      k_bhmc_begin<<<dim3(n_ds, C), 64, 0, stream>>>(base);
+      AUGUR_CHECK_LAUNCH();
+      for (int l = 1; l <= base.steps; ++l) launch_pass(BMODE_LEAPFROG, l);
+    }
     k_bhmc_end<<<dim3(n_ds, C), 64, 0, stream>>>(base);
     AUGUR_CHECK_LAUNCH();
     k_bump_sweep<<<1, 1, 0, stream>>>(sweep_dev.p);
@@ -1096,17 +1367,130 @@ struct BhlrSampler final : augur_sampler {
   void run_sweeps(int nsw) override {
     if (nsw <= 0) return;
     if (!cache_valid) refresh_cache();
-    if (use_graph && !graph_exec && prof_period == 0) {
-      AUGUR_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-      enqueue_sweep();
-      AUGUR_HIP(hipStreamEndCapture(stream, &graph));
-      AUGUR_HIP(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
-    }
     for (int i = 0; i < nsw; ++i) {
-      if (graph_exec && prof_period == 0) AUGUR_HIP(hipGraphLaunch(graph_exec, stream));
-      else enqueue_sweep();
+      // collection sweeps name their capture slots in kernel arguments, so
+      // they are launched directly; a phase change drops the graph
+      if (use_graph && prof_period == 0 && bs_phase != BS_COLLECT) {
+        if (!graph_exec) {
+          AUGUR_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+          enqueue_sweep();
+          AUGUR_HIP(hipStreamEndCapture(stream, &graph));
+          AUGUR_HIP(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+        }
+        AUGUR_HIP(hipGraphLaunch(graph_exec, stream));
+      } else {
+        enqueue_sweep();
+      }
       ++sweep;
       record_traces();
+      bs_after_sweep();
+    }
+  }
+
+  // ---- surrogate phases ----
+  void bs_configure(int train, double ridge, double min_accept, int window) {
+    const size_t slots = (size_t)train * base.steps;
+    const size_t bytes = 2 * sizeof(float) * slots * C * GW;
+    if (bytes > ((size_t)32 << 30))
+      fail(AUGUR_ERR_ARG, "%zu bytes of fitting pairs (train_sweeps x steps x chains x sum (D_i + 1) x 8) exceed 32 GiB",
+           bytes);
+    cap_t.alloc(slots * C * GW);
+    cap_g.alloc(slots * C * GW);
+    surr_w.alloc(WT);
+    surr_m.alloc(GW);
+    surr_g.alloc(GW);
+    std::vector<size_t> go(n_ds);
+    size_t tot = 0;
+    for (int i = 0; i < n_ds; ++i) {
+      go[i] = tot;
+      tot += 2 * (size_t)hds[i].d * hds[i].d;
+    }
+    gram.alloc(tot);
+    gram_off.alloc(n_ds);
+    gram_off.upload(go.data(), go.size(), stream);
+    smode.alloc(n_ds);
+    traj_smem = bsurr_traj_smem_bytes(dmax);
+    allow_lds(k_bsurr_traj, traj_smem);
+    allow_lds(k_bsurr_gram, sizeof(double) * kGramPairs * 2 * dmax);
+    AUGUR_HIP(hipStreamSynchronize(stream));  // go is released on return
+    bs_train = train;
+    bs_ridge = ridge;
+    bs_min_accept = min_accept;
+    bs_window = window;
+    bs_collected = 0;
+    bs_phase = BS_COLLECT;
+    drop_graph();
+  }
+
+  void bs_after_sweep() {
+    if (bs_phase == BS_COLLECT) {
+      if (++bs_collected >= bs_train) bs_fit();
+    } else if (bs_phase == BS_ACTIVE && bs_active > 0) {
+      ++bs_sweeps;
+      if (bs_window > 0 && ++bs_since >= bs_window) bs_check_window();
+    }
+  }
+
+  std::vector<double> theta_counts() {
+    std::vector<double> st(2 * (size_t)C * n_ds * 2);
+    stats_buf.download(st.data(), st.size(), stream);
+    AUGUR_HIP(hipStreamSynchronize(stream));
+    return std::vector<double>(st.begin() + (size_t)C * n_ds * 2, st.end());
+  }
+
+  void bs_fit() {
+    AUGUR_HIP(hipStreamSynchronize(stream));
+    const auto t0 = std::chrono::steady_clock::now();
+    const int slots = bs_collected * base.steps;
+    k_bsurr_mean<<<n_ds, 256, 0, stream>>>(base, cap_t.p, cap_g.p, slots, surr_m.p, surr_g.p);
+    AUGUR_CHECK_LAUNCH();
+    const dim3 gg(n_ds, (unsigned)((2 * (size_t)dmax * dmax + 256 * kGramPer - 1) / (256 * kGramPer)));
+    k_bsurr_gram<<<gg, 256, sizeof(double) * kGramPairs * 2 * dmax, stream>>>(
+        base, cap_t.p, cap_g.p, slots, surr_m.p, surr_g.p, gram.p, gram_off.p);
+    AUGUR_CHECK_LAUNCH();
+    k_bsurr_solve<<<n_ds, 256, 0, stream>>>(base, gram.p, gram_off.p, slots * C, bs_ridge, surr_w.p);
+    AUGUR_CHECK_LAUNCH();
+    hsmode.assign(n_ds, 1);
+    smode.upload(hsmode.data(), hsmode.size(), stream);
+    AUGUR_HIP(hipStreamSynchronize(stream));
+    bs_fit_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    base.smode = smode.p;
+    base.sw = surr_w.p;
+    base.sm = surr_m.p;
+    base.sg = surr_g.p;
+    bs_active = n_ds;
+    bs_phase = BS_ACTIVE;
+    bs_since = 0;
+    win0 = theta_counts();
+    drop_graph();
+  }
+
+  // Datasets whose acceptance over the last window fell below min_accept
+  // return to exact HMC for good.
+  void bs_check_window() {
+    const std::vector<double> st = theta_counts();
+    bool changed = false;
+    for (int i = 0; i < n_ds; ++i) {
+      if (!hsmode[i]) continue;
+      double a = 0.0, t = 0.0;
+      for (int c = 0; c < C; ++c) {
+        const size_t k = ((size_t)c * n_ds + i) * 2;
+        a += st[k] - win0[k];
+        t += st[k + 1] - win0[k + 1];
+      }
+      if (t > 0.0 && a < bs_min_accept * t) {
+        hsmode[i] = 0;
+        ++bs_fallbacks;
+        --bs_active;
+        changed = true;
+      }
+    }
+    win0 = st;
+    bs_since = 0;
+    if (changed) {
+      smode.upload(hsmode.data(), hsmode.size(), stream);
+      AUGUR_HIP(hipStreamSynchronize(stream));
+      drop_graph();
     }
   }
 
@@ -1136,6 +1520,7 @@ struct BhlrSampler final : augur_sampler {
       AUGUR_CHECK_LAUNCH();
       ++sweep;
       record_traces();
+      bs_after_sweep();
     }
     k_badapt<<<(n_ds + 63) / 64, 64, 0, stream>>>(base, dad.p, target, 1);
     AUGUR_CHECK_LAUNCH();
@@ -1223,8 +1608,10 @@ augur_status augur_bhlr_create(const augur_bhlr_desc* D, augur_sampler** out) {
         b.d = D->n_features[i];
         b.soff = soff;
         b.goff = goff;
+        b.woff = (int)s->WT;
         soff += b.d + 2;
         goff += b.d + 1;
+        s->WT += (size_t)b.d * b.d;
         s->dmax = std::max(s->dmax, b.d);
         xtot += (size_t)padded_rows(b.n) * b.d;
         ytot += (size_t)padded_rows(b.n);
@@ -1355,6 +1742,85 @@ augur_status augur_bhlr_step_sizes(augur_sampler* s, double* eps, double* v_var,
     if (eps) b->eps_ds.download(eps, b->n_ds, b->stream);
     if (v_var) b->vvar_ds.download(v_var, b->n_ds, b->stream);
     AUGUR_HIP(hipStreamSynchronize(b->stream));
+  });
+}
+
+
+augur_status augur_bhlr_surrogate(augur_sampler* s, const augur_bsurr_cfg* cfg) {
+  return guard([&] {
+    auto* b = dynamic_cast<BhlrSampler*>(s);
+    if (!b || !cfg) fail(AUGUR_ERR_ARG, "needs a batched HLR sampler and a configuration");
+    if (b->bs_phase != BhlrSampler::BS_OFF) fail(AUGUR_ERR_STATE, "surrogate already configured");
+    if (cfg->train_sweeps < 1 || cfg->window < 0 || !(cfg->ridge >= 0.0) || !(cfg->min_accept >= 0.0))
+      fail(AUGUR_ERR_ARG, "need train_sweeps >= 1, window >= 0, ridge >= 0 and min_accept >= 0");
+    AUGUR_HIP(hipSetDevice(b->device));
+    b->bs_configure(cfg->train_sweeps, cfg->ridge, cfg->min_accept, cfg->window);
+  });
+}
+
+augur_status augur_bhlr_surrogate_info(augur_sampler* s, double* info, size_t bytes) {
+  return guard([&] {
+    auto* b = dynamic_cast<BhlrSampler*>(s);
+    if (!b || !info) fail(AUGUR_ERR_ARG, "needs a batched HLR sampler and an output");
+    if (bytes < 7 * sizeof(double)) fail(AUGUR_ERR_ARG, "info needs 7 doubles");
+    info[0] = b->bs_phase;
+    info[1] = b->bs_active;
+    info[2] = b->bs_fallbacks;
+    info[3] = b->bs_fit_ms;
+    info[4] = (double)b->bs_collected * b->base.steps * b->C;
+    info[5] = (double)b->bs_sweeps;
+    info[6] = b->bs_collected;
+  });
+}
+
+augur_status augur_bhlr_surrogate_modes(augur_sampler* s, int32_t* modes, int n) {
+  return guard([&] {
+    auto* b = dynamic_cast<BhlrSampler*>(s);
+    if (!b || !modes || n < b->n_ds) fail(AUGUR_ERR_ARG, "needs a batched HLR sampler and n >= n_datasets");
+    for (int i = 0; i < b->n_ds; ++i) modes[i] = b->bs_phase == BhlrSampler::BS_ACTIVE ? b->hsmode[i] : 0;
+  });
+}
+
+augur_status augur_bhlr_surrogate_map(augur_sampler* s, int dataset, double* w, double* mean_theta,
+                                      double* mean_grad) {
+  return guard([&] {
+    auto* b = dynamic_cast<BhlrSampler*>(s);
+    if (!b) fail(AUGUR_ERR_ARG, "not a batched HLR sampler");
+    if (b->bs_phase != BhlrSampler::BS_ACTIVE) fail(AUGUR_ERR_STATE, "no fitted surrogate");
+    if (dataset < 0 || dataset >= b->n_ds) fail(AUGUR_ERR_ARG, "dataset %d out of range", dataset);
+    const BDs& d = b->hds[dataset];
+    AUGUR_HIP(hipSetDevice(b->device));
+    AUGUR_HIP(hipStreamSynchronize(b->stream));
+    if (w) {
+      std::vector<float> h((size_t)d.d * d.d);
+      AUGUR_HIP(hipMemcpy(h.data(), b->surr_w.p + d.woff, h.size() * sizeof(float), hipMemcpyDeviceToHost));
+      for (size_t e = 0; e < h.size(); ++e) w[e] = h[e];
+    }
+    if (mean_theta)
+      AUGUR_HIP(hipMemcpy(mean_theta, b->surr_m.p + d.goff, d.d * sizeof(double), hipMemcpyDeviceToHost));
+    if (mean_grad)
+      AUGUR_HIP(hipMemcpy(mean_grad, b->surr_g.p + d.goff, d.d * sizeof(double), hipMemcpyDeviceToHost));
+  });
+}
+
+augur_status augur_bhlr_surrogate_pairs(augur_sampler* s, int dataset, float* theta, float* grad, int cap,
+                                        int* n_pairs) {
+  return guard([&] {
+    auto* b = dynamic_cast<BhlrSampler*>(s);
+    if (!b || !n_pairs) fail(AUGUR_ERR_ARG, "needs a batched HLR sampler and n_pairs");
+    if (b->bs_phase == BhlrSampler::BS_OFF) fail(AUGUR_ERR_STATE, "no surrogate configured");
+    if (dataset < 0 || dataset >= b->n_ds) fail(AUGUR_ERR_ARG, "dataset %d out of range", dataset);
+    const BDs& d = b->hds[dataset];
+    const int np = b->bs_collected * b->base.steps * b->C;
+    *n_pairs = np;
+    if (!theta && !grad) return;
+    if (cap < np) fail(AUGUR_ERR_ARG, "room for %d pairs, %d collected", cap, np);
+    AUGUR_HIP(hipSetDevice(b->device));
+    AUGUR_HIP(hipStreamSynchronize(b->stream));
+    // pair q = slot * C + chain: row q of the output, the dataset's d values
+    const size_t sp = b->GW * sizeof(float), dp = (size_t)d.d * sizeof(float);
+    if (theta) AUGUR_HIP(hipMemcpy2D(theta, dp, b->cap_t.p + d.goff, sp, dp, np, hipMemcpyDeviceToHost));
+    if (grad) AUGUR_HIP(hipMemcpy2D(grad, dp, b->cap_g.p + d.goff, sp, dp, np, hipMemcpyDeviceToHost));
   });
 }
 

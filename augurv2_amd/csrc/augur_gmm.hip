@@ -6,12 +6,15 @@
 //
 //   k_gmm_stats_sum  the statistics of every (chain, component) summed
 //               over the blocks of the statistics kernel, FP64, fixed order.
-//   k_gmm_conj  one block per (component, chain): draws pis_k's Gamma
+//   k_gmm_conj  one wave per (component, chain): draws pis_k's Gamma
 //               (Core/CgConj.hs:171-183), mu_k | cov_k
 //               (cbits/augur_dist.c:633-669), cov_k | mu_k
-//               (cbits/augur_dist.c:740-757, Bartlett), all in FP64, and builds
-//               the component's table for the z step: L_k^-1 (packed lower
+//               (cbits/augur_dist.c:740-757, Bartlett), all in FP64 with the
+//               matrices in registers (lane i: row i), and builds the
+//               component's table for the z step: L_k^-1 (packed lower
 //               triangle), mu_k and -sum log diag L_k, stored FP32.
+//               (k_gmm_prec0 / k_gmm_tables build cov0^-1 and the tables at
+//               creation and after a set of mu or cov.)
 //   k_gmm_z     point-parallel categorical draw (Core/CgDisc.hs:122-144) in
 //               log space with max subtraction: logit_k = log pis_k
 //               - 0.5 log det cov_k - 0.5 |L_k^-1 (y - mu_k)|^2, tables read
@@ -1156,11 +1159,12 @@ __global__ __launch_bounds__(kGmmThreads) void k_gmm_z_mfma(GmmDev m, uint32_t s
 // Sufficient statistics of z around the table's mu (the mu of the z step):
 // per (chain, block) and component, count, sum e and sum e e^T (packed lower)
 // with e = y - mu_z.  Each wave takes 64 points and forms
-// S[k][q] += sum_p [z_p = k] F_p[q], F_p = (1, e_p, e_p e_p^T), as a one-hot
-// product on the FP32 matrix core (V_MFMA_F32_16X16X4_F32: A = [z_p = k],
-// 16 components x 4 points; B = F, 4 points x 16 entries) from a per-wave LDS
-// slab of the residuals.  No atomics: FP32 per block in a fixed order, FP64
-// across blocks in k_gmm_conj.
+// S[k][q] += sum_p [z_p = k] F_p[q], F_p = (1, e_p, e_p e_p^T), as one-hot
+// products on the matrix core (V_MFMA_F32_16X16X32_BF16: A = [z_p = k],
+// 16 components x 32 points, exact in BF16; B = F, 32 points x 16 entries,
+// each entry split into three BF16 parts) from a per-wave LDS slab of the
+// residuals.  No atomics: FP32 per block in a fixed order, FP64 across
+// blocks in k_gmm_stats_sum.
 template <int DM, int KM>
 __global__ __launch_bounds__(kGmmThreads) void k_gmm_stats_t(GmmDev m) {
   extern __shared__ __align__(16) unsigned char smem[];

@@ -236,6 +236,45 @@ class BhlrSampler(Sampler):
         L.call("augur_bhlr_tuning", self._h, C.byref(rows), C.byref(n), L.ptr(cr), L.ptr(cu), 16)
         return rows.value, dict(zip(cr[:n.value].tolist(), cu[:n.value].tolist()))
 
+    def surrogate(self, train_sweeps, ridge=1e-6, min_accept=0.3, window=50):
+        """Linear gradient surrogate for every dataset: the next train_sweeps
+        sweeps collect (theta, data gradient) pairs, each dataset's map is
+        then fitted, and later sweeps run surrogate trajectories with an
+        exact Metropolis test; a dataset below min_accept over a window of
+        sweeps returns to exact HMC."""
+        cfg = L.BsurrCfg(int(train_sweeps), float(ridge), float(min_accept), int(window))
+        L.call("augur_bhlr_surrogate", self._h, C.byref(cfg))
+
+    def surrogate_info(self):
+        info = np.zeros(7)
+        L.call("augur_bhlr_surrogate_info", self._h, L.ptr(info), info.nbytes)
+        keys = ("phase", "datasets_on_surrogate", "fallbacks", "fit_ms", "pairs_per_dataset",
+                "surrogate_sweeps", "collection_sweeps")
+        return dict(zip(keys, info.tolist()))
+
+    def surrogate_modes(self):
+        m = np.zeros(len(self.xs), np.int32)
+        L.call("augur_bhlr_surrogate_modes", self._h, L.ptr(m), len(self.xs))
+        return m
+
+    def surrogate_map(self, i):
+        """(W, m, gbar) of dataset i: g(theta) ~ gbar + W (theta - m)."""
+        d = int(self.dims[i])
+        w, m, g = np.zeros((d, d)), np.zeros(d), np.zeros(d)
+        L.call("augur_bhlr_surrogate_map", self._h, int(i), L.ptr(w), L.ptr(m), L.ptr(g))
+        return w, m, g
+
+    def surrogate_pairs(self, i):
+        """Dataset i's fitting pairs (theta, data gradient), FP32 rows in
+        (slot, chain) order."""
+        d = int(self.dims[i])
+        n = C.c_int(0)
+        L.call("augur_bhlr_surrogate_pairs", self._h, int(i), None, None, 0, C.byref(n))
+        t = np.zeros((n.value, d), np.float32)
+        g = np.zeros((n.value, d), np.float32)
+        L.call("augur_bhlr_surrogate_pairs", self._h, int(i), L.ptr(t), L.ptr(g), n.value, C.byref(n))
+        return t, g
+
     def acceptance(self):
         """[block (v, theta)][chain][dataset] acceptance rates."""
         k = len(self.xs)

@@ -203,6 +203,49 @@ augur_status augur_bhlr_adapt(augur_sampler* s, int n_sweeps, double target_acce
  * either pointer may be null). */
 augur_status augur_bhlr_step_sizes(augur_sampler* s, double* eps, double* v_var, int n);
 
+/* No counterpart in the reference.  Linear gradient surrogate for every
+ * dataset of a batched sampler (the single-dataset sampler's least-squares
+ * term, without the network):
+ *   1. the next train_sweeps sweeps (ordinary or warmup) run exact HMC and
+ *      keep every leapfrog point with its data gradient (FP32) on the device;
+ *   2. then each dataset's data gradient is fitted on the device as
+ *      g(theta) ~ gbar + W (theta - m), m and gbar the pairs' means, W by
+ *      least squares on standardised theta with `ridge` added to the
+ *      correlation matrix's diagonal (FP64 normal equations, Cholesky);
+ *   3. later sweeps move each dataset's theta by a leapfrog trajectory with
+ *      that gradient plus the exact prior gradient (its own step size,
+ *      jitter and count), then run one exact data pass at the proposal for
+ *      the unchanged FP64 Metropolis test, so the target is unchanged;
+ *   4. a dataset whose acceptance rate over a window of `window` sweeps
+ *      (0: never checked) falls below min_accept returns to exact HMC for
+ *      good.
+ * The pairs take train_sweeps x steps x chains x sum (D_i + 1) x 8 bytes
+ * (at most 32 GiB) and stay for augur_bhlr_surrogate_pairs. */
+typedef struct {
+  int train_sweeps;
+  double ridge;       /* e.g. 1e-6 */
+  double min_accept;
+  int window;
+} augur_bsurr_cfg;
+augur_status augur_bhlr_surrogate(augur_sampler* s, const augur_bsurr_cfg* cfg);
+/* info[7]: phase (0 off, 1 collecting, 2 surrogate), datasets on the
+ * surrogate, datasets that fell back, fit time in ms, pairs per dataset,
+ * surrogate sweeps run, collection sweeps run. */
+augur_status augur_bhlr_surrogate_info(augur_sampler* s, double* info, size_t bytes);
+/* Per dataset: 1 on the surrogate, 0 exact (n >= n_datasets). */
+augur_status augur_bhlr_surrogate_modes(augur_sampler* s, int32_t* modes, int n);
+/* Dataset i's fitted map: W (D_i x D_i, row o gives output coordinate o; the
+ * FP32 values the trajectories use), m and gbar (D_i each); any pointer may
+ * be null. */
+augur_status augur_bhlr_surrogate_map(augur_sampler* s, int dataset, double* w, double* mean_theta,
+                                      double* mean_grad);
+/* Dataset i's fitting pairs, row q = slot * chains + chain with slot =
+ * collection sweep * steps + leapfrog pass: theta as evaluated (FP32) and
+ * its data gradient (FP32), D_i values per row.  *n_pairs receives the
+ * count; with both pointers null only the count is returned. */
+augur_status augur_bhlr_surrogate_pairs(augur_sampler* s, int dataset, float* theta, float* grad, int cap,
+                                        int* n_pairs);
+
 /* hmvgmm.rv: pis ~ Dirichlet(alpha); mu[k] ~ MvNormal(mu0, cov0);
  * cov[k] ~ InvWishart(df, scale0); z[n] ~ Categorical(pis);
  * y[n] ~ MvNormal(mu[z[n]], cov[z[n]]).  sample_pi / sample_cov = 0 keep

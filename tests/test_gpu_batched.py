@@ -198,3 +198,151 @@ def test_autotuner_picks_a_timed_candidate(rt):
     s.step(2)
     assert np.isfinite(s.log_density()).all()
     s.close()
+
+
+def _surrogate_sampler(rt, rng, C, L, seed, train, **kw):
+    xs, ys = _datasets(rng, SHAPES)
+    s = rt.BhlrSampler(xs, ys, chains=C, seed=seed, v_block=("mwg", 1.0), theta_block=(0.1, 0.01, L),
+                       rows_per_block=512, use_graph=True)
+    _init(s, rng, C)
+    s.adapt(100, 0.8)
+    s.surrogate(train_sweeps=train, **kw)
+    return s, xs, ys
+
+
+def test_batched_surrogate_fit_is_least_squares_on_the_pairs(rt):
+    """Each dataset's map is the least-squares fit of its captured (theta,
+    data gradient) pairs: means of theta and of the gradient, then the
+    regression on standardised theta with 1e-6 added to the correlation
+    matrix's diagonal, as numpy solves it in FP64 (W kept in FP32)."""
+    rng = np.random.default_rng(5)
+    C, L, train = 2, -8, 30
+    s, _, _ = _surrogate_sampler(rt, rng, C, L, 3, train, ridge=1e-6, min_accept=0.0, window=0)
+    assert s.surrogate_info()["phase"] == 1
+    s.step(train)
+    info = s.surrogate_info()
+    assert info["phase"] == 2 and info["datasets_on_surrogate"] == len(SHAPES)
+    assert info["pairs_per_dataset"] == train * 8 * C
+    for i, (n, d) in enumerate(SHAPES):
+        t, g = s.surrogate_pairs(i)
+        assert t.shape == (train * 8 * C, d) and np.isfinite(t).all() and np.isfinite(g).all()
+        w, m, gbar = s.surrogate_map(i)
+        t64, g64 = t.astype(np.float64), g.astype(np.float64)
+        np.testing.assert_allclose(m, t64.mean(0), rtol=1e-12, atol=1e-15)
+        np.testing.assert_allclose(gbar, g64.mean(0), rtol=1e-12, atol=1e-12 * np.abs(g64).max())
+        dt, dg = t64 - m, g64 - gbar
+        sd = np.sqrt((dt ** 2).mean(0))
+        sd[sd == 0] = 1.0
+        z = dt / sd
+        r = z.T @ z / len(z) + 1e-6 * np.eye(d)
+        b = np.linalg.solve(r, z.T @ dg / len(z))
+        w_ref = (b / sd[:, None]).T
+        np.testing.assert_allclose(w, w_ref, rtol=1e-4, atol=1e-5 * np.abs(w_ref).max(),
+                                   err_msg=f"dataset {i}")
+        # the map explains the pairs' gradients (the posterior is near Gaussian)
+        resid = dg - dt @ w.T
+        assert (resid ** 2).sum() < 0.1 * (dg ** 2).sum(), i
+    s.close()
+
+
+def test_batched_surrogate_sweeps_match_oracle(rt, oracle):
+    """Surrogate sweeps (one trajectory kernel, one exact pass at the
+    proposal, the FP64 Metropolis test) equal the oracle's step with the
+    device's fitted map, dataset by dataset and chain by chain."""
+    from oracle import bsurr
+    rng = np.random.default_rng(6)
+    C, L, seed, train = 2, -8, 9, 20
+    s, xs, ys = _surrogate_sampler(rt, rng, C, L, seed, train, ridge=1e-6, min_accept=0.0, window=0)
+    s.step(train + 3)
+    assert s.surrogate_info()["phase"] == 2
+    eps, vv = s.step_sizes()
+    maps = [bsurr.linear_grad(*s.surrogate_map(i)) for i in range(len(SHAPES))]
+    models = [oracle.Hlr(bf16_round(x).astype(np.float64), y, prior_kind=0, prior_a=1.0,
+                         prior_b=1.0, ref_compat=False) for x, y in zip(xs, ys)]
+    states = {(i, c): s.get(f"state:{i}", c).copy() for i in range(len(SHAPES)) for c in range(C)}
+    accepted = 0
+    for k in range(3):
+        sweep = s.sweeps()
+        s.step(1)
+        for i in range(len(SHAPES)):
+            for c in range(C):
+                st = states[i, c]
+                gc = i * C + c
+                models[i].mwg_v_step(st, 0, vv[i], seed, gc, sweep)
+                accepted += bsurr.hmc_step(models[i], st, maps[i], eps[i], L, seed, gc, sweep)
+                np.testing.assert_allclose(s.get(f"state:{i}", c), st, rtol=2e-5, atol=1e-6,
+                                           err_msg=f"sweep {k} dataset {i} chain {c}")
+    assert accepted >= len(SHAPES) * C  # the surrogate proposals are mostly accepted
+    s.close()
+
+
+def test_batched_surrogate_falls_back_to_exact_hmc(rt, oracle):
+    """With an acceptance floor no window can meet, every dataset returns to
+    exact HMC after one window, and its later sweeps are the exact step."""
+    rng = np.random.default_rng(8)
+    C, L, seed, train = 2, -8, 13, 10
+    s, xs, ys = _surrogate_sampler(rt, rng, C, L, seed, train, ridge=1e-6, min_accept=1.01, window=2)
+    s.step(train + 2)
+    info = s.surrogate_info()
+    assert info["datasets_on_surrogate"] == 0 and info["fallbacks"] == len(SHAPES)
+    assert not s.surrogate_modes().any()
+    eps, vv = s.step_sizes()
+    models = [oracle.Hlr(bf16_round(x).astype(np.float64), y, prior_kind=0, prior_a=1.0,
+                         prior_b=1.0, ref_compat=False) for x, y in zip(xs, ys)]
+    sweep = s.sweeps()
+    states = {(i, c): s.get(f"state:{i}", c).copy() for i in range(len(SHAPES)) for c in range(C)}
+    s.step(1)
+    for i in range(len(SHAPES)):
+        for c in range(C):
+            st = states[i, c]
+            gc = i * C + c
+            models[i].mwg_v_step(st, 0, vv[i], seed, gc, sweep)
+            models[i].hmc_step(st, 2, 1, 0.1, eps[i], L, seed, gc, sweep)
+            np.testing.assert_allclose(s.get(f"state:{i}", c), st, rtol=2e-5, atol=1e-6,
+                                       err_msg=f"dataset {i} chain {c}")
+    s.close()
+
+
+def test_batched_surrogate_posterior_matches_exact_sampler(rt):
+    """Posterior means of theta from surrogate sweeps agree with exact
+    batched HMC on the same datasets within 4.5 combined Monte Carlo
+    standard errors (every coordinate of three datasets), at acceptance
+    above 0.6."""
+    from augurv2_amd import diagnostics as dg
+    rng = np.random.default_rng(12)
+    shapes = [(4000, 16), (12000, 32), (8000, 64)]
+    xs, ys = _datasets(rng, shapes)
+    C, L, n = 4, -10, 1500
+    draws = []
+    for surrogate in (False, True):
+        s = rt.BhlrSampler(xs, ys, chains=C, seed=21 + surrogate, v_block=("mwg", 1.0),
+                           theta_block=(0.1, 0.01, L), rows_per_block=512, use_graph=True)
+        for i, (nn, d) in enumerate(shapes):
+            for c in range(C):
+                s.set(f"state:{i}", np.concatenate([rng.standard_normal(d) * 0.05, [0.0, 1.0]]), chain=c)
+        s.adapt(400, 0.8)
+        if surrogate:
+            s.surrogate(train_sweeps=200, ridge=1e-6, min_accept=0.3, window=100)
+            s.step(200)
+        s.trace("state", n)
+        s.step(n)
+        tr = s.read_trace("state")  # (n, C, SW)
+        if surrogate:
+            info = s.surrogate_info()
+            assert info["datasets_on_surrogate"] == len(shapes), info
+            acc = s.acceptance()[1]
+            assert acc.mean() > 0.6, acc
+        draws.append(tr)
+        s.close()
+    off = 0
+    worst = 0.0
+    for nn, d in shapes:
+        for j in range(d):
+            xa = draws[0][:, :, off + j].T
+            xb = draws[1][:, :, off + j].T
+            assert dg.rhat(xa) < 1.05 and dg.rhat(xb) < 1.05
+            se = np.hypot(dg.mcse_mean(xa), dg.mcse_mean(xb))
+            z = abs(xa.mean() - xb.mean()) / se
+            worst = max(worst, z)
+        off += d + 2
+    assert worst < 4.5, worst
