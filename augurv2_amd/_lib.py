@@ -122,6 +122,7 @@ SIGNATURES = {
     "augur_bhlr_surrogate": [_H, _P],
     "augur_bhlr_surrogate_info": [_H, _P, _S],
     "augur_bhlr_surrogate_modes": [_H, _P, _I],
+    "augur_bhlr_surrogate_set_mode": [_H, _I, _I],
     "augur_bhlr_surrogate_map": [_H, _I, _P, _P, _P],
     "augur_bhlr_surrogate_pairs": [_H, _I, _P, _P, _I, _P],
     "augur_lda_predictive_ll": [_H, _I, _P, _P, _P],

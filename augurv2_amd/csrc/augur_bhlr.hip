@@ -1201,6 +1201,14 @@ struct BhlrSampler final : augur_sampler {
   DevBuf<size_t> gram_off;
   DevBuf<int> smode;
   std::vector<int> hsmode;
+  // Layout of the leapfrog passes before the last while some datasets run
+  // the surrogate: only the exact datasets' rows, in blocks of R_ex rows
+  // (a few small datasets would otherwise leave most CUs idle), with their
+  // own block table and partials.  The other datasets have no blocks.
+  DevBuf<BDs> dds_ex;
+  DevBuf<int> blk_ds_ex;
+  DevBuf<double> part_ex;
+  int nblk_ex = 0, R_ex = 0;
   std::vector<double> win0;  // theta-block (accepted, proposed) [C][n_ds] at the window's start
   size_t traj_smem = 0;
 
@@ -1273,17 +1281,24 @@ struct BhlrSampler final : augur_sampler {
     base.x_stream = getenv("AUGUR_BHLR_TEMPORAL") == nullptr;
   }
 
-  void launch_pass(int mode, int l) {
+  // exact_only: the exact datasets' layout (dds_ex) instead of every block
+  void launch_pass(int mode, int l, bool exact_only = false) {
     BArgs A = base;
     A.mode = mode;
     A.l = l;
+    if (exact_only) {
+      A.ds = dds_ex.p;
+      A.blk_ds = blk_ds_ex.p;
+      A.rows_per_blk = R_ex;
+      A.part = part_ex.p;
+    }
     if (bs_phase == BS_COLLECT && mode == BMODE_LEAPFROG) {
       const size_t slot = (size_t)bs_collected * base.steps + (size_t)(l - 1);
       A.cap_t = cap_t.p + slot * C * GW;
       A.cap_g = cap_g.p + slot * C * GW;
     }
     prof_begin();
-    pass<<<nblk_total, kBThreads, pass_smem, stream>>>(A);
+    pass<<<exact_only ? nblk_ex : nblk_total, kBThreads, pass_smem, stream>>>(A);
     AUGUR_CHECK_LAUNCH();
     prof_end();
     k_bhlr_finish<<<(A.nval + 255) / 256, 256, 0, stream>>>(A);
@@ -1352,6 +1367,13 @@ struct BhlrSampler final : augur_sampler {
       AUGUR_CHECK_LAUNCH();
     }
     if (bs_phase == BS_ACTIVE && bs_active == n_ds) {
+      launch_pass(BMODE_LEAPFROG, base.steps);
+    } else if (bs_phase == BS_ACTIVE && bs_active > 0) {
+      // datasets back on exact HMC: their blocks only until the last pass,
+      // which every dataset runs (the surrogate ones evaluate their proposal)
+      k_bhmc_begin<<<dim3(n_ds, C), 64, 0, stream>>>(base);
+      AUGUR_CHECK_LAUNCH();
+      for (int l = 1; l < base.steps; ++l) launch_pass(BMODE_LEAPFROG, l, true);
       launch_pass(BMODE_LEAPFROG, base.steps);
     } else {
       k_bhmc_begin<<<dim3(n_ds, C), 64, 0, stream>>>(base);
@@ -1487,11 +1509,44 @@ struct BhlrSampler final : augur_sampler {
     }
     win0 = st;
     bs_since = 0;
-    if (changed) {
-      smode.upload(hsmode.data(), hsmode.size(), stream);
-      AUGUR_HIP(hipStreamSynchronize(stream));
-      drop_graph();
+    if (changed) bs_modes_changed();
+  }
+
+  // hsmode changed: upload it and rebuild the exact datasets' layout, in
+  // blocks of R_ex rows (a multiple of kRowPad, at most R) chosen so that
+  // their rows make about 512 blocks
+  void bs_modes_changed() {
+    AUGUR_HIP(hipStreamSynchronize(stream));  // no launch still reads the old layout
+    drop_graph();  // the graph's passes name the old layout
+    long long rows = 0;
+    bs_active = 0;
+    for (int i = 0; i < n_ds; ++i) {
+      bs_active += hsmode[i];
+      if (!hsmode[i]) rows += hds[i].n;
     }
+    const long long want = (rows + 511) / 512;
+    R_ex = (int)std::min<long long>(R, std::max<long long>(kRowPad, (want + kRowPad - 1) / kRowPad * kRowPad));
+    std::vector<BDs> ex = hds;
+    std::vector<int> table;
+    int blk = 0;
+    size_t poff = 0;
+    for (int i = 0; i < n_ds; ++i) {
+      BDs& d = ex[i];
+      d.nblk = hsmode[i] ? 0 : (d.n + R_ex - 1) / R_ex;
+      d.blk0 = blk;
+      d.poff = (int)poff;
+      for (int b = 0; b < d.nblk; ++b) table.push_back(i);
+      blk += d.nblk;
+      poff += (size_t)C * (d.d + 2) * d.nblk;
+    }
+    nblk_ex = blk;
+    smode.upload(hsmode.data(), hsmode.size(), stream);
+    dds_ex.alloc(ex.size());
+    dds_ex.upload(ex.data(), ex.size(), stream);
+    blk_ds_ex.alloc(table.size());
+    blk_ds_ex.upload(table.data(), table.size(), stream);
+    part_ex.alloc(poff);
+    AUGUR_HIP(hipStreamSynchronize(stream));  // the host vectors go out of scope
   }
 
   // Warmup sweeps that adapt each dataset's theta step size (and the MWG
@@ -1778,6 +1833,20 @@ augur_status augur_bhlr_surrogate_modes(augur_sampler* s, int32_t* modes, int n)
     auto* b = dynamic_cast<BhlrSampler*>(s);
     if (!b || !modes || n < b->n_ds) fail(AUGUR_ERR_ARG, "needs a batched HLR sampler and n >= n_datasets");
     for (int i = 0; i < b->n_ds; ++i) modes[i] = b->bs_phase == BhlrSampler::BS_ACTIVE ? b->hsmode[i] : 0;
+  });
+}
+
+augur_status augur_bhlr_surrogate_set_mode(augur_sampler* s, int dataset, int mode) {
+  return guard([&] {
+    auto* b = dynamic_cast<BhlrSampler*>(s);
+    if (!b) fail(AUGUR_ERR_ARG, "not a batched HLR sampler");
+    if (b->bs_phase != BhlrSampler::BS_ACTIVE) fail(AUGUR_ERR_STATE, "no fitted surrogate");
+    if (dataset < 0 || dataset >= b->n_ds) fail(AUGUR_ERR_ARG, "dataset %d out of range", dataset);
+    if (mode != 0 && mode != 1) fail(AUGUR_ERR_ARG, "mode must be 0 (exact HMC) or 1 (surrogate)");
+    if (b->hsmode[dataset] == mode) return;
+    AUGUR_HIP(hipSetDevice(b->device));
+    b->hsmode[dataset] = mode;
+    b->bs_modes_changed();
   });
 }
 

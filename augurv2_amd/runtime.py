@@ -257,6 +257,10 @@ class BhlrSampler(Sampler):
         L.call("augur_bhlr_surrogate_modes", self._h, L.ptr(m), len(self.xs))
         return m
 
+    def surrogate_set_mode(self, i, mode):
+        """Dataset i on exact HMC (0) or on its fitted surrogate (1)."""
+        L.call("augur_bhlr_surrogate_set_mode", self._h, int(i), int(mode))
+
     def surrogate_map(self, i):
         """(W, m, gbar) of dataset i: g(theta) ~ gbar + W (theta - m)."""
         d = int(self.dims[i])
@@ -275,11 +279,16 @@ class BhlrSampler(Sampler):
         L.call("augur_bhlr_surrogate_pairs", self._h, int(i), L.ptr(t), L.ptr(g), n.value, C.byref(n))
         return t, g
 
-    def acceptance(self):
-        """[block (v, theta)][chain][dataset] acceptance rates."""
+    def acceptance_counts(self):
+        """[block (v, theta)][chain][dataset][accepted, proposed] since creation."""
         k = len(self.xs)
         out = np.zeros((2, self.chains, k, 2))
         L.call("augur_stats", self._h, L.ptr(out), out.nbytes)
+        return out
+
+    def acceptance(self):
+        """[block (v, theta)][chain][dataset] acceptance rates."""
+        out = self.acceptance_counts()
         with np.errstate(invalid="ignore"):
             return out[..., 0] / out[..., 1]
 

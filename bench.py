@@ -115,6 +115,23 @@ def _summary(draws, keep=None):
     return float(np.min(summ["ess_bulk"])), float(np.max(summ["rhat"])), summ
 
 
+def _dataset_summaries(draws, offsets, dims):
+    """Per dataset of a batched trace (sweeps, chains, sum (D_i + 2)): min
+    bulk-ESS and max R-hat over theta and v, and the per-coordinate means and
+    MCSEs, computed on a pool of threads (numpy releases the GIL)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    def one(i):
+        o, d = int(offsets[i]), int(dims[i])
+        return _summary(draws[:, :, o:o + d + 2], list(range(d)) + [d + 1])
+
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        res = list(ex.map(one, range(len(dims))))
+    ess = np.array([r[0] for r in res])
+    rhat = np.array([r[1] for r in res])
+    return ess, rhat, [r[2] for r in res]
+
+
 def run_hlr(steps, warmup, seed=2024):
     from augurv2_amd import runtime, workloads
     from augurv2_amd._lib import HMC_BLOCK_THETA, HMC_BLOCK_V
@@ -458,7 +475,8 @@ def run_surrogate(iters=5000, burn=1000, train=1000, seed=77, ablation=True):
     return res
 
 
-def run_batched(sweeps=2000, warmup=1000, chains=4, seed=31, plain=True, plain_sample=6):
+def run_batched(sweeps=2000, warmup=1000, chains=4, seed=31, plain=True, plain_sample=6,
+                surrogate_sweeps=0, surrogate_train=200):
     """256 HLR datasets (N_i ~ U[1k, 200k], D_i in {16, 32, 64}) in one
     batched sampler (BF16 X, autotuned rows per block, hipGraph sweep), 20
     leapfrog steps of 0.005 with the step jittered, MWG [v] ~ Normal(v, 1.0);
@@ -467,6 +485,14 @@ def run_batched(sweeps=2000, warmup=1000, chains=4, seed=31, plain=True, plain_s
     5 on the matrix cores.  Chains start at v = 1, 1.25, 1.5, ... (a draw of
     v from its InvGamma(1, 1) prior can be 10^3, from where a chain random-
     walks down for longer than the warmup).
+
+    With surrogate_sweeps > 0 the same sampler then turns on the linear
+    gradient surrogate: surrogate_train exact sweeps collect each dataset's
+    (theta, gradient) pairs, the maps are fitted, and surrogate_sweeps
+    sweeps run surrogate trajectories with the exact FP64 Metropolis test
+    (a dataset whose acceptance falls below 0.6 over 100 sweeps returns to
+    exact HMC).  Collection and fit are inside its timed span; its posterior
+    means are compared with the exact run's, coordinate by coordinate.
 
     Beside it the plain sampler (FP32 X, one dataset at a time) on a
     stratified sample of datasets, with each dataset's adapted step size and
@@ -498,14 +524,24 @@ def run_batched(sweeps=2000, warmup=1000, chains=4, seed=31, plain=True, plain_s
     el = time.perf_counter() - t0
     draws = s.read_trace("state")  # (sweeps, C, sum(D_i + 2))
     acc = s.acceptance()
+    dims = np.array([x.shape[1] for x in xs])
+    srun = None
+    if surrogate_sweeps:
+        log(f"batched ({chains} chains): surrogate, {surrogate_train} collection sweeps, "
+            f"{surrogate_sweeps} timed")
+        acc_before = s.acceptance_counts()
+        t0 = time.perf_counter()
+        s.surrogate(train_sweeps=surrogate_train, ridge=1e-6, min_accept=0.6, window=100)
+        s.step(surrogate_train)  # exact sweeps that record the pairs, then the fit
+        t_fit = time.perf_counter() - t0
+        s.trace("state", surrogate_sweeps)
+        s.step(surrogate_sweeps)
+        el_s = time.perf_counter() - t0
+        srun = {"draws": s.read_trace("state"), "elapsed_s": el_s, "collect_fit_s": t_fit,
+                "info": s.surrogate_info(), "modes": s.surrogate_modes(),
+                "acc": s.acceptance_counts() - acc_before}
     s.close()
-    ess_i, rhat_i = [], []
-    for i, x in enumerate(xs):
-        o, d = int(s.offsets[i]), x.shape[1]
-        e, r, _ = _summary(draws[:, :, o:o + d + 2], list(range(d)) + [d + 1])
-        ess_i.append(e)
-        rhat_i.append(r)
-    ess_i, rhat_i = np.array(ess_i), np.array(rhat_i)
+    ess_i, rhat_i, summ_i = _dataset_summaries(draws, s.offsets, dims)
     conv = rhat_i < RHAT_MAX
     out = {"datasets": len(xs), "chains": chains, "rows_total": int(sum(x.shape[0] for x in xs)),
            "data_pass": ("matrix cores: BF16 V_MFMA, FP32 accumulation" if chains >= 5
@@ -522,6 +558,42 @@ def run_batched(sweeps=2000, warmup=1000, chains=4, seed=31, plain=True, plain_s
            "min_converged_dataset_ess": float(ess_i[conv].min()) if conv.any() else None,
            "median_dataset_ess": float(np.median(ess_i)),
            "rhat_max": float(np.max(rhat_i)), "accept_theta": float(np.nanmean(acc[1]))}
+    sout = None
+    if srun:
+        ess_s, rhat_s, summ_s = _dataset_summaries(srun["draws"], s.offsets, dims)
+        conv_s = rhat_s < RHAT_MAX
+        # acceptance over the collection and surrogate sweeps only
+        with np.errstate(invalid="ignore"):
+            acc_s = srun["acc"][..., 0] / srun["acc"][..., 1]
+        # posterior means against the exact run, datasets converged in both
+        zs = []
+        for i in np.nonzero(conv & conv_s)[0]:
+            zs.append(np.abs(summ_i[i]["mean"] - summ_s[i]["mean"])
+                      / np.hypot(summ_i[i]["mcse"], summ_s[i]["mcse"]))
+        zs = np.concatenate(zs) if zs else np.zeros(0)
+        modes = srun["modes"]
+        sout = {"collection_sweeps": surrogate_train, "sweeps": surrogate_sweeps,
+                "elapsed_s": srun["elapsed_s"], "collect_fit_s": srun["collect_fit_s"],
+                "ms_per_sweep": 1e3 * (srun["elapsed_s"] - srun["collect_fit_s"]) / surrogate_sweeps,
+                "fit_ms": srun["info"]["fit_ms"],
+                "datasets_on_surrogate": int(modes.sum()),
+                "fallback_datasets": [{"dataset": int(i), "rows": int(xs[i].shape[0]),
+                                       "d": int(xs[i].shape[1])} for i in np.nonzero(modes == 0)[0]],
+                "accept_theta": float(np.nanmean(acc_s[1])),
+                "n_rhat_below_1_01": int(conv_s.sum()),
+                "not_converged": [{"dataset": int(i), "rows": int(xs[i].shape[0]), "d": int(xs[i].shape[1]),
+                                   "rhat": float(rhat_s[i])} for i in np.nonzero(~conv_s)[0]],
+                "rhat_max": float(np.max(rhat_s)),
+                "sum_converged_dataset_min_ess_per_s": float(ess_s[conv_s].sum() / srun["elapsed_s"]),
+                "median_dataset_ess": float(np.median(ess_s)),
+                "moments_vs_exact": {
+                    "coordinates": int(zs.size),
+                    "frac_within_3_mcse": float(np.mean(zs < 3.0)) if zs.size else None,
+                    "max_abs_z": float(zs.max()) if zs.size else None,
+                    "expected_frac_beyond_3_if_equal": 0.0027,
+                    "what": "per coordinate of theta, b, v of every dataset converged in both runs: "
+                            "|mean_exact - mean_surrogate| / hypot(MCSE_exact, MCSE_surrogate)"}}
+        out["surrogate"] = sout
     if not plain:
         return out
     # the plain sampler, one dataset at a time, on datasets spread over the
@@ -529,7 +601,7 @@ def run_batched(sweeps=2000, warmup=1000, chains=4, seed=31, plain=True, plain_s
     work = np.array([x.shape[0] * x.shape[1] for x in xs], np.float64)
     order = [i for i in np.argsort(work) if conv[i]]
     pick = [order[j] for j in np.linspace(0, len(order) - 1, plain_sample).astype(int)]
-    p_sweeps = max(200, sweeps // 4)
+    p_sweeps = max(200, sweeps // 2)
     rows_p = []
     for i in pick:
         p = runtime.HlrSampler(xs[i], ys[i], [(MWG_BLOCK_V, 0.0, float(vvar_i[i]), 0),
@@ -551,6 +623,12 @@ def run_batched(sweeps=2000, warmup=1000, chains=4, seed=31, plain=True, plain_s
                        "plain_s_to_batched_ess": float(ess_i[i]) / (e_p / t_p)})
     samp_t = sum(r["plain_s_to_batched_ess"] for r in rows_p)
     seq_est = samp_t * float(work[conv].sum() / work[pick].sum())
+    if sout:
+        # the same extrapolation to the surrogate run's min-ESS per dataset
+        samp_s = sum(float(ess_s[r["dataset"]]) / r["plain_min_ess_per_s"] for r in rows_p)
+        seq_s = samp_s * float(work[conv_s].sum() / work[pick].sum())
+        sout["plain_sequential_s_to_same_ess_est"] = seq_s
+        sout["min_ess_per_s_ratio_vs_plain_sequential"] = seq_s / srun["elapsed_s"]
     out.update({"plain_sample": rows_p, "plain_sweeps": p_sweeps,
                 "plain_sequential_s_to_same_ess_est": seq_est,
                 "min_ess_per_s_ratio_vs_plain_sequential": seq_est / el,
@@ -572,6 +650,8 @@ def main():
                     "HLR leg: lda, surrogate, batched, gmm, generic (default: all)")
     ap.add_argument("--batched-sweeps", type=int, default=2000,
                     help="timed sweeps of the batched legs (warmup: half as many)")
+    ap.add_argument("--batched-surrogate-sweeps", type=int, default=4000,
+                    help="timed surrogate sweeps after each batched leg (0: none)")
     ap.add_argument("--surrogate-iters", type=int, default=5000,
                     help="timed iterations of each surrogate-leg sampler (shorter for profiling)")
     ap.add_argument("--surrogate-exact", type=int, default=2000,
@@ -625,11 +705,14 @@ def main():
     surr = (run_surrogate(iters=args.surrogate_iters, burn=half, train=half)
             if "surrogate" in legs else None)
     bs = max(1, args.batched_sweeps)
-    bat = run_batched(sweeps=bs, warmup=max(1, bs // 2)) if "batched" in legs else None
+    bss = max(0, args.batched_surrogate_sweeps)
+    bat = (run_batched(sweeps=bs, warmup=max(1, bs // 2), surrogate_sweeps=bss)
+           if "batched" in legs else None)
     if bat:
         log(f"batched: {bat}")
     # 16 chains per dataset: the chains-by-rows products on the matrix cores
-    bat16 = run_batched(sweeps=bs, warmup=max(1, bs // 2), chains=16) if "batched" in legs else None
+    bat16 = (run_batched(sweeps=bs, warmup=max(1, bs // 2), chains=16, surrogate_sweeps=bss)
+             if "batched" in legs else None)
     if bat16:
         log(f"batched (16 chains): {bat16}")
     ok = hlr["rhat_max"] < RHAT_MAX
@@ -651,6 +734,18 @@ def main():
                                              "ratio_whole_job", "config")}
         vs_plain["accelerated_min_ess_per_s"] = surr["surrogate"]["min_ess_per_s"]
         vs_plain["plain_min_ess_per_s"] = surr["plain"]["min_ess_per_s"]
+    for name, b in (("batched_plus_surrogate_4_chains", bat), ("batched_plus_surrogate_16_chains", bat16)):
+        bsr = b.get("surrogate") if b else None
+        if bsr and "min_ess_per_s_ratio_vs_plain_sequential" in bsr:
+            vs_plain = vs_plain or {}
+            vs_plain[name] = {
+                "ratio_vs_plain_sequential": bsr["min_ess_per_s_ratio_vs_plain_sequential"],
+                "ratio_exact_batched_vs_plain_sequential": b["min_ess_per_s_ratio_vs_plain_sequential"],
+                "datasets_at_rhat_below_1_01": bsr["n_rhat_below_1_01"],
+                "moments_frac_within_3_mcse": bsr["moments_vs_exact"]["frac_within_3_mcse"],
+                "config": f"256 HLR datasets, {b['chains']} chains each; timed span = "
+                          f"{bsr['collection_sweeps']} exact collection sweeps + fit + "
+                          f"{bsr['sweeps']} surrogate sweeps; plain = one dataset at a time"}
     out = {
         "metric": "min-ESS/sec on 1 GPU (HLR HMC, R-hat<1.01) vs plain GPU sampler; LDA tokens/sec",
         "value": raw if ok else None,

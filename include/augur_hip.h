@@ -234,6 +234,11 @@ augur_status augur_bhlr_surrogate(augur_sampler* s, const augur_bsurr_cfg* cfg);
 augur_status augur_bhlr_surrogate_info(augur_sampler* s, double* info, size_t bytes);
 /* Per dataset: 1 on the surrogate, 0 exact (n >= n_datasets). */
 augur_status augur_bhlr_surrogate_modes(augur_sampler* s, int32_t* modes, int n);
+/* Puts dataset i on exact HMC (mode 0) or back on its fitted surrogate
+ * (mode 1), e.g. to keep a dataset whose posterior the linear map cannot
+ * follow on exact trajectories.  Needs a fitted surrogate.  Sweeps after it
+ * run the exact datasets' leapfrog passes over their row blocks only. */
+augur_status augur_bhlr_surrogate_set_mode(augur_sampler* s, int dataset, int mode);
 /* Dataset i's fitted map: W (D_i x D_i, row o gives output coordinate o; the
  * FP32 values the trajectories use), m and gbar (D_i each); any pointer may
  * be null. */

@@ -276,6 +276,56 @@ def test_batched_surrogate_sweeps_match_oracle(rt, oracle):
     s.close()
 
 
+@pytest.mark.parametrize("graph", [True, False])
+def test_batched_surrogate_mixed_modes_match_oracle(rt, oracle, graph):
+    """Some datasets on exact HMC, the rest on the surrogate: the exact
+    datasets' leapfrog passes run over their row blocks only, the last pass
+    over every block.  Each dataset's sweep equals the oracle's step of its
+    own mode (exact HMC or the surrogate trajectory), chain by chain; the
+    mode can be switched back."""
+    from oracle import bsurr
+    rng = np.random.default_rng(16)
+    C, L, seed, train = 2, -8, 19, 20
+    xs, ys = _datasets(rng, SHAPES)
+    s = rt.BhlrSampler(xs, ys, chains=C, seed=seed, v_block=("mwg", 1.0), theta_block=(0.1, 0.01, L),
+                       rows_per_block=512, use_graph=graph)
+    _init(s, rng, C)
+    s.adapt(100, 0.8)
+    s.surrogate(train_sweeps=train, ridge=1e-6, min_accept=0.0, window=0)
+    s.step(train + 1)
+    exact = {1, 2, 5}  # 1000 x 32 (two row blocks), 2500 x 64 (five), 64 x 32 (one)
+    for i in exact:
+        s.surrogate_set_mode(i, 0)
+    modes = s.surrogate_modes()
+    assert [int(m) for m in modes] == [0 if i in exact else 1 for i in range(len(SHAPES))]
+    assert s.surrogate_info()["datasets_on_surrogate"] == len(SHAPES) - len(exact)
+    eps, vv = s.step_sizes()
+    maps = [bsurr.linear_grad(*s.surrogate_map(i)) for i in range(len(SHAPES))]
+    models = [oracle.Hlr(bf16_round(x).astype(np.float64), y, prior_kind=0, prior_a=1.0,
+                         prior_b=1.0, ref_compat=False) for x, y in zip(xs, ys)]
+    states = {(i, c): s.get(f"state:{i}", c).copy() for i in range(len(SHAPES)) for c in range(C)}
+    for k in range(4):
+        if k == 2:  # dataset 2 back on its surrogate
+            s.surrogate_set_mode(2, 1)
+            exact.discard(2)
+        sweep = s.sweeps()
+        s.step(1)
+        for i in range(len(SHAPES)):
+            for c in range(C):
+                st = states[i, c]
+                gc = i * C + c
+                models[i].mwg_v_step(st, 0, vv[i], seed, gc, sweep)
+                if i in exact:
+                    models[i].hmc_step(st, 2, 1, 0.1, eps[i], L, seed, gc, sweep)
+                else:
+                    bsurr.hmc_step(models[i], st, maps[i], eps[i], L, seed, gc, sweep)
+                np.testing.assert_allclose(s.get(f"state:{i}", c), st, rtol=2e-5, atol=1e-6,
+                                           err_msg=f"sweep {k} dataset {i} chain {c} exact {i in exact}")
+    with pytest.raises(RuntimeError):
+        s.surrogate_set_mode(len(SHAPES), 0)
+    s.close()
+
+
 def test_batched_surrogate_falls_back_to_exact_hmc(rt, oracle):
     """With an acceptance floor no window can meet, every dataset returns to
     exact HMC after one window, and its later sweeps are the exact step."""
