@@ -507,9 +507,32 @@ __global__ __launch_bounds__(kPassThreads, kPassBlocksPerCU) void k_hlr_pass_t(P
 // theta is requested before the first tiles, so waiting for it never waits on
 // X, and each wave starts on its first tile while the later ones arrive.
 constexpr int kRowFeat = 8;  // features per lane in the register form
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
 // rows per lane per tile and tiles in flight: 32 KB of X per block either way
 __host__ __device__ constexpr int row_rpt(int cp) { return 2; }
 __host__ __device__ constexpr int row_ring(int cp) { return 2; }
+
+// s_waitcnt immediate that waits until at most n vector-memory loads are
+// outstanding (gfx9 encoding: vmcnt bits 3:0 and 15:14; expcnt and lgkmcnt
+// left at their maxima, i.e. not waited on)
+__host__ __device__ constexpr int vmcnt_imm(int n) {
+  return (n & 15) | (7 << 4) | (15 << 8) | (((n >> 4) & 3) << 14);
+}
+
+// Chain whose row term lane q (of a 16-lane row group) evaluates in the
+// matrix-core form of the D = 128, 8-chain pass, and its inverse.  With q's
+// bits b0..b3, rs_chain(q) = b1 ^ 3 b2 ^ 6 b3 (XOR): the lanes q ^ 15, q ^ 7,
+// q ^ 3 and q ^ 1 differ from lane q by 4, 2, 1 and 0 in it, which is what
+// the reduce-scatter's four levels need; lanes 2 h and 2 h + 1 share a chain
+// (slots 0 and 1), and h -> rs_chain(2 h) is a bijection of 0..7.
+__device__ inline int rs_chain(int q) {
+  return (((q >> 1) & 1) * 1) ^ (((q >> 2) & 1) * 3) ^ (((q >> 3) & 1) * 6);
+}
+__device__ inline int rs_lane_of_chain(int c) {
+  // h with rs_chain(2 h) = c: g(h) for h = 0..7 is 0 1 3 2 6 7 5 4
+  constexpr unsigned kInv = 0u | 1u << 3 | 3u << 6 | 2u << 9 | 7u << 12 | 6u << 15 | 4u << 18 | 5u << 21;
+  return (kInv >> (3 * c)) & 7;
+}
 
 __device__ inline float f4c(const float4& v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w; }
 
@@ -520,7 +543,23 @@ constexpr int kHlrRowPad = 256;
 // Blocks per CU the register form is compiled for: at D * CP > 256 its
 // theta and gradient shares (2 CP F floats) and tile ring need up to 256
 // registers a lane, so two blocks (two waves per SIMD) instead of three.
-template <int D, int CP>
+//
+// MB (D = 128, 8 chains; 108 -> 91 us a pass at N = 1M with the changes
+// below, measured against the vector form on one box):
+//  - phase C on the matrix cores.  A wave's four row groups are the K = 4
+//    rows of V_MFMA_F32_16X16X4_F32: lane l supplies A[m = l % 16][k = l /
+//    16] = x[row l / 16][8 m + j] (the value it already holds, product j
+//    covering features 8 m + j) and B[k][n = l % 16] = the adjoint of (row
+//    l / 16, chain n), zero for n >= 8.  The 16 products of a tile (2 row
+//    slots x 8 feature offsets j) accumulate in FP32 as phase C does; lane l
+//    ends with G[8 (4 (l / 16) + r) + j][chain l % 16] in acc[j][r].  Phase
+//    C's 144 vector operations per lane and tile become 16 matrix-core
+//    instructions, which run beside the vector work of phase B;
+//  - phase B ends in a reduce-scatter instead of 16 all-reduces: each lane
+//    keeps the chains in a lane-dependent order (rs_chain), so that four
+//    DPP levels leave it the one (row, chain) sum whose row term it
+//    evaluates (16 DPP adds per tile instead of 64, and no selects).
+template <int D, int CP, bool MB>
 __global__ __launch_bounds__(kPassThreads, (D * CP > 256 ? 2 : kPassBlocksPerCU)) void k_hlr_pass_r(PassArgs A) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int F = kRowFeat, RPT = row_rpt(CP);
@@ -533,6 +572,7 @@ __global__ __launch_bounds__(kPassThreads, (D * CP > 256 ? 2 : kPassBlocksPerCU)
   constexpr int GS = CP * F + 1;                  // LDS floats per lane (padded)
   static_assert(D % F == 0 && LPR >= 2 && LPR <= 16 && LPR % CP == 0, "shape");
   static_assert(kHlrRowPad % TRr == 0, "tiles end inside the row padding");
+  static_assert(!MB || (LPR == 16 && CP == 8 && RPT == 2 && F == 8), "matrix-core phase C shape");
 
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int q = tid % LPR;          // feature group: features F q .. F q + F - 1
@@ -542,13 +582,18 @@ __global__ __launch_bounds__(kPassThreads, (D * CP > 256 ? 2 : kPassBlocksPerCU)
   const int G = gridDim.x, n = A.n;
   const int tiles = (n + TRr - 1) / TRr;
 
-  // theta share of this lane (features F q .., every chain) and b of my_c
+  // MB: lane q's registers hold the chains in the order c ^ gq (see the
+  // reduce-scatter below); it evaluates the row term of (slot q & 1, chain gq)
+  const int gq = MB ? rs_chain(q) : 0;
+  // theta share of this lane (features F q .., every chain) and b of its chain
   float th[CP][F];
 #pragma unroll
   for (int c = 0; c < CP; ++c)
 #pragma unroll
-    for (int j = 0; j < F; ++j) th[c][j] = A.beta32[c * (D + 1) + F * q + j];
-  const float b_c = A.has_b ? A.beta32[my_c * (D + 1) + D] : 0.0f;
+    for (int j = 0; j < F; ++j) th[c][j] = A.beta32[(MB ? c ^ gq : c) * (D + 1) + F * q + j];
+  const float b_c = A.has_b ? A.beta32[(MB ? gq : my_c) * (D + 1) + D] : 0.0f;
+  // MB: lanes whose pairs are (slot 0 / 1, chain q) for the B operands
+  const int src0 = gbase + 2 * rs_lane_of_chain(q & 7);
 
   // PD tiles in flight in NB = PD + 1 register buffers: the refill of a slot
   // goes to the buffer the previous slot consumed, so the loads can be issued
@@ -568,25 +613,37 @@ __global__ __launch_bounds__(kPassThreads, (D * CP > 256 ? 2 : kPassBlocksPerCU)
       yd[i] = A.y[row];
     }
   };
+  // Every prefetch is issued, a tile past the last one clamped to it (its
+  // rows are valid memory, loaded again and not used), so each path
+  // through the loop issues the same loads and the compiler's waits can
+  // count the ring exactly: theta and b, requested before the ring's PD
+  // tiles (3 loads a row), are complete once at most PD * RPT * 3 loads are
+  // outstanding.  With conditional prefetches the waits in the row loop
+  // fell back to zero outstanding loads once per pass over the ring.
+  const int last = tiles - 1;  // n >= 1, so tiles >= 1
 #pragma unroll
-  for (int k = 0; k < PD; ++k) {
-    const int t = blockIdx.x + k * G;
-    if (t < tiles) prefetch(t, pre[k], ypre[k]);
-  }
+  for (int k = 0; k < PD; ++k) prefetch(min((int)blockIdx.x + k * G, last), pre[k], ypre[k]);
+  __builtin_amdgcn_s_waitcnt(vmcnt_imm(PD * RPT * 3));
 
-  float g[CP][F];
+  // vector form: lane gradient shares g[c][j]; matrix-core form: acc[j]
+  float g[MB ? 1 : CP][MB ? 1 : F];
+  f32x4_t acc[MB ? F : 1];
 #pragma unroll
-  for (int c = 0; c < CP; ++c)
+  for (int c = 0; c < (MB ? 1 : CP); ++c)
 #pragma unroll
-    for (int j = 0; j < F; ++j) g[c][j] = 0.0f;
+    for (int j = 0; j < (MB ? 1 : F); ++j) g[c][j] = 0.0f;
+#pragma unroll
+  for (int j = 0; j < (MB ? F : 1); ++j) acc[j] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
   double llacc = 0.0, gbacc = 0.0;
 
   for (int base = blockIdx.x; base < tiles; base += NB * G) {
 #pragma unroll
     for (int slot = 0; slot < NB; ++slot) {
       const int tile = base + slot * G;
-      if (tile >= tiles) break;
-      if (tile + PD * G < tiles) prefetch(tile + PD * G, pre[(slot + PD) % NB], ypre[(slot + PD) % NB]);
+      // issued even when this slot has no tile, so that every path through
+      // the loop issues the same loads
+      prefetch(min(tile + PD * G, last), pre[(slot + PD) % NB], ypre[(slot + PD) % NB]);
+      if (tile >= tiles) continue;
       auto xr = [&](int i, int j) { return f4c(pre[slot][i][j >> 2], j & 3); };
       const int* yv = ypre[slot];
       // phase B: shares of t[i][c] = x_i . theta_c, completed over the row's lanes
@@ -602,12 +659,45 @@ __global__ __launch_bounds__(kPassThreads, (D * CP > 256 ? 2 : kPassBlocksPerCU)
           s1 = fmaf(xr(i, 5), th[c][5], s1);
           s0 = fmaf(xr(i, 6), th[c][6], s0);
           s1 = fmaf(xr(i, 7), th[c][7], s1);
-          tp[i * CP + c] = row_sum<LPR>(s0 + s1);
+          tp[i * CP + c] = MB ? s0 + s1 : row_sum<LPR>(s0 + s1);
         }
-      // adjoints: pair m = q + s LPR is row slot m / CP, chain m % CP = my_c
       float aown[SPL];
+      if constexpr (MB) {
+        // Reduce-scatter of the 16 (slot, chain) shares over the row's 16
+        // lanes: register 8 s + c of lane q holds pair (s, c ^ gq), and at
+        // each level the partner lane holds the pair of register k in k ^ 4
+        // (row mirror, q ^ 15), k ^ 2 (half-row mirror, q ^ 7), k ^ 1
+        // (q ^ 3), then the same pairs (q ^ 1): 16 DPP adds instead of 64,
+        // leaving t of (slot s, chain gq) in register 8 s.
 #pragma unroll
-      for (int s = 0; s < SPL; ++s) {
+        for (int i = 0; i < 2; ++i) {
+#pragma unroll
+          for (int c = 0; c < 4; ++c) tp[8 * i + c] += dpp_f<0x140>(tp[8 * i + c + 4]);
+#pragma unroll
+          for (int c = 0; c < 2; ++c) tp[8 * i + c] += dpp_f<0x141>(tp[8 * i + c + 2]);
+          tp[8 * i] += dpp_f<0x1B>(tp[8 * i + 1]);
+          tp[8 * i] += dpp_f<0xB1>(tp[8 * i]);
+        }
+        // bit blends rather than selects: the compiler turns a select of two
+        // array elements into a dynamically indexed load, which sends the
+        // whole array to scratch memory
+        const int i = q & 1, im = -i;
+        const float tv = __int_as_float(__float_as_int(tp[0]) ^ ((__float_as_int(tp[0]) ^ __float_as_int(tp[8])) & im));
+        const int yy = yv[0] ^ ((yv[0] ^ yv[1]) & im);
+        const bool ok = tile * TRr + i * RG + rsub < n;
+        float a;
+        const float lr = log_sigmoid_pm(tv + b_c, yy, &a);
+        if (ok) {
+          llacc += (double)lr;
+          gbacc += (double)a;
+        } else {
+          a = 0.0f;
+        }
+        aown[0] = a;
+      }
+      // adjoints: pair m = q + s LPR is row slot m / CP, chain m % CP = my_c
+#pragma unroll
+      for (int s = 0; s < (MB ? 0 : SPL); ++s) {
         const int m = q + s * LPR;
         float tv = tp[0];
 #pragma unroll
@@ -627,13 +717,24 @@ __global__ __launch_bounds__(kPassThreads, (D * CP > 256 ? 2 : kPassBlocksPerCU)
         }
         aown[s] = a;
       }
-      // phase C: g[c] += a[i][c] x_i over the lane's features
+      if constexpr (MB) {
+        // B operands: the adjoints of (slot 0, chain q) and (slot 1, chain q)
+        const float a0 = __shfl(aown[0], src0), a1 = __shfl(aown[0], src0 + 1);
+        const float b0 = q < CP ? a0 : 0.0f, b1 = q < CP ? a1 : 0.0f;
 #pragma unroll
-      for (int m = 0; m < NC; ++m) {
-        const float av = __shfl(aown[m / LPR], gbase + m % LPR);
-        const int i = m / CP, c = m % CP;
+        for (int j = 0; j < F; ++j) {
+          acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(xr(0, j), b0, acc[j], 0, 0, 0);
+          acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(xr(1, j), b1, acc[j], 0, 0, 0);
+        }
+      } else {
+        // phase C: g[c] += a[i][c] x_i over the lane's features
 #pragma unroll
-        for (int j = 0; j < F; ++j) g[c][j] = fmaf(av, xr(i, j), g[c][j]);
+        for (int m = 0; m < NC; ++m) {
+          const float av = __shfl(aown[m / LPR], gbase + m % LPR);
+          const int i = m / CP, c = m % CP;
+#pragma unroll
+          for (int j = 0; j < F; ++j) g[c][j] = fmaf(av, xr(i, j), g[c][j]);
+        }
       }
     }
   }
@@ -642,29 +743,58 @@ __global__ __launch_bounds__(kPassThreads, (D * CP > 256 ? 2 : kPassBlocksPerCU)
   constexpr int VS = D + 2;
   float* gl = (float*)smem;                                      // [256][GS]
   double* red = (double*)(smem + ((sizeof(float) * kPassThreads * GS + 15) & ~(size_t)15));  // [2][4][CP]
+  if constexpr (MB) {
+    // the wave's sums: gl[(wave * CP + c) * D + feature]
+    if (q < CP) {
 #pragma unroll
-  for (int c = 0; c < CP; ++c)
+      for (int j = 0; j < F; ++j)
 #pragma unroll
-    for (int j = 0; j < F; ++j) gl[tid * GS + c * F + j] = g[c][j];
-  // ll and intercept gradient: lanes with the same chain (lane = c mod CP)
+        for (int r = 0; r < 4; ++r) gl[(wv * CP + q) * D + F * (4 * (lane >> 4) + r) + j] = acc[j][r];
+    }
+  } else {
 #pragma unroll
-  for (int off = 32; off >= CP; off >>= 1) {
-    gbacc += __shfl_xor(gbacc, off);
-    llacc += __shfl_xor(llacc, off);
+    for (int c = 0; c < CP; ++c)
+#pragma unroll
+      for (int j = 0; j < F; ++j) gl[tid * GS + c * F + j] = g[c][j];
   }
-  if (lane < CP) {
-    red[wv * CP + lane] = gbacc;
-    red[4 * CP + wv * CP + lane] = llacc;
+  // ll and intercept gradient: lanes with the same chain (lane = c mod CP;
+  // MB: lanes 2 h and 2 h + 1 of every row group hold chain rs_chain(2 h))
+  if constexpr (MB) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int off = k == 0 ? 32 : k == 1 ? 16 : 1;
+      gbacc += __shfl_xor(gbacc, off);
+      llacc += __shfl_xor(llacc, off);
+    }
+    if (lane < 16 && !(lane & 1)) {
+      red[wv * CP + gq] = gbacc;
+      red[4 * CP + wv * CP + gq] = llacc;
+    }
+  } else {
+#pragma unroll
+    for (int off = 32; off >= CP; off >>= 1) {
+      gbacc += __shfl_xor(gbacc, off);
+      llacc += __shfl_xor(llacc, off);
+    }
+    if (lane < CP) {
+      red[wv * CP + lane] = gbacc;
+      red[4 * CP + wv * CP + lane] = llacc;
+    }
   }
   __syncthreads();
   double* pb = A.part + (size_t)blockIdx.x * CP * VS;
   for (int v = tid; v < CP * D; v += kPassThreads) {
     const int c = v / D, j = v % D, qq = j / F, jj = j % F;
     double s = 0.0;
+    if constexpr (MB) {
 #pragma unroll
-    for (int w = 0; w < kPassThreads / 64; ++w)
+      for (int w = 0; w < kPassThreads / 64; ++w) s += (double)gl[(w * CP + c) * D + j];
+    } else {
 #pragma unroll
-      for (int k = 0; k < 64 / LPR; ++k) s += (double)gl[(w * 64 + k * LPR + qq) * GS + c * F + jj];
+      for (int w = 0; w < kPassThreads / 64; ++w)
+#pragma unroll
+        for (int k = 0; k < 64 / LPR; ++k) s += (double)gl[(w * 64 + k * LPR + qq) * GS + c * F + jj];
+    }
     pb[c * VS + j] = s;
   }
   if (tid < CP) {
@@ -712,13 +842,19 @@ PassKernel pick_pass(int d, int cp) {
 }
 
 // Register form of the pass for (d, cp), if compiled.
+// At D = 128 and 8 chains phase C runs on the matrix cores unless
+// AUGUR_HLR_VALU_BACK=1.
 PassKernel pick_pass_r(int d, int cp) {
-#define AUGUR_PASS(D_, C_) if (d == D_ && cp == C_) return k_hlr_pass_r<D_, C_>;
+#define AUGUR_PASS(D_, C_) if (d == D_ && cp == C_) return k_hlr_pass_r<D_, C_, false>;
   AUGUR_PASS(16, 1) AUGUR_PASS(16, 2)
   AUGUR_PASS(32, 1) AUGUR_PASS(32, 2) AUGUR_PASS(32, 4)
   AUGUR_PASS(64, 1) AUGUR_PASS(64, 2) AUGUR_PASS(64, 4)
-  AUGUR_PASS(128, 1) AUGUR_PASS(128, 2) AUGUR_PASS(128, 4) AUGUR_PASS(128, 8)
+  AUGUR_PASS(128, 1) AUGUR_PASS(128, 2) AUGUR_PASS(128, 4)
 #undef AUGUR_PASS
+  if (d == 128 && cp == 8) {
+    const char* e = getenv("AUGUR_HLR_VALU_BACK");
+    return (e && atoi(e)) ? k_hlr_pass_r<128, 8, false> : k_hlr_pass_r<128, 8, true>;
+  }
   return nullptr;
 }
 
