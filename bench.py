@@ -472,7 +472,59 @@ def run_surrogate(iters=5000, burn=1000, train=1000, seed=77, ablation=True):
                            "mlp_min_ess_per_s": q["min_ess_per_s"]}
     for mode, r in out.items():
         res[mode] = {k: v for k, v in r.items() if not k.startswith("post_")}
+    if ablation:
+        res["ablation_skewed_posterior"] = run_surrogate_skewed()
     return res
+
+
+def run_surrogate_skewed(sweeps=3000, seed=1):
+    """The network's contribution where the linear map falls short: HLR with
+    N = 300 rows, D = 8 and coefficients of scale 1.5 (a small, strongly
+    informative dataset: a skewed posterior, a gradient far from linear in
+    theta).  MWG [v] (*) HMC [theta] with 10 jittered leapfrog steps of 0.06,
+    4 chains; exact HMC, the linear surrogate and the network surrogate, each
+    300 warmup + 300 collection sweeps, then `sweeps` timed sweeps."""
+    from augurv2_amd import _lib, runtime
+    from augurv2_amd._lib import HMC_BLOCK_THETA, MWG_BLOCK_V
+    rng = np.random.default_rng(seed)
+    n, d, C = 300, 8, 4
+    x = rng.standard_normal((n, d)).astype(np.float32)
+    beta = rng.standard_normal(d) * 1.5
+    y = (rng.random(n) < 1 / (1 + np.exp(-(x @ beta)))).astype(np.int32)
+    keep = list(range(d)) + [d + 1]
+    out = {}
+    for mode in ("exact", "linear_only", "mlp"):
+        s = runtime.HlrSampler(x, y, [(MWG_BLOCK_V, 0.0, 0.5, 0), (HMC_BLOCK_THETA, 0.6, 0.06, 0)],
+                               chains=C, seed=5, ref_compat=False)
+        for c in range(C):
+            s.set("state", np.concatenate([np.zeros(d), [0.0], [1.0]]), chain=c)
+        s.step(300)
+        t0 = time.perf_counter()
+        if mode != "exact":
+            s.enable_surrogate(1, hidden=64, train_sweeps=300, max_pairs=400_000, epochs=60, batch=256,
+                               lr=2e-3, min_accept=0.0, window=100_000, seed=3,
+                               linear_only=(mode == "linear_only"))
+        s.step(300)
+        before = np.zeros((2, C, 2))
+        _lib.call("augur_stats", s._h, _lib.ptr(before), before.nbytes)
+        s.trace("state", sweeps)
+        t1 = time.perf_counter()
+        s.step(sweeps)
+        t2 = time.perf_counter()
+        after = np.zeros((2, C, 2))
+        _lib.call("augur_stats", s._h, _lib.ptr(after), after.nbytes)
+        ess, rhat, _ = _summary(s.read_trace("state"), keep)
+        info = s.surrogate_info() if mode != "exact" else {}
+        s.close()
+        acc = float((after[1, :, 0] - before[1, :, 0]).sum() / (after[1, :, 1] - before[1, :, 1]).sum())
+        out[mode] = {"accept": acc, "min_ess": ess, "rhat_max": rhat, "ms_per_sweep": 1e3 * (t2 - t1) / sweeps,
+                     "min_ess_per_s": ess / (t2 - t0) if rhat < RHAT_MAX else None,
+                     "min_ess_per_s_sampling_only": ess / (t2 - t1) if rhat < RHAT_MAX else None,
+                     "network_loss": info.get("loss"), "linear_resid_frac": info.get("resid_frac")}
+    out["workload"] = ("HLR N=300, D=8, coefficients ~ N(0, 1.5^2); 10 jittered leapfrog steps of 0.06; "
+                       "min-ESS over the timed sweeps, per second from the start of collection (training "
+                       "included) and per second of the timed sweeps alone")
+    return out
 
 
 def run_batched(sweeps=2000, warmup=1000, chains=4, seed=31, plain=True, plain_sample=6,
