@@ -48,6 +48,7 @@ typedef float gmm_f32x4 __attribute__((ext_vector_type(4)));
 
 struct GmmDev {
   int n, d, k, G, S;       // S = 1 + d + d(d+1)/2 statistics per component
+  int Gs;                  // blocks per chain of the statistics kernel (rows of part)
   const float* y;          // n x d
   const double* alpha;     // k
   const double* mu0;       // d
@@ -63,7 +64,7 @@ struct GmmDev {
   double* prec0;   // d x d          cov0^-1
   int TS, DM;
   int cus;         // compute units of the device
-  float* part;     // C x G x k x S
+  float* part;     // C x Gs x k x S
   int32_t* z;      // C x n
   // the mu block's move when it is not ConjGibbs (GMM_MU_*)
   int mu_mode, ref_compat, mu_fixed_steps;
@@ -342,17 +343,17 @@ __global__ __launch_bounds__(64 * kSumSlices) void k_gmm_stats_sum(GmmDev m, dou
   const int i = blockIdx.x * 64 + vi, k = blockIdx.y, c = blockIdx.z, S = m.S;
   double s = 0.0;
   if (i < S) {
-    const float* col = m.part + ((size_t)c * m.G * m.k + k) * S + i;
+    const float* col = m.part + ((size_t)c * m.Gs * m.k + k) * S + i;
     const size_t row = (size_t)m.k * S;
     int g = sl;
-    for (; g + 7 * kSumSlices < m.G; g += 8 * kSumSlices) {
+    for (; g + 7 * kSumSlices < m.Gs; g += 8 * kSumSlices) {
       float t[8];
 #pragma unroll
       for (int u = 0; u < 8; ++u) t[u] = col[(size_t)(g + u * kSumSlices) * row];
 #pragma unroll
       for (int u = 0; u < 8; ++u) s += (double)t[u];
     }
-    for (; g < m.G; g += kSumSlices) s += (double)col[(size_t)g * row];
+    for (; g < m.Gs; g += kSumSlices) s += (double)col[(size_t)g * row];
   }
   red[sl][vi] = s;
   __syncthreads();
@@ -1294,8 +1295,167 @@ __global__ __launch_bounds__(kGmmThreads) void k_gmm_stats_t(GmmDev m) {
     }
     __syncthreads();
   }
-  float* dst = m.part + ((size_t)c * m.G + blockIdx.x) * K * S;
+  float* dst = m.part + ((size_t)c * m.Gs + blockIdx.x) * K * S;
   for (int i = tid; i < K * S; i += kGmmThreads) dst[i] = red[i];
+}
+
+// Sorted form of the statistics for padded dimension 16.  The one-hot
+// products of k_gmm_stats_t spend 31 of 32 matrix-core products on zeros and
+// most of their vector work splitting e e^T into BF16 parts.  Here each block
+// takes tiles of 256 points (one per thread: its y row, read coalesced, and
+// its z, both requested one tile ahead), sorts them by component with a
+// counting sort from wave ballots (position = component, then wave, then
+// lane: the same order, so the same FP32 sums, on every run), and writes the
+// residuals e = y - mu_z in that order to LDS.  Each wave then accumulates
+// its components' segments with FP32 matrix-core outer products,
+// V_MFMA_F32_16X16X4_F32 on four points at a time: lane l supplies e_p[l %
+// 16] of point p = the (l / 16)-th of the four as both A (16 dims x 4
+// points) and B (4 points x 16 dims), adding sum e e^T, and with B = 1
+// (valid points) sum e.  Wave w owns components w, w + 4, ...: its sums stay
+// in accumulation registers over the block's tiles, one row of partials per
+// block and component as k_gmm_stats_t writes them.  Grid (Gs, chains), one
+// resident round: block (x, c) is dispatched at x + Gs c, on XCD x mod 8 for
+// every chain when Gs is a multiple of 8, so a tile's y rows read by one
+// chain are in that XCD's L2 for the others.
+template <int KM>
+__global__ __launch_bounds__(kGmmThreads) void k_gmm_stats_s(GmmDev m) {
+  constexpr int DM = 16, TS = tri(DM) + DM + 1, TP = kGmmThreads, NW = kGmmThreads / 64;
+  constexpr int KPW = (KM + NW - 1) / NW;  // components per wave
+  __shared__ __align__(16) float es[TP * DM];  // the tile's residuals, sorted by component
+  __shared__ float mus[KM * DM];               // the table's mu of every component
+  __shared__ int wcnt[NW][KM];                 // per wave and component: count, then offset
+  __shared__ int koff[KM + 1];                 // component segments in es
+  const int K = m.k, d = m.d, S = m.S, c = blockIdx.y, tid = threadIdx.x;
+  const int lane = tid & 63, wv = tid >> 6, dim = lane & 15, quad = lane >> 4;
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (int i = tid; i < K * DM; i += kGmmThreads) {
+    const int k = i / DM, j = i % DM;
+    mus[i] = j < d ? m.tab[((size_t)c * K + k) * TS + tri(DM) + j] : 0.0f;
+  }
+  gmm_f32x4 see[KPW], se[KPW];
+  float cnt[KPW];
+#pragma unroll
+  for (int j = 0; j < KPW; ++j) {
+    see[j] = gmm_f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    se[j] = gmm_f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    cnt[j] = 0.0f;
+  }
+  const int32_t* __restrict__ zc = m.z + (size_t)c * m.n;
+  const int tiles = (m.n + TP - 1) / TP;
+  // this thread's point of tile t: z (-1 past the end) and its y row
+  float yr[DM];
+  int zk = -1;
+  auto fetch = [&](int t, float (&row)[DM], int& zo) {
+    const int p = t * TP + tid;
+    zo = -1;
+#pragma unroll
+    for (int i = 0; i < DM; ++i) row[i] = 0.0f;
+    if (t < tiles && p < m.n) {
+      zo = zc[p];
+      const float* yp = m.y + (size_t)p * d;
+      if (d == DM) {
+#pragma unroll
+        for (int i = 0; i < DM; i += 4) {
+          const float4 q = *(const float4*)(yp + i);
+          row[i] = q.x; row[i + 1] = q.y; row[i + 2] = q.z; row[i + 3] = q.w;
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < DM; ++i) row[i] = i < d ? yp[i] : 0.0f;
+      }
+    }
+  };
+  fetch(blockIdx.x, yr, zk);
+  __syncthreads();  // mus
+  for (int t = blockIdx.x; t < tiles; t += gridDim.x) {
+    // counting sort: rank among this wave's lanes of the same component (a
+    // value outside 0 .. K - 1 matches none and counts nowhere)
+    int myrank = 0;
+#pragma unroll 8
+    for (int k = 0; k < KM; ++k) {
+      if (k < K) {
+        const uint64_t b = __ballot(zk == k);
+        if (zk == k) myrank = __popcll(b & below);
+        if (lane == 0) wcnt[wv][k] = __popcll(b);
+      }
+    }
+    __syncthreads();
+    if (wv == 0) {
+      // component k = lane (KM <= 64): totals, an exclusive scan over the
+      // components, then the waves' offsets
+      const int k = lane;
+      int tot = 0;
+      if (k < K)
+#pragma unroll
+        for (int w = 0; w < NW; ++w) tot += wcnt[w][k];
+      int incl = tot;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(incl, o);
+        if (lane >= o) incl += v;
+      }
+      if (k < K) {
+        int off = incl - tot;
+        koff[k] = off;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+          const int cw = wcnt[w][k];
+          wcnt[w][k] = off;
+          off += cw;
+        }
+      }
+      if (k == K - 1) koff[K] = incl;
+    }
+    __syncthreads();
+    if (zk >= 0 && zk < K) {
+      float* dst = es + (wcnt[wv][zk] + myrank) * DM;
+      const float* mk = mus + zk * DM;
+#pragma unroll
+      for (int i = 0; i < DM; i += 4) {
+        float4 v;
+        v.x = yr[i] - mk[i];
+        v.y = yr[i + 1] - mk[i + 1];
+        v.z = yr[i + 2] - mk[i + 2];
+        v.w = yr[i + 3] - mk[i + 3];
+        *(float4*)(dst + i) = v;
+      }
+    }
+    // the next tile's point, requested before this tile's products
+    fetch(t + gridDim.x, yr, zk);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < KPW; ++j) {
+      const int k = wv + NW * j;
+      if (k < K) {  // uniform over the wave
+        const int beg = koff[k], end = koff[k + 1];
+        cnt[j] += (float)(end - beg);
+        for (int i0 = beg; i0 < end; i0 += 4) {
+          const int pt = i0 + quad;
+          const bool ok = pt < end;
+          const float e = ok ? es[min(pt, TP - 1) * DM + dim] : 0.0f;
+          see[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(e, e, see[j], 0, 0, 0);
+          se[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(e, ok ? 1.0f : 0.0f, se[j], 0, 0, 0);
+        }
+      }
+    }
+    __syncthreads();  // es, wcnt and koff are rewritten by the next tile
+  }
+  // lane l holds sum over points of e[4 (l / 16) + r] e[l % 16] in see[j][r]
+  // and sum e[4 (l / 16) + r] (every column alike) in se[j][r]
+  float* dst = m.part + ((size_t)c * m.Gs + blockIdx.x) * K * S;
+#pragma unroll
+  for (int j = 0; j < KPW; ++j) {
+    const int k = wv + NW * j;
+    if (k >= K) continue;
+    float* row = dst + (size_t)k * S;
+    if (lane == 0) row[0] = cnt[j];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int a = 4 * quad + r, b = dim;
+      if (a < d && b == 0) row[1 + a] = se[j][r];
+      if (a < d && b <= a) row[1 + d + tri(a) + b] = see[j][r];
+    }
+  }
 }
 
 // Dynamic LDS of k_gmm_stats_t: the per-wave slabs, reused for the block's
@@ -1506,10 +1666,19 @@ struct GmmSampler final : augur_sampler {
     AUGUR_CHECK_LAUNCH();
   }
 
-  // statistics of the current z around the tables' mu
+  // statistics of the current z around the tables' mu: the sorted form for
+  // padded dimension 16 and up to 32 components unless
+  // AUGUR_GMM_STATS_ONEHOT=1 (at 64 components its 16 per wave need more
+  // registers than one wave per SIMD leaves)
   void launch_stats() {
-    pick_stats(m.DM, pad_comp(m.k))<<<dim3(m.G, chains), kGmmThreads,
-                                      gmm_stats_smem_bytes(m.DM, pad_comp(m.k), m.S), stream>>>(m);
+    const int KM = pad_comp(m.k);
+    const char* e = getenv("AUGUR_GMM_STATS_ONEHOT");
+    if (m.DM == 16 && KM <= 32 && !(e && atoi(e))) {
+      StatsKernel k = KM == 8 ? k_gmm_stats_s<8> : KM == 16 ? k_gmm_stats_s<16> : k_gmm_stats_s<32>;
+      k<<<dim3(m.Gs, chains), kGmmThreads, 0, stream>>>(m);
+    } else {
+      pick_stats(m.DM, KM)<<<dim3(m.Gs, chains), kGmmThreads, gmm_stats_smem_bytes(m.DM, KM, m.S), stream>>>(m);
+    }
     AUGUR_CHECK_LAUNCH();
   }
 
@@ -1666,6 +1835,18 @@ static void gmm_setup(GmmSampler* s, int n, int d, int k, int chains) {
     AUGUR_HIP(hipDeviceGetAttribute(&m.cus, hipDeviceAttributeMultiprocessorCount, dev));
     m.cus = std::max(m.cus, 1);
   }
+  // the sorted statistics kernel keeps its sums in registers over its tiles:
+  // one resident round of blocks over all chains (a second round would
+  // double its time), a multiple of 8 per chain
+  m.Gs = m.G;
+  if (m.DM == 16 && pad_comp(k) <= 32) {
+    const void* sk = pad_comp(k) == 8 ? (const void*)k_gmm_stats_s<8>
+                   : pad_comp(k) == 16 ? (const void*)k_gmm_stats_s<16> : (const void*)k_gmm_stats_s<32>;
+    int bpc = 0;
+    AUGUR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, sk, kGmmThreads, 0));
+    const int g = std::max(1, bpc) * m.cus / chains;
+    m.Gs = std::max(1, std::min(want, g >= 8 ? g / 8 * 8 : g));
+  }
   s->chains = chains;
   s->y.alloc((size_t)n * d);
   s->alpha.alloc(k); s->mu0.alloc(d); s->cov0.alloc((size_t)d * d); s->scale0.alloc((size_t)d * d);
@@ -1674,7 +1855,7 @@ static void gmm_setup(GmmSampler* s, int n, int d, int k, int chains) {
   s->tab.alloc((size_t)chains * k * m.TS);
   s->prec.alloc((size_t)chains * k * d * d);
   s->prec0.alloc((size_t)d * d);
-  s->part.alloc((size_t)chains * m.G * k * m.S);
+  s->part.alloc((size_t)chains * m.Gs * k * m.S);
   s->ssum.alloc((size_t)chains * k * m.S);
   s->z.alloc((size_t)chains * n);
   s->ll.alloc((size_t)chains * (1 + m.G + k));  // per-chain sums, data partials, priors
