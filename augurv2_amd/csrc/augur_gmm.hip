@@ -1302,7 +1302,7 @@ __global__ __launch_bounds__(kGmmThreads) void k_gmm_stats_t(GmmDev m) {
 // Sorted form of the statistics for padded dimension 16.  The one-hot
 // products of k_gmm_stats_t spend 31 of 32 matrix-core products on zeros and
 // most of their vector work splitting e e^T into BF16 parts.  Here each block
-// takes tiles of 256 points (one per thread: its y row, read coalesced, and
+// takes tiles of NT points (one per thread: its y row, read coalesced, and
 // its z, both requested one tile ahead), sorts them by component with a
 // counting sort from wave ballots (position = component, then wave, then
 // lane: the same order, so the same FP32 sums, on every run), and writes the
@@ -1311,15 +1311,15 @@ __global__ __launch_bounds__(kGmmThreads) void k_gmm_stats_t(GmmDev m) {
 // V_MFMA_F32_16X16X4_F32 on four points at a time: lane l supplies e_p[l %
 // 16] of point p = the (l / 16)-th of the four as both A (16 dims x 4
 // points) and B (4 points x 16 dims), adding sum e e^T, and with B = 1
-// (valid points) sum e.  Wave w owns components w, w + 4, ...: its sums stay
+// (valid points) sum e.  Wave w of NW owns components w, w + NW, ...: its sums stay
 // in accumulation registers over the block's tiles, one row of partials per
 // block and component as k_gmm_stats_t writes them.  Grid (Gs, chains), one
 // resident round: block (x, c) is dispatched at x + Gs c, on XCD x mod 8 for
 // every chain when Gs is a multiple of 8, so a tile's y rows read by one
 // chain are in that XCD's L2 for the others.
-template <int KM>
-__global__ __launch_bounds__(kGmmThreads) void k_gmm_stats_s(GmmDev m) {
-  constexpr int DM = 16, TS = tri(DM) + DM + 1, TP = kGmmThreads, NW = kGmmThreads / 64;
+template <int KM, int NT>
+__global__ __launch_bounds__(NT) void k_gmm_stats_s(GmmDev m) {
+  constexpr int DM = 16, TS = tri(DM) + DM + 1, TP = NT, NW = NT / 64;
   constexpr int KPW = (KM + NW - 1) / NW;  // components per wave
   __shared__ __align__(16) float es[TP * DM];  // the tile's residuals, sorted by component
   __shared__ float mus[KM * DM];               // the table's mu of every component
@@ -1328,7 +1328,7 @@ __global__ __launch_bounds__(kGmmThreads) void k_gmm_stats_s(GmmDev m) {
   const int K = m.k, d = m.d, S = m.S, c = blockIdx.y, tid = threadIdx.x;
   const int lane = tid & 63, wv = tid >> 6, dim = lane & 15, quad = lane >> 4;
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  for (int i = tid; i < K * DM; i += kGmmThreads) {
+  for (int i = tid; i < K * DM; i += NT) {
     const int k = i / DM, j = i % DM;
     mus[i] = j < d ? m.tab[((size_t)c * K + k) * TS + tri(DM) + j] : 0.0f;
   }
@@ -1586,6 +1586,15 @@ static int pad_comp(int k) { return k <= 8 ? 8 : (k <= 16 ? 16 : (k <= 32 ? 32 :
 typedef void (*ZKernel)(GmmDev, uint32_t, uint32_t, uint32_t);
 typedef void (*StatsKernel)(GmmDev);
 
+// Threads (and points per tile) of the sorted statistics kernel: 512 gave
+// 0.727 ms a GMM sweep at N = 1M, K = 32, 4 chains, against 0.741 ms for
+// 256 and for 1024 (same box).
+constexpr int kStatsThreads = 512;
+static StatsKernel k_stats_s(int KM) {
+  return KM == 8 ? k_gmm_stats_s<8, kStatsThreads> : KM == 16 ? k_gmm_stats_s<16, kStatsThreads>
+                                                   : k_gmm_stats_s<32, kStatsThreads>;
+}
+
 static ZKernel pick_z(int DM, int KM) {
 #define AUGUR_Z(D_, K_) if (DM == D_ && KM == K_) return k_gmm_z<D_, K_>;
   AUGUR_Z(4, 8) AUGUR_Z(4, 16) AUGUR_Z(4, 32) AUGUR_Z(4, 64)
@@ -1674,8 +1683,7 @@ struct GmmSampler final : augur_sampler {
     const int KM = pad_comp(m.k);
     const char* e = getenv("AUGUR_GMM_STATS_ONEHOT");
     if (m.DM == 16 && KM <= 32 && !(e && atoi(e))) {
-      StatsKernel k = KM == 8 ? k_gmm_stats_s<8> : KM == 16 ? k_gmm_stats_s<16> : k_gmm_stats_s<32>;
-      k<<<dim3(m.Gs, chains), kGmmThreads, 0, stream>>>(m);
+      k_stats_s(KM)<<<dim3(m.Gs, chains), kStatsThreads, 0, stream>>>(m);
     } else {
       pick_stats(m.DM, KM)<<<dim3(m.Gs, chains), kGmmThreads, gmm_stats_smem_bytes(m.DM, KM, m.S), stream>>>(m);
     }
@@ -1840,12 +1848,12 @@ static void gmm_setup(GmmSampler* s, int n, int d, int k, int chains) {
   // double its time), a multiple of 8 per chain
   m.Gs = m.G;
   if (m.DM == 16 && pad_comp(k) <= 32) {
-    const void* sk = pad_comp(k) == 8 ? (const void*)k_gmm_stats_s<8>
-                   : pad_comp(k) == 16 ? (const void*)k_gmm_stats_s<16> : (const void*)k_gmm_stats_s<32>;
     int bpc = 0;
-    AUGUR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, sk, kGmmThreads, 0));
+    AUGUR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, (const void*)k_stats_s(pad_comp(k)),
+                                                           kStatsThreads, 0));
     const int g = std::max(1, bpc) * m.cus / chains;
-    m.Gs = std::max(1, std::min(want, g >= 8 ? g / 8 * 8 : g));
+    const int tiles = (n + kStatsThreads - 1) / kStatsThreads;
+    m.Gs = std::max(1, std::min(tiles, g >= 8 ? g / 8 * 8 : g));
   }
   s->chains = chains;
   s->y.alloc((size_t)n * d);

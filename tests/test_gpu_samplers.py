@@ -356,11 +356,13 @@ def test_gmm_draws_do_not_depend_on_the_z_grid(rt, monkeypatch):
 
 @pytest.mark.parametrize("d,k,n,sample_pi,sample_cov", [
     (4, 3, 3000, True, True), (4, 3, 3000, False, False),
-    # the benchmark's shape (padded dimension 16, 32 components: the
-    # matrix-core statistics kernel with two 16-component blocks, the 16 x 16
-    # FP64 conjugate draws and the matrix-core z step)
+    # the benchmark's shape (padded dimension 16, 32 components: the sorted
+    # matrix-core statistics kernel, the 16 x 16 FP64 conjugate draws and
+    # the matrix-core z step)
     (16, 32, 6000, True, True),
     (16, 20, 4000, True, True),   # 20 of 32 padded components live
+    (12, 10, 3000, True, True),   # 12 of 16 padded dimensions: the sorted kernel's zero columns
+    (16, 40, 4000, True, True),   # 64 padded components: the one-hot statistics kernel
     (8, 40, 4000, True, True),    # padded dimension 8, 64 padded components
 ])
 def test_gmm_sweeps_match_oracle(rt, oracle, d, k, n, sample_pi, sample_cov):
