@@ -702,8 +702,9 @@ def main():
                     "HLR leg: lda, surrogate, batched, gmm, generic (default: all)")
     ap.add_argument("--batched-sweeps", type=int, default=2000,
                     help="timed sweeps of the batched legs (warmup: half as many)")
-    ap.add_argument("--batched-surrogate-sweeps", type=int, default=4000,
-                    help="timed surrogate sweeps after each batched leg (0: none)")
+    ap.add_argument("--batched-surrogate-sweeps", type=int, default=10_000,
+                    help="timed surrogate sweeps after the 4-chain batched leg, half as many after "
+                         "the 16-chain one (0: none)")
     ap.add_argument("--surrogate-iters", type=int, default=5000,
                     help="timed iterations of each surrogate-leg sampler (shorter for profiling)")
     ap.add_argument("--surrogate-exact", type=int, default=2000,
@@ -763,7 +764,7 @@ def main():
     if bat:
         log(f"batched: {bat}")
     # 16 chains per dataset: the chains-by-rows products on the matrix cores
-    bat16 = (run_batched(sweeps=bs, warmup=max(1, bs // 2), chains=16, surrogate_sweeps=bss)
+    bat16 = (run_batched(sweeps=bs, warmup=max(1, bs // 2), chains=16, surrogate_sweeps=bss // 2)
              if "batched" in legs else None)
     if bat16:
         log(f"batched (16 chains): {bat16}")
