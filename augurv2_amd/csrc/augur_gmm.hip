@@ -994,6 +994,22 @@ __global__ __launch_bounds__(kGmmThreads) void k_gmm_z(GmmDev m, uint32_t seed, 
 
 typedef __bf16 gmm_bf16x8 __attribute__((ext_vector_type(8)));
 
+// V_PERMLANE16_SWAP_B32 / V_PERMLANE32_SWAP_B32 on a register pair (rows of
+// 16 lanes, checked on the device with an in-bounds probe): after
+// swap_rows16, x = [x.r0, y.r0, x.r2, y.r2] and y = [x.r1, y.r1, x.r3, y.r3];
+// after swap_rows32, x = [x.r0, x.r1, y.r0, y.r1] and y = [x.r2, x.r3, y.r2,
+// y.r3].  Vector-unit exchanges, no LDS.
+__device__ inline void swap_rows16(float& x, float& y) {
+  const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(y), false, false);
+  x = __uint_as_float(r[0]);
+  y = __uint_as_float(r[1]);
+}
+__device__ inline void swap_rows32(float& x, float& y) {
+  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(y), false, false);
+  x = __uint_as_float(r[0]);
+  y = __uint_as_float(r[1]);
+}
+
 // x = h + m + l, each part rounded to BF16 to nearest even by the hardware
 // conversion; the residuals are exact in FP32, so the parts carry x's 24
 // significand bits.
@@ -1030,7 +1046,7 @@ __global__ __launch_bounds__(kGmmThreads) void k_gmm_z_mfma(GmmDev m, uint32_t s
   extern __shared__ __align__(16) unsigned char smem[];
   // the small arrays first: LDS reads take a 16-bit immediate offset, so this
   // keeps almost every per-component address a constant offset from one base
-  float* cvec = reinterpret_cast<float*>(smem);         // [KM][16] c_k
+  float* cvec = reinterpret_cast<float*>(smem);         // [KM][16] -c_k
   float* cst = cvec + KM * 16;                          // [KM] log pi_k + log-det term
   uint4* aop = reinterpret_cast<uint4*>(cst + KM);      // [KM][3][32] parts of L^-1 per lane
   const int K = m.k, d = m.d, c = blockIdx.y, tid = threadIdx.x;
@@ -1069,7 +1085,7 @@ __global__ __launch_bounds__(kGmmThreads) void k_gmm_z_mfma(GmmDev m, uint32_t s
     const float* t = gtab + k * TS;
     double sum = 0.0;
     for (int j = 0; j <= i; ++j) sum += (double)t[tri(i) + j] * (double)t[tri(DM) + j];
-    cvec[e2] = (float)sum;
+    cvec[e2] = (float)(-sum);
   }
   __syncthreads();
 
@@ -1109,26 +1125,28 @@ __global__ __launch_bounds__(kGmmThreads) void k_gmm_z_mfma(GmmDev m, uint32_t s
       if (kk < K) {
         const gmm_bf16x8 a1 = __builtin_bit_cast(gmm_bf16x8, aop[kk * 96 + la]);
         const gmm_bf16x8 a2 = __builtin_bit_cast(gmm_bf16x8, aop[kk * 96 + lb]);
-        const float4 cv = *reinterpret_cast<const float4*>(cvec + kk * 16 + 4 * kg);
+        // -c_k of this lane's 4 rows: the products accumulate onto it, so the
+        // accumulator ends as L^-1 x - c_k without a subtraction per value
+        const gmm_f32x4 cneg = *reinterpret_cast<const gmm_f32x4*>(cvec + kk * 16 + 4 * kg);
         // qp[j]: this lane's 4 rows of ||L^-1 x - c_k||^2 for point n of tile j
         float qp[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          gmm_f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1[j], acc, 0, 0, 0);
+          gmm_f32x4 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1[j], cneg, 0, 0, 0);
           acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, b2[j], acc, 0, 0, 0);
           acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b3[j], acc, 0, 0, 0);
-          const float d0 = acc[0] - cv.x, d1 = acc[1] - cv.y, d2 = acc[2] - cv.z, d3 = acc[3] - cv.w;
-          qp[j] = fmaf(d0, d0, fmaf(d1, d1, fmaf(d2, d2, d3 * d3)));
+          qp[j] = fmaf(acc[0], acc[0], fmaf(acc[1], acc[1], fmaf(acc[2], acc[2], acc[3] * acc[3])));
         }
-        // transpose-reduce over the 4 lanes of point n (kg = 0..3): after the
-        // xor-16 exchange a lane holds tiles kg & 1 and 2 + (kg & 1) summed
-        // over its lane pair, after the xor-32 exchange tile kg over all four
-        const bool b0 = kg & 1, b1s = kg >> 1;
-        float k0 = b0 ? qp[1] : qp[0], k1 = b0 ? qp[3] : qp[2];
-        k0 += __shfl_xor(b0 ? qp[0] : qp[1], 16);
-        k1 += __shfl_xor(b0 ? qp[2] : qp[3], 16);
-        const float q = (b1s ? k1 : k0) + __shfl_xor(b1s ? k0 : k1, 32);
+        // transpose-reduce over the 4 lanes of point n (row kg = 0..3 of 16
+        // lanes) with the row-swapping permutes: after the two 16-row swaps
+        // row kg holds tiles kg & 1 (k0) and 2 + (kg & 1) (k1) summed over
+        // its row pair, after the 32-row swap tile kg over all four rows
+        float t0 = qp[0], t1 = qp[1], t2 = qp[2], t3 = qp[3];
+        swap_rows16(t0, t1);
+        swap_rows16(t2, t3);
+        float k0 = t0 + t1, k1 = t2 + t3;
+        swap_rows32(k0, k1);
+        const float q = k0 + k1;
         const float l = cst[kk] - 0.5f * q;
         lg[kk] = l;
         mx = fmaxf(mx, l);
