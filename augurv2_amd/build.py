@@ -31,10 +31,13 @@ def sources():
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
 
 
-# Per-file flags.  augur_gmm.hip: matrix-core results in ordinary VGPRs
-# rather than accumulation registers (its z step otherwise copies every
-# product back with v_accvgpr_read: 512 instructions per 64 points).
-FILE_FLAGS = {"augur_gmm.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]}
+# Per-file flags: matrix-core results in ordinary VGPRs rather than
+# accumulation registers.  augur_gmm.hip: its z step otherwise copies every
+# product back with v_accvgpr_read (512 instructions per 64 points);
+# augur_bhlr.hip: the matrix-core data pass drops from 161 + 36 registers
+# (two waves per SIMD) to 164 (three) and loses its 243 copies.
+_VGPR_MFMA = ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]
+FILE_FLAGS = {"augur_gmm.hip": _VGPR_MFMA, "augur_bhlr.hip": _VGPR_MFMA}
 
 
 def _compile(src):
