@@ -110,14 +110,38 @@ __device__ inline bool bsurr_skip(const BArgs& A, int di) {
 // Nontemporal 16- / 8-byte loads of packed BF16 X.
 typedef unsigned v4u_t __attribute__((ext_vector_type(4)));
 typedef unsigned v2u_t __attribute__((ext_vector_type(2)));
+// s_waitcnt immediate that waits until at most n vector-memory loads are
+// outstanding (gfx9 encoding: vmcnt bits 3:0 and 15:14; expcnt and lgkmcnt
+// left at their maxima, i.e. not waited on)
+__host__ __device__ constexpr int vmcnt_imm(int n) {
+  return (n & 15) | (7 << 4) | (15 << 8) | (((n >> 4) & 3) << 14);
+}
+
+// The datasets' X and y pointers come from the device table of BDs, so the
+// compiler cannot tell their address space and would load through them with
+// flat instructions, which count against both the vector-memory and the LDS
+// counters (every wait on one then waits on the other).  X and y are global
+// memory: these casts make the loads global ones.
+#define AUGUR_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ inline const AUGUR_GLOBAL T* gmem(const T* p) {
+  return (const AUGUR_GLOBAL T*)p;
+}
+
 __device__ inline uint4 ld_x(const uint4* p, int nt) {
-  if (!nt) return *p;
-  const v4u_t v = __builtin_nontemporal_load((const v4u_t*)p);
+  if (!nt) {
+    const AUGUR_GLOBAL uint4* g = gmem(p);
+    return *g;
+  }
+  const v4u_t v = __builtin_nontemporal_load(gmem((const v4u_t*)p));
   return make_uint4(v[0], v[1], v[2], v[3]);
 }
 __device__ inline uint2 ld_x(const uint2* p, int nt) {
-  if (!nt) return *p;
-  const v2u_t v = __builtin_nontemporal_load((const v2u_t*)p);
+  if (!nt) {
+    const AUGUR_GLOBAL uint2* g = gmem(p);
+    return *g;
+  }
+  const v2u_t v = __builtin_nontemporal_load(gmem((const v2u_t*)p));
   return make_uint2(v[0], v[1]);
 }
 
@@ -168,13 +192,15 @@ __device__ void bhlr_body(const BArgs& A, const BDs& ds, int di, unsigned char* 
   int ypre = 0;
   auto prefetch = [&](int r0) {
     const int rows = min(TR, rend - r0);
-    const uint4* s8 = (const uint4*)(ds.x + (size_t)r0 * D);
+    const AUGUR_GLOBAL uint4* s8 = gmem((const uint4*)(ds.x + (size_t)r0 * D));
 #pragma unroll
     for (int k = 0; k < NVT; ++k) {
       const int i = tid + k * kBThreads;
-      pre[k] = (i < NV && i < rows * (D / 8)) ? s8[i] : make_uint4(0u, 0u, 0u, 0u);
+      uint4 v = make_uint4(0u, 0u, 0u, 0u);
+      if (i < NV && i < rows * (D / 8)) v = s8[i];
+      pre[k] = v;
     }
-    ypre = (tid < TR && tid < rows) ? (int)ds.y[r0 + tid] : 0;
+    ypre = (tid < TR && tid < rows) ? (int)*gmem(ds.y + r0 + tid) : 0;
   };
   if (rbeg < rend) prefetch(rbeg);
   for (int r0 = rbeg; r0 < rend; r0 += TR) {
@@ -394,12 +420,16 @@ __device__ void bhlr_body_r(const BArgs& A, const BDs& ds, unsigned char* smem) 
   const int rend = min(ds.n, rbeg + A.rows_per_blk);
   const int ntile = rend > rbeg ? (rend - rbeg + TRr - 1) / TRr : 0;
 
+  // theta of every padded chain (a real chain's value loaded for the padding,
+  // then zeroed): the same loads on every path, see below
   float th[CP][F];
 #pragma unroll
   for (int c = 0; c < CP; ++c)
 #pragma unroll
-    for (int j = 0; j < F; ++j)
-      th[c][j] = c < A.C ? A.beta32[(size_t)c * A.GW + ds.goff + F * q + j] : 0.0f;
+    for (int j = 0; j < F; ++j) {
+      const float v = A.beta32[(size_t)min(c, A.C - 1) * A.GW + ds.goff + F * q + j];
+      th[c][j] = c < A.C ? v : 0.0f;
+    }
 
   xword pre[PD][RPT];
   int ypre[PD][RPT];
@@ -412,12 +442,19 @@ __device__ void bhlr_body_r(const BArgs& A, const BDs& ds, unsigned char* smem) 
     for (int i = 0; i < RPT; ++i) {
       const int row = rbeg + t * TRr + i * RG + rsub;
       dst[i] = ld_x((const xword*)(ds.x + (size_t)row * D + F * q), A.x_stream);
-      yd[i] = ds.y[row];
+      yd[i] = *gmem(ds.y + row);
     }
   };
+  // Every prefetch is issued, a tile past the block's last one clamped to it
+  // (valid rows, loaded again and not used), so each path through the loop
+  // issues the same loads and the compiler's waits count the ring exactly:
+  // theta, requested before the ring (2 loads a row), is complete once at
+  // most PD * RPT * 2 loads are outstanding.  With conditional prefetches the
+  // row loop waited for zero outstanding loads at every slot.
+  const int last = max(ntile - 1, 0);  // rbeg < n: ntile >= 1
 #pragma unroll
-  for (int k = 0; k < PD; ++k)
-    if (k < ntile) prefetch(k, pre[k], ypre[k]);
+  for (int k = 0; k < PD; ++k) prefetch(min(k, last), pre[k], ypre[k]);
+  __builtin_amdgcn_s_waitcnt(vmcnt_imm(PD * RPT * 2));
 
   float g[CP][F];
 #pragma unroll
@@ -430,7 +467,11 @@ __device__ void bhlr_body_r(const BArgs& A, const BDs& ds, unsigned char* smem) 
 #pragma unroll
     for (int slot = 0; slot < PD; ++slot) {
       const int t = base + slot;
-      if (t >= ntile) break;
+      if (t >= ntile) {  // uniform: the block's remaining slots are empty
+        // the refill is still issued, so every path issues the same loads
+        prefetch(min(t + PD, last), pre[slot], ypre[slot]);
+        continue;
+      }
       float xr[RPT][F];
 #pragma unroll
       for (int i = 0; i < RPT; ++i)
@@ -483,7 +524,7 @@ __device__ void bhlr_body_r(const BArgs& A, const BDs& ds, unsigned char* smem) 
 #pragma unroll
         for (int j = 0; j < F; ++j) g[c][j] = fmaf(av, xr[i][j], g[c][j]);
       }
-      if (t + PD < ntile) prefetch(t + PD, pre[slot], ypre[slot]);
+      prefetch(min(t + PD, last), pre[slot], ypre[slot]);
     }
   }
 
@@ -642,7 +683,10 @@ __device__ void bhlr_body_mfma(const BArgs& A, const BDs& ds, unsigned char* sme
   for (int s = 0; s < KC; ++s)
 #pragma unroll
     for (int e = 0; e < KE; ++e) {
-      const float b = live ? A.beta32[(size_t)li * A.GW + ds.goff + KB * s + KE * kg + e] : 0.0f;
+      // a real chain's value loaded for the padding columns, then zeroed:
+      // the same loads on every path
+      const float bv = A.beta32[(size_t)min(li, A.C - 1) * A.GW + ds.goff + KB * s + KE * kg + e];
+      const float b = live ? bv : 0.0f;
       __bf16 h, m, l;
       split_bf16x3(b, h, m, l);
       bh[s][e] = h;
@@ -673,16 +717,26 @@ __device__ void bhlr_body_mfma(const BArgs& A, const BDs& ds, unsigned char* sme
     const uint16_t* xr = ds.x + (size_t)(t0 + li) * D + KE * kg;
 #pragma unroll
     for (int s = 0; s < KC; ++s) qd[s] = ld_x((const braw*)(xr + KB * s), A.x_stream);
-    yd = *(const uint32_t*)(ds.y + t0 + 4 * kg);
+    yd = *gmem((const uint32_t*)(ds.y + t0 + 4 * kg));
   };
+  // Every tile load is issued, one past the wave's last tile clamped to it
+  // (tile 0 when the wave has none: rows w0 .. w0 + 15 lie inside the padded
+  // rows), so each path through the loop issues the same loads and the
+  // compiler's waits count the ring exactly; theta, requested before the
+  // ring, is complete once at most PD (KC + 1) loads are outstanding.
+  const int lastw = max(ntw - 1, 0);
 #pragma unroll
-  for (int k = 0; k < PD; ++k)
-    if (k < ntw) load_tile(k, qn[k], yn[k]);
+  for (int k = 0; k < PD; ++k) load_tile(min(k, lastw), qn[k], yn[k]);
+  __builtin_amdgcn_s_waitcnt(vmcnt_imm(PD * (KC + 1)));
   for (int base = 0; base < ntw; base += PD) {
 #pragma unroll
   for (int slot = 0; slot < PD; ++slot) {
     const int tt = base + slot;
-    if (tt >= ntw) break;
+    if (tt >= ntw) {  // uniform over the wave: its remaining slots are empty
+      // the refill is still issued, so every path issues the same loads
+      load_tile(min(tt + PD, lastw), qn[slot], yn[slot]);
+      continue;
+    }
     const int r0 = w0 + 64 * tt;
     // phase B: A operand = row r0 + li, features KB s + KE kg .. + KE - 1
 #pragma unroll
@@ -697,7 +751,7 @@ __device__ void bhlr_body_mfma(const BArgs& A, const BDs& ds, unsigned char* sme
       t = MfmaB<KB>::mma(xa, bl[s], t);
     }
     // the slot's registers are free once the products have read them
-    if (tt + PD < ntw) load_tile(tt + PD, qn[slot], yn[slot]);
+    load_tile(min(tt + PD, lastw), qn[slot], yn[slot]);
     // rows r0 + 4 kg + r, chain li
     bf16x4_t ah, am, al;
 #pragma unroll
