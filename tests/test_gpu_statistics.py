@@ -64,6 +64,48 @@ def test_hlr_posterior_matches_oracle(rt, oracle):
         _agree(g[:, :, j].T, o[:, :, j].T)
 
 
+def test_hlr_reference_target_posterior_matches_oracle(rt, oracle):
+    """The reference's own target (ref_compat: normal_ll = -log var - t^2 /
+    (2 var), its normal_dotarg2 and MWG's extra Hastings term, cbits/augur_dist.c
+    :372-394, Core/CgUserProp.hs:130-141) under the schedule of
+    examples/hlr.py on hlr.rv: MWG [v] ~ Normal(v, 1.0) (*) HMC [b] (*) HMC
+    [theta], v ~ Exponential(1).  GPU chains and independent oracle chains
+    agree on posterior means and variances within 3 MCSE."""
+    from augurv2_amd._lib import HMC_BLOCK_B, HMC_BLOCK_THETA, MWG_BLOCK_V
+    rng = np.random.default_rng(23)
+    n, d, C, iters, burn = 400, 3, 4, 3000, 300
+    x = rng.standard_normal((n, d)).astype(np.float32)
+    y = (rng.random(n) < 1 / (1 + np.exp(-(x @ np.array([1.0, -0.5, 0.25]) + 0.3)))).astype(np.int32)
+    blocks = [(MWG_BLOCK_V, 0.0, 1.0, 0), (HMC_BLOCK_B, 0.3, 0.05, 0), (HMC_BLOCK_THETA, 0.3, 0.05, 0)]
+    s = rt.HlrSampler(x, y, blocks, chains=C, seed=6, prior_kind=1, prior_a=1.0,
+                      has_intercept=True, ref_compat=True)
+    init = [np.array([0.1 * c, -0.1, 0.0, 0.0, 1.0]) for c in range(C)]
+    for c in range(C):
+        s.set("state", init[c], chain=c)
+    s.step(burn)
+    s.trace("state", iters)
+    s.step(iters)
+    g = s.read_trace("state")
+    acc_v = s.acceptance()[0]
+    s.close()
+    assert 0.05 < np.nanmean(acc_v) < 0.95, acc_v  # v moves
+    m = oracle.Hlr(x.astype(np.float64), y, prior_kind=1, prior_a=1.0, has_intercept=True,
+                   ref_compat=True)
+    o = np.zeros_like(g)
+    for c in range(C):
+        st = init[c].copy()
+        for it in range(burn + iters):
+            m.mwg_v_step(st, 0, 1.0, 98, c, it)  # independent streams
+            m.hmc_step(st, 1, 1, 0.3, 0.05, 0, 98, c, it)
+            m.hmc_step(st, 2, 2, 0.3, 0.05, 0, 98, c, it)
+            if it >= burn:
+                o[it - burn, c] = st
+    for j in range(d + 2):
+        assert diagnostics.rhat(g[:, :, j].T) < 1.05
+        assert diagnostics.rhat(o[:, :, j].T) < 1.05
+        _agree(g[:, :, j].T, o[:, :, j].T)
+
+
 def test_lda_posterior_matches_oracle(rt, oracle):
     """Label-invariant summaries: per-document topic concentration
     sum_k theta_dk^2 and the probability that two fixed tokens share a topic."""
