@@ -129,19 +129,13 @@ __device__ inline const AUGUR_GLOBAL T* gmem(const T* p) {
 }
 
 __device__ inline uint4 ld_x(const uint4* p, int nt) {
-  if (!nt) {
-    const AUGUR_GLOBAL uint4* g = gmem(p);
-    return *g;
-  }
-  const v4u_t v = __builtin_nontemporal_load(gmem((const v4u_t*)p));
+  const AUGUR_GLOBAL v4u_t* g = gmem((const v4u_t*)p);
+  const v4u_t v = nt ? __builtin_nontemporal_load(g) : *g;
   return make_uint4(v[0], v[1], v[2], v[3]);
 }
 __device__ inline uint2 ld_x(const uint2* p, int nt) {
-  if (!nt) {
-    const AUGUR_GLOBAL uint2* g = gmem(p);
-    return *g;
-  }
-  const v2u_t v = __builtin_nontemporal_load(gmem((const v2u_t*)p));
+  const AUGUR_GLOBAL v2u_t* g = gmem((const v2u_t*)p);
+  const v2u_t v = nt ? __builtin_nontemporal_load(g) : *g;
   return make_uint2(v[0], v[1]);
 }
 
@@ -192,13 +186,13 @@ __device__ void bhlr_body(const BArgs& A, const BDs& ds, int di, unsigned char* 
   int ypre = 0;
   auto prefetch = [&](int r0) {
     const int rows = min(TR, rend - r0);
-    const AUGUR_GLOBAL uint4* s8 = gmem((const uint4*)(ds.x + (size_t)r0 * D));
+    const AUGUR_GLOBAL v4u_t* s8 = gmem((const v4u_t*)(ds.x + (size_t)r0 * D));
 #pragma unroll
     for (int k = 0; k < NVT; ++k) {
       const int i = tid + k * kBThreads;
-      uint4 v = make_uint4(0u, 0u, 0u, 0u);
+      v4u_t v = {0u, 0u, 0u, 0u};
       if (i < NV && i < rows * (D / 8)) v = s8[i];
-      pre[k] = v;
+      pre[k] = make_uint4(v[0], v[1], v[2], v[3]);
     }
     ypre = (tid < TR && tid < rows) ? (int)*gmem(ds.y + r0 + tid) : 0;
   };
