@@ -575,6 +575,231 @@ __global__ __launch_bounds__(kBThreads) void k_bhlr_pass_r(BArgs A) {
   }
 }
 
+// The register form at 4 chains and D = 16, 32, 64 with its a x phase on the
+// matrix cores.  Lanes and loads as bhlr_body_r (LPR = 4 or 8 lanes a row, F
+// features each, 2 rows a tile); what changes is the arithmetic after the
+// partial dot products:
+//  - each lane holds theta in a lane-dependent chain order, slot s of lane q
+//    being chain (q & 3) ^ s ^ x with x = 3 in the second quad of an 8-lane
+//    row group, else 0.  Then the quad partner q ^ 2 holds in its slots 2, 3
+//    the two chains that q keeps in its slots 0, 1, and the partner q ^ 1 in
+//    slot 1 the chain that q keeps in slot 0: a reduce-scatter over the quad
+//    in 3 DPP adds a row (quad_perm 2,3,0,1 then 1,0,3,2), with no selects.
+//    At 8 lanes a row the half-row mirror (q <-> 7 - q) pairs lanes holding
+//    the same chain, and each lane keeps one row: row 0 in the first quad,
+//    row 1 in the second.  Each (row, chain) term is computed by one lane.
+//  - g += a x runs on V_MFMA_F32_4X4X1_16B_F32: 16 blocks of 4 lanes, block b
+//    taking A[m] from lane 4b + m and B[n] from lane 4b + n, and leaving
+//    D[m][n] in register m of lane 4b + n (checked on the device).  A quad is
+//    one row's lanes (or half of them), so with A = the row's adjoint of the
+//    chain of quad lane m and B = the lane's feature j, register m of lane q
+//    accumulates g[chain of slot m][F q + j]: 16 products a tile replace 64
+//    vector FMAs and 8 lane shuffles.
+// FP32 arithmetic and FP64 accumulation of the row terms as bhlr_body_r.
+#ifndef AUGUR_RQ_PD
+#define AUGUR_RQ_PD 4
+#endif
+template <int D, bool LL>
+__device__ void bhlr_body_rq(const BArgs& A, const BDs& ds, unsigned char* smem) {
+  constexpr int CP = 4;
+  constexpr int LPR = (D / 8 > CP) ? D / 8 : CP;
+  constexpr int F = D / LPR;
+  constexpr int FW = F / 2;
+  constexpr int RPT = kBrRpt, PD = AUGUR_RQ_PD;
+  constexpr int RG = kBThreads / LPR;
+  constexpr int TRr = RG * RPT;
+  constexpr int GS = CP * F + 1;
+  constexpr int VS = D + 2;
+  static_assert((LPR == 4 || LPR == 8) && (F == 4 || F == 8) && RPT == 2, "shape");
+  typedef typename std::conditional<FW == 4, uint4, uint2>::type xword;
+  typedef float f4 __attribute__((ext_vector_type(4)));
+
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int q = tid % LPR, rsub = tid / LPR;
+  const int hi = (LPR == 8) ? (q >> 2) : 0;  // second quad of the row's lanes
+  const int cx = hi ? 3 : 0;
+  const int p = q & 3;
+  const int my_c = p ^ cx;  // chain of slot 0: the one this lane's row terms use
+  const int local = blockIdx.x - ds.blk0;
+  const int rbeg = local * A.rows_per_blk;
+  const int rend = min(ds.n, rbeg + A.rows_per_blk);
+  const int ntile = rend > rbeg ? (rend - rbeg + TRr - 1) / TRr : 0;
+
+  float th[CP][F];  // th[s] = theta of chain p ^ s ^ cx (0 for padding chains)
+#pragma unroll
+  for (int s = 0; s < CP; ++s) {
+    const int c = p ^ s ^ cx;
+#pragma unroll
+    for (int j = 0; j < F; ++j) {
+      const float v = A.beta32[(size_t)min(c, A.C - 1) * A.GW + ds.goff + F * q + j];
+      th[s][j] = c < A.C ? v : 0.0f;
+    }
+  }
+
+  // Row addresses as a block-uniform base (scalar registers) plus a 32-bit
+  // lane offset that stays fixed over the tiles, so the loads take the
+  // scalar-base form and the ring costs no per-lane address arithmetic.  At 8
+  // lanes a row each lane loads the y of the one row whose term it computes.
+  constexpr int NY = (LPR == 8) ? 1 : RPT;  // y loads a tile
+  const uint32_t xlane = (uint32_t)((rsub * D + F * q) * 2);
+  const uint32_t ylane = (uint32_t)(rsub + (LPR == 8 ? hi * RG : 0));
+  xword pre[PD][RPT];
+  int ypre[PD][NY];
+  static_assert(kRowPad % TRr == 0, "tiles within the padding");
+  auto prefetch = [&](int t, xword (&dst)[RPT], int (&yd)[NY]) {
+    const size_t r0 = (size_t)(rbeg + t * TRr);
+#pragma unroll
+    for (int i = 0; i < RPT; ++i)
+      dst[i] = ld_x((const xword*)((const char*)ds.x + (r0 + (size_t)i * RG) * (D * 2) + xlane),
+                    A.x_stream);
+#pragma unroll
+    for (int i = 0; i < NY; ++i) yd[i] = *gmem(ds.y + r0 + (size_t)i * RG + ylane);
+  };
+  // the same unconditional, clamped ring as bhlr_body_r
+  const int last = max(ntile - 1, 0);
+#pragma unroll
+  for (int k = 0; k < PD; ++k) prefetch(min(k, last), pre[k], ypre[k]);
+  __builtin_amdgcn_s_waitcnt(vmcnt_imm(PD * (RPT + NY)));
+
+  f4 acc[F];
+#pragma unroll
+  for (int j = 0; j < F; ++j) acc[j] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+  double llacc = 0.0;
+
+  for (int base = 0; base < ntile; base += PD) {
+#pragma unroll
+    for (int slot = 0; slot < PD; ++slot) {
+      const int t = base + slot;
+      if (t >= ntile) {
+        prefetch(min(t + PD, last), pre[slot], ypre[slot]);
+        continue;
+      }
+      float xr[RPT][F];
+#pragma unroll
+      for (int i = 0; i < RPT; ++i)
+#pragma unroll
+        for (int k = 0; k < FW; ++k) {
+          const uint32_t w = xword_at(pre[slot][i], k);
+          xr[i][2 * k] = __uint_as_float(w << 16);
+          xr[i][2 * k + 1] = __uint_as_float(w & 0xffff0000u);
+        }
+      // partial dot products of the lane's F features, slot order
+      float tp[RPT][CP];
+#pragma unroll
+      for (int i = 0; i < RPT; ++i)
+#pragma unroll
+        for (int s = 0; s < CP; ++s) {
+          float acc_t = xr[i][0] * th[s][0];
+#pragma unroll
+          for (int j = 1; j < F; ++j) acc_t = fmaf(xr[i][j], th[s][j], acc_t);
+          tp[i][s] = acc_t;
+        }
+      // quad reduce-scatter: slot 0 of every lane ends with its chain's sum
+      // over the quad, for both rows
+#pragma unroll
+      for (int i = 0; i < RPT; ++i) {
+        tp[i][0] += dpp_f<0x4E>(tp[i][2]);  // quad_perm 2,3,0,1
+        tp[i][1] += dpp_f<0x4E>(tp[i][3]);
+        tp[i][0] += dpp_f<0xB1>(tp[i][1]);  // quad_perm 1,0,3,2
+      }
+      float aq[RPT];  // adjoint of (row i, chain my_c), as quad lane p supplies it
+      if constexpr (LPR == 8) {
+        // the other quad holds the same chain: half-row mirror, one row each
+        const float t0 = tp[0][0] + dpp_f<0x141>(tp[0][0]);
+        const float t1 = tp[1][0] + dpp_f<0x141>(tp[1][0]);
+        const float tv = hi ? t1 : t0;
+        const int yy = ypre[slot][0];
+        const bool ok = my_c < A.C && rbeg + t * TRr + hi * RG + rsub < rend;
+        float a;
+        if constexpr (LL) {
+          const float lr = log_sigmoid_pm(tv, yy, &a);
+          llacc += (double)(ok ? lr : 0.0f);
+        } else {
+          a = sigmoid_adjoint_pm(tv, yy);
+        }
+        a = ok ? a : 0.0f;
+        const float ax = dpp_f<0x141>(a);  // the other row's, same chain
+        aq[0] = hi ? ax : a;
+        aq[1] = hi ? a : ax;
+      } else {
+#pragma unroll
+        for (int i = 0; i < RPT; ++i) {
+          const bool ok = my_c < A.C && rbeg + t * TRr + i * RG + rsub < rend;
+          float a;
+          if constexpr (LL) {
+            const float lr = log_sigmoid_pm(tp[i][0], ypre[slot][i], &a);
+            llacc += (double)(ok ? lr : 0.0f);
+          } else {
+            a = sigmoid_adjoint_pm(tp[i][0], ypre[slot][i]);
+          }
+          a = ok ? a : 0.0f;
+          aq[i] = a;
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < RPT; ++i)
+#pragma unroll
+        for (int j = 0; j < F; ++j) acc[j] = __builtin_amdgcn_mfma_f32_4x4x1f32(aq[i], xr[i][j], acc[j], 0, 0, 0);
+      prefetch(min(t + PD, last), pre[slot], ypre[slot]);
+    }
+  }
+
+  // partials of this block, as bhlr_body_r: register m of acc[j] is the
+  // chain p' ^ cx of quad lane m, i.e. chain m ^ cx
+  float* gl = (float*)smem;  // [256][GS]
+  double* red = (double*)(smem + ((sizeof(float) * kBThreads * GS + 15) & ~(size_t)15));  // [2][4][CP]
+#pragma unroll
+  for (int m = 0; m < CP; ++m)
+#pragma unroll
+    for (int j = 0; j < F; ++j) gl[tid * GS + (m ^ cx) * F + j] = acc[j][m];
+  // lanes of one chain: equal q across the wave, then (8 lanes a row) the
+  // mirrored quad lane 7 - q = q ^ 7; lane c < 4 then holds chain c's sum
+  if constexpr (LL) {
+#pragma unroll
+    for (int off = 32; off >= LPR; off >>= 1) llacc += __shfl_xor(llacc, off);
+    if constexpr (LPR == 8) llacc += __shfl_xor(llacc, 7);
+    if (lane < CP) red[wv * CP + lane] = llacc;
+  }
+  __syncthreads();
+  double* part = A.part + ds.poff;
+  for (int v = tid; v < A.C * D; v += kBThreads) {
+    const int c = v / D, j = v % D, qq = j / F, jj = j % F;
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < kBThreads / 64; ++w)
+#pragma unroll
+      for (int k = 0; k < 64 / LPR; ++k) s += (double)gl[(w * 64 + k * LPR + qq) * GS + c * F + jj];
+    part[(size_t)(c * VS + j) * ds.nblk + local] = s;
+  }
+  // the model has no intercept: its slot carries 0; the log-density only
+  // when this pass is asked for it (0 otherwise, and the finish ignores it)
+  if (tid < A.C) {
+    double sl = 0.0;
+    if constexpr (LL) {
+#pragma unroll
+      for (int w = 0; w < kBThreads / 64; ++w) sl += red[w * CP + tid];
+    }
+    part[(size_t)(tid * VS + D) * ds.nblk + local] = 0.0;
+    part[(size_t)(tid * VS + D + 1) * ds.nblk + local] = sl;
+  }
+}
+
+// LL = false: the gradient only, for a trajectory's leapfrog passes before
+// its last (the log-density is read by the cache pass and the last pass).
+template <bool LL>
+__global__ __launch_bounds__(kBThreads) void k_bhlr_pass_rq(BArgs A) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int di = A.blk_ds[blockIdx.x];
+  if (bsurr_skip(A, di)) return;
+  const BDs ds = A.ds[di];
+  switch (ds.d) {
+    case 16: bhlr_body_rq<16, LL>(A, ds, smem); break;
+    case 32: bhlr_body_rq<32, LL>(A, ds, smem); break;
+    case 64: bhlr_body_rq<64, LL>(A, ds, smem); break;
+    default: bhlr_body_r<128, 4>(A, ds, smem); break;
+  }
+}
+
 // LDS of the register form: the lanes' gradient shares, then the chains' sums.
 size_t bhlr_r_smem_bytes(int cp) {
   int f = 8;  // F * CP is largest at F = 8
@@ -654,7 +879,7 @@ template <> struct MfmaB<16> {
   }
 };
 
-template <int D>
+template <int D, bool LL>
 __device__ void bhlr_body_mfma(const BArgs& A, const BDs& ds, unsigned char* smem) {
   constexpr int KB = (D % 32 == 0) ? 32 : 16;  // features per phase-B product
   constexpr int KE = KB / 4;                   // features per lane per product
@@ -753,9 +978,13 @@ __device__ void bhlr_body_mfma(const BArgs& A, const BDs& ds, unsigned char* sme
       const int rr = r0 + 4 * kg + r;
       const bool ok = rr < rend && live;
       float a;
-      const float lr = log_sigmoid_pm(t[r], (int)((yw >> (8 * r)) & 0xffu), &a);
-      if (ok) ll += (double)lr;
-      else a = 0.0f;
+      if constexpr (LL) {
+        const float lr = log_sigmoid_pm(t[r], (int)((yw >> (8 * r)) & 0xffu), &a);
+        if (ok) ll += (double)lr;
+      } else {
+        a = sigmoid_adjoint_pm(t[r], (int)((yw >> (8 * r)) & 0xffu));
+      }
+      a = ok ? a : 0.0f;
       __bf16 h, m, l;
       split_bf16x3(a, h, m, l);
       ah[r] = h;
@@ -811,16 +1040,18 @@ __device__ void bhlr_body_mfma(const BArgs& A, const BDs& ds, unsigned char* sme
   }
 }
 
+// LL = false: the gradient only (leapfrog passes before a trajectory's last)
+template <bool LL>
 __global__ __launch_bounds__(kBThreads) void k_bhlr_pass_mfma(BArgs A) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int di = A.blk_ds[blockIdx.x];
   if (bsurr_skip(A, di)) return;
   const BDs ds = A.ds[di];
   switch (ds.d) {
-    case 16: bhlr_body_mfma<16>(A, ds, smem); break;
-    case 32: bhlr_body_mfma<32>(A, ds, smem); break;
-    case 64: bhlr_body_mfma<64>(A, ds, smem); break;
-    default: bhlr_body_mfma<128>(A, ds, smem); break;
+    case 16: bhlr_body_mfma<16, LL>(A, ds, smem); break;
+    case 32: bhlr_body_mfma<32, LL>(A, ds, smem); break;
+    case 64: bhlr_body_mfma<64, LL>(A, ds, smem); break;
+    default: bhlr_body_mfma<128, LL>(A, ds, smem); break;
   }
 }
 
@@ -1206,12 +1437,22 @@ static BPassKernel pick_bpass(int cp) {
   }
 }
 
+// At 3-4 chains the form with its a x phase on the matrix cores, unless
+// AUGUR_BHLR_R_VALU=1 (all-vector register form).
 static BPassKernel pick_bpass_r(int cp) {
+  const char* valu = getenv("AUGUR_BHLR_R_VALU");
   switch (cp) {
     case 1: return k_bhlr_pass_r<1>;
     case 2: return k_bhlr_pass_r<2>;
-    default: return k_bhlr_pass_r<4>;
+    default: return (valu && atoi(valu)) ? k_bhlr_pass_r<4> : k_bhlr_pass_rq<true>;
   }
+}
+
+// The gradient-only kernel of the same form for the leapfrog passes before a
+// trajectory's last, or null where the form has none.
+static BPassKernel pick_bpass_r_grad(int cp) {
+  const char* valu = getenv("AUGUR_BHLR_R_VALU");
+  return (cp == 4 && !(valu && atoi(valu))) ? k_bhlr_pass_rq<false> : nullptr;
 }
 
 struct BhlrSampler final : augur_sampler {
@@ -1229,6 +1470,7 @@ struct BhlrSampler final : augur_sampler {
   DevBuf<uint32_t> sweep_dev;
   BArgs base{};
   BPassKernel pass = nullptr;
+  BPassKernel pass_grad = nullptr;  // leapfrog passes before the last (null: pass)
   size_t pass_smem = 0;
   bool mfma = false;
   bool cache_valid = false, use_graph = false;
@@ -1345,8 +1587,9 @@ struct BhlrSampler final : augur_sampler {
       A.cap_t = cap_t.p + slot * C * GW;
       A.cap_g = cap_g.p + slot * C * GW;
     }
+    const BPassKernel k = (mode == BMODE_LEAPFROG && l < base.steps && pass_grad) ? pass_grad : pass;
     prof_begin();
-    pass<<<exact_only ? nblk_ex : nblk_total, kBThreads, pass_smem, stream>>>(A);
+    k<<<exact_only ? nblk_ex : nblk_total, kBThreads, pass_smem, stream>>>(A);
     AUGUR_CHECK_LAUNCH();
     prof_end();
     k_bhlr_finish<<<(A.nval + 255) / 256, 256, 0, stream>>>(A);
@@ -1785,16 +2028,19 @@ augur_status augur_bhlr_create(const augur_bhlr_desc* D, augur_sampler** out) {
       const int cpn = s->next_cp();
       s->mfma = cpn >= 8 && !(vec && atoi(vec));
       if (s->mfma) {
-        s->pass = k_bhlr_pass_mfma;
+        s->pass = k_bhlr_pass_mfma<true>;
+        s->pass_grad = k_bhlr_pass_mfma<false>;
         s->pass_smem = bhlr_mfma_smem_bytes(s->dmax);
       } else if (cpn <= 4 && !(lds && atoi(lds))) {
         s->pass = pick_bpass_r(cpn);
+        s->pass_grad = pick_bpass_r_grad(cpn);
         s->pass_smem = bhlr_r_smem_bytes(cpn);
       } else {
         s->pass = pick_bpass(cpn);
         s->pass_smem = bhlr_smem_bytes(s->dmax, cpn);
       }
       allow_lds(s->pass, s->pass_smem);
+      if (s->pass_grad) allow_lds(s->pass_grad, s->pass_smem);
       // state starts at theta = 0, v = 1
       std::vector<double> st(C * s->SW, 0.0);
       for (size_t c = 0; c < C; ++c)

@@ -121,10 +121,17 @@ __device__ inline bool v_update(int K, double S, double v0, int ref_compat, int 
 // FP32 on the hardware exp2 / log2 / reciprocal (about one ulp each).  The
 // exponent -|t| log2(e) carries its rounding error and the tail of log2(e)
 // as a first-order correction (2^(h + l) = 2^h (1 + l ln 2)), so e keeps
-// about one ulp for any |t|; log1p(e) = e log(u) / (u - 1) (Goldberg 1991,
-// Thm. 4: u - 1 is exact, and the quotient cancels the rounding of u), e
-// itself when u rounds to 1.
-__device__ inline float log_sigmoid_pm(float t, int y, float* a_out) {
+// about one ulp for any |t|; log1p(e) = log(u) + (e - (u - 1)) / u: u - 1
+// is exact (u in [1, 2]) and so is e - (u - 1), the rounding error of 1 + e,
+// so the first-order term ln(1 + r/u) ~ r/u restores what the rounding of u
+// took (the error left is O(r^2), below an ulp); it reuses 1/u, and when u
+// rounds to 1 it gives e itself.  (Goldberg 1991, Thm. 4's e log(u) / (u - 1)
+// needs one more reciprocal.)
+struct SigParts {
+  float e, u, inv;  // exp(-|t|), 1 + e, 1 / u
+};
+
+__device__ inline SigParts sig_parts(float t) {
   constexpr float kL2eHi = 1.44269502162933349609375f;  // log2(e) rounded to FP32
   constexpr float kL2eLo = 1.92596299e-08f;             // log2(e) - kL2eHi
   const float at = fabsf(t);
@@ -133,16 +140,26 @@ __device__ inline float log_sigmoid_pm(float t, int y, float* a_out) {
   const float e0 = __builtin_amdgcn_exp2f(h);
   const float e = fmaf(e0, l * 0.693147180559945309f, e0);
   const float u = 1.0f + e;
-  const float inv = __builtin_amdgcn_rcpf(u);
-  const float se = e * inv;
+  return {e, u, __builtin_amdgcn_rcpf(u)};
+}
+
+__device__ inline float sig_adjoint(float t, int y, const SigParts& p) {
+  const float se = p.e * p.inv;
   const bool pos = t >= 0.0f;
-  *a_out = y ? (pos ? se : inv) : -(pos ? inv : se);
-  const float um1 = u - 1.0f;
-  const float l1p = um1 == 0.0f ? e
-                                : e * (__builtin_amdgcn_logf(u) * 0.693147180559945309f) * __builtin_amdgcn_rcpf(um1);
+  return y ? (pos ? se : p.inv) : -(pos ? p.inv : se);
+}
+
+__device__ inline float log_sigmoid_pm(float t, int y, float* a_out) {
+  const SigParts p = sig_parts(t);
+  *a_out = sig_adjoint(t, y, p);
+  const float l1p = fmaf(p.e - (p.u - 1.0f), p.inv, __builtin_amdgcn_logf(p.u) * 0.693147180559945309f);
   const float st = y ? t : -t;
   return fminf(st, 0.0f) - l1p;
 }
+
+// The adjoint alone, the same value log_sigmoid_pm stores in *a_out: a
+// trajectory's leapfrog passes before its last need the gradient only.
+__device__ inline float sigmoid_adjoint_pm(float t, int y) { return sig_adjoint(t, y, sig_parts(t)); }
 
 template <int CTRL>
 __device__ inline float dpp_f(float v) {

@@ -699,12 +699,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=5, help="untimed warmup steps of the headline")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--only", default="", help="comma list of legs to run besides the headline "
-                    "HLR leg: lda, surrogate, batched, gmm, generic (default: all)")
+                    "HLR leg: lda, surrogate, batched, gmm, generic (default: all); with "
+                    "--skip-headline also batched4 (the 4-chain batched leg alone; batched is then "
+                    "the 16-chain one) and stream")
     ap.add_argument("--batched-sweeps", type=int, default=2000,
                     help="timed sweeps of the batched legs (warmup: half as many)")
-    ap.add_argument("--batched-surrogate-sweeps", type=int, default=10_000,
+    ap.add_argument("--batched-surrogate-sweeps", type=int, default=None,
                     help="timed surrogate sweeps after the 4-chain batched leg, half as many after "
-                         "the 16-chain one (0: none)")
+                         "the 16-chain one (0: none; default 10000, or 0 with --skip-headline)")
     ap.add_argument("--surrogate-iters", type=int, default=5000,
                     help="timed iterations of each surrogate-leg sampler (shorter for profiling)")
     ap.add_argument("--surrogate-exact", type=int, default=2000,
@@ -728,6 +730,7 @@ def main():
         raise SystemExit("bench: --steps must be at least 1")
     legs = set(args.only.split(",")) if args.only else {"lda", "surrogate", "batched", "gmm", "generic"}
     if args.skip_headline:
+        bss_only = max(0, args.batched_surrogate_sweeps or 0)
         for leg, fn in (("lda", lambda: run_lda(cpu=False)),
                         ("gmm", lambda: run_gmm(sweeps=args.gmm_sweeps, cpu=False)),
                         ("surrogate", lambda: run_surrogate(iters=args.surrogate_iters,
@@ -736,7 +739,12 @@ def main():
                                                             ablation=False)),
                         ("batched", lambda: run_batched(sweeps=args.batched_sweeps,
                                                         warmup=max(1, args.batched_sweeps // 2),
-                                                        chains=16, plain=False)),
+                                                        chains=16, plain=False,
+                                                        surrogate_sweeps=bss_only // 2)),
+                        ("batched4", lambda: run_batched(sweeps=args.batched_sweeps,
+                                                         warmup=max(1, args.batched_sweeps // 2),
+                                                         chains=4, plain=False,
+                                                         surrogate_sweeps=bss_only)),
                         ("generic", lambda: run_generic(run_hlr(1, 1), sweeps=args.generic_sweeps,
                                                         warmup=50)),
                         ("stream", lambda: log(str(stream_bandwidth(256, 3))))):
@@ -758,7 +766,7 @@ def main():
     surr = (run_surrogate(iters=args.surrogate_iters, burn=half, train=half)
             if "surrogate" in legs else None)
     bs = max(1, args.batched_sweeps)
-    bss = max(0, args.batched_surrogate_sweeps)
+    bss = max(0, 10_000 if args.batched_surrogate_sweeps is None else args.batched_surrogate_sweeps)
     bat = (run_batched(sweeps=bs, warmup=max(1, bs // 2), surrogate_sweeps=bss)
            if "batched" in legs else None)
     if bat:

@@ -44,20 +44,23 @@ def _init(s, rng, C):
     return starts
 
 
-@pytest.mark.parametrize("lds", [False, True])
+@pytest.mark.parametrize("C,form", [(2, "register"), (2, "lds"), (4, "register_mfma"),
+                                    (4, "register"), (3, "register_mfma"), (1, "register")])
 @pytest.mark.parametrize("L", [8, -8])
-def test_batched_matches_oracle(rt, oracle, monkeypatch, L, lds):
+def test_batched_matches_oracle(rt, oracle, monkeypatch, L, C, form):
     """L > 0: fixed step and count; L < 0: the reference's step jitter with
     |L| steps (every chain runs the same number of passes).  The register
-    form of the pass, and the LDS-staged form forced.  HMC on v with
-    steps of 0.01: at 0.05 its leapfrog is unstable on dataset 3 and the
-    oracle alone turns a 1e-7 nudge of theta into 28 times this test's
-    tolerance within three sweeps; at 0.01 the same nudge stays below a
-    quarter of it on every dataset."""
-    monkeypatch.setenv("AUGUR_BHLR_PASS_LDS", "1" if lds else "0")
+    form of the pass (at 3-4 chains with its a x phase on the matrix cores,
+    or all on the vector units when forced), and the LDS-staged form
+    forced.  HMC on v with steps of 0.01: at 0.05 its leapfrog is unstable
+    on dataset 3 and the oracle alone turns a 1e-7 nudge of theta into 28
+    times this test's tolerance within three sweeps; at 0.01 the same nudge
+    stays below a quarter of it on every dataset."""
+    monkeypatch.setenv("AUGUR_BHLR_PASS_LDS", "1" if form == "lds" else "0")
+    monkeypatch.setenv("AUGUR_BHLR_R_VALU", "1" if form == "register" else "0")
     rng = np.random.default_rng(0)
     xs, ys = _datasets(rng, SHAPES)
-    C, EPS = 2, 0.01
+    EPS = 0.01
     s = rt.BhlrSampler(xs, ys, chains=C, seed=7, v_block=(0.2, 0.01, 0), theta_block=(0.1, EPS, L),
                        rows_per_block=512, use_graph=True)
     starts = _init(s, rng, C)
