@@ -29,6 +29,7 @@
 // are exactly what the pass computed when that point was proposed), so an
 // iteration of L leapfrog steps costs L data passes instead of the
 // reference's L + 3 (L + 1 gradients and 2 log-densities).
+#include <climits>
 #include <chrono>
 #include <mutex>
 
@@ -559,7 +560,10 @@ constexpr int kHlrRowPad = 256;
 //    keeps the chains in a lane-dependent order (rs_chain), so that four
 //    DPP levels leave it the one (row, chain) sum whose row term it
 //    evaluates (16 DPP adds per tile instead of 64, and no selects).
-template <int D, int CP, bool MB>
+// LL = false: the gradient only, for the leapfrog passes before the shortest
+// trajectory's last (the log-density is read by the cache pass and by each
+// chain's last pass)
+template <int D, int CP, bool MB, bool LL = true>
 __global__ __launch_bounds__(kPassThreads, (D * CP > 256 ? 2 : kPassBlocksPerCU)) void k_hlr_pass_r(PassArgs A) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int F = kRowFeat, RPT = row_rpt(CP);
@@ -686,13 +690,14 @@ __global__ __launch_bounds__(kPassThreads, (D * CP > 256 ? 2 : kPassBlocksPerCU)
         const int yy = yv[0] ^ ((yv[0] ^ yv[1]) & im);
         const bool ok = tile * TRr + i * RG + rsub < n;
         float a;
-        const float lr = log_sigmoid_pm(tv + b_c, yy, &a);
-        if (ok) {
-          llacc += (double)lr;
-          gbacc += (double)a;
+        if constexpr (LL) {
+          const float lr = log_sigmoid_pm(tv + b_c, yy, &a);
+          if (ok) llacc += (double)lr;
         } else {
-          a = 0.0f;
+          a = sigmoid_adjoint_pm(tv + b_c, yy);
         }
+        a = ok ? a : 0.0f;
+        gbacc += (double)a;
         aown[0] = a;
       }
       // adjoints: pair m = q + s LPR is row slot m / CP, chain m % CP = my_c
@@ -708,13 +713,14 @@ __global__ __launch_bounds__(kPassThreads, (D * CP > 256 ? 2 : kPassBlocksPerCU)
         for (int k = 1; k < RPT; ++k) yy = (i == k) ? yv[k] : yy;
         const bool ok = m < NC && tile * TRr + i * RG + rsub < n;
         float a;
-        const float lr = log_sigmoid_pm(tv + b_c, yy, &a);
-        if (ok) {
-          llacc += (double)lr;
-          gbacc += (double)a;
+        if constexpr (LL) {
+          const float lr = log_sigmoid_pm(tv + b_c, yy, &a);
+          if (ok) llacc += (double)lr;
         } else {
-          a = 0.0f;
+          a = sigmoid_adjoint_pm(tv + b_c, yy);
         }
+        a = ok ? a : 0.0f;
+        gbacc += (double)a;
         aown[s] = a;
       }
       if constexpr (MB) {
@@ -854,6 +860,21 @@ PassKernel pick_pass_r(int d, int cp) {
   if (d == 128 && cp == 8) {
     const char* e = getenv("AUGUR_HLR_VALU_BACK");
     return (e && atoi(e)) ? k_hlr_pass_r<128, 8, false> : k_hlr_pass_r<128, 8, true>;
+  }
+  return nullptr;
+}
+
+// The gradient-only kernel of pick_pass_r's form, if compiled.
+PassKernel pick_pass_r_grad(int d, int cp) {
+#define AUGUR_PASS(D_, C_) if (d == D_ && cp == C_) return k_hlr_pass_r<D_, C_, false, false>;
+  AUGUR_PASS(16, 1) AUGUR_PASS(16, 2)
+  AUGUR_PASS(32, 1) AUGUR_PASS(32, 2) AUGUR_PASS(32, 4)
+  AUGUR_PASS(64, 1) AUGUR_PASS(64, 2) AUGUR_PASS(64, 4)
+  AUGUR_PASS(128, 1) AUGUR_PASS(128, 2) AUGUR_PASS(128, 4)
+#undef AUGUR_PASS
+  if (d == 128 && cp == 8) {
+    const char* e = getenv("AUGUR_HLR_VALU_BACK");
+    return (e && atoi(e)) ? k_hlr_pass_r<128, 8, false, false> : k_hlr_pass_r<128, 8, true, false>;
   }
   return nullptr;
 }
@@ -1122,6 +1143,7 @@ struct HlrSampler final : augur_sampler {
   int G = 1;
   int G64 = 1;                     // grid of the FP64 log-density (launch_ll64)
   PassKernel fast_pass = nullptr;  // specialised pass for (d, cp), if compiled
+  PassKernel fast_pass_grad = nullptr;  // its gradient-only form, if any
   size_t fast_smem = 0;            // its dynamic LDS bytes
   std::vector<augur_hmc_block> sched;
   DevBuf<float> x, beta32;
@@ -1187,8 +1209,9 @@ struct HlrSampler final : augur_sampler {
     return A;
   }
 
-  void launch_pass(const PassArgs& A) {
-    PassKernel k = fast_pass;
+  // grad_only: a leapfrog pass before every chain's last (no log-density)
+  void launch_pass(const PassArgs& A, bool grad_only = false) {
+    PassKernel k = (grad_only && fast_pass_grad) ? fast_pass_grad : fast_pass;
     size_t smem = fast_smem;
     if (!k) {
       k = k_hlr_pass;
@@ -1375,11 +1398,12 @@ struct HlrSampler final : augur_sampler {
     H.cache_g = cache_g.p; H.cache_ll = cache_ll.p; H.cache_g_w = cache_g.p; H.cache_ll_w = cache_ll.p;
     H.gprop = gprop.p; H.llprop = llprop.p; H.state = state.p; H.prop = prop.p; H.p0 = p0.p;
     H.p = p.p; H.k0 = k0.p; H.beta32 = beta32.p; H.stats = stats_buf.p + (size_t)bi * chains * 2;
-    int lmax = 0;
+    int lmax = 0, lmin = INT_MAX;
     for (int c = 0; c < chains; ++c) {
       hmc_jitter(Stream{seed, (uint32_t)c, sweep, tag + 1}, blk.sim_len, blk.step_size, blk.fixed_steps,
                  &H.hs.step[c], &H.hs.steps[c]);
       lmax = std::max(lmax, H.hs.steps[c]);
+      lmin = std::min(lmin, H.hs.steps[c]);
     }
     if (fuse_v) k_hmc_vblock_begin<<<chains, 64, 0, stream>>>(*fuse_v, H);
     else k_hmc_begin<<<chains, 64, 0, stream>>>(H);
@@ -1398,7 +1422,7 @@ struct HlrSampler final : augur_sampler {
         A.cap_row = n_pairs;
         n_pairs += chains;
       }
-      launch_pass(A);
+      launch_pass(A, l < lmin);
     }
     k_hmc_end<<<chains, 64, 0, stream>>>(H);
     AUGUR_CHECK_LAUNCH();
@@ -1470,6 +1494,7 @@ struct HlrSampler final : augur_sampler {
     if (fast_pass && !(lds && atoi(lds))) {
       if (PassKernel r = pick_pass_r(d, cp)) {
         fast_pass = r;
+        fast_pass_grad = pick_pass_r_grad(d, cp);
         fast_smem = pass_r_smem_bytes(cp);
         tiles = (n + pass_r_rows(d, cp) - 1) / pass_r_rows(d, cp);
         ring = row_ring(cp);
