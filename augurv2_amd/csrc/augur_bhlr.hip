@@ -334,18 +334,34 @@ __global__ __launch_bounds__(256) void k_bhlr_finish(BArgs A) {
   const BDs ds = A.ds[di];
   const int D = ds.d, VS = D + 2;
   const int lv = v - ds.voff;
-  const double* pv = A.part + ds.poff + (size_t)lv * ds.nblk;
-  double s = 0.0;
-  for (int b = 0; b < ds.nblk; ++b) s += pv[b];
   const int c = lv / VS, j = lv % VS;
-  const size_t ci = (size_t)c * A.n_ds + di;
   if (j == D) return;  // no intercept in the batched model
+  const size_t ci = (size_t)c * A.n_ds + di;
+  const size_t gi = (size_t)c * A.GW + ds.goff + j;
+  const size_t si = (size_t)c * A.SW + ds.soff;
+  // the leapfrog update's operands are requested with the partials
+  const bool upd = j < D && A.mode != BMODE_CACHE && !(A.smode && A.smode[di]);
+  double var = 1.0, xv = 0.0, p_old = 0.0;
+  if (upd) {
+    var = A.state[si + D + 1];
+    xv = A.prop[si + j];
+    p_old = A.p[gi];
+  }
+  // eight independent sums (blocks b = k mod 8), combined in a fixed order:
+  // eight loads in flight instead of one chain of dependent adds
+  const double* pv = A.part + ds.poff + (size_t)lv * ds.nblk;
+  double s8[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int b0 = 0; b0 < ds.nblk; b0 += 8) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (b0 + k < ds.nblk) s8[k] += pv[b0 + k];
+  }
+  const double s = ((s8[0] + s8[1]) + (s8[2] + s8[3])) + ((s8[4] + s8[5]) + (s8[6] + s8[7]));
   if (j == D + 1) {
     if (A.mode == BMODE_CACHE) A.cache_ll[ci] = s;
     else if (A.l == A.steps) A.llprop[ci] = s;
     return;
   }
-  const size_t gi = (size_t)c * A.GW + ds.goff + j;
   if (A.mode == BMODE_CACHE) {
     A.cache_g[gi] = s;
     return;
@@ -354,18 +370,15 @@ __global__ __launch_bounds__(256) void k_bhlr_finish(BArgs A) {
   if (A.l == L) A.gprop[gi] = s;
   // a surrogate dataset's last pass only evaluates its proposal: the
   // trajectory kernel has already moved theta and the momentum
-  if (A.smode && A.smode[di]) return;
+  if (!upd) return;
   if (A.cap_t) {  // a fitting pair: the point this pass evaluated, its data gradient
     A.cap_t[gi] = A.beta32[gi];
     A.cap_g[gi] = (float)s;
   }
-  const size_t si = (size_t)c * A.SW + ds.soff;
-  const double var = A.state[si + D + 1];
-  double xv = A.prop[si + j];
   const double g = s + (-xv / var);
   const double step = theta_step(A, (uint32_t)(di * A.C + c));
   const double coef = (A.l == L) ? 0.5 * step : step;
-  const double pj = A.p[gi] + coef * g;
+  const double pj = p_old + coef * g;
   A.p[gi] = pj;
   if (A.l < L) {
     xv += step * pj;
@@ -1627,11 +1640,17 @@ struct BhlrSampler final : augur_sampler {
       }
       BArgs A = base;
       A.mode = BMODE_CACHE;
-      pass<<<nblk_total, kBThreads, pass_smem, stream>>>(A);  // warm
-      AUGUR_CHECK_LAUNCH();
+      // a leapfrog step's cost: the pass and the finish, whose work grows
+      // with the number of blocks
+      auto step_once = [&]() {
+        pass<<<nblk_total, kBThreads, pass_smem, stream>>>(A);
+        AUGUR_CHECK_LAUNCH();
+        k_bhlr_finish<<<(A.nval + 255) / 256, 256, 0, stream>>>(A);
+        AUGUR_CHECK_LAUNCH();
+      };
+      step_once();  // warm
       AUGUR_HIP(hipEventRecord(e0, stream));
-      for (int r = 0; r < 3; ++r) pass<<<nblk_total, kBThreads, pass_smem, stream>>>(A);
-      AUGUR_CHECK_LAUNCH();
+      for (int r = 0; r < 3; ++r) step_once();
       AUGUR_HIP(hipEventRecord(e1, stream));
       AUGUR_HIP(hipEventSynchronize(e1));
       float ms = 0.0f;
