@@ -864,6 +864,21 @@ __device__ inline void split_bf16x3(float a, __bf16& h, __bf16& m, __bf16& l) {
   l = (__bf16)(r1 - (float)m);
 }
 
+// split_bf16x3 of two values at once, each part of the pair packed in one
+// 32-bit word (element 0 in the low half): one V_CVT_PK_BF16_F32 per part
+// for both values, the same roundings and subtractions as split_bf16x3.
+__device__ inline void split_bf16x3_pair(float a0, float a1, uint32_t& h, uint32_t& m, uint32_t& l) {
+  typedef float f2_t __attribute__((ext_vector_type(2)));
+  typedef __bf16 b2_t __attribute__((ext_vector_type(2)));
+  auto cvt = [](float x, float y) { return __builtin_bit_cast(uint32_t, __builtin_convertvector(f2_t{x, y}, b2_t)); };
+  auto lo = [](uint32_t w) { return __uint_as_float(w << 16); };
+  auto hi = [](uint32_t w) { return __uint_as_float(w & 0xffff0000u); };
+  h = cvt(a0, a1);
+  const float r0 = a0 - lo(h), r1 = a1 - hi(h);
+  m = cvt(r0, r1);
+  l = cvt(r0 - lo(m), r1 - hi(m));
+}
+
 constexpr int kMfmaCols = 16;
 constexpr int kMfRing = 4;  // 16-row tiles in flight per wave in the matrix-core form
 
@@ -985,7 +1000,7 @@ __device__ void bhlr_body_mfma(const BArgs& A, const BDs& ds, unsigned char* sme
     // the slot's registers are free once the products have read them
     load_tile(min(tt + PD, lastw), qn[slot], yn[slot]);
     // rows r0 + 4 kg + r, chain li
-    bf16x4_t ah, am, al;
+    float av[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int rr = r0 + 4 * kg + r;
@@ -997,13 +1012,16 @@ __device__ void bhlr_body_mfma(const BArgs& A, const BDs& ds, unsigned char* sme
       } else {
         a = sigmoid_adjoint_pm(t[r], (int)((yw >> (8 * r)) & 0xffu));
       }
-      a = ok ? a : 0.0f;
-      __bf16 h, m, l;
-      split_bf16x3(a, h, m, l);
-      ah[r] = h;
-      am[r] = m;
-      al[r] = l;
+      av[r] = ok ? a : 0.0f;
     }
+    // three BF16 parts of the 4 adjoints, two at a time
+    uint32_t h01, m01, l01, h23, m23, l23;
+    split_bf16x3_pair(av[0], av[1], h01, m01, l01);
+    split_bf16x3_pair(av[2], av[3], h23, m23, l23);
+    typedef uint32_t u2_t __attribute__((ext_vector_type(2)));
+    const bf16x4_t ah = __builtin_bit_cast(bf16x4_t, u2_t{h01, h23});
+    const bf16x4_t am = __builtin_bit_cast(bf16x4_t, u2_t{m01, m23});
+    const bf16x4_t al = __builtin_bit_cast(bf16x4_t, u2_t{l01, l23});
     // the wave's tile copy is complete before any lane reads it
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();

@@ -143,10 +143,12 @@ __device__ inline SigParts sig_parts(float t) {
   return {e, u, __builtin_amdgcn_rcpf(u)};
 }
 
+// y = 1: sigmoid(-t); y = 0: -sigmoid(t).  The magnitude is the small value
+// e / u exactly when y = 1 and t >= 0 or y = 0 and t < 0, else 1 / u.
 __device__ inline float sig_adjoint(float t, int y, const SigParts& p) {
   const float se = p.e * p.inv;
-  const bool pos = t >= 0.0f;
-  return y ? (pos ? se : p.inv) : -(pos ? p.inv : se);
+  const float mag = ((y != 0) == (t >= 0.0f)) ? se : p.inv;
+  return y ? mag : -mag;
 }
 
 __device__ inline float log_sigmoid_pm(float t, int y, float* a_out) {
