@@ -6,8 +6,11 @@
 // Exponential(a); theta_i[k] ~ Normal(0, v_i); y_i[n] ~ Bernoulli(sigmoid(
 // dotprod(theta_i, x_i[n]))), with C chains per dataset and the schedule
 // HMC [v] | MWG [v] (*) HMC [theta] (fixed leapfrog count for theta, step
-// optionally jittered).  Up to 4 chains per dataset the pass is vector FP32
-// (below); from 5 chains on it is a BF16 matrix-core GEMM (bhlr_body_mfma).
+// optionally jittered).  Up to 4 chains per dataset the pass is FP32 on the
+// vector units (bhlr_body_r; at 3-4 chains with its a x phase on FP32
+// matrix-core outer products, bhlr_body_rq); from 5 chains on it is a BF16
+// matrix-core GEMM (bhlr_body_mfma).  The leapfrog passes before a
+// trajectory's last run gradient-only instantiations (no log-density).
 //
 // Layout in HBM: every dataset's X stored BF16, row-major, back to back; y as
 // uint8.  The pass kernel gives each dataset ceil(N_i / R) blocks of R rows (R
