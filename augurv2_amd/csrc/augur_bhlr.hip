@@ -725,7 +725,9 @@ __device__ void bhlr_body_rq(const BArgs& A, const BDs& ds, unsigned char* smem)
         const float t1 = tp[1][0] + dpp_f<0x141>(tp[1][0]);
         const float tv = hi ? t1 : t0;
         const int yy = ypre[slot][0];
-        const bool ok = my_c < A.C && rbeg + t * TRr + hi * RG + rsub < rend;
+        // rows past the dataset's end are zero rows of X and the padding
+        // chains' gradients are never read: only the log-density needs the mask
+        const bool ok = !LL || (my_c < A.C && rbeg + t * TRr + hi * RG + rsub < rend);
         float a;
         if constexpr (LL) {
           const float lr = log_sigmoid_pm(tv, yy, &a);
@@ -740,7 +742,7 @@ __device__ void bhlr_body_rq(const BArgs& A, const BDs& ds, unsigned char* smem)
       } else {
 #pragma unroll
         for (int i = 0; i < RPT; ++i) {
-          const bool ok = my_c < A.C && rbeg + t * TRr + i * RG + rsub < rend;
+          const bool ok = !LL || (my_c < A.C && rbeg + t * TRr + i * RG + rsub < rend);
           float a;
           if constexpr (LL) {
             const float lr = log_sigmoid_pm(tp[i][0], ypre[slot][i], &a);
