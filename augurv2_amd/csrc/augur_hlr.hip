@@ -1494,7 +1494,9 @@ struct HlrSampler final : augur_sampler {
     if (fast_pass && !(lds && atoi(lds))) {
       if (PassKernel r = pick_pass_r(d, cp)) {
         fast_pass = r;
-        fast_pass_grad = pick_pass_r_grad(d, cp);
+        // AUGUR_HLR_GRAD_PASS=0: every pass computes the log-density too
+        const char* gp = getenv("AUGUR_HLR_GRAD_PASS");
+        fast_pass_grad = (gp && atoi(gp) == 0) ? nullptr : pick_pass_r_grad(d, cp);
         fast_smem = pass_r_smem_bytes(cp);
         tiles = (n + pass_r_rows(d, cp) - 1) / pass_r_rows(d, cp);
         ring = row_ring(cp);
