@@ -2081,6 +2081,9 @@ augur_status augur_bhlr_create(const augur_bhlr_desc* D, augur_sampler** out) {
         s->pass = pick_bpass(cpn);
         s->pass_smem = bhlr_smem_bytes(s->dmax, cpn);
       }
+      // AUGUR_BHLR_GRAD_PASS=0: every pass computes the log-density too
+      if (const char* gp = getenv("AUGUR_BHLR_GRAD_PASS"))
+        if (atoi(gp) == 0) s->pass_grad = nullptr;
       allow_lds(s->pass, s->pass_smem);
       if (s->pass_grad) allow_lds(s->pass_grad, s->pass_smem);
       // state starts at theta = 0, v = 1

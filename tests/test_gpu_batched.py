@@ -399,3 +399,27 @@ def test_batched_surrogate_posterior_matches_exact_sampler(rt):
             worst = max(worst, z)
         off += d + 2
     assert worst < 4.5, worst
+
+
+@pytest.mark.parametrize("C", [4, 3, 16])
+def test_gradient_only_passes_give_the_same_trajectories(rt, monkeypatch, C):
+    """The leapfrog passes before a trajectory's last run kernels that skip
+    the log-density (register form with matrix-core a x at 3-4 chains,
+    BF16 matrix cores at 16).  Their adjoints are the same values, so every
+    state must equal, bit for bit, a run whose passes all compute the
+    log-density (AUGUR_BHLR_GRAD_PASS=0)."""
+    rng = np.random.default_rng(5)
+    xs, ys = _datasets(rng, SHAPES)
+    runs = []
+    for grad_pass in ("1", "0"):
+        monkeypatch.setenv("AUGUR_BHLR_GRAD_PASS", grad_pass)
+        s = rt.BhlrSampler(xs, ys, chains=C, seed=29, v_block=("mwg", 0.5), theta_block=(0.1, 0.01, -8),
+                           rows_per_block=512, use_graph=True)
+        _init(s, np.random.default_rng(6), C)
+        s.step(4)
+        runs.append([s.get(f"state:{i}", c).copy() for i in range(len(SHAPES)) for c in range(C)])
+        acc = s.acceptance()[1]
+        s.close()
+    assert acc.mean() > 0.3
+    for a, b in zip(*runs):
+        np.testing.assert_array_equal(a, b)

@@ -185,6 +185,36 @@ def test_hlr_iterations_match_oracle(rt, oracle, fixed, intercept):
     s.close()
 
 
+@pytest.mark.parametrize("d,C,intercept", [(64, 4, False), (64, 4, True), (128, 8, False), (32, 2, True)])
+def test_hlr_gradient_only_passes_give_the_same_trajectories(rt, monkeypatch, d, C, intercept):
+    """The register-form passes before the shortest chain's last leapfrog
+    step skip the log-density (the intercept's gradient sum is kept).  The
+    adjoints are the same values, so every state must equal, bit for bit,
+    a run whose passes all compute the log-density (AUGUR_HLR_GRAD_PASS=0);
+    the step jitter gives the chains different step counts."""
+    from augurv2_amd._lib import HMC_BLOCK_V, HMC_BLOCK_B, HMC_BLOCK_THETA
+    rng = np.random.default_rng(40 + d)
+    x, y = _hlr_data(rng, 3000, d)
+    blocks = [(HMC_BLOCK_V, 1.0, 0.05, 0)]
+    if intercept:
+        blocks.append((HMC_BLOCK_B, 0.1, 0.01, 0))
+    blocks.append((HMC_BLOCK_THETA, 0.1, 0.005, 0))
+    starts = [np.concatenate([rng.standard_normal(d) * 0.1, [0.1 * intercept], [0.8 + 0.1 * c]]) for c in range(C)]
+    runs = []
+    for grad_pass in ("1", "0"):
+        monkeypatch.setenv("AUGUR_HLR_GRAD_PASS", grad_pass)
+        s = rt.HlrSampler(x, y, blocks, chains=C, seed=77, has_intercept=intercept)
+        for c in range(C):
+            s.set("state", starts[c], chain=c)
+        s.step(5)
+        runs.append([s.get("state", c).copy() for c in range(C)])
+        acc = s.acceptance()
+        s.close()
+    assert acc[-1].mean() > 0.3
+    for a, b in zip(*runs):
+        np.testing.assert_array_equal(a, b)
+
+
 def test_hlr_v_block_last_matches_oracle(rt, oracle):
     """HMC [theta] (*) HMC [v]: a v block with no data block after it runs as
     its own launch instead of sharing the next block's first launch."""
