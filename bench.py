@@ -840,7 +840,9 @@ def main():
         "batched": bat,
         "batched_16_chains": bat16,
         "generic_lowering": gen,
-        "roofline": {"kernel": "k_hlr_pass_r<64,4>", "bound": "hbm", "achieved": achieved,
+        "roofline": {"kernel": "k_hlr_pass_r<64,4,false,LL> (LL = false: gradient-only leapfrog "
+                               "passes; true: cache and last passes; avg_us over both)",
+                     "bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": 1024.0 * (PASS_FETCH_KIB * FETCH_SCALE + PASS_WRITE_KIB * WRITE_SCALE),
                      "avg_us": hlr["pass_avg_us"], "launches_timed": hlr["pass_launches_timed"],
